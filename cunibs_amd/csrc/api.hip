@@ -1,0 +1,799 @@
+// The C ABI (include/cunibs_amd.h): the per-mesh context, the block solve, and kernel-level
+// entry points. No exception crosses the boundary; each call returns a status code and leaves
+// its message in a thread-local buffer.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/cunibs_amd.h"
+#include "coil.hpp"
+#include "prims.hpp"
+#include "solve.hpp"
+
+namespace {
+
+thread_local std::string g_error;
+
+template <typename F>
+int32_t guarded(F&& f) {
+    try {
+        f();
+        g_error.clear();
+        return CN_OK;
+    } catch (const cn::InvalidArgument& e) {
+        g_error = e.what();
+        return CN_EINVAL;
+    } catch (const cn::NotConverged& e) {
+        g_error = e.what();
+        return CN_ENOCONV;
+    } catch (const cn::HipError& e) {
+        g_error = e.what();
+        return CN_EHIP;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return CN_EINTERNAL;
+    } catch (...) {
+        g_error = "unknown error";
+        return CN_EINTERNAL;
+    }
+}
+
+// fem/assembly.py:15-26.
+const double kDefaultSigma[11] = {NAN, 0.126, 0.275, 1.654, 0.010, 0.465, 0.5, 0.008, 0.025, 0.6, 0.16};
+constexpr double kMu0Over4Pi = 1e-7;  // fem/placement.py:16
+
+int device_cu_count() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 256;
+    return cus > 0 ? cus : 256;
+}
+
+}  // namespace
+
+struct cn_context {
+    hipStream_t s = nullptr;
+    cn_options opts{};
+    std::int64_t n = 0, m = 0, n_skin = 0;
+    int ground = 0;
+    int n_cu = 256;
+    cn::DeviceBuffer<double> nodes, bary, skin_a, skin_b, skin_c, skin_n;
+    cn::DeviceBuffer<std::int32_t> tets, tags, skin, node_perm, tet_perm;
+    cn::DeviceBuffer<float> g32, vols32, neg_vc;
+    cn::DeviceBuffer<std::int32_t> n2c_ptr, n2c_idx;
+    // full stiffness (optional) and the reduced operator
+    cn::DeviceBuffer<std::int32_t> indptr, indices;
+    cn::DeviceBuffer<double> data;
+    std::int64_t nnz_full = 0;
+    cn::DeviceBuffer<std::int32_t> rptr, rcol;
+    cn::DeviceBuffer<double> rval64;
+    cn::DeviceBuffer<float> rval32;
+    std::int64_t nnz_red = 0;
+    cn::Hierarchy hier;
+    std::unique_ptr<cn::BlockSolver> solver;
+    // summary region
+    cn::DeviceBuffer<std::int32_t> region_idx;
+    std::int64_t n_region = 0;
+    double region_volume = 0.0;
+    // per-block scratch (grow-only)
+    cn::DeviceBuffer<double> d_centers, d_handles, d_dists, d_tf, d_origins;
+    cn::DeviceBuffer<std::int32_t> d_degen;
+    cn::DeviceBuffer<float> d_dip, d_dadt_nodes, d_dadt_elm, d_B, d_E, d_magn, d_rhs;
+    cn::DeviceBuffer<double> d_X, d_v, d_x0;
+
+    cn::CsrView reduced_view() const {
+        return {rptr.get(), rcol.get(), rval32.get(), n - 1, n - 1, nnz_red};
+    }
+    ~cn_context() {
+        solver.reset();
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+namespace {
+
+__global__ void region_index_kernel(const std::int32_t* __restrict__ tags, std::int64_t m, int tag,
+                                    std::int32_t* __restrict__ flag) {
+    const std::int64_t e = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < m) flag[e] = tags[e] == tag ? 1 : 0;
+}
+
+__global__ void region_compact_kernel(const std::int32_t* __restrict__ flag, const std::int32_t* __restrict__ pos,
+                                      std::int64_t m, std::int32_t* __restrict__ idx) {
+    const std::int64_t e = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < m && flag[e]) idx[pos[e]] = static_cast<std::int32_t>(e);
+}
+
+__global__ void region_volume_kernel(const std::int32_t* __restrict__ idx, std::int64_t nr,
+                                     const float* __restrict__ vols, double* __restrict__ out) {
+    // One block, fixed order: thread strides then a tree.
+    __shared__ double s[cn::kBlock];
+    double acc = 0.0;
+    for (std::int64_t j = threadIdx.x; j < nr; j += cn::kBlock) acc += static_cast<double>(vols[idx[j]]);
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = cn::kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = s[0];
+}
+
+__global__ void product_kernel(const double* __restrict__ a, const double* __restrict__ b, std::int64_t n,
+                               double* __restrict__ o) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = a[i] * b[i];
+}
+
+void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const std::int32_t* tet_nodes,
+                   const std::int32_t* tet_tags, std::int64_t m, const std::int32_t* skin_tris, std::int64_t n_skin) {
+    using namespace cn;
+    require(n >= 2, "the mesh needs at least two nodes");
+    require(m >= 1, "the mesh needs at least one tetrahedron");
+    require(n < (1ll << 31) && 4 * m < (1ll << 31), "mesh too large for int32 indexing");
+    for (std::int64_t i = 0; i < 3 * n; ++i)
+        require(std::isfinite(nodes_mm[i]), "node coordinates must be finite");
+    for (std::int64_t i = 0; i < 4 * m; ++i)
+        require(tet_nodes[i] >= 0 && tet_nodes[i] < n, "tetrahedron references a node outside the mesh");
+    for (std::int64_t i = 0; i < 3 * n_skin; ++i)
+        require(skin_tris[i] >= 0 && skin_tris[i] < n, "skin triangle references a node outside the mesh");
+    const hipStream_t s = c.s;
+    c.n = n;
+    c.m = m;
+    c.n_skin = n_skin;
+    c.n_cu = device_cu_count();
+
+    DeviceBuffer<double> nodes_in = upload(nodes_mm, 3 * n, s, "nodes");
+    DeviceBuffer<std::int32_t> tets_in = upload(tet_nodes, 4 * m, s, "tets");
+    DeviceBuffer<std::int32_t> tags_in = upload(tet_tags, m, s, "tags");
+    c.skin.reset(std::max<std::int64_t>(3 * n_skin, 1), "skin");
+    h2d(c.skin.get(), skin_tris, 3 * n_skin, s);
+    c.node_perm.reset(n, "node perm");
+    c.tet_perm.reset(m, "tet perm");
+    DeviceBuffer<std::int32_t> inverse(n, "inverse perm");
+    spatial_order(nodes_in.get(), n, tets_in.get(), m, c.node_perm.get(), c.tet_perm.get(), inverse.get(), s);
+    c.nodes.reset(3 * n, "nodes");
+    c.tets.reset(4 * m, "tets");
+    c.tags.reset(m, "tags");
+    apply_spatial_order(nodes_in.get(), tets_in.get(), tags_in.get(), n, m, c.node_perm.get(), c.tet_perm.get(),
+                        inverse.get(), c.nodes.get(), c.tets.get(), c.tags.get(), c.skin.get(), n_skin, s);
+    nodes_in.release();
+    tets_in.release();
+    tags_in.release();
+
+    DeviceBuffer<double> g64(12 * m, "g64"), vols64(m, "vols64"), cond(m, "cond");
+    p1_gradients(c.nodes.get(), c.tets.get(), m, g64.get(), vols64.get(), s);
+    const int bad = count_bad_volumes(vols64.get(), m, s);
+    if (bad) {
+        throw InvalidArgument(std::to_string(bad) + (bad == 1 ? " tetrahedron has" : " tetrahedra have") +
+                              " zero or non-finite volume");
+    }
+    std::vector<double> lut;
+    if (c.opts.conductivity != nullptr && c.opts.n_conductivity > 0)
+        lut.assign(c.opts.conductivity, c.opts.conductivity + c.opts.n_conductivity);
+    else
+        lut.assign(kDefaultSigma, kDefaultSigma + 11);
+    const int unknown = conductivity(c.tags.get(), m, lut.data(), static_cast<int>(lut.size()), cond.get(), s);
+    if (unknown) throw InvalidArgument(std::to_string(unknown) + " tetrahedra have unknown tags");
+
+    c.n2c_ptr.reset(n + 1, "n2c ptr");
+    c.n2c_idx.reset(4 * m, "n2c idx");
+    node2corner(c.tets.get(), m, 4, n, c.n2c_ptr.get(), c.n2c_idx.get(), s);
+    DeviceBuffer<std::int32_t> indptr, indices;
+    const std::int64_t nnz = stiffness_pattern(c.tets.get(), c.n2c_ptr.get(), c.n2c_idx.get(), n, indptr, indices, s);
+    DeviceBuffer<double> data(std::max<std::int64_t>(nnz, 1), "stiffness");
+    DeviceBuffer<double> scale(m, "vol*sigma");  // fem/assembly.py:131, FP64 product
+    product_kernel<<<grid_for(m), kBlock, 0, s>>>(vols64.get(), cond.get(), m, scale.get());
+    stiffness_values(g64.get(), scale.get(), c.tets.get(), c.n2c_ptr.get(), c.n2c_idx.get(), n, indptr.get(),
+                     indices.get(), data.get(), s);
+    c.nnz_full = nnz;
+    c.ground = lowest_z_node(c.nodes.get(), n, s);
+    c.nnz_red = reduce_grounded(indptr.get(), indices.get(), data.get(), n, c.ground, c.rptr, c.rcol, c.rval64,
+                                c.rval32, s);
+    if (c.opts.keep_stiffness) {
+        c.indptr = std::move(indptr);
+        c.indices = std::move(indices);
+        c.data = std::move(data);
+    } else {
+        indptr.release();
+        indices.release();
+        data.release();
+    }
+
+    c.g32.reset(12 * m, "g32");
+    c.vols32.reset(m, "vols32");
+    c.neg_vc.reset(m, "neg_vc");
+    c.bary.reset(3 * m, "barycentres");
+    resident_arrays(g64.get(), vols64.get(), cond.get(), c.nodes.get(), c.tets.get(), m, c.g32.get(), c.vols32.get(),
+                    c.neg_vc.get(), c.bary.get(), s);
+    g64.release();
+    vols64.release();
+    cond.release();
+    scale.release();
+
+    // AMG hierarchy on the FP32 reduced operator.
+    AmgParams prm;
+    prm.min_coarse_rows = c.opts.min_coarse_rows;
+    prm.max_levels = c.opts.max_levels;
+    prm.rounds = c.opts.rounds;
+    prm.degree = c.opts.degree;
+    prm.lower_ratio = c.opts.lower_ratio;
+    c.hier = build_hierarchy(c.reduced_view(), prm, s);
+    c.solver = std::make_unique<BlockSolver>(Operator{c.reduced_view(), c.rval64.get()}, c.hier, s);
+
+    // Skin surface for the placement scan.
+    c.skin_a.reset(std::max<std::int64_t>(3 * n_skin, 1), "skin a");
+    c.skin_b.reset(std::max<std::int64_t>(3 * n_skin, 1), "skin b");
+    c.skin_c.reset(std::max<std::int64_t>(3 * n_skin, 1), "skin c");
+    c.skin_n.reset(std::max<std::int64_t>(3 * n_skin, 1), "skin normals");
+    if (n_skin > 0) {
+        triangle_vertices(c.nodes.get(), c.skin.get(), n_skin, c.skin_a.get(), c.skin_b.get(), c.skin_c.get(), s);
+        skin_normals(c.nodes.get(), c.skin.get(), n_skin, n, c.skin_n.get(), s);
+    }
+
+    // Summary region (metrics.py:246-259).
+    DeviceBuffer<std::int32_t> flag(m + 1, "region flag"), pos(m + 1, "region pos");
+    check(hipMemsetAsync(flag.get() + m, 0, sizeof(std::int32_t), s), "memset");
+    region_index_kernel<<<grid_for(m), kBlock, 0, s>>>(c.tags.get(), m, c.opts.region_tag, flag.get());
+    exclusive_scan_i32(flag.get(), pos.get(), m + 1, s);
+    std::int32_t nr = 0;
+    d2h(&nr, pos.get() + m, 1, s);
+    sync(s, "region size");
+    c.n_region = nr;
+    c.region_idx.reset(std::max<std::int64_t>(nr, 1), "region idx");
+    region_compact_kernel<<<grid_for(m), kBlock, 0, s>>>(flag.get(), pos.get(), m, c.region_idx.get());
+    DeviceBuffer<double> vol(1, "region volume");
+    region_volume_kernel<<<1, kBlock, 0, s>>>(c.region_idx.get(), nr, c.vols32.get(), vol.get());
+    d2h(&c.region_volume, vol.get(), 1, s);
+    sync(s, "context build");
+}
+
+// Dipoles of one placement in its coil frame, FP64 as fem/placement.py:294-313, then FP32 pack
+// [s | m | m x s | |s|^2] centred at the coil origin.
+void pack_dipoles(const double* tf, const double* pos_m, const double* mom, std::int64_t n_dip, float* out,
+                  double* origin) {
+    double rot[3][3], trans[3];
+    for (int r = 0; r < 3; ++r) {
+        for (int q = 0; q < 3; ++q) rot[r][q] = tf[4 * r + q];
+        trans[r] = tf[4 * r + 3];
+    }
+    const double o[3] = {trans[0] * 1e-3, trans[1] * 1e-3, trans[2] * 1e-3};
+    for (int r = 0; r < 3; ++r) origin[r] = o[r];
+    for (std::int64_t j = 0; j < n_dip; ++j) {
+        double s[3], mm[3];
+        for (int r = 0; r < 3; ++r) {
+            double acc = 0.0, am = 0.0;
+            for (int q = 0; q < 3; ++q) {
+                acc += pos_m[3 * j + q] * 1e3 * rot[r][q];
+                am += mom[3 * j + q] * rot[r][q];
+            }
+            s[r] = (acc + trans[r]) * 1e-3 - o[r];
+            mm[r] = am;
+        }
+        const double mxs[3] = {mm[1] * s[2] - mm[2] * s[1], mm[2] * s[0] - mm[0] * s[2], mm[0] * s[1] - mm[1] * s[0]};
+        float* p = out + 10 * j;
+        for (int r = 0; r < 3; ++r) {
+            p[r] = static_cast<float>(s[r]);
+            p[3 + r] = static_cast<float>(mm[r]);
+            p[6 + r] = static_cast<float>(mxs[r]);
+        }
+        p[9] = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];  // from the FP32 positions, as the reference
+    }
+}
+
+void place_block(cn_context& c, const double* centers, const double* handles, const double* dists, int k,
+                 double* tf_host, std::int32_t* degen_host) {
+    using namespace cn;
+    const hipStream_t s = c.s;
+    require(c.n_skin > 0, "the mesh has no scalp surface (tag 1005) to place a coil on");
+    for (int i = 0; i < 3 * k; ++i)
+        require(std::isfinite(centers[i]) && (handles == nullptr || std::isfinite(handles[i])),
+                "center_mm and handle_mm must be finite 3-vectors");
+    for (int i = 0; i < k; ++i) require(std::isfinite(dists[i]), "distance_mm must be finite");
+    c.d_centers.ensure(3 * k, "centres");
+    c.d_dists.ensure(k, "distances");
+    c.d_tf.ensure(16 * k, "transforms");
+    c.d_degen.ensure(k, "degenerate");
+    h2d(c.d_centers.get(), centers, 3 * k, s);
+    h2d(c.d_dists.get(), dists, k, s);
+    const double* dh = nullptr;
+    if (handles != nullptr) {
+        c.d_handles.ensure(3 * k, "handles");
+        h2d(c.d_handles.get(), handles, 3 * k, s);
+        dh = c.d_handles.get();
+    }
+    place_transforms(c.d_centers.get(), dh, c.d_dists.get(), k, c.skin_a.get(), c.skin_b.get(), c.skin_c.get(),
+                     c.skin_n.get(), static_cast<int>(c.n_skin), c.d_tf.get(), c.d_degen.get(), c.n_cu, s);
+    d2h(tf_host, c.d_tf.get(), 16 * k, s);
+    d2h(degen_host, c.d_degen.get(), k, s);
+    sync(s, "placement");
+}
+
+void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int64_t n_dip, const double* centers,
+                 const double* handles, const double* dists, int k, double didt, const cn_solve_options* sopt,
+                 const double* x0, cn_block_outputs* out) {
+    using namespace cn;
+    require(k >= 1 && k <= kMaxBlock, "k must be in [1, 32]");
+    require(n_dip >= 1, "the coil needs at least one dipole");
+    require(std::isfinite(didt), "didt must be finite");
+    for (int i = 0; i < 3 * k; ++i) {
+        require(std::isfinite(centers[i]) && (handles == nullptr || std::isfinite(handles[i])),
+                "center_mm and handle_mm must be finite 3-vectors");
+    }
+    const hipStream_t s = c.s;
+    const int K = padded_width(k);
+    const std::int64_t n = c.n, m = c.m, nr = n - 1;
+
+    // 1. coil frames (place.cu), degenerate handles rejected with the placement's index.
+    std::vector<double> tf(16 * k);
+    std::vector<std::int32_t> degen(k);
+    place_block(c, centers, handles, dists, k, tf.data(), degen.data());
+    if (out && out->transforms) std::memcpy(out->transforms, tf.data(), sizeof(double) * 16 * k);
+    if (out && out->degenerate) std::memcpy(out->degenerate, degen.data(), sizeof(std::int32_t) * k);
+    if (handles != nullptr) {
+        for (int i = 0; i < k; ++i) {
+            if (degen[i]) {
+                throw InvalidArgument("Degenerate coil placement at index " + std::to_string(i) + " of " +
+                                      std::to_string(k) +
+                                      ": handle_mm lies on the outward normal through the scalp projection of "
+                                      "center_mm, so the coil's rotation about the normal is undefined.");
+            }
+        }
+    }
+    // 2. dA/dt at nodes, element average, RHS.
+    std::vector<float> dip(static_cast<std::size_t>(10 * n_dip * k));
+    std::vector<double> origins(3 * k);
+    for (int i = 0; i < k; ++i) pack_dipoles(tf.data() + 16 * i, pos_m, mom, n_dip, dip.data() + 10 * n_dip * i, origins.data() + 3 * i);
+    c.d_dip.ensure(dip.size(), "dipoles");
+    c.d_origins.ensure(3 * k, "origins");
+    h2d(c.d_dip.get(), dip.data(), dip.size(), s);
+    h2d(c.d_origins.get(), origins.data(), 3 * k, s);
+    c.d_dadt_nodes.ensure(3 * n * k, "dadt nodes");
+    c.d_dadt_elm.ensure(3 * m * k, "dadt elements");
+    c.d_B.ensure(nr * K, "rhs block");
+    const float scale = static_cast<float>(didt) * static_cast<float>(kMu0Over4Pi);
+    dadt_nbody(c.d_dip.get(), static_cast<int>(n_dip), c.nodes.get(), n, c.d_origins.get(), k, scale,
+               c.d_dadt_nodes.get(), s);
+    element_average(c.d_dadt_nodes.get(), c.tets.get(), n, m, k, c.d_dadt_elm.get(), s);
+    float* rhs_full = nullptr;
+    if (out && out->rhs) {
+        c.d_rhs.ensure(n * k, "rhs full");
+        rhs_full = c.d_rhs.get();
+    }
+    assemble_rhs(c.d_dadt_elm.get(), c.g32.get(), c.neg_vc.get(), c.n2c_ptr.get(), c.n2c_idx.get(), n, m, k, K,
+                 c.ground, c.d_B.get(), rhs_full, s);
+    // 3. block PCG.
+    SolveOptions opt;
+    if (sopt) {
+        opt.tolerance = sopt->tolerance;
+        opt.max_iters = sopt->max_iters;
+        opt.max_refine = sopt->max_refine;
+        opt.drop_tol = sopt->drop_tol;
+        opt.use_graph = sopt->use_graph != 0;
+        opt.check_every = std::max(1, sopt->check_every);
+    }
+    c.d_X.ensure(nr * K, "solution");
+    const double* dx0 = nullptr;
+    if (x0 != nullptr) {
+        // (k, n_red) placement-major host guess -> (n_red, K) row-major device block
+        std::vector<double> blk(static_cast<std::size_t>(nr * K), 0.0);
+        for (int ci = 0; ci < k; ++ci)
+            for (std::int64_t r = 0; r < nr; ++r) blk[r * K + ci] = x0[ci * nr + r];
+        c.d_x0.ensure(nr * K, "x0");
+        h2d(c.d_x0.get(), blk.data(), blk.size(), s);
+        dx0 = c.d_x0.get();
+    }
+    SolveStats st = c.solver->solve(c.d_B.get(), c.d_X.get(), dx0, k, opt);
+    // 4. reconstruction and metrics.
+    c.d_magn.ensure(m * k, "magnE");
+    float* dE = nullptr;
+    if (out && out->E) {
+        c.d_E.ensure(3 * m * k, "E");
+        dE = c.d_E.get();
+    }
+    reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), c.d_dadt_elm.get(), n, m, k, K, c.ground, dE, c.d_magn.get(), s);
+    if (out) {
+        if (out->magnE) d2h(out->magnE, c.d_magn.get(), m * k, s);
+        if (out->E) d2h(out->E, c.d_E.get(), 3 * m * k, s);
+        if (out->dadt_elm) d2h(out->dadt_elm, c.d_dadt_elm.get(), 3 * m * k, s);
+        if (out->dadt_nodes) d2h(out->dadt_nodes, c.d_dadt_nodes.get(), 3 * n * k, s);
+        if (out->rhs) d2h(out->rhs, c.d_rhs.get(), n * k, s);
+        if (out->v) {
+            c.d_v.ensure(n * k, "potential");
+            scatter_potential(c.d_X.get(), n, k, K, c.ground, c.d_v.get(), s);
+            d2h(out->v, c.d_v.get(), n * k, s);
+        }
+        if (out->metrics)
+            region_metrics(c.d_magn.get(), m, k, c.region_idx.get(), c.n_region, c.vols32.get(), c.bary.get(),
+                           out->metrics, s);
+        for (int i = 0; i < k; ++i) {
+            if (out->iterations) out->iterations[i] = st.column_iterations[i];
+            if (out->relative_residual) out->relative_residual[i] = st.relative_residual[i];
+        }
+        out->total_iterations = st.iterations;
+        out->refinements = st.refinements;
+    }
+    sync(s, "block outputs");
+    for (int i = 0; i < k; ++i) {
+        if (!(st.relative_residual[i] <= opt.tolerance)) {
+            char buf[256];
+            std::snprintf(buf, sizeof buf,
+                          "PCG did not converge (column %d of the block): %d iterations, relative residual %.3e > "
+                          "tolerance %.3e",
+                          i, st.column_iterations[i] ? st.column_iterations[i] : st.iterations,
+                          st.relative_residual[i], opt.tolerance);
+            throw NotConverged(buf);
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cn_last_error(void) { return g_error.c_str(); }
+int32_t cn_version(void) { return 1; }
+
+int32_t cn_block_sizes(int32_t* out, int32_t cap) {
+    for (int i = 0; i < cn::kNumBlockWidths && i < cap; ++i) out[i] = cn::kBlockWidths[i];
+    return cn::kNumBlockWidths;
+}
+
+int32_t cn_max_block(void) { return cn::kMaxBlock; }
+
+void cn_default_options(cn_options* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->min_coarse_rows = 128;
+    o->max_levels = 50;
+    o->rounds = 2;
+    o->degree = 2;
+    o->lower_ratio = 12.0f;
+    o->keep_stiffness = 0;
+    o->conductivity = nullptr;
+    o->n_conductivity = 0;
+    o->region_tag = 2;
+}
+
+void cn_default_solve_options(cn_solve_options* o) {
+    o->tolerance = 1e-6;
+    o->max_iters = 2000;
+    o->max_refine = 8;
+    o->drop_tol = 1e-6;
+    o->use_graph = 1;
+    o->check_every = 4;
+}
+
+int32_t cn_context_create(const double* nodes_mm, int64_t n_nodes, const int32_t* tet_nodes, const int32_t* tet_tags,
+                          int64_t n_tets, const int32_t* skin_tris, int64_t n_skin, const cn_options* opts,
+                          cn_context** out) {
+    return guarded([&] {
+        cn::require(out != nullptr && nodes_mm != nullptr && tet_nodes != nullptr && tet_tags != nullptr,
+                    "null argument");
+        cn::require(n_skin == 0 || skin_tris != nullptr, "null skin_tris with n_skin > 0");
+        auto c = std::make_unique<cn_context>();
+        if (opts)
+            c->opts = *opts;
+        else
+            cn_default_options(&c->opts);
+        cn::require(c->opts.degree >= 1 && c->opts.degree <= cn::kMaxDegree, "degree must be in [1, 6]");
+        cn::require(c->opts.lower_ratio > 1.0f, "lower_ratio must exceed 1");
+        cn::check(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking), "create stream");
+        build_context(*c, nodes_mm, n_nodes, tet_nodes, tet_tags, n_tets, skin_tris, n_skin);
+        *out = c.release();
+    });
+}
+
+void cn_context_destroy(cn_context* ctx) {
+    if (ctx) {
+        (void)hipStreamSynchronize(ctx->s);
+        delete ctx;
+    }
+}
+
+int32_t cn_context_info(const cn_context* c, cn_info* o) {
+    return guarded([&] {
+        cn::require(c && o, "null argument");
+        std::memset(o, 0, sizeof(*o));
+        o->n_nodes = c->n;
+        o->n_tets = c->m;
+        o->n_skin = c->n_skin;
+        o->n_reduced = c->n - 1;
+        o->nnz_reduced = c->nnz_red;
+        o->nnz_full = c->nnz_full;
+        o->ground_node = c->ground;
+        o->n_levels = static_cast<int32_t>(std::min<std::size_t>(c->hier.levels.size(), CN_MAX_LEVELS));
+        for (int i = 0; i < o->n_levels; ++i) {
+            o->level_rows[i] = c->hier.levels[i].a.n;
+            o->level_nnz[i] = c->hier.levels[i].a.nnz;
+        }
+        o->level_rows[o->n_levels] = c->hier.coarse.n;
+        o->level_nnz[o->n_levels] = c->hier.coarse.nnz;
+        o->n_region = c->n_region;
+        o->region_volume_m3 = c->region_volume;
+        o->n_cu = c->n_cu;
+    });
+}
+
+int32_t cn_context_mesh(const cn_context* c, double* nodes_mm, int32_t* tets, int32_t* tags, int32_t* skin,
+                        int32_t* node_perm, int32_t* tet_perm) {
+    return guarded([&] {
+        cn::require(c != nullptr, "null context");
+        if (nodes_mm) cn::d2h(nodes_mm, c->nodes.get(), 3 * c->n, c->s);
+        if (tets) cn::d2h(tets, c->tets.get(), 4 * c->m, c->s);
+        if (tags) cn::d2h(tags, c->tags.get(), c->m, c->s);
+        if (skin) cn::d2h(skin, c->skin.get(), 3 * c->n_skin, c->s);
+        if (node_perm) cn::d2h(node_perm, c->node_perm.get(), c->n, c->s);
+        if (tet_perm) cn::d2h(tet_perm, c->tet_perm.get(), c->m, c->s);
+        cn::sync(c->s, "mesh readback");
+    });
+}
+
+int32_t cn_context_geometry(const cn_context* c, float* vols, double* bary, float* g, float* neg_vc) {
+    return guarded([&] {
+        cn::require(c != nullptr, "null context");
+        if (vols) cn::d2h(vols, c->vols32.get(), c->m, c->s);
+        if (bary) cn::d2h(bary, c->bary.get(), 3 * c->m, c->s);
+        if (g) cn::d2h(g, c->g32.get(), 12 * c->m, c->s);
+        if (neg_vc) cn::d2h(neg_vc, c->neg_vc.get(), c->m, c->s);
+        cn::sync(c->s, "geometry readback");
+    });
+}
+
+int32_t cn_context_stiffness(const cn_context* c, int32_t* indptr, int32_t* indices, double* data) {
+    return guarded([&] {
+        cn::require(c != nullptr, "null context");
+        cn::require(static_cast<bool>(c->indptr), "the context was built without keep_stiffness");
+        if (indptr) cn::d2h(indptr, c->indptr.get(), c->n + 1, c->s);
+        if (indices) cn::d2h(indices, c->indices.get(), c->nnz_full, c->s);
+        if (data) cn::d2h(data, c->data.get(), c->nnz_full, c->s);
+        cn::sync(c->s, "stiffness readback");
+    });
+}
+
+int32_t cn_context_reduced(const cn_context* c, int32_t* ptr, int32_t* col, double* val) {
+    return guarded([&] {
+        cn::require(c != nullptr, "null context");
+        if (ptr) cn::d2h(ptr, c->rptr.get(), c->n, c->s);
+        if (col) cn::d2h(col, c->rcol.get(), c->nnz_red, c->s);
+        if (val) cn::d2h(val, c->rval64.get(), c->nnz_red, c->s);
+        cn::sync(c->s, "reduced readback");
+    });
+}
+
+int32_t cn_context_level(const cn_context* c, int32_t level, int64_t* n_rows, int64_t* nnz, int64_t* n_coarse,
+                         int64_t* p_nnz, int32_t* a_ptr, int32_t* a_col, float* a_val, float* dinv, int32_t* agg,
+                         int32_t* p_ptr, int32_t* p_col, float* p_val) {
+    return guarded([&] {
+        cn::require(c != nullptr, "null context");
+        const int nl = static_cast<int>(c->hier.levels.size());
+        cn::require(level >= 0 && level <= nl, "level out of range");
+        const hipStream_t s = c->s;
+        if (level == nl) {
+            const cn::CsrF& a = c->hier.coarse;
+            if (n_rows) *n_rows = a.n;
+            if (nnz) *nnz = a.nnz;
+            if (n_coarse) *n_coarse = 0;
+            if (p_nnz) *p_nnz = 0;
+            if (a_ptr) cn::d2h(a_ptr, a.ptr.get(), a.n + 1, s);
+            if (a_col) cn::d2h(a_col, a.col.get(), a.nnz, s);
+            if (a_val) cn::d2h(a_val, a.val.get(), a.nnz, s);
+        } else {
+            const cn::Level& L = c->hier.levels[level];
+            if (n_rows) *n_rows = L.a.n;
+            if (nnz) *nnz = L.a.nnz;
+            if (n_coarse) *n_coarse = L.n_coarse;
+            if (p_nnz) *p_nnz = L.p.nnz;
+            if (a_ptr) cn::d2h(a_ptr, L.a.ptr, L.a.n + 1, s);
+            if (a_col) cn::d2h(a_col, L.a.col, L.a.nnz, s);
+            if (a_val) cn::d2h(a_val, L.a.val, L.a.nnz, s);
+            if (dinv) cn::d2h(dinv, L.dinv.get(), L.a.n, s);
+            if (agg) cn::d2h(agg, L.agg.get(), L.a.n, s);
+            if (p_ptr) cn::d2h(p_ptr, L.p.ptr.get(), L.p.n + 1, s);
+            if (p_col) cn::d2h(p_col, L.p.col.get(), L.p.nnz, s);
+            if (p_val) cn::d2h(p_val, L.p.val.get(), L.p.nnz, s);
+        }
+        cn::sync(s, "level readback");
+    });
+}
+
+int32_t cn_context_skin_normals(const cn_context* c, double* out) {
+    return guarded([&] {
+        cn::require(c != nullptr && out != nullptr, "null argument");
+        cn::d2h(out, c->skin_n.get(), 3 * c->n_skin, c->s);
+        cn::sync(c->s, "skin normals readback");
+    });
+}
+
+int32_t cn_solve_block(cn_context* c, const double* dip_pos_m, const double* dip_moment, int64_t n_dip,
+                       const double* centers_mm, const double* handles_mm, const double* distances_mm, int32_t k,
+                       double didt, const cn_solve_options* opt, const double* x0, cn_block_outputs* out) {
+    return guarded([&] {
+        cn::require(c && dip_pos_m && dip_moment && centers_mm && distances_mm, "null argument");
+        solve_block(*c, dip_pos_m, dip_moment, n_dip, centers_mm, handles_mm, distances_mm, k, didt, opt, x0, out);
+    });
+}
+
+// ---- kernel-level entry points ----
+
+int32_t cn_k_p1_gradients(const double* nodes_mm, int64_t n, const int32_t* tets, int64_t m, double* g, double* vols) {
+    return guarded([&] {
+        hipStream_t s = nullptr;
+        auto dn = cn::upload(nodes_mm, 3 * n, s, "nodes");
+        auto dt = cn::upload(tets, 4 * m, s, "tets");
+        cn::DeviceBuffer<double> dg(12 * m, "g"), dv(m, "vols");
+        cn::p1_gradients(dn.get(), dt.get(), m, dg.get(), dv.get(), s);
+        cn::d2h(g, dg.get(), 12 * m, s);
+        cn::d2h(vols, dv.get(), m, s);
+        cn::sync(s, "gradients");
+    });
+}
+
+int32_t cn_k_morton_keys(const double* pts, int64_t n, uint64_t* keys) {
+    return guarded([&] {
+        hipStream_t s = nullptr;
+        auto dp = cn::upload(pts, 3 * n, s, "points");
+        cn::DeviceBuffer<std::uint64_t> dk(std::max<int64_t>(n, 1), "keys");
+        cn::morton_keys(dp.get(), n, dk.get(), s);
+        cn::d2h(keys, dk.get(), n, s);
+        cn::sync(s, "morton");
+    });
+}
+
+int32_t cn_k_tet_lowest_node(const int32_t* inverse, int64_t n, const int32_t* tets, int64_t m, int32_t* lowest) {
+    return guarded([&] {
+        hipStream_t s = nullptr;
+        auto di = cn::upload(inverse, n, s, "inverse");
+        auto dt = cn::upload(tets, 4 * m, s, "tets");
+        cn::DeviceBuffer<std::uint32_t> dl(std::max<int64_t>(m, 1), "lowest");
+        cn::tet_lowest_node(di.get(), dt.get(), m, dl.get(), s);
+        cn::d2h(reinterpret_cast<std::uint32_t*>(lowest), dl.get(), m, s);
+        cn::sync(s, "tet lowest");
+    });
+}
+
+int32_t cn_k_node2corner(const int32_t* elems, int64_t n_elem, int32_t corners, int64_t n_nodes, int32_t* ptr,
+                         int32_t* idx) {
+    return guarded([&] {
+        hipStream_t s = nullptr;
+        auto de = cn::upload(elems, n_elem * corners, s, "elements");
+        cn::DeviceBuffer<std::int32_t> dp(n_nodes + 1, "ptr"), di(std::max<int64_t>(n_elem * corners, 1), "idx");
+        cn::node2corner(de.get(), n_elem, corners, n_nodes, dp.get(), di.get(), s);
+        cn::d2h(ptr, dp.get(), n_nodes + 1, s);
+        cn::d2h(idx, di.get(), n_elem * corners, s);
+        cn::sync(s, "node2corner");
+    });
+}
+
+static cn::CsrF upload_csr(int64_t n, int64_t n_cols, const int32_t* ptr, const int32_t* col, const float* val,
+                           hipStream_t s) {
+    cn::CsrF a;
+    a.n = n;
+    a.n_cols = n_cols;
+    a.nnz = ptr[n];
+    a.ptr = cn::upload(ptr, n + 1, s, "ptr");
+    a.col.reset(std::max<int64_t>(a.nnz, 1), "col");
+    a.val.reset(std::max<int64_t>(a.nnz, 1), "val");
+    cn::h2d(a.col.get(), col, a.nnz, s);
+    cn::h2d(a.val.get(), val, a.nnz, s);
+    return a;
+}
+
+int32_t cn_k_l1_dinv(int64_t n, const int32_t* ptr, const int32_t* col, const float* val, float* dinv) {
+    return guarded([&] {
+        hipStream_t s = nullptr;
+        cn::CsrF a = upload_csr(n, n, ptr, col, val, s);
+        cn::DeviceBuffer<float> d(std::max<int64_t>(n, 1), "dinv");
+        cn::l1_dinv(a.view(), d.get(), s);
+        cn::d2h(dinv, d.get(), n, s);
+        cn::sync(s, "l1");
+    });
+}
+
+int32_t cn_k_select_size4(int64_t n, const int32_t* ptr, const int32_t* col, const float* val, int32_t* agg,
+                          int64_t* n_coarse) {
+    return guarded([&] {
+        hipStream_t s = nullptr;
+        cn::CsrF a = upload_csr(n, n, ptr, col, val, s);
+        cn::DeviceBuffer<std::int32_t> d(std::max<int64_t>(n, 1), "agg");
+        *n_coarse = cn::select_size4(a.view(), d.get(), s);
+        cn::d2h(agg, d.get(), n, s);
+        cn::sync(s, "select_size4");
+    });
+}
+
+int32_t cn_k_spgemm(int64_t a_rows, int64_t b_rows, int64_t b_cols, const int32_t* a_ptr, const int32_t* a_col,
+                    const float* a_val, const int32_t* b_ptr, const int32_t* b_col, const float* b_val, int32_t* c_ptr,
+                    int32_t* c_col, float* c_val, int64_t* c_nnz) {
+    return guarded([&] {
+        hipStream_t s = nullptr;
+        cn::CsrF a = upload_csr(a_rows, b_rows, a_ptr, a_col, a_val, s);
+        cn::CsrF b = upload_csr(b_rows, b_cols, b_ptr, b_col, b_val, s);
+        cn::CsrF c = cn::spgemm(a.view(), b.view(), s);
+        *c_nnz = c.nnz;
+        cn::d2h(c_ptr, c.ptr.get(), a_rows + 1, s);
+        if (c_col) cn::d2h(c_col, c.col.get(), c.nnz, s);
+        if (c_val) cn::d2h(c_val, c.val.get(), c.nnz, s);
+        cn::sync(s, "spgemm");
+    });
+}
+
+int32_t cn_k_spmm(cn_context* c, const float* X, float* Y, int32_t K) {
+    return guarded([&] {
+        cn::require(c != nullptr, "null context");
+        cn::require(cn::padded_width(K) == K, "K must be a compiled block width");
+        const int64_t nk = (c->n - 1) * K;
+        auto dx = cn::upload(X, nk, c->s, "X");
+        cn::DeviceBuffer<float> dy(nk, "Y");
+        c->solver->spmm(dx.get(), dy.get(), K);
+        cn::check(hipGetLastError(), "spmm launch");
+        cn::d2h(Y, dy.get(), nk, c->s);
+        cn::sync(c->s, "spmm");
+    });
+}
+
+int32_t cn_k_vcycle(cn_context* c, const float* B, float* X, int32_t K) {
+    return guarded([&] {
+        cn::require(c != nullptr, "null context");
+        cn::require(cn::padded_width(K) == K, "K must be a compiled block width");
+        const int64_t nk = (c->n - 1) * K;
+        auto db = cn::upload(B, nk, c->s, "B");
+        cn::DeviceBuffer<float> dx(nk, "X");
+        c->solver->vcycle(db.get(), dx.get(), K);
+        cn::check(hipGetLastError(), "vcycle launch");
+        cn::d2h(X, dx.get(), nk, c->s);
+        cn::sync(c->s, "vcycle");
+    });
+}
+
+int32_t cn_k_dadt(const float* s_, const float* mp, const float* sn, int64_t n_dip, const float* r, int64_t n,
+                  float didt, float mu0_4pi, float* out) {
+    return guarded([&] {
+        hipStream_t s = nullptr;
+        std::vector<float> pack(static_cast<std::size_t>(10 * n_dip));
+        for (int64_t j = 0; j < n_dip; ++j) {
+            for (int q = 0; q < 3; ++q) pack[10 * j + q] = s_[3 * j + q];
+            for (int q = 0; q < 6; ++q) pack[10 * j + 3 + q] = mp[6 * j + q];
+            pack[10 * j + 9] = sn[j];
+        }
+        std::vector<double> nodes_mm(static_cast<std::size_t>(3 * n));
+        for (int64_t i = 0; i < 3 * n; ++i) nodes_mm[i] = static_cast<double>(r[i]) * 1e3;
+        auto dd = cn::upload(pack.data(), pack.size(), s, "dipoles");
+        auto dn = cn::upload(nodes_mm.data(), nodes_mm.size(), s, "targets");
+        const double zero[3] = {0.0, 0.0, 0.0};
+        auto dorig = cn::upload(zero, 3, s, "origin");
+        cn::DeviceBuffer<float> dout(std::max<int64_t>(3 * n, 1), "out");
+        cn::dadt_nbody(dd.get(), static_cast<int>(n_dip), dn.get(), n, dorig.get(), 1, didt * mu0_4pi, dout.get(), s);
+        cn::d2h(out, dout.get(), 3 * n, s);
+        cn::sync(s, "dadt");
+    });
+}
+
+int32_t cn_k_place(cn_context* c, const double* centers, const double* handles, const double* dists, int32_t n_pl,
+                   double* out, int32_t* degenerate) {
+    return guarded([&] {
+        cn::require(c != nullptr && n_pl >= 1, "bad argument");
+        std::vector<std::int32_t> deg(n_pl);
+        place_block(*c, centers, handles, dists, n_pl, out, deg.data());
+        if (degenerate) std::memcpy(degenerate, deg.data(), sizeof(std::int32_t) * n_pl);
+    });
+}
+
+int32_t cn_bench_spmm(cn_context* c, int32_t K, int32_t reps, int32_t fused, double* ms) {
+    return guarded([&] {
+        cn::require(c != nullptr && reps >= 1, "bad argument");
+        *ms = c->solver->bench_spmm(K, reps, fused != 0);
+    });
+}
+
+int32_t cn_bench_vcycle(cn_context* c, int32_t K, int32_t reps, double* ms) {
+    return guarded([&] {
+        cn::require(c != nullptr && reps >= 1, "bad argument");
+        *ms = c->solver->bench_vcycle(K, reps);
+    });
+}
+
+}  // extern "C"

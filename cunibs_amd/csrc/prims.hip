@@ -1,0 +1,61 @@
+// Setup-time primitives: stable radix sorts and integer scans over rocPRIM's device algorithms.
+// These are exact by definition (integer keys, stable order), so which library runs them cannot
+// change a result; every floating-point reduction in the library is written out by hand instead.
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "prims.hpp"
+
+namespace cn {
+
+template <typename K>
+static void sort_pairs_impl(const K* keys_in, K* keys_out, const std::int32_t* vals_in,
+                            std::int32_t* vals_out, std::int64_t n, int end_bit, hipStream_t s) {
+    if (n <= 0) return;
+    std::size_t bytes = 0;
+    check(rocprim::radix_sort_pairs(nullptr, bytes, keys_in, keys_out, vals_in, vals_out,
+                                    static_cast<std::size_t>(n), 0, end_bit, s),
+          "radix sort (size)");
+    DeviceBuffer<unsigned char> tmp(bytes ? bytes : 1, "radix sort scratch");
+    check(rocprim::radix_sort_pairs(tmp.get(), bytes, keys_in, keys_out, vals_in, vals_out,
+                                    static_cast<std::size_t>(n), 0, end_bit, s),
+          "radix sort");
+    sync(s, "radix sort");  // the scratch dies with this frame
+}
+
+void sort_pairs_u64(const std::uint64_t* keys_in, std::uint64_t* keys_out,
+                    const std::int32_t* vals_in, std::int32_t* vals_out, std::int64_t n,
+                    int end_bit, hipStream_t s) {
+    sort_pairs_impl(keys_in, keys_out, vals_in, vals_out, n, end_bit, s);
+}
+
+void sort_pairs_u32(const std::uint32_t* keys_in, std::uint32_t* keys_out,
+                    const std::int32_t* vals_in, std::int32_t* vals_out, std::int64_t n,
+                    int end_bit, hipStream_t s) {
+    sort_pairs_impl(keys_in, keys_out, vals_in, vals_out, n, end_bit, s);
+}
+
+void exclusive_scan_i32(const std::int32_t* in, std::int32_t* out, std::int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    std::size_t bytes = 0;
+    check(rocprim::exclusive_scan(nullptr, bytes, in, out, std::int32_t(0),
+                                  static_cast<std::size_t>(n), rocprim::plus<std::int32_t>(), s),
+          "scan (size)");
+    DeviceBuffer<unsigned char> tmp(bytes ? bytes : 1, "scan scratch");
+    check(rocprim::exclusive_scan(tmp.get(), bytes, in, out, std::int32_t(0),
+                                  static_cast<std::size_t>(n), rocprim::plus<std::int32_t>(), s),
+          "scan");
+    sync(s, "scan");
+}
+
+__global__ void iota_kernel(std::int32_t* out, std::int64_t n) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = static_cast<std::int32_t>(i);
+}
+
+void iota_i32(std::int32_t* out, std::int64_t n, hipStream_t s) {
+    if (const unsigned g = grid_for(n)) iota_kernel<<<g, kBlock, 0, s>>>(out, n);
+}
+
+}  // namespace cn

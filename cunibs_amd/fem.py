@@ -1,0 +1,462 @@
+"""Device-side solver context and block solve, over the C ABI.
+
+Mirrors the reference's FEM layer (src/cunibs/fem/solve.py): ``build_context`` (:690-746) becomes
+``SolverContext(mesh)``, ``solve_placements_block`` (:865-979) keeps its name and its
+(center_mm, pos_ydir_mm, distance_mm) site triples. All device state lives in the HIP library;
+this module only moves host arrays across the boundary.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+from typing import Sequence
+
+import numpy as np
+
+from cunibs_amd import _lib
+from cunibs_amd._lib import SolverConvergenceError, check, lib, ptr
+
+DEFAULT_TOLERANCE = 1e-6  # fem/solve.py:56
+DEFAULT_MAX_ITERS = 2000  # fem/solve.py:57
+BLOCK_SIZES: tuple[int, ...] = ()
+MAX_BLOCK = 32
+MU0_OVER_4PI = 1e-7  # fem/placement.py:16
+
+TISSUE_CONDUCTIVITY: dict[int, float] = {
+    1: 0.126, 2: 0.275, 3: 1.654, 4: 0.010, 5: 0.465,
+    6: 0.5, 7: 0.008, 8: 0.025, 9: 0.6, 10: 0.16,
+}  # fem/assembly.py:15-26
+GM_TAG = 2
+
+METRIC_FIELDS = (
+    "peak_magnE", "peak_x", "peak_y", "peak_z", "cog_x", "cog_y", "cog_z", "region_volume_m3",
+    "focality_0.5", "focality_0.75", "mean", "std", "p50", "p95", "p99", "p99.9",
+)
+
+
+def _blocks() -> tuple[int, ...]:
+    global BLOCK_SIZES, MAX_BLOCK
+    if not BLOCK_SIZES:
+        BLOCK_SIZES = _lib.block_sizes()
+        MAX_BLOCK = _lib.max_block()
+    return BLOCK_SIZES
+
+
+@dataclass(frozen=True)
+class AggregationParams:
+    """fem/solve.py:137-150."""
+
+    min_coarse_rows: int = 128
+    max_levels: int = 50
+    rounds: int = 2
+
+
+@dataclass(frozen=True)
+class SmootherParams:
+    """fem/solve.py:153-168."""
+
+    degree: int = 2
+    lower_ratio: float = 12.0
+
+
+@dataclass
+class PlacementResult:
+    transform: np.ndarray
+    magnE: np.ndarray | None
+    E: np.ndarray | None
+    v: np.ndarray | None
+    metrics: np.ndarray | None
+    iterations: int
+    relative_residual: float
+
+
+class SolverContext:
+    """Per-mesh GPU state: reordered mesh, FP64 assembly, grounded operator, AMG hierarchy.
+
+    The mesh is laid out along a Morton curve first (fem/solve.py:645-662); every per-node and
+    per-element array a solve returns is indexed in that order, available as ``self.nodes_mm``,
+    ``self.tet_nodes``, ``self.tet_tags`` and ``self.skin_tris``.
+    """
+
+    def __init__(
+        self,
+        nodes_mm: np.ndarray,
+        tet_nodes: np.ndarray,
+        tet_tags: np.ndarray,
+        skin_tris: np.ndarray,
+        *,
+        aggregation: AggregationParams | None = None,
+        smoother: SmootherParams | None = None,
+        keep_stiffness: bool = False,
+        conductivity: dict[int, float] | None = None,
+        region_tag: int = GM_TAG,
+    ) -> None:
+        _blocks()
+        agg = aggregation or AggregationParams()
+        sm = smoother or SmootherParams()
+        nodes = np.ascontiguousarray(nodes_mm, dtype=np.float64).reshape(-1, 3)
+        tets = np.ascontiguousarray(tet_nodes, dtype=np.int32).reshape(-1, 4)
+        tags = np.ascontiguousarray(tet_tags, dtype=np.int32).reshape(-1)
+        skin = np.ascontiguousarray(skin_tris, dtype=np.int32).reshape(-1, 3)
+        if tags.shape[0] != tets.shape[0]:
+            raise ValueError("tet_tags must have one entry per tetrahedron")
+        opts = _lib.default_options()
+        opts.min_coarse_rows = agg.min_coarse_rows
+        opts.max_levels = agg.max_levels
+        opts.rounds = agg.rounds
+        opts.degree = sm.degree
+        opts.lower_ratio = sm.lower_ratio
+        opts.keep_stiffness = int(keep_stiffness)
+        opts.region_tag = int(region_tag)
+        self._sigma = None
+        if conductivity is not None:
+            top = max(conductivity)
+            lut = np.full(top + 1, np.nan)
+            for t, s in conductivity.items():
+                lut[t] = s
+            self._sigma = lut
+            opts.conductivity = lut.ctypes.data
+            opts.n_conductivity = lut.shape[0]
+        handle = ctypes.c_void_p()
+        check(lib().cn_context_create(ptr(nodes), nodes.shape[0], ptr(tets), ptr(tags), tets.shape[0],
+                                      ptr(skin), skin.shape[0], ctypes.byref(opts), ctypes.byref(handle)),
+              "cn_context_create")
+        self._h = handle
+        info = _lib.Info()
+        check(lib().cn_context_info(self._h, ctypes.byref(info)), "cn_context_info")
+        self.info = info
+        self.n_nodes = int(info.n_nodes)
+        self.n_tets = int(info.n_tets)
+        self.n_skin = int(info.n_skin)
+        self.ground_node = int(info.ground_node)
+        self.n_region = int(info.n_region)
+        self.region_volume_m3 = float(info.region_volume_m3)
+        self.region_tag = int(region_tag)
+        self.hierarchy_sizes = [int(info.level_rows[i]) for i in range(info.n_levels + 1)]
+        self._mesh: tuple[np.ndarray, ...] | None = None
+        self._geom: tuple[np.ndarray, ...] | None = None
+
+    # ---- lifetime ----
+    def free(self) -> None:
+        if getattr(self, "_h", None):
+            lib().cn_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self) -> None:
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if not self._h:
+            raise RuntimeError("the solver context has been freed")
+        return self._h
+
+    # ---- host copies of device state (cached) ----
+    def _mesh_arrays(self):
+        if self._mesh is None:
+            nodes = np.empty((self.n_nodes, 3), np.float64)
+            tets = np.empty((self.n_tets, 4), np.int32)
+            tags = np.empty(self.n_tets, np.int32)
+            skin = np.empty((self.n_skin, 3), np.int32)
+            nperm = np.empty(self.n_nodes, np.int32)
+            tperm = np.empty(self.n_tets, np.int32)
+            check(lib().cn_context_mesh(self.handle, ptr(nodes), ptr(tets), ptr(tags), ptr(skin), ptr(nperm),
+                                        ptr(tperm)), "cn_context_mesh")
+            self._mesh = (nodes, tets, tags, skin, nperm, tperm)
+        return self._mesh
+
+    @property
+    def nodes_mm(self) -> np.ndarray:
+        return self._mesh_arrays()[0]
+
+    @property
+    def tet_nodes(self) -> np.ndarray:
+        return self._mesh_arrays()[1]
+
+    @property
+    def tet_tags(self) -> np.ndarray:
+        return self._mesh_arrays()[2]
+
+    @property
+    def skin_tris(self) -> np.ndarray:
+        return self._mesh_arrays()[3]
+
+    @property
+    def node_perm(self) -> np.ndarray:
+        return self._mesh_arrays()[4]
+
+    @property
+    def tet_perm(self) -> np.ndarray:
+        return self._mesh_arrays()[5]
+
+    def _geometry(self):
+        if self._geom is None:
+            vols = np.empty(self.n_tets, np.float32)
+            bary = np.empty((self.n_tets, 3), np.float64)
+            g = np.empty((self.n_tets, 4, 3), np.float32)
+            nvc = np.empty(self.n_tets, np.float32)
+            check(lib().cn_context_geometry(self.handle, ptr(vols), ptr(bary), ptr(g), ptr(nvc)),
+                  "cn_context_geometry")
+            self._geom = (vols, bary, g, nvc)
+        return self._geom
+
+    @property
+    def vols(self) -> np.ndarray:
+        return self._geometry()[0]
+
+    @property
+    def tet_barycenters_mm(self) -> np.ndarray:
+        return self._geometry()[1]
+
+    @property
+    def g(self) -> np.ndarray:
+        return self._geometry()[2]
+
+    @property
+    def neg_vc(self) -> np.ndarray:
+        return self._geometry()[3]
+
+    def stiffness(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        nnz = int(self.info.nnz_full)
+        indptr = np.empty(self.n_nodes + 1, np.int32)
+        indices = np.empty(nnz, np.int32)
+        data = np.empty(nnz, np.float64)
+        check(lib().cn_context_stiffness(self.handle, ptr(indptr), ptr(indices), ptr(data)),
+              "cn_context_stiffness")
+        return indptr, indices, data
+
+    def reduced(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        nnz = int(self.info.nnz_reduced)
+        p = np.empty(self.n_nodes, np.int32)
+        c = np.empty(nnz, np.int32)
+        v = np.empty(nnz, np.float64)
+        check(lib().cn_context_reduced(self.handle, ptr(p), ptr(c), ptr(v)), "cn_context_reduced")
+        return p, c, v
+
+    def level(self, i: int) -> dict[str, np.ndarray | int]:
+        n = ctypes.c_int64()
+        nnz = ctypes.c_int64()
+        nc = ctypes.c_int64()
+        pn = ctypes.c_int64()
+        check(lib().cn_context_level(self.handle, i, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(nc),
+                                     ctypes.byref(pn), *([None] * 8)), "cn_context_level")
+        n, nnz, nc, pn = n.value, nnz.value, nc.value, pn.value
+        out = {"n": n, "nnz": nnz, "n_coarse": nc, "p_nnz": pn,
+               "a_ptr": np.empty(n + 1, np.int32), "a_col": np.empty(nnz, np.int32),
+               "a_val": np.empty(nnz, np.float32)}
+        coarse = i == int(self.info.n_levels)
+        if not coarse:
+            out.update({"dinv": np.empty(n, np.float32), "agg": np.empty(n, np.int32),
+                        "p_ptr": np.empty(n + 1, np.int32), "p_col": np.empty(pn, np.int32),
+                        "p_val": np.empty(pn, np.float32)})
+        check(lib().cn_context_level(
+            self.handle, i, None, None, None, None, ptr(out["a_ptr"]), ptr(out["a_col"]), ptr(out["a_val"]),
+            ptr(out.get("dinv")), ptr(out.get("agg")), ptr(out.get("p_ptr")), ptr(out.get("p_col")),
+            ptr(out.get("p_val"))), "cn_context_level")
+        return out
+
+    def skin_normals(self) -> np.ndarray:
+        out = np.empty((self.n_skin, 3), np.float64)
+        check(lib().cn_context_skin_normals(self.handle, ptr(out)), "cn_context_skin_normals")
+        return out
+
+    # ---- kernels on the context ----
+    def spmm(self, X: np.ndarray) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        K = X.shape[1]
+        Y = np.empty_like(X)
+        check(lib().cn_k_spmm(self.handle, ptr(X), ptr(Y), K), "cn_k_spmm")
+        return Y
+
+    def vcycle(self, B: np.ndarray) -> np.ndarray:
+        B = np.ascontiguousarray(B, dtype=np.float32)
+        X = np.empty_like(B)
+        check(lib().cn_k_vcycle(self.handle, ptr(B), ptr(X), B.shape[1]), "cn_k_vcycle")
+        return X
+
+    def place(self, centers, handles, dists) -> tuple[np.ndarray, np.ndarray]:
+        c = np.ascontiguousarray(centers, np.float64).reshape(-1, 3)
+        h = None if handles is None else np.ascontiguousarray(handles, np.float64).reshape(-1, 3)
+        d = np.ascontiguousarray(dists, np.float64).reshape(-1)
+        out = np.empty((c.shape[0], 4, 4), np.float64)
+        deg = np.empty(c.shape[0], np.int32)
+        check(lib().cn_k_place(self.handle, ptr(c), ptr(h), ptr(d), c.shape[0], ptr(out), ptr(deg)), "cn_k_place")
+        return out, deg
+
+    def bench_spmm(self, K: int, reps: int = 20, fused_gram: bool = True) -> float:
+        ms = ctypes.c_double()
+        check(lib().cn_bench_spmm(self.handle, K, reps, int(fused_gram), ctypes.byref(ms)), "cn_bench_spmm")
+        return float(ms.value)
+
+    def bench_vcycle(self, K: int, reps: int = 10) -> float:
+        ms = ctypes.c_double()
+        check(lib().cn_bench_vcycle(self.handle, K, reps, ctypes.byref(ms)), "cn_bench_vcycle")
+        return float(ms.value)
+
+
+def solve_placements_block(
+    ctx: SolverContext,
+    dip_pos_m: np.ndarray,
+    dip_moment: np.ndarray,
+    sites: Sequence[tuple[np.ndarray, np.ndarray, float]],
+    didt: float,
+    *,
+    magnitude: bool = True,
+    vectors: bool = False,
+    potential: bool = False,
+    metrics: bool = True,
+    tolerance: float = DEFAULT_TOLERANCE,
+    max_iters: int = DEFAULT_MAX_ITERS,
+    x0: np.ndarray | None = None,
+    extra: dict | None = None,
+    use_graph: bool = True,
+) -> list[PlacementResult]:
+    """Solve up to MAX_BLOCK placements as one block (fem/solve.py:865-979)."""
+    _blocks()
+    k = len(sites)
+    if k < 1 or k > MAX_BLOCK:
+        raise ValueError(f"solve_placements_block: k={k} must be in [1, {MAX_BLOCK}]")
+    pos = np.ascontiguousarray(dip_pos_m, np.float64).reshape(-1, 3)
+    mom = np.ascontiguousarray(dip_moment, np.float64).reshape(-1, 3)
+    if pos.shape != mom.shape:
+        raise ValueError("dipole positions and moments must have the same shape")
+    centers = np.ascontiguousarray([np.asarray(s[0], np.float64).reshape(3) for s in sites])
+    handles = np.ascontiguousarray([np.asarray(s[1], np.float64).reshape(3) for s in sites])
+    dists = np.ascontiguousarray([float(s[2]) for s in sites], np.float64)
+    m, n = ctx.n_tets, ctx.n_nodes
+    out = _lib.BlockOutputs()
+    tf = np.empty((k, 4, 4), np.float64)
+    it = np.empty(k, np.int32)
+    rel = np.empty(k, np.float64)
+    mag = np.empty((k, m), np.float32) if magnitude else None
+    E = np.empty((k, m, 3), np.float32) if vectors else None
+    v = np.empty((k, n), np.float64) if potential else None
+    met = np.empty((k, _lib.METRICS_LEN), np.float64) if metrics else None
+    out.transforms = ptr(tf)
+    out.iterations = ptr(it)
+    out.relative_residual = ptr(rel)
+    out.magnE = ptr(mag)
+    out.E = ptr(E)
+    out.v = ptr(v)
+    out.metrics = ptr(met)
+    ex = {}
+    if extra is not None:
+        for key, shape, dt in (("dadt_elm", (k, m, 3), np.float32), ("dadt_nodes", (k, n, 3), np.float32),
+                               ("rhs", (k, n), np.float32)):
+            if key in extra:
+                ex[key] = np.empty(shape, dt)
+                setattr(out, key, ptr(ex[key]))
+    opt = _lib.default_solve_options()
+    opt.tolerance = float(tolerance)
+    opt.max_iters = int(max_iters)
+    opt.use_graph = int(use_graph)
+    x0a = None
+    if x0 is not None:
+        x0a = np.ascontiguousarray(x0, np.float64).reshape(k, n - 1)
+    status = lib().cn_solve_block(ctx.handle, ptr(pos), ptr(mom), pos.shape[0], ptr(centers), ptr(handles),
+                                  ptr(dists), k, float(didt), ctypes.byref(opt), ptr(x0a), ctypes.byref(out))
+    check(status, "cn_solve_block")
+    if extra is not None:
+        extra.update(ex)
+        extra["total_iterations"] = int(out.total_iterations)
+        extra["refinements"] = int(out.refinements)
+    return [
+        PlacementResult(
+            transform=tf[i], magnE=None if mag is None else mag[i], E=None if E is None else E[i],
+            v=None if v is None else v[i], metrics=None if met is None else met[i],
+            iterations=int(it[i]), relative_residual=float(rel[i]),
+        )
+        for i in range(k)
+    ]
+
+
+def metrics_dict(rec: np.ndarray, region: str = "gray_matter") -> dict:
+    """A metric record (CN_METRICS_LEN doubles) as the reference's FieldMetrics (metrics.py:28-37)."""
+    return {
+        "region": region,
+        "peak_magnE": float(rec[0]),
+        "peak_location_mm": np.array(rec[1:4], dtype=np.float64),
+        "center_of_gravity_mm": np.array(rec[4:7], dtype=np.float64),
+        "region_volume_m3": float(rec[7]),
+        "focality_m3": {"0.5": float(rec[8]), "0.75": float(rec[9])},
+        "distribution": {"mean": float(rec[10]), "std": float(rec[11]), "p50": float(rec[12]),
+                         "p95": float(rec[13]), "p99": float(rec[14]), "p99.9": float(rec[15])},
+    }
+
+
+def dadt_nbody(s: np.ndarray, mp: np.ndarray, sn: np.ndarray, r: np.ndarray, didt: float,
+               mu0_4pi: float = MU0_OVER_4PI) -> np.ndarray:
+    """Raw dipole N-body kernel (bindings.cpp:194-206 dadt_nbody)."""
+    s = np.ascontiguousarray(s, np.float32)
+    mp = np.ascontiguousarray(mp, np.float32)
+    sn = np.ascontiguousarray(sn, np.float32)
+    r = np.ascontiguousarray(r, np.float32)
+    out = np.empty_like(r)
+    check(lib().cn_k_dadt(ptr(s), ptr(mp), ptr(sn), s.shape[0], ptr(r), r.shape[0], float(didt), float(mu0_4pi),
+                          ptr(out)), "cn_k_dadt")
+    return out
+
+
+def p1_gradients(nodes_mm: np.ndarray, tet_nodes: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    nodes = np.ascontiguousarray(nodes_mm, np.float64)
+    tets = np.ascontiguousarray(tet_nodes, np.int32)
+    g = np.empty((tets.shape[0], 4, 3), np.float64)
+    v = np.empty(tets.shape[0], np.float64)
+    check(lib().cn_k_p1_gradients(ptr(nodes), nodes.shape[0], ptr(tets), tets.shape[0], ptr(g), ptr(v)),
+          "cn_k_p1_gradients")
+    return g, v
+
+
+def morton_keys(points_mm: np.ndarray) -> np.ndarray:
+    pts = np.ascontiguousarray(points_mm, np.float64)
+    keys = np.empty(pts.shape[0], np.uint64)
+    check(lib().cn_k_morton_keys(ptr(pts), pts.shape[0], ptr(keys)), "cn_k_morton_keys")
+    return keys
+
+
+def node2corner(elems: np.ndarray, n_nodes: int) -> tuple[np.ndarray, np.ndarray]:
+    e = np.ascontiguousarray(elems, np.int32)
+    ptr_ = np.empty(n_nodes + 1, np.int32)
+    idx = np.empty(e.size, np.int32)
+    check(lib().cn_k_node2corner(ptr(e), e.shape[0], e.shape[1], n_nodes, ptr(ptr_), ptr(idx)), "cn_k_node2corner")
+    return ptr_, idx
+
+
+def l1_dinv(n, a_ptr, a_col, a_val) -> np.ndarray:
+    out = np.empty(n, np.float32)
+    check(lib().cn_k_l1_dinv(n, ptr(np.ascontiguousarray(a_ptr, np.int32)), ptr(np.ascontiguousarray(a_col, np.int32)),
+                             ptr(np.ascontiguousarray(a_val, np.float32)), ptr(out)), "cn_k_l1_dinv")
+    return out
+
+
+def select_size4(n, a_ptr, a_col, a_val) -> tuple[np.ndarray, int]:
+    agg = np.empty(n, np.int32)
+    nc = ctypes.c_int64()
+    check(lib().cn_k_select_size4(n, ptr(np.ascontiguousarray(a_ptr, np.int32)),
+                                  ptr(np.ascontiguousarray(a_col, np.int32)),
+                                  ptr(np.ascontiguousarray(a_val, np.float32)), ptr(agg), ctypes.byref(nc)),
+          "cn_k_select_size4")
+    return agg, int(nc.value)
+
+
+def spgemm(a_ptr, a_col, a_val, b_ptr, b_col, b_val, b_cols: int):
+    a_ptr = np.ascontiguousarray(a_ptr, np.int32)
+    a_col = np.ascontiguousarray(a_col, np.int32)
+    a_val = np.ascontiguousarray(a_val, np.float32)
+    b_ptr = np.ascontiguousarray(b_ptr, np.int32)
+    b_col = np.ascontiguousarray(b_col, np.int32)
+    b_val = np.ascontiguousarray(b_val, np.float32)
+    n = a_ptr.shape[0] - 1
+    c_ptr = np.empty(n + 1, np.int32)
+    nnz = ctypes.c_int64()
+    args = (n, b_ptr.shape[0] - 1, b_cols, ptr(a_ptr), ptr(a_col), ptr(a_val), ptr(b_ptr), ptr(b_col), ptr(b_val))
+    check(lib().cn_k_spgemm(*args, ptr(c_ptr), None, None, ctypes.byref(nnz)), "cn_k_spgemm")
+    c_col = np.empty(nnz.value, np.int32)
+    c_val = np.empty(nnz.value, np.float32)
+    check(lib().cn_k_spgemm(*args, ptr(c_ptr), ptr(c_col), ptr(c_val), ctypes.byref(nnz)), "cn_k_spgemm")
+    return c_ptr, c_col, c_val

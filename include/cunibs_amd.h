@@ -1,0 +1,200 @@
+/*
+ * cunibs_amd: C ABI of the MI355X TMS E-field FEM library.
+ *
+ * The reference binds its CUDA kernels through a nanobind module (cuNIBS
+ * src/cunibs/solver/bindings.cpp:30-676) that takes CuPy device arrays. This ABI replaces it with
+ * plain pointers and sizes, no exceptions across the boundary, and two layers:
+ *
+ *   1. A coarse layer the Python front end drives: a context built once per head mesh
+ *      (replacing fem/solve.py:690-746, build_context) and a block solve of up to 32 placements
+ *      (replacing fem/solve.py:865-979, solve_placements_block, and the metric reduction of
+ *      simulation.py:330-376). Inputs and outputs are host arrays; every call is synchronous.
+ *   2. Kernel-level entry points mirroring individual reference bindings, used by parity tests.
+ *      They also take host arrays and copy in and out.
+ *
+ * Errors: every function returns a status code; cn_last_error() gives the message of the calling
+ * thread's last failure. CN_EINVAL maps to Python ValueError, CN_EHIP to RuntimeError,
+ * CN_ENOCONV to SolverConvergenceError (fem/solve.py:368-392).
+ *
+ * Threading: one thread at a time per context; contexts are independent.
+ *
+ * Indices are int32 and 0-based; coordinates are millimetres; arrays are C-contiguous.
+ */
+#ifndef CUNIBS_AMD_H
+#define CUNIBS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CN_OK 0
+#define CN_EINVAL 1
+#define CN_EHIP 2
+#define CN_ENOCONV 3
+#define CN_EINTERNAL 4
+
+/* Length of one placement's region-metric record, cn_solve_block's `metrics` output. Layout:
+ * [0] peak |E|, [1..3] peak location mm, [4..6] |E|-weighted centre of gravity mm,
+ * [7] region volume m^3, [8] focality at 0.5 x p99.9 (m^3), [9] focality at 0.75 x p99.9,
+ * [10] volume-weighted mean, [11] std, [12..15] p50, p95, p99, p99.9 (metrics.py:209-243). */
+#define CN_METRICS_LEN 16
+
+typedef struct cn_context cn_context;
+
+/* Message of this thread's last failed call ("" if none). */
+const char* cn_last_error(void);
+int32_t cn_version(void);
+/* Compiled block widths, increasing (solver/__init__.py BLOCK_SIZES, bindings.cpp:31-35).
+ * Writes up to cap entries, returns the count. */
+int32_t cn_block_sizes(int32_t* out, int32_t cap);
+/* Largest compiled block width (fem/solve.py:505 MAX_BLOCK). */
+int32_t cn_max_block(void);
+
+typedef struct {
+    int32_t min_coarse_rows; /* fem/solve.py:146, default 128 */
+    int32_t max_levels;      /* default 50 */
+    int32_t rounds;          /* pairwise passes per level, default 2 */
+    int32_t degree;          /* Chebyshev degree, default 2 (1..6) */
+    float lower_ratio;       /* Chebyshev interval [1/lower_ratio, 1], default 12 */
+    int32_t keep_stiffness;  /* keep the full FP64 stiffness for cn_context_stiffness */
+    const double* conductivity; /* sigma by tag (NaN = unknown tag); NULL: fem/assembly.py:15-26 */
+    int32_t n_conductivity;
+    int32_t region_tag;      /* tag summarised by cn_solve_block's metrics; default 2 (GM) */
+} cn_options;
+
+void cn_default_options(cn_options* out);
+
+/* Build the per-mesh state on the GPU: Morton layout, FP64 P1 assembly, grounding, AMG
+ * hierarchy, skin normals (fem/solve.py:690-746). Rejects zero-volume elements and unknown tags
+ * with CN_EINVAL. */
+int32_t cn_context_create(const double* nodes_mm, int64_t n_nodes, const int32_t* tet_nodes,
+                          const int32_t* tet_tags, int64_t n_tets, const int32_t* skin_tris,
+                          int64_t n_skin, const cn_options* opts, cn_context** out);
+void cn_context_destroy(cn_context* ctx);
+
+#define CN_MAX_LEVELS 32
+
+typedef struct {
+    int64_t n_nodes, n_tets, n_skin;
+    int64_t n_reduced, nnz_reduced, nnz_full;
+    int32_t ground_node;
+    int32_t n_levels; /* smoothed levels; the coarsest operator follows them */
+    int64_t level_rows[CN_MAX_LEVELS + 1];
+    int64_t level_nnz[CN_MAX_LEVELS + 1];
+    int64_t n_region;
+    double region_volume_m3;
+    int32_t n_cu;
+} cn_info;
+
+int32_t cn_context_info(const cn_context* ctx, cn_info* out);
+
+/* The reordered mesh every result is indexed by (fem/solve.py:645-662), and the permutations
+ * from the input order. Any pointer may be NULL. */
+int32_t cn_context_mesh(const cn_context* ctx, double* nodes_mm, int32_t* tet_nodes,
+                        int32_t* tet_tags, int32_t* skin_tris, int32_t* node_perm,
+                        int32_t* tet_perm);
+/* Per-element resident arrays: volumes (m^3, FP32), barycentres (mm, FP64), P1 gradients
+ * (m, 4, 3) FP32, -vol*sigma FP32. Any pointer may be NULL. */
+int32_t cn_context_geometry(const cn_context* ctx, float* vols, double* barycenters_mm, float* g,
+                            float* neg_vc);
+/* Full FP64 stiffness CSR (needs keep_stiffness): indptr (n+1), indices, data (nnz_full). */
+int32_t cn_context_stiffness(const cn_context* ctx, int32_t* indptr, int32_t* indices, double* data);
+/* Reduced (grounded) operator: ptr (n_red+1), col, FP64 values (nnz_reduced). */
+int32_t cn_context_reduced(const cn_context* ctx, int32_t* ptr, int32_t* col, double* val);
+/* One hierarchy level: operator A (FP32 CSR), l1 dinv, aggregates, prolongator P (FP32 CSR).
+ * Call with NULL arrays first to read the sizes. level == n_levels gives the coarsest operator
+ * (A only). */
+int32_t cn_context_level(const cn_context* ctx, int32_t level, int64_t* n_rows, int64_t* nnz,
+                         int64_t* n_coarse, int64_t* p_nnz, int32_t* a_ptr, int32_t* a_col,
+                         float* a_val, float* dinv, int32_t* agg, int32_t* p_ptr, int32_t* p_col,
+                         float* p_val);
+/* Smoothed skin-triangle normals (n_skin, 3) FP64 (fem/solve.py:665-687). */
+int32_t cn_context_skin_normals(const cn_context* ctx, double* out);
+
+typedef struct {
+    double tolerance;  /* relative true residual per column, default 1e-6 */
+    int32_t max_iters; /* default 2000 */
+    int32_t max_refine;
+    double drop_tol;
+    int32_t use_graph;
+    int32_t check_every;
+} cn_solve_options;
+
+void cn_default_solve_options(cn_solve_options* out);
+
+/* Host outputs of a block solve, placement-major; NULL = not wanted. */
+typedef struct {
+    double* transforms;        /* (k, 4, 4) coil-to-head affine, mm */
+    float* magnE;              /* (k, n_tets) */
+    float* E;                  /* (k, n_tets, 3) */
+    double* v;                 /* (k, n_nodes), ground node 0 */
+    double* metrics;           /* (k, CN_METRICS_LEN), region_tag's summary */
+    int32_t* iterations;       /* (k) inner PCG iterations until each column froze */
+    double* relative_residual; /* (k) final FP64 true residual / ||b|| */
+    float* dadt_elm;           /* (k, n_tets, 3) element dA/dt */
+    float* dadt_nodes;         /* (k, n_nodes, 3) nodal dA/dt */
+    float* rhs;                /* (k, n_nodes) right-hand side before grounding */
+    int32_t* degenerate;       /* (k) 1 where the handle left the frame undefined */
+    int32_t total_iterations;  /* out: inner iterations of the block */
+    int32_t refinements;       /* out: FP64 refinement rounds */
+} cn_block_outputs;
+
+/* Solve k <= 32 placements as one block (fem/solve.py:865-979): coil frames, dA/dt, RHS, block
+ * PCG, E reconstruction (raw recovery), region metrics. Dipoles in metres / A m^2; centres and
+ * handles (k, 3) mm; distances (k) mm; didt in A/s. A degenerate handle is CN_EINVAL naming the
+ * placement (fem/placement.py:182-215). x0 (optional, (k, n_reduced) FP64) warm-starts PCG. */
+int32_t cn_solve_block(cn_context* ctx, const double* dip_pos_m, const double* dip_moment,
+                       int64_t n_dip, const double* centers_mm, const double* handles_mm,
+                       const double* distances_mm, int32_t k, double didt,
+                       const cn_solve_options* opt, const double* x0, cn_block_outputs* out);
+
+/* ---- kernel-level entry points (reference binding each one stands in for) ---- */
+
+/* bindings.cpp:175-192 p1_gradients -> (m,4,3) gradients 1/m and volumes m^3, FP64. */
+int32_t cn_k_p1_gradients(const double* nodes_mm, int64_t n_nodes, const int32_t* tet_nodes,
+                          int64_t n_tets, double* g, double* vols);
+/* fem/solve.py:315-323 Morton keys. */
+int32_t cn_k_morton_keys(const double* points_mm, int64_t n, uint64_t* keys);
+/* bindings.cpp:246-259 tet_lowest_node. */
+int32_t cn_k_tet_lowest_node(const int32_t* inverse, int64_t n_nodes, const int32_t* tet_nodes,
+                             int64_t n_tets, int32_t* lowest);
+/* fem/assembly.py:142-162 build_node2corner: ptr (n_nodes+1), idx (n_elem*corners). */
+int32_t cn_k_node2corner(const int32_t* elems, int64_t n_elem, int32_t corners, int64_t n_nodes,
+                         int32_t* ptr, int32_t* idx);
+/* bindings.cpp:354-370 l1_dinv (FP32 CSR). */
+int32_t cn_k_l1_dinv(int64_t n, const int32_t* ptr, const int32_t* col, const float* val,
+                     float* dinv);
+/* bindings.cpp:85-104 select_size4: one SIZE_4 pass; returns the aggregate count in *n_coarse. */
+int32_t cn_k_select_size4(int64_t n, const int32_t* ptr, const int32_t* col, const float* val,
+                          int32_t* agg, int64_t* n_coarse);
+/* Sparse product C = A B (FP32, fixed association), the Galerkin building block that replaces
+ * cupyx SpGEMM (fem/solve.py:119-133). Call with c_col == NULL to read *c_nnz (c_ptr filled). */
+int32_t cn_k_spgemm(int64_t a_rows, int64_t b_rows, int64_t b_cols, const int32_t* a_ptr,
+                    const int32_t* a_col, const float* a_val, const int32_t* b_ptr,
+                    const int32_t* b_col, const float* b_val, int32_t* c_ptr, int32_t* c_col,
+                    float* c_val, int64_t* c_nnz);
+/* Y = A X on the context's reduced FP32 operator; X, Y (n_reduced, K), K in the block sizes. */
+int32_t cn_k_spmm(cn_context* ctx, const float* X, float* Y, int32_t K);
+/* X = M B, one zero-guess V-cycle (vcycle.cu:650-663 apply / apply_block). */
+int32_t cn_k_vcycle(cn_context* ctx, const float* B, float* X, int32_t K);
+/* bindings.cpp:194-206 dadt_nbody: targets r (n,3) metres relative to the coil origin, dipoles
+ * s (D,3), packed moments mp (D,6) = [m | m x s], |s|^2 (D); out (n,3). FP32. */
+int32_t cn_k_dadt(const float* s, const float* mp, const float* sn, int64_t n_dip, const float* r,
+                  int64_t n, float didt, float mu0_4pi, float* out);
+/* bindings.cpp:657-675 place_transforms on the context's scalp; handles may be NULL. */
+int32_t cn_k_place(cn_context* ctx, const double* centers_mm, const double* handles_mm,
+                   const double* distances_mm, int32_t n_pl, double* out, int32_t* degenerate);
+
+/* Average device time (ms) of `reps` launches of the block solve's SpMM (with its fused Gram)
+ * at width K, timed with HIP events on the library's stream. For roofline reporting. */
+int32_t cn_bench_spmm(cn_context* ctx, int32_t K, int32_t reps, int32_t fused_gram, double* ms);
+/* Same for one V-cycle at width K. */
+int32_t cn_bench_vcycle(cn_context* ctx, int32_t K, int32_t reps, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUNIBS_AMD_H */
