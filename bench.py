@@ -1,0 +1,190 @@
+"""Placements/s of the TMS E-field path on one MI355X.
+
+Workload: the 1.5 mm perturbed 5-shell head (~1.0M nodes, ~5.8M tetrahedra) and the 16 x 16 cap
+grid of 256 placements, solved to relative residual 1e-6 in blocks of ``--block-k`` placements.
+A step is one block: coil frames, dA/dt N-body, RHS, block PCG, E reconstruction and the
+gray-matter metrics. Inputs are resident on the GPU (the context is built before the timed region).
+
+Printed as one JSON line. Besides the contract fields it carries the plain single-placement GPU
+PCG rate (k = 1) measured in the same run, the GM |E| error of sample placements against the FP64
+oracle (CG to 1e-12), the SpMM roofline (``roofline``) and the CPU oracle timed on this host
+(``cpu_baseline``).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "placements/s @ rel.res 1e-6, GM |E| err<=1e-3 vs FP64; batched+warm vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmm_bytes(n: int, nnz: int, k: int) -> float:
+    """Algorithmic bytes of Y = A X, FP32 values, int32 indices: matrix once, X once, Y once."""
+    return 8.0 * nnz + 4.0 * (n + 1) + 8.0 * n * k
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=32)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--block-k", type=int, default=8)
+    ap.add_argument("--spacing", type=float, default=1.5)
+    ap.add_argument("--plain", type=int, default=16, help="placements timed at k=1")
+    ap.add_argument("--accuracy", type=int, default=2, help="placements checked against the FP64 oracle")
+    ap.add_argument("--cpu-sample", type=int, default=2, help="placements the CPU oracle solves")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the placement sweep shards "
+              "over GPUs by placement, see DESIGN.md", file=sys.stderr)
+        return 2
+
+    from cunibs_amd import fem, synthetic
+
+    t0 = time.time()
+    head = synthetic.shell_head(args.spacing, perturb_rms_mm=2.0, seed=1)
+    coil = synthetic.figure8_coil(seed=0)
+    sites = synthetic.grid_sites()
+    log(f"mesh {head.nodes_mm.shape[0]} nodes {head.tet_nodes.shape[0]} tets in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    ctx = fem.SolverContext(head.nodes_mm, head.tet_nodes, head.tet_tags, head.skin_tris)
+    setup_s = time.time() - t0
+    log(f"context {setup_s:.2f}s hierarchy {ctx.hierarchy_sizes}")
+    k = args.block_k
+    pos, mom = coil.positions_m, coil.moments
+
+    def block(i0: int, kk: int):
+        chunk = [sites[(i0 + j) % len(sites)] for j in range(kk)]
+        return fem.solve_placements_block(ctx, pos, mom, chunk, 1e6, magnitude=False, metrics=True)
+
+    for w in range(args.warmup):
+        block(w * k, k)
+    iters = []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        res = block(s * k, k)
+        iters.extend(r.iterations for r in res)
+    elapsed = time.perf_counter() - t0
+    value = args.steps * k / elapsed
+    log(f"batched k={k}: {value:.1f} placements/s, {1e3 * elapsed / args.steps:.1f} ms/step, "
+        f"iterations mean {np.mean(iters):.1f}")
+
+    # Plain single-placement GPU PCG on the same sites.
+    plain_iters = []
+    block(0, 1)
+    t0 = time.perf_counter()
+    for i in range(args.plain):
+        res = block(i, 1)
+        plain_iters.append(res[0].iterations)
+    plain = args.plain / (time.perf_counter() - t0)
+    log(f"plain k=1: {plain:.1f} placements/s, iterations mean {np.mean(plain_iters):.1f}")
+
+    rounds = {}
+    for kk in (k, 1):
+        ex: dict = {}
+        fem.solve_placements_block(ctx, pos, mom, sites[:kk], 1e6, magnitude=False, metrics=False, extra=ex)
+        rounds[kk] = {"iterations": ex["round_iterations"], "true_residual": ex["round_residual"]}
+        log(f"rounds k={kk}: {rounds[kk]}")
+
+    # Roofline of the block solve's dominant kernel: SpMM with the fused FP64 Gram.
+    n_red = int(ctx.info.n_reduced)
+    nnz = int(ctx.info.nnz_reduced)
+    K = k if k in fem.BLOCK_SIZES else min(b for b in fem.BLOCK_SIZES if b >= k)
+    spmm_ms = ctx.bench_spmm(K, 50, True)
+    achieved = spmm_bytes(n_red, nnz, K) / (spmm_ms * 1e-3) / 1e9
+    spmm_plain_ms = ctx.bench_spmm(K, 50, False)
+    vc_ms = ctx.bench_vcycle(K, 10)
+    log(f"spmm K={K}: fused {spmm_ms * 1e3:.1f} us ({achieved:.0f} GB/s), plain {spmm_plain_ms * 1e3:.1f} us, "
+        f"vcycle {vc_ms * 1e3:.1f} us")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "placements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (fp64 Gram/refinement)",
+        "data": "synthetic (seeded perturbed 5-shell head, figure-8 coil)",
+        "config": {
+            "workload": f"{args.spacing} mm perturbed 5-shell head, 16x16 cap-grid sweep, block_k={k}",
+            "n_nodes": ctx.n_nodes, "n_tets": ctx.n_tets, "nnz_reduced": nnz,
+            "hierarchy": ctx.hierarchy_sizes, "block_k": k, "tolerance": 1e-6, "recovery": "raw",
+            "warm_start": "none",
+        },
+        "plain_placements_per_s": round(plain, 3),
+        "speedup_vs_plain": round(value / plain, 3),
+        "iterations_mean": round(float(np.mean(iters)), 2),
+        "plain_iterations_mean": round(float(np.mean(plain_iters)), 2),
+        "setup_s": round(setup_s, 3),
+        "refinement_rounds": rounds,
+        "roofline": {
+            "kernel": f"spmm_kernel<{K},gram>",
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "avg_us": round(spmm_ms * 1e3, 2),
+            "algorithmic_bytes": spmm_bytes(n_red, nnz, K),
+            "spmm_no_gram_us": round(spmm_plain_ms * 1e3, 2),
+            "vcycle_us": round(vc_ms * 1e3, 1),
+        },
+    }
+
+    if not args.no_cpu:
+        import oracle
+
+        ncores = min(os.cpu_count() or 1, 16)
+        oracle.set_threads(ncores)
+        t0 = time.time()
+        octx = oracle.build_context(head.nodes_mm, head.tet_nodes, head.tet_tags, head.skin_tris)
+        log(f"oracle context {time.time() - t0:.1f}s")
+        # Accuracy: GM |E| of sample placements against the FP64 oracle solved to 1e-12.
+        errs = []
+        gm = octx.tet_tags == fem.GM_TAG
+        for i in range(args.accuracy):
+            gres = block(i * 37, 1)[0]
+            ref = oracle.solve_site(octx, pos, mom, sites[(i * 37) % len(sites)], 1e6, tol=1e-12)
+            g = fem.solve_placements_block(ctx, pos, mom, [sites[(i * 37) % len(sites)]], 1e6)[0]
+            errs.append(float(np.linalg.norm(g.magnE[gm] - ref["magnE"][gm]) / np.linalg.norm(ref["magnE"][gm])))
+            del gres
+        out["gm_magnE_rel_err_vs_fp64"] = max(errs) if errs else None
+        # CPU baseline: the oracle's FP64 AMG-PCG path to 1e-6, per placement.
+        t0 = time.time()
+        for i in range(args.cpu_sample):
+            oracle.solve_site(octx, pos, mom, sites[i], 1e6, tol=1e-6)
+        cpu_s = time.time() - t0
+        out["cpu_baseline"] = {
+            "value": round(args.cpu_sample / cpu_s, 4),
+            "unit": "placements/s",
+            "cores": ncores,
+            "kind": "port",
+            "sample": f"{args.cpu_sample} placements of the same sweep (dA/dt, RHS, FP64 AMG-PCG to 1e-6, "
+                      f"reconstruction) after an untimed oracle setup",
+        }
+        log(f"cpu baseline {out['cpu_baseline']['value']} placements/s on {ncores} threads; GM err {errs}")
+    print(json.dumps(out), flush=True)
+    ctx.free()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

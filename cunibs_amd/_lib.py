@@ -64,6 +64,7 @@ class BlockOutputs(ctypes.Structure):
         ("transforms", _P), ("magnE", _P), ("E", _P), ("v", _P), ("metrics", _P),
         ("iterations", _P), ("relative_residual", _P), ("dadt_elm", _P), ("dadt_nodes", _P),
         ("rhs", _P), ("degenerate", _P), ("total_iterations", _I32), ("refinements", _I32),
+        ("round_iterations", _I32 * 8), ("round_residual", _D * 8),
     ]
 
 
@@ -92,6 +93,7 @@ _SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "cn_k_select_size4": (_I32, [_I64, _P, _P, _P, _P, _P]),
     "cn_k_spgemm": (_I32, [_I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "cn_k_spmm": (_I32, [_P, _P, _P, _I32]),
+    "cn_k_spmm_diff": (_I32, [_P, _P, _P, _I32]),
     "cn_k_vcycle": (_I32, [_P, _P, _P, _I32]),
     "cn_k_dadt": (_I32, [_P, _P, _P, _I64, _P, _I64, _F, _F, _P]),
     "cn_k_place": (_I32, [_P, _P, _P, _P, _I32, _P, _P]),

@@ -278,47 +278,47 @@ __device__ __forceinline__ unsigned hash_slot(int key, unsigned mask) {
     return (static_cast<unsigned>(key) * 2654435761u) & mask;
 }
 
-// Inserts key; returns its slot. The table never fills: capacity is at least twice the row's
-// candidate count.
-__device__ __forceinline__ int hash_insert(int* keys, unsigned mask, int key, int* inserted) {
+// Inserts key; returns its slot, or -1 when `cap` probes found neither the key nor a free slot
+// (the table is full). The probe count is bounded, so a full table cannot spin.
+__device__ __forceinline__ int hash_insert(int* keys, int cap, int key, int* inserted) {
+    const unsigned mask = static_cast<unsigned>(cap - 1);
     unsigned s = hash_slot(key, mask);
-    while (true) {
+    *inserted = 0;
+    for (int probe = 0; probe < cap; ++probe) {
         const int prev = atomicCAS(&keys[s], -1, key);
         if (prev == -1) {
             *inserted = 1;
             return static_cast<int>(s);
         }
-        if (prev == key) {
-            *inserted = 0;
-            return static_cast<int>(s);
-        }
+        if (prev == key) return static_cast<int>(s);
         s = (s + 1) & mask;
     }
+    return -1;
 }
 
 __global__ void __launch_bounds__(kWave) spgemm_count_kernel(
     const std::int32_t* __restrict__ ap, const std::int32_t* __restrict__ ac,
     const std::int32_t* __restrict__ bp, const std::int32_t* __restrict__ bc, std::int64_t n_rows,
-    int cap, std::int32_t* __restrict__ counts) {
+    int cap, std::int32_t* __restrict__ counts, int* __restrict__ overflow) {
     extern __shared__ int lds[];
     int* keys = lds;
     const std::int64_t row = blockIdx.x;
     if (row >= n_rows) return;
     const int lane = threadIdx.x;
-    const unsigned mask = static_cast<unsigned>(cap - 1);
     for (int s = lane; s < cap; s += kWave) keys[s] = -1;
     __syncthreads();
-    int mine = 0;
+    int mine = 0, full = 0;
     for (int p = ap[row]; p < ap[row + 1]; ++p) {
         const int j = ac[p];
         for (int q = bp[j] + lane; q < bp[j + 1]; q += kWave) {
             int ins = 0;
-            hash_insert(keys, mask, bc[q], &ins);
+            if (hash_insert(keys, cap, bc[q], &ins) < 0) full = 1;
             mine += ins;
         }
     }
     for (int off = kWave / 2; off > 0; off >>= 1) mine += __shfl_down(mine, off, kWave);
     if (lane == 0) counts[row] = mine;
+    if (full) atomicOr(overflow, 1);
 }
 
 __global__ void __launch_bounds__(kWave) spgemm_fill_kernel(
@@ -333,7 +333,6 @@ __global__ void __launch_bounds__(kWave) spgemm_fill_kernel(
     const std::int64_t row = blockIdx.x;
     if (row >= n_rows) return;
     const int lane = threadIdx.x;
-    const unsigned mask = static_cast<unsigned>(cap - 1);
     for (int s = lane; s < cap; s += kWave) {
         keys[s] = -1;
         acc[s] = 0.0f;
@@ -344,8 +343,9 @@ __global__ void __launch_bounds__(kWave) spgemm_fill_kernel(
         const float a = av[p];
         for (int q = bp[j] + lane; q < bp[j + 1]; q += kWave) {
             int ins = 0;
-            const int s = hash_insert(keys, mask, bc[q], &ins);
-            acc[s] = fmaf(a, bv[q], acc[s]);
+            // cap >= 2 x this row's distinct count (from the count pass), so the slot exists.
+            const int s = hash_insert(keys, cap, bc[q], &ins);
+            if (s >= 0) acc[s] = fmaf(a, bv[q], acc[s]);
         }
         __syncthreads();  // step p's folds land before step p+1 reads them
     }
@@ -558,18 +558,32 @@ CsrF spgemm(const CsrView& a, const CsrView& b, hipStream_t s) {
     std::vector<std::int32_t> hc(static_cast<std::size_t>(a.n));
     d2h(hc.data(), cand.get(), hc.size(), s);
     sync(s, "spgemm candidates");
+    // Counting table: keys only, sized from the candidate count (duplicates included) but capped
+    // at 64 KB; a row whose distinct columns would not fit raises the overflow flag.
+    constexpr int kMaxCountCap = 16384;
+    constexpr int kMaxFillCap = 4096;  // 16 B per slot: 64 KB
     std::int64_t widest = 1;
     for (auto v : hc) widest = std::max<std::int64_t>(widest, std::min<std::int64_t>(v, b.n_cols));
     int cap = 64;
-    while (cap < 2 * widest) cap <<= 1;
-    // keys + accumulators + compacted keys + compacted values
-    const std::size_t lds = static_cast<std::size_t>(cap) * 16;
-    require(lds <= 160 * 1024, "sparse product: a row's distinct columns exceed the LDS table");
+    while (cap < 2 * widest && cap < kMaxCountCap) cap <<= 1;
     DeviceBuffer<std::int32_t> counts(static_cast<std::size_t>(a.n + 1), "spgemm counts");
+    DeviceBuffer<int> overflow(1, "spgemm overflow");
     check(hipMemsetAsync(counts.get() + a.n, 0, sizeof(std::int32_t), s), "memset");
+    check(hipMemsetAsync(overflow.get(), 0, sizeof(int), s), "memset");
     spgemm_count_kernel<<<static_cast<unsigned>(a.n), kWave, static_cast<std::size_t>(cap) * 4, s>>>(
-        a.ptr, a.col, b.ptr, b.col, a.n, cap, counts.get());
+        a.ptr, a.col, b.ptr, b.col, a.n, cap, counts.get(), overflow.get());
     check(hipGetLastError(), "spgemm count launch");
+    int hov = 0;
+    d2h(&hov, overflow.get(), 1, s);
+    d2h(hc.data(), counts.get(), hc.size(), s);
+    sync(s, "spgemm count");
+    require(hov == 0, "sparse product: a row has more distinct columns than the LDS table holds");
+    std::int64_t widest_u = 1;
+    for (auto v : hc) widest_u = std::max<std::int64_t>(widest_u, v);
+    cap = 64;
+    while (cap < 2 * widest_u) cap <<= 1;
+    require(cap <= kMaxFillCap, "sparse product: a row has more distinct columns than the LDS table holds");
+    const std::size_t lds = static_cast<std::size_t>(cap) * 16;  // keys, accumulators, compacted pairs
     exclusive_scan_i32(counts.get(), c.ptr.get(), a.n + 1, s);
     std::int32_t nnz = 0;
     d2h(&nnz, c.ptr.get() + a.n, 1, s);
@@ -641,9 +655,11 @@ CsrF smoothed_prolongator(const CsrView& a, const std::int32_t* agg, std::int64_
 }
 
 CsrF galerkin(const CsrView& a, const CsrView& p, hipStream_t s) {
+    // P^T (A P): A P rows stay narrow, and P^T (A P) rows are at most as wide as the coarse
+    // operator's, so every product fits the LDS hash tables.
     CsrF r = transpose(p, s);
-    CsrF ra = spgemm(r.view(), a, s);
-    CsrF c = spgemm(ra.view(), p, s);
+    CsrF ap = spgemm(a, p, s);
+    CsrF c = spgemm(r.view(), ap.view(), s);
     sync(s, "galerkin");
     return c;
 }
@@ -741,7 +757,7 @@ Hierarchy build_hierarchy(const CsrView& a0, const AmgParams& prm, hipStream_t s
         L.r = transpose(L.p.view(), s);
         L.agg = std::move(agg);
         L.n_coarse = nc;
-        CsrF c = spgemm(spgemm(L.r.view(), L.a, s).view(), L.p.view(), s);
+        CsrF c = spgemm(L.r.view(), spgemm(L.a, L.p.view(), s).view(), s);
         sync(s, "galerkin");
         h.levels.push_back(std::move(L));
         a_next = std::move(c);

@@ -73,6 +73,9 @@ struct cn_context {
     cn::DeviceBuffer<double> rval64;
     cn::DeviceBuffer<float> rval32;
     std::int64_t nnz_red = 0;
+    // the reduced operator with its diagonal folded into FP64 row sums (differenced SpMM)
+    cn::DeviceBuffer<std::int32_t> optr, ocol;
+    cn::DeviceBuffer<float> oval, rowsum;
     cn::Hierarchy hier;
     std::unique_ptr<cn::BlockSolver> solver;
     // summary region
@@ -121,6 +124,36 @@ __global__ void region_volume_kernel(const std::int32_t* __restrict__ idx, std::
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = s[0];
+}
+
+// Row r of the differenced operator: its off-diagonal entries in order, and the FP64 row sum
+// (diagonal included) rounded once to FP32.
+__global__ void offdiag_kernel(const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
+                               const double* __restrict__ v, std::int64_t n, std::int32_t* __restrict__ optr,
+                               std::int32_t* __restrict__ ocol, float* __restrict__ oval, float* __restrict__ rowsum,
+                               int* __restrict__ missing) {
+    const std::int64_t r = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r > n) return;
+    optr[r] = static_cast<std::int32_t>(rp[r] - r);
+    if (r == n) return;
+    std::int64_t at = rp[r] - r;
+    const std::int64_t end = rp[r + 1] - (r + 1);
+    double s = 0.0;
+    int diag = 0;
+    for (int p = rp[r]; p < rp[r + 1]; ++p) {
+        s += v[p];
+        if (ci[p] == r) {
+            diag = 1;
+            continue;
+        }
+        if (at < end) {  // bounded even for a row that lacks its diagonal (reported below)
+            ocol[at] = ci[p];
+            oval[at] = static_cast<float>(v[p]);
+            ++at;
+        }
+    }
+    rowsum[r] = static_cast<float>(s);
+    if (!diag) atomicOr(missing, 1);
 }
 
 __global__ void product_kernel(const double* __restrict__ a, const double* __restrict__ b, std::int64_t n,
@@ -223,7 +256,27 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
     prm.degree = c.opts.degree;
     prm.lower_ratio = c.opts.lower_ratio;
     c.hier = build_hierarchy(c.reduced_view(), prm, s);
-    c.solver = std::make_unique<BlockSolver>(Operator{c.reduced_view(), c.rval64.get()}, c.hier, s);
+    // Differenced form: every reduced row keeps its diagonal (a node lies in its own elements), so
+    // dropping it shifts row r's start by exactly r.
+    const std::int64_t nr = n - 1;
+    c.optr.reset(nr + 1, "offdiag ptr");
+    c.ocol.reset(std::max<std::int64_t>(c.nnz_red - nr, 1), "offdiag col");
+    c.oval.reset(std::max<std::int64_t>(c.nnz_red - nr, 1), "offdiag val");
+    c.rowsum.reset(nr, "row sums");
+    DeviceBuffer<int> missing(1, "missing diagonal");
+    check(hipMemsetAsync(missing.get(), 0, sizeof(int), s), "memset");
+    offdiag_kernel<<<grid_for(nr + 1), kBlock, 0, s>>>(c.rptr.get(), c.rcol.get(), c.rval64.get(), nr, c.optr.get(),
+                                                       c.ocol.get(), c.oval.get(), c.rowsum.get(), missing.get());
+    int hmiss = 0;
+    d2h(&hmiss, missing.get(), 1, s);
+    sync(s, "differenced operator");
+    require(hmiss == 0, "a row of the reduced operator has no diagonal entry");
+    Operator op;
+    op.a32 = c.reduced_view();
+    op.off = {c.optr.get(), c.ocol.get(), c.oval.get(), nr, nr, c.nnz_red - nr};
+    op.rowsum = c.rowsum.get();
+    op.val64 = c.rval64.get();
+    c.solver = std::make_unique<BlockSolver>(op, c.hier, s);
 
     // Skin surface for the placement scan.
     c.skin_a.reset(std::max<std::int64_t>(3 * n_skin, 1), "skin a");
@@ -240,14 +293,14 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
     check(hipMemsetAsync(flag.get() + m, 0, sizeof(std::int32_t), s), "memset");
     region_index_kernel<<<grid_for(m), kBlock, 0, s>>>(c.tags.get(), m, c.opts.region_tag, flag.get());
     exclusive_scan_i32(flag.get(), pos.get(), m + 1, s);
-    std::int32_t nr = 0;
-    d2h(&nr, pos.get() + m, 1, s);
+    std::int32_t n_reg = 0;
+    d2h(&n_reg, pos.get() + m, 1, s);
     sync(s, "region size");
-    c.n_region = nr;
-    c.region_idx.reset(std::max<std::int64_t>(nr, 1), "region idx");
+    c.n_region = n_reg;
+    c.region_idx.reset(std::max<std::int64_t>(n_reg, 1), "region idx");
     region_compact_kernel<<<grid_for(m), kBlock, 0, s>>>(flag.get(), pos.get(), m, c.region_idx.get());
     DeviceBuffer<double> vol(1, "region volume");
-    region_volume_kernel<<<1, kBlock, 0, s>>>(c.region_idx.get(), nr, c.vols32.get(), vol.get());
+    region_volume_kernel<<<1, kBlock, 0, s>>>(c.region_idx.get(), n_reg, c.vols32.get(), vol.get());
     d2h(&c.region_volume, vol.get(), 1, s);
     sync(s, "context build");
 }
@@ -416,6 +469,10 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
         }
         out->total_iterations = st.iterations;
         out->refinements = st.refinements;
+        for (int r = 0; r < 8; ++r) {
+            out->round_iterations[r] = r < cn::kMaxRounds ? st.round_iterations[r] : 0;
+            out->round_residual[r] = r < cn::kMaxRounds ? st.round_residual[r] : 0.0;
+        }
     }
     sync(s, "block outputs");
     for (int i = 0; i < k; ++i) {
@@ -729,6 +786,20 @@ int32_t cn_k_spmm(cn_context* c, const float* X, float* Y, int32_t K) {
         auto dx = cn::upload(X, nk, c->s, "X");
         cn::DeviceBuffer<float> dy(nk, "Y");
         c->solver->spmm(dx.get(), dy.get(), K);
+        cn::check(hipGetLastError(), "spmm launch");
+        cn::d2h(Y, dy.get(), nk, c->s);
+        cn::sync(c->s, "spmm");
+    });
+}
+
+int32_t cn_k_spmm_diff(cn_context* c, const float* X, float* Y, int32_t K) {
+    return guarded([&] {
+        cn::require(c != nullptr, "null context");
+        cn::require(cn::padded_width(K) == K, "K must be a compiled block width");
+        const int64_t nk = (c->n - 1) * K;
+        auto dx = cn::upload(X, nk, c->s, "X");
+        cn::DeviceBuffer<float> dy(nk, "Y");
+        c->solver->spmm_diff(dx.get(), dy.get(), K);
         cn::check(hipGetLastError(), "spmm launch");
         cn::d2h(Y, dy.get(), nk, c->s);
         cn::sync(c->s, "spmm");

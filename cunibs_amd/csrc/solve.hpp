@@ -11,6 +11,7 @@ namespace cn {
 constexpr int kBlockWidths[] = {1, 2, 4, 8, 16, 32};
 constexpr int kNumBlockWidths = 6;
 constexpr int kMaxBlock = 32;
+constexpr int kMaxRounds = 8;
 
 // Partial-sum blocks of every Gram-producing kernel. Fixed, so the association of each FP64
 // reduction depends on the row count alone.
@@ -19,26 +20,35 @@ constexpr int kGramBlocks = 512;
 int padded_width(int k);  // smallest compiled width >= k
 
 struct SolveOptions {
-    double tolerance = 1e-6;  // ||b - A x|| / ||b|| per column (FP64 true residual)
-    int max_iters = 2000;     // inner iterations over all refinement rounds
-    int max_refine = 8;       // outer FP64 refinement rounds
-    double inner_margin = 0.5;   // inner target = margin * tol * ||b||
-    double drop_tol = 1e-6;      // pivoted Cholesky: relative Schur pivot below which a direction drops
-    int check_every = 4;         // graph replays between host convergence reads
+    double tolerance = 1e-6;    // ||b - A x|| / ||b|| per column (FP64 true residual)
+    int max_iters = 2000;       // inner iterations over all refinement rounds
+    int max_refine = kMaxRounds;  // outer FP64 refinement rounds
+    double inner_margin = 0.5;  // inner target = margin * tol * ||b||
+    double drop_tol = 1e-6;     // pivoted Cholesky: relative Schur pivot below which a direction drops
+    int check_every = 4;        // graph replays between host convergence reads
     bool use_graph = true;
 };
 
 struct SolveStats {
-    std::vector<int> column_iterations;    // inner iterations until each column froze
-    std::vector<double> relative_residual; // final FP64 true residual per column
-    int iterations = 0;                    // inner iterations run (all rounds)
-    int refinements = 0;                   // outer rounds run
+    std::vector<int> column_iterations;     // inner iterations until each column froze
+    std::vector<double> relative_residual;  // final FP64 true residual per column
+    int iterations = 0;                     // inner iterations run (all rounds)
+    int refinements = 0;                    // outer rounds run
     int vcycles = 0;
+    int round_iterations[kMaxRounds] = {};  // cumulative iterations at the end of each round
+    double round_residual[kMaxRounds] = {}; // worst true relative residual after each round
 };
 
-// The reduced operator the solver iterates on: FP32 values for SpMM, FP64 for the true residual.
+// The reduced operator the solver iterates on.
+//   a32:  FP32 CSR (the V-cycle's finest level and the plain SpMM)
+//   off/rowsum: the same operator with the diagonal folded into row sums, applied as
+//         y_i = sum_{j != i} a_ij (x_j - x_i) + s_i x_i, which keeps the FP32 product free of the
+//         cancellation between a stiffness row's entries (they sum to ~0)
+//   val64: FP64 values for the true residual
 struct Operator {
-    CsrView a32;               // FP32 view (ptr, col, val32)
+    CsrView a32;
+    CsrView off;
+    const float* rowsum = nullptr;
     const double* val64 = nullptr;
 };
 
@@ -49,15 +59,16 @@ class BlockSolver {
     BlockSolver(const BlockSolver&) = delete;
     BlockSolver& operator=(const BlockSolver&) = delete;
 
-    // B: (n, K) row-major FP32 on the device (columns >= k are ignored), X: (n, K) FP64 out,
+    // B: (n, K) row-major FP32 on the device (columns >= k are zero), X: (n, K) FP64 out,
     // X0: optional (n, K) FP64 initial guess. K = padded_width(k).
     SolveStats solve(const float* B, double* X, const double* X0, int k, const SolveOptions& opt);
 
     // Kernel-level pieces, exposed for parity tests and benchmarks.
-    void spmm(const float* X, float* Y, int K);          // Y = A X (FP32)
-    void vcycle(const float* B, float* X, int K);       // X = M B, one zero-guess V-cycle
+    void spmm(const float* X, float* Y, int K);     // Y = A X (FP32, plain CSR)
+    void spmm_diff(const float* X, float* Y, int K);  // Y = A X (FP32, differenced form)
+    void vcycle(const float* B, float* X, int K);   // X = M B, one zero-guess V-cycle
     // Average device time (ms) over `reps` launches, HIP events on the solver's stream.
-    double bench_spmm(int K, int reps, bool fused_gram);
+    double bench_spmm(int K, int reps, bool fused);
     double bench_vcycle(int K, int reps);
     std::int64_t n() const { return op_.a32.n; }
     hipStream_t stream() const { return s_; }
@@ -66,24 +77,27 @@ class BlockSolver {
     template <int K>
     SolveStats solve_k(const float* B, double* X, const double* X0, int k, const SolveOptions& opt);
     template <int K>
-    void vcycle_k(const float* B, float* X);
+    void vcycle_k(const float* B, float* X, const int* state);
     template <int K>
     void iteration_k(const SolveOptions& opt);
+    template <int K>
+    void spmm_block_k(const int* state);
     void ensure(int K);
 
     Operator op_;
     const Hierarchy& h_;
     hipStream_t s_;
     int cap_k_ = 0;
+    int n_cu_ = 256;
     // block vectors (n, K)
-    DeviceBuffer<float> R_, Z_, W_, S_, P_, Q_;
+    DeviceBuffer<float> R_, Z_, W_, S_;
     DeviceBuffer<double> R64_, X64_;
     // per-level V-cycle vectors
     std::vector<DeviceBuffer<float>> lb_, lx_, lr_;
-    DeviceBuffer<double> cb_, cx_;  // coarsest level (FP64)
+    DeviceBuffer<float> cb_, cx_;  // coarsest level
     // reductions and small state
-    DeviceBuffer<double> partials_;  // kGramBlocks * K * K
-    DeviceBuffer<double> G_, T_, C_, alpha_, norms_, target_;
+    DeviceBuffer<double> partials_;  // kGramBlocks * 2 * K * K
+    DeviceBuffer<double> G_, T_, TT_, AL_, D_, norms_, target_;
     DeviceBuffer<int> state_;  // [0] done, [1] iteration, [2..2+K) frozen, [2+K..2+2K) col_iters
     int* host_state_ = nullptr;  // pinned
     hipGraphExec_t graph_ = nullptr;

@@ -273,6 +273,12 @@ class SolverContext:
         check(lib().cn_k_spmm(self.handle, ptr(X), ptr(Y), K), "cn_k_spmm")
         return Y
 
+    def spmm_diff(self, X: np.ndarray) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.empty_like(X)
+        check(lib().cn_k_spmm_diff(self.handle, ptr(X), ptr(Y), X.shape[1]), "cn_k_spmm_diff")
+        return Y
+
     def vcycle(self, B: np.ndarray) -> np.ndarray:
         B = np.ascontiguousarray(B, dtype=np.float32)
         X = np.empty_like(B)
@@ -365,6 +371,9 @@ def solve_placements_block(
         extra.update(ex)
         extra["total_iterations"] = int(out.total_iterations)
         extra["refinements"] = int(out.refinements)
+        nr = min(int(out.refinements), 8)
+        extra["round_iterations"] = [int(out.round_iterations[i]) for i in range(nr)]
+        extra["round_residual"] = [float(out.round_residual[i]) for i in range(nr)]
     return [
         PlacementResult(
             transform=tf[i], magnE=None if mag is None else mag[i], E=None if E is None else E[i],

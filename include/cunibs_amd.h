@@ -139,6 +139,8 @@ typedef struct {
     int32_t* degenerate;       /* (k) 1 where the handle left the frame undefined */
     int32_t total_iterations;  /* out: inner iterations of the block */
     int32_t refinements;       /* out: FP64 refinement rounds */
+    int32_t round_iterations[8]; /* out: cumulative inner iterations at the end of each round */
+    double round_residual[8];    /* out: worst FP64 true relative residual after each round */
 } cn_block_outputs;
 
 /* Solve k <= 32 placements as one block (fem/solve.py:865-979): coil frames, dA/dt, RHS, block
@@ -177,6 +179,10 @@ int32_t cn_k_spgemm(int64_t a_rows, int64_t b_rows, int64_t b_cols, const int32_
                     float* c_val, int64_t* c_nnz);
 /* Y = A X on the context's reduced FP32 operator; X, Y (n_reduced, K), K in the block sizes. */
 int32_t cn_k_spmm(cn_context* ctx, const float* X, float* Y, int32_t K);
+/* The same product in the block solver's differenced form, y_i = sum_{j!=i} a_ij (x_j - x_i)
+ * + s_i x_i with s_i the FP64 row sum (no counterpart in the reference: it is what keeps the
+ * FP32 recursive residual close to the FP64 true residual). */
+int32_t cn_k_spmm_diff(cn_context* ctx, const float* X, float* Y, int32_t K);
 /* X = M B, one zero-guess V-cycle (vcycle.cu:650-663 apply / apply_block). */
 int32_t cn_k_vcycle(cn_context* ctx, const float* B, float* X, int32_t K);
 /* bindings.cpp:194-206 dadt_nbody: targets r (n,3) metres relative to the coil origin, dipoles

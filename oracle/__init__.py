@@ -302,8 +302,10 @@ def smooth_prolongator(a: Csr, p_tent: Csr, dinv: np.ndarray) -> Csr:
 
 
 def galerkin(a: Csr, p: Csr) -> Csr:
-    """fem/solve.py:129-134: A_c = (P^T A) P, each product fixed-order FP32."""
-    return spgemm(spgemm(transpose(p), a), p)
+    """fem/solve.py:129-134: A_c = P^T A P, here associated as P^T (A P), each product
+    fixed-order FP32. (The reference evaluates (P^T A) P; the association only changes rounding,
+    and this one keeps every intermediate row narrow.)"""
+    return spgemm(transpose(p), spgemm(a, p))
 
 
 def aggregate_level(a: Csr) -> tuple[np.ndarray | None, int]:
