@@ -85,7 +85,7 @@ struct cn_context {
     // per-block scratch (grow-only)
     cn::DeviceBuffer<double> d_centers, d_handles, d_dists, d_tf, d_origins;
     cn::DeviceBuffer<std::int32_t> d_degen;
-    cn::DeviceBuffer<float> d_dip, d_dadt_nodes, d_dadt_elm, d_B, d_E, d_magn, d_rhs;
+    cn::DeviceBuffer<float> d_dip, d_dadt_nodes, d_dadt_elm, d_q, d_B, d_E, d_magn, d_rhs, d_stage;
     cn::DeviceBuffer<double> d_X, d_v, d_x0;
 
     cn::CsrView reduced_view() const {
@@ -405,20 +405,21 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
     c.d_origins.ensure(3 * k, "origins");
     h2d(c.d_dip.get(), dip.data(), dip.size(), s);
     h2d(c.d_origins.get(), origins.data(), 3 * k, s);
-    c.d_dadt_nodes.ensure(3 * n * k, "dadt nodes");
-    c.d_dadt_elm.ensure(3 * m * k, "dadt elements");
+    c.d_dadt_nodes.ensure(3 * n * K, "dadt nodes");
+    c.d_dadt_elm.ensure(3 * m * K, "dadt elements");
+    c.d_q.ensure(4 * m * K, "corner terms");
     c.d_B.ensure(nr * K, "rhs block");
     const float scale = static_cast<float>(didt) * static_cast<float>(kMu0Over4Pi);
-    dadt_nbody(c.d_dip.get(), static_cast<int>(n_dip), c.nodes.get(), n, c.d_origins.get(), k, scale,
+    dadt_nbody(c.d_dip.get(), static_cast<int>(n_dip), c.nodes.get(), n, c.d_origins.get(), k, K, scale,
                c.d_dadt_nodes.get(), s);
-    element_average(c.d_dadt_nodes.get(), c.tets.get(), n, m, k, c.d_dadt_elm.get(), s);
+    element_pass(c.d_dadt_nodes.get(), c.tets.get(), c.g32.get(), c.neg_vc.get(), m, k, K, c.d_dadt_elm.get(),
+                 c.d_q.get(), s);
     float* rhs_full = nullptr;
     if (out && out->rhs) {
         c.d_rhs.ensure(n * k, "rhs full");
         rhs_full = c.d_rhs.get();
     }
-    assemble_rhs(c.d_dadt_elm.get(), c.g32.get(), c.neg_vc.get(), c.n2c_ptr.get(), c.n2c_idx.get(), n, m, k, K,
-                 c.ground, c.d_B.get(), rhs_full, s);
+    node_rhs(c.d_q.get(), c.n2c_ptr.get(), c.n2c_idx.get(), n, k, K, c.ground, c.d_B.get(), rhs_full, s);
     // 3. block PCG.
     SolveOptions opt;
     if (sopt) {
@@ -442,18 +443,25 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
     }
     SolveStats st = c.solver->solve(c.d_B.get(), c.d_X.get(), dx0, k, opt);
     // 4. reconstruction and metrics.
-    c.d_magn.ensure(m * k, "magnE");
+    c.d_magn.ensure(m * K, "magnE");
     float* dE = nullptr;
     if (out && out->E) {
-        c.d_E.ensure(3 * m * k, "E");
+        c.d_E.ensure(3 * m * K, "E");
         dE = c.d_E.get();
     }
-    reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), c.d_dadt_elm.get(), n, m, k, K, c.ground, dE, c.d_magn.get(), s);
+    reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), c.d_dadt_elm.get(), m, k, K, c.ground, dE, c.d_magn.get(), s);
     if (out) {
-        if (out->magnE) d2h(out->magnE, c.d_magn.get(), m * k, s);
-        if (out->E) d2h(out->E, c.d_E.get(), 3 * m * k, s);
-        if (out->dadt_elm) d2h(out->dadt_elm, c.d_dadt_elm.get(), 3 * m * k, s);
-        if (out->dadt_nodes) d2h(out->dadt_nodes, c.d_dadt_nodes.get(), 3 * n * k, s);
+        // Placement-fastest device arrays -> placement-major host arrays through one staging buffer.
+        auto fetch = [&](float* host, const float* dev, std::int64_t rows, int inner) {
+            c.d_stage.ensure(rows * inner * k, "output staging");
+            unpack(dev, rows, inner, k, K, c.d_stage.get(), s);
+            d2h(host, c.d_stage.get(), rows * inner * k, s);
+            sync(s, "output copy");
+        };
+        if (out->magnE) fetch(out->magnE, c.d_magn.get(), m, 1);
+        if (out->E) fetch(out->E, c.d_E.get(), m, 3);
+        if (out->dadt_elm) fetch(out->dadt_elm, c.d_dadt_elm.get(), m, 3);
+        if (out->dadt_nodes) fetch(out->dadt_nodes, c.d_dadt_nodes.get(), n, 3);
         if (out->rhs) d2h(out->rhs, c.d_rhs.get(), n * k, s);
         if (out->v) {
             c.d_v.ensure(n * k, "potential");
@@ -461,7 +469,7 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
             d2h(out->v, c.d_v.get(), n * k, s);
         }
         if (out->metrics)
-            region_metrics(c.d_magn.get(), m, k, c.region_idx.get(), c.n_region, c.vols32.get(), c.bary.get(),
+            region_metrics(c.d_magn.get(), m, k, K, c.region_idx.get(), c.n_region, c.vols32.get(), c.bary.get(),
                            out->metrics, s);
         for (int i = 0; i < k; ++i) {
             if (out->iterations) out->iterations[i] = st.column_iterations[i];
@@ -837,7 +845,7 @@ int32_t cn_k_dadt(const float* s_, const float* mp, const float* sn, int64_t n_d
         const double zero[3] = {0.0, 0.0, 0.0};
         auto dorig = cn::upload(zero, 3, s, "origin");
         cn::DeviceBuffer<float> dout(std::max<int64_t>(3 * n, 1), "out");
-        cn::dadt_nbody(dd.get(), static_cast<int>(n_dip), dn.get(), n, dorig.get(), 1, didt * mu0_4pi, dout.get(), s);
+        cn::dadt_nbody(dd.get(), static_cast<int>(n_dip), dn.get(), n, dorig.get(), 1, 1, didt * mu0_4pi, dout.get(), s);
         cn::d2h(out, dout.get(), 3 * n, s);
         cn::sync(s, "dadt");
     });
@@ -853,10 +861,10 @@ int32_t cn_k_place(cn_context* c, const double* centers, const double* handles, 
     });
 }
 
-int32_t cn_bench_spmm(cn_context* c, int32_t K, int32_t reps, int32_t fused, double* ms) {
+int32_t cn_bench_spmm(cn_context* c, int32_t K, int32_t reps, int32_t mode, int32_t sub, double* ms) {
     return guarded([&] {
-        cn::require(c != nullptr && reps >= 1, "bad argument");
-        *ms = c->solver->bench_spmm(K, reps, fused != 0);
+        cn::require(c != nullptr && reps >= 1 && ms != nullptr, "bad argument");
+        *ms = c->solver->bench_spmm(K, reps, mode, sub);
     });
 }
 

@@ -171,9 +171,10 @@ constexpr int kNodesPerThread = 4;
 
 // dadt.cu:27-104 restated for CDNA: dipole tiles staged in LDS, 4 nodes per thread in registers,
 // one placement per blockIdx.y. w_j = |r - s_j|^-3 from rn + sn - 2 r.s; rsqrtf, FP32.
+// Output (n, 3, K): the K placements of a node component are contiguous.
 __global__ void __launch_bounds__(kDadtBlock) dadt_kernel(const float* __restrict__ dip, int n_dip,
                                                           const double* __restrict__ nodes_mm, std::int64_t n,
-                                                          const double* __restrict__ origin, float scale,
+                                                          const double* __restrict__ origin, float scale, int K,
                                                           float* __restrict__ out) {
     __shared__ float s_s[kDipTile * 3];
     __shared__ float s_mp[kDipTile * 6];
@@ -230,59 +231,62 @@ __global__ void __launch_bounds__(kDadtBlock) dadt_kernel(const float* __restric
             }
         }
     }
-    float* o = out + static_cast<std::int64_t>(c) * n * 3;
 #pragma unroll
     for (int k = 0; k < kNodesPerThread; ++k) {
         const std::int64_t node = t + k * T;
         if (node < n) {
-            o[3 * node] = scale * ((wy[k] * rz[k] - wz[k] * ry[k]) - px[k]);
-            o[3 * node + 1] = scale * ((wz[k] * rx[k] - wx[k] * rz[k]) - py[k]);
-            o[3 * node + 2] = scale * ((wx[k] * ry[k] - wy[k] * rx[k]) - pz[k]);
+            float* o = out + node * 3 * K + c;
+            o[0] = scale * ((wy[k] * rz[k] - wz[k] * ry[k]) - px[k]);
+            o[K] = scale * ((wz[k] * rx[k] - wx[k] * rz[k]) - py[k]);
+            o[2 * K] = scale * ((wx[k] * ry[k] - wy[k] * rx[k]) - pz[k]);
         }
     }
 }
 
-// dadt_element.cu:6-21: mean of the four nodal vectors, (k, m, 3) from (k, n, 3).
-__global__ void element_average_kernel(const float* __restrict__ nodal, const std::int32_t* __restrict__ tets,
-                                       std::int64_t n, std::int64_t m, int k, float* __restrict__ elm) {
+// Element pass over (element, placement), placements fastest so the K lanes of an element read
+// and write contiguous memory:
+//   dadt_elm = mean of the four nodal vectors       (dadt_element.cu:6-21)
+//   q[corner] = dadt_elm . (g[corner] * neg_vc)     (rhs.cu:65-80, the staged corner terms, FP32)
+// Placements beyond k (block padding) get zeros, so their right-hand-side columns are zero.
+__global__ void element_pass_kernel(const float* __restrict__ nodal, const std::int32_t* __restrict__ tets,
+                                    const float* __restrict__ g, const float* __restrict__ neg_vc, std::int64_t m,
+                                    int k, int K, float* __restrict__ elm, float* __restrict__ q) {
     const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= m * k) return;
-    const std::int64_t c = t / m, e = t % m;
-    const int4 tt = reinterpret_cast<const int4*>(tets)[e];
-    const float* v = nodal + c * n * 3;
-    float* o = elm + (c * m + e) * 3;
-    for (int d = 0; d < 3; ++d)
-        o[d] = 0.25f * (((v[3 * (std::int64_t)tt.x + d] + v[3 * (std::int64_t)tt.y + d]) + v[3 * (std::int64_t)tt.z + d]) +
-                        v[3 * (std::int64_t)tt.w + d]);
-}
-
-// rhs.cu:65-104 in one pass: per node and placement, the corner terms
-// q = dadt . (g * neg_vc) in FP32 (as the staged corner kernel rounds them) summed in node2corner
-// order; written to the solver block with the ground row dropped.
-__global__ void rhs_kernel(const float* __restrict__ elm, const float* __restrict__ g, const float* __restrict__ neg_vc,
-                           const std::int32_t* __restrict__ ptr, const std::int32_t* __restrict__ idx,
-                           std::int64_t n, std::int64_t m, int k, int K, int ground, float* __restrict__ B,
-                           float* __restrict__ b_full) {
-    const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= n * k) return;
-    const std::int64_t c = t / n, node = t % n;
-    const float* d = elm + c * m * 3;
-    float acc = 0.f;
-    for (int p = ptr[node]; p < ptr[node + 1]; ++p) {
-        const std::int64_t corner = idx[p];
-        const std::int64_t e = corner >> 2;
-        const float w = neg_vc[e];
-        const float w0 = g[3 * corner] * w, w1 = g[3 * corner + 1] * w, w2 = g[3 * corner + 2] * w;
-        const float q = fmaf(d[3 * e + 2], w2, fmaf(d[3 * e + 1], w1, d[3 * e] * w0));
-        acc += q;
+    if (t >= m * K) return;
+    const std::int64_t e = t / K;
+    const int c = static_cast<int>(t % K);
+    float d[3] = {0.f, 0.f, 0.f};
+    if (c < k) {
+        const int4 tt = reinterpret_cast<const int4*>(tets)[e];
+        const std::int64_t stride = 3 * static_cast<std::int64_t>(K);
+        for (int a = 0; a < 3; ++a) {
+            const float v0 = nodal[tt.x * stride + a * K + c], v1 = nodal[tt.y * stride + a * K + c];
+            const float v2 = nodal[tt.z * stride + a * K + c], v3 = nodal[tt.w * stride + a * K + c];
+            d[a] = 0.25f * (((v0 + v1) + v2) + v3);
+        }
     }
-    if (b_full != nullptr) b_full[c * n + node] = acc;
-    if (node != ground) B[(node - (node > ground)) * K + c] = acc;
+    for (int a = 0; a < 3; ++a) elm[(e * 3 + a) * K + c] = d[a];
+    const float w = neg_vc[e];
+    for (int i = 0; i < 4; ++i) {
+        const float* gc = g + (4 * e + i) * 3;
+        const float w0 = gc[0] * w, w1 = gc[1] * w, w2 = gc[2] * w;
+        q[(4 * e + i) * K + c] = fmaf(d[2], w2, fmaf(d[1], w1, d[0] * w0));
+    }
 }
 
-__global__ void zero_f32_kernel(std::int64_t count, float* __restrict__ out) {
-    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < count) out[i] = 0.0f;
+// Node gather (rhs.cu:84-104): b[node] = sum of its corners' q in node2corner order, written to
+// the solver block with the ground row dropped, and optionally to (k, n) full-node storage.
+__global__ void node_rhs_kernel(const float* __restrict__ q, const std::int32_t* __restrict__ ptr,
+                                const std::int32_t* __restrict__ idx, std::int64_t n, int k, int K, int ground,
+                                float* __restrict__ B, float* __restrict__ b_full) {
+    const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n * K) return;
+    const std::int64_t node = t / K;
+    const int c = static_cast<int>(t % K);
+    float acc = 0.f;
+    for (int p = ptr[node]; p < ptr[node + 1]; ++p) acc += q[static_cast<std::int64_t>(idx[p]) * K + c];
+    if (b_full != nullptr && c < k) b_full[c * n + node] = acc;
+    if (node != ground) B[(node - (node > ground)) * K + c] = acc;
 }
 
 }  // namespace
@@ -302,27 +306,21 @@ void place_transforms(const double* centers, const double* handles, const double
 }
 
 void dadt_nbody(const float* dip, int n_dip, const double* nodes_mm, std::int64_t n, const double* origins, int k,
-                float scale, float* out, hipStream_t s) {
+                int K, float scale, float* out, hipStream_t s) {
     if (n <= 0 || k <= 0) return;
     const std::int64_t threads = (n + kNodesPerThread - 1) / kNodesPerThread;
     const unsigned gx = grid_for(threads, kDadtBlock);
-    dadt_kernel<<<dim3(gx, k), kDadtBlock, 0, s>>>(dip, n_dip, nodes_mm, n, origins, scale, out);
+    dadt_kernel<<<dim3(gx, k), kDadtBlock, 0, s>>>(dip, n_dip, nodes_mm, n, origins, scale, K, out);
 }
 
-void element_average(const float* nodal, const std::int32_t* tets, std::int64_t n, std::int64_t m, int k, float* elm,
-                     hipStream_t s) {
-    if (const unsigned g = grid_for(m * k)) element_average_kernel<<<g, kBlock, 0, s>>>(nodal, tets, n, m, k, elm);
+void element_pass(const float* nodal, const std::int32_t* tets, const float* g, const float* neg_vc, std::int64_t m,
+                  int k, int K, float* elm, float* q, hipStream_t s) {
+    if (const unsigned gr = grid_for(m * K)) element_pass_kernel<<<gr, kBlock, 0, s>>>(nodal, tets, g, neg_vc, m, k, K, elm, q);
 }
 
-void assemble_rhs(const float* elm, const float* g, const float* neg_vc, const std::int32_t* ptr,
-                  const std::int32_t* idx, std::int64_t n, std::int64_t m, int k, int K, int ground, float* B,
-                  float* b_full, hipStream_t s) {
-    const std::int64_t nk = (n - 1) * K;
-    if (K > k) {  // padding columns stay zero
-        if (const unsigned g2 = grid_for(nk)) zero_f32_kernel<<<g2, kBlock, 0, s>>>(nk, B);
-    }
-    if (const unsigned gr = grid_for(n * k))
-        rhs_kernel<<<gr, kBlock, 0, s>>>(elm, g, neg_vc, ptr, idx, n, m, k, K, ground, B, b_full);
+void node_rhs(const float* q, const std::int32_t* ptr, const std::int32_t* idx, std::int64_t n, int k, int K,
+              int ground, float* B, float* b_full, hipStream_t s) {
+    if (const unsigned gr = grid_for(n * K)) node_rhs_kernel<<<gr, kBlock, 0, s>>>(q, ptr, idx, n, k, K, ground, B, b_full);
 }
 
 }  // namespace cn

@@ -1,4 +1,9 @@
 // Coil-side and post-processing launchers.
+//
+// Per-placement device arrays of a block are laid out placement-fastest with the padded block width
+// K: nodal dA/dt (n, 3, K), element dA/dt (m, 3, K), corner terms (4m, K), E (m, 3, K), |E| (m, K).
+// The K lanes that serve one node or element then touch contiguous memory. unpack() turns such an
+// array into the placement-major (k, rows, inner) layout the C ABI hands out.
 #pragma once
 #include "kernels.hpp"
 
@@ -10,21 +15,22 @@ void place_transforms(const double* centers, const double* handles, const double
                       int n_cu, hipStream_t s);
 // dip: per placement n_dip x [s(3) m(3) m x s(3) |s|^2] FP32, centred at that placement's origin.
 void dadt_nbody(const float* dip, int n_dip, const double* nodes_mm, std::int64_t n, const double* origins, int k,
-                float scale, float* out, hipStream_t s);
-void element_average(const float* nodal, const std::int32_t* tets, std::int64_t n, std::int64_t m, int k, float* elm,
-                     hipStream_t s);
-void assemble_rhs(const float* elm, const float* g, const float* neg_vc, const std::int32_t* ptr,
-                  const std::int32_t* idx, std::int64_t n, std::int64_t m, int k, int K, int ground, float* B,
-                  float* b_full, hipStream_t s);
+                int K, float scale, float* out, hipStream_t s);
+void element_pass(const float* nodal, const std::int32_t* tets, const float* g, const float* neg_vc, std::int64_t m,
+                  int k, int K, float* elm, float* q, hipStream_t s);
+void node_rhs(const float* q, const std::int32_t* ptr, const std::int32_t* idx, std::int64_t n, int k, int K,
+              int ground, float* B, float* b_full, hipStream_t s);
 
 // post.hip
 void scatter_potential(const double* X, std::int64_t n, int k, int K, int ground, double* v, hipStream_t s);
-void reconstruct(const double* X, const std::int32_t* tets, const float* g, const float* elm, std::int64_t n,
-                 std::int64_t m, int k, int K, int ground, float* E, float* magn, hipStream_t s);
+void reconstruct(const double* X, const std::int32_t* tets, const float* g, const float* elm, std::int64_t m, int k,
+                 int K, int ground, float* E, float* magn, hipStream_t s);
+// (rows, inner, K) -> (k, rows, inner)
+void unpack(const float* in, std::int64_t rows, int inner, int k, int K, float* out, hipStream_t s);
 
 constexpr int kMetricsLen = 16;
-// Gray-matter summary per placement (metrics.py:209-243): see include/cunibs_amd.h for the layout.
-void region_metrics(const float* magn, std::int64_t m, int k, const std::int32_t* region_idx, std::int64_t n_region,
-                    const float* vols, const double* bary, double* out_host, hipStream_t s);
+// Region summary per placement (metrics.py:209-243) from |E| laid out (m, K).
+void region_metrics(const float* magn, std::int64_t m, int k, int K, const std::int32_t* region_idx,
+                    std::int64_t n_region, const float* vols, const double* bary, double* out_host, hipStream_t s);
 
 }  // namespace cn

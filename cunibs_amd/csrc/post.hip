@@ -23,13 +23,15 @@ __global__ void scatter_kernel(const double* __restrict__ X, std::int64_t n, int
 }
 
 // reconstruct.cu:36-78: E = -sum_i v_i g_i - dadt_elm with the gradient in FP64, |E| per element.
+// Threads over (element, placement), placements fastest; E (m, 3, K), |E| (m, K).
 __global__ void reconstruct_kernel(const double* __restrict__ X, const std::int32_t* __restrict__ tets,
                                    const float* __restrict__ g, const float* __restrict__ elm, std::int64_t m, int k,
                                    int K, int ground, float* __restrict__ E, float* __restrict__ magn) {
     const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= m * k) return;
-    const int c = static_cast<int>(t / m);
-    const std::int64_t e = t % m;
+    if (t >= m * K) return;
+    const std::int64_t e = t / K;
+    const int c = static_cast<int>(t % K);
+    if (c >= k) return;
     const int4 tt = reinterpret_cast<const int4*>(tets)[e];
     const int nodes[4] = {tt.x, tt.y, tt.z, tt.w};
     const float* ge = g + 12 * e;
@@ -40,16 +42,24 @@ __global__ void reconstruct_kernel(const double* __restrict__ X, const std::int3
         gy = fma(vi, static_cast<double>(ge[3 * i + 1]), gy);
         gz = fma(vi, static_cast<double>(ge[3 * i + 2]), gz);
     }
-    const float* d = elm + t * 3;
-    const double ex = -gx - static_cast<double>(d[0]);
-    const double ey = -gy - static_cast<double>(d[1]);
-    const double ez = -gz - static_cast<double>(d[2]);
+    const double ex = -gx - static_cast<double>(elm[(e * 3) * K + c]);
+    const double ey = -gy - static_cast<double>(elm[(e * 3 + 1) * K + c]);
+    const double ez = -gz - static_cast<double>(elm[(e * 3 + 2) * K + c]);
     if (E != nullptr) {
-        E[3 * t] = static_cast<float>(ex);
-        E[3 * t + 1] = static_cast<float>(ey);
-        E[3 * t + 2] = static_cast<float>(ez);
+        E[(e * 3) * K + c] = static_cast<float>(ex);
+        E[(e * 3 + 1) * K + c] = static_cast<float>(ey);
+        E[(e * 3 + 2) * K + c] = static_cast<float>(ez);
     }
-    magn[t] = static_cast<float>(sqrt(fma(ez, ez, fma(ey, ey, ex * ex))));
+    magn[e * K + c] = static_cast<float>(sqrt(fma(ez, ez, fma(ey, ey, ex * ex))));
+}
+
+__global__ void unpack_kernel(const float* __restrict__ in, std::int64_t rows, int inner, int k, int K,
+                              float* __restrict__ out) {
+    const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const std::int64_t per = rows * inner;
+    if (t >= per * k) return;
+    const std::int64_t c = t / per, ri = t % per;  // ri = r * inner + d
+    out[t] = in[ri * K + c];
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -58,11 +68,11 @@ __global__ void reconstruct_kernel(const double* __restrict__ X, const std::int3
 // ------------------------------------------------------------------------------------------------
 constexpr int kStatBlocks = 256;
 
-__global__ void gather_region_kernel(const float* __restrict__ magn, std::int64_t m, const std::int32_t* __restrict__ idx,
+__global__ void gather_region_kernel(const float* __restrict__ magn, int K, const std::int32_t* __restrict__ idx,
                                      std::int64_t nr, int c, float* __restrict__ vals, std::uint32_t* __restrict__ keys) {
     const std::int64_t j = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= nr) return;
-    const float v = magn[static_cast<std::int64_t>(c) * m + idx[j]];
+    const float v = magn[static_cast<std::int64_t>(idx[j]) * K + c];
     vals[j] = v;
     keys[j] = __float_as_uint(v);  // |E| >= 0: the bit pattern orders like the value
 }
@@ -251,15 +261,19 @@ void scatter_potential(const double* X, std::int64_t n, int k, int K, int ground
     if (const unsigned g = grid_for(n * k)) scatter_kernel<<<g, kBlock, 0, s>>>(X, n, k, K, ground, v);
 }
 
-void reconstruct(const double* X, const std::int32_t* tets, const float* g, const float* elm, std::int64_t n,
-                 std::int64_t m, int k, int K, int ground, float* E, float* magn, hipStream_t s) {
-    (void)n;
-    if (const unsigned gr = grid_for(m * k))
+void reconstruct(const double* X, const std::int32_t* tets, const float* g, const float* elm, std::int64_t m, int k,
+                 int K, int ground, float* E, float* magn, hipStream_t s) {
+    if (const unsigned gr = grid_for(m * K))
         reconstruct_kernel<<<gr, kBlock, 0, s>>>(X, tets, g, elm, m, k, K, ground, E, magn);
 }
 
-void region_metrics(const float* magn, std::int64_t m, int k, const std::int32_t* idx, std::int64_t nr,
+void unpack(const float* in, std::int64_t rows, int inner, int k, int K, float* out, hipStream_t s) {
+    if (const unsigned gr = grid_for(rows * inner * k)) unpack_kernel<<<gr, kBlock, 0, s>>>(in, rows, inner, k, K, out);
+}
+
+void region_metrics(const float* magn, std::int64_t m, int k, int K, const std::int32_t* idx, std::int64_t nr,
                     const float* vols, const double* bary, double* out_host, hipStream_t s) {
+    (void)m;
     std::memset(out_host, 0, sizeof(double) * kMetricsLen * k);
     if (nr <= 0) return;
     DeviceBuffer<float> vals(static_cast<std::size_t>(nr), "region values");
@@ -277,7 +291,7 @@ void region_metrics(const float* magn, std::int64_t m, int k, const std::int32_t
     std::vector<double> hp(8 * kStatBlocks), hv(kStatBlocks);
     for (int c = 0; c < k; ++c) {
         double* o = out_host + kMetricsLen * c;
-        gather_region_kernel<<<grid_for(nr), kBlock, 0, s>>>(magn, m, idx, nr, c, vals.get(), keys.get());
+        gather_region_kernel<<<grid_for(nr), kBlock, 0, s>>>(magn, K, idx, nr, c, vals.get(), keys.get());
         stats_kernel<<<kStatBlocks, kBlock, 0, s>>>(vals.get(), idx, vols, bary, nr, part.get());
         d2h(hp.data(), part.get(), hp.size(), s);
         sync(s, "region stats");

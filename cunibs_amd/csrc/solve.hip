@@ -95,131 +95,220 @@ __device__ __forceinline__ void block_vec_store(double v, double* __restrict__ o
     if (threadIdx.x < K) out[threadIdx.x] = acc[threadIdx.x];
 }
 
-// sum_{j in [j, e) step SUB} av[j] * X[ci[j] * K + c], four gathers in flight per step; the fold
-// order is the plain loop's.
-template <int K, int SUB>
-__device__ __forceinline__ float row_gather(const std::int32_t* __restrict__ ci, const float* __restrict__ av,
-                                            int j, int e, const float* __restrict__ X, int c) {
-    float acc = 0.0f;
-    for (; j + 3 * SUB < e; j += 4 * SUB) {
-        const std::int64_t c0 = ci[j], c1 = ci[j + SUB], c2 = ci[j + 2 * SUB], c3 = ci[j + 3 * SUB];
-        const float a0 = av[j], a1 = av[j + SUB], a2 = av[j + 2 * SUB], a3 = av[j + 3 * SUB];
-        const float x0 = X[c0 * K + c], x1 = X[c1 * K + c], x2 = X[c2 * K + c], x3 = X[c3 * K + c];
-        acc = fmaf(a0, x0, acc);
-        acc = fmaf(a1, x1, acc);
-        acc = fmaf(a2, x2, acc);
-        acc = fmaf(a3, x3, acc);
-    }
-    for (; j < e; j += SUB) acc = fmaf(av[j], X[static_cast<std::int64_t>(ci[j]) * K + c], acc);
-    return acc;
+// ---------------------------------------------------------------------------------------------
+// Vectorised row kernels. Each lane carries V consecutive columns of the (n, K) operands (16-byte
+// loads for K >= 4), so a row needs G = K / V lane groups; SUB sub-lanes per group split the row's
+// nonzeros and are folded by a fixed xor tree. L = G * SUB lanes per row.
+// ---------------------------------------------------------------------------------------------
+template <int K>
+constexpr int vec_width() {
+    return K >= 4 ? 4 : K;
 }
 
-// The same over the differenced off-diagonal operator: sum av[j] * (X[ci[j]] - xi).
-template <int K, int SUB>
-__device__ __forceinline__ float row_gather_diff(const std::int32_t* __restrict__ ci, const float* __restrict__ av,
-                                                 int j, int e, const float* __restrict__ X, int c, float xi) {
-    float acc = 0.0f;
-    for (; j + 3 * SUB < e; j += 4 * SUB) {
-        const std::int64_t c0 = ci[j], c1 = ci[j + SUB], c2 = ci[j + 2 * SUB], c3 = ci[j + 3 * SUB];
-        const float a0 = av[j], a1 = av[j + SUB], a2 = av[j + 2 * SUB], a3 = av[j + 3 * SUB];
-        const float x0 = X[c0 * K + c], x1 = X[c1 * K + c], x2 = X[c2 * K + c], x3 = X[c3 * K + c];
-        acc = fmaf(a0, x0 - xi, acc);
-        acc = fmaf(a1, x1 - xi, acc);
-        acc = fmaf(a2, x2 - xi, acc);
-        acc = fmaf(a3, x3 - xi, acc);
+template <int V>
+__device__ __forceinline__ void vload(const float* __restrict__ p, float (&v)[V]) {
+    if constexpr (V == 4) {
+        const float4 t = *reinterpret_cast<const float4*>(p);
+        v[0] = t.x;
+        v[1] = t.y;
+        v[2] = t.z;
+        v[3] = t.w;
+    } else if constexpr (V == 2) {
+        const float2 t = *reinterpret_cast<const float2*>(p);
+        v[0] = t.x;
+        v[1] = t.y;
+    } else {
+        v[0] = *p;
     }
-    for (; j < e; j += SUB) acc = fmaf(av[j], X[static_cast<std::int64_t>(ci[j]) * K + c] - xi, acc);
-    return acc;
+}
+
+template <int V>
+__device__ __forceinline__ void vstore(float* p, const float (&v)[V]) {
+    if constexpr (V == 4) {
+        *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    } else if constexpr (V == 2) {
+        *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
+    } else {
+        *p = v[0];
+    }
+}
+
+// acc[v] = sum over j in [j, e) step SUB of av[j] * (X[ci[j] * K + c0 + v] (- xi[v] when DIFF)),
+// two gathers in flight per step, folded in the plain loop's order.
+template <int K, int V, int SUB, bool DIFF>
+__device__ __forceinline__ void row_gather_v(const std::int32_t* __restrict__ ci, const float* __restrict__ av, int j,
+                                             int e, const float* __restrict__ X, int c0, const float (&xi)[V],
+                                             float (&acc)[V]) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = 0.0f;
+    for (; j + SUB < e; j += 2 * SUB) {
+        const std::int64_t k0 = ci[j], k1 = ci[j + SUB];
+        const float a0 = av[j], a1 = av[j + SUB];
+        float x0[V], x1[V];
+        vload<V>(X + k0 * K + c0, x0);
+        vload<V>(X + k1 * K + c0, x1);
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] = fmaf(a0, DIFF ? x0[v] - xi[v] : x0[v], acc[v]);
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] = fmaf(a1, DIFF ? x1[v] - xi[v] : x1[v], acc[v]);
+    }
+    if (j < e) {
+        const std::int64_t k0 = ci[j];
+        const float a0 = av[j];
+        float x0[V];
+        vload<V>(X + k0 * K + c0, x0);
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] = fmaf(a0, DIFF ? x0[v] - xi[v] : x0[v], acc[v]);
+    }
+}
+
+constexpr int kRowThreads = 256;
+constexpr int kRowWaves = kRowThreads / kWave;
+constexpr int kRowIters = 4;  // row groups each wave walks per block
+
+template <int K, int SUB>
+constexpr std::int64_t rows_per_vblock() {
+    return static_cast<std::int64_t>(kRowWaves) * (kWave / ((K / vec_width<K>()) * SUB)) * kRowIters;
+}
+
+enum RowMode : int { kResidual = 0, kCheby = 1, kSpmm = 2, kSpmmDiff = 3 };
+
+// One kernel for every SpMV-shaped pass:
+//   kResidual  out = b - A x
+//   kCheby     out = c_cur x + c_prev x_prev + alpha dinv (b - A x); out may alias x_prev
+//   kSpmm      out = A x
+//   kSpmmDiff  out = sum_{j != i} a_ij (x_j - x_i) + s_i x_i over the differenced operator, with
+//              `diag` holding the row sums s_i
+template <int K, int SUB, int MODE>
+__global__ void __launch_bounds__(kRowThreads) rowk_kernel(
+    std::int64_t n, const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
+    const float* __restrict__ av, const float* __restrict__ diag, const float* __restrict__ b,
+    const float* __restrict__ x, const float* x_prev, float c_cur, float c_prev, float alpha, float* out,
+    const int* __restrict__ state) {
+    if (finished(state)) return;
+    constexpr int V = vec_width<K>();
+    constexpr int G = K / V;
+    constexpr int L = G * SUB;
+    constexpr int RPW = kWave / L;
+    const int lb = logical_block();
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
+    const int l = lane % L, grp = l % G, sub = l / G;
+    const int c0 = grp * V;
+    const std::int64_t lo = static_cast<std::int64_t>(lb) * rows_per_vblock<K, SUB>();
+    const std::int64_t hi = min(lo + rows_per_vblock<K, SUB>(), n);
+    for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW; r0 < hi; r0 += kRowWaves * RPW) {
+        const std::int64_t row = r0 + lane / L;
+        const bool valid = row < hi;
+        const std::int64_t at = row * K + c0;
+        float xi[V], acc[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) xi[v] = 0.0f;
+        if constexpr (MODE == kSpmmDiff) {
+            if (valid) vload<V>(x + at, xi);
+        }
+        if (valid) {
+            row_gather_v<K, V, SUB, MODE == kSpmmDiff>(ci, av, rp[row] + sub, rp[row + 1], x, c0, xi, acc);
+        } else {
+#pragma unroll
+            for (int v = 0; v < V; ++v) acc[v] = 0.0f;
+        }
+#pragma unroll
+        for (int off = G; off < L; off <<= 1) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) acc[v] += __shfl_xor(acc[v], off, kWave);
+        }
+        if (!valid || sub != 0) continue;
+        float o[V];
+        if constexpr (MODE == kSpmm) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) o[v] = acc[v];
+        } else if constexpr (MODE == kSpmmDiff) {
+            const float s = diag[row];
+#pragma unroll
+            for (int v = 0; v < V; ++v) o[v] = fmaf(s, xi[v], acc[v]);
+        } else {
+            float bb[V];
+            vload<V>(b + at, bb);
+            if constexpr (MODE == kResidual) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) o[v] = bb[v] - acc[v];
+            } else {
+                float cur[V], prev[V];
+                vload<V>(x + at, cur);
+                vload<V>(x_prev + at, prev);  // read before the store below: out may alias x_prev
+                const float d = alpha * diag[row];
+#pragma unroll
+                for (int v = 0; v < V; ++v) o[v] = fmaf(d, bb[v] - acc[v], fmaf(c_prev, prev[v], c_cur * cur[v]));
+            }
+        }
+        vstore<V>(out + at, o);
+    }
+}
+
+// G1 = A^T B1 (and G2 = A^T B2), FP64 partials of one logical block: lane a of a row group holds
+// column a of A and accumulates row a of G from the B rows, which every lane of the group reads
+// whole (a broadcast). No cross-lane traffic until the block reduction.
+constexpr int kGramRowThreads = 256;
+
+template <int K, bool TWO>
+__global__ void __launch_bounds__(kGramRowThreads) gram_rows_kernel(
+    std::int64_t n, const float* __restrict__ A, const float* __restrict__ B1, const float* __restrict__ B2,
+    double* __restrict__ partials, std::int64_t rows_per_block, const int* __restrict__ state) {
+    if (finished(state)) return;
+    constexpr int RPW = kWave / K;
+    constexpr int WAVES = kGramRowThreads / kWave;
+    constexpr int V = vec_width<K>();
+    const int lb = logical_block();
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
+    const int a = lane % K, rg = lane / K;
+    const std::int64_t lo = static_cast<std::int64_t>(lb) * rows_per_block;
+    const std::int64_t hi = min(lo + rows_per_block, n);
+    double g1[K], g2[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) g1[c] = g2[c] = 0.0;
+    for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW; r0 < hi; r0 += WAVES * RPW) {
+        const std::int64_t row = r0 + rg;
+        if (row < hi) {
+            const double av = static_cast<double>(A[row * K + a]);
+#pragma unroll
+            for (int c = 0; c < K; c += V) {
+                float bv[V];
+                vload<V>(B1 + row * K + c, bv);
+#pragma unroll
+                for (int v = 0; v < V; ++v) g1[c + v] = fma(av, static_cast<double>(bv[v]), g1[c + v]);
+                if constexpr (TWO) {
+                    vload<V>(B2 + row * K + c, bv);
+#pragma unroll
+                    for (int v = 0; v < V; ++v) g2[c + v] = fma(av, static_cast<double>(bv[v]), g2[c + v]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+        g1[c] = wave_rows_sum<K>(g1[c]);
+        if constexpr (TWO) g2[c] = wave_rows_sum<K>(g2[c]);
+    }
+    // Block partial in wave order: lanes a < K of each wave add row a.
+    __shared__ double acc[(TWO ? 2 : 1) * K * K];
+    for (int e = threadIdx.x; e < (TWO ? 2 : 1) * K * K; e += blockDim.x) acc[e] = 0.0;
+    __syncthreads();
+    for (int w = 0; w < WAVES; ++w) {
+        if (wave == w && lane < K) {
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                acc[a * K + c] += g1[c];
+                if constexpr (TWO) acc[K * K + a * K + c] += g2[c];
+            }
+        }
+        __syncthreads();
+    }
+    double* out = partials + static_cast<std::int64_t>(lb) * (TWO ? 2 : 1) * K * K;
+    for (int e = threadIdx.x; e < (TWO ? 2 : 1) * K * K; e += blockDim.x) out[e] = acc[e];
 }
 
 constexpr int kGramThreads = 1024;
 constexpr int kGramWaves = kGramThreads / kWave;
-
-// ---------------------------------------------------------------------------------------------
-// Plain SpMM: Y = A X over the FP32 CSR, X and Y (n, K) row-major.
-// ---------------------------------------------------------------------------------------------
-template <int K>
-__global__ void __launch_bounds__(kGramThreads) spmm_kernel(std::int64_t n, const std::int32_t* __restrict__ rp,
-                                                            const std::int32_t* __restrict__ ci,
-                                                            const float* __restrict__ av, const float* __restrict__ X,
-                                                            float* __restrict__ Y, std::int64_t rows_per_block) {
-    constexpr int L = spmm_lanes<K>();
-    constexpr int SUB = L / K;
-    constexpr int RPW = kWave / L;
-    const int lb = logical_block();
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
-    const int l = lane % L, c = l % K, sub = l / K;
-    const std::int64_t lo = static_cast<std::int64_t>(lb) * rows_per_block;
-    const std::int64_t hi = min(lo + rows_per_block, n);
-    for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW; r0 < hi; r0 += kGramWaves * RPW) {
-        const std::int64_t row = r0 + lane / L;
-        const bool valid = row < hi;
-        float acc = valid ? row_gather<K, SUB>(ci, av, rp[row] + sub, rp[row + 1], X, c) : 0.0f;
-#pragma unroll
-        for (int off = K; off < L; off <<= 1) acc += __shfl_xor(acc, off, kWave);
-        if (valid && sub == 0) Y[row * K + c] = acc;
-    }
-}
-
-// ---------------------------------------------------------------------------------------------
-// Block-PCG SpMM: S = A W over the differenced operator, with the FP64 partials of G = W^T S and
-// H = W^T R in one pass. partials[lb] = [G (K x K) | H (K x K)]. With FUSED false only S is formed
-// (the differenced SpMM on its own).
-// ---------------------------------------------------------------------------------------------
-template <int K, bool FUSED>
-__global__ void __launch_bounds__(kGramThreads) spmm_block_kernel(
-    std::int64_t n, const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
-    const float* __restrict__ av, const float* __restrict__ rowsum, const float* __restrict__ W,
-    const float* __restrict__ R, float* __restrict__ S, double* __restrict__ partials,
-    std::int64_t rows_per_block, const int* __restrict__ state) {
-    if (finished(state)) return;
-    constexpr int L = spmm_lanes<K>();
-    constexpr int SUB = L / K;
-    constexpr int RPW = kWave / L;
-    const int lb = logical_block();
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
-    const int l = lane % L, c = l % K, sub = l / K;
-    const int group = lane - l;
-    const std::int64_t lo = static_cast<std::int64_t>(lb) * rows_per_block;
-    const std::int64_t hi = min(lo + rows_per_block, n);
-    double g[K], h[K];
-    if constexpr (FUSED) {
-#pragma unroll
-        for (int a = 0; a < K; ++a) g[a] = h[a] = 0.0;
-    }
-    for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW; r0 < hi; r0 += kGramWaves * RPW) {
-        const std::int64_t row = r0 + lane / L;
-        const bool valid = row < hi;
-        const std::int64_t at = row * K + c;
-        const float xi = valid ? W[at] : 0.0f;
-        float acc = valid ? row_gather_diff<K, SUB>(ci, av, rp[row] + sub, rp[row + 1], W, c, xi) : 0.0f;
-#pragma unroll
-        for (int off = K; off < L; off <<= 1) acc += __shfl_xor(acc, off, kWave);
-        const float s = valid ? fmaf(rowsum[row], xi, acc) : 0.0f;
-        if (valid && sub == 0) S[at] = s;
-        if constexpr (FUSED) {
-            const bool own = valid && sub == 0;
-            const double sd = own ? static_cast<double>(s) : 0.0;
-            const double rd = own ? static_cast<double>(R[at]) : 0.0;
-#pragma unroll
-            for (int a = 0; a < K; ++a) {
-                const double wa = static_cast<double>(__shfl(xi, group + a, kWave));
-                g[a] = fma(wa, sd, g[a]);
-                h[a] = fma(wa, rd, h[a]);
-            }
-        }
-    }
-    if constexpr (FUSED) {
-#pragma unroll
-        for (int a = 0; a < K; ++a) {
-            g[a] = wave_rows_sum<L>(g[a]);
-            h[a] = wave_rows_sum<L>(h[a]);
-        }
-        double* out = partials + static_cast<std::int64_t>(lb) * 2 * K * K;
-        block_gram_store<K, kGramWaves>(g, out);
-        block_gram_store<K, kGramWaves>(h, out + K * K);
-    }
-}
 
 // FP64 true residual R64 = B - A X64 with the FP64 operator, and per-column ||R64||^2 partials.
 template <int K>
@@ -258,55 +347,9 @@ __global__ void __launch_bounds__(kGramThreads) residual64_kernel(
 }
 
 // ---------------------------------------------------------------------------------------------
-// V-cycle kernels (solver/amg/vcycle.cu:124-311), K-wide, FP32. SUB lanes per column split each
-// row's nonzeros, chosen per level from its shape (wide coarse rows take more lanes).
+// V-cycle kernels (solver/amg/vcycle.cu:124-311), K-wide, FP32. The smoother and residual passes
+// are rowk_kernel; SUB is chosen per level from its shape (wide coarse rows take more lanes).
 // ---------------------------------------------------------------------------------------------
-constexpr int kRowThreads = 256;
-constexpr int kRowWaves = kRowThreads / kWave;
-constexpr int kRowIters = 4;  // row groups each wave walks per block
-
-template <int K, int SUB>
-constexpr std::int64_t rows_per_row_block() {
-    return static_cast<std::int64_t>(kRowWaves) * (kWave / (K * SUB)) * kRowIters;
-}
-
-// MODE 0: residual  out = b - A x
-// MODE 1: Chebyshev out = c_cur x + c_prev x_prev + alpha dinv (b - A x); out may alias x_prev.
-template <int K, int SUB, int MODE>
-__global__ void __launch_bounds__(kRowThreads) smoother_kernel(
-    std::int64_t n, const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
-    const float* __restrict__ av, const float* __restrict__ dinv, const float* __restrict__ b,
-    const float* __restrict__ x, const float* x_prev, float c_cur, float c_prev, float alpha, float* out,
-    const int* __restrict__ state) {
-    if (finished(state)) return;
-    constexpr int L = K * SUB;
-    constexpr int RPW = kWave / L;
-    const int lb = logical_block();
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
-    const int l = lane % L, c = l % K, sub = l / K;
-    const std::int64_t lo = static_cast<std::int64_t>(lb) * rows_per_row_block<K, SUB>();
-    const std::int64_t hi = min(lo + rows_per_row_block<K, SUB>(), n);
-    for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW; r0 < hi; r0 += kRowWaves * RPW) {
-        const std::int64_t row = r0 + lane / L;
-        const bool valid = row < hi;
-        float acc = valid ? row_gather<K, SUB>(ci, av, rp[row] + sub, rp[row + 1], x, c) : 0.0f;
-#pragma unroll
-        for (int off = K; off < L; off <<= 1) acc += __shfl_xor(acc, off, kWave);
-        if (valid && sub == 0) {
-            const std::int64_t at = row * K + c;
-            const float rhs = b[at] - acc;
-            if constexpr (MODE == 0) {
-                out[at] = rhs;
-            } else {
-                const float d = alpha * dinv[row];
-                const float cur = x[at];
-                const float prev = x_prev[at];  // read before the store below: out may alias x_prev
-                out[at] = fmaf(d, rhs, fmaf(c_prev, prev, c_cur * cur));
-            }
-        }
-    }
-}
-
 template <int K>
 __global__ void jacobi_zero_kernel(std::int64_t n, const float* __restrict__ dinv, const float* __restrict__ b,
                                    float alpha, float* __restrict__ x, const int* __restrict__ state) {
@@ -322,28 +365,48 @@ __global__ void __launch_bounds__(kRowThreads) restrict_kernel(
     const float* __restrict__ rv, const float* __restrict__ fine, float* __restrict__ out,
     const int* __restrict__ state) {
     if (finished(state)) return;
-    constexpr int SUB = kWave / K;
+    constexpr int V = vec_width<K>();
+    constexpr int G = K / V;
+    constexpr int SUB = kWave / G;
     const std::int64_t row = static_cast<std::int64_t>(blockIdx.x) * kRowWaves + (threadIdx.x >> 6);
     const int lane = threadIdx.x & (kWave - 1);
-    const int c = lane % K, sub = lane / K;
-    float acc = (row < n_coarse) ? row_gather<K, SUB>(ci, rv, rp[row] + sub, rp[row + 1], fine, c) : 0.0f;
+    const int grp = lane % G, sub = lane / G;
+    const float zero[V] = {};
+    float acc[V];
+    if (row < n_coarse) {
+        row_gather_v<K, V, SUB, false>(ci, rv, rp[row] + sub, rp[row + 1], fine, grp * V, zero, acc);
+    } else {
 #pragma unroll
-    for (int off = K; off < kWave; off <<= 1) acc += __shfl_xor(acc, off, kWave);
-    if (row < n_coarse && sub == 0) out[row * K + c] = acc;
+        for (int v = 0; v < V; ++v) acc[v] = 0.0f;
+    }
+#pragma unroll
+    for (int off = G; off < kWave; off <<= 1) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] += __shfl_xor(acc[v], off, kWave);
+    }
+    if (row < n_coarse && sub == 0) vstore<V>(out + row * K + grp * V, acc);
 }
 
-// x_fine += P x_coarse.
+// x_fine += P x_coarse, one lane per (row, column group).
 template <int K>
 __global__ void __launch_bounds__(kRowThreads) prolongate_kernel(
     std::int64_t n, const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
     const float* __restrict__ pv, const float* __restrict__ coarse, float* __restrict__ fine,
     const int* __restrict__ state) {
     if (finished(state)) return;
+    constexpr int V = vec_width<K>();
+    constexpr int G = K / V;
     const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= n * K) return;
-    const std::int64_t row = t / K;
-    const int c = static_cast<int>(t % K);
-    fine[t] += row_gather<K, 1>(ci, pv, rp[row], rp[row + 1], coarse, c);
+    if (t >= n * G) return;
+    const std::int64_t row = t / G;
+    const int c0 = static_cast<int>(t % G) * V;
+    const float zero[V] = {};
+    float acc[V], cur[V];
+    row_gather_v<K, V, 1, false>(ci, pv, rp[row], rp[row + 1], coarse, c0, zero, acc);
+    vload<V>(fine + row * K + c0, cur);
+#pragma unroll
+    for (int v = 0; v < V; ++v) cur[v] += acc[v];
+    vstore<V>(fine + row * K + c0, cur);
 }
 
 // Coarsest level in FP64: x = Ainv b, one wavefront per row of the dense inverse.
@@ -381,17 +444,39 @@ __global__ void __launch_bounds__(kRowThreads) coarse_kernel(int n, const double
 // Block-vector kernels of the PCG iteration.
 // ---------------------------------------------------------------------------------------------
 
+// Per-column partials of a (K)-vector held as V entries by the lanes of column group `grp`:
+// summed over the wave's rows, then added wave by wave into the block's slot.
+template <int K, int V, int WAVES>
+__device__ __forceinline__ void block_cols_store(double (&v)[V], int grp, double* __restrict__ out) {
+    constexpr int G = K / V;
+    __shared__ double acc[K];
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int u = 0; u < V; ++u) v[u] = wave_rows_sum<G>(v[u]);
+    if (threadIdx.x < K) acc[threadIdx.x] = 0.0;
+    __syncthreads();
+    for (int w = 0; w < WAVES; ++w) {
+        if (wave == w && lane < G) {
+#pragma unroll
+            for (int u = 0; u < V; ++u) acc[grp * V + u] += v[u];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < K) out[threadIdx.x] = acc[threadIdx.x];
+}
+
 // X64 += W a (FP64), R -= S a (FP32), partials of ||R_c||^2. Frozen columns have a zero column
-// in `a`, which leaves X and R exactly unchanged.
+// in `a`, which leaves X and R exactly unchanged. A lane owns V columns of one row and reads the
+// row's W and S whole (the other lanes of the row read the same cache lines).
 template <int K>
-__global__ void __launch_bounds__(kGramThreads) update_xr_kernel(
+__global__ void __launch_bounds__(kRowThreads) update_xr_kernel(
     std::int64_t n, const float* __restrict__ W, const float* __restrict__ S, const double* __restrict__ a,
     double* __restrict__ X, float* __restrict__ R, double* __restrict__ partials, std::int64_t rows_per_block,
     const int* __restrict__ state) {
     if (finished(state)) return;
-    constexpr int L = spmm_lanes<K>();
-    constexpr int RPW = kWave / L;
-    constexpr int SUB = L / K;
+    constexpr int V = vec_width<K>();
+    constexpr int G = K / V;
+    constexpr int RPW = kWave / G;
     __shared__ double al[K * K];
     __shared__ float alf[K * K];
     for (int e = threadIdx.x; e < K * K; e += blockDim.x) {
@@ -401,98 +486,83 @@ __global__ void __launch_bounds__(kGramThreads) update_xr_kernel(
     __syncthreads();
     const int lb = logical_block();
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
-    const int l = lane % L, c = l % K, sub = l / K;
-    const int group = lane - l;
+    const int grp = lane % G, c0 = grp * V;
     const std::int64_t lo = static_cast<std::int64_t>(lb) * rows_per_block;
     const std::int64_t hi = min(lo + rows_per_block, n);
-    double nrm = 0.0;
-    for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW; r0 < hi; r0 += kGramWaves * RPW) {
-        const std::int64_t row = r0 + lane / L;
-        const bool valid = row < hi;
-        const std::int64_t at = row * K + c;
-        const float w_own = valid ? W[at] : 0.0f;
-        const float s_own = valid ? S[at] : 0.0f;
-        double dx = 0.0;
-        float dr = 0.0f;
+    double nrm[V];
 #pragma unroll
-        for (int b = 0; b < K; ++b) {
-            const float wb = __shfl(w_own, group + b, kWave);
-            const float sb = __shfl(s_own, group + b, kWave);
-            dx = fma(static_cast<double>(wb), al[b * K + c], dx);
-            dr = fmaf(sb, alf[b * K + c], dr);
+    for (int v = 0; v < V; ++v) nrm[v] = 0.0;
+    for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW; r0 < hi; r0 += kRowWaves * RPW) {
+        const std::int64_t row = r0 + lane / G;
+        if (row >= hi) continue;
+        double dx[V];
+        float dr[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            dx[v] = 0.0;
+            dr[v] = 0.0f;
         }
-        if (valid && sub == 0) {
-            X[at] += dx;
-            const float r = R[at] - dr;
-            R[at] = r;
-            nrm = fma(static_cast<double>(r), static_cast<double>(r), nrm);
+#pragma unroll
+        for (int b0 = 0; b0 < K; b0 += V) {
+            float wb[V], sb[V];
+            vload<V>(W + row * K + b0, wb);
+            vload<V>(S + row * K + b0, sb);
+#pragma unroll
+            for (int u = 0; u < V; ++u) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    dx[v] = fma(static_cast<double>(wb[u]), al[(b0 + u) * K + c0 + v], dx[v]);
+                    dr[v] = fmaf(sb[u], alf[(b0 + u) * K + c0 + v], dr[v]);
+                }
+            }
         }
+        const std::int64_t at = row * K + c0;
+        float rr[V];
+        vload<V>(R + at, rr);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            X[at + v] += dx[v];
+            rr[v] -= dr[v];
+            nrm[v] = fma(static_cast<double>(rr[v]), static_cast<double>(rr[v]), nrm[v]);
+        }
+        vstore<V>(R + at, rr);
     }
-    nrm = wave_rows_sum<L>(nrm);
-    block_vec_store<K, kGramWaves>(nrm, partials + static_cast<std::int64_t>(lb) * K);
+    block_cols_store<K, V, kRowWaves>(nrm, grp, partials + static_cast<std::int64_t>(lb) * K);
 }
 
-// partials of E = S^T Z.
-template <int K>
-__global__ void __launch_bounds__(kGramThreads) gram_kernel(std::int64_t n, const float* __restrict__ A,
-                                                            const float* __restrict__ B,
-                                                            double* __restrict__ partials,
-                                                            std::int64_t rows_per_block,
-                                                            const int* __restrict__ state) {
-    if (finished(state)) return;
-    constexpr int L = spmm_lanes<K>();
-    constexpr int RPW = kWave / L;
-    constexpr int SUB = L / K;
-    const int lb = logical_block();
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
-    const int l = lane % L, c = l % K, sub = l / K;
-    const int group = lane - l;
-    const std::int64_t lo = static_cast<std::int64_t>(lb) * rows_per_block;
-    const std::int64_t hi = min(lo + rows_per_block, n);
-    double g[K];
-#pragma unroll
-    for (int a = 0; a < K; ++a) g[a] = 0.0;
-    for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW; r0 < hi; r0 += kGramWaves * RPW) {
-        const std::int64_t row = r0 + lane / L;
-        const bool valid = row < hi;
-        const std::int64_t at = row * K + c;
-        const float a_own = valid ? A[at] : 0.0f;
-        const double bb = (valid && sub == 0) ? static_cast<double>(B[at]) : 0.0;
-#pragma unroll
-        for (int a = 0; a < K; ++a) {
-            const float aa = __shfl(a_own, group + a, kWave);
-            g[a] = fma(static_cast<double>(aa), bb, g[a]);
-        }
-    }
-#pragma unroll
-    for (int a = 0; a < K; ++a) g[a] = wave_rows_sum<L>(g[a]);
-    block_gram_store<K, kGramWaves>(g, partials + static_cast<std::int64_t>(lb) * K * K);
-}
-
-// W <- Z - W D, in place (each lane reads its row's W before any lane of the row writes it).
+// W <- Z - W D, in place: every lane of a row loads the row's W before any of them stores (the
+// stores depend on the loads, and a row's lanes share one wavefront).
 template <int K>
 __global__ void __launch_bounds__(kRowThreads) update_w_kernel(std::int64_t n, const float* __restrict__ Z,
                                                                const double* __restrict__ D, float* W,
                                                                const int* __restrict__ state) {
     if (finished(state)) return;
-    constexpr int L = spmm_lanes<K>();
+    constexpr int V = vec_width<K>();
+    constexpr int G = K / V;
     __shared__ float dm[K * K];
     for (int e = threadIdx.x; e < K * K; e += blockDim.x) dm[e] = static_cast<float>(D[e]);
     __syncthreads();
-    const int lane = threadIdx.x & (kWave - 1);
-    const int l = lane % L, c = l % K, sub = l / K;
-    const int group = lane - l;
-    const std::int64_t row = (static_cast<std::int64_t>(blockIdx.x) * kRowThreads + threadIdx.x) / L;
+    const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * kRowThreads + threadIdx.x;
+    const std::int64_t row = t / G;
     const bool valid = row < n;
-    const std::int64_t at = row * K + c;
-    const float w_own = valid ? W[at] : 0.0f;
-    float acc = valid ? Z[at] : 0.0f;
+    const int c0 = static_cast<int>(t % G) * V;
+    float acc[V];
 #pragma unroll
-    for (int b = 0; b < K; ++b) {
-        const float wb = __shfl(w_own, group + b, kWave);
-        acc = fmaf(-wb, dm[b * K + c], acc);
+    for (int v = 0; v < V; ++v) acc[v] = 0.0f;
+    if (valid) vload<V>(Z + row * K + c0, acc);
+#pragma unroll
+    for (int b0 = 0; b0 < K; b0 += V) {
+        float wb[V];
+#pragma unroll
+        for (int u = 0; u < V; ++u) wb[u] = 0.0f;
+        if (valid) vload<V>(W + row * K + b0, wb);
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) acc[v] = fmaf(-wb[u], dm[(b0 + u) * K + c0 + v], acc[v]);
+        }
     }
-    if (valid && sub == 0) W[at] = acc;
+    if (valid) vstore<V>(W + row * K + c0, acc);
 }
 
 __global__ void cast_f64_f32_kernel(std::int64_t count, const double* __restrict__ in, float* __restrict__ out) {
@@ -703,29 +773,40 @@ __global__ void column_norms_kernel(const double* __restrict__ partials, int nb,
 
 std::int64_t gram_rows(std::int64_t n) { return (n + kGramBlocks - 1) / kGramBlocks; }
 
-// Lanes per column for a level's row kernels: enough that each lane walks a few nonzeros, more
-// when the level is too small to fill the machine (solver/amg/vcycle.cu:104-120 makes the same
-// trade for its threads-per-row).
-int choose_sub(std::int64_t n_rows, std::int64_t nnz, int K, int n_cu) {
+// Sub-lanes per column group (G = K / V groups per row) for a level's row kernels: enough that
+// each lane walks a few nonzeros, more when the level is too small to fill the machine
+// (solver/amg/vcycle.cu:104-120 makes the same trade for its threads-per-row).
+int choose_sub(std::int64_t n_rows, std::int64_t nnz, int G, int n_cu) {
     const std::int64_t per_row = n_rows > 0 ? (nnz + n_rows - 1) / n_rows : 1;
     int sub = 1;
-    while (K * sub * 2 <= kWave) {
+    while (G * sub * 2 <= kWave) {
         const bool pays = per_row >= 4 * (sub * 2);
-        const bool starved = n_rows * K * sub < static_cast<std::int64_t>(n_cu) * 8 * kWave;
+        const bool starved = n_rows * G * sub < static_cast<std::int64_t>(n_cu) * 8 * kWave;
         if (!pays && !starved) break;
         sub *= 2;
     }
     return sub;
 }
 
+template <int K>
+constexpr int col_groups() {
+    return K / vec_width<K>();
+}
+
+// rowk_kernel<K, sub, MODE> over the rows of `a`; `diag` is dinv (kCheby) or the row sums
+// (kSpmmDiff).
 template <int K, int MODE>
-void launch_smoother_sub(int sub, const CsrView& a, const float* dinv, const float* b, const float* x, const float* xp,
-                         float cc, float cp, float al, float* out, const int* state, hipStream_t s) {
+void launch_rows(int sub, const CsrView& a, const float* diag, const float* b, const float* x, const float* xp,
+                 float cc, float cp, float al, float* out, const int* state, hipStream_t s) {
     auto go = [&](auto subc) {
         constexpr int SUB = decltype(subc)::value;
-        if constexpr (K * SUB <= kWave) {
-            const unsigned g = xcd_grid((a.n + rows_per_row_block<K, SUB>() - 1) / rows_per_row_block<K, SUB>());
-            if (g) smoother_kernel<K, SUB, MODE><<<g, kRowThreads, 0, s>>>(a.n, a.ptr, a.col, a.val, dinv, b, x, xp, cc, cp, al, out, state);
+        if constexpr (col_groups<K>() * SUB <= kWave) {
+            const unsigned g = xcd_grid((a.n + rows_per_vblock<K, SUB>() - 1) / rows_per_vblock<K, SUB>());
+            if (g)
+                rowk_kernel<K, SUB, MODE>
+                    <<<g, kRowThreads, 0, s>>>(a.n, a.ptr, a.col, a.val, diag, b, x, xp, cc, cp, al, out, state);
+        } else {
+            throw InvalidArgument("sub-lanes per column group exceed a wavefront");
         }
     };
     switch (sub) {
@@ -758,14 +839,14 @@ void run_vcycle(const Hierarchy& h, const std::vector<int>& subs, std::vector<De
         if (x0 == nullptr) {
             jacobi_zero_kernel<K><<<grid_for(L.a.n * K), kBlock, 0, s>>>(L.a.n, L.dinv.get(), bi, ch.alpha0, target(0), state);
         } else {
-            launch_smoother_sub<K, 1>(sub, L.a, L.dinv.get(), bi, x0, x0, 1.0f, 0.0f, ch.alpha0, target(0), state, s);
+            launch_rows<K, kCheby>(sub, L.a, L.dinv.get(), bi, x0, x0, 1.0f, 0.0f, ch.alpha0, target(0), state, s);
         }
         float* cur = target(0);
         const float* prev = (x0 == nullptr) ? cur : other;
         for (int st = 1; st < ch.degree; ++st) {
             float* out = target(st);
             const float cprev = (st == 1 && x0 == nullptr) ? 0.0f : ch.c_prev[st - 1];
-            launch_smoother_sub<K, 1>(sub, L.a, L.dinv.get(), bi, cur, prev, ch.c_cur[st - 1], cprev, ch.alpha[st - 1],
+            launch_rows<K, kCheby>(sub, L.a, L.dinv.get(), bi, cur, prev, ch.c_cur[st - 1], cprev, ch.alpha[st - 1],
                                       out, state, s);
             prev = cur;
             cur = out;
@@ -777,7 +858,7 @@ void run_vcycle(const Hierarchy& h, const std::vector<int>& subs, std::vector<De
         const Level& L = h.levels[i];
         const float* bi = (i == 0) ? B : lb[i].get();
         smooth(i, bi, nullptr, odd ? lx[i].get() : lr[i].get(), odd ? lr[i].get() : lx[i].get(), nullptr);
-        launch_smoother_sub<K, 0>(subs[i], L.a, L.dinv.get(), bi, lx[i].get(), lx[i].get(), 0.f, 0.f, 0.f, lr[i].get(),
+        launch_rows<K, kResidual>(subs[i], L.a, L.dinv.get(), bi, lx[i].get(), lx[i].get(), 0.f, 0.f, 0.f, lr[i].get(),
                                   state, s);
         float* next_b = (i + 1 < nl) ? lb[i + 1].get() : cb;
         restrict_kernel<K><<<grid_for(L.n_coarse, kRowWaves), kRowThreads, 0, s>>>(
@@ -789,7 +870,7 @@ void run_vcycle(const Hierarchy& h, const std::vector<int>& subs, std::vector<De
     for (int i = nl - 1; i >= 0; --i) {
         const Level& L = h.levels[i];
         const float* bi = (i == 0) ? B : lb[i].get();
-        prolongate_kernel<K><<<grid_for(L.a.n * K), kBlock, 0, s>>>(L.a.n, L.p.ptr.get(), L.p.col.get(), L.p.val.get(),
+        prolongate_kernel<K><<<grid_for(L.a.n * col_groups<K>()), kBlock, 0, s>>>(L.a.n, L.p.ptr.get(), L.p.col.get(), L.p.val.get(),
                                                                     xc, lx[i].get(), state);
         xc = smooth(i, bi, lx[i].get(), lr[i].get(), lx[i].get(), (i == 0) ? X : nullptr);
     }
@@ -824,8 +905,15 @@ BlockSolver::BlockSolver(const Operator& op, const Hierarchy& h, hipStream_t s) 
 }
 
 BlockSolver::~BlockSolver() {
-    if (graph_) (void)hipGraphExecDestroy(graph_);
+    drop_graphs();
     if (host_state_) (void)hipHostFree(host_state_);
+}
+
+void BlockSolver::drop_graphs() {
+    for (auto& g : graphs_) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
 }
 
 void BlockSolver::ensure(int K) {
@@ -859,10 +947,7 @@ void BlockSolver::ensure(int K) {
     target_.reset(K, "targets");
     state_.reset(2 + 2 * kMaxBlock, "solver state");
     cap_k_ = K;
-    if (graph_) {
-        (void)hipGraphExecDestroy(graph_);
-        graph_ = nullptr;
-    }
+    drop_graphs();  // the captured iterations named the buffers just replaced
 }
 
 template <int K>
@@ -870,16 +955,24 @@ void BlockSolver::vcycle_k(const float* B, float* X, const int* state) {
     // Lanes per column of each level's row kernels: a property of the level and the width alone.
     std::vector<int> subs(h_.levels.size());
     for (std::size_t i = 0; i < h_.levels.size(); ++i)
-        subs[i] = choose_sub(h_.levels[i].a.n, h_.levels[i].a.nnz, K, n_cu_);
+        subs[i] = choose_sub(h_.levels[i].a.n, h_.levels[i].a.nnz, col_groups<K>(), n_cu_);
     run_vcycle<K>(h_, subs, lb_, lx_, lr_, cb_.get(), cx_.get(), B, X, state, s_);
 }
 
 template <int K>
+int BlockSolver::fine_sub() const {
+    if (fine_sub_override_ > 0) return fine_sub_override_;
+    return choose_sub(op_.off.n, op_.off.nnz, col_groups<K>(), n_cu_);
+}
+
+// S = A W (differenced), then the partials of G = W^T S and H = W^T R.
+template <int K>
 void BlockSolver::spmm_block_k(const int* state) {
     const std::int64_t nn = n();
-    spmm_block_kernel<K, true><<<kGramBlocks, kGramThreads, 0, s_>>>(nn, op_.off.ptr, op_.off.col, op_.off.val,
-                                                                     op_.rowsum, W_.get(), R_.get(), S_.get(),
-                                                                     partials_.get(), gram_rows(nn), state);
+    launch_rows<K, kSpmmDiff>(fine_sub<K>(), op_.off, op_.rowsum, nullptr, W_.get(), nullptr, 0.f, 0.f, 0.f,
+                              S_.get(), state, s_);
+    gram_rows_kernel<K, true><<<kGramBlocks, kGramRowThreads, 0, s_>>>(nn, W_.get(), S_.get(), R_.get(),
+                                                                        partials_.get(), gram_rows(nn), state);
 }
 
 template <int K>
@@ -891,13 +984,14 @@ void BlockSolver::iteration_k(const SolveOptions& opt) {
     spmm_block_k<K>(st);
     coef_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, opt.drop_tol, G_.get(), T_.get(),
                                                 TT_.get(), AL_.get(), stw);
-    update_xr_kernel<K><<<kGramBlocks, kGramThreads, 0, s_>>>(nn, W_.get(), S_.get(), AL_.get(), X64_.get(), R_.get(),
-                                                              partials_.get(), rpb, st);
+    update_xr_kernel<K><<<kGramBlocks, kRowThreads, 0, s_>>>(nn, W_.get(), S_.get(), AL_.get(), X64_.get(), R_.get(),
+                                                             partials_.get(), rpb, st);
     converge_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, target_.get(), norms_.get(), stw);
     vcycle_k<K>(R_.get(), Z_.get(), st);
-    gram_kernel<K><<<kGramBlocks, kGramThreads, 0, s_>>>(nn, S_.get(), Z_.get(), partials_.get(), rpb, st);
+    gram_rows_kernel<K, false><<<kGramBlocks, kGramRowThreads, 0, s_>>>(nn, S_.get(), Z_.get(), nullptr,
+                                                                         partials_.get(), rpb, st);
     dmat_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, TT_.get(), D_.get(), stw);
-    update_w_kernel<K><<<grid_for(nn * spmm_lanes<K>(), kRowThreads), kRowThreads, 0, s_>>>(nn, Z_.get(), D_.get(),
+    update_w_kernel<K><<<grid_for(nn * col_groups<K>(), kRowThreads), kRowThreads, 0, s_>>>(nn, Z_.get(), D_.get(),
                                                                                            W_.get(), st);
 }
 
@@ -954,18 +1048,20 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
         cast_f64_f32_kernel<<<grid_for(nk), kBlock, 0, s_>>>(nk, R64_.get(), R_.get());
         vcycle_k<K>(R_.get(), Z_.get(), nullptr);
         d2d(W_.get(), Z_.get(), static_cast<std::size_t>(nk), s_);
-        if (opt.use_graph && (graph_ == nullptr || graph_k_ != K)) {
-            if (graph_) {
-                (void)hipGraphExecDestroy(graph_);
-                graph_ = nullptr;
+        int slot = 0;
+        while (kBlockWidths[slot] != K) ++slot;
+        if (opt.use_graph && (graphs_[slot] == nullptr || graph_drop_[slot] != opt.drop_tol)) {
+            if (graphs_[slot]) {
+                (void)hipGraphExecDestroy(graphs_[slot]);
+                graphs_[slot] = nullptr;
             }
             hipGraph_t g = nullptr;
             check(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal), "begin capture");
             iteration_k<K>(opt);
             check(hipStreamEndCapture(s_, &g), "end capture");
-            check(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0), "graph instantiate");
+            check(hipGraphInstantiate(&graphs_[slot], g, nullptr, nullptr, 0), "graph instantiate");
             (void)hipGraphDestroy(g);
-            graph_k_ = K;
+            graph_drop_[slot] = opt.drop_tol;
         }
         // Bounded by the launches the host issues, not only by the device counter.
         int launched = total_iters;
@@ -974,7 +1070,7 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
             const int batch = std::min(opt.check_every, opt.max_iters - launched);
             for (int b = 0; b < batch; ++b) {
                 if (opt.use_graph)
-                    check(hipGraphLaunch(graph_, s_), "graph launch");
+                    check(hipGraphLaunch(graphs_[slot], s_), "graph launch");
                 else
                     iteration_k<K>(opt);
             }
@@ -1018,21 +1114,19 @@ SolveStats BlockSolver::solve(const float* B, double* X, const double* X0, int k
 }
 
 void BlockSolver::spmm(const float* X, float* Y, int K) {
-    const std::int64_t nn = n();
     dispatch_k(K, [&](auto kc) {
         constexpr int KK = decltype(kc)::value;
-        spmm_kernel<KK><<<kGramBlocks, kGramThreads, 0, s_>>>(nn, op_.a32.ptr, op_.a32.col, op_.a32.val, X, Y,
-                                                              gram_rows(nn));
+        const int sub = fine_sub_override_ > 0 ? fine_sub_override_
+                                               : choose_sub(op_.a32.n, op_.a32.nnz, col_groups<KK>(), n_cu_);
+        launch_rows<KK, kSpmm>(sub, op_.a32, nullptr, nullptr, X, nullptr, 0.f, 0.f, 0.f, Y, nullptr, s_);
     });
 }
 
 void BlockSolver::spmm_diff(const float* X, float* Y, int K) {
-    const std::int64_t nn = n();
     dispatch_k(K, [&](auto kc) {
         constexpr int KK = decltype(kc)::value;
-        spmm_block_kernel<KK, false><<<kGramBlocks, kGramThreads, 0, s_>>>(nn, op_.off.ptr, op_.off.col, op_.off.val,
-                                                                           op_.rowsum, X, nullptr, Y, nullptr,
-                                                                           gram_rows(nn), nullptr);
+        launch_rows<KK, kSpmmDiff>(fine_sub<KK>(), op_.off, op_.rowsum, nullptr, X, nullptr, 0.f, 0.f, 0.f, Y,
+                                   nullptr, s_);
     });
 }
 
@@ -1041,15 +1135,25 @@ void BlockSolver::vcycle(const float* B, float* X, int K) {
     dispatch_k(K, [&](auto kc) { vcycle_k<decltype(kc)::value>(B, X, nullptr); });
 }
 
-double BlockSolver::bench_spmm(int K, int reps, bool fused) {
+double BlockSolver::bench_spmm(int K, int reps, int mode, int sub) {
     require(padded_width(K) == K, "K must be a compiled block width");
+    require(mode >= 0 && mode <= 2, "mode must be 0, 1 or 2");
+    require(sub >= 0 && sub <= kWave && (sub & (sub - 1)) == 0, "sub must be 0 or a power of two <= 64");
+    require(sub == 0 || (K / (K >= 4 ? 4 : K)) * sub <= kWave, "sub-lanes exceed a wavefront for this K");
     ensure(K);
     const std::int64_t nk = n() * K;
     fill_pattern_kernel<<<grid_for(nk), kBlock, 0, s_>>>(nk, W_.get());
     fill_pattern_kernel<<<grid_for(nk), kBlock, 0, s_>>>(nk, R_.get());
+    struct Restore {
+        int& v;
+        ~Restore() { v = 0; }
+    } restore{fine_sub_override_};
+    fine_sub_override_ = sub;
     auto launch = [&] {
-        if (fused)
+        if (mode == 2)
             dispatch_k(K, [&](auto kc) { spmm_block_k<decltype(kc)::value>(nullptr); });
+        else if (mode == 1)
+            spmm_diff(W_.get(), S_.get(), K);
         else
             spmm(W_.get(), S_.get(), K);
     };

@@ -68,7 +68,9 @@ class BlockSolver {
     void spmm_diff(const float* X, float* Y, int K);  // Y = A X (FP32, differenced form)
     void vcycle(const float* B, float* X, int K);   // X = M B, one zero-guess V-cycle
     // Average device time (ms) over `reps` launches, HIP events on the solver's stream.
-    double bench_spmm(int K, int reps, bool fused);
+    // mode 0: plain SpMM; 1: differenced SpMM (the iteration's S = A W); 2: 1 followed by the Gram
+    // pass G = W^T S, H = W^T R. sub > 0 forces the sub-lanes per column group of the fine level.
+    double bench_spmm(int K, int reps, int mode, int sub);
     double bench_vcycle(int K, int reps);
     std::int64_t n() const { return op_.a32.n; }
     hipStream_t stream() const { return s_; }
@@ -82,6 +84,8 @@ class BlockSolver {
     void iteration_k(const SolveOptions& opt);
     template <int K>
     void spmm_block_k(const int* state);
+    template <int K>
+    int fine_sub() const;
     void ensure(int K);
 
     Operator op_;
@@ -89,6 +93,7 @@ class BlockSolver {
     hipStream_t s_;
     int cap_k_ = 0;
     int n_cu_ = 256;
+    int fine_sub_override_ = 0;  // bench_spmm's forced sub-lane count while it runs
     // block vectors (n, K)
     DeviceBuffer<float> R_, Z_, W_, S_;
     DeviceBuffer<double> R64_, X64_;
@@ -100,8 +105,12 @@ class BlockSolver {
     DeviceBuffer<double> G_, T_, TT_, AL_, D_, norms_, target_;
     DeviceBuffer<int> state_;  // [0] done, [1] iteration, [2..2+K) frozen, [2+K..2+2K) col_iters
     int* host_state_ = nullptr;  // pinned
-    hipGraphExec_t graph_ = nullptr;
-    int graph_k_ = 0;
+    // One captured iteration per block width, kept for the solver's lifetime: every buffer the
+    // iteration names is solver-owned and reallocated only when the capacity grows (which drops
+    // them all).
+    hipGraphExec_t graphs_[kNumBlockWidths] = {};
+    double graph_drop_[kNumBlockWidths] = {};  // the drop tolerance each graph was captured with
+    void drop_graphs();
 };
 
 }  // namespace cn

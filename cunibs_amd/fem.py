@@ -294,9 +294,10 @@ class SolverContext:
         check(lib().cn_k_place(self.handle, ptr(c), ptr(h), ptr(d), c.shape[0], ptr(out), ptr(deg)), "cn_k_place")
         return out, deg
 
-    def bench_spmm(self, K: int, reps: int = 20, fused_gram: bool = True) -> float:
+    def bench_spmm(self, K: int, reps: int = 20, mode: int = 1, sub: int = 0) -> float:
+        """Average ms of the fine-level product: mode 0 plain SpMM, 1 differenced SpMM, 2 with Grams."""
         ms = ctypes.c_double()
-        check(lib().cn_bench_spmm(self.handle, K, reps, int(fused_gram), ctypes.byref(ms)), "cn_bench_spmm")
+        check(lib().cn_bench_spmm(self.handle, K, reps, mode, sub, ctypes.byref(ms)), "cn_bench_spmm")
         return float(ms.value)
 
     def bench_vcycle(self, K: int, reps: int = 10) -> float:

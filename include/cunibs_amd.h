@@ -193,9 +193,12 @@ int32_t cn_k_dadt(const float* s, const float* mp, const float* sn, int64_t n_di
 int32_t cn_k_place(cn_context* ctx, const double* centers_mm, const double* handles_mm,
                    const double* distances_mm, int32_t n_pl, double* out, int32_t* degenerate);
 
-/* Average device time (ms) of `reps` launches of the block solve's SpMM (with its fused Gram)
- * at width K, timed with HIP events on the library's stream. For roofline reporting. */
-int32_t cn_bench_spmm(cn_context* ctx, int32_t K, int32_t reps, int32_t fused_gram, double* ms);
+/* Average device time (ms) of `reps` launches of the block solve's fine-level product at width K,
+ * timed with HIP events on the library's stream. For roofline reporting and launch tuning.
+ * mode 0: plain CSR SpMM Y = A X; 1: the differenced SpMM of the block iteration (S = A W);
+ * 2: mode 1 followed by its Gram pass (W^T S, W^T R). sub = 0 picks the sub-lanes per column
+ * group the solver uses; a power of two forces that count. */
+int32_t cn_bench_spmm(cn_context* ctx, int32_t K, int32_t reps, int32_t mode, int32_t sub, double* ms);
 /* Same for one V-cycle at width K. */
 int32_t cn_bench_vcycle(cn_context* ctx, int32_t K, int32_t reps, double* ms);
 
