@@ -20,8 +20,10 @@ from cunibs_amd._lib import SolverConvergenceError, check, lib, ptr
 
 DEFAULT_TOLERANCE = 1e-6  # fem/solve.py:56
 DEFAULT_MAX_ITERS = 2000  # fem/solve.py:57
-BLOCK_SIZES: tuple[int, ...] = ()
-MAX_BLOCK = 32
+# Compiled block widths, read from the library (solver/__init__.py BLOCK_SIZES there). Importing
+# this module loads the HIP library; there is no CPU fallback.
+BLOCK_SIZES: tuple[int, ...] = _lib.block_sizes()
+MAX_BLOCK: int = _lib.max_block()
 MU0_OVER_4PI = 1e-7  # fem/placement.py:16
 
 TISSUE_CONDUCTIVITY: dict[int, float] = {
@@ -37,10 +39,6 @@ METRIC_FIELDS = (
 
 
 def _blocks() -> tuple[int, ...]:
-    global BLOCK_SIZES, MAX_BLOCK
-    if not BLOCK_SIZES:
-        BLOCK_SIZES = _lib.block_sizes()
-        MAX_BLOCK = _lib.max_block()
     return BLOCK_SIZES
 
 

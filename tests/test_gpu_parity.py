@@ -1,0 +1,267 @@
+"""HIP library against the CPU oracle on the same seeded inputs (4 mm five-shell head, 56k nodes).
+
+Integer and index work (layout, CSR pattern, aggregates) and the FP64 assembly must be bit-exact;
+FP32 products within the stated tolerances; the end-to-end GM |E| within 1e-3 of the FP64 oracle
+(CG to 1e-12), the bar the throughput metric is quoted at. Every call goes through the C ABI.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GM_ERR_BAR = 1e-3
+
+
+def rel(a, b) -> float:
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def octx(head4, oracle_mod):
+    return oracle_mod.build_context(head4.nodes_mm, head4.tet_nodes, head4.tet_tags, head4.skin_tris)
+
+
+@pytest.fixture(scope="module")
+def gctx(head4):
+    from cunibs_amd import fem
+    ctx = fem.SolverContext(head4.nodes_mm, head4.tet_nodes, head4.tet_tags, head4.skin_tris, keep_stiffness=True)
+    yield ctx
+    ctx.free()
+
+
+def test_mesh_layout_is_bit_exact(gctx, octx):
+    """fem/solve.py:645-662: Morton node order, tets by lowest renumbered node."""
+    np.testing.assert_array_equal(gctx.node_perm, octx.node_perm)
+    np.testing.assert_array_equal(gctx.tet_perm, octx.tet_perm)
+    np.testing.assert_array_equal(gctx.nodes_mm, octx.nodes_mm)
+    np.testing.assert_array_equal(gctx.tet_nodes, octx.tet_nodes)
+    np.testing.assert_array_equal(gctx.tet_tags, octx.tet_tags)
+    np.testing.assert_array_equal(gctx.skin_tris, octx.skin_tris)
+
+
+def test_stiffness_is_bit_exact(gctx, octx):
+    ip, ii, dd = gctx.stiffness()
+    np.testing.assert_array_equal(ip, octx.indptr)
+    np.testing.assert_array_equal(ii, octx.indices)
+    np.testing.assert_array_equal(dd, octx.data)
+
+
+def test_grounded_operator_is_bit_exact(gctx, octx):
+    rp, rc, rv = gctx.reduced()
+    assert gctx.ground_node == octx.ground
+    np.testing.assert_array_equal(rp, octx.reduced.ptr)
+    np.testing.assert_array_equal(rc, octx.reduced.col)
+    np.testing.assert_array_equal(rv, octx.reduced_val64)
+
+
+def test_amg_hierarchy_is_bit_exact(gctx, octx):
+    """Aggregates, l1 diagonals, prolongators and Galerkin operators of every level."""
+    assert gctx.hierarchy_sizes == octx.hier.sizes
+    for i, lv in enumerate(octx.hier.levels):
+        g = gctx.level(i)
+        np.testing.assert_array_equal(g["agg"], lv.agg)
+        np.testing.assert_array_equal(g["a_ptr"], lv.a.ptr)
+        np.testing.assert_array_equal(g["a_col"], lv.a.col)
+        np.testing.assert_array_equal(g["a_val"], lv.a.val.astype(np.float32))
+        np.testing.assert_array_equal(g["dinv"], lv.dinv)
+        np.testing.assert_array_equal(g["p_col"], lv.p.col)
+        np.testing.assert_array_equal(g["p_val"], lv.p.val)
+    gc = gctx.level(len(octx.hier.levels))
+    np.testing.assert_array_equal(gc["a_col"], octx.hier.coarse.col)
+    np.testing.assert_array_equal(gc["a_val"], octx.hier.coarse.val)
+
+
+def test_skin_normals(gctx, octx):
+    np.testing.assert_allclose(gctx.skin_normals(), octx.skin_normals, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("K", [1, 2, 4, 8, 16, 32])
+def test_block_spmm_plain_and_differenced(gctx, octx, oracle_mod, K):
+    """FP32 CSR x K products against the FP64 product of the same FP32 matrix."""
+    n = octx.reduced.n
+    X = np.random.default_rng(K).standard_normal((n, K)).astype(np.float32)
+    ref = np.stack([oracle_mod.spmv(octx.reduced, octx.reduced.val.astype(np.float64), X[:, c].astype(np.float64))
+                    for c in range(K)], 1)
+    assert rel(gctx.spmm(X), ref) < 1e-6
+    assert rel(gctx.spmm_diff(X), ref) < 1e-6
+
+
+@pytest.mark.parametrize("K", [1, 4, 8, 32])
+def test_vcycle_against_the_fp64_vcycle(gctx, octx, oracle_mod, K):
+    n = octx.reduced.n
+    B = np.random.default_rng(100 + K).standard_normal((n, K)).astype(np.float32)
+    got = gctx.vcycle(B)
+    ref = np.stack([oracle_mod.vcycle_apply(octx.hier, B[:, c].astype(np.float64)) for c in range(K)], 1)
+    assert rel(got, ref) < 1e-5
+
+
+def test_placement_transforms(gctx, octx, oracle_mod):
+    from cunibs_amd import synthetic
+    sites = synthetic.cap_sites(12, seed=5)
+    c = np.array([s[0] for s in sites]); h = np.array([s[1] for s in sites]); d = np.array([s[2] for s in sites])
+    tf, deg = gctx.place(c, h, d)
+    otf, odeg = oracle_mod.place_transforms(c, h, d, octx.nodes_mm, octx.skin_tris, octx.skin_normals)
+    np.testing.assert_array_equal(deg, odeg)
+    np.testing.assert_allclose(tf, otf, rtol=0, atol=1e-9)
+
+
+@pytest.fixture(scope="module")
+def single(gctx, octx, oracle_mod, coil):
+    from cunibs_amd import fem, synthetic
+    site = synthetic.SPHERE_SITE
+    ref = oracle_mod.solve_site(octx, coil.positions_m, coil.moments, site, 1e6, tol=1e-12)
+    extra = {"dadt_elm": 1, "rhs": 1}
+    res = fem.solve_placements_block(gctx, coil.positions_m, coil.moments, [site], 1e6, potential=True,
+                                     vectors=True, extra=extra)[0]
+    return res, ref, extra
+
+
+def test_single_placement_end_to_end(single, octx):
+    """The 4 mm sphere-head case: dA/dt, RHS, solve, E, |E| against the FP64 oracle."""
+    res, ref, extra = single
+    assert rel(extra["dadt_elm"][0], ref["dadt_elm"]) < 1e-5
+    assert rel(extra["rhs"][0], ref["b"]) < 1e-5
+    assert res.relative_residual <= 1e-6
+    np.testing.assert_allclose(res.transform, ref["transform"], atol=1e-9)
+    gm = octx.tet_tags == 2
+    assert rel(res.magnE[gm], ref["magnE"][gm]) < GM_ERR_BAR * 0.1
+    assert rel(res.v, ref["v"]) < 1e-3
+    assert rel(res.E, ref["E"]) < 1e-3
+    np.testing.assert_allclose(np.linalg.norm(res.E.astype(np.float64), axis=1), res.magnE, rtol=1e-5)
+
+
+def test_true_residual_in_fp64(single, gctx):
+    """The reported residual is the true FP64 residual ||b - A x|| / ||b|| of the returned potential."""
+    import scipy.sparse as sp
+    res, _, extra = single
+    rp, rc, rv = gctx.reduced()
+    a = sp.csr_matrix((rv, rc, rp), shape=(len(rp) - 1,) * 2)
+    keep = np.arange(gctx.n_nodes) != gctx.ground_node
+    b = extra["rhs"][0][keep].astype(np.float64)
+    x = res.v[keep] - res.v[gctx.ground_node]
+    r = np.linalg.norm(b - a @ x) / np.linalg.norm(b)
+    assert r <= 1e-6
+    assert r == pytest.approx(res.relative_residual, rel=1e-3, abs=1e-12)
+
+
+def test_gm_metrics_match_the_oracle_summary(single, gctx, oracle_mod):
+    """metrics.py:209-243 on the GPU's own |E|: the device reduction equals the NumPy restatement."""
+    from cunibs_amd import fem
+    res, _, _ = single
+    gm = np.nonzero(gctx.tet_tags == 2)[0]
+    vols = gctx.vols[gm]
+    bary = gctx.tet_barycenters_mm[gm]
+    want = oracle_mod.summarize(res.magnE[gm], vols, bary, volume_m3=float(vols.astype(np.float64).sum()))
+    got = fem.metrics_dict(res.metrics)
+    assert got["peak_magnE"] == want["peak_magnE"]
+    np.testing.assert_allclose(got["peak_location_mm"], want["peak_location_mm"], rtol=1e-12)
+    np.testing.assert_allclose(got["center_of_gravity_mm"], want["center_of_gravity_mm"], rtol=1e-9)
+    assert got["region_volume_m3"] == pytest.approx(want["region_volume_m3"], rel=1e-9)
+    for key in ("mean", "std", "p50", "p95", "p99", "p99.9"):
+        assert got["distribution"][key] == pytest.approx(want["distribution"][key], rel=1e-6), key
+    for key in ("0.5", "0.75"):
+        assert got["focality_m3"][key] == pytest.approx(want["focality_m3"][key], rel=1e-6), key
+
+
+@pytest.fixture(scope="module")
+def block8(gctx, coil):
+    from cunibs_amd import fem, synthetic
+    sites = synthetic.cap_sites(8, seed=3)
+    return sites, fem.solve_placements_block(gctx, coil.positions_m, coil.moments, sites, 1e6)
+
+
+def test_block_of_eight_against_the_oracle(block8, octx, oracle_mod, coil):
+    sites, res = block8
+    gm = octx.tet_tags == 2
+    for i in (0, 5):
+        ref = oracle_mod.solve_site(octx, coil.positions_m, coil.moments, sites[i], 1e6, tol=1e-12)
+        assert rel(res[i].magnE[gm], ref["magnE"][gm]) < GM_ERR_BAR * 0.1
+    assert all(r.relative_residual <= 1e-6 for r in res)
+
+
+def test_block_solve_is_bitwise_reproducible(block8, gctx, coil):
+    """Fixed-order reductions: the same block gives the same bits."""
+    from cunibs_amd import fem
+    sites, res = block8
+    again = fem.solve_placements_block(gctx, coil.positions_m, coil.moments, sites, 1e6)
+    for a, b in zip(res, again):
+        np.testing.assert_array_equal(a.magnE, b.magnE)
+        np.testing.assert_array_equal(a.metrics, b.metrics)
+        assert a.iterations == b.iterations
+
+
+@pytest.mark.parametrize("k", [1, 3, 5])
+def test_partial_blocks_agree_with_the_full_block(block8, gctx, coil, k):
+    """Padded widths (3 -> 4, 5 -> 8) and single placements give the same field within the
+    block method's tolerance (the reference's PARITY = 2e-5, tests/test_block_solve.py:20)."""
+    from cunibs_amd import fem
+    sites, res = block8
+    sub = fem.solve_placements_block(gctx, coil.positions_m, coil.moments, sites[:k], 1e6)
+    for i in range(k):
+        assert rel(sub[i].magnE, res[i].magnE) < 2e-5
+
+
+def test_degenerate_placement_raises(gctx, coil):
+    """place.cu:153-230: a handle along the scalp normal names the placement in a ValueError."""
+    from cunibs_amd import fem
+    c = np.array([0.0, 0.0, 95.0])
+    with pytest.raises(ValueError):
+        fem.solve_placements_block(gctx, coil.positions_m, coil.moments, [(c, c + [0, 0, 10.0], 4.0)], 1e6)
+
+
+def test_missed_tolerance_raises(gctx, coil):
+    """tests/test_block_solve.py:321-376: an unreachable tolerance within a small iteration cap
+    raises SolverConvergenceError (forced from the host; every kernel exits normally)."""
+    from cunibs_amd import fem, synthetic
+    with pytest.raises(fem.SolverConvergenceError):
+        fem.solve_placements_block(gctx, coil.positions_m, coil.moments, [synthetic.SPHERE_SITE], 1e6,
+                                   tolerance=1e-14, max_iters=3)
+
+
+# ------------------------------------------------------------------------------------------------
+# Kernel-level entry points (one per reference binding)
+# ------------------------------------------------------------------------------------------------
+
+def test_kernel_gradients_morton_node2corner(head4, oracle_mod):
+    from cunibs_amd import fem
+    g, v = fem.p1_gradients(head4.nodes_mm, head4.tet_nodes)
+    og, ov = oracle_mod.p1_gradients(head4.nodes_mm, head4.tet_nodes)
+    np.testing.assert_array_equal(g, og)
+    np.testing.assert_array_equal(v, ov)
+    np.testing.assert_array_equal(fem.morton_keys(head4.nodes_mm), oracle_mod.morton_keys(head4.nodes_mm))
+    p, i = fem.node2corner(head4.tet_nodes, head4.nodes_mm.shape[0])
+    op, oi = oracle_mod.node2corner(head4.tet_nodes, head4.nodes_mm.shape[0])
+    np.testing.assert_array_equal(p, op)
+    np.testing.assert_array_equal(i, oi)
+
+
+def test_kernel_l1_size4_spgemm(octx, oracle_mod):
+    from cunibs_amd import fem
+    a = octx.hier.levels[0].a
+    np.testing.assert_array_equal(fem.l1_dinv(a.n, a.ptr, a.col, a.val), oracle_mod.l1_dinv(a))
+    agg, nc = fem.select_size4(a.n, a.ptr, a.col, a.val)
+    oagg, onc = oracle_mod.select_size4(a)
+    assert nc == onc
+    np.testing.assert_array_equal(agg, oagg)
+    p = octx.hier.levels[0].p
+    cp_, cc, cv = fem.spgemm(a.ptr, a.col, a.val, p.ptr, p.col, p.val, p.n_cols)
+    want = oracle_mod.spgemm(a, p)
+    np.testing.assert_array_equal(cp_, want.ptr)
+    np.testing.assert_array_equal(cc, want.col)
+    np.testing.assert_array_equal(cv, want.val)
+
+
+def test_kernel_dadt_nbody(oracle_mod):
+    """solver/coil/dadt.cu:27-104 against the FP64 closed form (tests/test_placement.py:334-346)."""
+    from cunibs_amd import fem
+    rng = np.random.default_rng(12)
+    s = rng.normal(0, 0.02, (300, 3))
+    m = rng.normal(0, 1e-3, (300, 3))
+    r = rng.normal(0, 0.04, (5000, 3)) + [0, 0, -0.07]
+    mp = np.concatenate([m, np.cross(m, s)], 1)
+    sn = (s * s).sum(1)
+    got = fem.dadt_nbody(s, mp, sn, r, 1e6)
+    assert rel(got, oracle_mod.dadt_direct(s, m, r, 1e6)) < 1e-5
