@@ -48,6 +48,8 @@ def main() -> int:
     ap.add_argument("--accuracy", type=int, default=2, help="placements checked against the FP64 oracle")
     ap.add_argument("--cpu-sample", type=int, default=2, help="placements the CPU oracle solves")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--warm-basis", type=int, default=256,
+                    help="training placements whose solutions form the warm-start basis (0: cold only)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the placement sweep shards "
@@ -68,21 +70,45 @@ def main() -> int:
     k = args.block_k
     pos, mom = coil.positions_m, coil.moments
 
-    def block(i0: int, kk: int):
+    def block(i0: int, kk: int, warm: bool = False):
         chunk = [sites[(i0 + j) % len(sites)] for j in range(kk)]
-        return fem.solve_placements_block(ctx, pos, mom, chunk, 1e6, magnitude=False, metrics=True)
+        return fem.solve_placements_block(ctx, pos, mom, chunk, 1e6, magnitude=False, metrics=True,
+                                          warm_start=warm)
 
-    for w in range(args.warmup):
-        block(w * k, k)
-    iters = []
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        res = block(s * k, k)
-        iters.extend(r.iterations for r in res)
-    elapsed = time.perf_counter() - t0
+    def sweep(warm: bool):
+        for w in range(args.warmup):
+            block(w * k, k, warm)
+        its = []
+        t_start = time.perf_counter()
+        for s in range(args.steps):
+            res = block(s * k, k, warm)
+            its.extend(r.iterations for r in res)
+        return time.perf_counter() - t_start, its
+
+    cold_elapsed, cold_iters = sweep(False)
+    cold_value = args.steps * k / cold_elapsed
+    log(f"batched cold k={k}: {cold_value:.1f} placements/s, {1e3 * cold_elapsed / args.steps:.1f} ms/step, "
+        f"iterations mean {np.mean(cold_iters):.1f}")
+    train_s = 0.0
+    rank = 0
+    if args.warm_basis > 0:
+        # Warm-start basis: solutions of training placements drawn uniformly on the same cap
+        # (a different seed from any sweep), solved once, then the Galerkin projector.
+        t0 = time.perf_counter()
+        ctx.warm_reset(args.warm_basis)
+        train = synthetic.cap_sites(args.warm_basis, seed=7)
+        for i in range(0, len(train), 16):
+            fem.solve_placements_block(ctx, pos, mom, train[i:i + 16], 1e6, metrics=False, magnitude=False,
+                                       collect=True)
+        rank = ctx.warm_finalize()
+        train_s = time.perf_counter() - t0
+        log(f"warm basis: {args.warm_basis} training solves, rank {rank}, {train_s:.2f}s")
+        elapsed, iters = sweep(True)
+    else:
+        elapsed, iters = cold_elapsed, cold_iters
     value = args.steps * k / elapsed
-    log(f"batched k={k}: {value:.1f} placements/s, {1e3 * elapsed / args.steps:.1f} ms/step, "
-        f"iterations mean {np.mean(iters):.1f}")
+    log(f"batched{' warm' if args.warm_basis > 0 else ''} k={k}: {value:.1f} placements/s, "
+        f"{1e3 * elapsed / args.steps:.1f} ms/step, iterations mean {np.mean(iters):.1f}")
 
     # Plain single-placement GPU PCG on the same sites.
     plain_iters = [0]
@@ -133,8 +159,14 @@ def main() -> int:
             "workload": f"{args.spacing} mm perturbed 5-shell head, 16x16 cap-grid sweep, block_k={k}",
             "n_nodes": ctx.n_nodes, "n_tets": ctx.n_tets, "nnz_reduced": nnz,
             "hierarchy": ctx.hierarchy_sizes, "block_k": k, "tolerance": 1e-6, "recovery": "raw",
-            "warm_start": "none",
+            "warm_start": (f"Galerkin projection on {args.warm_basis} solver-generated solutions (BF16 basis, "
+                           f"MFMA expansion)") if args.warm_basis > 0 else "none",
         },
+        "batched_cold_placements_per_s": round(cold_value, 3),
+        "cold_iterations_mean": round(float(np.mean(cold_iters)), 2),
+        "warm_basis_rank": rank,
+        "warm_training_s": round(train_s, 3),
+        "value_incl_training": round(args.steps * k / (elapsed + train_s), 3),
         "plain_placements_per_s": round(plain, 3),
         "speedup_vs_plain": round(value / plain, 3),
         "iterations_mean": round(float(np.mean(iters)), 2),
@@ -171,7 +203,8 @@ def main() -> int:
         for i in range(args.accuracy):
             gres = block(i * 37, 1)[0]
             ref = oracle.solve_site(octx, pos, mom, sites[(i * 37) % len(sites)], 1e6, tol=1e-12)
-            g = fem.solve_placements_block(ctx, pos, mom, [sites[(i * 37) % len(sites)]], 1e6)[0]
+            g = fem.solve_placements_block(ctx, pos, mom, [sites[(i * 37) % len(sites)]], 1e6,
+                                           warm_start=args.warm_basis > 0)[0]
             errs.append(float(np.linalg.norm(g.magnE[gm] - ref["magnE"][gm]) / np.linalg.norm(ref["magnE"][gm])))
             del gres
         out["gm_magnE_rel_err_vs_fp64"] = max(errs) if errs else None

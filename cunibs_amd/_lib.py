@@ -16,6 +16,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "libcunibs_amd.so"
 
 CN_OK, CN_EINVAL, CN_EHIP, CN_ENOCONV, CN_EINTERNAL = 0, 1, 2, 3, 4
+RECOVERY_CODES = {"raw": 0, "spr_global": 1, "spr_tissue": 2, "harmonic": 3}
 METRICS_LEN = 16
 MAX_LEVELS = 32
 
@@ -55,7 +56,7 @@ class Info(ctypes.Structure):
 class SolveOptions(ctypes.Structure):
     _fields_ = [
         ("tolerance", _D), ("max_iters", _I32), ("max_refine", _I32), ("drop_tol", _D),
-        ("use_graph", _I32), ("check_every", _I32),
+        ("use_graph", _I32), ("check_every", _I32), ("recovery", _I32), ("warm_start", _I32), ("collect", _I32),
     ]
 
 
@@ -63,7 +64,7 @@ class BlockOutputs(ctypes.Structure):
     _fields_ = [
         ("transforms", _P), ("magnE", _P), ("E", _P), ("v", _P), ("metrics", _P),
         ("iterations", _P), ("relative_residual", _P), ("dadt_elm", _P), ("dadt_nodes", _P),
-        ("rhs", _P), ("degenerate", _P), ("total_iterations", _I32), ("refinements", _I32),
+        ("rhs", _P), ("degenerate", _P), ("E_slots", _P), ("x0", _P), ("total_iterations", _I32), ("refinements", _I32),
         ("round_iterations", _I32 * 8), ("round_residual", _D * 8),
     ]
 
@@ -85,6 +86,12 @@ _SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "cn_context_level": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "cn_context_skin_normals": (_I32, [_P, _P]),
     "cn_solve_block": (_I32, [_P, _P, _P, _I64, _P, _P, _P, _I32, _D, _P, _P, _P]),
+    "cn_recovery_build": (_I32, [_P, _I32, _P, _P]),
+    "cn_warm_reset": (_I32, [_P, _I32]),
+    "cn_warm_count": (_I32, [_P, _P]),
+    "cn_warm_gram": (_I32, [_P, _P]),
+    "cn_warm_set_projector": (_I32, [_P, _P, _I32]),
+    "cn_recovery_operator": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
     "cn_k_p1_gradients": (_I32, [_P, _I64, _P, _I64, _P, _P]),
     "cn_k_morton_keys": (_I32, [_P, _I64, _P]),
     "cn_k_tet_lowest_node": (_I32, [_P, _I64, _P, _I64, _P]),

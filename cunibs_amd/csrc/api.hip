@@ -12,6 +12,8 @@
 #include "../../include/cunibs_amd.h"
 #include "coil.hpp"
 #include "prims.hpp"
+#include "recovery.hpp"
+#include "warm.hpp"
 #include "solve.hpp"
 
 namespace {
@@ -87,6 +89,11 @@ struct cn_context {
     cn::DeviceBuffer<std::int32_t> d_degen;
     cn::DeviceBuffer<float> d_dip, d_dadt_nodes, d_dadt_elm, d_q, d_B, d_E, d_magn, d_rhs, d_stage;
     cn::DeviceBuffer<double> d_X, d_v, d_x0;
+    // recovery operators, built on first use (index: CN_RECOVERY_*), and their block buffers
+    std::unique_ptr<cn::RecoveryOp> recovery[4];
+    cn::DeviceBuffer<float> d_slots, d_Eraw;
+    int max_tag = 0;
+    cn::WarmBasis warm;
 
     cn::CsrView reduced_view() const {
         return {rptr.get(), rcol.get(), rval32.get(), n - 1, n - 1, nnz_red};
@@ -212,6 +219,7 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
         lut.assign(kDefaultSigma, kDefaultSigma + 11);
     const int unknown = conductivity(c.tags.get(), m, lut.data(), static_cast<int>(lut.size()), cond.get(), s);
     if (unknown) throw InvalidArgument(std::to_string(unknown) + " tetrahedra have unknown tags");
+    c.max_tag = static_cast<int>(lut.size()) - 1;  // every tag indexes the table
 
     c.n2c_ptr.reset(n + 1, "n2c ptr");
     c.n2c_idx.reset(4 * m, "n2c idx");
@@ -366,6 +374,20 @@ void place_block(cn_context& c, const double* centers, const double* handles, co
     sync(s, "placement");
 }
 
+const cn::RecoveryOp* ensure_recovery(cn_context& c, int mode) {
+    using namespace cn;
+    require(mode >= CN_RECOVERY_RAW && mode <= CN_RECOVERY_HARMONIC, "unknown recovery mode");
+    require(mode != CN_RECOVERY_HARMONIC, "harmonic recovery is not implemented in this build");
+    require(mode != CN_RECOVERY_RAW, "the raw mode has no recovery operator");
+    if (!c.recovery[mode]) {
+        auto op = std::make_unique<RecoveryOp>();
+        build_spr(*op, mode, c.nodes.get(), c.tets.get(), c.tags.get(), c.vols32.get(), c.n2c_ptr.get(),
+                  c.n2c_idx.get(), c.n, c.m, c.max_tag, c.s);
+        c.recovery[mode] = std::move(op);
+    }
+    return c.recovery[mode].get();
+}
+
 void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int64_t n_dip, const double* centers,
                  const double* handles, const double* dists, int k, double didt, const cn_solve_options* sopt,
                  const double* x0, cn_block_outputs* out) {
@@ -440,16 +462,43 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
         c.d_x0.ensure(nr * K, "x0");
         h2d(c.d_x0.get(), blk.data(), blk.size(), s);
         dx0 = c.d_x0.get();
+    } else if (sopt && sopt->warm_start) {
+        c.d_x0.ensure(nr * K, "x0");
+        c.warm.initial_guess(c.d_B.get(), k, K, c.d_x0.get(), s);
+        dx0 = c.d_x0.get();
+    }
+    if (out && out->x0) {
+        // (n_red, K) device block -> (k, n_red) host, zeros for a cold start
+        std::vector<double> blk(static_cast<std::size_t>(nr * K), 0.0);
+        if (dx0) {
+            d2h(blk.data(), dx0, blk.size(), s);
+            sync(s, "x0 copy");
+        }
+        for (int ci = 0; ci < k; ++ci)
+            for (std::int64_t r = 0; r < nr; ++r) out->x0[ci * nr + r] = blk[r * K + ci];
     }
     SolveStats st = c.solver->solve(c.d_B.get(), c.d_X.get(), dx0, k, opt);
-    // 4. reconstruction and metrics.
+    if (sopt && sopt->collect) c.warm.append(c.d_X.get(), k, K, s);
+    // 4. reconstruction, recovery and metrics.
+    const int mode = sopt ? sopt->recovery : CN_RECOVERY_RAW;
+    const RecoveryOp* rec = mode == CN_RECOVERY_RAW ? nullptr : ensure_recovery(c, mode);
     c.d_magn.ensure(m * K, "magnE");
     float* dE = nullptr;
     if (out && out->E) {
         c.d_E.ensure(3 * m * K, "E");
         dE = c.d_E.get();
     }
-    reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), c.d_dadt_elm.get(), m, k, K, c.ground, dE, c.d_magn.get(), s);
+    if (rec == nullptr) {
+        reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), c.d_dadt_elm.get(), m, k, K, c.ground, dE, c.d_magn.get(),
+                    s);
+    } else {
+        // The raw field feeds the patch fits; |E| and E then describe the recovered field.
+        c.d_Eraw.ensure(3 * m * K, "raw E");
+        c.d_slots.ensure(3 * rec->n_slots * K, "recovered slots");
+        reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), c.d_dadt_elm.get(), m, k, K, c.ground, c.d_Eraw.get(),
+                    c.d_magn.get(), s);
+        apply_spr(*rec, c.d_Eraw.get(), m, k, K, c.d_slots.get(), dE, c.d_magn.get(), s);
+    }
     if (out) {
         // Placement-fastest device arrays -> placement-major host arrays through one staging buffer.
         auto fetch = [&](float* host, const float* dev, std::int64_t rows, int inner) {
@@ -460,6 +509,10 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
         };
         if (out->magnE) fetch(out->magnE, c.d_magn.get(), m, 1);
         if (out->E) fetch(out->E, c.d_E.get(), m, 3);
+        if (out->E_slots) {
+            require(rec != nullptr, "E_slots needs a recovery mode");
+            fetch(out->E_slots, c.d_slots.get(), rec->n_slots, 3);
+        }
         if (out->dadt_elm) fetch(out->dadt_elm, c.d_dadt_elm.get(), m, 3);
         if (out->dadt_nodes) fetch(out->dadt_nodes, c.d_dadt_nodes.get(), n, 3);
         if (out->rhs) d2h(out->rhs, c.d_rhs.get(), n * k, s);
@@ -524,6 +577,9 @@ void cn_default_options(cn_options* o) {
 }
 
 void cn_default_solve_options(cn_solve_options* o) {
+    o->recovery = CN_RECOVERY_RAW;
+    o->warm_start = 0;
+    o->collect = 0;
     o->tolerance = 1e-6;
     o->max_iters = 2000;
     o->max_refine = 8;
@@ -872,6 +928,84 @@ int32_t cn_bench_vcycle(cn_context* c, int32_t K, int32_t reps, double* ms) {
     return guarded([&] {
         cn::require(c != nullptr && reps >= 1, "bad argument");
         *ms = c->solver->bench_vcycle(K, reps);
+    });
+}
+
+
+int32_t cn_recovery_build(cn_context* c, int32_t mode, int64_t* n_slots, int32_t* n_fallback) {
+    return guarded([&] {
+        cn::require(c != nullptr, "null context");
+        const cn::RecoveryOp* op = ensure_recovery(*c, mode);
+        if (n_slots) *n_slots = op->n_slots;
+        if (n_fallback) *n_fallback = op->n_fallback;
+    });
+}
+
+int32_t cn_recovery_operator(const cn_context* c, int32_t mode, int32_t* ptr, int32_t* idx, float* w,
+                             int32_t* slot_of_corner, int32_t* slot_node, int32_t* slot_tag, int32_t* boundary,
+                             int32_t* status) {
+    return guarded([&] {
+        cn::require(c != nullptr && mode >= 1 && mode <= 3 && c->recovery[mode] != nullptr,
+                    "recovery operator not built (call cn_recovery_build first)");
+        const cn::RecoveryOp& op = *c->recovery[mode];
+        const hipStream_t s = c->s;
+        const std::int64_t nc = 4 * c->m;
+        if (ptr) cn::d2h(ptr, op.ptr_view, op.n_slots + 1, s);
+        if (idx) cn::d2h(idx, op.idx_view, nc, s);
+        if (w) cn::d2h(w, op.w.get(), nc, s);
+        if (slot_of_corner) cn::d2h(slot_of_corner, op.soc_view, nc, s);
+        if (slot_node) cn::d2h(slot_node, op.slot_node.get(), op.n_slots, s);
+        if (slot_tag && mode == CN_RECOVERY_SPR_TISSUE) cn::d2h(slot_tag, op.slot_tag.get(), op.n_slots, s);
+        if (boundary) cn::d2h(boundary, op.boundary.get(), c->n, s);
+        if (status) cn::d2h(status, op.status.get(), op.n_slots, s);
+        cn::sync(s, "recovery operator copy");
+    });
+}
+
+int32_t cn_warm_reset(cn_context* c, int32_t capacity) {
+    return guarded([&] {
+        cn::require(c != nullptr, "null context");
+        c->warm.reset(c->n - 1, capacity);
+    });
+}
+
+int32_t cn_warm_count(const cn_context* c, int32_t* count) {
+    return guarded([&] {
+        cn::require(c != nullptr && count != nullptr, "null argument");
+        *count = c->warm.count;
+    });
+}
+
+int32_t cn_warm_gram(cn_context* c, double* G) {
+    return guarded([&] {
+        cn::require(c != nullptr && G != nullptr, "null argument");
+        const int S = c->warm.count;
+        cn::require(S >= 1, "the warm-start basis is empty");
+        const std::int64_t nr = c->n - 1;
+        const int chunks = (S + 31) / 32;
+        std::vector<cn::DeviceBuffer<float>> w(chunks);
+        for (int i = 0; i < chunks; ++i) {
+            w[i].reset(static_cast<std::size_t>(nr) * 32, "basis chunk");
+            c->warm.chunk(32 * i, w[i].get(), c->s);
+        }
+        cn::DeviceBuffer<float> aw(static_cast<std::size_t>(nr) * 32, "A W chunk");
+        std::vector<double> tile(32 * 32);
+        for (int j = 0; j < chunks; ++j) {
+            c->solver->spmm_diff(w[j].get(), aw.get(), 32);
+            for (int i = 0; i < chunks; ++i) {
+                c->solver->cross_gram32(w[i].get(), aw.get(), tile.data());
+                for (int a = 0; a < 32 && 32 * i + a < S; ++a)
+                    for (int b = 0; b < 32 && 32 * j + b < S; ++b)
+                        G[static_cast<std::int64_t>(32 * i + a) * S + 32 * j + b] = tile[a * 32 + b];
+            }
+        }
+    });
+}
+
+int32_t cn_warm_set_projector(cn_context* c, const double* P, int32_t count) {
+    return guarded([&] {
+        cn::require(c != nullptr && P != nullptr, "null argument");
+        c->warm.set_projector(P, count, c->s);
     });
 }
 

@@ -1,0 +1,39 @@
+// Patch recovery operators (recovery.hip).
+#pragma once
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace cn {
+
+// Mode numbers of the C ABI (CN_RECOVERY_*).
+constexpr int kRecoveryRaw = 0;
+constexpr int kRecoverySprGlobal = 1;
+constexpr int kRecoverySprTissue = 2;
+constexpr int kRecoveryHarmonic = 3;
+
+// One SPR operator: a corner CSR over slots, a weight per corner, and the maps back to the mesh.
+// For "spr_global" the CSR and the corner-to-slot map are the context's node2corner and
+// connectivity (views, not copies).
+struct RecoveryOp {
+    int mode = kRecoveryRaw;
+    std::int64_t n_slots = 0;
+    int n_fallback = 0;  // slots that took the volume-weighted average
+    DeviceBuffer<std::int32_t> ptr, idx, soc, slot_node, slot_tag, boundary, status;
+    DeviceBuffer<float> w;
+    const std::int32_t* ptr_view = nullptr;
+    const std::int32_t* idx_view = nullptr;
+    const std::int32_t* soc_view = nullptr;
+};
+
+// mark[node] = 1 on the outer boundary of the tetrahedral volume (faces with a single owner).
+void outer_boundary_nodes(const std::int32_t* tets, std::int64_t m, const std::int32_t* n2c_ptr,
+                          const std::int32_t* n2c_idx, std::int64_t n, std::int32_t* mark, hipStream_t s);
+void build_spr(RecoveryOp& op, int mode, const double* nodes, const std::int32_t* tets, const std::int32_t* tags,
+               const float* vols, const std::int32_t* n2c_ptr, const std::int32_t* n2c_idx, std::int64_t n,
+               std::int64_t m, int max_tag, hipStream_t s);
+// E (m, 3, K) -> slots (n_slots, 3, K), E_out (m, 3, K) (may be null), magn (m, K).
+void apply_spr(const RecoveryOp& op, const float* E, std::int64_t m, int k, int K, float* slots, float* E_out,
+               float* magn, hipStream_t s);
+
+}  // namespace cn

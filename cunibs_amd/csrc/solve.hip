@@ -134,31 +134,40 @@ __device__ __forceinline__ void vstore(float* p, const float (&v)[V]) {
 }
 
 // acc[v] = sum over j in [j, e) step SUB of av[j] * (X[ci[j] * K + c0 + v] (- xi[v] when DIFF)),
-// two gathers in flight per step, folded in the plain loop's order.
-template <int K, int V, int SUB, bool DIFF>
+// folded in the plain loop's order. Entries go in batches of U: the U index/value loads, then the
+// U gathers, are all issued before the first use, so a typical row is two memory round trips.
+template <int K, int V, int SUB, bool DIFF, int U = 4>
 __device__ __forceinline__ void row_gather_v(const std::int32_t* __restrict__ ci, const float* __restrict__ av, int j,
                                              int e, const float* __restrict__ X, int c0, const float (&xi)[V],
                                              float (&acc)[V]) {
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v] = 0.0f;
-    for (; j + SUB < e; j += 2 * SUB) {
-        const std::int64_t k0 = ci[j], k1 = ci[j + SUB];
-        const float a0 = av[j], a1 = av[j + SUB];
-        float x0[V], x1[V];
-        vload<V>(X + k0 * K + c0, x0);
-        vload<V>(X + k1 * K + c0, x1);
+    for (; j < e; j += U * SUB) {
+        std::int64_t cc[U];
+        float aa[U];
 #pragma unroll
-        for (int v = 0; v < V; ++v) acc[v] = fmaf(a0, DIFF ? x0[v] - xi[v] : x0[v], acc[v]);
+        for (int u = 0; u < U; ++u) {
+            const bool ok = j + u * SUB < e;
+            cc[u] = ok ? ci[j + u * SUB] : 0;
+            aa[u] = ok ? av[j + u * SUB] : 0.0f;
+        }
+        float xx[U][V];
 #pragma unroll
-        for (int v = 0; v < V; ++v) acc[v] = fmaf(a1, DIFF ? x1[v] - xi[v] : x1[v], acc[v]);
-    }
-    if (j < e) {
-        const std::int64_t k0 = ci[j];
-        const float a0 = av[j];
-        float x0[V];
-        vload<V>(X + k0 * K + c0, x0);
+        for (int u = 0; u < U; ++u) {
+            if (j + u * SUB < e) {
+                vload<V>(X + cc[u] * K + c0, xx[u]);
+            } else {
 #pragma unroll
-        for (int v = 0; v < V; ++v) acc[v] = fmaf(a0, DIFF ? x0[v] - xi[v] : x0[v], acc[v]);
+                for (int v = 0; v < V; ++v) xx[u][v] = 0.0f;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (j + u * SUB < e) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) acc[v] = fmaf(aa[u], DIFF ? xx[u][v] - xi[v] : xx[u][v], acc[v]);
+            }
+        }
     }
 }
 
@@ -248,15 +257,20 @@ __global__ void __launch_bounds__(kRowThreads) rowk_kernel(
 // G1 = A^T B1 (and G2 = A^T B2), FP64 partials of one logical block: lane a of a row group holds
 // column a of A and accumulates row a of G from the B rows, which every lane of the group reads
 // whole (a broadcast). No cross-lane traffic until the block reduction.
-constexpr int kGramRowThreads = 256;
+// Block size of the streaming block-vector kernels: as many waves as the accumulators' registers
+// allow, since each wave walks its rows with one load round trip per step.
+template <int K, bool TWO>
+constexpr int gram_threads() {
+    return (TWO ? 2 : 1) * K <= 16 ? 1024 : ((TWO ? 2 : 1) * K <= 32 ? 512 : 256);
+}
 
 template <int K, bool TWO>
-__global__ void __launch_bounds__(kGramRowThreads) gram_rows_kernel(
+__global__ void __launch_bounds__((gram_threads<K, TWO>())) gram_rows_kernel(
     std::int64_t n, const float* __restrict__ A, const float* __restrict__ B1, const float* __restrict__ B2,
     double* __restrict__ partials, std::int64_t rows_per_block, const int* __restrict__ state) {
     if (finished(state)) return;
     constexpr int RPW = kWave / K;
-    constexpr int WAVES = kGramRowThreads / kWave;
+    constexpr int WAVES = gram_threads<K, TWO>() / kWave;
     constexpr int V = vec_width<K>();
     const int lb = logical_block();
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
@@ -469,7 +483,12 @@ __device__ __forceinline__ void block_cols_store(double (&v)[V], int grp, double
 // in `a`, which leaves X and R exactly unchanged. A lane owns V columns of one row and reads the
 // row's W and S whole (the other lanes of the row read the same cache lines).
 template <int K>
-__global__ void __launch_bounds__(kRowThreads) update_xr_kernel(
+constexpr int xr_threads() {
+    return K <= 16 ? 1024 : 512;
+}
+
+template <int K>
+__global__ void __launch_bounds__(xr_threads<K>()) update_xr_kernel(
     std::int64_t n, const float* __restrict__ W, const float* __restrict__ S, const double* __restrict__ a,
     double* __restrict__ X, float* __restrict__ R, double* __restrict__ partials, std::int64_t rows_per_block,
     const int* __restrict__ state) {
@@ -492,7 +511,8 @@ __global__ void __launch_bounds__(kRowThreads) update_xr_kernel(
     double nrm[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) nrm[v] = 0.0;
-    for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW; r0 < hi; r0 += kRowWaves * RPW) {
+    constexpr int WAVES = xr_threads<K>() / kWave;
+    for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW; r0 < hi; r0 += WAVES * RPW) {
         const std::int64_t row = r0 + lane / G;
         if (row >= hi) continue;
         double dx[V];
@@ -527,7 +547,7 @@ __global__ void __launch_bounds__(kRowThreads) update_xr_kernel(
         }
         vstore<V>(R + at, rr);
     }
-    block_cols_store<K, V, kRowWaves>(nrm, grp, partials + static_cast<std::int64_t>(lb) * K);
+    block_cols_store<K, V, WAVES>(nrm, grp, partials + static_cast<std::int64_t>(lb) * K);
 }
 
 // W <- Z - W D, in place: every lane of a row loads the row's W before any of them stores (the
@@ -595,7 +615,15 @@ __device__ void reduce_partials(const double* __restrict__ partials, int nb, int
         const int e = t % E, q = t / E;
         const int b0 = (nb * q) / Q, b1 = (nb * (q + 1)) / Q;
         double s = 0.0;
-        for (int b = b0; b < b1; ++b) s += partials[static_cast<std::int64_t>(b) * stride + off + e];
+        int b = b0;
+        for (; b + 8 <= b1; b += 8) {  // eight loads in flight, added in ascending order
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = partials[static_cast<std::int64_t>(b + u) * stride + off + e];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += v[u];
+        }
+        for (; b < b1; ++b) s += partials[static_cast<std::int64_t>(b) * stride + off + e];
         chunk[q * E + e] = s;
     }
     __syncthreads();
@@ -762,6 +790,14 @@ __global__ void __launch_bounds__(kSmallThreads) converge_kernel(const double* _
         state[1] = it;
         if (all) state[0] = 1;
     }
+}
+
+template <int E>
+__global__ void __launch_bounds__(kSmallThreads) sum_partials_kernel(const double* __restrict__ partials, int nb,
+                                                                     double* __restrict__ out) {
+    __shared__ double v[E];
+    reduce_partials<E>(partials, nb, E, 0, v);
+    for (int e = threadIdx.x; e < E; e += blockDim.x) out[e] = v[e];
 }
 
 template <int K>
@@ -971,7 +1007,7 @@ void BlockSolver::spmm_block_k(const int* state) {
     const std::int64_t nn = n();
     launch_rows<K, kSpmmDiff>(fine_sub<K>(), op_.off, op_.rowsum, nullptr, W_.get(), nullptr, 0.f, 0.f, 0.f,
                               S_.get(), state, s_);
-    gram_rows_kernel<K, true><<<kGramBlocks, kGramRowThreads, 0, s_>>>(nn, W_.get(), S_.get(), R_.get(),
+    gram_rows_kernel<K, true><<<kGramBlocks, gram_threads<K, true>(), 0, s_>>>(nn, W_.get(), S_.get(), R_.get(),
                                                                         partials_.get(), gram_rows(nn), state);
 }
 
@@ -984,11 +1020,11 @@ void BlockSolver::iteration_k(const SolveOptions& opt) {
     spmm_block_k<K>(st);
     coef_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, opt.drop_tol, G_.get(), T_.get(),
                                                 TT_.get(), AL_.get(), stw);
-    update_xr_kernel<K><<<kGramBlocks, kRowThreads, 0, s_>>>(nn, W_.get(), S_.get(), AL_.get(), X64_.get(), R_.get(),
+    update_xr_kernel<K><<<kGramBlocks, xr_threads<K>(), 0, s_>>>(nn, W_.get(), S_.get(), AL_.get(), X64_.get(), R_.get(),
                                                              partials_.get(), rpb, st);
     converge_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, target_.get(), norms_.get(), stw);
     vcycle_k<K>(R_.get(), Z_.get(), st);
-    gram_rows_kernel<K, false><<<kGramBlocks, kGramRowThreads, 0, s_>>>(nn, S_.get(), Z_.get(), nullptr,
+    gram_rows_kernel<K, false><<<kGramBlocks, gram_threads<K, false>(), 0, s_>>>(nn, S_.get(), Z_.get(), nullptr,
                                                                          partials_.get(), rpb, st);
     dmat_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, TT_.get(), D_.get(), stw);
     update_w_kernel<K><<<grid_for(nn * col_groups<K>(), kRowThreads), kRowThreads, 0, s_>>>(nn, Z_.get(), D_.get(),
@@ -1128,6 +1164,16 @@ void BlockSolver::spmm_diff(const float* X, float* Y, int K) {
         launch_rows<KK, kSpmmDiff>(fine_sub<KK>(), op_.off, op_.rowsum, nullptr, X, nullptr, 0.f, 0.f, 0.f, Y,
                                    nullptr, s_);
     });
+}
+
+void BlockSolver::cross_gram32(const float* A, const float* B, double* out_host) {
+    ensure(32);
+    const std::int64_t nn = n();
+    gram_rows_kernel<32, false><<<kGramBlocks, gram_threads<32, false>(), 0, s_>>>(nn, A, B, nullptr, partials_.get(),
+                                                                               gram_rows(nn), nullptr);
+    sum_partials_kernel<32 * 32><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, G_.get());
+    d2h(out_host, G_.get(), 32 * 32, s_);
+    sync(s_, "cross gram");
 }
 
 void BlockSolver::vcycle(const float* B, float* X, int K) {

@@ -67,6 +67,8 @@ class BlockSolver {
     void spmm(const float* X, float* Y, int K);     // Y = A X (FP32, plain CSR)
     void spmm_diff(const float* X, float* Y, int K);  // Y = A X (FP32, differenced form)
     void vcycle(const float* B, float* X, int K);   // X = M B, one zero-guess V-cycle
+    // out (32 x 32, FP64, host) = A^T B for (n, 32) FP32 blocks, fixed-order reduction.
+    void cross_gram32(const float* A, const float* B, double* out_host);
     // Average device time (ms) over `reps` launches, HIP events on the solver's stream.
     // mode 0: plain SpMM; 1: differenced SpMM (the iteration's S = A W); 2: 1 followed by the Gram
     // pass G = W^T S, H = W^T R. sub > 0 forces the sub-lanes per column group of the fine level.

@@ -68,6 +68,7 @@ class PlacementResult:
     metrics: np.ndarray | None
     iterations: int
     relative_residual: float
+    E_slots: np.ndarray | None = None
 
 
 class SolverContext:
@@ -292,6 +293,67 @@ class SolverContext:
         check(lib().cn_k_place(self.handle, ptr(c), ptr(h), ptr(d), c.shape[0], ptr(out), ptr(deg)), "cn_k_place")
         return out, deg
 
+    # ---- reduced-basis warm start ----
+    def warm_reset(self, capacity: int) -> None:
+        check(lib().cn_warm_reset(self.handle, int(capacity)), "cn_warm_reset")
+
+    def warm_count(self) -> int:
+        c = ctypes.c_int32()
+        check(lib().cn_warm_count(self.handle, ctypes.byref(c)), "cn_warm_count")
+        return int(c.value)
+
+    def warm_gram(self) -> np.ndarray:
+        """G = W^T A W of the collected basis (FP64)."""
+        s = self.warm_count()
+        g = np.empty((s, s), np.float64)
+        check(lib().cn_warm_gram(self.handle, ptr(g)), "cn_warm_gram")
+        return g
+
+    def warm_set_projector(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float64)
+        if p.ndim != 2 or p.shape[0] != p.shape[1]:
+            raise ValueError("the projector must be square")
+        check(lib().cn_warm_set_projector(self.handle, ptr(p), p.shape[0]), "cn_warm_set_projector")
+
+    def warm_finalize(self, rcond: float = 1e-7) -> int:
+        """Pseudo-inverse of the Galerkin matrix by a symmetric eigendecomposition, dropping
+        directions below rcond x the largest eigenvalue (near-duplicate snapshots); returns the rank
+        kept."""
+        g = self.warm_gram()
+        g = 0.5 * (g + g.T)
+        lam, v = np.linalg.eigh(g)
+        keep = lam > rcond * lam.max()
+        p = (v[:, keep] / lam[keep]) @ v[:, keep].T
+        self.warm_set_projector(p)
+        self.warm_projector = p
+        return int(keep.sum())
+
+    def recovery_build(self, mode: str) -> tuple[int, int]:
+        """Build (once) the recovery operator of ``mode``; returns (n_slots, n_fallback)."""
+        code = _lib.RECOVERY_CODES[mode]
+        n_slots = ctypes.c_int64()
+        n_fb = ctypes.c_int32()
+        check(lib().cn_recovery_build(self.handle, code, ctypes.byref(n_slots), ctypes.byref(n_fb)),
+              "cn_recovery_build")
+        return int(n_slots.value), int(n_fb.value)
+
+    def recovery_operator(self, mode: str) -> dict[str, np.ndarray]:
+        """The recovery operator's arrays (built on first use), for inspection and tests."""
+        n_slots, n_fb = self.recovery_build(mode)
+        nc = 4 * self.n_tets
+        out = {
+            "ptr": np.empty(n_slots + 1, np.int32), "idx": np.empty(nc, np.int32), "w": np.empty(nc, np.float32),
+            "slot_of_corner": np.empty(nc, np.int32), "slot_node": np.empty(n_slots, np.int32),
+            "slot_tag": np.empty(n_slots, np.int32) if mode == "spr_tissue" else None,
+            "boundary": np.empty(self.n_nodes, np.int32), "status": np.empty(n_slots, np.int32),
+        }
+        check(lib().cn_recovery_operator(self.handle, _lib.RECOVERY_CODES[mode], ptr(out["ptr"]), ptr(out["idx"]),
+                                         ptr(out["w"]), ptr(out["slot_of_corner"]), ptr(out["slot_node"]),
+                                         ptr(out["slot_tag"]), ptr(out["boundary"]), ptr(out["status"])),
+              "cn_recovery_operator")
+        out["n_fallback"] = n_fb
+        return out
+
     def bench_spmm(self, K: int, reps: int = 20, mode: int = 1, sub: int = 0) -> float:
         """Average ms of the fine-level product: mode 0 plain SpMM, 1 differenced SpMM, 2 with Grams."""
         ms = ctypes.c_double()
@@ -320,8 +382,14 @@ def solve_placements_block(
     x0: np.ndarray | None = None,
     extra: dict | None = None,
     use_graph: bool = True,
+    recovery: str = "raw",
+    slots: bool = False,
+    warm_start: bool = False,
+    collect: bool = False,
 ) -> list[PlacementResult]:
-    """Solve up to MAX_BLOCK placements as one block (fem/solve.py:865-979)."""
+    """Solve up to MAX_BLOCK placements as one block (fem/solve.py:865-979). ``recovery`` selects
+    the field E, |E| and the metrics describe ("raw", "spr_global", "spr_tissue"); ``slots``
+    also returns the recovered field on its slots."""
     _blocks()
     k = len(sites)
     if k < 1 or k > MAX_BLOCK:
@@ -352,11 +420,23 @@ def solve_placements_block(
     ex = {}
     if extra is not None:
         for key, shape, dt in (("dadt_elm", (k, m, 3), np.float32), ("dadt_nodes", (k, n, 3), np.float32),
-                               ("rhs", (k, n), np.float32)):
+                               ("rhs", (k, n), np.float32), ("x0", (k, n - 1), np.float64)):
             if key in extra:
                 ex[key] = np.empty(shape, dt)
                 setattr(out, key, ptr(ex[key]))
+    if recovery not in _lib.RECOVERY_CODES:
+        raise ValueError(f"unknown recovery {recovery!r}")
+    if slots and recovery == "raw":
+        raise ValueError("slots=True needs a recovery mode")
+    e_slots = None
+    if slots:
+        n_slots, _ = ctx.recovery_build(recovery)
+        e_slots = np.empty((k, n_slots, 3), np.float32)
+        out.E_slots = ptr(e_slots)
     opt = _lib.default_solve_options()
+    opt.recovery = _lib.RECOVERY_CODES[recovery]
+    opt.warm_start = int(warm_start)
+    opt.collect = int(collect)
     opt.tolerance = float(tolerance)
     opt.max_iters = int(max_iters)
     opt.use_graph = int(use_graph)
@@ -378,6 +458,7 @@ def solve_placements_block(
             transform=tf[i], magnE=None if mag is None else mag[i], E=None if E is None else E[i],
             v=None if v is None else v[i], metrics=None if met is None else met[i],
             iterations=int(it[i]), relative_residual=float(rel[i]),
+            E_slots=None if e_slots is None else e_slots[i],
         )
         for i in range(k)
     ]

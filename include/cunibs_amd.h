@@ -120,6 +120,9 @@ typedef struct {
     double drop_tol;
     int32_t use_graph;
     int32_t check_every;
+    int32_t recovery;  /* CN_RECOVERY_*: the field E, |E| and the metrics describe; default raw */
+    int32_t warm_start; /* 1: start PCG from the reduced-basis guess (cn_warm_*); default 0 */
+    int32_t collect;    /* 1: append this block's solutions to the warm-start basis; default 0 */
 } cn_solve_options;
 
 void cn_default_solve_options(cn_solve_options* out);
@@ -137,14 +140,45 @@ typedef struct {
     float* dadt_nodes;         /* (k, n_nodes, 3) nodal dA/dt */
     float* rhs;                /* (k, n_nodes) right-hand side before grounding */
     int32_t* degenerate;       /* (k) 1 where the handle left the frame undefined */
+    float* E_slots;            /* (k, n_slots, 3) recovered field on its slots (recovery != raw) */
+    double* x0;                /* (k, n_reduced) the initial guess PCG started from (zeros if cold) */
     int32_t total_iterations;  /* out: inner iterations of the block */
     int32_t refinements;       /* out: FP64 refinement rounds */
     int32_t round_iterations[8]; /* out: cumulative inner iterations at the end of each round */
     double round_residual[8];    /* out: worst FP64 true relative residual after each round */
 } cn_block_outputs;
 
+/* Recovery modes (fem/recovery.py:77-100): the raw per-element field, or superconvergent patch
+ * recovery over each node's incident elements (spr_global) or each (node, tissue) pair's
+ * (spr_tissue). Harmonic recovery is not provided yet (CN_EINVAL). */
+#define CN_RECOVERY_RAW 0
+#define CN_RECOVERY_SPR_GLOBAL 1
+#define CN_RECOVERY_SPR_TISSUE 2
+#define CN_RECOVERY_HARMONIC 3
+
+/* Build (once; cached on the context) the operator of a recovery mode (fem/recovery.py:385-453,
+ * ensure_recovery). Reports the slot count and how many slots took the volume-weighted average.*/
+int32_t cn_recovery_build(cn_context* ctx, int32_t mode, int64_t* n_slots, int32_t* n_fallback);
+/* Copy a built operator back: ptr (n_slots+1), idx and w (4 n_tets: corner ids and weights),
+ * slot_of_corner (4 n_tets), slot_node and slot_tag (n_slots; slot_tag only for spr_tissue),
+ * boundary (n_nodes), status (n_slots: 0 fitted, 1 averaged). Any pointer may be NULL. */
+int32_t cn_recovery_operator(const cn_context* ctx, int32_t mode, int32_t* ptr, int32_t* idx, float* w,
+                             int32_t* slot_of_corner, int32_t* slot_node, int32_t* slot_tag,
+                             int32_t* boundary, int32_t* status);
+
+/* Reduced-basis warm start. No reference counterpart (its nearest analogue is the Galerkin
+ * initial guess of uq/conductivity/run.py:154-200). A basis W of up to 512 solver-generated
+ * solutions (BF16) is collected by solves with opt->collect = 1; cn_warm_gram returns the
+ * Galerkin matrix G = W^T A W; the caller hands back P, a (pseudo-)inverse of G, with
+ * cn_warm_set_projector; solves with opt->warm_start = 1 then start from x0 = W P W^T b
+ * (expansion as a BF16 MFMA GEMM) and still iterate to the same tolerance. */
+int32_t cn_warm_reset(cn_context* ctx, int32_t capacity);
+int32_t cn_warm_count(const cn_context* ctx, int32_t* count);
+int32_t cn_warm_gram(cn_context* ctx, double* G); /* (count, count) row-major */
+int32_t cn_warm_set_projector(cn_context* ctx, const double* P, int32_t count);
+
 /* Solve k <= 32 placements as one block (fem/solve.py:865-979): coil frames, dA/dt, RHS, block
- * PCG, E reconstruction (raw recovery), region metrics. Dipoles in metres / A m^2; centres and
+ * PCG, E reconstruction, recovery (opt->recovery; built on first use), region metrics. Dipoles in metres / A m^2; centres and
  * handles (k, 3) mm; distances (k) mm; didt in A/s. A degenerate handle is CN_EINVAL naming the
  * placement (fem/placement.py:182-215). x0 (optional, (k, n_reduced) FP64) warm-starts PCG. */
 int32_t cn_solve_block(cn_context* ctx, const double* dip_pos_m, const double* dip_moment,

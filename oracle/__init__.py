@@ -665,3 +665,117 @@ def set_threads(n: int) -> None:
 
 def num_threads() -> int:
     return int(lib().or_num_threads())
+
+
+# ------------------------------------------------------------------------------------------------
+# Superconvergent patch recovery (fem/recovery.py:120-453, solver/fem/recovery.cu:67-280,620-687)
+# ------------------------------------------------------------------------------------------------
+
+SPR_MAX_LEBESGUE = 8.0  # solver/fem/recovery.cu:36
+
+
+def outer_boundary_nodes(tet_nodes: np.ndarray, n_nodes: int) -> np.ndarray:
+    """Nodes of faces owned by exactly one tetrahedron (fem/recovery.py:120-141)."""
+    t = np.asarray(tet_nodes, np.int64)
+    faces = np.concatenate([t[:, [1, 2, 3]], t[:, [0, 2, 3]], t[:, [0, 1, 3]], t[:, [0, 1, 2]]])
+    faces = np.sort(faces, axis=1)
+    uniq, counts = np.unique(faces, axis=0, return_counts=True)
+    mark = np.zeros(n_nodes, dtype=np.int32)
+    mark[uniq[counts == 1].ravel()] = 1
+    return mark
+
+
+def tissue_slots(tet_nodes: np.ndarray, tet_tags: np.ndarray):
+    """Corners grouped by (node, tag), stable (fem/recovery.py:187-228)."""
+    t = np.asarray(tet_nodes, np.int64).ravel()
+    tags = np.repeat(np.asarray(tet_tags, np.int64), 4)
+    stride = int(tags.max()) + 1
+    key = t * stride + tags
+    order = np.argsort(key, kind="stable")
+    sk = key[order]
+    head = np.r_[True, sk[1:] != sk[:-1]]
+    starts = np.flatnonzero(head)
+    slot_sorted = np.cumsum(head) - 1
+    ptr = np.r_[starts, sk.size].astype(np.int32)
+    soc = np.empty(sk.size, np.int32)
+    soc[order] = slot_sorted
+    return (ptr, order.astype(np.int32), soc, (sk[starts] // stride).astype(np.int32),
+            (sk[starts] % stride).astype(np.int32))
+
+
+def spr_operator(nodes_mm, tet_nodes, tet_tags, vols, mode: str) -> dict:
+    """One weight per corner: the linear fit over the slot's incident barycentres in the
+    node-centred, radius-scaled basis, evaluated at the node; the volume-weighted average on the
+    patch volume's boundary, for fewer than four elements, a pivot <= 1e-12 A_00 or a Lebesgue
+    constant above 8 (solver/fem/recovery.cu:192-280)."""
+    nodes = np.asarray(nodes_mm, np.float64)
+    tets = np.asarray(tet_nodes, np.int32)
+    n, m = nodes.shape[0], tets.shape[0]
+    boundary = outer_boundary_nodes(tets, n)
+    if mode == "spr_global":
+        ptr, idx = node2corner(tets, n)
+        soc = tets.ravel().astype(np.int32)
+        slot_node = np.arange(n, dtype=np.int32)
+        slot_tag = None
+    elif mode == "spr_tissue":
+        ptr, idx, soc, slot_node, slot_tag = tissue_slots(tets, tet_tags)
+        rep = slot_node[:-1][slot_node[1:] == slot_node[:-1]]
+        boundary[rep] = 1
+    else:
+        raise ValueError(f"not an SPR mode: {mode}")
+    n_slots = slot_node.shape[0]
+    counts = np.diff(ptr)
+    owner = np.repeat(np.arange(n_slots), counts)
+    e = idx >> 2
+    p = nodes[tets[e]]
+    b = (((p[:, 0] + p[:, 1]) + p[:, 2]) + p[:, 3]) * 0.25
+    xn = nodes[slot_node[owner]]
+    d = b - xn
+    h = np.zeros(n_slots)
+    np.maximum.at(h, owner, np.abs(d).max(axis=1))
+    fit = (counts >= 4) & (boundary[slot_node] == 0) & (h > 0)
+    hs = np.where(h > 0, h, 1.0)[owner]
+    q = np.column_stack([np.ones(len(owner)), d[:, 0] / hs, d[:, 1] / hs, d[:, 2] / hs])
+    a = np.zeros((n_slots, 4, 4))
+    np.add.at(a, owner, q[:, :, None] * q[:, None, :])
+    # Cholesky per slot with the pivot test, then A z = e_0.
+    l = np.zeros_like(a)
+    floor = 1e-12 * a[:, 0, 0]
+    for j in range(4):
+        dj = a[:, j, j] - (l[:, j, :j] ** 2).sum(axis=1)
+        fit &= dj > floor
+        l[:, j, j] = np.sqrt(np.where(dj > 0, dj, 1.0))
+        for i in range(j + 1, 4):
+            l[:, i, j] = (a[:, i, j] - (l[:, i, :j] * l[:, j, :j]).sum(axis=1)) / l[:, j, j]
+    y = np.zeros((n_slots, 4))
+    for i in range(4):
+        y[:, i] = ((1.0 if i == 0 else 0.0) - (l[:, i, :i] * y[:, :i]).sum(axis=1)) / l[:, i, i]
+    z = np.zeros((n_slots, 4))
+    for i in range(3, -1, -1):
+        z[:, i] = (y[:, i] - (l[:, i + 1:, i] * z[:, i + 1:]).sum(axis=1)) / l[:, i, i]
+    w_fit = (z[owner] * q).sum(axis=1)
+    leb = np.zeros(n_slots)
+    np.add.at(leb, owner, np.abs(w_fit))
+    fit &= leb <= SPR_MAX_LEBESGUE
+    v = np.asarray(vols, np.float32).astype(np.float64)[e]
+    tot = np.zeros(n_slots)
+    np.add.at(tot, owner, v)
+    by_vol = tot[owner] > 0
+    w_avg = np.where(by_vol, v / np.where(tot[owner] > 0, tot[owner], 1.0), 1.0 / counts[owner])
+    w = np.where(fit[owner], w_fit, w_avg).astype(np.float32)
+    return {"ptr": ptr.astype(np.int32), "idx": idx.astype(np.int32), "w": w, "slot_of_corner": soc,
+            "slot_node": slot_node, "slot_tag": slot_tag, "boundary": boundary,
+            "status": (~fit).astype(np.int32)}
+
+
+def spr_apply(op: dict, E: np.ndarray):
+    """E*_s = sum_c w_c E[c >> 2]; E_out = mean of each element's four corner slots
+    (solver/fem/recovery.cu:620-687), FP64 here."""
+    counts = np.diff(op["ptr"])
+    owner = np.repeat(np.arange(counts.size), counts)
+    contrib = op["w"].astype(np.float64)[:, None] * np.asarray(E, np.float64)[op["idx"] >> 2]
+    slots = np.zeros((counts.size, 3))
+    np.add.at(slots, owner, contrib)
+    soc = op["slot_of_corner"].reshape(-1, 4)
+    e_out = slots[soc].mean(axis=1)
+    return slots, e_out, np.linalg.norm(e_out, axis=1)

@@ -158,7 +158,8 @@ def test_gm_metrics_match_the_oracle_summary(single, gctx, oracle_mod):
     got = fem.metrics_dict(res.metrics)
     assert got["peak_magnE"] == want["peak_magnE"]
     np.testing.assert_allclose(got["peak_location_mm"], want["peak_location_mm"], rtol=1e-12)
-    np.testing.assert_allclose(got["center_of_gravity_mm"], want["center_of_gravity_mm"], rtol=1e-9)
+    # The restatement forms w * |E| in FP32 as the reference does; the device accumulates in FP64.
+    np.testing.assert_allclose(got["center_of_gravity_mm"], want["center_of_gravity_mm"], rtol=1e-6)
     assert got["region_volume_m3"] == pytest.approx(want["region_volume_m3"], rel=1e-9)
     for key in ("mean", "std", "p50", "p95", "p99", "p99.9"):
         assert got["distribution"][key] == pytest.approx(want["distribution"][key], rel=1e-6), key

@@ -1,0 +1,66 @@
+"""Reduced-basis warm start: the initial guess is the Galerkin projection W P W^T b onto the span
+of collected solutions (BF16 basis, MFMA expansion), PCG still iterates to the same tolerance,
+and the iteration count drops."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16_round(x: np.ndarray) -> np.ndarray:
+    """FP32 -> BF16 (round to nearest even) -> FP32, the basis' storage rounding."""
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+@pytest.fixture(scope="module")
+def trained(head4, coil):
+    from cunibs_amd import fem, synthetic
+    ctx = fem.SolverContext(head4.nodes_mm, head4.tet_nodes, head4.tet_tags, head4.skin_tris)
+    ctx.warm_reset(64)
+    train = synthetic.cap_sites(48, seed=11)
+    sols = []
+    for i in range(0, 48, 16):
+        res = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, train[i:i + 16], 1e6,
+                                         potential=True, collect=True)
+        sols.extend(r.v for r in res)
+    assert ctx.warm_count() == 48
+    rank = ctx.warm_finalize()
+    yield ctx, train, np.array(sols), rank
+    ctx.free()
+
+
+def test_initial_guess_is_the_galerkin_projection(trained, coil):
+    from cunibs_amd import fem, synthetic
+    ctx, _, sols, rank = trained
+    assert rank >= 40
+    keep = np.arange(ctx.n_nodes) != ctx.ground_node
+    w = bf16_round(sols[:, keep].astype(np.float32)).astype(np.float64).T  # (n_red, S)
+    sites = synthetic.cap_sites(4, seed=12)
+    extra = {"x0": 1, "rhs": 1}
+    fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites, 1e6, warm_start=True, extra=extra)
+    for i in range(4):
+        b = extra["rhs"][i][keep].astype(np.float64)
+        want = w @ (ctx.warm_projector @ (w.T @ b))
+        got = extra["x0"][i]
+        assert np.linalg.norm(got - want) <= 1e-4 * np.linalg.norm(want)
+
+
+def test_warm_start_keeps_the_answer_and_saves_iterations(trained, coil):
+    from cunibs_amd import fem, synthetic
+    ctx, train, _, _ = trained
+    sites = synthetic.cap_sites(8, seed=13)
+    cold = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites, 1e6)
+    warm = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites, 1e6, warm_start=True)
+    for a, b in zip(cold, warm):
+        assert b.relative_residual <= 1e-6
+        assert np.linalg.norm(a.magnE - b.magnE) <= 2e-5 * np.linalg.norm(a.magnE)
+    assert np.mean([r.iterations for r in warm]) < np.mean([r.iterations for r in cold])
+    # A placement whose solution is in the basis starts within the BF16 rounding of the answer.
+    again_c = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, train[:4], 1e6)
+    again_w = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, train[:4], 1e6, warm_start=True)
+    assert max(r.iterations for r in again_w) <= min(r.iterations for r in again_c) - 5
