@@ -92,6 +92,7 @@ _SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "cn_warm_gram": (_I32, [_P, _P]),
     "cn_warm_set_projector": (_I32, [_P, _P, _I32]),
     "cn_recovery_operator": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "cn_recovery_patches": (_I32, [_P, _P, _P, _P, _P, _P]),
     "cn_k_p1_gradients": (_I32, [_P, _I64, _P, _I64, _P, _P]),
     "cn_k_morton_keys": (_I32, [_P, _I64, _P]),
     "cn_k_tet_lowest_node": (_I32, [_P, _I64, _P, _I64, _P]),

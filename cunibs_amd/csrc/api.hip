@@ -377,12 +377,14 @@ void place_block(cn_context& c, const double* centers, const double* handles, co
 const cn::RecoveryOp* ensure_recovery(cn_context& c, int mode) {
     using namespace cn;
     require(mode >= CN_RECOVERY_RAW && mode <= CN_RECOVERY_HARMONIC, "unknown recovery mode");
-    require(mode != CN_RECOVERY_HARMONIC, "harmonic recovery is not implemented in this build");
     require(mode != CN_RECOVERY_RAW, "the raw mode has no recovery operator");
     if (!c.recovery[mode]) {
         auto op = std::make_unique<RecoveryOp>();
-        build_spr(*op, mode, c.nodes.get(), c.tets.get(), c.tags.get(), c.vols32.get(), c.n2c_ptr.get(),
-                  c.n2c_idx.get(), c.n, c.m, c.max_tag, c.s);
+        if (mode == CN_RECOVERY_HARMONIC)
+            build_harmonic(*op, c.nodes.get(), c.tets.get(), c.tags.get(), c.n, c.m, c.max_tag, c.s);
+        else
+            build_spr(*op, mode, c.nodes.get(), c.tets.get(), c.tags.get(), c.vols32.get(), c.n2c_ptr.get(),
+                      c.n2c_idx.get(), c.n, c.m, c.max_tag, c.s);
         c.recovery[mode] = std::move(op);
     }
     return c.recovery[mode].get();
@@ -491,6 +493,11 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
     if (rec == nullptr) {
         reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), c.d_dadt_elm.get(), m, k, K, c.ground, dE, c.d_magn.get(),
                     s);
+    } else if (rec->mode == kRecoveryHarmonic) {
+        // The potential and the exact nodal dA/dt feed the fits; |E| and E describe the result.
+        c.d_slots.ensure(3 * rec->n_slots * K, "recovered slots");
+        apply_harmonic(*rec, c.d_X.get(), c.ground, c.d_dadt_nodes.get(), m, k, K, c.d_slots.get(), dE, c.d_magn.get(),
+                       s);
     } else {
         // The raw field feeds the patch fits; |E| and E then describe the recovered field.
         c.d_Eraw.ensure(3 * m * K, "raw E");
@@ -955,8 +962,8 @@ int32_t cn_recovery_operator(const cn_context* c, int32_t mode, int32_t* ptr, in
         if (w) cn::d2h(w, op.w.get(), nc, s);
         if (slot_of_corner) cn::d2h(slot_of_corner, op.soc_view, nc, s);
         if (slot_node) cn::d2h(slot_node, op.slot_node.get(), op.n_slots, s);
-        if (slot_tag && mode == CN_RECOVERY_SPR_TISSUE) cn::d2h(slot_tag, op.slot_tag.get(), op.n_slots, s);
-        if (boundary) cn::d2h(boundary, op.boundary.get(), c->n, s);
+        if (slot_tag && mode != CN_RECOVERY_SPR_GLOBAL) cn::d2h(slot_tag, op.slot_tag.get(), op.n_slots, s);
+        if (boundary && mode != CN_RECOVERY_HARMONIC) cn::d2h(boundary, op.boundary.get(), c->n, s);
         if (status) cn::d2h(status, op.status.get(), op.n_slots, s);
         cn::sync(s, "recovery operator copy");
     });
@@ -1006,6 +1013,21 @@ int32_t cn_warm_set_projector(cn_context* c, const double* P, int32_t count) {
     return guarded([&] {
         cn::require(c != nullptr && P != nullptr, "null argument");
         c->warm.set_projector(P, count, c->s);
+    });
+}
+
+int32_t cn_recovery_patches(const cn_context* c, int64_t* nnz, int32_t* n_undetermined, int32_t* pptr,
+                            int32_t* pidx, float* w3) {
+    return guarded([&] {
+        cn::require(c != nullptr && c->recovery[CN_RECOVERY_HARMONIC] != nullptr,
+                    "harmonic operator not built (call cn_recovery_build first)");
+        const cn::RecoveryOp& op = *c->recovery[CN_RECOVERY_HARMONIC];
+        if (nnz) *nnz = op.nnz_patch;
+        if (n_undetermined) *n_undetermined = op.n_undetermined;
+        if (pptr) cn::d2h(pptr, op.pptr.get(), op.n_slots + 1, c->s);
+        if (pidx) cn::d2h(pidx, op.pidx.get(), op.nnz_patch, c->s);
+        if (w3) cn::d2h(w3, op.w3.get(), 3 * op.nnz_patch, c->s);
+        cn::sync(c->s, "patch copy");
     });
 }
 

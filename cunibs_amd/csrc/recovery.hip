@@ -15,6 +15,9 @@
 // same layout as the rest of the block pipeline. Every sum walks its CSR segment in order.
 #include "recovery.hpp"
 
+#include <algorithm>
+#include <vector>
+
 #include "prims.hpp"
 
 namespace cn {
@@ -284,6 +287,41 @@ void outer_boundary_nodes(const std::int32_t* tets, std::int64_t m, const std::i
     if (const unsigned g = grid_for(4 * m)) outer_boundary_kernel<<<g, kBlock, 0, s>>>(tets, m, n2c_ptr, n2c_idx, mark);
 }
 
+// Corners grouped by (node, tag), stable: the node2corner order refined by tissue
+// (fem/recovery.py:187-228). Fills ptr/idx/soc/slot_node/slot_tag and the views.
+void tissue_slots(RecoveryOp& op, const std::int32_t* tets, const std::int32_t* tags, std::int64_t n, std::int64_t m,
+                  int max_tag, hipStream_t s) {
+    const std::int64_t nc = 4 * m;
+    const std::uint64_t stride = static_cast<std::uint64_t>(max_tag) + 1;
+    op.stride = stride;
+    DeviceBuffer<std::uint64_t> keys(nc, "slot keys"), skeys(nc, "sorted slot keys");
+    DeviceBuffer<std::int32_t> corner(nc, "corners"), flag(nc, "heads"), pos(nc, "head scan");
+    op.idx.reset(nc, "slot corners");
+    op.soc.reset(nc, "slot of corner");
+    tissue_keys_kernel<<<grid_for(nc), kBlock, 0, s>>>(tets, tags, m, stride, keys.get());
+    iota_i32(corner.get(), nc, s);
+    int bits = 1;
+    while (bits < 64 && ((static_cast<std::uint64_t>(n) * stride) >> bits) != 0) ++bits;
+    sort_pairs_u64(keys.get(), skeys.get(), corner.get(), op.idx.get(), nc, bits, s);
+    head_flags_kernel<<<grid_for(nc), kBlock, 0, s>>>(skeys.get(), nc, flag.get());
+    exclusive_scan_i32(flag.get(), pos.get(), nc, s);
+    std::int32_t last_pos = 0, last_flag = 0;
+    d2h(&last_pos, pos.get() + (nc - 1), 1, s);
+    d2h(&last_flag, flag.get() + (nc - 1), 1, s);
+    sync(s, "slot count");
+    op.n_slots = last_pos + last_flag;
+    op.ptr.reset(op.n_slots + 1, "slot ptr");
+    op.slot_node.reset(op.n_slots, "slot node");
+    op.slot_tag.reset(op.n_slots, "slot tag");
+    slots_fill_kernel<<<grid_for(nc), kBlock, 0, s>>>(skeys.get(), flag.get(), pos.get(), op.idx.get(), nc, stride,
+                                                      op.ptr.get(), op.soc.get(), op.slot_node.get(),
+                                                      op.slot_tag.get());
+    op.ptr_view = op.ptr.get();
+    op.idx_view = op.idx.get();
+    op.soc_view = op.soc.get();
+    sync(s, "tissue slots");  // the scratch above is freed on return
+}
+
 void build_spr(RecoveryOp& op, int mode, const double* nodes, const std::int32_t* tets, const std::int32_t* tags,
                const float* vols, const std::int32_t* n2c_ptr, const std::int32_t* n2c_idx, std::int64_t n,
                std::int64_t m, int max_tag, hipStream_t s) {
@@ -300,34 +338,8 @@ void build_spr(RecoveryOp& op, int mode, const double* nodes, const std::int32_t
         op.slot_node.reset(static_cast<std::size_t>(n), "slot node");
         iota_i32(op.slot_node.get(), n, s);
     } else {
-        const std::uint64_t stride = static_cast<std::uint64_t>(max_tag) + 1;
-        DeviceBuffer<std::uint64_t> keys(nc, "slot keys"), skeys(nc, "sorted slot keys");
-        DeviceBuffer<std::int32_t> corner(nc, "corners"), flag(nc, "heads"), pos(nc, "head scan");
-        op.idx.reset(nc, "slot corners");
-        op.soc.reset(nc, "slot of corner");
-        tissue_keys_kernel<<<grid_for(nc), kBlock, 0, s>>>(tets, tags, m, stride, keys.get());
-        iota_i32(corner.get(), nc, s);
-        int bits = 1;
-        while (bits < 64 && ((static_cast<std::uint64_t>(n) * stride) >> bits) != 0) ++bits;
-        sort_pairs_u64(keys.get(), skeys.get(), corner.get(), op.idx.get(), nc, bits, s);
-        head_flags_kernel<<<grid_for(nc), kBlock, 0, s>>>(skeys.get(), nc, flag.get());
-        exclusive_scan_i32(flag.get(), pos.get(), nc, s);
-        std::int32_t last_pos = 0, last_flag = 0;
-        d2h(&last_pos, pos.get() + (nc - 1), 1, s);
-        d2h(&last_flag, flag.get() + (nc - 1), 1, s);
-        sync(s, "slot count");
-        op.n_slots = last_pos + last_flag;
-        op.ptr.reset(op.n_slots + 1, "slot ptr");
-        op.slot_node.reset(op.n_slots, "slot node");
-        op.slot_tag.reset(op.n_slots, "slot tag");
-        slots_fill_kernel<<<grid_for(nc), kBlock, 0, s>>>(skeys.get(), flag.get(), pos.get(), op.idx.get(), nc, stride,
-                                                          op.ptr.get(), op.soc.get(), op.slot_node.get(),
-                                                          op.slot_tag.get());
+        tissue_slots(op, tets, tags, n, m, max_tag, s);
         mixed_nodes_kernel<<<grid_for(op.n_slots), kBlock, 0, s>>>(op.slot_node.get(), op.n_slots, op.boundary.get());
-        op.ptr_view = op.ptr.get();
-        op.idx_view = op.idx.get();
-        op.soc_view = op.soc.get();
-        sync(s, "tissue slots");  // the scratch above is freed on return
     }
     op.w.reset(nc, "recovery weights");
     op.status.reset(static_cast<std::size_t>(op.n_slots), "recovery status");
@@ -345,6 +357,302 @@ void apply_spr(const RecoveryOp& op, const float* E, std::int64_t m, int k, int 
                float* magn, hipStream_t s) {
     if (const unsigned g = grid_for(op.n_slots * K))
         recover_slots_kernel<<<g, kBlock, 0, s>>>(E, op.w.get(), op.ptr_view, op.idx_view, op.n_slots, k, K, slots);
+    if (const unsigned g = grid_for(m * K))
+        recover_elements_kernel<<<g, kBlock, 0, s>>>(slots, op.soc_view, m, k, K, E_out, magn);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Harmonic recovery (fem/recovery.py:23-50,269-383; solver/fem/recovery.cu:290-610 of the
+// reference). The potential is fitted over a same-tissue node patch around each (node, tissue)
+// slot in the space of harmonic polynomials -- degree <= 3 (16 terms), else degree <= 2 (9), else
+// linear (4) -- in the node-centred frame scaled by the patch half-width h; the gradient at the
+// node is the linear coefficients / h, so each patch node m gets three weights and
+//     E[s] = -sum_m w[s, m] v[m] - dA/dt(x_node).
+// A rung is rejected when its normal matrix has a pivot <= 1e-12 A_00, or when its gradient can
+// amplify the potential by more than 50 (h max_c sum_m |w[m, c]|); status 0/1/2 is the rung taken,
+// 3 no gradient at all (zero weights, E = -dA/dt there).
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+constexpr double kMaxGradientAmplification = 50.0;
+constexpr int kMinPatchNodes = 16;  // below this a slot's patch grows to its neighbours' rings
+constexpr int kFitWaves = 4;
+constexpr int kFitChunk = 64;
+constexpr int kMaxTerms = 16;
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Harmonic polynomials up to degree 3 in (u, v, t): 1, the linear terms, the five quadratic and
+// the seven cubic solid harmonics (each satisfies Laplace's equation).
+template <int N>
+__device__ __forceinline__ void harmonic_basis(double u, double v, double t, double* p) {
+    p[0] = 1.0;
+    p[1] = u;
+    p[2] = v;
+    p[3] = t;
+    if constexpr (N >= 9) {
+        p[4] = u * u - v * v;
+        p[5] = v * v - t * t;
+        p[6] = u * v;
+        p[7] = u * t;
+        p[8] = v * t;
+    }
+    if constexpr (N >= 16) {
+        const double uu = u * u, vv = v * v, tt = t * t;
+        p[9] = t * (2.0 * tt - 3.0 * uu - 3.0 * vv);
+        p[10] = u * (4.0 * tt - uu - vv);
+        p[11] = v * (4.0 * tt - uu - vv);
+        p[12] = t * (uu - vv);
+        p[13] = u * v * t;
+        p[14] = u * (uu - 3.0 * vv);
+        p[15] = v * (3.0 * uu - vv);
+    }
+}
+
+template <int N>
+constexpr int min_patch() {
+    return N == 16 ? 20 : (N == 9 ? 9 : 4);
+}
+
+// One rung of the fit for one slot, one wavefront. Normal-matrix entries (lower triangle) are
+// dealt round the lanes and summed over the patch in order from an LDS-staged chunk of basis
+// values; lane 0 factors and solves for rows 1..3 of the inverse; the lanes then write the
+// weights of their patch entries and the amplification sums are folded by a fixed xor tree.
+template <int N>
+__device__ bool fit_rung(const double* __restrict__ nodes, const std::int32_t* __restrict__ pidx, int begin, int end,
+                         const double (&xn)[3], double h, float* __restrict__ w3, double* q, double* a, double* z,
+                         int* flag, int lane) {
+    if (end - begin < min_patch<N>()) return false;
+    constexpr int kEntries = N * (N + 1) / 2;
+    constexpr int kPer = (kEntries + kWave - 1) / kWave;
+    int ri[kPer], ci[kPer];
+    double acc[kPer];
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const int k = lane + kWave * e;
+        int i = 0;
+        while ((i + 1) * (i + 2) / 2 <= k) ++i;
+        ri[e] = i;
+        ci[e] = k - i * (i + 1) / 2;
+        acc[e] = 0.0;
+    }
+    for (int base = begin; base < end; base += kFitChunk) {
+        const int cnt = min(kFitChunk, end - base);
+        if (lane < cnt) {
+            const std::int64_t mm = pidx[base + lane];
+            harmonic_basis<N>((nodes[3 * mm] * 1e-3 - xn[0]) / h, (nodes[3 * mm + 1] * 1e-3 - xn[1]) / h,
+                              (nodes[3 * mm + 2] * 1e-3 - xn[2]) / h, q + lane * N);
+        }
+        wave_sync();
+        for (int p = 0; p < cnt; ++p) {
+#pragma unroll
+            for (int e = 0; e < kPer; ++e)
+                if (lane + kWave * e < kEntries) acc[e] = fma(q[p * N + ri[e]], q[p * N + ci[e]], acc[e]);
+        }
+        wave_sync();
+    }
+#pragma unroll
+    for (int e = 0; e < kPer; ++e)
+        if (lane + kWave * e < kEntries) a[ri[e] * N + ci[e]] = acc[e];
+    wave_sync();
+    if (lane == 0) {
+        int ok = 1;
+        const double floor_pivot = 1e-12 * a[0];
+        for (int j = 0; j < N && ok; ++j) {
+            double d = a[j * N + j];
+            for (int k = 0; k < j; ++k) d = fma(-a[j * N + k], a[j * N + k], d);
+            if (!(d > floor_pivot)) {
+                ok = 0;
+                break;
+            }
+            const double ljj = sqrt(d);
+            a[j * N + j] = ljj;
+            for (int i = j + 1; i < N; ++i) {
+                double v = a[i * N + j];
+                for (int k = 0; k < j; ++k) v = fma(-a[i * N + k], a[j * N + k], v);
+                a[i * N + j] = v / ljj;
+            }
+        }
+        if (ok) {
+            for (int c = 0; c < 3; ++c) {
+                double y[N];
+                for (int i = 0; i < N; ++i) {
+                    double v = (i == c + 1) ? 1.0 : 0.0;
+                    for (int k = 0; k < i; ++k) v = fma(-a[i * N + k], y[k], v);
+                    y[i] = v / a[i * N + i];
+                }
+                for (int i = N - 1; i >= 0; --i) {
+                    double v = y[i];
+                    for (int k = i + 1; k < N; ++k) v = fma(-a[k * N + i], z[c * N + k], v);
+                    z[c * N + i] = v / a[i * N + i];
+                }
+            }
+        }
+        *flag = ok;
+    }
+    wave_sync();
+    if (!*flag) return false;
+    double sum[3] = {0.0, 0.0, 0.0};
+    for (int p = begin + lane; p < end; p += kWave) {
+        const std::int64_t mm = pidx[p];
+        double qq[N];
+        harmonic_basis<N>((nodes[3 * mm] * 1e-3 - xn[0]) / h, (nodes[3 * mm + 1] * 1e-3 - xn[1]) / h,
+                          (nodes[3 * mm + 2] * 1e-3 - xn[2]) / h, qq);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double g = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) g = fma(z[c * N + i], qq[i], g);
+            sum[c] += fabs(g);
+            w3[3 * static_cast<std::int64_t>(p) + c] = static_cast<float>(g / h);
+        }
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1)
+        for (int c = 0; c < 3; ++c) sum[c] += __shfl_xor(sum[c], off, kWave);
+    return fmax(sum[0], fmax(sum[1], sum[2])) <= kMaxGradientAmplification;
+}
+
+__global__ void __launch_bounds__(kFitWaves * kWave) hpr_fit_kernel(
+    const double* __restrict__ nodes, const std::int32_t* __restrict__ pptr, const std::int32_t* __restrict__ pidx,
+    const std::int32_t* __restrict__ slot_node, std::int64_t n_slots, float* __restrict__ w3,
+    std::int32_t* __restrict__ status) {
+    __shared__ double q_s[kFitWaves][kFitChunk * kMaxTerms];
+    __shared__ double a_s[kFitWaves][kMaxTerms * kMaxTerms];
+    __shared__ double z_s[kFitWaves][3 * kMaxTerms];
+    __shared__ int flag_s[kFitWaves];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
+    const std::int64_t s = static_cast<std::int64_t>(blockIdx.x) * kFitWaves + wave;
+    if (s >= n_slots) return;  // wave-uniform: no block barrier follows
+    const int begin = pptr[s], end = pptr[s + 1];
+    const std::int64_t node = slot_node[s];
+    // In metres: the weights differentiate, so E comes out in V/m.
+    const double xn[3] = {nodes[3 * node] * 1e-3, nodes[3 * node + 1] * 1e-3, nodes[3 * node + 2] * 1e-3};
+    double h = 0.0;
+    for (int p = begin + lane; p < end; p += kWave) {
+        const std::int64_t mm = pidx[p];
+        for (int c = 0; c < 3; ++c) h = fmax(h, fabs(nodes[3 * mm + c] * 1e-3 - xn[c]));
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) h = fmax(h, __shfl_xor(h, off, kWave));
+    int rung = 3;
+    if (h > 0.0) {
+        double* q = q_s[wave];
+        double* a = a_s[wave];
+        double* z = z_s[wave];
+        int* flag = &flag_s[wave];
+        if (fit_rung<16>(nodes, pidx, begin, end, xn, h, w3, q, a, z, flag, lane))
+            rung = 0;
+        else if (fit_rung<9>(nodes, pidx, begin, end, xn, h, w3, q, a, z, flag, lane))
+            rung = 1;
+        else if (fit_rung<4>(nodes, pidx, begin, end, xn, h, w3, q, a, z, flag, lane))
+            rung = 2;
+    }
+    if (rung == 3) {
+        for (int p = begin + lane; p < end; p += kWave)
+            for (int c = 0; c < 3; ++c) w3[3 * static_cast<std::int64_t>(p) + c] = 0.0f;
+    }
+    if (lane == 0) status[s] = rung;
+}
+
+// E_slots[s, d, c] = -sum_p w3[p, d] v[pidx[p], c] - dA/dt[node, d, c], FP64 accumulation in patch
+// order; v is the reduced potential with the ground node's zero restored.
+__global__ void hpr_grad_kernel(const double* __restrict__ X, int ground, const float* __restrict__ w3,
+                                const std::int32_t* __restrict__ pptr, const std::int32_t* __restrict__ pidx,
+                                const std::int32_t* __restrict__ slot_node, const float* __restrict__ dadt,
+                                std::int64_t n_slots, int K, float* __restrict__ slots) {
+    const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n_slots * K) return;
+    const std::int64_t s = t / K;
+    const int c = static_cast<int>(t % K);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int p = pptr[s]; p < pptr[s + 1]; ++p) {
+        const int mm = pidx[p];
+        const double vm = (mm == ground) ? 0.0 : X[static_cast<std::int64_t>(mm - (mm > ground)) * K + c];
+        a0 = fma(static_cast<double>(w3[3 * static_cast<std::int64_t>(p)]), vm, a0);
+        a1 = fma(static_cast<double>(w3[3 * static_cast<std::int64_t>(p) + 1]), vm, a1);
+        a2 = fma(static_cast<double>(w3[3 * static_cast<std::int64_t>(p) + 2]), vm, a2);
+    }
+    const std::int64_t node = slot_node[s];
+    const float* da = dadt + node * 3 * K + c;
+    float* dst = slots + s * 3 * K + c;
+    dst[0] = static_cast<float>(-a0 - static_cast<double>(da[0]));
+    dst[K] = static_cast<float>(-a1 - static_cast<double>(da[K]));
+    dst[2 * K] = static_cast<float>(-a2 - static_cast<double>(da[2 * K]));
+}
+
+}  // namespace
+
+void build_harmonic(RecoveryOp& op, const double* nodes, const std::int32_t* tets, const std::int32_t* tags,
+                    std::int64_t n, std::int64_t m, int max_tag, hipStream_t s) {
+    op.mode = kRecoveryHarmonic;
+    tissue_slots(op, tets, tags, n, m, max_tag, s);
+    const std::int64_t ns = op.n_slots;
+    // First ring of each slot: the distinct nodes of its own tetrahedra, sorted.
+    DeviceBuffer<std::int32_t> r1_ptr, r1_idx;
+    const std::int64_t nnz1 = stiffness_pattern(tets, op.ptr.get(), op.idx.get(), ns, r1_ptr, r1_idx, s);
+    std::vector<std::int32_t> rp(static_cast<std::size_t>(ns + 1)), ri(static_cast<std::size_t>(nnz1));
+    std::vector<std::int32_t> sn(static_cast<std::size_t>(ns)), st(static_cast<std::size_t>(ns));
+    d2h(rp.data(), r1_ptr.get(), ns + 1, s);
+    d2h(ri.data(), r1_idx.get(), nnz1, s);
+    d2h(sn.data(), op.slot_node.get(), ns, s);
+    d2h(st.data(), op.slot_tag.get(), ns, s);
+    sync(s, "first rings");
+    // Slots below kMinPatchNodes grow to the union of their same-tissue neighbours' first rings
+    // (the slot is its own neighbour), sorted and deduplicated.
+    std::vector<std::uint64_t> key(static_cast<std::size_t>(ns));
+    for (std::int64_t i = 0; i < ns; ++i) key[i] = static_cast<std::uint64_t>(sn[i]) * op.stride + st[i];
+    std::vector<std::int32_t> pp(static_cast<std::size_t>(ns + 1), 0), pi;
+    pi.reserve(static_cast<std::size_t>(nnz1) + static_cast<std::size_t>(ns));
+    std::vector<std::int32_t> cand;
+    for (std::int64_t i = 0; i < ns; ++i) {
+        const int b = rp[i], e = rp[i + 1];
+        if (e - b >= kMinPatchNodes) {
+            pi.insert(pi.end(), ri.begin() + b, ri.begin() + e);
+        } else {
+            cand.clear();
+            for (int j = b; j < e; ++j) {
+                const std::uint64_t want = static_cast<std::uint64_t>(ri[j]) * op.stride + st[i];
+                const auto it = std::lower_bound(key.begin(), key.end(), want);
+                if (it == key.end() || *it != want) throw std::runtime_error("harmonic patch: missing neighbour slot");
+                const std::int64_t nb = it - key.begin();
+                cand.insert(cand.end(), ri.begin() + rp[nb], ri.begin() + rp[nb + 1]);
+            }
+            std::sort(cand.begin(), cand.end());
+            cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+            pi.insert(pi.end(), cand.begin(), cand.end());
+        }
+        pp[i + 1] = static_cast<std::int32_t>(pi.size());
+    }
+    op.nnz_patch = static_cast<std::int64_t>(pi.size());
+    op.pptr.reset(static_cast<std::size_t>(ns + 1), "patch ptr");
+    op.pidx.reset(std::max<std::size_t>(pi.size(), 1), "patch nodes");
+    h2d(op.pptr.get(), pp.data(), pp.size(), s);
+    h2d(op.pidx.get(), pi.data(), pi.size(), s);
+    op.w3.reset(std::max<std::size_t>(3 * pi.size(), 1), "harmonic weights");
+    op.status.reset(static_cast<std::size_t>(std::max<std::int64_t>(ns, 1)), "harmonic status");
+    if (const unsigned g = grid_for(ns, kFitWaves))
+        hpr_fit_kernel<<<g, kFitWaves * kWave, 0, s>>>(nodes, op.pptr.get(), op.pidx.get(), op.slot_node.get(), ns,
+                                                       op.w3.get(), op.status.get());
+    std::vector<std::int32_t> status(static_cast<std::size_t>(ns));
+    d2h(status.data(), op.status.get(), ns, s);
+    sync(s, "harmonic weights");
+    op.n_fallback = 0;
+    op.n_undetermined = 0;
+    for (auto v : status) {
+        op.n_fallback += v >= 2 ? 1 : 0;
+        op.n_undetermined += v == 3 ? 1 : 0;
+    }
+}
+
+void apply_harmonic(const RecoveryOp& op, const double* X, int ground, const float* dadt_nodes, std::int64_t m, int k,
+                    int K, float* slots, float* E_out, float* magn, hipStream_t s) {
+    if (const unsigned g = grid_for(op.n_slots * K))
+        hpr_grad_kernel<<<g, kBlock, 0, s>>>(X, ground, op.w3.get(), op.pptr.get(), op.pidx.get(), op.slot_node.get(),
+                                             dadt_nodes, op.n_slots, K, slots);
     if (const unsigned g = grid_for(m * K))
         recover_elements_kernel<<<g, kBlock, 0, s>>>(slots, op.soc_view, m, k, K, E_out, magn);
 }

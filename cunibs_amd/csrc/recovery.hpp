@@ -21,6 +21,12 @@ struct RecoveryOp {
     int n_fallback = 0;  // slots that took the volume-weighted average
     DeviceBuffer<std::int32_t> ptr, idx, soc, slot_node, slot_tag, boundary, status;
     DeviceBuffer<float> w;
+    std::uint64_t stride = 0;  // tissue slot key = node * stride + tag
+    // harmonic mode: node patch per slot and three gradient weights per patch entry
+    DeviceBuffer<std::int32_t> pptr, pidx;
+    DeviceBuffer<float> w3;
+    std::int64_t nnz_patch = 0;
+    int n_undetermined = 0;
     const std::int32_t* ptr_view = nullptr;
     const std::int32_t* idx_view = nullptr;
     const std::int32_t* soc_view = nullptr;
@@ -32,6 +38,14 @@ void outer_boundary_nodes(const std::int32_t* tets, std::int64_t m, const std::i
 void build_spr(RecoveryOp& op, int mode, const double* nodes, const std::int32_t* tets, const std::int32_t* tags,
                const float* vols, const std::int32_t* n2c_ptr, const std::int32_t* n2c_idx, std::int64_t n,
                std::int64_t m, int max_tag, hipStream_t s);
+void tissue_slots(RecoveryOp& op, const std::int32_t* tets, const std::int32_t* tags, std::int64_t n, std::int64_t m,
+                  int max_tag, hipStream_t s);
+// Harmonic recovery of the potential (fem/recovery.py:269-383, solver/fem/recovery.cu:290-610).
+void build_harmonic(RecoveryOp& op, const double* nodes, const std::int32_t* tets, const std::int32_t* tags,
+                    std::int64_t n, std::int64_t m, int max_tag, hipStream_t s);
+// X (n-1, K) FP64 reduced potential with `ground` removed; dadt_nodes (n, 3, K) FP32.
+void apply_harmonic(const RecoveryOp& op, const double* X, int ground, const float* dadt_nodes, std::int64_t m, int k,
+                    int K, float* slots, float* E_out, float* magn, hipStream_t s);
 // E (m, 3, K) -> slots (n_slots, 3, K), E_out (m, 3, K) (may be null), magn (m, K).
 void apply_spr(const RecoveryOp& op, const float* E, std::int64_t m, int k, int K, float* slots, float* E_out,
                float* magn, hipStream_t s);

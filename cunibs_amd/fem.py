@@ -341,17 +341,30 @@ class SolverContext:
         """The recovery operator's arrays (built on first use), for inspection and tests."""
         n_slots, n_fb = self.recovery_build(mode)
         nc = 4 * self.n_tets
+        spr = mode != "harmonic"
         out = {
-            "ptr": np.empty(n_slots + 1, np.int32), "idx": np.empty(nc, np.int32), "w": np.empty(nc, np.float32),
+            "ptr": np.empty(n_slots + 1, np.int32), "idx": np.empty(nc, np.int32),
+            "w": np.empty(nc, np.float32) if spr else None,
             "slot_of_corner": np.empty(nc, np.int32), "slot_node": np.empty(n_slots, np.int32),
-            "slot_tag": np.empty(n_slots, np.int32) if mode == "spr_tissue" else None,
-            "boundary": np.empty(self.n_nodes, np.int32), "status": np.empty(n_slots, np.int32),
+            "slot_tag": np.empty(n_slots, np.int32) if mode != "spr_global" else None,
+            "boundary": np.empty(self.n_nodes, np.int32) if spr else None, "status": np.empty(n_slots, np.int32),
         }
         check(lib().cn_recovery_operator(self.handle, _lib.RECOVERY_CODES[mode], ptr(out["ptr"]), ptr(out["idx"]),
                                          ptr(out["w"]), ptr(out["slot_of_corner"]), ptr(out["slot_node"]),
                                          ptr(out["slot_tag"]), ptr(out["boundary"]), ptr(out["status"])),
               "cn_recovery_operator")
         out["n_fallback"] = n_fb
+        if not spr:
+            nnz = ctypes.c_int64()
+            und = ctypes.c_int32()
+            check(lib().cn_recovery_patches(self.handle, ctypes.byref(nnz), ctypes.byref(und), None, None, None),
+                  "cn_recovery_patches")
+            out["pptr"] = np.empty(n_slots + 1, np.int32)
+            out["pidx"] = np.empty(nnz.value, np.int32)
+            out["w3"] = np.empty((nnz.value, 3), np.float32)
+            check(lib().cn_recovery_patches(self.handle, None, None, ptr(out["pptr"]), ptr(out["pidx"]),
+                                            ptr(out["w3"])), "cn_recovery_patches")
+            out["n_undetermined"] = int(und.value)
         return out
 
     def bench_spmm(self, K: int, reps: int = 20, mode: int = 1, sub: int = 0) -> float:

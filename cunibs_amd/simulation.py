@@ -20,7 +20,7 @@ from cunibs_amd import fem
 from cunibs_amd.mesh import VOLUME_KEY_TO_LABEL, HeadMesh, load_mesh
 
 RECOVERY_MODES: tuple[str, ...] = ("raw", "spr_global", "spr_tissue", "harmonic")  # fem/recovery.py:77
-SUPPORTED_RECOVERY: tuple[str, ...] = ("raw", "spr_global", "spr_tissue")
+SUPPORTED_RECOVERY: tuple[str, ...] = RECOVERY_MODES
 TISSUE_SLOT_MODES: tuple[str, ...] = ("spr_tissue", "harmonic")
 REGIONS: dict[str, int] = {label: tag for tag, label in VOLUME_KEY_TO_LABEL.items()}
 _DEFAULT_REGION = "gray_matter"
@@ -341,7 +341,7 @@ class Subject:
 
     def iter_simulate(self, coil: Coil, placements: Sequence[Placement], didt: float = 1e6, *,
                       magnitude: bool = False, vectors: bool = False, potential: bool = False,
-                      nodal: bool = False, recovery: str = "raw",
+                      nodal: bool = False, recovery: str = "harmonic",
                       block_k: int | None = None) -> Iterator[FieldResult]:
         """Stream one FieldResult per placement, in order (simulation.py:454-509). Placements are
         solved in blocks of ``block_k`` (default MAX_BLOCK, clamped to [1, MAX_BLOCK]) that share
@@ -357,7 +357,7 @@ class Subject:
 
     def simulate(self, coil: Coil, placement: Placement, didt: float = 1e6, *, magnitude: bool = False,
                  vectors: bool = False, potential: bool = False, nodal: bool = False,
-                 recovery: str = "raw", block_k: int | None = None) -> FieldResult:
+                 recovery: str = "harmonic", block_k: int | None = None) -> FieldResult:
         """One placement (simulation.py:511-546)."""
         if not isinstance(placement, Placement):
             raise TypeError("simulate() takes a single Placement. Use Subject.iter_simulate(...) to stream "

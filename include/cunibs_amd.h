@@ -148,9 +148,10 @@ typedef struct {
     double round_residual[8];    /* out: worst FP64 true relative residual after each round */
 } cn_block_outputs;
 
-/* Recovery modes (fem/recovery.py:77-100): the raw per-element field, or superconvergent patch
+/* Recovery modes (fem/recovery.py:77-100): the raw per-element field; superconvergent patch
  * recovery over each node's incident elements (spr_global) or each (node, tissue) pair's
- * (spr_tissue). Harmonic recovery is not provided yet (CN_EINVAL). */
+ * (spr_tissue); or the harmonic-polynomial fit of the potential over same-tissue node patches
+ * (harmonic, the reference's default). */
 #define CN_RECOVERY_RAW 0
 #define CN_RECOVERY_SPR_GLOBAL 1
 #define CN_RECOVERY_SPR_TISSUE 2
@@ -159,12 +160,17 @@ typedef struct {
 /* Build (once; cached on the context) the operator of a recovery mode (fem/recovery.py:385-453,
  * ensure_recovery). Reports the slot count and how many slots took the volume-weighted average.*/
 int32_t cn_recovery_build(cn_context* ctx, int32_t mode, int64_t* n_slots, int32_t* n_fallback);
-/* Copy a built operator back: ptr (n_slots+1), idx and w (4 n_tets: corner ids and weights),
- * slot_of_corner (4 n_tets), slot_node and slot_tag (n_slots; slot_tag only for spr_tissue),
- * boundary (n_nodes), status (n_slots: 0 fitted, 1 averaged). Any pointer may be NULL. */
+/* Copy a built operator back: ptr (n_slots+1), idx and w (4 n_tets: corner ids and weights; SPR
+ * modes), slot_of_corner (4 n_tets), slot_node and slot_tag (n_slots; slot_tag for the tissue
+ * modes), boundary (n_nodes; SPR modes), status (n_slots: SPR 0 fitted / 1 averaged; harmonic
+ * 0 cubic / 1 quadratic / 2 linear / 3 none). Any pointer may be NULL. */
 int32_t cn_recovery_operator(const cn_context* ctx, int32_t mode, int32_t* ptr, int32_t* idx, float* w,
                              int32_t* slot_of_corner, int32_t* slot_node, int32_t* slot_tag,
                              int32_t* boundary, int32_t* status);
+/* The harmonic operator's node patches: pptr (n_slots+1), pidx (nnz) and the three gradient
+ * weights per entry w3 (nnz, 3); call with NULL arrays first to read nnz. */
+int32_t cn_recovery_patches(const cn_context* ctx, int64_t* nnz, int32_t* n_undetermined, int32_t* pptr,
+                            int32_t* pidx, float* w3);
 
 /* Reduced-basis warm start. No reference counterpart (its nearest analogue is the Galerkin
  * initial guess of uq/conductivity/run.py:154-200). A basis W of up to 512 solver-generated

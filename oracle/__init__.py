@@ -779,3 +779,102 @@ def spr_apply(op: dict, E: np.ndarray):
     soc = op["slot_of_corner"].reshape(-1, 4)
     e_out = slots[soc].mean(axis=1)
     return slots, e_out, np.linalg.norm(e_out, axis=1)
+
+
+# ------------------------------------------------------------------------------------------------
+# Harmonic recovery of the potential (fem/recovery.py:23-50,230-383; solver/fem/recovery.cu:290-610)
+# ------------------------------------------------------------------------------------------------
+
+HPR_MIN_PATCH_NODES = 16  # fem/recovery.py:237
+HPR_MAX_AMPLIFICATION = 50.0  # solver/fem/recovery.cu:44
+HPR_RUNGS = ((16, 20), (9, 9), (4, 4))  # (terms, smallest patch), widest first
+
+
+def harmonic_basis(u, v, t, n_terms: int) -> np.ndarray:
+    """1, u, v, t; the five quadratic and seven cubic harmonic polynomials (each solves
+    Laplace's equation), truncated to n_terms."""
+    cols = [np.ones_like(u), u, v, t]
+    if n_terms >= 9:
+        cols += [u * u - v * v, v * v - t * t, u * v, u * t, v * t]
+    if n_terms >= 16:
+        uu, vv, tt = u * u, v * v, t * t
+        cols += [t * (2 * tt - 3 * uu - 3 * vv), u * (4 * tt - uu - vv), v * (4 * tt - uu - vv),
+                 t * (uu - vv), u * v * t, u * (uu - 3 * vv), v * (3 * uu - vv)]
+    return np.stack(cols, axis=-1)
+
+
+def harmonic_patches(tet_nodes, tet_tags, n_nodes: int):
+    """Per (node, tissue) slot: the distinct nodes of its own tetrahedra, grown to the union of its
+    same-tissue neighbours' first rings when fewer than 16 (fem/recovery.py:230-295)."""
+    tets = np.asarray(tet_nodes, np.int64)
+    ptr, idx, soc, slot_node, slot_tag = tissue_slots(tets, tet_tags)
+    ns = slot_node.shape[0]
+    owner = np.repeat(np.arange(ns, dtype=np.int64), np.diff(ptr))
+    pairs = np.unique(owner[:, None] * n_nodes + tets[idx >> 2])  # (slot, node) of every corner's tet
+    r_slot, r_node = pairs // n_nodes, pairs % n_nodes
+    rp = np.zeros(ns + 1, np.int64)
+    rp[1:] = np.cumsum(np.bincount(r_slot, minlength=ns))
+    stride = int(np.max(tet_tags)) + 1
+    key = slot_node.astype(np.int64) * stride + slot_tag
+    pp = [0]
+    pi = []
+    for s in range(ns):
+        ring = r_node[rp[s]:rp[s + 1]]
+        if ring.size >= HPR_MIN_PATCH_NODES:
+            pi.append(ring)
+        else:
+            nb = np.searchsorted(key, ring * stride + slot_tag[s])
+            pi.append(np.unique(np.concatenate([r_node[rp[b]:rp[b + 1]] for b in nb])))
+        pp.append(pp[-1] + pi[-1].size)
+    return (np.asarray(pp, np.int32), np.concatenate(pi).astype(np.int32), soc, slot_node, slot_tag)
+
+
+def harmonic_operator(nodes_mm, tet_nodes, tet_tags) -> dict:
+    """Three gradient weights per patch node: the widest harmonic rung whose normal matrix has all
+    Cholesky pivots above 1e-12 A_00 and whose amplification h max_c sum|g_c| stays <= 50
+    (solver/fem/recovery.cu:520-564). status: 0 cubic, 1 quadratic, 2 linear, 3 none."""
+    from scipy.linalg import solve_triangular
+    nodes = np.asarray(nodes_mm, np.float64) * 1e-3  # metres: E in V/m
+    pptr, pidx, soc, slot_node, slot_tag = harmonic_patches(tet_nodes, tet_tags, nodes.shape[0])
+    w3 = np.zeros((pidx.size, 3), np.float32)
+    status = np.full(slot_node.size, 3, np.int32)
+    for s in range(slot_node.size):
+        b, e = pptr[s], pptr[s + 1]
+        d = nodes[pidx[b:e]] - nodes[slot_node[s]]
+        h = np.abs(d).max()
+        if not h > 0:
+            continue
+        x = d / h
+        for rung, (n_terms, min_patch) in enumerate(HPR_RUNGS):
+            if e - b < min_patch:
+                continue
+            q = harmonic_basis(x[:, 0], x[:, 1], x[:, 2], n_terms)
+            a = q.T @ q
+            try:
+                l = np.linalg.cholesky(a)
+            except np.linalg.LinAlgError:
+                continue
+            if not (np.diag(l) ** 2 > 1e-12 * a[0, 0]).all():
+                continue
+            rhs = np.zeros((n_terms, 3))
+            rhs[1, 0] = rhs[2, 1] = rhs[3, 2] = 1.0
+            z = solve_triangular(l.T, solve_triangular(l, rhs, lower=True), lower=False)
+            g = q @ z  # (patch, 3) gradient in the scaled frame
+            if np.abs(g).sum(axis=0).max() <= HPR_MAX_AMPLIFICATION:
+                w3[b:e] = (g / h).astype(np.float32)
+                status[s] = rung
+                break
+    return {"pptr": pptr, "pidx": pidx, "w3": w3, "status": status, "slot_of_corner": soc,
+            "slot_node": slot_node, "slot_tag": slot_tag}
+
+
+def harmonic_apply(op: dict, v: np.ndarray, dadt_nodes: np.ndarray):
+    """E[s] = -sum_m w[s, m] v[m] - dA/dt(x_node) (solver/fem/recovery.cu:571-610), then each
+    element's mean over its four corner slots."""
+    counts = np.diff(op["pptr"])
+    owner = np.repeat(np.arange(counts.size), counts)
+    grad = np.zeros((counts.size, 3))
+    np.add.at(grad, owner, op["w3"].astype(np.float64) * np.asarray(v, np.float64)[op["pidx"], None])
+    slots = -grad - np.asarray(dadt_nodes, np.float64)[op["slot_node"]]
+    e_out = slots[op["slot_of_corner"].reshape(-1, 4)].mean(axis=1)
+    return slots, e_out, np.linalg.norm(e_out, axis=1)

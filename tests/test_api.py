@@ -37,8 +37,7 @@ def test_recovery_modes():
     with pytest.raises(ValueError):
         validate_recovery("bogus")
     assert validate_recovery("spr_tissue") == "spr_tissue"
-    with pytest.raises(NotImplementedError):
-        validate_recovery("harmonic")
+    assert validate_recovery("harmonic") == "harmonic"
 
 
 def test_iter_simulate_checks_arguments_before_any_device_work(head8, coil):
@@ -53,7 +52,7 @@ def test_iter_simulate_checks_arguments_before_any_device_work(head8, coil):
     with pytest.raises(ValueError):
         subj.iter_simulate(c, [p], recovery="nope")
     with pytest.raises(ValueError):
-        subj.iter_simulate(c, [p], nodal=True)
+        subj.iter_simulate(c, [p], nodal=True, recovery="raw")
     with pytest.raises(TypeError):
         subj.iter_simulate(c, [(1, 2, 3)])
     assert subj._ctx is None
@@ -109,7 +108,7 @@ def test_subject_end_to_end_matches_the_block_solver(head4, coil):
     placements = [Placement(c, h, d) for c, h, d in sites]
     c = Coil(coil.positions_m, coil.moments, name="fig8")
     with Subject(HeadMesh(head4.nodes_mm, head4.tet_nodes, head4.tet_tags, head4.skin_tris)) as subj:
-        res = list(subj.iter_simulate(c, placements, magnitude=True, block_k=2))
+        res = list(subj.iter_simulate(c, placements, magnitude=True, block_k=2, recovery="raw"))
         assert [r.placement for r in res] == placements
         ref = fem.solve_placements_block(subj.context, c.positions_m, c.moments, sites[:2], 1e6)
         for a, b in zip(res[:2], ref):
@@ -121,5 +120,12 @@ def test_subject_end_to_end_matches_the_block_solver(head4, coil):
             assert r.summary["peak_magnE"] == host["peak_magnE"]
             assert r.summary["distribution"]["p99"] == pytest.approx(host["distribution"]["p99"], rel=1e-6)
             assert r.peak_magnE("csf") > 0
-        one = subj.simulate(c, placements[0], magnitude=True)
+        one = subj.simulate(c, placements[0], magnitude=True, recovery="raw")
         assert np.linalg.norm(one.magnE - res[0].magnE) / np.linalg.norm(res[0].magnE) < 2e-5
+        # The default is the reference's: harmonic recovery, with the nodal field on demand.
+        h = subj.simulate(c, placements[0], magnitude=True, nodal=True)
+        assert h.recovery == "harmonic" and h.E_slots is not None
+        nf = h.nodal_field("gray_matter")
+        assert nf.shape == (subj.context.n_nodes, 3) and np.isfinite(nf).any() and np.isnan(nf).any()
+        with pytest.raises(ValueError):
+            h.nodal_field("all")
