@@ -90,7 +90,19 @@ _SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "cn_warm_reset": (_I32, [_P, _I32]),
     "cn_warm_count": (_I32, [_P, _P]),
     "cn_warm_gram": (_I32, [_P, _P]),
+    "cn_warm_set_modes": (_I32, [_P, _P, _I32, _I32]),
+    "cn_warm_mode_grams": (_I32, [_P, _P, _P]),
     "cn_warm_set_projector": (_I32, [_P, _P, _I32]),
+    "cn_warm_set_coefficients": (_I32, [_P, _P, _I32, _I32]),
+    "cn_warm_modes": (_I32, [_P, _P, _P]),
+    "cn_mlp_create": (_I32, [_I32, _I32, _I32, ctypes.c_uint32, _P]),
+    "cn_mlp_destroy": (None, [_P]),
+    "cn_mlp_n_params": (_I32, [_P, _P]),
+    "cn_mlp_set_params": (_I32, [_P, _P]),
+    "cn_mlp_get_params": (_I32, [_P, _P]),
+    "cn_mlp_gradients": (_I32, [_P, _P, _P, _I32, _P, _P]),
+    "cn_mlp_train": (_I32, [_P, _P, _P, _I64, _I32, _I32, _F, ctypes.c_uint32, _P]),
+    "cn_mlp_predict": (_I32, [_P, _P, _I32, _P]),
     "cn_recovery_operator": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
     "cn_recovery_patches": (_I32, [_P, _P, _P, _P, _P, _P]),
     "cn_k_p1_gradients": (_I32, [_P, _I64, _P, _I64, _P, _P]),

@@ -13,6 +13,7 @@
 #include "coil.hpp"
 #include "prims.hpp"
 #include "recovery.hpp"
+#include "mlp.hpp"
 #include "warm.hpp"
 #include "solve.hpp"
 
@@ -94,6 +95,8 @@ struct cn_context {
     cn::DeviceBuffer<float> d_slots, d_Eraw;
     int max_tag = 0;
     cn::WarmBasis warm;
+    std::vector<double> warm_coef;  // coefficients for the next warm_start = 2 solve
+    int warm_coef_k = 0;
 
     cn::CsrView reduced_view() const {
         return {rptr.get(), rcol.get(), rval32.get(), n - 1, n - 1, nnz_red};
@@ -464,10 +467,16 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
         c.d_x0.ensure(nr * K, "x0");
         h2d(c.d_x0.get(), blk.data(), blk.size(), s);
         dx0 = c.d_x0.get();
-    } else if (sopt && sopt->warm_start) {
+    } else if (sopt && sopt->warm_start == 1) {
         c.d_x0.ensure(nr * K, "x0");
         c.warm.initial_guess(c.d_B.get(), k, K, c.d_x0.get(), s);
         dx0 = c.d_x0.get();
+    } else if (sopt && sopt->warm_start == 2) {
+        require(c.warm_coef_k == k, "warm_start = 2 needs cn_warm_set_coefficients for exactly this block's k");
+        c.d_x0.ensure(nr * K, "x0");
+        c.warm.guess_from_coefficients(c.warm_coef.data(), k, K, c.d_x0.get(), s);
+        dx0 = c.d_x0.get();
+        c.warm_coef_k = 0;
     }
     if (out && out->x0) {
         // (n_red, K) device block -> (k, n_red) host, zeros for a cold start
@@ -1009,10 +1018,145 @@ int32_t cn_warm_gram(cn_context* c, double* G) {
     });
 }
 
-int32_t cn_warm_set_projector(cn_context* c, const double* P, int32_t count) {
+int32_t cn_warm_set_modes(cn_context* c, const double* M, int32_t count, int32_t R) {
+    return guarded([&] {
+        cn::require(c != nullptr && M != nullptr, "null argument");
+        c->warm.set_modes(M, count, R, c->s);
+    });
+}
+
+// Phi^T A Phi and Phi^T A W through 32-column chunks: differenced SpMM, then FP64 cross Grams.
+int32_t cn_warm_mode_grams(cn_context* c, double* G_modes, double* G_cross) {
+    return guarded([&] {
+        cn::require(c != nullptr && c->warm.ready, "warm-start modes not set");
+        const int R = c->warm.used_modes, S = c->warm.count;
+        const std::int64_t nr = c->n - 1;
+        const int rc = (R + 31) / 32, sc = (S + 31) / 32;
+        std::vector<cn::DeviceBuffer<float>> phi(rc);
+        for (int i = 0; i < rc; ++i) {
+            phi[i].reset(static_cast<std::size_t>(nr) * 32, "mode chunk");
+            c->warm.mode_chunk(32 * i, phi[i].get(), c->s);
+        }
+        cn::DeviceBuffer<float> x(static_cast<std::size_t>(nr) * 32, "chunk"), ax(static_cast<std::size_t>(nr) * 32, "A chunk");
+        std::vector<double> tile(32 * 32);
+        auto fill = [&](double* out, int cols_total, int rows_total, int ci, int cj) {
+            for (int a = 0; a < 32 && 32 * ci + a < rows_total; ++a)
+                for (int b = 0; b < 32 && 32 * cj + b < cols_total; ++b)
+                    out[static_cast<std::int64_t>(32 * ci + a) * cols_total + 32 * cj + b] = tile[a * 32 + b];
+        };
+        if (G_modes) {
+            for (int j = 0; j < rc; ++j) {
+                c->solver->spmm_diff(phi[j].get(), ax.get(), 32);
+                for (int i = 0; i < rc; ++i) {
+                    c->solver->cross_gram32(phi[i].get(), ax.get(), tile.data());
+                    fill(G_modes, R, R, i, j);
+                }
+            }
+        }
+        if (G_cross) {
+            for (int j = 0; j < sc; ++j) {
+                c->warm.chunk(32 * j, x.get(), c->s);
+                c->solver->spmm_diff(x.get(), ax.get(), 32);
+                for (int i = 0; i < rc; ++i) {
+                    c->solver->cross_gram32(phi[i].get(), ax.get(), tile.data());
+                    fill(G_cross, S, R, i, j);
+                }
+            }
+        }
+    });
+}
+
+int32_t cn_warm_set_projector(cn_context* c, const double* P, int32_t R) {
     return guarded([&] {
         cn::require(c != nullptr && P != nullptr, "null argument");
-        c->warm.set_projector(P, count, c->s);
+        c->warm.set_projector(P, R, c->s);
+    });
+}
+
+int32_t cn_warm_set_coefficients(cn_context* c, const double* coef, int32_t k, int32_t R) {
+    return guarded([&] {
+        cn::require(c != nullptr && coef != nullptr, "null argument");
+        cn::require(c->warm.ready && R == c->warm.used_modes, "coefficients must match the current modes");
+        cn::require(k >= 1 && k <= 16, "k must be in [1, 16]");
+        c->warm_coef.assign(coef, coef + static_cast<std::size_t>(k) * R);
+        c->warm_coef_k = k;
+    });
+}
+
+int32_t cn_warm_modes(const cn_context* c, uint16_t* out, int32_t* width) {
+    return guarded([&] {
+        cn::require(c != nullptr && c->warm.ready, "warm-start modes not set");
+        if (width) *width = c->warm.n_modes;
+        if (out) {
+            cn::d2h(out, c->warm.modes.get(), static_cast<std::size_t>(c->n - 1) * c->warm.n_modes, c->s);
+            cn::sync(c->s, "modes copy");
+        }
+    });
+}
+
+struct cn_mlp {
+    hipStream_t s = nullptr;
+    std::unique_ptr<cn::Mlp> net;
+    ~cn_mlp() {
+        net.reset();
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+int32_t cn_mlp_create(int32_t n_in, int32_t hidden, int32_t n_out, uint32_t seed, cn_mlp** out) {
+    return guarded([&] {
+        cn::require(out != nullptr, "null argument");
+        auto m = std::make_unique<cn_mlp>();
+        cn::check(hipStreamCreateWithFlags(&m->s, hipStreamNonBlocking), "create stream");
+        m->net = std::make_unique<cn::Mlp>(n_in, hidden, n_out, seed, m->s);
+        *out = m.release();
+    });
+}
+
+void cn_mlp_destroy(cn_mlp* net) { delete net; }
+
+int32_t cn_mlp_n_params(const cn_mlp* net, int64_t* n) {
+    return guarded([&] {
+        cn::require(net != nullptr && n != nullptr, "null argument");
+        *n = net->net->n_params();
+    });
+}
+
+int32_t cn_mlp_set_params(cn_mlp* net, const float* params) {
+    return guarded([&] {
+        cn::require(net != nullptr && params != nullptr, "null argument");
+        net->net->set_params(params);
+    });
+}
+
+int32_t cn_mlp_get_params(const cn_mlp* net, float* params) {
+    return guarded([&] {
+        cn::require(net != nullptr && params != nullptr, "null argument");
+        net->net->get_params(params);
+    });
+}
+
+int32_t cn_mlp_gradients(cn_mlp* net, const float* X, const float* T, int32_t B, float* grads, double* loss) {
+    return guarded([&] {
+        cn::require(net != nullptr && X != nullptr && T != nullptr && grads != nullptr && B >= 1, "bad argument");
+        const double l = net->net->gradients(X, T, B, grads);
+        if (loss) *loss = l;
+    });
+}
+
+int32_t cn_mlp_train(cn_mlp* net, const float* X, const float* T, int64_t N, int32_t epochs, int32_t batch, float lr,
+                     uint32_t seed, double* loss) {
+    return guarded([&] {
+        cn::require(net != nullptr && X != nullptr && T != nullptr, "null argument");
+        const double l = net->net->train(X, T, N, epochs, batch, lr, seed);
+        if (loss) *loss = l;
+    });
+}
+
+int32_t cn_mlp_predict(cn_mlp* net, const float* X, int32_t B, float* Y) {
+    return guarded([&] {
+        cn::require(net != nullptr && X != nullptr && Y != nullptr && B >= 1, "bad argument");
+        net->net->predict(X, B, Y);
     });
 }
 

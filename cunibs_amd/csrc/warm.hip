@@ -1,18 +1,20 @@
 // Reduced-basis warm start for the block PCG.
 //
-// The basis W (n_red x S) is a set of solutions the solver itself produced, stored in BF16 row-major
-// (row r holds the S basis values of unknown r). The basis is defined as that BF16 array, so every
-// product below sees the same W exactly:
-//   Galerkin matrix   G  = W^T (A W)                       (FP64, built once; A applied in FP32)
-//   projection        c  = P (W^T B),  P = pseudo-inverse of G (supplied by the caller)
-//   expansion         x0 = W c                              (BF16 MFMA, FP32 accumulation)
-// x0 is the A-norm best approximation of A^-1 B in span(W), up to the rounding of P and of c,
-// which is carried as a BF16 high part plus a BF16 low part. PCG then iterates from x0 to the
-// same tolerance against ||b||, so the warm start changes only the iteration count.
+// Snapshots W (n_red x S, S <= 512): solutions the solver itself produced, BF16, row-major (row r
+// holds the S values of unknown r). From the Galerkin matrix G = W^T A W (FP64) the caller takes
+// the leading eigenpairs and hands back M = V_R Lambda_R^(-1/2); the modes
+//   Phi = W M          (n_red x R, R <= 256, BF16; an MFMA GEMM with M as BF16 hi + lo halves)
+// are then A-orthonormal up to their BF16 rounding, which the caller removes with the small
+// projector P = (Phi^T A Phi)^-1 (FP64, close to the identity). A block then starts from
+//   x0 = Phi c,   c = P Phi^T B   (Galerkin projection: the A-norm best point of span(Phi))
+//   or c supplied by the caller  (e.g. predicted by the warm-start network from the pose),
+// the expansion being a BF16 MFMA GEMM with FP32 accumulation and c as BF16 hi + lo halves. PCG
+// then iterates from x0 to the same tolerance against ||b||, so only the iteration count changes.
 //
 // No reference counterpart: the reference's closest analogue is the Galerkin-projected initial
 // guess of its conductivity-UQ loop (uq/conductivity/run.py:154-200).
-#include <hip/hip_bf16.h>
+#include <cstring>
+#include <vector>
 
 #include "warm.hpp"
 
@@ -73,30 +75,30 @@ __global__ void __launch_bounds__(512) project_kernel(const unsigned short* __re
     }
 }
 
-// WtB[s, c] = sum over blocks (ascending) of the partials; then c = P WtB (FP64) split into BF16
-// hi/lo halves laid out for the expansion: coef[(half * S16 + s) * 16 + c], zero-padded to
-// S16 = ceil(S / 16) * 16 rows and 16 columns.
+// c = P (Phi^T B): the partials summed over blocks in order, then the R x R projector (FP64);
+// c is split into BF16 hi/lo halves laid out for the expansion: coef[(half * R + s) * 16 + col]
+// (R a multiple of 16, columns >= k zero).
 template <int K>
-__global__ void coefficients_kernel(const double* __restrict__ partials, int nb, int S, const double* __restrict__ P,
-                                    int k, unsigned short* __restrict__ coef, int S16) {
-    extern __shared__ double wtb[];  // S * K
-    for (int e = threadIdx.x; e < S * K; e += blockDim.x) {
+__global__ void coefficients_kernel(const double* __restrict__ partials, int nb, int R,
+                                    const double* __restrict__ P, int k, unsigned short* __restrict__ coef) {
+    extern __shared__ double ptb[];  // R * K
+    for (int e = threadIdx.x; e < R * K; e += blockDim.x) {
         double acc = 0.0;
-        for (int b = 0; b < nb; ++b) acc += partials[static_cast<std::int64_t>(b) * S * K + e];
-        wtb[e] = acc;
+        for (int b = 0; b < nb; ++b) acc += partials[static_cast<std::int64_t>(b) * R * K + e];
+        ptb[e] = acc;
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < S16 * 16; e += blockDim.x) {
-        const int s = e / 16, c = e % 16;
+    for (int e = threadIdx.x; e < R * 16; e += blockDim.x) {
+        const int sidx = e / 16, c = e % 16;
         double v = 0.0;
-        if (s < S && c < k) {
-            for (int j = 0; j < S; ++j) v = fma(P[static_cast<std::int64_t>(s) * S + j], wtb[j * K + c], v);
+        if (c < k) {
+            for (int j = 0; j < R; ++j) v = fma(P[static_cast<std::int64_t>(sidx) * R + j], ptb[j * K + c], v);
         }
         const float vf = static_cast<float>(v);
         const unsigned short hi = bf16_bits(vf);
         const unsigned short lo = bf16_bits(vf - bf16_value(hi));
         coef[e] = hi;
-        coef[S16 * 16 + e] = lo;
+        coef[R * 16 + e] = lo;
     }
 }
 
@@ -144,6 +146,39 @@ __global__ void __launch_bounds__(256) expand_kernel(const unsigned short* __res
     }
 }
 
+// Modes (n, R) BF16 = W (n, S) M (S, R): one wavefront per 16 x 16 output tile, S stepped by 16,
+// M carried as BF16 hi/lo halves mhl[(half * S16 + s) * R + j] (zero rows past S).
+__global__ void __launch_bounds__(256) modes_kernel(const unsigned short* __restrict__ basis, std::int64_t n, int S,
+                                                   int S16, const unsigned short* __restrict__ mhl, int R,
+                                                   unsigned short* __restrict__ modes) {
+    const int lane = threadIdx.x & 63;
+    const std::int64_t tile = static_cast<std::int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int tiles_r = R / 16;
+    const std::int64_t r0 = (tile / tiles_r) * 16;
+    const int j0 = static_cast<int>(tile % tiles_r) * 16;
+    if (r0 >= n) return;
+    const int i = lane & 15, q = lane >> 4;
+    const std::int64_t row = r0 + i;
+    floatx4_t acc = {0.f, 0.f, 0.f, 0.f};
+    for (int s0 = 0; s0 < S16; s0 += 16) {
+        bf16x4_t a, bh, bl;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int sidx = s0 + 4 * q + j;
+            a[j] = (row < n && sidx < S) ? static_cast<short>(basis[row * S + sidx]) : static_cast<short>(0);
+            bh[j] = static_cast<short>(mhl[static_cast<std::int64_t>(sidx) * R + j0 + i]);
+            bl[j] = static_cast<short>(mhl[static_cast<std::int64_t>(S16 + sidx) * R + j0 + i]);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bl, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const std::int64_t r = r0 + 4 * q + j;
+        if (r < n) modes[r * R + j0 + i] = bf16_bits(acc[j]);
+    }
+}
+
 }  // namespace
 
 void WarmBasis::reset(std::int64_t n_rows, int capacity) {
@@ -152,6 +187,8 @@ void WarmBasis::reset(std::int64_t n_rows, int capacity) {
     cap = capacity;
     count = 0;
     ready = false;
+    n_modes = used_modes = 0;
+    modes.reset(0);
     basis.reset(static_cast<std::size_t>(n) * cap, "warm basis");
     check(hipMemset(basis.get(), 0, sizeof(unsigned short) * n * cap), "warm basis clear");
 }
@@ -171,41 +208,80 @@ void WarmBasis::chunk(int s0, float* out, hipStream_t s) const {
     if (const unsigned g = grid_for(n * 32)) gather_chunk_kernel<<<g, kBlock, 0, s>>>(basis.get(), n, cap, s0, out);
 }
 
-void WarmBasis::set_projector(const double* P_host, int S, hipStream_t s) {
-    require(S == count && S >= 1, "projector size must equal the number of basis columns");
-    P.reset(static_cast<std::size_t>(S) * S, "warm projector");
-    h2d(P.get(), P_host, static_cast<std::size_t>(S) * S, s);
-    sync(s, "warm projector");
+// Host BF16 rounding, the same rule as the device's (round to nearest even).
+static unsigned short host_bf16(float x) {
+    unsigned u;
+    std::memcpy(&u, &x, sizeof u);
+    return static_cast<unsigned short>((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+static float host_bf16_value(unsigned short b) {
+    const unsigned u = static_cast<unsigned>(b) << 16;
+    float x;
+    std::memcpy(&x, &u, sizeof x);
+    return x;
+}
+
+void WarmBasis::set_modes(const double* M_host, int S, int R, hipStream_t s) {
+    require(S == count && S >= 1, "the mode matrix must have one row per basis column");
+    require(R >= 1 && R <= kMaxWarmModes, "mode count must be in [1, 256]");
+    const int S16 = (S + 15) / 16 * 16;
+    const int R16 = (R + 15) / 16 * 16;
+    std::vector<unsigned short> mhl(static_cast<std::size_t>(2) * S16 * R16, 0);
+    for (int a = 0; a < S; ++a)
+        for (int j = 0; j < R; ++j) {
+            const float v = static_cast<float>(M_host[static_cast<std::size_t>(a) * R + j]);
+            const unsigned short hi = host_bf16(v);
+            mhl[static_cast<std::size_t>(a) * R16 + j] = hi;
+            mhl[static_cast<std::size_t>(S16 + a) * R16 + j] = host_bf16(v - host_bf16_value(hi));
+        }
+    DeviceBuffer<unsigned short> dm(mhl.size(), "mode matrix");
+    h2d(dm.get(), mhl.data(), mhl.size(), s);
+    n_modes = R16;
+    modes.reset(static_cast<std::size_t>(n) * R16, "warm modes");
+    const std::int64_t tiles = (n + 15) / 16 * (R16 / 16);
+    modes_kernel<<<static_cast<unsigned>((tiles + 3) / 4), 256, 0, s>>>(basis.get(), n, cap, S16, dm.get(), R16,
+                                                                       modes.get());
+    used_modes = R;
+    // P = I until the caller supplies the BF16-rounding correction.
+    std::vector<double> eye(static_cast<std::size_t>(R16) * R16, 0.0);
+    for (int j = 0; j < R; ++j) eye[static_cast<std::size_t>(j) * R16 + j] = 1.0;
+    P.reset(eye.size(), "warm projector");
+    h2d(P.get(), eye.data(), eye.size(), s);
+    sync(s, "warm modes");
     ready = true;
 }
 
+void WarmBasis::set_projector(const double* P_host, int R, hipStream_t s) {
+    require(ready && R == used_modes, "the projector must be R x R for the current modes");
+    std::vector<double> pad(static_cast<std::size_t>(n_modes) * n_modes, 0.0);
+    for (int a = 0; a < R; ++a)
+        for (int b = 0; b < R; ++b) pad[static_cast<std::size_t>(a) * n_modes + b] = P_host[static_cast<std::size_t>(a) * R + b];
+    h2d(P.get(), pad.data(), pad.size(), s);
+    sync(s, "warm projector");
+}
+
+void WarmBasis::mode_chunk(int j0, float* out, hipStream_t s) const {
+    if (const unsigned g = grid_for(n * 32)) gather_chunk_kernel<<<g, kBlock, 0, s>>>(modes.get(), n, n_modes, j0, out);
+}
+
+void WarmBasis::expand(int k, int K, double* x0, hipStream_t s) {
+    const std::int64_t tiles = (n + 15) / 16;
+    expand_kernel<<<grid_for(tiles, 4), 256, 0, s>>>(modes.get(), n, n_modes, n_modes, coef.get(), k, K, x0);
+}
+
 void WarmBasis::initial_guess(const float* B, int k, int K, double* x0, hipStream_t s) {
-    require(ready, "warm start requested but no projector is set");
-    const int S = cap;  // row stride; columns >= count are zero in the basis and in P's image
-    const int S_used = count;
-    const int S16 = (S + 15) / 16 * 16;
+    require(ready, "warm start requested but no modes are set");
+    const int R = n_modes;
     const int nb = kProjectBlocks;
     const std::int64_t rpb = (n + nb - 1) / nb;
-    partials.ensure(static_cast<std::size_t>(nb) * S * K, "warm partials");
-    coef.ensure(static_cast<std::size_t>(2) * S16 * 16, "warm coefficients");
-    Pfull.ensure(static_cast<std::size_t>(S) * S, "warm projector (padded)");
-    // P padded to the basis stride: rows/columns >= count are zero.
-    if (padded_for != S_used) {
-        std::vector<double> host(static_cast<std::size_t>(S) * S, 0.0);
-        std::vector<double> small(static_cast<std::size_t>(S_used) * S_used);
-        d2h(small.data(), P.get(), small.size(), s);
-        sync(s, "projector");
-        for (int a = 0; a < S_used; ++a)
-            for (int b = 0; b < S_used; ++b) host[static_cast<std::size_t>(a) * S + b] = small[a * S_used + b];
-        h2d(Pfull.get(), host.data(), host.size(), s);
-        padded_for = S_used;
-    }
-    const int threads = (S + 63) / 64 * 64;
+    partials.ensure(static_cast<std::size_t>(nb) * R * K, "warm partials");
+    coef.ensure(static_cast<std::size_t>(2) * R * 16, "warm coefficients");
+    const int threads = (R + 63) / 64 * 64;
     auto run = [&](auto kc) {
         constexpr int KK = decltype(kc)::value;
-        project_kernel<KK><<<nb, threads, 0, s>>>(basis.get(), n, S, B, rpb, partials.get());
-        coefficients_kernel<KK><<<1, 512, sizeof(double) * S * KK, s>>>(partials.get(), nb, S, Pfull.get(), k,
-                                                                        coef.get(), S16);
+        project_kernel<KK><<<nb, threads, 0, s>>>(modes.get(), n, R, B, rpb, partials.get());
+        coefficients_kernel<KK><<<1, 512, sizeof(double) * R * KK, s>>>(partials.get(), nb, R, P.get(), k, coef.get());
     };
     switch (K) {
         case 1: run(std::integral_constant<int, 1>{}); break;
@@ -215,8 +291,24 @@ void WarmBasis::initial_guess(const float* B, int k, int K, double* x0, hipStrea
         case 16: run(std::integral_constant<int, 16>{}); break;
         default: throw InvalidArgument("warm start supports block widths up to 16");
     }
-    const std::int64_t tiles = (n + 15) / 16;
-    expand_kernel<<<grid_for(tiles, 4), 256, 0, s>>>(basis.get(), n, S, S16, coef.get(), k, K, x0);
+    expand(k, K, x0, s);
+}
+
+void WarmBasis::guess_from_coefficients(const double* c_host, int k, int K, double* x0, hipStream_t s) {
+    require(ready, "warm start requested but no modes are set");
+    require(k <= 16 && K <= 16, "supplied-coefficient warm start supports block widths up to 16");
+    const int R = n_modes;
+    std::vector<unsigned short> hl(static_cast<std::size_t>(2) * R * 16, 0);
+    for (int c = 0; c < k; ++c)
+        for (int j = 0; j < used_modes; ++j) {
+            const float v = static_cast<float>(c_host[static_cast<std::size_t>(c) * used_modes + j]);
+            const unsigned short hi = host_bf16(v);
+            hl[static_cast<std::size_t>(j) * 16 + c] = hi;
+            hl[static_cast<std::size_t>(R + j) * 16 + c] = host_bf16(v - host_bf16_value(hi));
+        }
+    coef.ensure(hl.size(), "warm coefficients");
+    h2d(coef.get(), hl.data(), hl.size(), s);
+    expand(k, K, x0, s);
 }
 
 }  // namespace cn

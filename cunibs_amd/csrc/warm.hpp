@@ -7,6 +7,7 @@
 namespace cn {
 
 constexpr int kMaxWarmBasis = 512;
+constexpr int kMaxWarmModes = 256;
 constexpr int kProjectBlocks = 1024;
 
 struct WarmBasis {
@@ -14,9 +15,11 @@ struct WarmBasis {
     int cap = 0;         // basis capacity (row stride of `basis`)
     int count = 0;       // columns filled
     bool ready = false;  // a projector matching `count` has been set
-    int padded_for = -1;
+    int n_modes = 0;     // mode buffer width (a multiple of 16)
+    int used_modes = 0;  // modes set by the caller
     DeviceBuffer<unsigned short> basis;  // (n, cap) BF16 bits, row-major
-    DeviceBuffer<double> P, Pfull, partials;
+    DeviceBuffer<unsigned short> modes;  // (n, n_modes) BF16 bits, row-major
+    DeviceBuffer<double> P, partials;    // P: (n_modes, n_modes), identity-padded
     DeviceBuffer<unsigned short> coef;
 
     void reset(std::int64_t n_rows, int capacity);
@@ -24,10 +27,18 @@ struct WarmBasis {
     int append(const double* X, int k, int K, hipStream_t s);
     // Columns [s0, s0 + 32) as an (n, 32) FP32 block (zero past the capacity).
     void chunk(int s0, float* out, hipStream_t s) const;
-    // P: (S, S) FP64 host array, S = count: the pseudo-inverse of the Galerkin matrix W^T A W.
-    void set_projector(const double* P_host, int S, hipStream_t s);
-    // x0 (n, K) FP64 = W P W^T B for the (n, K) FP32 block B (columns >= k of x0 are zero).
+    // Modes Phi = W M for M (count x R) FP64 host, row-major; resets P to the identity.
+    void set_modes(const double* M_host, int S, int R, hipStream_t s);
+    // Columns [j0, j0 + 32) of the modes as an (n, 32) FP32 block.
+    void mode_chunk(int j0, float* out, hipStream_t s) const;
+    // P (R x R) FP64 host: the inverse of Phi^T A Phi.
+    void set_projector(const double* P_host, int R, hipStream_t s);
+    // x0 (n, K) FP64 = Phi P Phi^T B for the (n, K) FP32 block B (columns >= k of x0 are zero).
     void initial_guess(const float* B, int k, int K, double* x0, hipStream_t s);
+    // x0 = Phi c for caller-supplied coefficients c (k x R) FP64 host.
+    void guess_from_coefficients(const double* c_host, int k, int K, double* x0, hipStream_t s);
+    // x0 = Phi c from the coefficients already in `coef`.
+    void expand(int k, int K, double* x0, hipStream_t s);
 };
 
 }  // namespace cn

@@ -309,24 +309,44 @@ class SolverContext:
         check(lib().cn_warm_gram(self.handle, ptr(g)), "cn_warm_gram")
         return g
 
-    def warm_set_projector(self, p: np.ndarray) -> None:
-        p = np.ascontiguousarray(p, np.float64)
-        if p.ndim != 2 or p.shape[0] != p.shape[1]:
-            raise ValueError("the projector must be square")
-        check(lib().cn_warm_set_projector(self.handle, ptr(p), p.shape[0]), "cn_warm_set_projector")
-
-    def warm_finalize(self, rcond: float = 1e-7) -> int:
-        """Pseudo-inverse of the Galerkin matrix by a symmetric eigendecomposition, dropping
-        directions below rcond x the largest eigenvalue (near-duplicate snapshots); returns the rank
-        kept."""
+    def warm_finalize(self, rcond: float = 1e-7, max_modes: int = 256) -> int:
+        """Modes of the collected snapshots: the leading eigenpairs (V, lambda) of the Galerkin
+        matrix G = W^T A W, at most max_modes and above rcond x the largest (near-duplicate
+        snapshots drop out), give the A-orthonormal modes Phi = W V lambda^-1/2 (formed on the
+        device in BF16); P = (Phi^T A Phi)^-1 removes their BF16 rounding. Returns the mode count."""
         g = self.warm_gram()
         g = 0.5 * (g + g.T)
         lam, v = np.linalg.eigh(g)
-        keep = lam > rcond * lam.max()
-        p = (v[:, keep] / lam[keep]) @ v[:, keep].T
-        self.warm_set_projector(p)
-        self.warm_projector = p
-        return int(keep.sum())
+        order = np.argsort(lam)[::-1]
+        lam, v = lam[order], v[:, order]
+        r = int(min(max_modes, (lam > rcond * lam[0]).sum()))
+        m = np.ascontiguousarray(v[:, :r] / np.sqrt(lam[:r]))
+        check(lib().cn_warm_set_modes(self.handle, ptr(m), m.shape[0], r), "cn_warm_set_modes")
+        gpp = np.empty((r, r), np.float64)
+        gpw = np.empty((r, m.shape[0]), np.float64)
+        check(lib().cn_warm_mode_grams(self.handle, ptr(gpp), ptr(gpw)), "cn_warm_mode_grams")
+        p = np.ascontiguousarray(np.linalg.inv(0.5 * (gpp + gpp.T)))
+        check(lib().cn_warm_set_projector(self.handle, ptr(p), r), "cn_warm_set_projector")
+        self.warm_state = {"modes": r, "P": p, "G_modes": gpp, "G_cross": gpw}
+        return r
+
+    def warm_targets(self) -> np.ndarray:
+        """Galerkin coefficients of every snapshot in the modes, (count, R): P Phi^T A W."""
+        st = self.warm_state
+        return np.ascontiguousarray((st["P"] @ st["G_cross"]).T)
+
+    def warm_set_coefficients(self, c: np.ndarray) -> None:
+        c = np.ascontiguousarray(c, np.float64)
+        check(lib().cn_warm_set_coefficients(self.handle, ptr(c), c.shape[0], c.shape[1]),
+              "cn_warm_set_coefficients")
+
+    def warm_modes(self) -> np.ndarray:
+        """The modes as FP32 values of their BF16 bits, (n_reduced, width)."""
+        w = ctypes.c_int32()
+        check(lib().cn_warm_modes(self.handle, None, ctypes.byref(w)), "cn_warm_modes")
+        bits = np.empty((self.n_nodes - 1, w.value), np.uint16)
+        check(lib().cn_warm_modes(self.handle, ptr(bits), ctypes.byref(w)), "cn_warm_modes")
+        return (bits.astype(np.uint32) << 16).view(np.float32)
 
     def recovery_build(self, mode: str) -> tuple[int, int]:
         """Build (once) the recovery operator of ``mode``; returns (n_slots, n_fallback)."""
@@ -448,7 +468,7 @@ def solve_placements_block(
         out.E_slots = ptr(e_slots)
     opt = _lib.default_solve_options()
     opt.recovery = _lib.RECOVERY_CODES[recovery]
-    opt.warm_start = int(warm_start)
+    opt.warm_start = int(warm_start)  # True: Galerkin projection; 2: coefficients set beforehand
     opt.collect = int(collect)
     opt.tolerance = float(tolerance)
     opt.max_iters = int(max_iters)

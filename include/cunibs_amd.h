@@ -121,7 +121,7 @@ typedef struct {
     int32_t use_graph;
     int32_t check_every;
     int32_t recovery;  /* CN_RECOVERY_*: the field E, |E| and the metrics describe; default raw */
-    int32_t warm_start; /* 1: start PCG from the reduced-basis guess (cn_warm_*); default 0 */
+    int32_t warm_start; /* 1: Galerkin reduced-basis guess, 2: supplied coefficients (cn_warm_*); 0 */
     int32_t collect;    /* 1: append this block's solutions to the warm-start basis; default 0 */
 } cn_solve_options;
 
@@ -173,15 +173,40 @@ int32_t cn_recovery_patches(const cn_context* ctx, int64_t* nnz, int32_t* n_unde
                             int32_t* pidx, float* w3);
 
 /* Reduced-basis warm start. No reference counterpart (its nearest analogue is the Galerkin
- * initial guess of uq/conductivity/run.py:154-200). A basis W of up to 512 solver-generated
- * solutions (BF16) is collected by solves with opt->collect = 1; cn_warm_gram returns the
- * Galerkin matrix G = W^T A W; the caller hands back P, a (pseudo-)inverse of G, with
- * cn_warm_set_projector; solves with opt->warm_start = 1 then start from x0 = W P W^T b
- * (expansion as a BF16 MFMA GEMM) and still iterate to the same tolerance. */
+ * initial guess of uq/conductivity/run.py:154-200). Solves with opt->collect = 1 append their
+ * solutions (BF16) to a snapshot basis W of up to 512 columns. cn_warm_gram returns
+ * G = W^T A W (count x count); the caller hands back M = V Lambda^-1/2 of its leading R <= 256
+ * eigenpairs, and cn_warm_set_modes forms the modes Phi = W M (BF16 MFMA GEMM). cn_warm_mode_grams
+ * returns Phi^T A Phi (R x R) and Phi^T A W (R x count); cn_warm_set_projector takes
+ * P = (Phi^T A Phi)^-1. A solve with opt->warm_start = 1 starts from x0 = Phi P Phi^T b; with
+ * warm_start = 2 from x0 = Phi c for the k x R coefficients given to cn_warm_set_coefficients
+ * (e.g. predicted by the warm-start network). PCG iterates to the same tolerance either way. */
 int32_t cn_warm_reset(cn_context* ctx, int32_t capacity);
 int32_t cn_warm_count(const cn_context* ctx, int32_t* count);
-int32_t cn_warm_gram(cn_context* ctx, double* G); /* (count, count) row-major */
-int32_t cn_warm_set_projector(cn_context* ctx, const double* P, int32_t count);
+int32_t cn_warm_gram(cn_context* ctx, double* G);
+int32_t cn_warm_set_modes(cn_context* ctx, const double* M, int32_t count, int32_t R);
+int32_t cn_warm_mode_grams(cn_context* ctx, double* G_modes, double* G_cross);
+int32_t cn_warm_set_projector(cn_context* ctx, const double* P, int32_t R);
+int32_t cn_warm_set_coefficients(cn_context* ctx, const double* c, int32_t k, int32_t R);
+/* The modes as BF16 bits, (n_reduced, width) row-major; width (a multiple of 16 >= R) is
+ * written to *width; call with out = NULL to read it. */
+int32_t cn_warm_modes(const cn_context* ctx, uint16_t* out, int32_t* width);
+
+/* The warm-start network: n_in (<= 16) inputs -> hidden -> hidden -> n_out, GELU, trained on the
+ * GPU in FP32 (Adam, mean squared error), inference with BF16 weights on MFMA. No reference
+ * counterpart. Parameters are one flat FP32 array: W1 (16 x hidden, rows >= n_in zero), b1,
+ * W2 (hidden x hidden), b2, W3 (hidden x n_out), b3. */
+typedef struct cn_mlp cn_mlp;
+int32_t cn_mlp_create(int32_t n_in, int32_t hidden, int32_t n_out, uint32_t seed, cn_mlp** out);
+void cn_mlp_destroy(cn_mlp* net);
+int32_t cn_mlp_n_params(const cn_mlp* net, int64_t* n);
+int32_t cn_mlp_set_params(cn_mlp* net, const float* params);
+int32_t cn_mlp_get_params(const cn_mlp* net, float* params);
+/* Loss and gradients of one batch X (B x n_in), T (B x n_out). */
+int32_t cn_mlp_gradients(cn_mlp* net, const float* X, const float* T, int32_t B, float* grads, double* loss);
+int32_t cn_mlp_train(cn_mlp* net, const float* X, const float* T, int64_t N, int32_t epochs, int32_t batch,
+                     float lr, uint32_t seed, double* loss);
+int32_t cn_mlp_predict(cn_mlp* net, const float* X, int32_t B, float* Y);
 
 /* Solve k <= 32 placements as one block (fem/solve.py:865-979): coil frames, dA/dt, RHS, block
  * PCG, E reconstruction, recovery (opt->recovery; built on first use), region metrics. Dipoles in metres / A m^2; centres and

@@ -878,3 +878,60 @@ def harmonic_apply(op: dict, v: np.ndarray, dadt_nodes: np.ndarray):
     slots = -grad - np.asarray(dadt_nodes, np.float64)[op["slot_node"]]
     e_out = slots[op["slot_of_corner"].reshape(-1, 4)].mean(axis=1)
     return slots, e_out, np.linalg.norm(e_out, axis=1)
+
+
+# ------------------------------------------------------------------------------------------------
+# Warm-start network (FP64 restatement of cunibs_amd/csrc/mlp.hip's training arithmetic)
+# ------------------------------------------------------------------------------------------------
+
+MLP_IN_PAD = 16
+
+
+def mlp_unpack(params: np.ndarray, n_hidden: int, n_out: int):
+    p = np.asarray(params, np.float64)
+    shapes = [(MLP_IN_PAD, n_hidden), (n_hidden, n_hidden), (n_hidden, n_out)]
+    out, at = [], 0
+    for r, c in shapes:
+        w = p[at:at + r * c].reshape(r, c)
+        at += r * c
+        b = p[at:at + c]
+        at += c
+        out.append((w, b))
+    assert at == p.size
+    return out
+
+
+def _gelu(x):
+    from scipy.special import erf
+    return 0.5 * x * (1.0 + erf(x / np.sqrt(2.0)))
+
+
+def _gelu_grad(x):
+    from scipy.special import erf
+    return 0.5 * (1.0 + erf(x / np.sqrt(2.0))) + x * np.exp(-0.5 * x * x) / np.sqrt(2.0 * np.pi)
+
+
+def mlp_forward(params, X, n_hidden: int, n_out: int):
+    (w1, b1), (w2, b2), (w3, b3) = mlp_unpack(params, n_hidden, n_out)
+    x = np.zeros((X.shape[0], MLP_IN_PAD))
+    x[:, :X.shape[1]] = X
+    z1 = x @ w1 + b1
+    h1 = _gelu(z1)
+    z2 = h1 @ w2 + b2
+    h2 = _gelu(z2)
+    return x, z1, h1, z2, h2, h2 @ w3 + b3
+
+
+def mlp_loss_and_gradients(params, X, T, n_hidden: int, n_out: int):
+    """Mean squared error over the batch and its gradient, in the flat parameter layout."""
+    (w1, b1), (w2, b2), (w3, b3) = mlp_unpack(params, n_hidden, n_out)
+    x, z1, h1, z2, h2, y = mlp_forward(params, X, n_hidden, n_out)
+    d = y - T
+    loss = float((d * d).mean())
+    dy = 2.0 * d / d.size
+    g3, gb3 = h2.T @ dy, dy.sum(0)
+    dz2 = (dy @ w3.T) * _gelu_grad(z2)
+    g2, gb2 = h1.T @ dz2, dz2.sum(0)
+    dz1 = (dz2 @ w2.T) * _gelu_grad(z1)
+    g1, gb1 = x.T @ dz1, dz1.sum(0)
+    return loss, np.concatenate([g1.ravel(), gb1, g2.ravel(), gb2, g3.ravel(), gb3])

@@ -34,20 +34,38 @@ def trained(head4, coil):
     ctx.free()
 
 
-def test_initial_guess_is_the_galerkin_projection(trained, coil):
-    from cunibs_amd import fem, synthetic
-    ctx, _, sols, rank = trained
+def test_modes_are_a_orthonormal_up_to_bf16(trained):
+    """Phi^T A Phi = I up to the modes' BF16 rounding, which P removes."""
+    ctx, _, _, rank = trained
+    g = ctx.warm_state["G_modes"]
     assert rank >= 40
+    assert np.abs(g - np.eye(rank)).max() < 5e-2
+    np.testing.assert_allclose(ctx.warm_state["P"] @ g, np.eye(rank), atol=1e-8)
+
+
+def test_initial_guess_is_the_galerkin_projection(trained, coil):
+    """x0 = Phi P Phi^T b, evaluated on the host from the device's modes, and the MFMA expansion of
+    the coefficients agrees with it."""
+    from cunibs_amd import fem, synthetic
+    ctx, _, _, rank = trained
     keep = np.arange(ctx.n_nodes) != ctx.ground_node
-    w = bf16_round(sols[:, keep].astype(np.float32)).astype(np.float64).T  # (n_red, S)
+    phi = ctx.warm_modes()[:, :rank].astype(np.float64)
+    p = ctx.warm_state["P"]
     sites = synthetic.cap_sites(4, seed=12)
     extra = {"x0": 1, "rhs": 1}
     fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites, 1e6, warm_start=True, extra=extra)
+    coefs = []
     for i in range(4):
         b = extra["rhs"][i][keep].astype(np.float64)
-        want = w @ (ctx.warm_projector @ (w.T @ b))
-        got = extra["x0"][i]
-        assert np.linalg.norm(got - want) <= 1e-4 * np.linalg.norm(want)
+        c = p @ (phi.T @ b)
+        coefs.append(c)
+        want = phi @ c
+        assert np.linalg.norm(extra["x0"][i] - want) <= 1e-4 * np.linalg.norm(want)
+    # Supplied coefficients (warm_start = 2) expand to the same guess.
+    ctx.warm_set_coefficients(np.array(coefs))
+    extra2 = {"x0": 1}
+    fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites, 1e6, warm_start=2, extra=extra2)
+    assert np.linalg.norm(extra2["x0"] - extra["x0"]) <= 1e-5 * np.linalg.norm(extra["x0"])
 
 
 def test_warm_start_keeps_the_answer_and_saves_iterations(trained, coil):
