@@ -14,6 +14,7 @@
 #include "prims.hpp"
 #include "recovery.hpp"
 #include "mlp.hpp"
+#include "uq.hpp"
 #include "warm.hpp"
 #include "solve.hpp"
 
@@ -97,6 +98,20 @@ struct cn_context {
     cn::WarmBasis warm;
     std::vector<double> warm_coef;  // coefficients for the next warm_start = 2 solve
     int warm_coef_k = 0;
+    // conductivity uncertainty: per-tissue operators and RHS of one placement
+    struct Uq {
+        bool ready = false;
+        std::vector<int> tags;
+        cn::DeviceBuffer<double> vals, off, rowsum;  // (T, nnz_red), (T, nnz_off), (T, n_red) FP64
+        cn::DeviceBuffer<float> rhs;                  // (T, n_red)
+        cn::DeviceBuffer<float> dadt_elm1, dadt_nodes1;  // the placement's dA/dt, one column
+        cn::DeviceBuffer<double> sigma;               // (K, T)
+        cn::DeviceBuffer<float> off_k, rowsum_k;      // (nnz_off, K), (n_red, K)
+        cn::DeviceBuffer<double> sum, sumsq;          // |E| moments over every sample solved
+        std::int64_t count = 0;
+        double transform[16] = {};
+        cn::MultiOperator mop;
+    } uq;
 
     cn::CsrView reduced_view() const {
         return {rptr.get(), rcol.get(), rval32.get(), n - 1, n - 1, nnz_red};
@@ -375,6 +390,173 @@ void place_block(cn_context& c, const double* centers, const double* handles, co
     d2h(tf_host, c.d_tf.get(), 16 * k, s);
     d2h(degen_host, c.d_degen.get(), k, s);
     sync(s, "placement");
+}
+
+// Per-tissue unit-conductivity operators on the reduced pattern and per-tissue RHS of one placement
+// (uq/conductivity/assembly.py:37-146): K(sigma) = sum_t sigma_t K_t, b(sigma) = sum_t sigma_t b_t.
+void uq_prepare(cn_context& c, const double* pos_m, const double* mom, std::int64_t n_dip, const double* center,
+                const double* handle, double dist, double didt) {
+    using namespace cn;
+    const hipStream_t s = c.s;
+    const std::int64_t n = c.n, m = c.m, nr = n - 1, nnz_off = c.nnz_red - nr;
+    // tags present in the mesh, ascending
+    std::vector<std::int32_t> tags(m);
+    d2h(tags.data(), c.tags.get(), m, s);
+    sync(s, "tags");
+    std::vector<int> present;
+    {
+        std::vector<char> seen(c.max_tag + 1, 0);
+        for (auto t : tags) seen[t] = 1;
+        for (int t = 0; t <= c.max_tag; ++t)
+            if (seen[t]) present.push_back(t);
+    }
+    require(static_cast<int>(present.size()) <= kMaxTissues, "more than 16 tissues");
+    const int T = static_cast<int>(present.size());
+    auto& u = c.uq;
+    u = cn_context::Uq{};
+    u.tags = present;
+    u.vals.reset(static_cast<std::size_t>(T) * c.nnz_red, "tissue values");
+    u.off.reset(static_cast<std::size_t>(T) * std::max<std::int64_t>(nnz_off, 1), "tissue off-diagonals");
+    u.rowsum.reset(static_cast<std::size_t>(T) * nr, "tissue row sums");
+    u.rhs.reset(static_cast<std::size_t>(T) * nr, "tissue rhs");
+    // Stiffness per tissue: gradients and pattern again (the context keeps neither in FP64).
+    DeviceBuffer<double> g64(12 * m, "g64"), vols64(m, "vols64"), scale(m, "scale");
+    p1_gradients(c.nodes.get(), c.tets.get(), m, g64.get(), vols64.get(), s);
+    DeviceBuffer<std::int32_t> indptr, indices;
+    const std::int64_t nnz = stiffness_pattern(c.tets.get(), c.n2c_ptr.get(), c.n2c_idx.get(), n, indptr, indices, s);
+    DeviceBuffer<double> data(std::max<std::int64_t>(nnz, 1), "tissue stiffness");
+    DeviceBuffer<std::int32_t> rp, rc;
+    DeviceBuffer<double> rv64;
+    DeviceBuffer<float> rv32;
+    for (int q = 0; q < T; ++q) {
+        tissue_scale(vols64.get(), c.tags.get(), m, present[q], scale.get(), s);
+        stiffness_values(g64.get(), scale.get(), c.tets.get(), c.n2c_ptr.get(), c.n2c_idx.get(), n, indptr.get(),
+                         indices.get(), data.get(), s);
+        const std::int64_t nz = reduce_grounded(indptr.get(), indices.get(), data.get(), n, c.ground, rp, rc, rv64, rv32, s);
+        require(nz == c.nnz_red, "tissue operator pattern differs from the reduced operator");
+        d2d(u.vals.get() + static_cast<std::int64_t>(q) * c.nnz_red, rv64.get(), c.nnz_red, s);
+        split_offdiag(c.rptr.get(), c.rcol.get(), rv64.get(), nr, u.off.get() + static_cast<std::int64_t>(q) * nnz_off,
+                      u.rowsum.get() + static_cast<std::int64_t>(q) * nr, s);
+    }
+    // The placement's dA/dt (one column), then the RHS of each tissue with unit conductivity.
+    std::vector<double> tf(16);
+    std::vector<std::int32_t> degen(1);
+    place_block(c, center, handle, &dist, 1, tf.data(), degen.data());
+    require(handle == nullptr || degen[0] == 0, "degenerate coil placement (handle along the scalp normal)");
+    std::copy(tf.begin(), tf.end(), u.transform);
+    std::vector<float> dip(static_cast<std::size_t>(10 * n_dip));
+    std::vector<double> origin(3);
+    pack_dipoles(tf.data(), pos_m, mom, n_dip, dip.data(), origin.data());
+    auto ddip = upload(dip.data(), dip.size(), s, "dipoles");
+    auto dorg = upload(origin.data(), 3, s, "origin");
+    u.dadt_nodes1.reset(3 * n, "uq dadt nodes");
+    u.dadt_elm1.reset(3 * m, "uq dadt elements");
+    dadt_nbody(ddip.get(), static_cast<int>(n_dip), c.nodes.get(), n, dorg.get(), 1, 1,
+               static_cast<float>(didt) * static_cast<float>(kMu0Over4Pi), u.dadt_nodes1.get(), s);
+    DeviceBuffer<float> negvc(m, "tissue neg_vc"), q(4 * m, "corner terms"), elm(3 * m, "elm");
+    for (int t = 0; t < T; ++t) {
+        tissue_neg_vc(c.vols32.get(), c.tags.get(), m, present[t], negvc.get(), s);
+        element_pass(u.dadt_nodes1.get(), c.tets.get(), c.g32.get(), negvc.get(), m, 1, 1, u.dadt_elm1.get(), q.get(), s);
+        node_rhs(q.get(), c.n2c_ptr.get(), c.n2c_idx.get(), n, 1, 1, c.ground,
+                 u.rhs.get() + static_cast<std::int64_t>(t) * nr, nullptr, s);
+    }
+    u.sum.reset(m, "|E| sum");
+    u.sumsq.reset(m, "|E| sum of squares");
+    check(hipMemsetAsync(u.sum.get(), 0, sizeof(double) * m, s), "moments clear");
+    check(hipMemsetAsync(u.sumsq.get(), 0, sizeof(double) * m, s), "moments clear");
+    sync(s, "uq prepare");
+    u.ready = true;
+}
+
+// k <= 16 conductivity samples (sigma: k x T, T in the order of the tags uq_prepare found) of the
+// prepared placement, as K independent CG chains under the nominal preconditioner.
+void uq_solve(cn_context& c, const double* sigma, int k, const cn_solve_options* sopt, cn_block_outputs* out) {
+    using namespace cn;
+    auto& u = c.uq;
+    require(u.ready, "call cn_uq_prepare first");
+    require(k >= 1 && k <= 16, "k must be in [1, 16]");
+    const int T = static_cast<int>(u.tags.size());
+    for (int i = 0; i < k * T; ++i) require(std::isfinite(sigma[i]) && sigma[i] > 0.0, "conductivities must be positive");
+    const hipStream_t s = c.s;
+    const int K = padded_width(k);
+    const std::int64_t n = c.n, m = c.m, nr = n - 1, nnz_off = c.nnz_red - nr;
+    std::vector<double> sg(static_cast<std::size_t>(K) * T, 0.0);
+    std::copy(sigma, sigma + static_cast<std::size_t>(k) * T, sg.begin());
+    u.sigma.ensure(sg.size(), "uq sigma");
+    h2d(u.sigma.get(), sg.data(), sg.size(), s);
+    u.off_k.ensure(static_cast<std::size_t>(std::max<std::int64_t>(nnz_off, 1)) * K, "sample off-diagonals");
+    u.rowsum_k.ensure(static_cast<std::size_t>(nr) * K, "sample row sums");
+    c.d_B.ensure(nr * K, "rhs block");
+    combine_tissues(u.off.get(), nnz_off, T, u.sigma.get(), k, K, u.off_k.get(), s);
+    combine_tissues(u.rowsum.get(), nr, T, u.sigma.get(), k, K, u.rowsum_k.get(), s);
+    combine_tissues(u.rhs.get(), nr, T, u.sigma.get(), k, K, c.d_B.get(), s);
+    u.mop.width = K;
+    u.mop.off_vals = u.off_k.get();
+    u.mop.rowsum = u.rowsum_k.get();
+    u.mop.tissue_vals = u.vals.get();
+    u.mop.n_tissue = T;
+    u.mop.sigma = u.sigma.get();
+    SolveOptions opt;
+    if (sopt) {
+        opt.tolerance = sopt->tolerance;
+        opt.max_iters = sopt->max_iters;
+        opt.max_refine = sopt->max_refine;
+        opt.drop_tol = sopt->drop_tol;
+        opt.use_graph = sopt->use_graph != 0;
+        opt.check_every = std::max(1, sopt->check_every);
+    }
+    c.d_X.ensure(nr * K, "solution");
+    c.solver->set_multi(&u.mop);
+    SolveStats st;
+    try {
+        st = c.solver->solve(c.d_B.get(), c.d_X.get(), nullptr, k, opt);
+    } catch (...) {
+        c.solver->set_multi(nullptr);
+        throw;
+    }
+    c.solver->set_multi(nullptr);
+    // |E| per sample: dA/dt is the same for every sample.
+    c.d_dadt_elm.ensure(3 * m * K, "dadt elements");
+    broadcast_columns(u.dadt_elm1.get(), 3 * m, K, c.d_dadt_elm.get(), s);
+    c.d_magn.ensure(m * K, "magnE");
+    float* dE = nullptr;
+    if (out && out->E) {
+        c.d_E.ensure(3 * m * K, "E");
+        dE = c.d_E.get();
+    }
+    reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), c.d_dadt_elm.get(), m, k, K, c.ground, dE, c.d_magn.get(), s);
+    accumulate_moments(c.d_magn.get(), m, k, K, u.sum.get(), u.sumsq.get(), s);
+    u.count += k;
+    if (out) {
+        auto fetch = [&](float* host, const float* dev, std::int64_t rows, int inner) {
+            c.d_stage.ensure(rows * inner * k, "output staging");
+            unpack(dev, rows, inner, k, K, c.d_stage.get(), s);
+            d2h(host, c.d_stage.get(), rows * inner * k, s);
+            sync(s, "output copy");
+        };
+        if (out->transforms)
+            for (int i = 0; i < k; ++i) std::memcpy(out->transforms + 16 * i, u.transform, sizeof(double) * 16);
+        if (out->magnE) fetch(out->magnE, c.d_magn.get(), m, 1);
+        if (out->E) fetch(out->E, c.d_E.get(), m, 3);
+        if (out->v) {
+            c.d_v.ensure(n * k, "potential");
+            scatter_potential(c.d_X.get(), n, k, K, c.ground, c.d_v.get(), s);
+            d2h(out->v, c.d_v.get(), n * k, s);
+        }
+        if (out->metrics)
+            region_metrics(c.d_magn.get(), m, k, K, c.region_idx.get(), c.n_region, c.vols32.get(), c.bary.get(),
+                           out->metrics, s);
+        for (int i = 0; i < k; ++i) {
+            if (out->iterations) out->iterations[i] = st.column_iterations[i];
+            if (out->relative_residual) out->relative_residual[i] = st.relative_residual[i];
+        }
+        out->total_iterations = st.iterations;
+        out->refinements = st.refinements;
+    }
+    sync(s, "uq outputs");
+    for (int i = 0; i < k; ++i)
+        if (!(st.relative_residual[i] <= opt.tolerance))
+            throw NotConverged("PCG did not converge (column " + std::to_string(i) + " of the sample block)");
 }
 
 const cn::RecoveryOp* ensure_recovery(cn_context& c, int mode) {
@@ -1088,7 +1270,7 @@ int32_t cn_warm_modes(const cn_context* c, uint16_t* out, int32_t* width) {
         cn::require(c != nullptr && c->warm.ready, "warm-start modes not set");
         if (width) *width = c->warm.n_modes;
         if (out) {
-            cn::d2h(out, c->warm.modes.get(), static_cast<std::size_t>(c->n - 1) * c->warm.n_modes, c->s);
+            cn::d2h(out, c->warm.modes.get(), static_cast<std::size_t>(2) * (c->n - 1) * c->warm.n_modes, c->s);
             cn::sync(c->s, "modes copy");
         }
     });
@@ -1172,6 +1354,44 @@ int32_t cn_recovery_patches(const cn_context* c, int64_t* nnz, int32_t* n_undete
         if (pidx) cn::d2h(pidx, op.pidx.get(), op.nnz_patch, c->s);
         if (w3) cn::d2h(w3, op.w3.get(), 3 * op.nnz_patch, c->s);
         cn::sync(c->s, "patch copy");
+    });
+}
+
+int32_t cn_uq_prepare(cn_context* c, const double* pos_m, const double* mom, int64_t n_dip, const double* center,
+                      const double* handle, double distance, double didt, int32_t* n_tissue, int32_t* tags) {
+    return guarded([&] {
+        cn::require(c != nullptr && pos_m != nullptr && mom != nullptr && center != nullptr && n_dip >= 1,
+                    "bad argument");
+        uq_prepare(*c, pos_m, mom, n_dip, center, handle, distance, didt);
+        if (n_tissue) *n_tissue = static_cast<int32_t>(c->uq.tags.size());
+        if (tags)
+            for (std::size_t i = 0; i < c->uq.tags.size(); ++i) tags[i] = c->uq.tags[i];
+    });
+}
+
+int32_t cn_uq_solve(cn_context* c, const double* sigma, int32_t k, const cn_solve_options* opt, cn_block_outputs* out) {
+    return guarded([&] {
+        cn::require(c != nullptr && sigma != nullptr, "null argument");
+        uq_solve(*c, sigma, k, opt, out);
+    });
+}
+
+int32_t cn_uq_moments(const cn_context* c, double* mean, double* var, int64_t* count) {
+    return guarded([&] {
+        cn::require(c != nullptr && c->uq.ready, "call cn_uq_prepare first");
+        const std::int64_t m = c->m, cnt = c->uq.count;
+        if (count) *count = cnt;
+        if (!mean && !var) return;
+        cn::require(cnt >= 1, "no samples solved yet");
+        std::vector<double> s1(m), s2(m);
+        cn::d2h(s1.data(), c->uq.sum.get(), m, c->s);
+        cn::d2h(s2.data(), c->uq.sumsq.get(), m, c->s);
+        cn::sync(c->s, "moments copy");
+        for (std::int64_t e = 0; e < m; ++e) {
+            const double mu = s1[e] / static_cast<double>(cnt);
+            if (mean) mean[e] = mu;
+            if (var) var[e] = cnt > 1 ? std::max(0.0, (s2[e] - static_cast<double>(cnt) * mu * mu) / (cnt - 1)) : 0.0;
+        }
     });
 }
 

@@ -171,6 +171,46 @@ __device__ __forceinline__ void row_gather_v(const std::int32_t* __restrict__ ci
     }
 }
 
+// The same over K matrices that share one pattern: column c reads its own value av[j * K + c].
+template <int K, int V, int SUB, int U = 4>
+__device__ __forceinline__ void row_gather_multi(const std::int32_t* __restrict__ ci, const float* __restrict__ av,
+                                                 int j, int e, const float* __restrict__ X, int c0,
+                                                 const float (&xi)[V], float (&acc)[V]) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = 0.0f;
+    for (; j < e; j += U * SUB) {
+        std::int64_t cc[U];
+        float aa[U][V], xx[U][V];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (j + u * SUB < e) {
+                cc[u] = ci[j + u * SUB];
+                vload<V>(av + static_cast<std::int64_t>(j + u * SUB) * K + c0, aa[u]);
+            } else {
+                cc[u] = 0;
+#pragma unroll
+                for (int v = 0; v < V; ++v) aa[u][v] = 0.0f;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (j + u * SUB < e) {
+                vload<V>(X + cc[u] * K + c0, xx[u]);
+            } else {
+#pragma unroll
+                for (int v = 0; v < V; ++v) xx[u][v] = 0.0f;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (j + u * SUB < e) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) acc[v] = fmaf(aa[u][v], xx[u][v] - xi[v], acc[v]);
+            }
+        }
+    }
+}
+
 constexpr int kRowThreads = 256;
 constexpr int kRowWaves = kRowThreads / kWave;
 constexpr int kRowIters = 4;  // row groups each wave walks per block
@@ -180,7 +220,7 @@ constexpr std::int64_t rows_per_vblock() {
     return static_cast<std::int64_t>(kRowWaves) * (kWave / ((K / vec_width<K>()) * SUB)) * kRowIters;
 }
 
-enum RowMode : int { kResidual = 0, kCheby = 1, kSpmm = 2, kSpmmDiff = 3 };
+enum RowMode : int { kResidual = 0, kCheby = 1, kSpmm = 2, kSpmmDiff = 3, kSpmmDiffMulti = 4 };
 
 // One kernel for every SpMV-shaped pass:
 //   kResidual  out = b - A x
@@ -188,6 +228,8 @@ enum RowMode : int { kResidual = 0, kCheby = 1, kSpmm = 2, kSpmmDiff = 3 };
 //   kSpmm      out = A x
 //   kSpmmDiff  out = sum_{j != i} a_ij (x_j - x_i) + s_i x_i over the differenced operator, with
 //              `diag` holding the row sums s_i
+//   kSpmmDiffMulti  the same for K matrices sharing the pattern: values (nnz, K) and row sums
+//              (n, K), column c of x multiplied by matrix c
 template <int K, int SUB, int MODE>
 __global__ void __launch_bounds__(kRowThreads) rowk_kernel(
     std::int64_t n, const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
@@ -212,11 +254,14 @@ __global__ void __launch_bounds__(kRowThreads) rowk_kernel(
         float xi[V], acc[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) xi[v] = 0.0f;
-        if constexpr (MODE == kSpmmDiff) {
+        if constexpr (MODE == kSpmmDiff || MODE == kSpmmDiffMulti) {
             if (valid) vload<V>(x + at, xi);
         }
         if (valid) {
-            row_gather_v<K, V, SUB, MODE == kSpmmDiff>(ci, av, rp[row] + sub, rp[row + 1], x, c0, xi, acc);
+            if constexpr (MODE == kSpmmDiffMulti)
+                row_gather_multi<K, V, SUB>(ci, av, rp[row] + sub, rp[row + 1], x, c0, xi, acc);
+            else
+                row_gather_v<K, V, SUB, MODE == kSpmmDiff>(ci, av, rp[row] + sub, rp[row + 1], x, c0, xi, acc);
         } else {
 #pragma unroll
             for (int v = 0; v < V; ++v) acc[v] = 0.0f;
@@ -235,6 +280,11 @@ __global__ void __launch_bounds__(kRowThreads) rowk_kernel(
             const float s = diag[row];
 #pragma unroll
             for (int v = 0; v < V; ++v) o[v] = fmaf(s, xi[v], acc[v]);
+        } else if constexpr (MODE == kSpmmDiffMulti) {
+            float sv[V];
+            vload<V>(diag + at, sv);
+#pragma unroll
+            for (int v = 0; v < V; ++v) o[v] = fmaf(sv[v], xi[v], acc[v]);
         } else {
             float bb[V];
             vload<V>(b + at, bb);
@@ -347,6 +397,49 @@ __global__ void __launch_bounds__(kGramThreads) residual64_kernel(
             const int e = rp[row + 1];
             for (int j = rp[row] + sub; j < e; j += SUB)
                 acc = fma(av[j], X[static_cast<std::int64_t>(ci[j]) * K + c], acc);
+        }
+#pragma unroll
+        for (int off = K; off < L; off <<= 1) acc += __shfl_xor(acc, off, kWave);
+        if (valid && sub == 0) {
+            const double r = static_cast<double>(B[row * K + c]) - acc;
+            R[row * K + c] = r;
+            nrm = fma(r, r, nrm);
+        }
+    }
+    nrm = wave_rows_sum<L>(nrm);
+    block_vec_store<K, kGramWaves>(nrm, partials + static_cast<std::int64_t>(lb) * K);
+}
+
+// The FP64 true residual for K matrices A_c = sum_t sigma[c, t] A_t sharing one pattern: the
+// per-tissue FP64 values tv (T, nnz) are combined per entry in tissue order.
+template <int K>
+__global__ void __launch_bounds__(kGramThreads) residual64_multi_kernel(
+    std::int64_t n, const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
+    const double* __restrict__ tv, std::int64_t nnz, int T, const double* __restrict__ sigma,
+    const float* __restrict__ B, const double* __restrict__ X, double* __restrict__ R, double* __restrict__ partials,
+    std::int64_t rows_per_block, bool x_zero) {
+    constexpr int L = spmm_lanes<K>();
+    constexpr int SUB = L / K;
+    constexpr int RPW = kWave / L;
+    const int lb = logical_block();
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
+    const int l = lane % L, c = l % K, sub = l / K;
+    const std::int64_t lo = static_cast<std::int64_t>(lb) * rows_per_block;
+    const std::int64_t hi = min(lo + rows_per_block, n);
+    double sg[kMaxTissues];
+    for (int t = 0; t < kMaxTissues; ++t) sg[t] = t < T ? sigma[c * T + t] : 0.0;
+    double nrm = 0.0;
+    for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW; r0 < hi; r0 += kGramWaves * RPW) {
+        const std::int64_t row = r0 + lane / L;
+        const bool valid = row < hi;
+        double acc = 0.0;
+        if (valid && !x_zero) {
+            const int e = rp[row + 1];
+            for (int j = rp[row] + sub; j < e; j += SUB) {
+                double a = 0.0;
+                for (int t = 0; t < T; ++t) a = fma(sg[t], tv[static_cast<std::int64_t>(t) * nnz + j], a);
+                acc = fma(a, X[static_cast<std::int64_t>(ci[j]) * K + c], acc);
+            }
         }
 #pragma unroll
         for (int off = K; off < L; off <<= 1) acc += __shfl_xor(acc, off, kWave);
@@ -641,7 +734,8 @@ template <int K>
 __global__ void __launch_bounds__(kSmallThreads) coef_kernel(const double* __restrict__ partials, int nb,
                                                              double drop_tol, double* __restrict__ G_out,
                                                              double* __restrict__ T, double* __restrict__ TT,
-                                                             double* __restrict__ AL, int* __restrict__ state) {
+                                                             double* __restrict__ AL, int* __restrict__ state,
+                                                             int independent) {
     if (finished(state)) return;
     __shared__ double A[K * K];   // scaled G, then its Schur complements; then Lp
     __shared__ double H[K * K];
@@ -657,6 +751,13 @@ __global__ void __launch_bounds__(kSmallThreads) coef_kernel(const double* __res
     reduce_partials<K * K>(partials, nb, 2 * K * K, 0, A);
     reduce_partials<K * K>(partials, nb, 2 * K * K, K * K, H);
     const int t = threadIdx.x;
+    if (independent) {  // K separate systems: each column is its own CG chain
+        if (t < K * K && (t / K) != (t % K)) {
+            A[t] = 0.0;
+            H[t] = 0.0;
+        }
+        __syncthreads();
+    }
     if (t < K * K) G_out[t] = A[t];
     if (t < K) {
         const double d = A[t * K + t];
@@ -748,11 +849,15 @@ __global__ void __launch_bounds__(kSmallThreads) coef_kernel(const double* __res
 template <int K>
 __global__ void __launch_bounds__(kSmallThreads) dmat_kernel(const double* __restrict__ partials, int nb,
                                                              const double* __restrict__ TT, double* __restrict__ D,
-                                                             int* __restrict__ state) {
+                                                             int* __restrict__ state, int independent) {
     if (finished(state)) return;
     __shared__ double E[K * K];
     reduce_partials<K * K>(partials, nb, K * K, 0, E);
     const int t = threadIdx.x;
+    if (independent) {
+        if (t < K * K && (t / K) != (t % K)) E[t] = 0.0;
+        __syncthreads();
+    }
     if (t < K * K) {
         const int i = t / K, c = t % K;
         double s = 0.0;
@@ -945,6 +1050,12 @@ BlockSolver::~BlockSolver() {
     if (host_state_) (void)hipHostFree(host_state_);
 }
 
+void BlockSolver::set_multi(const MultiOperator* m) {
+    if (m) require(m->n_tissue >= 1 && m->n_tissue <= kMaxTissues, "tissue count must be in [1, 16]");
+    multi_ = m;
+    drop_graphs();  // captured iterations name the operator's kernels and buffers
+}
+
 void BlockSolver::drop_graphs() {
     for (auto& g : graphs_) {
         if (g) (void)hipGraphExecDestroy(g);
@@ -1005,8 +1116,16 @@ int BlockSolver::fine_sub() const {
 template <int K>
 void BlockSolver::spmm_block_k(const int* state) {
     const std::int64_t nn = n();
-    launch_rows<K, kSpmmDiff>(fine_sub<K>(), op_.off, op_.rowsum, nullptr, W_.get(), nullptr, 0.f, 0.f, 0.f,
-                              S_.get(), state, s_);
+    if (multi_) {
+        require(multi_->width == K, "multi-matrix values were prepared for another block width");
+        CsrView off = op_.off;
+        off.val = multi_->off_vals;
+        launch_rows<K, kSpmmDiffMulti>(fine_sub<K>(), off, multi_->rowsum, nullptr, W_.get(), nullptr, 0.f, 0.f, 0.f,
+                                       S_.get(), state, s_);
+    } else {
+        launch_rows<K, kSpmmDiff>(fine_sub<K>(), op_.off, op_.rowsum, nullptr, W_.get(), nullptr, 0.f, 0.f, 0.f,
+                                  S_.get(), state, s_);
+    }
     gram_rows_kernel<K, true><<<kGramBlocks, gram_threads<K, true>(), 0, s_>>>(nn, W_.get(), S_.get(), R_.get(),
                                                                         partials_.get(), gram_rows(nn), state);
 }
@@ -1019,14 +1138,15 @@ void BlockSolver::iteration_k(const SolveOptions& opt) {
     int* stw = state_.get();
     spmm_block_k<K>(st);
     coef_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, opt.drop_tol, G_.get(), T_.get(),
-                                                TT_.get(), AL_.get(), stw);
+                                                TT_.get(), AL_.get(), stw, multi_ ? 1 : 0);
     update_xr_kernel<K><<<kGramBlocks, xr_threads<K>(), 0, s_>>>(nn, W_.get(), S_.get(), AL_.get(), X64_.get(), R_.get(),
                                                              partials_.get(), rpb, st);
     converge_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, target_.get(), norms_.get(), stw);
     vcycle_k<K>(R_.get(), Z_.get(), st);
     gram_rows_kernel<K, false><<<kGramBlocks, gram_threads<K, false>(), 0, s_>>>(nn, S_.get(), Z_.get(), nullptr,
                                                                          partials_.get(), rpb, st);
-    dmat_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, TT_.get(), D_.get(), stw);
+    dmat_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, TT_.get(), D_.get(), stw,
+                                                multi_ ? 1 : 0);
     update_w_kernel<K><<<grid_for(nn * col_groups<K>(), kRowThreads), kRowThreads, 0, s_>>>(nn, Z_.get(), D_.get(),
                                                                                            W_.get(), st);
 }
@@ -1044,8 +1164,13 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
 
     std::vector<double> bnorm(K), rnorm(K);
     auto true_residual = [&](bool zero) {
-        residual64_kernel<K><<<kGramBlocks, kGramThreads, 0, s_>>>(nn, op_.a32.ptr, op_.a32.col, op_.val64, B, Xi,
-                                                                   R64_.get(), partials_.get(), rpb, zero);
+        if (multi_)
+            residual64_multi_kernel<K><<<kGramBlocks, kGramThreads, 0, s_>>>(
+                nn, op_.a32.ptr, op_.a32.col, multi_->tissue_vals, op_.a32.nnz, multi_->n_tissue, multi_->sigma, B, Xi,
+                R64_.get(), partials_.get(), rpb, zero);
+        else
+            residual64_kernel<K><<<kGramBlocks, kGramThreads, 0, s_>>>(nn, op_.a32.ptr, op_.a32.col, op_.val64, B, Xi,
+                                                                       R64_.get(), partials_.get(), rpb, zero);
         column_norms_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, norms_.get());
         d2h(rnorm.data(), norms_.get(), K, s_);
         sync(s_, "true residual");

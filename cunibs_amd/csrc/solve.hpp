@@ -52,6 +52,20 @@ struct Operator {
     const double* val64 = nullptr;
 };
 
+constexpr int kMaxTissues = 16;
+
+// K matrices sharing the reduced operator's pattern (conductivity samples): column c of every
+// block is solved with A_c = sum_t sigma[c, t] A_t, as K independent CG chains (the block Grams
+// restricted to their diagonals) under the shared preconditioner.
+struct MultiOperator {
+    int width = 0;                     // K the values below were laid out for
+    const float* off_vals = nullptr;   // (nnz_off, K) differenced off-diagonal values
+    const float* rowsum = nullptr;     // (n, K) row sums
+    const double* tissue_vals = nullptr;  // (T, nnz) FP64 per-tissue values on the reduced pattern
+    int n_tissue = 0;
+    const double* sigma = nullptr;     // (K, T) device
+};
+
 class BlockSolver {
   public:
     BlockSolver(const Operator& op, const Hierarchy& h, hipStream_t s);
@@ -67,6 +81,8 @@ class BlockSolver {
     void spmm(const float* X, float* Y, int K);     // Y = A X (FP32, plain CSR)
     void spmm_diff(const float* X, float* Y, int K);  // Y = A X (FP32, differenced form)
     void vcycle(const float* B, float* X, int K);   // X = M B, one zero-guess V-cycle
+    // Multi-matrix mode (nullptr: the single operator). The pointed-to values must outlive the solves.
+    void set_multi(const MultiOperator* m);
     // out (32 x 32, FP64, host) = A^T B for (n, 32) FP32 blocks, fixed-order reduction.
     void cross_gram32(const float* A, const float* B, double* out_host);
     // Average device time (ms) over `reps` launches, HIP events on the solver's stream.
@@ -96,6 +112,7 @@ class BlockSolver {
     int cap_k_ = 0;
     int n_cu_ = 256;
     int fine_sub_override_ = 0;  // bench_spmm's forced sub-lane count while it runs
+    const MultiOperator* multi_ = nullptr;
     // block vectors (n, K)
     DeviceBuffer<float> R_, Z_, W_, S_;
     DeviceBuffer<double> R64_, X64_;

@@ -1,0 +1,24 @@
+// Conductivity-uncertainty kernels (uq.hip).
+#pragma once
+#include "kernels.hpp"
+
+namespace cn {
+
+// scale[e] = vols[e] if tags[e] == tag else 0 (FP64): the unit-conductivity stiffness of one tissue.
+void tissue_scale(const double* vols, const std::int32_t* tags, std::int64_t m, int tag, double* scale, hipStream_t s);
+// neg_vc[e] = -vols[e] if tags[e] == tag else 0 (FP32): the unit-conductivity RHS weights of one tissue.
+void tissue_neg_vc(const float* vols, const std::int32_t* tags, std::int64_t m, int tag, float* neg_vc, hipStream_t s);
+// Off-diagonal values (row order, FP64) and FP64 row sums of a reduced CSR.
+void split_offdiag(const std::int32_t* rp, const std::int32_t* ci, const double* v, std::int64_t n, double* off,
+                   double* rowsum, hipStream_t s);
+// out[i * K + c] = (float) sum_t sigma[c * T + t] in[t * count + i] for c < k, 0 for k <= c < K.
+void combine_tissues(const double* in, std::int64_t count, int T, const double* sigma, int k, int K, float* out,
+                     hipStream_t s);
+void combine_tissues(const float* in, std::int64_t count, int T, const double* sigma, int k, int K, float* out,
+                     hipStream_t s);
+// (rows, inner, 1) -> (rows, inner, K), every column a copy.
+void broadcast_columns(const float* in, std::int64_t rows_inner, int K, float* out, hipStream_t s);
+// sum[e] += x[e, c], sumsq[e] += x[e, c]^2 over c < k, FP64, columns in order.
+void accumulate_moments(const float* x, std::int64_t m, int k, int K, double* sum, double* sumsq, hipStream_t s);
+
+}  // namespace cn

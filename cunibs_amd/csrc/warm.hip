@@ -1,14 +1,17 @@
 // Reduced-basis warm start for the block PCG.
 //
-// Snapshots W (n_red x S, S <= 512): solutions the solver itself produced, BF16, row-major (row r
+// Snapshots W (n_red x S, S <= 512): solutions the solver itself produced, FP32, row-major (row r
 // holds the S values of unknown r). From the Galerkin matrix G = W^T A W (FP64) the caller takes
 // the leading eigenpairs and hands back M = V_R Lambda_R^(-1/2); the modes
-//   Phi = W M          (n_red x R, R <= 256, BF16; an MFMA GEMM with M as BF16 hi + lo halves)
-// are then A-orthonormal up to their BF16 rounding, which the caller removes with the small
-// projector P = (Phi^T A Phi)^-1 (FP64, close to the identity). A block then starts from
+//   Phi = W M          (n_red x R, R <= 256; an MFMA GEMM with W and M as BF16 hi + lo halves)
+// are stored as BF16 hi + lo pairs (16 significant bits): a basis rounded to BF16 alone carries
+// high-frequency noise whose residual A e, amplified by the operator's spread, is as large as b
+// itself, so the pair is what makes the guess worth starting from. The modes are A-orthonormal up
+// to their rounding, which the small projector P = (Phi^T A Phi)^-1 (FP64, close to the identity)
+// removes. A block then starts from
 //   x0 = Phi c,   c = P Phi^T B   (Galerkin projection: the A-norm best point of span(Phi))
 //   or c supplied by the caller  (e.g. predicted by the warm-start network from the pose),
-// the expansion being a BF16 MFMA GEMM with FP32 accumulation and c as BF16 hi + lo halves. PCG
+// the expansion being a BF16 MFMA GEMM with FP32 accumulation (hi.hi + hi.lo + lo.hi). PCG
 // then iterates from x0 to the same tolerance against ||b||, so only the iteration count changes.
 //
 // No reference counterpart: the reference's closest analogue is the Galerkin-projected initial
@@ -31,30 +34,45 @@ __device__ __forceinline__ unsigned short bf16_bits(float x) {
 
 __device__ __forceinline__ float bf16_value(unsigned short b) { return __uint_as_float(static_cast<unsigned>(b) << 16); }
 
-// basis[r * S + (s0 + c)] = bf16(X[r * K + c]) for c < k: append a block's solutions as columns.
+// basis[r * S + (s0 + c)] = X[r * K + c] (FP32) for c < k: append a block's solutions as columns.
 __global__ void append_kernel(const double* __restrict__ X, std::int64_t n, int k, int K, int S, int s0,
-                              unsigned short* __restrict__ basis) {
+                              float* __restrict__ basis) {
     const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= n * k) return;
     const std::int64_t r = t / k;
     const int c = static_cast<int>(t % k);
-    basis[r * S + s0 + c] = bf16_bits(static_cast<float>(X[r * K + c]));
+    basis[r * S + s0 + c] = static_cast<float>(X[r * K + c]);
+}
+
+// A BF16 hi + lo pair: value = hi + lo.
+__device__ __forceinline__ float pair_value(const unsigned short* __restrict__ hi, std::int64_t at, std::int64_t lo_off) {
+    return bf16_value(hi[at]) + bf16_value(hi[lo_off + at]);
 }
 
 // out (n, 32) FP32 = columns [s0, s0 + 32) of the basis (zero beyond S).
-__global__ void gather_chunk_kernel(const unsigned short* __restrict__ basis, std::int64_t n, int S, int s0,
+__global__ void gather_chunk_kernel(const float* __restrict__ basis, std::int64_t n, int S, int s0,
                                     float* __restrict__ out) {
     const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= n * 32) return;
     const std::int64_t r = t >> 5;
     const int c = static_cast<int>(t & 31);
-    out[t] = (s0 + c < S) ? bf16_value(basis[r * S + s0 + c]) : 0.0f;
+    out[t] = (s0 + c < S) ? basis[r * S + s0 + c] : 0.0f;
+}
+
+// The same over the modes' hi + lo pairs (row stride R, lo half at offset n * R).
+__global__ void gather_modes_kernel(const unsigned short* __restrict__ modes, std::int64_t n, int R, int j0,
+                                    float* __restrict__ out) {
+    const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n * 32) return;
+    const std::int64_t r = t >> 5;
+    const int c = static_cast<int>(t & 31);
+    out[t] = (j0 + c < R) ? pair_value(modes, r * R + j0 + c, n * R) : 0.0f;
 }
 
 // Partials of W^T B: one thread per basis column s (blockDim = S rounded up to a wave multiple),
 // the block walks its row range; B rows are read whole by every thread (a broadcast).
 template <int K>
-__global__ void __launch_bounds__(512) project_kernel(const unsigned short* __restrict__ basis, std::int64_t n, int S,
+__global__ void __launch_bounds__(512) project_kernel(const unsigned short* __restrict__ modes, std::int64_t n, int S,
                                                       const float* __restrict__ B, std::int64_t rows_per_block,
                                                       double* __restrict__ partials) {
     const int s = threadIdx.x;
@@ -65,7 +83,7 @@ __global__ void __launch_bounds__(512) project_kernel(const unsigned short* __re
     for (int c = 0; c < K; ++c) acc[c] = 0.0;
     if (s < S) {
         for (std::int64_t r = lo; r < hi; ++r) {
-            const double w = static_cast<double>(bf16_value(basis[r * S + s]));
+            const double w = static_cast<double>(pair_value(modes, r * S + s, n * S));
 #pragma unroll
             for (int c = 0; c < K; ++c) acc[c] = fma(w, static_cast<double>(B[r * K + c]), acc[c]);
         }
@@ -105,11 +123,11 @@ __global__ void coefficients_kernel(const double* __restrict__ partials, int nb,
 typedef __attribute__((__vector_size__(4 * sizeof(short)))) short bf16x4_t;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4_t;
 
-// x0 (n, K) FP64 = W c: one wavefront per 16-row tile, v_mfma_f32_16x16x16bf16_1k over the basis
-// in 16-column steps, the high and low coefficient halves accumulated into the same FP32 tile.
+// x0 (n, K) FP64 = Phi c: one wavefront per 16-row tile, v_mfma_f32_16x16x16bf16_1k over the modes
+// in 16-column steps; hi.hi, hi.lo and lo.hi products accumulate into the same FP32 tile.
 // Operand layout of the 16x16x16 BF16 MFMA (CDNA3/4 ISA): lane l supplies A[l % 16][4 (l / 16) + j]
 // and B[4 (l / 16) + j][l % 16], j = 0..3, and receives D[4 (l / 16) + j][l % 16].
-__global__ void __launch_bounds__(256) expand_kernel(const unsigned short* __restrict__ basis, std::int64_t n, int S,
+__global__ void __launch_bounds__(256) expand_kernel(const unsigned short* __restrict__ modes, std::int64_t n, int S,
                                                      int S16, const unsigned short* __restrict__ coef, int k, int K,
                                                      double* __restrict__ x0) {
     const int lane = threadIdx.x & 63;
@@ -120,11 +138,13 @@ __global__ void __launch_bounds__(256) expand_kernel(const unsigned short* __res
     const std::int64_t row = r0 + i;
     floatx4_t acc = {0.f, 0.f, 0.f, 0.f};
     for (int s0 = 0; s0 < S16; s0 += 16) {
-        bf16x4_t a;
+        bf16x4_t a, alo;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int s = s0 + 4 * q + j;
-            a[j] = (row < n && s < S) ? static_cast<short>(basis[row * S + s]) : static_cast<short>(0);
+            const bool ok = row < n && s < S;
+            a[j] = ok ? static_cast<short>(modes[row * S + s]) : static_cast<short>(0);
+            alo[j] = ok ? static_cast<short>(modes[n * S + row * S + s]) : static_cast<short>(0);
         }
         bf16x4_t bh, bl;
 #pragma unroll
@@ -135,6 +155,7 @@ __global__ void __launch_bounds__(256) expand_kernel(const unsigned short* __res
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bh, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo, bh, acc, 0, 0, 0);
     }
     // acc[j] = x0[r0 + 4 q + j][i]
     if (i < K) {
@@ -146,9 +167,10 @@ __global__ void __launch_bounds__(256) expand_kernel(const unsigned short* __res
     }
 }
 
-// Modes (n, R) BF16 = W (n, S) M (S, R): one wavefront per 16 x 16 output tile, S stepped by 16,
-// M carried as BF16 hi/lo halves mhl[(half * S16 + s) * R + j] (zero rows past S).
-__global__ void __launch_bounds__(256) modes_kernel(const unsigned short* __restrict__ basis, std::int64_t n, int S,
+// Modes (n, R) = W (n, S) M (S, R): one wavefront per 16 x 16 output tile, S stepped by 16, W split
+// into BF16 hi/lo on load, M carried as BF16 hi/lo halves mhl[(half * S16 + s) * R + j] (zero rows
+// past S); the FP32 result is stored as a BF16 hi + lo pair (lo half at offset n * R).
+__global__ void __launch_bounds__(256) modes_kernel(const float* __restrict__ basis, std::int64_t n, int S,
                                                    int S16, const unsigned short* __restrict__ mhl, int R,
                                                    unsigned short* __restrict__ modes) {
     const int lane = threadIdx.x & 63;
@@ -161,21 +183,29 @@ __global__ void __launch_bounds__(256) modes_kernel(const unsigned short* __rest
     const std::int64_t row = r0 + i;
     floatx4_t acc = {0.f, 0.f, 0.f, 0.f};
     for (int s0 = 0; s0 < S16; s0 += 16) {
-        bf16x4_t a, bh, bl;
+        bf16x4_t a, alo, bh, bl;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int sidx = s0 + 4 * q + j;
-            a[j] = (row < n && sidx < S) ? static_cast<short>(basis[row * S + sidx]) : static_cast<short>(0);
+            const float w = (row < n && sidx < S) ? basis[row * S + sidx] : 0.0f;
+            const unsigned short wh = bf16_bits(w);
+            a[j] = static_cast<short>(wh);
+            alo[j] = static_cast<short>(bf16_bits(w - bf16_value(wh)));
             bh[j] = static_cast<short>(mhl[static_cast<std::int64_t>(sidx) * R + j0 + i]);
             bl[j] = static_cast<short>(mhl[static_cast<std::int64_t>(S16 + sidx) * R + j0 + i]);
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bh, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo, bh, acc, 0, 0, 0);
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const std::int64_t r = r0 + 4 * q + j;
-        if (r < n) modes[r * R + j0 + i] = bf16_bits(acc[j]);
+        if (r < n) {
+            const unsigned short hi = bf16_bits(acc[j]);
+            modes[r * R + j0 + i] = hi;
+            modes[n * R + r * R + j0 + i] = bf16_bits(acc[j] - bf16_value(hi));
+        }
     }
 }
 
@@ -190,7 +220,7 @@ void WarmBasis::reset(std::int64_t n_rows, int capacity) {
     n_modes = used_modes = 0;
     modes.reset(0);
     basis.reset(static_cast<std::size_t>(n) * cap, "warm basis");
-    check(hipMemset(basis.get(), 0, sizeof(unsigned short) * n * cap), "warm basis clear");
+    check(hipMemset(basis.get(), 0, sizeof(float) * n * cap), "warm basis clear");
 }
 
 int WarmBasis::append(const double* X, int k, int K, hipStream_t s) {
@@ -238,7 +268,7 @@ void WarmBasis::set_modes(const double* M_host, int S, int R, hipStream_t s) {
     DeviceBuffer<unsigned short> dm(mhl.size(), "mode matrix");
     h2d(dm.get(), mhl.data(), mhl.size(), s);
     n_modes = R16;
-    modes.reset(static_cast<std::size_t>(n) * R16, "warm modes");
+    modes.reset(static_cast<std::size_t>(2) * n * R16, "warm modes");
     const std::int64_t tiles = (n + 15) / 16 * (R16 / 16);
     modes_kernel<<<static_cast<unsigned>((tiles + 3) / 4), 256, 0, s>>>(basis.get(), n, cap, S16, dm.get(), R16,
                                                                        modes.get());
@@ -262,7 +292,7 @@ void WarmBasis::set_projector(const double* P_host, int R, hipStream_t s) {
 }
 
 void WarmBasis::mode_chunk(int j0, float* out, hipStream_t s) const {
-    if (const unsigned g = grid_for(n * 32)) gather_chunk_kernel<<<g, kBlock, 0, s>>>(modes.get(), n, n_modes, j0, out);
+    if (const unsigned g = grid_for(n * 32)) gather_modes_kernel<<<g, kBlock, 0, s>>>(modes.get(), n, n_modes, j0, out);
 }
 
 void WarmBasis::expand(int k, int K, double* x0, hipStream_t s) {

@@ -17,8 +17,8 @@ struct WarmBasis {
     bool ready = false;  // a projector matching `count` has been set
     int n_modes = 0;     // mode buffer width (a multiple of 16)
     int used_modes = 0;  // modes set by the caller
-    DeviceBuffer<unsigned short> basis;  // (n, cap) BF16 bits, row-major
-    DeviceBuffer<unsigned short> modes;  // (n, n_modes) BF16 bits, row-major
+    DeviceBuffer<float> basis;           // (n, cap) FP32, row-major
+    DeviceBuffer<unsigned short> modes;  // (2, n, n_modes) BF16 hi then lo halves, row-major
     DeviceBuffer<double> P, partials;    // P: (n_modes, n_modes), identity-padded
     DeviceBuffer<unsigned short> coef;
 

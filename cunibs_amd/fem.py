@@ -341,12 +341,62 @@ class SolverContext:
               "cn_warm_set_coefficients")
 
     def warm_modes(self) -> np.ndarray:
-        """The modes as FP32 values of their BF16 bits, (n_reduced, width)."""
+        """The modes (n_reduced, width): hi + lo of their BF16 pairs, in FP64."""
         w = ctypes.c_int32()
         check(lib().cn_warm_modes(self.handle, None, ctypes.byref(w)), "cn_warm_modes")
-        bits = np.empty((self.n_nodes - 1, w.value), np.uint16)
+        bits = np.empty((2, self.n_nodes - 1, w.value), np.uint16)
         check(lib().cn_warm_modes(self.handle, ptr(bits), ctypes.byref(w)), "cn_warm_modes")
-        return (bits.astype(np.uint32) << 16).view(np.float32)
+        halves = (bits.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+        return halves[0] + halves[1]
+
+    # ---- conductivity uncertainty ----
+    def uq_prepare(self, dip_pos_m, dip_moment, site, didt: float) -> list[int]:
+        """Per-tissue operators and RHS of one placement; returns the tissue tags (the column order
+        of every sigma passed to uq_solve)."""
+        pos = np.ascontiguousarray(dip_pos_m, np.float64).reshape(-1, 3)
+        mom = np.ascontiguousarray(dip_moment, np.float64).reshape(-1, 3)
+        c = np.ascontiguousarray(site[0], np.float64).reshape(3)
+        h = None if site[1] is None else np.ascontiguousarray(site[1], np.float64).reshape(3)
+        nt = ctypes.c_int32()
+        tags = np.zeros(16, np.int32)
+        check(lib().cn_uq_prepare(self.handle, ptr(pos), ptr(mom), pos.shape[0], ptr(c), ptr(h), float(site[2]),
+                                  float(didt), ctypes.byref(nt), ptr(tags)), "cn_uq_prepare")
+        self.uq_tags = [int(t) for t in tags[:nt.value]]
+        return self.uq_tags
+
+    def uq_solve(self, sigma: np.ndarray, *, magnitude: bool = True, vectors: bool = False, potential: bool = False,
+                 metrics: bool = True, tolerance: float = DEFAULT_TOLERANCE,
+                 max_iters: int = DEFAULT_MAX_ITERS) -> list[PlacementResult]:
+        """k <= 16 conductivity samples (k x n_tissue, columns in uq_tags order)."""
+        sg = np.ascontiguousarray(sigma, np.float64)
+        if sg.ndim != 2 or sg.shape[1] != len(self.uq_tags):
+            raise ValueError("sigma must be (k, n_tissue) in the order uq_prepare returned")
+        k, m, n = sg.shape[0], self.n_tets, self.n_nodes
+        out = _lib.BlockOutputs()
+        tf = np.empty((k, 4, 4), np.float64)
+        it = np.empty(k, np.int32)
+        rel = np.empty(k, np.float64)
+        mag = np.empty((k, m), np.float32) if magnitude else None
+        E = np.empty((k, m, 3), np.float32) if vectors else None
+        v = np.empty((k, n), np.float64) if potential else None
+        met = np.empty((k, _lib.METRICS_LEN), np.float64) if metrics else None
+        out.transforms, out.iterations, out.relative_residual = ptr(tf), ptr(it), ptr(rel)
+        out.magnE, out.E, out.v, out.metrics = ptr(mag), ptr(E), ptr(v), ptr(met)
+        opt = _lib.default_solve_options()
+        opt.tolerance = float(tolerance)
+        opt.max_iters = int(max_iters)
+        check(lib().cn_uq_solve(self.handle, ptr(sg), k, ctypes.byref(opt), ctypes.byref(out)), "cn_uq_solve")
+        return [PlacementResult(transform=tf[i], magnE=None if mag is None else mag[i],
+                                E=None if E is None else E[i], v=None if v is None else v[i],
+                                metrics=None if met is None else met[i], iterations=int(it[i]),
+                                relative_residual=float(rel[i])) for i in range(k)]
+
+    def uq_moments(self) -> tuple[np.ndarray, np.ndarray, int]:
+        cnt = ctypes.c_int64()
+        mean = np.empty(self.n_tets, np.float64)
+        var = np.empty(self.n_tets, np.float64)
+        check(lib().cn_uq_moments(self.handle, ptr(mean), ptr(var), ctypes.byref(cnt)), "cn_uq_moments")
+        return mean, var, int(cnt.value)
 
     def recovery_build(self, mode: str) -> tuple[int, int]:
         """Build (once) the recovery operator of ``mode``; returns (n_slots, n_fallback)."""

@@ -174,7 +174,7 @@ int32_t cn_recovery_patches(const cn_context* ctx, int64_t* nnz, int32_t* n_unde
 
 /* Reduced-basis warm start. No reference counterpart (its nearest analogue is the Galerkin
  * initial guess of uq/conductivity/run.py:154-200). Solves with opt->collect = 1 append their
- * solutions (BF16) to a snapshot basis W of up to 512 columns. cn_warm_gram returns
+ * solutions (FP32) to a snapshot basis W of up to 512 columns. cn_warm_gram returns
  * G = W^T A W (count x count); the caller hands back M = V Lambda^-1/2 of its leading R <= 256
  * eigenpairs, and cn_warm_set_modes forms the modes Phi = W M (BF16 MFMA GEMM). cn_warm_mode_grams
  * returns Phi^T A Phi (R x R) and Phi^T A W (R x count); cn_warm_set_projector takes
@@ -188,9 +188,24 @@ int32_t cn_warm_set_modes(cn_context* ctx, const double* M, int32_t count, int32
 int32_t cn_warm_mode_grams(cn_context* ctx, double* G_modes, double* G_cross);
 int32_t cn_warm_set_projector(cn_context* ctx, const double* P, int32_t R);
 int32_t cn_warm_set_coefficients(cn_context* ctx, const double* c, int32_t k, int32_t R);
-/* The modes as BF16 bits, (n_reduced, width) row-major; width (a multiple of 16 >= R) is
- * written to *width; call with out = NULL to read it. */
+/* The modes as BF16 hi + lo pairs: out holds (2, n_reduced, width) BF16 bits, the high halves
+ * then the low halves, each row-major; width (a multiple of 16 >= R) is written to *width; call
+ * with out = NULL to read it. A mode value is hi + lo. */
 int32_t cn_warm_modes(const cn_context* ctx, uint16_t* out, int32_t* width);
+
+/* Conductivity uncertainty (uq/conductivity of the reference): for one placement, the per-tissue
+ * unit-conductivity operators K_t and right-hand sides b_t (assembly.py:37-146), so that a sample
+ * sigma gives K(sigma) = sum_t sigma_t K_t and b(sigma) = sum_t sigma_t b_t on the shared pattern.
+ * cn_uq_prepare returns the tissue tags (ascending, at most 16) that sigma's columns follow;
+ * cn_uq_solve solves k <= 16 samples (sigma: k x n_tissue) as independent CG chains under the
+ * nominal AMG preconditioner, with the same outputs as cn_solve_block, and accumulates the |E|
+ * moments that cn_uq_moments returns (per element mean and sample variance, FP64). */
+int32_t cn_uq_prepare(cn_context* ctx, const double* dip_pos_m, const double* dip_moment, int64_t n_dip,
+                      const double* center_mm, const double* handle_mm, double distance_mm, double didt,
+                      int32_t* n_tissue, int32_t* tags);
+int32_t cn_uq_solve(cn_context* ctx, const double* sigma, int32_t k, const cn_solve_options* opt,
+                    cn_block_outputs* out);
+int32_t cn_uq_moments(const cn_context* ctx, double* mean, double* var, int64_t* count);
 
 /* The warm-start network: n_in (<= 16) inputs -> hidden -> hidden -> n_out, GELU, trained on the
  * GPU in FP32 (Adam, mean squared error), inference with BF16 weights on MFMA. No reference

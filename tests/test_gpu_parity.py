@@ -206,11 +206,17 @@ def test_partial_blocks_agree_with_the_full_block(block8, gctx, coil, k):
 
 
 def test_degenerate_placement_raises(gctx, coil):
-    """place.cu:153-230: a handle along the scalp normal names the placement in a ValueError."""
+    """place.cu:153-230: a handle along the scalp normal names the placement in a ValueError. The
+    centre is a scalp node (its own projection) and the handle sits on the frame's normal there."""
     from cunibs_amd import fem
-    c = np.array([0.0, 0.0, 95.0])
-    with pytest.raises(ValueError):
-        fem.solve_placements_block(gctx, coil.positions_m, coil.moments, [(c, c + [0, 0, 10.0], 4.0)], 1e6)
+    c = gctx.nodes_mm[gctx.skin_tris[100, 0]]
+    tf, deg = gctx.place([c], [c + np.array([0.0, 50.0, 0.0])], [0.0])
+    assert deg[0] == 0
+    handle = c - 10.0 * tf[0, :3, 2]  # the frame's z axis is the inward normal
+    _, deg = gctx.place([c], [handle], [0.0])
+    assert deg[0] == 1
+    with pytest.raises(ValueError, match="index 0"):
+        fem.solve_placements_block(gctx, coil.positions_m, coil.moments, [(c, handle, 4.0)], 1e6)
 
 
 def test_missed_tolerance_raises(gctx, coil):
@@ -259,9 +265,11 @@ def test_kernel_dadt_nbody(oracle_mod):
     """solver/coil/dadt.cu:27-104 against the FP64 closed form (tests/test_placement.py:334-346)."""
     from cunibs_amd import fem
     rng = np.random.default_rng(12)
-    s = rng.normal(0, 0.02, (300, 3))
+    # Dipoles in a slab above the targets, as a coil sits outside the head (>= 25 mm apart here,
+    # so the FP32 |r - s|^2 = |r|^2 + |s|^2 - 2 r.s stays well conditioned).
+    s = np.column_stack([rng.normal(0, 0.02, (300, 2)), rng.uniform(0.0, 0.005, 300)])
     m = rng.normal(0, 1e-3, (300, 3))
-    r = rng.normal(0, 0.04, (5000, 3)) + [0, 0, -0.07]
+    r = np.column_stack([rng.normal(0, 0.04, (5000, 2)), rng.uniform(-0.12, -0.025, 5000)])
     mp = np.concatenate([m, np.cross(m, s)], 1)
     sn = (s * s).sum(1)
     got = fem.dadt_nbody(s, mp, sn, r, 1e6)

@@ -35,12 +35,12 @@ def trained(head4, coil):
 
 
 def test_modes_are_a_orthonormal_up_to_bf16(trained):
-    """Phi^T A Phi = I up to the modes' BF16 rounding, which P removes."""
+    """Phi^T A Phi = I up to the modes' rounding (BF16 hi + lo pairs), which P removes."""
     ctx, _, _, rank = trained
     g = ctx.warm_state["G_modes"]
     assert rank >= 40
-    assert np.abs(g - np.eye(rank)).max() < 5e-2
-    np.testing.assert_allclose(ctx.warm_state["P"] @ g, np.eye(rank), atol=1e-8)
+    assert np.abs(g - np.eye(rank)).max() < 1e-2
+    np.testing.assert_allclose(ctx.warm_state["P"] @ g, np.eye(rank), atol=1e-6)
 
 
 def test_initial_guess_is_the_galerkin_projection(trained, coil):
@@ -78,7 +78,7 @@ def test_warm_start_keeps_the_answer_and_saves_iterations(trained, coil):
         assert b.relative_residual <= 1e-6
         assert np.linalg.norm(a.magnE - b.magnE) <= 2e-5 * np.linalg.norm(a.magnE)
     assert np.mean([r.iterations for r in warm]) < np.mean([r.iterations for r in cold])
-    # A placement whose solution is in the basis starts within the BF16 rounding of the answer.
+    # A placement whose solution is in the basis starts within the modes' rounding of the answer.
     again_c = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, train[:4], 1e6)
     again_w = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, train[:4], 1e6, warm_start=True)
     assert max(r.iterations for r in again_w) <= min(r.iterations for r in again_c) - 5
