@@ -37,6 +37,36 @@ def spmm_diff_bytes(n: int, nnz: int, k: int) -> float:
     return 8.0 * nnz_off + 4.0 * (n + 1) + 4.0 * n + 8.0 * n * k
 
 
+COPY_BYTES = 512 << 20  # calibration copy: 512 MiB read + 512 MiB written per launch
+
+
+def kernels_only(ctx, coil, sites, k: int) -> int:
+    """The launches the HBM traffic counters are read on: 20 streaming copies of COPY_BYTES (the
+    counter calibration), 20 differenced SpMMs at width k (each after one untimed launch), then one
+    cold block of k placements with each recovery (raw, spr_global, harmonic), so that every kernel
+    of the path appears once with the same sizes as in the sweep."""
+    from cunibs_amd import fem
+
+    K = k if k in fem.BLOCK_SIZES else min(b for b in fem.BLOCK_SIZES if b >= k)
+    n_red, nnz = int(ctx.info.n_reduced), int(ctx.info.nnz_reduced)
+    copy_ms = ctx.bench_copy(COPY_BYTES, 20)
+    spmm_ms = ctx.bench_spmm(K, 20, mode=1)
+    iters = {}
+    for rec in ("raw", "spr_global", "harmonic"):
+        res = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites[:k], 1e6, magnitude=False,
+                                         metrics=True, recovery=rec)
+        iters[rec] = [r.iterations for r in res]
+    print(json.dumps({
+        "copy_bytes_read": COPY_BYTES, "copy_bytes_written": COPY_BYTES, "copy_us": round(copy_ms * 1e3, 2),
+        "copy_GBps": round(2 * COPY_BYTES / (copy_ms * 1e-3) / 1e9, 1),
+        "spmm_K": K, "n_reduced": n_red, "nnz_reduced": nnz,
+        "spmm_algorithmic_bytes": spmm_diff_bytes(n_red, nnz, K), "spmm_us": round(spmm_ms * 1e3, 2),
+        "block_iterations": iters,
+    }), flush=True)
+    ctx.free()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -50,6 +80,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--warm-basis", type=int, default=256,
                     help="training placements whose solutions form the warm-start basis (0: cold only)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the roofline launches (copy calibration + differenced SpMM), for counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the placement sweep shards "
@@ -69,6 +101,8 @@ def main() -> int:
     log(f"context {setup_s:.2f}s hierarchy {ctx.hierarchy_sizes}")
     k = args.block_k
     pos, mom = coil.positions_m, coil.moments
+    if args.kernels_only:
+        return kernels_only(ctx, coil, sites, k)
 
     def block(i0: int, kk: int, warm: bool = False):
         chunk = [sites[(i0 + j) % len(sites)] for j in range(kk)]
@@ -159,8 +193,9 @@ def main() -> int:
             "workload": f"{args.spacing} mm perturbed 5-shell head, 16x16 cap-grid sweep, block_k={k}",
             "n_nodes": ctx.n_nodes, "n_tets": ctx.n_tets, "nnz_reduced": nnz,
             "hierarchy": ctx.hierarchy_sizes, "block_k": k, "tolerance": 1e-6, "recovery": "raw",
-            "warm_start": (f"Galerkin projection on {args.warm_basis} solver-generated solutions (BF16 basis, "
-                           f"MFMA expansion)") if args.warm_basis > 0 else "none",
+            "warm_start": (f"Galerkin projection on {args.warm_basis} solver-generated solutions (FP32 snapshot "
+                           f"basis, A-orthonormal modes as BF16 hi+lo pairs, MFMA expansion)")
+            if args.warm_basis > 0 else "none",
         },
         "batched_cold_placements_per_s": round(cold_value, 3),
         "cold_iterations_mean": round(float(np.mean(cold_iters)), 2),
@@ -201,12 +236,10 @@ def main() -> int:
         errs = []
         gm = octx.tet_tags == fem.GM_TAG
         for i in range(args.accuracy):
-            gres = block(i * 37, 1)[0]
             ref = oracle.solve_site(octx, pos, mom, sites[(i * 37) % len(sites)], 1e6, tol=1e-12)
             g = fem.solve_placements_block(ctx, pos, mom, [sites[(i * 37) % len(sites)]], 1e6,
                                            warm_start=args.warm_basis > 0)[0]
             errs.append(float(np.linalg.norm(g.magnE[gm] - ref["magnE"][gm]) / np.linalg.norm(ref["magnE"][gm])))
-            del gres
         out["gm_magnE_rel_err_vs_fp64"] = max(errs) if errs else None
         # CPU baseline: the oracle's FP64 AMG-PCG path to 1e-6, per placement.
         t0 = time.time()

@@ -122,6 +122,7 @@ _SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "cn_k_place": (_I32, [_P, _P, _P, _P, _I32, _P, _P]),
     "cn_bench_spmm": (_I32, [_P, _I32, _I32, _I32, _I32, _P]),
     "cn_bench_vcycle": (_I32, [_P, _I32, _I32, _P]),
+    "cn_bench_copy": (_I32, [_P, _I64, _I32, _P]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

@@ -1129,6 +1129,13 @@ int32_t cn_bench_vcycle(cn_context* c, int32_t K, int32_t reps, double* ms) {
     });
 }
 
+int32_t cn_bench_copy(cn_context* c, int64_t bytes, int32_t reps, double* ms) {
+    return guarded([&] {
+        cn::require(c != nullptr && reps >= 1 && ms != nullptr, "bad argument");
+        *ms = c->solver->bench_copy(bytes, reps);
+    });
+}
+
 
 int32_t cn_recovery_build(cn_context* c, int32_t mode, int64_t* n_slots, int32_t* n_fallback) {
     return guarded([&] {

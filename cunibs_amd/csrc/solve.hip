@@ -1343,6 +1343,41 @@ double BlockSolver::bench_spmm(int K, int reps, int mode, int sub) {
     return static_cast<double>(ms) / reps;
 }
 
+// Streaming copy out[i] = in[i] over float4s: the calibration kernel of the HBM traffic counters
+// (FETCH_SIZE / WRITE_SIZE of a launch that moves a known number of bytes) and the practical
+// bandwidth ceiling beside the published peak.
+__global__ void __launch_bounds__(kBlock) stream_copy_kernel(std::int64_t n4, const float4* __restrict__ in,
+                                                             float4* __restrict__ out) {
+    const std::int64_t stride = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n4; i += stride)
+        out[i] = in[i];
+}
+
+double BlockSolver::bench_copy(std::int64_t bytes, int reps) {
+    require(bytes >= 16 && bytes % 16 == 0, "bytes must be a positive multiple of 16");
+    const std::int64_t n4 = bytes / 16;
+    DeviceBuffer<float> a(bytes / 4), b(bytes / 4);
+    fill_pattern_kernel<<<grid_for(bytes / 4), kBlock, 0, s_>>>(bytes / 4, a.get());
+    const int grid = static_cast<int>(std::min<std::int64_t>((n4 + kBlock - 1) / kBlock, 8192));
+    auto launch = [&] {
+        stream_copy_kernel<<<grid, kBlock, 0, s_>>>(n4, reinterpret_cast<const float4*>(a.get()),
+                                                    reinterpret_cast<float4*>(b.get()));
+    };
+    launch();
+    hipEvent_t e0, e1;
+    check(hipEventCreate(&e0), "event");
+    check(hipEventCreate(&e1), "event");
+    check(hipEventRecord(e0, s_), "event record");
+    for (int r = 0; r < reps; ++r) launch();
+    check(hipEventRecord(e1, s_), "event record");
+    check(hipEventSynchronize(e1), "event sync");
+    float ms = 0.f;
+    check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return static_cast<double>(ms) / reps;
+}
+
 double BlockSolver::bench_vcycle(int K, int reps) {
     require(padded_width(K) == K, "K must be a compiled block width");
     ensure(K);

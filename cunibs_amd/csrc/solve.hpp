@@ -90,6 +90,8 @@ class BlockSolver {
     // pass G = W^T S, H = W^T R. sub > 0 forces the sub-lanes per column group of the fine level.
     double bench_spmm(int K, int reps, int mode, int sub);
     double bench_vcycle(int K, int reps);
+    // Average ms of a streaming copy of `bytes` bytes (read `bytes`, write `bytes`).
+    double bench_copy(std::int64_t bytes, int reps);
     std::int64_t n() const { return op_.a32.n; }
     hipStream_t stream() const { return s_; }
 

@@ -448,6 +448,12 @@ class SolverContext:
         check(lib().cn_bench_vcycle(self.handle, K, reps, ctypes.byref(ms)), "cn_bench_vcycle")
         return float(ms.value)
 
+    def bench_copy(self, nbytes: int, reps: int = 20) -> float:
+        """Average ms of a streaming device copy of `nbytes` bytes (read and write)."""
+        ms = ctypes.c_double()
+        check(lib().cn_bench_copy(self.handle, int(nbytes), reps, ctypes.byref(ms)), "cn_bench_copy")
+        return float(ms.value)
+
 
 def solve_placements_block(
     ctx: SolverContext,

@@ -281,6 +281,10 @@ int32_t cn_k_place(cn_context* ctx, const double* centers_mm, const double* hand
 int32_t cn_bench_spmm(cn_context* ctx, int32_t K, int32_t reps, int32_t mode, int32_t sub, double* ms);
 /* Same for one V-cycle at width K. */
 int32_t cn_bench_vcycle(cn_context* ctx, int32_t K, int32_t reps, double* ms);
+/* Average device time (ms) of a streaming device-to-device copy of `bytes` bytes (a multiple of 16)
+ * on the library's stream: the calibration launch for the HBM traffic counters and the practical
+ * bandwidth ceiling. No counterpart in the reference (a measurement aid). */
+int32_t cn_bench_copy(cn_context* ctx, int64_t bytes, int32_t reps, double* ms);
 
 #ifdef __cplusplus
 }
