@@ -80,6 +80,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--warm-basis", type=int, default=256,
                     help="training placements whose solutions form the warm-start basis (0: cold only)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1: pick block width (4/8/16), SpMM tile and Chebyshev degree (2-6) on 32 training "
+                         "placements before the sweeps; 0: use --block-k, the automatic tile and degree 2")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the roofline launches (copy calibration + differenced SpMM), for counter runs")
     args = ap.parse_args()
@@ -112,14 +115,23 @@ def main() -> int:
     def sweep(warm: bool):
         for w in range(args.warmup):
             block(w * k, k, warm)
-        its = []
+        its, block_its = [], []
         t_start = time.perf_counter()
         for s in range(args.steps):
             res = block(s * k, k, warm)
             its.extend(r.iterations for r in res)
-        return time.perf_counter() - t_start, its
+            block_its.append(max(r.iterations for r in res))
+        return time.perf_counter() - t_start, its, block_its
 
-    cold_elapsed, cold_iters = sweep(False)
+    tune = None
+    if args.autotune:
+        from cunibs_amd import autotune
+
+        tune = autotune.autotune(ctx, pos, mom, synthetic.cap_sites(32, seed=13), 1e6, block_ks=(4, 8, 16))
+        k = tune.block_k
+        log(f"autotune {tune.seconds:.2f}s: block_k {k}, degree {tune.degree}, fine_sub {tune.fine_sub}; "
+            f"tile us {tune.tile_us}, degree rates {tune.degree_rates}, width rates {tune.width_rates}")
+    cold_elapsed, cold_iters, cold_block_iters = sweep(False)
     cold_value = args.steps * k / cold_elapsed
     log(f"batched cold k={k}: {cold_value:.1f} placements/s, {1e3 * cold_elapsed / args.steps:.1f} ms/step, "
         f"iterations mean {np.mean(cold_iters):.1f}")
@@ -137,17 +149,20 @@ def main() -> int:
         rank = ctx.warm_finalize()
         train_s = time.perf_counter() - t0
         log(f"warm basis: {args.warm_basis} training solves, rank {rank}, {train_s:.2f}s")
-        elapsed, iters = sweep(True)
+        elapsed, iters, block_iters = sweep(True)
     else:
-        elapsed, iters = cold_elapsed, cold_iters
+        elapsed, iters, block_iters = cold_elapsed, cold_iters, cold_block_iters
     value = args.steps * k / elapsed
     log(f"batched{' warm' if args.warm_basis > 0 else ''} k={k}: {value:.1f} placements/s, "
         f"{1e3 * elapsed / args.steps:.1f} ms/step, iterations mean {np.mean(iters):.1f}")
 
-    # Plain single-placement GPU PCG on the same sites.
+    # Plain single-placement GPU PCG on the same sites, at the reference's smoother and the
+    # automatic tile (no tuning, no warm start).
     plain_iters = [0]
     plain = float("nan")
     if args.plain > 0:
+        tuned = ctx.tuning
+        ctx.set_tuning(degree=2, lower_ratio=12.0, fine_sub=0)
         plain_iters = []
         block(0, 1)
         t0 = time.perf_counter()
@@ -156,6 +171,7 @@ def main() -> int:
             plain_iters.append(res[0].iterations)
         plain = args.plain / (time.perf_counter() - t0)
         log(f"plain k=1: {plain:.1f} placements/s, iterations mean {np.mean(plain_iters):.1f}")
+        ctx.set_tuning(**tuned)
 
     rounds = {}
     for kk in ((k, 1) if args.plain > 0 else (k,)):
@@ -199,6 +215,8 @@ def main() -> int:
         },
         "batched_cold_placements_per_s": round(cold_value, 3),
         "cold_iterations_mean": round(float(np.mean(cold_iters)), 2),
+        "cold_block_iterations_mean": round(float(np.mean(cold_block_iters)), 2),
+        "block_iterations_mean": round(float(np.mean(block_iters)), 2),
         "warm_basis_rank": rank,
         "warm_training_s": round(train_s, 3),
         "value_incl_training": round(args.steps * k / (elapsed + train_s), 3),
@@ -207,6 +225,7 @@ def main() -> int:
         "iterations_mean": round(float(np.mean(iters)), 2),
         "plain_iterations_mean": round(float(np.mean(plain_iters)), 2),
         "setup_s": round(setup_s, 3),
+        "autotune": tune.as_dict() if tune else None,
         "refinement_rounds": rounds,
         "roofline": {
             "kernel": f"rowk_kernel<{K},sub,kSpmmDiff> (differenced SpMM S = A W)",

@@ -53,6 +53,10 @@ class Info(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("degree", _I32), ("lower_ratio", _F), ("fine_sub", _I32)]
+
+
 class SolveOptions(ctypes.Structure):
     _fields_ = [
         ("tolerance", _D), ("max_iters", _I32), ("max_refine", _I32), ("drop_tol", _D),
@@ -79,6 +83,8 @@ _SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "cn_context_create": (_I32, [_P, _I64, _P, _P, _I64, _P, _I64, _P, _P]),
     "cn_context_destroy": (None, [_P]),
     "cn_context_info": (_I32, [_P, _P]),
+    "cn_context_get_tuning": (_I32, [_P, _P]),
+    "cn_context_set_tuning": (_I32, [_P, _P]),
     "cn_context_mesh": (_I32, [_P, _P, _P, _P, _P, _P, _P]),
     "cn_context_geometry": (_I32, [_P, _P, _P, _P, _P]),
     "cn_context_stiffness": (_I32, [_P, _P, _P, _P]),

@@ -1,38 +1,95 @@
-"""Launch-shape autotuning for a sweep: the block width (placements per block PCG) that gives the
-most placements per second on this mesh and GPU, measured, not modelled.
+"""Autotuner for a placement sweep: block width, fine-level SpMM tile and Chebyshev degree.
 
-The reference fixes its block at MAX_BLOCK = 8 (core/common.hpp:27-33); here the block method's
-iteration count and the cost of one iteration both change with the width (wider blocks converge in
-fewer iterations and stream the operator once for more right-hand sides, but their FP64 Gram work
-grows as K^2), so the best width depends on the mesh and is found by timing a block of each.
+The reference fixes all three: blocks of at most MAX_BLOCK = 8 placements (core/common.hpp:27-33),
+Chebyshev degree 2 with ratio 12 (fem/solve.py:153-168) and the V-cycle's threads per row from its
+`choose_tpr` rule (solver/amg/vcycle.cu:104-120). Here each is measured on the mesh at hand:
+
+* the tile (sub-lanes per 4-column group of the fine level's row kernels) changes only the speed
+  of one iteration, so it is chosen from kernel timings: the differenced SpMM with its Gram pass
+  plus one V-cycle, HIP-event timed on the library's stream;
+* the degree trades smoother cost against iterations, and the block width trades the operator
+  stream's reuse and the faster convergence of wide blocks against their K^2 Gram work, so both
+  are chosen by timing whole blocks over the same placements.
+
+Every candidate is timed after one untimed block that captures that setting's iteration graph.
+The context is left at the winning tile and degree; the winning width is returned.
 """
 
 from __future__ import annotations
 
 import time
 from collections.abc import Sequence
+from dataclasses import dataclass, field
 
 from cunibs_amd import fem
 
 
-def choose_block_k(ctx: fem.SolverContext, dip_pos_m, dip_moment, sites: Sequence, didt: float = 1e6,
-                   candidates: Sequence[int] = (8, 16), warm_start: bool = False, repeats: int = 1) -> tuple[int, dict]:
-    """Time `repeats` blocks of each candidate width on the first placements of `sites` (after an
-    untimed block that captures that width's iteration graph); returns the fastest width and the
-    measured placements/s of every candidate."""
-    rates: dict[int, float] = {}
-    for k in candidates:
-        if k < 1 or k > fem.MAX_BLOCK or len(sites) < k:
-            continue
-        chunk = list(sites[:k])
-        fem.solve_placements_block(ctx, dip_pos_m, dip_moment, chunk, didt, magnitude=False, metrics=True,
-                                   warm_start=warm_start)
-        t0 = time.perf_counter()
-        for _ in range(repeats):
-            fem.solve_placements_block(ctx, dip_pos_m, dip_moment, chunk, didt, magnitude=False, metrics=True,
-                                       warm_start=warm_start)
-        rates[k] = repeats * k / (time.perf_counter() - t0)
-    if not rates:
+@dataclass
+class TuningResult:
+    block_k: int
+    degree: int
+    fine_sub: int
+    seconds: float = 0.0
+    tile_us: dict = field(default_factory=dict)          # fine_sub -> us per iteration's fine work
+    degree_rates: dict = field(default_factory=dict)     # degree -> placements/s
+    width_rates: dict = field(default_factory=dict)      # block_k -> placements/s
+
+    def as_dict(self) -> dict:
+        return {"block_k": self.block_k, "degree": self.degree, "fine_sub": self.fine_sub,
+                "seconds": round(self.seconds, 3),
+                "tile_us": {str(k): round(v, 2) for k, v in self.tile_us.items()},
+                "degree_rates": {str(k): round(v, 2) for k, v in self.degree_rates.items()},
+                "width_rates": {str(k): round(v, 2) for k, v in self.width_rates.items()}}
+
+
+def _rate(ctx, pos, mom, sites, didt, k: int, warm_start: bool) -> float:
+    """Placements/s over `sites` in blocks of k, after one untimed block (graph capture)."""
+    def run(chunk):
+        fem.solve_placements_block(ctx, pos, mom, chunk, didt, magnitude=False, metrics=True, warm_start=warm_start)
+
+    run(list(sites[:k]))
+    t0 = time.perf_counter()
+    for i in range(0, len(sites), k):
+        run(list(sites[i:i + k]))
+    return len(sites) / (time.perf_counter() - t0)
+
+
+def autotune(ctx: fem.SolverContext, dip_pos_m, dip_moment, sites: Sequence, didt: float = 1e6, *,
+             block_ks: Sequence[int] = (4, 8, 16, 32), degrees: Sequence[int] = (2, 3, 4, 5, 6),
+             subs: Sequence[int] = (0, 1, 2, 4, 8), base_k: int = 8, warm_start: bool = False) -> TuningResult:
+    """Tune on `sites` (their count should be a multiple of every width, e.g. 32): the tile at
+    width `base_k`, then the degree at `base_k`, then the width. Widths above what a warm start
+    supports (16) are skipped when `warm_start` is set."""
+    t_start = time.perf_counter()
+    widths = [k for k in block_ks if 1 <= k <= fem.MAX_BLOCK and k <= len(sites) and not (warm_start and k > 16)]
+    if not widths:
         raise ValueError("no candidate block width fits the placements given")
-    best = max(rates, key=rates.get)
-    return best, rates
+    if base_k not in widths:
+        base_k = widths[len(widths) // 2]
+    K = min(b for b in fem.BLOCK_SIZES if b >= base_k)
+    res = TuningResult(block_k=base_k, degree=ctx.tuning["degree"], fine_sub=0)
+
+    # 1. tile: kernel time of one iteration's fine-level work at the base width
+    for sub in subs:
+        try:
+            ctx.set_tuning(fine_sub=sub)
+            res.tile_us[sub] = 1e3 * (ctx.bench_spmm(K, 10, mode=2) + ctx.bench_vcycle(K, 5))
+        except ValueError:
+            continue  # more sub-lanes than a wavefront holds at this width
+    res.fine_sub = min(res.tile_us, key=res.tile_us.get)
+    ctx.set_tuning(fine_sub=res.fine_sub)
+
+    # 2. degree: whole blocks at the base width
+    for d in degrees:
+        ctx.set_tuning(degree=d)
+        res.degree_rates[d] = _rate(ctx, dip_pos_m, dip_moment, sites, didt, base_k, warm_start)
+    res.degree = max(res.degree_rates, key=res.degree_rates.get)
+    ctx.set_tuning(degree=res.degree)
+
+    # 3. width: whole blocks over the same placements
+    for k in widths:
+        res.width_rates[k] = res.degree_rates[res.degree] if k == base_k else \
+            _rate(ctx, dip_pos_m, dip_moment, sites, didt, k, warm_start)
+    res.block_k = max(res.width_rates, key=res.width_rates.get)
+    res.seconds = time.perf_counter() - t_start
+    return res

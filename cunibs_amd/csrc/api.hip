@@ -813,6 +813,31 @@ void cn_context_destroy(cn_context* ctx) {
     }
 }
 
+int32_t cn_context_get_tuning(const cn_context* c, cn_tuning* o) {
+    return guarded([&] {
+        cn::require(c != nullptr && o != nullptr, "null argument");
+        o->degree = c->hier.cheb.degree;
+        o->lower_ratio = c->opts.lower_ratio;
+        o->fine_sub = c->solver->fine_sub_setting();
+    });
+}
+
+int32_t cn_context_set_tuning(cn_context* c, const cn_tuning* t) {
+    return guarded([&] {
+        cn::require(c != nullptr && t != nullptr, "null argument");
+        cn::require(t->degree >= 1 && t->degree <= cn::kMaxDegree, "degree must be in [1, 6]");
+        cn::require(t->lower_ratio > 1.0f, "lower_ratio must exceed 1");
+        cn::require(t->fine_sub >= 0 && t->fine_sub <= 64 && (t->fine_sub & (t->fine_sub - 1)) == 0,
+                    "fine_sub must be 0 or a power of two <= 64");
+        cn::check(hipStreamSynchronize(c->s), "sync before retuning");
+        c->hier.cheb = cn::chebyshev(t->degree, t->lower_ratio);
+        c->opts.degree = t->degree;
+        c->opts.lower_ratio = t->lower_ratio;
+        c->solver->set_fine_sub(t->fine_sub);
+        c->solver->invalidate();
+    });
+}
+
 int32_t cn_context_info(const cn_context* c, cn_info* o) {
     return guarded([&] {
         cn::require(c && o, "null argument");

@@ -1097,18 +1097,27 @@ void BlockSolver::ensure(int K) {
     drop_graphs();  // the captured iterations named the buffers just replaced
 }
 
+void BlockSolver::set_fine_sub(int sub) {
+    require(sub >= 0 && sub <= kWave && (sub & (sub - 1)) == 0, "sub must be 0 or a power of two <= 64");
+    fine_sub_set_ = sub;
+    fine_sub_override_ = sub;
+    drop_graphs();
+}
+
 template <int K>
 void BlockSolver::vcycle_k(const float* B, float* X, const int* state) {
-    // Lanes per column of each level's row kernels: a property of the level and the width alone.
+    // Lanes per column of each level's row kernels: a property of the level and the width alone,
+    // unless a fine-level count is in force (clamped to what one wavefront holds at this width).
     std::vector<int> subs(h_.levels.size());
     for (std::size_t i = 0; i < h_.levels.size(); ++i)
         subs[i] = choose_sub(h_.levels[i].a.n, h_.levels[i].a.nnz, col_groups<K>(), n_cu_);
+    if (!subs.empty() && fine_sub_override_ > 0) subs[0] = std::min(fine_sub_override_, kWave / col_groups<K>());
     run_vcycle<K>(h_, subs, lb_, lx_, lr_, cb_.get(), cx_.get(), B, X, state, s_);
 }
 
 template <int K>
 int BlockSolver::fine_sub() const {
-    if (fine_sub_override_ > 0) return fine_sub_override_;
+    if (fine_sub_override_ > 0) return std::min(fine_sub_override_, kWave / col_groups<K>());
     return choose_sub(op_.off.n, op_.off.nnz, col_groups<K>(), n_cu_);
 }
 
@@ -1277,7 +1286,7 @@ SolveStats BlockSolver::solve(const float* B, double* X, const double* X0, int k
 void BlockSolver::spmm(const float* X, float* Y, int K) {
     dispatch_k(K, [&](auto kc) {
         constexpr int KK = decltype(kc)::value;
-        const int sub = fine_sub_override_ > 0 ? fine_sub_override_
+        const int sub = fine_sub_override_ > 0 ? std::min(fine_sub_override_, kWave / col_groups<KK>())
                                                : choose_sub(op_.a32.n, op_.a32.nnz, col_groups<KK>(), n_cu_);
         launch_rows<KK, kSpmm>(sub, op_.a32, nullptr, nullptr, X, nullptr, 0.f, 0.f, 0.f, Y, nullptr, s_);
     });
@@ -1317,9 +1326,10 @@ double BlockSolver::bench_spmm(int K, int reps, int mode, int sub) {
     fill_pattern_kernel<<<grid_for(nk), kBlock, 0, s_>>>(nk, R_.get());
     struct Restore {
         int& v;
-        ~Restore() { v = 0; }
-    } restore{fine_sub_override_};
-    fine_sub_override_ = sub;
+        int keep;
+        ~Restore() { v = keep; }
+    } restore{fine_sub_override_, fine_sub_set_};
+    if (sub > 0) fine_sub_override_ = sub;
     auto launch = [&] {
         if (mode == 2)
             dispatch_k(K, [&](auto kc) { spmm_block_k<decltype(kc)::value>(nullptr); });

@@ -94,6 +94,12 @@ class BlockSolver {
     double bench_copy(std::int64_t bytes, int reps);
     std::int64_t n() const { return op_.a32.n; }
     hipStream_t stream() const { return s_; }
+    // Sub-lanes per column group of the fine level's row kernels (the SpMM tile); 0 = chosen from
+    // the level's shape. Drops the captured iterations.
+    void set_fine_sub(int sub);
+    int fine_sub_setting() const { return fine_sub_set_; }
+    // Forget every captured iteration (after a change to the hierarchy's smoother).
+    void invalidate() { drop_graphs(); }
 
   private:
     template <int K>
@@ -113,7 +119,8 @@ class BlockSolver {
     hipStream_t s_;
     int cap_k_ = 0;
     int n_cu_ = 256;
-    int fine_sub_override_ = 0;  // bench_spmm's forced sub-lane count while it runs
+    int fine_sub_set_ = 0;       // set_fine_sub's value (0: automatic)
+    int fine_sub_override_ = 0;  // the fine level's sub-lane count in force (bench_spmm may force one)
     const MultiOperator* multi_ = nullptr;
     // block vectors (n, K)
     DeviceBuffer<float> R_, Z_, W_, S_;

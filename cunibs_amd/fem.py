@@ -437,6 +437,22 @@ class SolverContext:
             out["n_undetermined"] = int(und.value)
         return out
 
+    @property
+    def tuning(self) -> dict:
+        """Smoother degree, Chebyshev ratio and fine-level SpMM tile (sub-lanes, 0 = automatic)."""
+        t = _lib.Tuning()
+        check(lib().cn_context_get_tuning(self.handle, ctypes.byref(t)), "cn_context_get_tuning")
+        return {"degree": int(t.degree), "lower_ratio": float(t.lower_ratio), "fine_sub": int(t.fine_sub)}
+
+    def set_tuning(self, degree: int | None = None, lower_ratio: float | None = None,
+                   fine_sub: int | None = None) -> None:
+        """Change the settings of `tuning` between solves (None keeps the current value)."""
+        cur = self.tuning
+        t = _lib.Tuning(cur["degree"] if degree is None else int(degree),
+                        cur["lower_ratio"] if lower_ratio is None else float(lower_ratio),
+                        cur["fine_sub"] if fine_sub is None else int(fine_sub))
+        check(lib().cn_context_set_tuning(self.handle, ctypes.byref(t)), "cn_context_set_tuning")
+
     def bench_spmm(self, K: int, reps: int = 20, mode: int = 1, sub: int = 0) -> float:
         """Average ms of the fine-level product: mode 0 plain SpMM, 1 differenced SpMM, 2 with Grams."""
         ms = ctypes.c_double()

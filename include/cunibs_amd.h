@@ -74,6 +74,21 @@ int32_t cn_context_create(const double* nodes_mm, int64_t n_nodes, const int32_t
                           int64_t n_skin, const cn_options* opts, cn_context** out);
 void cn_context_destroy(cn_context* ctx);
 
+/* Launch and smoother settings of a built context, changeable between solves (the autotuner's
+ * knobs; the reference fixes them: Chebyshev degree 2 and ratio 12 at fem/solve.py:153-168,
+ * threads per row by choose_tpr at solver/amg/vcycle.cu:104-120). Setting them drops the
+ * captured iterations; the next solve of each width captures again. */
+typedef struct {
+    int32_t degree;     /* Chebyshev degree of the V-cycle smoother, 1..6 */
+    float lower_ratio;  /* Chebyshev interval [1/lower_ratio, 1], > 1 */
+    int32_t fine_sub;   /* lanes per 4-column group of the fine level's row kernels (the SpMM
+                           tile): 0 = chosen from the level's shape, else a power of two <= 64,
+                           clamped to one wavefront per row at wide blocks */
+} cn_tuning;
+
+int32_t cn_context_get_tuning(const cn_context* ctx, cn_tuning* out);
+int32_t cn_context_set_tuning(cn_context* ctx, const cn_tuning* tuning);
+
 #define CN_MAX_LEVELS 32
 
 typedef struct {
