@@ -37,6 +37,30 @@ def spmm_diff_bytes(n: int, nnz: int, k: int) -> float:
     return 8.0 * nnz_off + 4.0 * (n + 1) + 4.0 * n + 8.0 * n * k
 
 
+def vcycle_bytes(info, K: int, degree: int) -> float:
+    """Algorithmic bytes of one zero-guess V-cycle at width K (FP32 vectors, int32 indices): per
+    smoothed level of n rows, nnz entries and p prolongator entries (n_c coarse rows) the first
+    Jacobi step (b, dinv in; x out), degree - 1 further pre-smoothing and degree post-smoothing
+    Chebyshev steps (matrix once, row pointers, dinv, x gathered once, x_prev, b, out), the
+    residual (matrix, b, x, out), the restriction (P^T once, fine residual in, coarse rhs out) and
+    the prolongation (P once, coarse x in, fine x read and written); then the coarsest FP64 inverse
+    (read once) with its rhs and solution."""
+    total = 0.0
+    for i in range(info.n_levels):
+        n, a, p = info.level_rows[i], info.level_nnz[i], info.level_p_nnz[i]
+        nc = info.level_rows[i + 1]
+        mat = 8.0 * a + 4.0 * (n + 1)
+        cheby = mat + 4.0 * n + 16.0 * n * K
+        total += 8.0 * n * K + 4.0 * n              # Jacobi step from zero
+        total += (2 * degree - 1) * cheby            # pre (degree - 1) and post (degree) steps
+        total += mat + 12.0 * n * K                  # residual
+        total += 8.0 * p + 4.0 * (nc + 1) + 4.0 * K * (n + nc)        # restriction
+        total += 8.0 * p + 4.0 * (n + 1) + 4.0 * K * nc + 8.0 * n * K  # prolongation
+    nco = info.level_rows[info.n_levels]
+    total += 8.0 * nco * nco + 8.0 * nco * K
+    return total
+
+
 COPY_BYTES = 512 << 20  # calibration copy: 512 MiB read + 512 MiB written per launch
 
 
@@ -83,6 +107,8 @@ def main() -> int:
     ap.add_argument("--autotune", type=int, default=1,
                     help="1: pick block width (4/8/16), SpMM tile and Chebyshev degree (2-6) on 32 training "
                          "placements before the sweeps; 0: use --block-k, the automatic tile and degree 2")
+    ap.add_argument("--isolated", type=int, default=0,
+                    help="1: also time the SpMM and V-cycle in isolated repeated launches (warm caches)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the roofline launches (copy calibration + differenced SpMM), for counter runs")
     args = ap.parse_args()
@@ -123,6 +149,7 @@ def main() -> int:
             block_its.append(max(r.iterations for r in res))
         return time.perf_counter() - t_start, its, block_its
 
+    ctx.timing_enable(True)
     tune = None
     if args.autotune:
         from cunibs_amd import autotune
@@ -131,7 +158,9 @@ def main() -> int:
         k = tune.block_k
         log(f"autotune {tune.seconds:.2f}s: block_k {k}, degree {tune.degree}, fine_sub {tune.fine_sub}; "
             f"tile us {tune.tile_us}, degree rates {tune.degree_rates}, width rates {tune.width_rates}")
+    ctx.timing_enable(True)  # reset the in-solve sums
     cold_elapsed, cold_iters, cold_block_iters = sweep(False)
+    cold_timing = ctx.timing_read()
     cold_value = args.steps * k / cold_elapsed
     log(f"batched cold k={k}: {cold_value:.1f} placements/s, {1e3 * cold_elapsed / args.steps:.1f} ms/step, "
         f"iterations mean {np.mean(cold_iters):.1f}")
@@ -149,9 +178,12 @@ def main() -> int:
         rank = ctx.warm_finalize()
         train_s = time.perf_counter() - t0
         log(f"warm basis: {args.warm_basis} training solves, rank {rank}, {train_s:.2f}s")
+        ctx.timing_enable(True)
         elapsed, iters, block_iters = sweep(True)
+        timing = ctx.timing_read()
     else:
         elapsed, iters, block_iters = cold_elapsed, cold_iters, cold_block_iters
+        timing = cold_timing
     value = args.steps * k / elapsed
     log(f"batched{' warm' if args.warm_basis > 0 else ''} k={k}: {value:.1f} placements/s, "
         f"{1e3 * elapsed / args.steps:.1f} ms/step, iterations mean {np.mean(iters):.1f}")
@@ -180,17 +212,26 @@ def main() -> int:
         rounds[kk] = {"iterations": ex["round_iterations"], "true_residual": ex["round_residual"]}
         log(f"rounds k={kk}: {rounds[kk]}")
 
-    # Roofline of the block solve's fine-level product (the operator every row kernel streams).
+    # Rooflines from the timed sweep itself: HIP events recorded inside the captured iteration
+    # around the differenced SpMM (S = A W) and the V-cycle, averaged over the sweep's iterations.
     n_red = int(ctx.info.n_reduced)
     nnz = int(ctx.info.nnz_reduced)
     K = k if k in fem.BLOCK_SIZES else min(b for b in fem.BLOCK_SIZES if b >= k)
-    spmm_ms = ctx.bench_spmm(K, 50, mode=1)
+    degree = ctx.tuning["degree"]
+    spmm_ms, vc_ms, it_ms = timing["spmm_ms"], timing["vcycle_ms"], timing["iteration_ms"]
     achieved = spmm_diff_bytes(n_red, nnz, K) / (spmm_ms * 1e-3) / 1e9
-    spmm_gram_ms = ctx.bench_spmm(K, 50, mode=2)
-    spmm_plain_ms = ctx.bench_spmm(K, 50, mode=0)
-    vc_ms = ctx.bench_vcycle(K, 10)
-    log(f"spmm K={K}: differenced {spmm_ms * 1e3:.1f} us ({achieved:.0f} GB/s), with Grams "
-        f"{spmm_gram_ms * 1e3:.1f} us, plain {spmm_plain_ms * 1e3:.1f} us, vcycle {vc_ms * 1e3:.1f} us")
+    vc_bytes = vcycle_bytes(ctx.info, K, degree)
+    vc_achieved = vc_bytes / (vc_ms * 1e-3) / 1e9
+    log(f"in-sweep ({timing['samples']} samples) K={K}: differenced SpMM {spmm_ms * 1e3:.1f} us "
+        f"({achieved:.0f} GB/s), V-cycle {vc_ms * 1e3:.1f} us ({vc_achieved:.0f} GB/s), "
+        f"iteration {it_ms * 1e3:.1f} us")
+    isolated = None
+    if args.isolated:
+        isolated = {"spmm_us": round(ctx.bench_spmm(K, 50, mode=1) * 1e3, 2),
+                    "spmm_with_gram_us": round(ctx.bench_spmm(K, 50, mode=2) * 1e3, 2),
+                    "plain_spmm_us": round(ctx.bench_spmm(K, 50, mode=0) * 1e3, 2),
+                    "vcycle_us": round(ctx.bench_vcycle(K, 10) * 1e3, 2)}
+        log(f"isolated launches: {isolated}")
 
     out = {
         "metric": METRIC,
@@ -228,7 +269,7 @@ def main() -> int:
         "autotune": tune.as_dict() if tune else None,
         "refinement_rounds": rounds,
         "roofline": {
-            "kernel": f"rowk_kernel<{K},sub,kSpmmDiff> (differenced SpMM S = A W)",
+            "kernel": f"rowk_kernel<{K},{ctx.tuning['fine_sub'] or 'auto'},kSpmmDiff> (differenced SpMM S = A W)",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
@@ -236,11 +277,24 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
             "avg_us": round(spmm_ms * 1e3, 2),
+            "samples": timing["samples"],
             "algorithmic_bytes": spmm_diff_bytes(n_red, nnz, K),
-            "with_gram_us": round(spmm_gram_ms * 1e3, 2),
-            "plain_spmm_us": round(spmm_plain_ms * 1e3, 2),
-            "vcycle_us": round(vc_ms * 1e3, 1),
         },
+        "vcycle_roofline": {
+            "bound": "hbm",
+            "achieved": round(vc_achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(vc_achieved / HBM_PEAK_GBS, 4),
+            "avg_us": round(vc_ms * 1e3, 2),
+            "algorithmic_bytes": vc_bytes,
+            "degree": degree,
+        },
+        "iteration_us": round(it_ms * 1e3, 2),
+        "cold_in_sweep_us": {"spmm": round(cold_timing["spmm_ms"] * 1e3, 2),
+                             "vcycle": round(cold_timing["vcycle_ms"] * 1e3, 2),
+                             "iteration": round(cold_timing["iteration_ms"] * 1e3, 2)},
+        "isolated_us": isolated,
     }
 
     if not args.no_cpu:

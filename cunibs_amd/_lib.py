@@ -50,6 +50,7 @@ class Info(ctypes.Structure):
         ("ground_node", _I32), ("n_levels", _I32),
         ("level_rows", _I64 * (MAX_LEVELS + 1)), ("level_nnz", _I64 * (MAX_LEVELS + 1)),
         ("n_region", _I64), ("region_volume_m3", _D), ("n_cu", _I32),
+        ("level_p_nnz", _I64 * MAX_LEVELS),
     ]
 
 
@@ -129,6 +130,8 @@ _SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "cn_bench_spmm": (_I32, [_P, _I32, _I32, _I32, _I32, _P]),
     "cn_bench_vcycle": (_I32, [_P, _I32, _I32, _P]),
     "cn_bench_copy": (_I32, [_P, _I64, _I32, _P]),
+    "cn_timing_enable": (_I32, [_P, _I32]),
+    "cn_timing_read": (_I32, [_P, _P]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

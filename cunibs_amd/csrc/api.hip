@@ -853,6 +853,7 @@ int32_t cn_context_info(const cn_context* c, cn_info* o) {
         for (int i = 0; i < o->n_levels; ++i) {
             o->level_rows[i] = c->hier.levels[i].a.n;
             o->level_nnz[i] = c->hier.levels[i].a.nnz;
+            o->level_p_nnz[i] = c->hier.levels[i].p.nnz;
         }
         o->level_rows[o->n_levels] = c->hier.coarse.n;
         o->level_nnz[o->n_levels] = c->hier.coarse.nnz;
@@ -1151,6 +1152,20 @@ int32_t cn_bench_vcycle(cn_context* c, int32_t K, int32_t reps, double* ms) {
     return guarded([&] {
         cn::require(c != nullptr && reps >= 1, "bad argument");
         *ms = c->solver->bench_vcycle(K, reps);
+    });
+}
+
+int32_t cn_timing_enable(cn_context* c, int32_t on) {
+    return guarded([&] {
+        cn::require(c != nullptr, "null context");
+        c->solver->set_timing(on != 0);
+    });
+}
+
+int32_t cn_timing_read(const cn_context* c, double* out) {
+    return guarded([&] {
+        cn::require(c != nullptr && out != nullptr, "null argument");
+        c->solver->timing(out);
     });
 }
 

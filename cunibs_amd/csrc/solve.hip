@@ -1047,6 +1047,8 @@ BlockSolver::BlockSolver(const Operator& op, const Hierarchy& h, hipStream_t s) 
 
 BlockSolver::~BlockSolver() {
     drop_graphs();
+    for (auto& e : ev_)
+        if (e) (void)hipEventDestroy(e);
     if (host_state_) (void)hipHostFree(host_state_);
 }
 
@@ -1097,6 +1099,44 @@ void BlockSolver::ensure(int K) {
     drop_graphs();  // the captured iterations named the buffers just replaced
 }
 
+void BlockSolver::mark(int i) {
+    if (timing_) check(hipEventRecord(ev_[i], s_), "timing event");
+}
+
+void BlockSolver::sample() {
+    float ms[3] = {};
+    check(hipEventElapsedTime(&ms[0], ev_[0], ev_[1]), "elapsed");
+    check(hipEventElapsedTime(&ms[1], ev_[2], ev_[3]), "elapsed");
+    check(hipEventElapsedTime(&ms[2], ev_[0], ev_[4]), "elapsed");
+    for (int i = 0; i < 3; ++i) t_sum_[i] += ms[i];
+    ++t_n_;
+}
+
+void BlockSolver::set_timing(bool on) {
+    t_n_ = 0;
+    t_sum_[0] = t_sum_[1] = t_sum_[2] = 0.0;
+    if (on == timing_) return;
+    drop_graphs();  // the captured iterations are recorded with or without the events
+    if (on && ev_[0] == nullptr)
+        for (auto& e : ev_) check(hipEventCreate(&e), "timing event");
+    if (on) {
+        // An event record must be capturable into a graph on this runtime; find out before a
+        // solve's capture depends on it.
+        hipGraph_t g = nullptr;
+        check(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal), "begin capture");
+        const hipError_t rec = hipEventRecord(ev_[0], s_);
+        const hipError_t end = hipStreamEndCapture(s_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        require(rec == hipSuccess && end == hipSuccess, "event records cannot be captured on this runtime");
+    }
+    timing_ = on;
+}
+
+void BlockSolver::timing(double* out) const {
+    out[0] = static_cast<double>(t_n_);
+    for (int i = 0; i < 3; ++i) out[1 + i] = t_n_ ? t_sum_[i] / static_cast<double>(t_n_) : 0.0;
+}
+
 void BlockSolver::set_fine_sub(int sub) {
     require(sub >= 0 && sub <= kWave && (sub & (sub - 1)) == 0, "sub must be 0 or a power of two <= 64");
     fine_sub_set_ = sub;
@@ -1135,6 +1175,7 @@ void BlockSolver::spmm_block_k(const int* state) {
         launch_rows<K, kSpmmDiff>(fine_sub<K>(), op_.off, op_.rowsum, nullptr, W_.get(), nullptr, 0.f, 0.f, 0.f,
                                   S_.get(), state, s_);
     }
+    mark(1);
     gram_rows_kernel<K, true><<<kGramBlocks, gram_threads<K, true>(), 0, s_>>>(nn, W_.get(), S_.get(), R_.get(),
                                                                         partials_.get(), gram_rows(nn), state);
 }
@@ -1145,19 +1186,23 @@ void BlockSolver::iteration_k(const SolveOptions& opt) {
     const std::int64_t rpb = gram_rows(nn);
     const int* st = state_.get();
     int* stw = state_.get();
+    mark(0);
     spmm_block_k<K>(st);
     coef_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, opt.drop_tol, G_.get(), T_.get(),
                                                 TT_.get(), AL_.get(), stw, multi_ ? 1 : 0);
     update_xr_kernel<K><<<kGramBlocks, xr_threads<K>(), 0, s_>>>(nn, W_.get(), S_.get(), AL_.get(), X64_.get(), R_.get(),
                                                              partials_.get(), rpb, st);
     converge_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, target_.get(), norms_.get(), stw);
+    mark(2);
     vcycle_k<K>(R_.get(), Z_.get(), st);
+    mark(3);
     gram_rows_kernel<K, false><<<kGramBlocks, gram_threads<K, false>(), 0, s_>>>(nn, S_.get(), Z_.get(), nullptr,
                                                                          partials_.get(), rpb, st);
     dmat_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, TT_.get(), D_.get(), stw,
                                                 multi_ ? 1 : 0);
     update_w_kernel<K><<<grid_for(nn * col_groups<K>(), kRowThreads), kRowThreads, 0, s_>>>(nn, Z_.get(), D_.get(),
                                                                                            W_.get(), st);
+    mark(4);
 }
 
 template <int K>
@@ -1248,6 +1293,7 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
             d2h(host_state_, state_.get(), 2 + 2 * K, s_);
             sync(s_, "convergence read");
             total_iters = host_state_[1];
+            if (timing_ && total_iters == launched) sample();  // the batch's last iteration ran in full
             if (host_state_[0]) {
                 done = true;
                 break;

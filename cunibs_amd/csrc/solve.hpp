@@ -100,6 +100,12 @@ class BlockSolver {
     int fine_sub_setting() const { return fine_sub_set_; }
     // Forget every captured iteration (after a change to the hierarchy's smoother).
     void invalidate() { drop_graphs(); }
+    // In-solve timing: with it on, each iteration (captured or not) records HIP events around the
+    // differenced SpMM, the V-cycle and the whole iteration, and every batch whose last iteration
+    // ran in full adds one sample. Switching drops the captured iterations and resets the sums.
+    void set_timing(bool on);
+    // out[0] samples, out[1..3] mean ms of the SpMM, the V-cycle and the iteration.
+    void timing(double* out) const;
 
   private:
     template <int K>
@@ -136,6 +142,12 @@ class BlockSolver {
     // One captured iteration per block width, kept for the solver's lifetime: every buffer the
     // iteration names is solver-owned and reallocated only when the capacity grows (which drops
     // them all).
+    bool timing_ = false;
+    hipEvent_t ev_[5] = {};
+    double t_sum_[3] = {};
+    std::int64_t t_n_ = 0;
+    void mark(int i);
+    void sample();
     hipGraphExec_t graphs_[kNumBlockWidths] = {};
     double graph_drop_[kNumBlockWidths] = {};  // the drop tolerance each graph was captured with
     void drop_graphs();

@@ -464,6 +464,17 @@ class SolverContext:
         check(lib().cn_bench_vcycle(self.handle, K, reps, ctypes.byref(ms)), "cn_bench_vcycle")
         return float(ms.value)
 
+    def timing_enable(self, on: bool = True) -> None:
+        """Record HIP events around the SpMM, the V-cycle and each iteration of later solves."""
+        check(lib().cn_timing_enable(self.handle, 1 if on else 0), "cn_timing_enable")
+
+    def timing_read(self) -> dict:
+        """Samples and mean ms of the in-solve differenced SpMM, V-cycle and iteration."""
+        out = np.zeros(4)
+        check(lib().cn_timing_read(self.handle, ptr(out)), "cn_timing_read")
+        return {"samples": int(out[0]), "spmm_ms": float(out[1]), "vcycle_ms": float(out[2]),
+                "iteration_ms": float(out[3])}
+
     def bench_copy(self, nbytes: int, reps: int = 20) -> float:
         """Average ms of a streaming device copy of `nbytes` bytes (read and write)."""
         ms = ctypes.c_double()

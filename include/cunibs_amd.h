@@ -101,6 +101,7 @@ typedef struct {
     int64_t n_region;
     double region_volume_m3;
     int32_t n_cu;
+    int64_t level_p_nnz[CN_MAX_LEVELS]; /* nonzeros of each smoothed level's prolongator */
 } cn_info;
 
 int32_t cn_context_info(const cn_context* ctx, cn_info* out);
@@ -296,6 +297,15 @@ int32_t cn_k_place(cn_context* ctx, const double* centers_mm, const double* hand
 int32_t cn_bench_spmm(cn_context* ctx, int32_t K, int32_t reps, int32_t mode, int32_t sub, double* ms);
 /* Same for one V-cycle at width K. */
 int32_t cn_bench_vcycle(cn_context* ctx, int32_t K, int32_t reps, double* ms);
+/* In-solve kernel timing, a measurement aid with no counterpart in the reference. When on, every
+ * block-PCG iteration (inside its captured graph) records HIP events on the library's stream
+ * around the differenced SpMM, the V-cycle and the whole iteration; each convergence check whose
+ * last iteration ran in full adds one sample. Switching on or off resets the sums and drops the
+ * captured iterations. Fails with CN_EINVAL if the runtime cannot capture event records. */
+int32_t cn_timing_enable(cn_context* ctx, int32_t on);
+/* out[0] samples, out[1] mean ms of the differenced SpMM, out[2] of the V-cycle, out[3] of the
+ * whole iteration. */
+int32_t cn_timing_read(const cn_context* ctx, double* out4);
 /* Average device time (ms) of a streaming device-to-device copy of `bytes` bytes (a multiple of 16)
  * on the library's stream: the calibration launch for the HBM traffic counters and the practical
  * bandwidth ceiling. No counterpart in the reference (a measurement aid). */

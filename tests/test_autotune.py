@@ -98,3 +98,24 @@ def test_autotuned_context_gives_the_same_field(tctx, octx, oracle_mod, coil):
         assert all(r.relative_residual <= 1e-6 for r in out)
     finally:
         tctx.set_tuning(degree=2, fine_sub=0)
+
+
+def test_in_solve_timing_leaves_the_answer_bitwise_unchanged(tctx, coil):
+    """Event records inside the captured iteration time the SpMM, the V-cycle and the iteration
+    without touching the numerics."""
+    from cunibs_amd import fem, synthetic
+    sites = synthetic.cap_sites(8, seed=21)
+    base = fem.solve_placements_block(tctx, coil.positions_m, coil.moments, sites, 1e6)
+    tctx.timing_enable(True)
+    try:
+        timed = fem.solve_placements_block(tctx, coil.positions_m, coil.moments, sites, 1e6)
+        t = tctx.timing_read()
+    finally:
+        tctx.timing_enable(False)
+    for a, b in zip(base, timed):
+        np.testing.assert_array_equal(a.magnE, b.magnE)
+        assert a.iterations == b.iterations
+    assert t["samples"] >= 1
+    assert 0 < t["spmm_ms"] < t["iteration_ms"]
+    assert 0 < t["vcycle_ms"] < t["iteration_ms"]
+    assert t["spmm_ms"] + t["vcycle_ms"] <= t["iteration_ms"]
