@@ -212,8 +212,9 @@ def main() -> int:
         rounds[kk] = {"iterations": ex["round_iterations"], "true_residual": ex["round_residual"]}
         log(f"rounds k={kk}: {rounds[kk]}")
 
-    # Rooflines from the timed sweep itself: HIP events recorded inside the captured iteration
-    # around the differenced SpMM (S = A W) and the V-cycle, averaged over the sweep's iterations.
+    # Rooflines from the timed sweep itself: device wall-clock stamps inside the captured
+    # iteration (the differenced SpMM's own blocks, and marks around the V-cycle), averaged over
+    # every iteration of the sweep.
     n_red = int(ctx.info.n_reduced)
     nnz = int(ctx.info.nnz_reduced)
     K = k if k in fem.BLOCK_SIZES else min(b for b in fem.BLOCK_SIZES if b >= k)

@@ -136,14 +136,20 @@ __device__ __forceinline__ void vstore(float* p, const float (&v)[V]) {
 // acc[v] = sum over j in [j, e) step SUB of av[j] * (X[ci[j] * K + c0 + v] (- xi[v] when DIFF)),
 // folded in the plain loop's order. Entries go in batches of U: the U index/value loads, then the
 // U gathers, are all issued before the first use, so a typical row is two memory round trips.
-template <int K, int V, int SUB, bool DIFF, int U = 4>
+// Entries per lane per batch: a stiffness row (about 15 entries) in one batch for SUB <= 2.
+template <int SUB>
+constexpr int gather_batch() {
+    return SUB <= 2 ? 8 : (SUB == 4 ? 4 : 2);
+}
+
+template <int K, int V, int SUB, bool DIFF, int U = gather_batch<SUB>()>
 __device__ __forceinline__ void row_gather_v(const std::int32_t* __restrict__ ci, const float* __restrict__ av, int j,
                                              int e, const float* __restrict__ X, int c0, const float (&xi)[V],
                                              float (&acc)[V]) {
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v] = 0.0f;
     for (; j < e; j += U * SUB) {
-        std::int64_t cc[U];
+        int cc[U];
         float aa[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -155,7 +161,7 @@ __device__ __forceinline__ void row_gather_v(const std::int32_t* __restrict__ ci
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (j + u * SUB < e) {
-                vload<V>(X + cc[u] * K + c0, xx[u]);
+                vload<V>(X + static_cast<std::int64_t>(cc[u]) * K + c0, xx[u]);
             } else {
 #pragma unroll
                 for (int v = 0; v < V; ++v) xx[u][v] = 0.0f;
@@ -235,13 +241,14 @@ __global__ void __launch_bounds__(kRowThreads) rowk_kernel(
     std::int64_t n, const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
     const float* __restrict__ av, const float* __restrict__ diag, const float* __restrict__ b,
     const float* __restrict__ x, const float* x_prev, float c_cur, float c_prev, float alpha, float* out,
-    const int* __restrict__ state) {
+    const int* __restrict__ state, unsigned long long* __restrict__ stamps) {
     if (finished(state)) return;
     constexpr int V = vec_width<K>();
     constexpr int G = K / V;
     constexpr int L = G * SUB;
     constexpr int RPW = kWave / L;
     const int lb = logical_block();
+    if (stamps != nullptr && threadIdx.x == 0) stamps[2 * lb] = wall_clock64();
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
     const int l = lane % L, grp = l % G, sub = l / G;
     const int c0 = grp * V;
@@ -250,12 +257,28 @@ __global__ void __launch_bounds__(kRowThreads) rowk_kernel(
     for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW; r0 < hi; r0 += kRowWaves * RPW) {
         const std::int64_t row = r0 + lane / L;
         const bool valid = row < hi;
+        const bool lead = valid && sub == 0;
         const std::int64_t at = row * K + c0;
         float xi[V], acc[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) xi[v] = 0.0f;
         if constexpr (MODE == kSpmmDiff || MODE == kSpmmDiffMulti) {
             if (valid) vload<V>(x + at, xi);
+        }
+        // The row's own operands go out before the gather, so their latency hides behind it
+        // (x_prev is read before the store below either way: out may alias it).
+        float bb[V], cur[V], prev[V], sv[V];
+        float dg = 0.0f;
+#pragma unroll
+        for (int v = 0; v < V; ++v) bb[v] = cur[v] = prev[v] = sv[v] = 0.0f;
+        if (lead) {
+            if constexpr (MODE == kResidual || MODE == kCheby) vload<V>(b + at, bb);
+            if constexpr (MODE == kCheby) {
+                vload<V>(x + at, cur);
+                vload<V>(x_prev + at, prev);
+            }
+            if constexpr (MODE == kCheby || MODE == kSpmmDiff) dg = diag[row];
+            if constexpr (MODE == kSpmmDiffMulti) vload<V>(diag + at, sv);
         }
         if (valid) {
             if constexpr (MODE == kSpmmDiffMulti)
@@ -271,36 +294,66 @@ __global__ void __launch_bounds__(kRowThreads) rowk_kernel(
 #pragma unroll
             for (int v = 0; v < V; ++v) acc[v] += __shfl_xor(acc[v], off, kWave);
         }
-        if (!valid || sub != 0) continue;
+        if (!lead) continue;
         float o[V];
         if constexpr (MODE == kSpmm) {
 #pragma unroll
             for (int v = 0; v < V; ++v) o[v] = acc[v];
         } else if constexpr (MODE == kSpmmDiff) {
-            const float s = diag[row];
 #pragma unroll
-            for (int v = 0; v < V; ++v) o[v] = fmaf(s, xi[v], acc[v]);
+            for (int v = 0; v < V; ++v) o[v] = fmaf(dg, xi[v], acc[v]);
         } else if constexpr (MODE == kSpmmDiffMulti) {
-            float sv[V];
-            vload<V>(diag + at, sv);
 #pragma unroll
             for (int v = 0; v < V; ++v) o[v] = fmaf(sv[v], xi[v], acc[v]);
+        } else if constexpr (MODE == kResidual) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) o[v] = bb[v] - acc[v];
         } else {
-            float bb[V];
-            vload<V>(b + at, bb);
-            if constexpr (MODE == kResidual) {
+            const float d = alpha * dg;
 #pragma unroll
-                for (int v = 0; v < V; ++v) o[v] = bb[v] - acc[v];
-            } else {
-                float cur[V], prev[V];
-                vload<V>(x + at, cur);
-                vload<V>(x_prev + at, prev);  // read before the store below: out may alias x_prev
-                const float d = alpha * diag[row];
-#pragma unroll
-                for (int v = 0; v < V; ++v) o[v] = fmaf(d, bb[v] - acc[v], fmaf(c_prev, prev[v], c_cur * cur[v]));
-            }
+            for (int v = 0; v < V; ++v) o[v] = fmaf(d, bb[v] - acc[v], fmaf(c_prev, prev[v], c_cur * cur[v]));
         }
         vstore<V>(out + at, o);
+    }
+    if (stamps != nullptr) {  // the block's end: after its last wavefront's stores
+        __syncthreads();
+        if (threadIdx.x == 0) stamps[2 * lb + 1] = wall_clock64();
+    }
+}
+
+// In-solve timing on the device wall clock. A stamp is 0 when the solve had already finished, so
+// the accumulator counts only work that ran.
+__global__ void stamp_kernel(unsigned long long* __restrict__ slot, const int* __restrict__ state) {
+    *slot = finished(state) ? 0ull : wall_clock64();
+}
+
+// sums[0] += SpMM span (earliest block start to latest block end over `blocks` block stamps),
+// sums[1] += V-cycle span, sums[2] += iteration span, sums[3] += 1, in wall-clock ticks; only for
+// iterations that ran in full.
+__global__ void __launch_bounds__(kBlock) timing_accumulate_kernel(const unsigned long long* __restrict__ spmm,
+                                                                   int blocks, const unsigned long long* __restrict__ marks,
+                                                                   double* __restrict__ sums) {
+    __shared__ unsigned long long lo[kBlock], hi[kBlock];
+    unsigned long long a = ~0ull, b = 0ull;
+    for (int i = threadIdx.x; i < blocks; i += kBlock) {
+        a = min(a, spmm[2 * i]);
+        b = max(b, spmm[2 * i + 1]);
+    }
+    lo[threadIdx.x] = a;
+    hi[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            lo[threadIdx.x] = min(lo[threadIdx.x], lo[threadIdx.x + w]);
+            hi[threadIdx.x] = max(hi[threadIdx.x], hi[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && marks[0] != 0 && marks[1] != 0 && marks[2] != 0 && marks[3] != 0 && hi[0] > lo[0]) {
+        sums[0] += static_cast<double>(hi[0] - lo[0]);
+        sums[1] += static_cast<double>(marks[2] - marks[1]);
+        sums[2] += static_cast<double>(marks[3] - marks[0]);
+        sums[3] += 1.0;
     }
 }
 
@@ -937,15 +990,18 @@ constexpr int col_groups() {
 // rowk_kernel<K, sub, MODE> over the rows of `a`; `diag` is dinv (kCheby) or the row sums
 // (kSpmmDiff).
 template <int K, int MODE>
-void launch_rows(int sub, const CsrView& a, const float* diag, const float* b, const float* x, const float* xp,
-                 float cc, float cp, float al, float* out, const int* state, hipStream_t s) {
+unsigned launch_rows(int sub, const CsrView& a, const float* diag, const float* b, const float* x, const float* xp,
+                     float cc, float cp, float al, float* out, const int* state, hipStream_t s,
+                     unsigned long long* stamps = nullptr) {
+    unsigned grid = 0;
     auto go = [&](auto subc) {
         constexpr int SUB = decltype(subc)::value;
         if constexpr (col_groups<K>() * SUB <= kWave) {
             const unsigned g = xcd_grid((a.n + rows_per_vblock<K, SUB>() - 1) / rows_per_vblock<K, SUB>());
+            grid = g;
             if (g)
-                rowk_kernel<K, SUB, MODE>
-                    <<<g, kRowThreads, 0, s>>>(a.n, a.ptr, a.col, a.val, diag, b, x, xp, cc, cp, al, out, state);
+                rowk_kernel<K, SUB, MODE><<<g, kRowThreads, 0, s>>>(a.n, a.ptr, a.col, a.val, diag, b, x, xp, cc, cp,
+                                                                    al, out, state, stamps);
         } else {
             throw InvalidArgument("sub-lanes per column group exceed a wavefront");
         }
@@ -959,6 +1015,7 @@ void launch_rows(int sub, const CsrView& a, const float* diag, const float* b, c
         case 32: go(std::integral_constant<int, 32>{}); break;
         default: go(std::integral_constant<int, 64>{}); break;
     }
+    return grid;
 }
 
 // One zero-guess V-cycle (solver/amg/vcycle.cu:560-648). `state` lets captured replays skip it.
@@ -1047,8 +1104,6 @@ BlockSolver::BlockSolver(const Operator& op, const Hierarchy& h, hipStream_t s) 
 
 BlockSolver::~BlockSolver() {
     drop_graphs();
-    for (auto& e : ev_)
-        if (e) (void)hipEventDestroy(e);
     if (host_state_) (void)hipHostFree(host_state_);
 }
 
@@ -1099,42 +1154,43 @@ void BlockSolver::ensure(int K) {
     drop_graphs();  // the captured iterations named the buffers just replaced
 }
 
-void BlockSolver::mark(int i) {
-    if (timing_) check(hipEventRecord(ev_[i], s_), "timing event");
+std::size_t BlockSolver::stamp_marks() const {
+    // room for two stamps per block of the finest row kernel at any tile (16 rows per block at least)
+    return 2 * static_cast<std::size_t>(n() / 16 + 64);
 }
 
-void BlockSolver::sample() {
-    float ms[3] = {};
-    check(hipEventElapsedTime(&ms[0], ev_[0], ev_[1]), "elapsed");
-    check(hipEventElapsedTime(&ms[1], ev_[2], ev_[3]), "elapsed");
-    check(hipEventElapsedTime(&ms[2], ev_[0], ev_[4]), "elapsed");
-    for (int i = 0; i < 3; ++i) t_sum_[i] += ms[i];
-    ++t_n_;
+unsigned long long* BlockSolver::stamps() { return timing_ ? stamps_.get() : nullptr; }
+
+void BlockSolver::mark(int i, const int* state) {
+    if (timing_) stamp_kernel<<<1, 1, 0, s_>>>(stamps_.get() + stamp_marks() + i, state);
 }
 
 void BlockSolver::set_timing(bool on) {
-    t_n_ = 0;
-    t_sum_[0] = t_sum_[1] = t_sum_[2] = 0.0;
-    if (on == timing_) return;
-    drop_graphs();  // the captured iterations are recorded with or without the events
-    if (on && ev_[0] == nullptr)
-        for (auto& e : ev_) check(hipEventCreate(&e), "timing event");
-    if (on) {
-        // An event record must be capturable into a graph on this runtime; find out before a
-        // solve's capture depends on it.
-        hipGraph_t g = nullptr;
-        check(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal), "begin capture");
-        const hipError_t rec = hipEventRecord(ev_[0], s_);
-        const hipError_t end = hipStreamEndCapture(s_, &g);
-        if (g) (void)hipGraphDestroy(g);
-        require(rec == hipSuccess && end == hipSuccess, "event records cannot be captured on this runtime");
+    if (on && stamps_.get() == nullptr) {
+        stamps_.reset(stamp_marks() + 4, "timing stamps");
+        t_sums_.reset(4, "timing sums");
+        int dev = 0, khz = 0;
+        check(hipGetDevice(&dev), "device");
+        check(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "wall clock rate");
+        require(khz > 0, "the device reports no wall-clock rate");
+        clock_khz_ = khz;
     }
+    if (t_sums_.get() != nullptr) {
+        zero_f64_kernel<<<1, kBlock, 0, s_>>>(4, t_sums_.get());
+        sync(s_, "timing reset");
+    }
+    if (on != timing_) drop_graphs();  // the captured iterations are recorded with or without the stamps
     timing_ = on;
 }
 
 void BlockSolver::timing(double* out) const {
-    out[0] = static_cast<double>(t_n_);
-    for (int i = 0; i < 3; ++i) out[1 + i] = t_n_ ? t_sum_[i] / static_cast<double>(t_n_) : 0.0;
+    double sums[4] = {};
+    if (t_sums_.get() != nullptr) {
+        d2h(sums, t_sums_.get(), 4, s_);
+        sync(s_, "timing read");
+    }
+    out[0] = sums[3];
+    for (int i = 0; i < 3; ++i) out[1 + i] = sums[3] > 0 ? sums[i] / sums[3] / clock_khz_ : 0.0;
 }
 
 void BlockSolver::set_fine_sub(int sub) {
@@ -1169,13 +1225,12 @@ void BlockSolver::spmm_block_k(const int* state) {
         require(multi_->width == K, "multi-matrix values were prepared for another block width");
         CsrView off = op_.off;
         off.val = multi_->off_vals;
-        launch_rows<K, kSpmmDiffMulti>(fine_sub<K>(), off, multi_->rowsum, nullptr, W_.get(), nullptr, 0.f, 0.f, 0.f,
-                                       S_.get(), state, s_);
+        spmm_grid_ = launch_rows<K, kSpmmDiffMulti>(fine_sub<K>(), off, multi_->rowsum, nullptr, W_.get(), nullptr,
+                                                    0.f, 0.f, 0.f, S_.get(), state, s_, stamps());
     } else {
-        launch_rows<K, kSpmmDiff>(fine_sub<K>(), op_.off, op_.rowsum, nullptr, W_.get(), nullptr, 0.f, 0.f, 0.f,
-                                  S_.get(), state, s_);
+        spmm_grid_ = launch_rows<K, kSpmmDiff>(fine_sub<K>(), op_.off, op_.rowsum, nullptr, W_.get(), nullptr, 0.f,
+                                               0.f, 0.f, S_.get(), state, s_, stamps());
     }
-    mark(1);
     gram_rows_kernel<K, true><<<kGramBlocks, gram_threads<K, true>(), 0, s_>>>(nn, W_.get(), S_.get(), R_.get(),
                                                                         partials_.get(), gram_rows(nn), state);
 }
@@ -1186,23 +1241,26 @@ void BlockSolver::iteration_k(const SolveOptions& opt) {
     const std::int64_t rpb = gram_rows(nn);
     const int* st = state_.get();
     int* stw = state_.get();
-    mark(0);
+    mark(0, st);
     spmm_block_k<K>(st);
     coef_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, opt.drop_tol, G_.get(), T_.get(),
                                                 TT_.get(), AL_.get(), stw, multi_ ? 1 : 0);
     update_xr_kernel<K><<<kGramBlocks, xr_threads<K>(), 0, s_>>>(nn, W_.get(), S_.get(), AL_.get(), X64_.get(), R_.get(),
                                                              partials_.get(), rpb, st);
     converge_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, target_.get(), norms_.get(), stw);
-    mark(2);
+    mark(1, st);
     vcycle_k<K>(R_.get(), Z_.get(), st);
-    mark(3);
+    mark(2, st);
     gram_rows_kernel<K, false><<<kGramBlocks, gram_threads<K, false>(), 0, s_>>>(nn, S_.get(), Z_.get(), nullptr,
                                                                          partials_.get(), rpb, st);
     dmat_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, TT_.get(), D_.get(), stw,
                                                 multi_ ? 1 : 0);
     update_w_kernel<K><<<grid_for(nn * col_groups<K>(), kRowThreads), kRowThreads, 0, s_>>>(nn, Z_.get(), D_.get(),
                                                                                            W_.get(), st);
-    mark(4);
+    mark(3, st);
+    if (timing_)
+        timing_accumulate_kernel<<<1, kBlock, 0, s_>>>(stamps_.get(), static_cast<int>(spmm_grid_),
+                                                       stamps_.get() + stamp_marks(), t_sums_.get());
 }
 
 template <int K>
@@ -1293,7 +1351,6 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
             d2h(host_state_, state_.get(), 2 + 2 * K, s_);
             sync(s_, "convergence read");
             total_iters = host_state_[1];
-            if (timing_ && total_iters == launched) sample();  // the batch's last iteration ran in full
             if (host_state_[0]) {
                 done = true;
                 break;

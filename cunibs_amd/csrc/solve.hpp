@@ -100,11 +100,12 @@ class BlockSolver {
     int fine_sub_setting() const { return fine_sub_set_; }
     // Forget every captured iteration (after a change to the hierarchy's smoother).
     void invalidate() { drop_graphs(); }
-    // In-solve timing: with it on, each iteration (captured or not) records HIP events around the
-    // differenced SpMM, the V-cycle and the whole iteration, and every batch whose last iteration
-    // ran in full adds one sample. Switching drops the captured iterations and resets the sums.
+    // In-solve timing on the device wall clock: with it on, the differenced SpMM's blocks stamp
+    // their start and end, stamp kernels mark the V-cycle and the iteration, and an accumulator
+    // kernel at the end of each iteration (captured with it) adds the spans of every iteration
+    // that ran in full. Switching drops the captured iterations; every call resets the sums.
     void set_timing(bool on);
-    // out[0] samples, out[1..3] mean ms of the SpMM, the V-cycle and the iteration.
+    // out[0] iterations sampled, out[1..3] mean ms of the SpMM, the V-cycle and the iteration.
     void timing(double* out) const;
 
   private:
@@ -143,11 +144,13 @@ class BlockSolver {
     // iteration names is solver-owned and reallocated only when the capacity grows (which drops
     // them all).
     bool timing_ = false;
-    hipEvent_t ev_[5] = {};
-    double t_sum_[3] = {};
-    std::int64_t t_n_ = 0;
-    void mark(int i);
-    void sample();
+    DeviceBuffer<unsigned long long> stamps_;  // SpMM block stamps, then 4 marks
+    DeviceBuffer<double> t_sums_;
+    double clock_khz_ = 1.0;
+    unsigned spmm_grid_ = 0;
+    std::size_t stamp_marks() const;
+    unsigned long long* stamps();
+    void mark(int i, const int* state);
     hipGraphExec_t graphs_[kNumBlockWidths] = {};
     double graph_drop_[kNumBlockWidths] = {};  // the drop tolerance each graph was captured with
     void drop_graphs();

@@ -465,7 +465,7 @@ class SolverContext:
         return float(ms.value)
 
     def timing_enable(self, on: bool = True) -> None:
-        """Record HIP events around the SpMM, the V-cycle and each iteration of later solves."""
+        """Time the SpMM, the V-cycle and each iteration of later solves on the device clock."""
         check(lib().cn_timing_enable(self.handle, 1 if on else 0), "cn_timing_enable")
 
     def timing_read(self) -> dict:

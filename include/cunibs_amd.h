@@ -298,13 +298,14 @@ int32_t cn_bench_spmm(cn_context* ctx, int32_t K, int32_t reps, int32_t mode, in
 /* Same for one V-cycle at width K. */
 int32_t cn_bench_vcycle(cn_context* ctx, int32_t K, int32_t reps, double* ms);
 /* In-solve kernel timing, a measurement aid with no counterpart in the reference. When on, every
- * block-PCG iteration (inside its captured graph) records HIP events on the library's stream
- * around the differenced SpMM, the V-cycle and the whole iteration; each convergence check whose
- * last iteration ran in full adds one sample. Switching on or off resets the sums and drops the
- * captured iterations. Fails with CN_EINVAL if the runtime cannot capture event records. */
+ * block-PCG iteration (captured in its graph) times itself on the device wall clock: the
+ * differenced SpMM's blocks stamp their start and end, one-thread stamp launches mark the V-cycle
+ * and the iteration, and an accumulator launch at the end of the iteration adds the spans of
+ * every iteration that ran in full (HIP events recorded inside a captured graph cannot be timed
+ * on this runtime). Every call resets the sums; switching drops the captured iterations. */
 int32_t cn_timing_enable(cn_context* ctx, int32_t on);
-/* out[0] samples, out[1] mean ms of the differenced SpMM, out[2] of the V-cycle, out[3] of the
- * whole iteration. */
+/* out[0] iterations sampled, out[1] mean ms of the differenced SpMM (earliest block start to
+ * latest block end), out[2] of the V-cycle, out[3] of the whole iteration. */
 int32_t cn_timing_read(const cn_context* ctx, double* out4);
 /* Average device time (ms) of a streaming device-to-device copy of `bytes` bytes (a multiple of 16)
  * on the library's stream: the calibration launch for the HBM traffic counters and the practical

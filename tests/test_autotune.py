@@ -101,8 +101,8 @@ def test_autotuned_context_gives_the_same_field(tctx, octx, oracle_mod, coil):
 
 
 def test_in_solve_timing_leaves_the_answer_bitwise_unchanged(tctx, coil):
-    """Event records inside the captured iteration time the SpMM, the V-cycle and the iteration
-    without touching the numerics."""
+    """Device-clock stamps inside the captured iteration time the SpMM, the V-cycle and the
+    iteration without touching the numerics."""
     from cunibs_amd import fem, synthetic
     sites = synthetic.cap_sites(8, seed=21)
     base = fem.solve_placements_block(tctx, coil.positions_m, coil.moments, sites, 1e6)
