@@ -105,6 +105,7 @@ _SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "cn_uq_prepare": (_I32, [_P, _P, _P, _I64, _P, _P, _D, _D, _P, _P]),
     "cn_uq_solve": (_I32, [_P, _P, _I32, _P, _P]),
     "cn_uq_moments": (_I32, [_P, _P, _P, _P]),
+    "cn_uq_guess_info": (_I32, [_P, _P]),
     "cn_mlp_create": (_I32, [_I32, _I32, _I32, ctypes.c_uint32, _P]),
     "cn_mlp_destroy": (None, [_P]),
     "cn_mlp_n_params": (_I32, [_P, _P]),

@@ -95,6 +95,7 @@ struct cn_context {
     std::unique_ptr<cn::RecoveryOp> recovery[4];
     cn::DeviceBuffer<float> d_slots, d_Eraw;
     int max_tag = 0;
+    std::vector<double> lut;  // conductivity by tag (S/m)
     cn::WarmBasis warm;
     std::vector<double> warm_coef;  // coefficients for the next warm_start = 2 solve
     int warm_coef_k = 0;
@@ -109,6 +110,12 @@ struct cn_context {
         cn::DeviceBuffer<float> off_k, rowsum_k;      // (nnz_off, K), (n_red, K)
         cn::DeviceBuffer<double> sum, sumsq;          // |E| moments over every sample solved
         std::int64_t count = 0;
+        // Initial guess of the draws: x_nom (n_red), sensitivities S (n_red, S_width) and the map
+        // from a draw's sigma to its Galerkin coefficients, M (T x T, rows of dropped directions 0).
+        cn::DeviceBuffer<double> x_nom, S;
+        int S_width = 0;
+        std::vector<double> guess_M;
+        int nominal_iterations = 0, sensitivity_iterations = 0, guess_rank = 0;
         double transform[16] = {};
         cn::MultiOperator mop;
     } uq;
@@ -238,6 +245,8 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
     const int unknown = conductivity(c.tags.get(), m, lut.data(), static_cast<int>(lut.size()), cond.get(), s);
     if (unknown) throw InvalidArgument(std::to_string(unknown) + " tetrahedra have unknown tags");
     c.max_tag = static_cast<int>(lut.size()) - 1;  // every tag indexes the table
+    c.opts.conductivity = nullptr;  // the caller's table need not outlive this call; keep a copy
+    c.lut = lut;
 
     c.n2c_ptr.reset(n + 1, "n2c ptr");
     c.n2c_idx.reset(4 * m, "n2c idx");
@@ -394,6 +403,116 @@ void place_block(cn_context& c, const double* centers, const double* handles, co
 
 // Per-tissue unit-conductivity operators on the reduced pattern and per-tissue RHS of one placement
 // (uq/conductivity/assembly.py:37-146): K(sigma) = sum_t sigma_t K_t, b(sigma) = sum_t sigma_t b_t.
+// Pseudo-inverse solve of a small SPD matrix by diagonally pivoted Cholesky on its unit-diagonal
+// scaling: directions whose pivot falls below `drop` are dropped. Returns the kept rank and
+// writes X (T x P) = E^+ C for C (T x P), zero rows for dropped directions.
+int pivoted_solve(const std::vector<double>& E, int T, const std::vector<double>& C, int P, double drop,
+                  std::vector<double>& X) {
+    std::vector<double> d(T, 0.0), a(static_cast<std::size_t>(T) * T, 0.0);
+    for (int i = 0; i < T; ++i) d[i] = E[i * T + i] > 0.0 ? 1.0 / std::sqrt(E[i * T + i]) : 0.0;
+    for (int i = 0; i < T; ++i)
+        for (int j = 0; j < T; ++j) a[i * T + j] = d[i] * E[i * T + j] * d[j];
+    std::vector<int> order;
+    std::vector<char> used(T, 0);
+    std::vector<double> L(static_cast<std::size_t>(T) * T, 0.0);  // rows: kept directions in pivot order
+    for (int step = 0; step < T; ++step) {
+        int best = -1;
+        double bv = drop;
+        for (int i = 0; i < T; ++i) {
+            if (used[i] || d[i] == 0.0) continue;
+            double v = a[i * T + i];
+            for (int r = 0; r < step; ++r) v -= L[r * T + i] * L[r * T + i];
+            if (v > bv) {
+                bv = v;
+                best = i;
+            }
+        }
+        if (best < 0) break;
+        used[best] = 1;
+        const double piv = std::sqrt(bv);
+        for (int i = 0; i < T; ++i) {
+            if (used[i] && i != best) continue;
+            double v = a[best * T + i];
+            for (int r = 0; r < step; ++r) v -= L[r * T + best] * L[r * T + i];
+            L[step * T + i] = v / piv;
+        }
+        order.push_back(best);
+    }
+    const int R = static_cast<int>(order.size());
+    X.assign(static_cast<std::size_t>(T) * P, 0.0);
+    // Solve (L_k L_k^T) y = D C restricted to the kept directions, in pivot coordinates.
+    for (int p = 0; p < P; ++p) {
+        std::vector<double> y(R);
+        for (int r = 0; r < R; ++r) {
+            double v = d[order[r]] * C[order[r] * P + p];
+            for (int q = 0; q < r; ++q) v -= L[q * T + order[r]] * y[q];
+            y[r] = v / L[r * T + order[r]];
+        }
+        for (int r = R - 1; r >= 0; --r) {
+            double v = y[r];
+            for (int q = r + 1; q < R; ++q) v -= L[r * T + order[q]] * y[q];
+            y[r] = v / L[r * T + order[r]];
+        }
+        for (int r = 0; r < R; ++r) X[order[r] * P + p] = d[order[r]] * y[r];
+    }
+    return R;
+}
+
+// The reference's initial guess of every draw (uq/conductivity/run.py:100-200): solve the nominal
+// system once for x_nom; the sensitivities S_t solve A_nom S_t = b_t - K_t x_nom (tolerance 1e-3,
+// run.py:46); a draw's residual at x_nom is r0 = sum_t sigma_t (b_t - K_t x_nom) since both the
+// matrix and the RHS are linear in sigma, so the Galerkin correction on span(S) is
+// x0 = x_nom + S E^+ (S^T R) sigma with E = S^T A_nom S. S^T R and E are formed once; a draw's
+// coefficients are a T x T product on the host.
+void uq_initial_guess(cn_context& c) {
+    using namespace cn;
+    auto& u = c.uq;
+    const hipStream_t s = c.s;
+    const std::int64_t nr = c.n - 1;
+    const int T = static_cast<int>(u.tags.size());
+    SolveOptions opt;
+    // nominal solve, one column
+    std::vector<double> sig0(T);
+    for (int t = 0; t < T; ++t) sig0[t] = c.lut[u.tags[t]];
+    DeviceBuffer<double> dsig(T, "nominal sigma");
+    h2d(dsig.get(), sig0.data(), T, s);
+    c.d_B.ensure(nr * 1, "rhs block");
+    combine_tissues(u.rhs.get(), nr, T, dsig.get(), 1, 1, c.d_B.get(), s);
+    c.d_X.ensure(nr * 1, "solution");
+    SolveStats st = c.solver->solve(c.d_B.get(), c.d_X.get(), nullptr, 1, opt);
+    u.nominal_iterations = st.iterations;
+    u.x_nom.reset(nr, "nominal solution");
+    d2d(u.x_nom.get(), c.d_X.get(), nr, s);
+    // sensitivities
+    const int K = padded_width(T);
+    DeviceBuffer<double> R(static_cast<std::size_t>(nr) * K, "sensitivity rhs");
+    c.d_B.ensure(nr * K, "rhs block");
+    sensitivity_rhs(c.rptr.get(), c.rcol.get(), u.vals.get(), c.nnz_red, u.rhs.get(), u.x_nom.get(), nr, T, K, R.get(),
+                    c.d_B.get(), s);
+    std::vector<double> RR(static_cast<std::size_t>(T) * T);
+    gram64(R.get(), R.get(), nr, T, K, RR.data(), s);
+    u.S_width = K;
+    u.guess_M.assign(static_cast<std::size_t>(T) * T, 0.0);
+    u.guess_rank = 0;
+    for (int t = 0; t < T; ++t)
+        if (!(RR[t * T + t] > 0.0)) return;  // a tissue the field does not see: x_nom alone
+    SolveOptions sopt;
+    sopt.tolerance = 1e-3;
+    c.d_X.ensure(nr * K, "solution");
+    st = c.solver->solve(c.d_B.get(), c.d_X.get(), nullptr, T, sopt);
+    u.sensitivity_iterations = st.iterations;
+    u.S.reset(static_cast<std::size_t>(nr) * K, "sensitivities");
+    d2d(u.S.get(), c.d_X.get(), static_cast<std::size_t>(nr) * K, s);
+    DeviceBuffer<double> AS(static_cast<std::size_t>(nr) * K, "A S");
+    spmm64(c.rptr.get(), c.rcol.get(), c.rval64.get(), nr, u.S.get(), T, K, AS.get(), s);
+    std::vector<double> E(static_cast<std::size_t>(T) * T), C(static_cast<std::size_t>(T) * T);
+    gram64(u.S.get(), AS.get(), nr, T, K, E.data(), s);
+    gram64(u.S.get(), R.get(), nr, T, K, C.data(), s);
+    for (int i = 0; i < T; ++i)
+        for (int j = 0; j < i; ++j) E[i * T + j] = E[j * T + i] = 0.5 * (E[i * T + j] + E[j * T + i]);
+    u.guess_rank = pivoted_solve(E, T, C, T, 1e-12, u.guess_M);
+}
+
 void uq_prepare(cn_context& c, const double* pos_m, const double* mom, std::int64_t n_dip, const double* center,
                 const double* handle, double dist, double didt) {
     using namespace cn;
@@ -465,6 +584,7 @@ void uq_prepare(cn_context& c, const double* pos_m, const double* mom, std::int6
     check(hipMemsetAsync(u.sum.get(), 0, sizeof(double) * m, s), "moments clear");
     check(hipMemsetAsync(u.sumsq.get(), 0, sizeof(double) * m, s), "moments clear");
     sync(s, "uq prepare");
+    uq_initial_guess(c);
     u.ready = true;
 }
 
@@ -506,10 +626,28 @@ void uq_solve(cn_context& c, const double* sigma, int k, const cn_solve_options*
         opt.check_every = std::max(1, sopt->check_every);
     }
     c.d_X.ensure(nr * K, "solution");
+    const double* dx0 = nullptr;
+    if (sopt && sopt->warm_start != 0) {
+        // x0 = x_nom + S M sigma per draw (M zero when the field sees some tissue not at all).
+        std::vector<double> coef(static_cast<std::size_t>(T) * K, 0.0);
+        for (int q = 0; q < T; ++q)
+            for (int col = 0; col < k; ++col) {
+                double v = 0.0;
+                for (int t = 0; t < T; ++t) v += u.guess_M[q * T + t] * sigma[col * T + t];
+                coef[q * K + col] = v;
+            }
+        DeviceBuffer<double> dcoef(coef.size(), "guess coefficients");
+        h2d(dcoef.get(), coef.data(), coef.size(), s);
+        c.d_x0.ensure(nr * K, "initial guess");
+        expand_guess(u.x_nom.get(), u.guess_rank > 0 ? u.S.get() : u.x_nom.get(), nr, u.guess_rank > 0 ? T : 0,
+                     u.S_width, dcoef.get(), k, K, c.d_x0.get(), s);
+        sync(s, "uq initial guess");
+        dx0 = c.d_x0.get();
+    }
     c.solver->set_multi(&u.mop);
     SolveStats st;
     try {
-        st = c.solver->solve(c.d_B.get(), c.d_X.get(), nullptr, k, opt);
+        st = c.solver->solve(c.d_B.get(), c.d_X.get(), dx0, k, opt);
     } catch (...) {
         c.solver->set_multi(nullptr);
         throw;
@@ -1401,6 +1539,16 @@ int32_t cn_recovery_patches(const cn_context* c, int64_t* nnz, int32_t* n_undete
         if (pidx) cn::d2h(pidx, op.pidx.get(), op.nnz_patch, c->s);
         if (w3) cn::d2h(w3, op.w3.get(), 3 * op.nnz_patch, c->s);
         cn::sync(c->s, "patch copy");
+    });
+}
+
+int32_t cn_uq_guess_info(const cn_context* c, int32_t* out) {
+    return guarded([&] {
+        cn::require(c != nullptr && out != nullptr, "null argument");
+        cn::require(c->uq.ready, "call cn_uq_prepare first");
+        out[0] = c->uq.nominal_iterations;
+        out[1] = c->uq.sensitivity_iterations;
+        out[2] = c->uq.guess_rank;
     });
 }
 

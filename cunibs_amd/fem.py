@@ -366,8 +366,10 @@ class SolverContext:
 
     def uq_solve(self, sigma: np.ndarray, *, magnitude: bool = True, vectors: bool = False, potential: bool = False,
                  metrics: bool = True, tolerance: float = DEFAULT_TOLERANCE,
-                 max_iters: int = DEFAULT_MAX_ITERS) -> list[PlacementResult]:
-        """k <= 16 conductivity samples (k x n_tissue, columns in uq_tags order)."""
+                 max_iters: int = DEFAULT_MAX_ITERS, initial_guess: bool = True) -> list[PlacementResult]:
+        """k <= 16 conductivity samples (k x n_tissue, columns in uq_tags order). With
+        ``initial_guess`` each draw starts from the nominal solution plus its Galerkin correction
+        on the sensitivity span (uq/conductivity/run.py:154-200), else from zero."""
         sg = np.ascontiguousarray(sigma, np.float64)
         if sg.ndim != 2 or sg.shape[1] != len(self.uq_tags):
             raise ValueError("sigma must be (k, n_tissue) in the order uq_prepare returned")
@@ -385,11 +387,19 @@ class SolverContext:
         opt = _lib.default_solve_options()
         opt.tolerance = float(tolerance)
         opt.max_iters = int(max_iters)
+        opt.warm_start = 1 if initial_guess else 0
         check(lib().cn_uq_solve(self.handle, ptr(sg), k, ctypes.byref(opt), ctypes.byref(out)), "cn_uq_solve")
         return [PlacementResult(transform=tf[i], magnE=None if mag is None else mag[i],
                                 E=None if E is None else E[i], v=None if v is None else v[i],
                                 metrics=None if met is None else met[i], iterations=int(it[i]),
                                 relative_residual=float(rel[i])) for i in range(k)]
+
+    def uq_guess_info(self) -> dict:
+        """Iterations of the nominal and sensitivity solves behind the draws' initial guess, and
+        the rank of its correction."""
+        out = np.zeros(3, np.int32)
+        check(lib().cn_uq_guess_info(self.handle, ptr(out)), "cn_uq_guess_info")
+        return {"nominal_iterations": int(out[0]), "sensitivity_iterations": int(out[1]), "rank": int(out[2])}
 
     def uq_moments(self) -> tuple[np.ndarray, np.ndarray, int]:
         cnt = ctypes.c_int64()

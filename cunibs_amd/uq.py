@@ -31,6 +31,9 @@ class UQResult:
     iterations: np.ndarray     # (n,)
     magnE_mean: np.ndarray     # (m,) per-element mean of |E|
     magnE_var: np.ndarray      # (m,) per-element sample variance
+    focality: np.ndarray | None = None       # (n,) GM volume (m^3) at |E| >= 0.5 x p99.9 per sample
+    peak_location: np.ndarray | None = None  # (n, 3) mm
+    guess: dict | None = None                # nominal / sensitivity iterations, correction rank
 
     @property
     def gm_peak_mean(self) -> float:
@@ -42,15 +45,18 @@ class UQResult:
 
 
 def run_conductivity_uq(ctx: fem.SolverContext, dip_pos_m, dip_moment, site, didt: float, n_samples: int, *,
-                        log_sd: float = 0.2, seed: int = 0, block: int = 16, sigma: np.ndarray | None = None
-                        ) -> UQResult:
+                        log_sd: float = 0.2, seed: int = 0, block: int = 16, sigma: np.ndarray | None = None,
+                        initial_guess: bool = True) -> UQResult:
     tags = ctx.uq_prepare(dip_pos_m, dip_moment, site, didt)
     if sigma is None:
         sigma = sample_conductivities(tags, n_samples, log_sd, seed)
-    peaks, iters = [], []
+    peaks, iters, foc, loc = [], [], [], []
     for i in range(0, sigma.shape[0], block):
-        res = ctx.uq_solve(sigma[i:i + block], magnitude=False)
+        res = ctx.uq_solve(sigma[i:i + block], magnitude=False, initial_guess=initial_guess)
         peaks.extend(float(r.metrics[0]) for r in res)
+        loc.extend(np.asarray(r.metrics[1:4], np.float64) for r in res)
+        foc.extend(float(r.metrics[8]) for r in res)
         iters.extend(r.iterations for r in res)
     mean, var, _ = ctx.uq_moments()
-    return UQResult(sigma, tags, np.array(peaks), np.array(iters), mean, var)
+    return UQResult(sigma, tags, np.array(peaks), np.array(iters), mean, var, np.array(foc), np.array(loc),
+                    ctx.uq_guess_info())

@@ -212,16 +212,22 @@ int32_t cn_warm_modes(const cn_context* ctx, uint16_t* out, int32_t* width);
 /* Conductivity uncertainty (uq/conductivity of the reference): for one placement, the per-tissue
  * unit-conductivity operators K_t and right-hand sides b_t (assembly.py:37-146), so that a sample
  * sigma gives K(sigma) = sum_t sigma_t K_t and b(sigma) = sum_t sigma_t b_t on the shared pattern.
- * cn_uq_prepare returns the tissue tags (ascending, at most 16) that sigma's columns follow;
+ * cn_uq_prepare returns the tissue tags (ascending, at most 16) that sigma's columns follow, and
+ * builds the draws' initial guess (run.py:100-200): the nominal solution x_nom, the sensitivities
+ * S_t (A_nom S_t = b_t - K_t x_nom, to 1e-3) and the Galerkin correction on their span.
  * cn_uq_solve solves k <= 16 samples (sigma: k x n_tissue) as independent CG chains under the
- * nominal AMG preconditioner, with the same outputs as cn_solve_block, and accumulates the |E|
- * moments that cn_uq_moments returns (per element mean and sample variance, FP64). */
+ * nominal AMG preconditioner, from x0 = x_nom + S E^+ S^T r0(sigma) when opt->warm_start != 0
+ * (from zero otherwise), with the same outputs as cn_solve_block, and accumulates the |E|
+ * moments that cn_uq_moments returns (per element mean and sample variance, FP64).
+ * cn_uq_guess_info: out[0] iterations of the nominal solve, out[1] of the sensitivity block,
+ * out[2] rank of the correction (0: x_nom alone). */
 int32_t cn_uq_prepare(cn_context* ctx, const double* dip_pos_m, const double* dip_moment, int64_t n_dip,
                       const double* center_mm, const double* handle_mm, double distance_mm, double didt,
                       int32_t* n_tissue, int32_t* tags);
 int32_t cn_uq_solve(cn_context* ctx, const double* sigma, int32_t k, const cn_solve_options* opt,
                     cn_block_outputs* out);
 int32_t cn_uq_moments(const cn_context* ctx, double* mean, double* var, int64_t* count);
+int32_t cn_uq_guess_info(const cn_context* ctx, int32_t* out3);
 
 /* The warm-start network: n_in (<= 16) inputs -> hidden -> hidden -> n_out, GELU, trained on the
  * GPU in FP32 (Adam, mean squared error), inference with BF16 weights on MFMA. No reference

@@ -50,5 +50,14 @@ def test_uq_samples_match_per_sample_fp64_solves(head4, coil, oracle_mod):
         stack = np.stack([r.magnE.astype(np.float64) for r in res])
         np.testing.assert_allclose(mean, stack.mean(axis=0), rtol=1e-12, atol=1e-12)
         np.testing.assert_allclose(var, stack.var(axis=0, ddof=1), rtol=1e-6, atol=1e-9 * stack.max() ** 2)
+        # The initial guess (run.py:154-200) changes where a draw starts, not where it lands: from
+        # zero the same draws reach the same field, in more iterations.
+        info = ctx.uq_guess_info()
+        assert info["rank"] == len(tags) and info["nominal_iterations"] > 0
+        cold = ctx.uq_solve(sigma, initial_guess=False)
+        for a, b in zip(res, cold):
+            assert np.linalg.norm(a.magnE - b.magnE) <= 2e-5 * np.linalg.norm(b.magnE)
+        assert sum(r.iterations for r in res) < sum(r.iterations for r in cold)
+        assert res[0].iterations <= 2  # the nominal draw starts at x_nom
     finally:
         ctx.free()
