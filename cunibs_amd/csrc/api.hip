@@ -100,10 +100,20 @@ struct cn_context {
     std::vector<double> warm_coef;  // coefficients for the next warm_start = 2 solve
     int warm_coef_k = 0;
     // conductivity uncertainty: per-tissue operators and RHS of one placement
-    struct Uq {
+    // Unit-conductivity decomposition of the reduced operator by tissue, K(sigma) = sum_t sigma_t K_t
+    // (uq/conductivity/assembly.py:37-146), built on first use by the conductivity-UQ and the
+    // per-placement-conductivity solves. Tags ascending, at most 16.
+    struct Tissues {
         bool ready = false;
         std::vector<int> tags;
         cn::DeviceBuffer<double> vals, off, rowsum;  // (T, nnz_red), (T, nnz_off), (T, n_red) FP64
+    } tissue;
+    // Per-placement conductivities of the current block: (K, T) sigma, combined operator.
+    cn::DeviceBuffer<double> blk_sigma;
+    cn::DeviceBuffer<float> blk_sigma_tag, blk_off, blk_rowsum;
+    cn::MultiOperator blk_mop;
+    struct Uq {
+        bool ready = false;
         cn::DeviceBuffer<float> rhs;                  // (T, n_red)
         cn::DeviceBuffer<float> dadt_elm1, dadt_nodes1;  // the placement's dA/dt, one column
         cn::DeviceBuffer<double> sigma;               // (K, T)
@@ -401,8 +411,54 @@ void place_block(cn_context& c, const double* centers, const double* handles, co
     sync(s, "placement");
 }
 
-// Per-tissue unit-conductivity operators on the reduced pattern and per-tissue RHS of one placement
-// (uq/conductivity/assembly.py:37-146): K(sigma) = sum_t sigma_t K_t, b(sigma) = sum_t sigma_t b_t.
+// Per-tissue unit-conductivity operators on the reduced pattern (uq/conductivity/assembly.py:37-146):
+// K(sigma) = sum_t sigma_t K_t, as FP64 values, differenced off-diagonals and row sums.
+void ensure_tissues(cn_context& c) {
+    using namespace cn;
+    if (c.tissue.ready) return;
+    const hipStream_t s = c.s;
+    const std::int64_t n = c.n, m = c.m, nr = n - 1, nnz_off = c.nnz_red - nr;
+    // tags present in the mesh, ascending
+    std::vector<std::int32_t> tags(m);
+    d2h(tags.data(), c.tags.get(), m, s);
+    sync(s, "tags");
+    std::vector<int> present;
+    {
+        std::vector<char> seen(c.max_tag + 1, 0);
+        for (auto t : tags) seen[t] = 1;
+        for (int t = 0; t <= c.max_tag; ++t)
+            if (seen[t]) present.push_back(t);
+    }
+    require(static_cast<int>(present.size()) <= kMaxTissues, "more than 16 tissues");
+    const int T = static_cast<int>(present.size());
+    auto& u = c.tissue;
+    u.tags = present;
+    u.vals.reset(static_cast<std::size_t>(T) * c.nnz_red, "tissue values");
+    u.off.reset(static_cast<std::size_t>(T) * std::max<std::int64_t>(nnz_off, 1), "tissue off-diagonals");
+    u.rowsum.reset(static_cast<std::size_t>(T) * nr, "tissue row sums");
+    // Stiffness per tissue: gradients and pattern again (the context keeps neither in FP64).
+    DeviceBuffer<double> g64(12 * m, "g64"), vols64(m, "vols64"), scale(m, "scale");
+    p1_gradients(c.nodes.get(), c.tets.get(), m, g64.get(), vols64.get(), s);
+    DeviceBuffer<std::int32_t> indptr, indices;
+    const std::int64_t nnz = stiffness_pattern(c.tets.get(), c.n2c_ptr.get(), c.n2c_idx.get(), n, indptr, indices, s);
+    DeviceBuffer<double> data(std::max<std::int64_t>(nnz, 1), "tissue stiffness");
+    DeviceBuffer<std::int32_t> rp, rc;
+    DeviceBuffer<double> rv64;
+    DeviceBuffer<float> rv32;
+    for (int q = 0; q < T; ++q) {
+        tissue_scale(vols64.get(), c.tags.get(), m, present[q], scale.get(), s);
+        stiffness_values(g64.get(), scale.get(), c.tets.get(), c.n2c_ptr.get(), c.n2c_idx.get(), n, indptr.get(),
+                         indices.get(), data.get(), s);
+        const std::int64_t nz = reduce_grounded(indptr.get(), indices.get(), data.get(), n, c.ground, rp, rc, rv64, rv32, s);
+        require(nz == c.nnz_red, "tissue operator pattern differs from the reduced operator");
+        d2d(u.vals.get() + static_cast<std::int64_t>(q) * c.nnz_red, rv64.get(), c.nnz_red, s);
+        split_offdiag(c.rptr.get(), c.rcol.get(), rv64.get(), nr, u.off.get() + static_cast<std::int64_t>(q) * nnz_off,
+                      u.rowsum.get() + static_cast<std::int64_t>(q) * nr, s);
+    }
+    sync(s, "tissue operators");
+    u.ready = true;
+}
+
 // Pseudo-inverse solve of a small SPD matrix by diagonally pivoted Cholesky on its unit-diagonal
 // scaling: directions whose pivot falls below `drop` are dropped. Returns the kept rank and
 // writes X (T x P) = E^+ C for C (T x P), zero rows for dropped directions.
@@ -469,11 +525,11 @@ void uq_initial_guess(cn_context& c) {
     auto& u = c.uq;
     const hipStream_t s = c.s;
     const std::int64_t nr = c.n - 1;
-    const int T = static_cast<int>(u.tags.size());
+    const int T = static_cast<int>(c.tissue.tags.size());
     SolveOptions opt;
     // nominal solve, one column
     std::vector<double> sig0(T);
-    for (int t = 0; t < T; ++t) sig0[t] = c.lut[u.tags[t]];
+    for (int t = 0; t < T; ++t) sig0[t] = c.lut[c.tissue.tags[t]];
     DeviceBuffer<double> dsig(T, "nominal sigma");
     h2d(dsig.get(), sig0.data(), T, s);
     c.d_B.ensure(nr * 1, "rhs block");
@@ -487,7 +543,7 @@ void uq_initial_guess(cn_context& c) {
     const int K = padded_width(T);
     DeviceBuffer<double> R(static_cast<std::size_t>(nr) * K, "sensitivity rhs");
     c.d_B.ensure(nr * K, "rhs block");
-    sensitivity_rhs(c.rptr.get(), c.rcol.get(), u.vals.get(), c.nnz_red, u.rhs.get(), u.x_nom.get(), nr, T, K, R.get(),
+    sensitivity_rhs(c.rptr.get(), c.rcol.get(), c.tissue.vals.get(), c.nnz_red, u.rhs.get(), u.x_nom.get(), nr, T, K, R.get(),
                     c.d_B.get(), s);
     std::vector<double> RR(static_cast<std::size_t>(T) * T);
     gram64(R.get(), R.get(), nr, T, K, RR.data(), s);
@@ -518,45 +574,12 @@ void uq_prepare(cn_context& c, const double* pos_m, const double* mom, std::int6
     using namespace cn;
     const hipStream_t s = c.s;
     const std::int64_t n = c.n, m = c.m, nr = n - 1, nnz_off = c.nnz_red - nr;
-    // tags present in the mesh, ascending
-    std::vector<std::int32_t> tags(m);
-    d2h(tags.data(), c.tags.get(), m, s);
-    sync(s, "tags");
-    std::vector<int> present;
-    {
-        std::vector<char> seen(c.max_tag + 1, 0);
-        for (auto t : tags) seen[t] = 1;
-        for (int t = 0; t <= c.max_tag; ++t)
-            if (seen[t]) present.push_back(t);
-    }
-    require(static_cast<int>(present.size()) <= kMaxTissues, "more than 16 tissues");
+    ensure_tissues(c);
+    const std::vector<int>& present = c.tissue.tags;
     const int T = static_cast<int>(present.size());
     auto& u = c.uq;
     u = cn_context::Uq{};
-    u.tags = present;
-    u.vals.reset(static_cast<std::size_t>(T) * c.nnz_red, "tissue values");
-    u.off.reset(static_cast<std::size_t>(T) * std::max<std::int64_t>(nnz_off, 1), "tissue off-diagonals");
-    u.rowsum.reset(static_cast<std::size_t>(T) * nr, "tissue row sums");
     u.rhs.reset(static_cast<std::size_t>(T) * nr, "tissue rhs");
-    // Stiffness per tissue: gradients and pattern again (the context keeps neither in FP64).
-    DeviceBuffer<double> g64(12 * m, "g64"), vols64(m, "vols64"), scale(m, "scale");
-    p1_gradients(c.nodes.get(), c.tets.get(), m, g64.get(), vols64.get(), s);
-    DeviceBuffer<std::int32_t> indptr, indices;
-    const std::int64_t nnz = stiffness_pattern(c.tets.get(), c.n2c_ptr.get(), c.n2c_idx.get(), n, indptr, indices, s);
-    DeviceBuffer<double> data(std::max<std::int64_t>(nnz, 1), "tissue stiffness");
-    DeviceBuffer<std::int32_t> rp, rc;
-    DeviceBuffer<double> rv64;
-    DeviceBuffer<float> rv32;
-    for (int q = 0; q < T; ++q) {
-        tissue_scale(vols64.get(), c.tags.get(), m, present[q], scale.get(), s);
-        stiffness_values(g64.get(), scale.get(), c.tets.get(), c.n2c_ptr.get(), c.n2c_idx.get(), n, indptr.get(),
-                         indices.get(), data.get(), s);
-        const std::int64_t nz = reduce_grounded(indptr.get(), indices.get(), data.get(), n, c.ground, rp, rc, rv64, rv32, s);
-        require(nz == c.nnz_red, "tissue operator pattern differs from the reduced operator");
-        d2d(u.vals.get() + static_cast<std::int64_t>(q) * c.nnz_red, rv64.get(), c.nnz_red, s);
-        split_offdiag(c.rptr.get(), c.rcol.get(), rv64.get(), nr, u.off.get() + static_cast<std::int64_t>(q) * nnz_off,
-                      u.rowsum.get() + static_cast<std::int64_t>(q) * nr, s);
-    }
     // The placement's dA/dt (one column), then the RHS of each tissue with unit conductivity.
     std::vector<double> tf(16);
     std::vector<std::int32_t> degen(1);
@@ -595,7 +618,7 @@ void uq_solve(cn_context& c, const double* sigma, int k, const cn_solve_options*
     auto& u = c.uq;
     require(u.ready, "call cn_uq_prepare first");
     require(k >= 1 && k <= 16, "k must be in [1, 16]");
-    const int T = static_cast<int>(u.tags.size());
+    const int T = static_cast<int>(c.tissue.tags.size());
     for (int i = 0; i < k * T; ++i) require(std::isfinite(sigma[i]) && sigma[i] > 0.0, "conductivities must be positive");
     const hipStream_t s = c.s;
     const int K = padded_width(k);
@@ -607,13 +630,13 @@ void uq_solve(cn_context& c, const double* sigma, int k, const cn_solve_options*
     u.off_k.ensure(static_cast<std::size_t>(std::max<std::int64_t>(nnz_off, 1)) * K, "sample off-diagonals");
     u.rowsum_k.ensure(static_cast<std::size_t>(nr) * K, "sample row sums");
     c.d_B.ensure(nr * K, "rhs block");
-    combine_tissues(u.off.get(), nnz_off, T, u.sigma.get(), k, K, u.off_k.get(), s);
-    combine_tissues(u.rowsum.get(), nr, T, u.sigma.get(), k, K, u.rowsum_k.get(), s);
+    combine_tissues(c.tissue.off.get(), nnz_off, T, u.sigma.get(), k, K, u.off_k.get(), s);
+    combine_tissues(c.tissue.rowsum.get(), nr, T, u.sigma.get(), k, K, u.rowsum_k.get(), s);
     combine_tissues(u.rhs.get(), nr, T, u.sigma.get(), k, K, c.d_B.get(), s);
     u.mop.width = K;
     u.mop.off_vals = u.off_k.get();
     u.mop.rowsum = u.rowsum_k.get();
-    u.mop.tissue_vals = u.vals.get();
+    u.mop.tissue_vals = c.tissue.vals.get();
     u.mop.n_tissue = T;
     u.mop.sigma = u.sigma.get();
     SolveOptions opt;
@@ -759,8 +782,44 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
     const float scale = static_cast<float>(didt) * static_cast<float>(kMu0Over4Pi);
     dadt_nbody(c.d_dip.get(), static_cast<int>(n_dip), c.nodes.get(), n, c.d_origins.get(), k, K, scale,
                c.d_dadt_nodes.get(), s);
+    // Per-placement conductivities: the tissue decomposition of the operator and a (K, n_sigma)
+    // table for the RHS; padded columns take the context's table.
+    const bool per_sigma = sopt != nullptr && sopt->sigma != nullptr;
+    if (per_sigma) {
+        ensure_tissues(c);
+        const auto& tg = c.tissue.tags;
+        const int T = static_cast<int>(tg.size());
+        const int ns = sopt->n_sigma;
+        require(ns > tg.back(), "sigma rows must cover every tissue tag of the mesh (n_sigma > largest tag)");
+        std::vector<float> tab(static_cast<std::size_t>(K) * ns, 0.0f);
+        std::vector<double> sg(static_cast<std::size_t>(K) * T, 0.0);
+        for (int col = 0; col < K; ++col) {
+            for (int t = 0; t < T; ++t) {
+                const double v = col < k ? sopt->sigma[static_cast<std::size_t>(col) * ns + tg[t]] : c.lut[tg[t]];
+                require(std::isfinite(v) && v > 0.0, "conductivities of the tissues present must be positive");
+                sg[static_cast<std::size_t>(col) * T + t] = v;
+                tab[static_cast<std::size_t>(col) * ns + tg[t]] = static_cast<float>(v);
+            }
+        }
+        c.blk_sigma.ensure(sg.size(), "block sigma");
+        c.blk_sigma_tag.ensure(tab.size(), "block sigma by tag");
+        h2d(c.blk_sigma.get(), sg.data(), sg.size(), s);
+        h2d(c.blk_sigma_tag.get(), tab.data(), tab.size(), s);
+        const std::int64_t nnz_off = c.nnz_red - nr;
+        c.blk_off.ensure(static_cast<std::size_t>(std::max<std::int64_t>(nnz_off, 1)) * K, "block off-diagonals");
+        c.blk_rowsum.ensure(static_cast<std::size_t>(nr) * K, "block row sums");
+        combine_tissues(c.tissue.off.get(), nnz_off, T, c.blk_sigma.get(), k, K, c.blk_off.get(), s);
+        combine_tissues(c.tissue.rowsum.get(), nr, T, c.blk_sigma.get(), k, K, c.blk_rowsum.get(), s);
+        c.blk_mop.width = K;
+        c.blk_mop.off_vals = c.blk_off.get();
+        c.blk_mop.rowsum = c.blk_rowsum.get();
+        c.blk_mop.tissue_vals = c.tissue.vals.get();
+        c.blk_mop.n_tissue = T;
+        c.blk_mop.sigma = c.blk_sigma.get();
+    }
     element_pass(c.d_dadt_nodes.get(), c.tets.get(), c.g32.get(), c.neg_vc.get(), m, k, K, c.d_dadt_elm.get(),
-                 c.d_q.get(), s);
+                 c.d_q.get(), s, c.vols32.get(), c.tags.get(), per_sigma ? c.blk_sigma_tag.get() : nullptr,
+                 per_sigma ? sopt->n_sigma : 0);
     float* rhs_full = nullptr;
     if (out && out->rhs) {
         c.d_rhs.ensure(n * k, "rhs full");
@@ -808,7 +867,19 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
         for (int ci = 0; ci < k; ++ci)
             for (std::int64_t r = 0; r < nr; ++r) out->x0[ci * nr + r] = blk[r * K + ci];
     }
-    SolveStats st = c.solver->solve(c.d_B.get(), c.d_X.get(), dx0, k, opt);
+    SolveStats st;
+    if (per_sigma) {
+        c.solver->set_multi(&c.blk_mop);
+        try {
+            st = c.solver->solve(c.d_B.get(), c.d_X.get(), dx0, k, opt);
+        } catch (...) {
+            c.solver->set_multi(nullptr);
+            throw;
+        }
+        c.solver->set_multi(nullptr);
+    } else {
+        st = c.solver->solve(c.d_B.get(), c.d_X.get(), dx0, k, opt);
+    }
     if (sopt && sopt->collect) c.warm.append(c.d_X.get(), k, K, s);
     // 4. reconstruction, recovery and metrics.
     const int mode = sopt ? sopt->recovery : CN_RECOVERY_RAW;
@@ -916,6 +987,8 @@ void cn_default_solve_options(cn_solve_options* o) {
     o->recovery = CN_RECOVERY_RAW;
     o->warm_start = 0;
     o->collect = 0;
+    o->sigma = nullptr;
+    o->n_sigma = 0;
     o->tolerance = 1e-6;
     o->max_iters = 2000;
     o->max_refine = 8;
@@ -1558,9 +1631,9 @@ int32_t cn_uq_prepare(cn_context* c, const double* pos_m, const double* mom, int
         cn::require(c != nullptr && pos_m != nullptr && mom != nullptr && center != nullptr && n_dip >= 1,
                     "bad argument");
         uq_prepare(*c, pos_m, mom, n_dip, center, handle, distance, didt);
-        if (n_tissue) *n_tissue = static_cast<int32_t>(c->uq.tags.size());
+        if (n_tissue) *n_tissue = static_cast<int32_t>(c->tissue.tags.size());
         if (tags)
-            for (std::size_t i = 0; i < c->uq.tags.size(); ++i) tags[i] = c->uq.tags[i];
+            for (std::size_t i = 0; i < c->tissue.tags.size(); ++i) tags[i] = c->tissue.tags[i];
     });
 }
 

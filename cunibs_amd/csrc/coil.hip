@@ -284,9 +284,13 @@ __global__ void __launch_bounds__(kDadtWaves * 64) dadt_kernel(const float* __re
 //   dadt_elm = mean of the four nodal vectors       (dadt_element.cu:6-21)
 //   q[corner] = dadt_elm . (g[corner] * neg_vc)     (rhs.cu:65-80, the staged corner terms, FP32)
 // Placements beyond k (block padding) get zeros, so their right-hand-side columns are zero.
+// With `sigma_tag` (K x n_sigma, FP32), placement c weighs element e by -(vol_e * sigma_c[tag_e])
+// instead of neg_vc[e] (the same FP32 product the context's neg_vc holds for its own table).
 __global__ void element_pass_kernel(const float* __restrict__ nodal, const std::int32_t* __restrict__ tets,
                                     const float* __restrict__ g, const float* __restrict__ neg_vc, std::int64_t m,
-                                    int k, int K, float* __restrict__ elm, float* __restrict__ q) {
+                                    int k, int K, float* __restrict__ elm, float* __restrict__ q,
+                                    const float* __restrict__ vols, const std::int32_t* __restrict__ tags,
+                                    const float* __restrict__ sigma_tag, int n_sigma) {
     const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= m * K) return;
     const std::int64_t e = t / K;
@@ -302,7 +306,7 @@ __global__ void element_pass_kernel(const float* __restrict__ nodal, const std::
         }
     }
     for (int a = 0; a < 3; ++a) elm[(e * 3 + a) * K + c] = d[a];
-    const float w = neg_vc[e];
+    const float w = sigma_tag != nullptr ? -(vols[e] * sigma_tag[c * n_sigma + tags[e]]) : neg_vc[e];
     for (int i = 0; i < 4; ++i) {
         const float* gc = g + (4 * e + i) * 3;
         const float w0 = gc[0] * w, w1 = gc[1] * w, w2 = gc[2] * w;
@@ -351,8 +355,11 @@ void dadt_nbody(const float* dip, int n_dip, const double* nodes_mm, std::int64_
 }
 
 void element_pass(const float* nodal, const std::int32_t* tets, const float* g, const float* neg_vc, std::int64_t m,
-                  int k, int K, float* elm, float* q, hipStream_t s) {
-    if (const unsigned gr = grid_for(m * K)) element_pass_kernel<<<gr, kBlock, 0, s>>>(nodal, tets, g, neg_vc, m, k, K, elm, q);
+                  int k, int K, float* elm, float* q, hipStream_t s, const float* vols, const std::int32_t* tags,
+                  const float* sigma_tag, int n_sigma) {
+    if (const unsigned gr = grid_for(m * K))
+        element_pass_kernel<<<gr, kBlock, 0, s>>>(nodal, tets, g, neg_vc, m, k, K, elm, q, vols, tags, sigma_tag,
+                                                  n_sigma);
 }
 
 void node_rhs(const float* q, const std::int32_t* ptr, const std::int32_t* idx, std::int64_t n, int k, int K,

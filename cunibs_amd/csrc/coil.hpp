@@ -16,8 +16,11 @@ void place_transforms(const double* centers, const double* handles, const double
 // dip: per placement n_dip x [s(3) m(3) m x s(3) |s|^2] FP32, centred at that placement's origin.
 void dadt_nbody(const float* dip, int n_dip, const double* nodes_mm, std::int64_t n, const double* origins, int k,
                 int K, float scale, float* out, hipStream_t s);
+// Element average of the nodal dA/dt and the RHS corner terms; sigma_tag (K x n_sigma FP32, with
+// vols and tags) gives each placement its own conductivities instead of neg_vc.
 void element_pass(const float* nodal, const std::int32_t* tets, const float* g, const float* neg_vc, std::int64_t m,
-                  int k, int K, float* elm, float* q, hipStream_t s);
+                  int k, int K, float* elm, float* q, hipStream_t s, const float* vols = nullptr,
+                  const std::int32_t* tags = nullptr, const float* sigma_tag = nullptr, int n_sigma = 0);
 void node_rhs(const float* q, const std::int32_t* ptr, const std::int32_t* idx, std::int64_t n, int k, int K,
               int ground, float* B, float* b_full, hipStream_t s);
 

@@ -512,10 +512,13 @@ def solve_placements_block(
     slots: bool = False,
     warm_start: bool = False,
     collect: bool = False,
+    sigma: np.ndarray | None = None,
 ) -> list[PlacementResult]:
     """Solve up to MAX_BLOCK placements as one block (fem/solve.py:865-979). ``recovery`` selects
-    the field E, |E| and the metrics describe ("raw", "spr_global", "spr_tissue"); ``slots``
-    also returns the recovered field on its slots."""
+    the field E, |E| and the metrics describe ("raw", "spr_global", "spr_tissue", "harmonic");
+    ``slots`` also returns the recovered field on its slots. ``sigma`` (k x n_tags, S/m by tag,
+    see `conductivity_rows`) gives every placement its own conductivities; the block then runs as
+    independent CG chains, each on its own operator, under the context's AMG hierarchy."""
     _blocks()
     k = len(sites)
     if k < 1 or k > MAX_BLOCK:
@@ -566,6 +569,13 @@ def solve_placements_block(
     opt.tolerance = float(tolerance)
     opt.max_iters = int(max_iters)
     opt.use_graph = int(use_graph)
+    sig = None
+    if sigma is not None:
+        sig = np.ascontiguousarray(sigma, np.float64)
+        if sig.ndim != 2 or sig.shape[0] != k:
+            raise ValueError("sigma must be (k, n_tags): one conductivity row per placement")
+        opt.sigma = ptr(sig)
+        opt.n_sigma = int(sig.shape[1])
     x0a = None
     if x0 is not None:
         x0a = np.ascontiguousarray(x0, np.float64).reshape(k, n - 1)
@@ -588,6 +598,22 @@ def solve_placements_block(
         )
         for i in range(k)
     ]
+
+
+def conductivity_rows(overrides: Sequence[dict | None], table: dict | None = None) -> np.ndarray:
+    """(k, 11) conductivity-by-tag rows for ``solve_placements_block(sigma=...)``: the table
+    (fem/assembly.py:15-26) with each placement's {tag: S/m} overrides applied."""
+    base = TISSUE_CONDUCTIVITY if table is None else table
+    width = max(base) + 1
+    rows = np.full((len(overrides), width), np.nan)
+    for i, ov in enumerate(overrides):
+        for t, v in base.items():
+            rows[i, t] = v
+        for t, v in (ov or {}).items():
+            if not 0 <= int(t) < width:
+                raise ValueError(f"tag {t} is outside the conductivity table")
+            rows[i, int(t)] = float(v)
+    return rows
 
 
 def metrics_dict(rec: np.ndarray, region: str = "gray_matter") -> dict:

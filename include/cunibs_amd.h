@@ -139,6 +139,13 @@ typedef struct {
     int32_t recovery;  /* CN_RECOVERY_*: the field E, |E| and the metrics describe; default raw */
     int32_t warm_start; /* 1: Galerkin reduced-basis guess, 2: supplied coefficients (cn_warm_*); 0 */
     int32_t collect;    /* 1: append this block's solutions to the warm-start basis; default 0 */
+    /* Per-placement conductivities (k x n_sigma, S/m by tag; NULL: the context's table for every
+     * placement). Each placement then has its own operator K(sigma_c) = sum_t sigma_c,t K_t and
+     * RHS (uq/conductivity/assembly.py:37-146 decomposes them the same way); the block runs as
+     * independent CG chains under the context's AMG hierarchy. n_sigma must exceed every tag
+     * present in the mesh. */
+    const double* sigma;
+    int32_t n_sigma;
 } cn_solve_options;
 
 void cn_default_solve_options(cn_solve_options* out);

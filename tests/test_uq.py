@@ -61,3 +61,36 @@ def test_uq_samples_match_per_sample_fp64_solves(head4, coil, oracle_mod):
         assert res[0].iterations <= 2  # the nominal draw starts at x_nom
     finally:
         ctx.free()
+
+
+
+@pytest.mark.gpu
+def test_per_placement_conductivities_match_fp64_solves(head4, coil, oracle_mod):
+    """solve_placements_block(sigma=...): every placement of a block with its own conductivities,
+    against the FP64 oracle built with that placement's table; the nominal row reproduces the
+    plain solve."""
+    from cunibs_amd import fem, synthetic
+    ctx = fem.SolverContext(head4.nodes_mm, head4.tet_nodes, head4.tet_tags, head4.skin_tris)
+    try:
+        sites = synthetic.cap_sites(3, seed=9)
+        overrides = [None, {1: 0.2, 2: 0.4, 4: 0.006}, {3: 1.2, 5: 0.3}]
+        rows = fem.conductivity_rows(overrides)
+        res = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites, 1e6, sigma=rows)
+        plain = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites[:1], 1e6)[0]
+        assert np.linalg.norm(res[0].magnE - plain.magnE) <= 2e-5 * np.linalg.norm(plain.magnE)
+        gm = ctx.tet_tags == 2
+        for i in (1, 2):
+            table = dict(fem.TISSUE_CONDUCTIVITY)
+            table.update(overrides[i])
+            octx = oracle_mod.build_context(head4.nodes_mm, head4.tet_nodes, head4.tet_tags, head4.skin_tris,
+                                            conductivity=table)
+            ref = oracle_mod.solve_site(octx, coil.positions_m, coil.moments, sites[i], 1e6, tol=1e-12)
+            assert res[i].relative_residual <= 1e-6
+            err = np.linalg.norm(res[i].magnE[gm] - ref["magnE"][gm]) / np.linalg.norm(ref["magnE"][gm])
+            assert err < 1e-4
+        with pytest.raises(ValueError):
+            bad = rows.copy()
+            bad[1, 3] = -1.0
+            fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites, 1e6, sigma=bad)
+    finally:
+        ctx.free()
