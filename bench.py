@@ -91,6 +91,75 @@ def kernels_only(ctx, coil, sites, k: int) -> int:
     return 0
 
 
+def learned_leg(ctx, pos, mom, n_train: int, n_test: int, log_sd: float = 0.2) -> dict:
+    """The learned warm start on poses and conductivities (distance U[2, 8] mm, the five tissues of
+    the head log-normal around the table): n_train solver-generated solutions, each with its own
+    conductivities, form the snapshot basis and the training set; the network maps pose and log
+    conductivities to the solution's coefficients in the 256 leading A-orthonormal modes; on
+    n_test held-out placements it warm-starts PCG, which iterates to the same tolerance."""
+    from cunibs_amd import fem, learned, synthetic, uq
+
+    tags = [1, 2, 3, 4, 5]
+
+    def draws(n, seed):
+        sites = synthetic.cap_sites(n, seed=seed, distance_range_mm=(2.0, 8.0))
+        sig = uq.sample_conductivities(tags, n, log_sd, seed=seed + 1)
+        rows = fem.conductivity_rows([dict(zip(tags, r)) for r in sig])
+        return sites, rows
+
+    t0 = time.perf_counter()
+    tr_sites, tr_rows = draws(n_train, 31)
+    ctx.warm_reset(n_train)
+    for i in range(0, n_train, 16):
+        fem.solve_placements_block(ctx, pos, mom, tr_sites[i:i + 16], 1e6, metrics=False, magnitude=False,
+                                   collect=True, sigma=tr_rows[i:i + 16])
+    t_solve = time.perf_counter() - t0
+    rank = ctx.warm_finalize(max_modes=256)
+    t_basis = time.perf_counter() - t0 - t_solve
+    feats = lambda sites, rows: np.stack([learned.pose_features(s, r[1:11]) for s, r in zip(sites, rows)])
+    net = learned.WarmStartNetwork(n_out=(rank + 15) // 16 * 16, hidden=512, seed=0)
+    T = np.zeros((n_train, net.n_out), np.float32)
+    T[:, :rank] = ctx.warm_targets()[:, :rank]
+    t1 = time.perf_counter()
+    loss = net.fit(feats(tr_sites, tr_rows), T, epochs=300)
+    net.modes = rank
+    t_train = time.perf_counter() - t1
+
+    te_sites, te_rows = draws(n_test, 37)
+    X = feats(te_sites, te_rows)
+
+    def run(mode):
+        its, block_its, mags = [], [], []
+        t = time.perf_counter()
+        for i in range(0, n_test, 16):
+            s, r = te_sites[i:i + 16], te_rows[i:i + 16]
+            if mode == "learned":
+                c = net.predict(X[i:i + 16])[:, :rank].astype(np.float64)
+                ctx.warm_set_coefficients(c)
+            res = fem.solve_placements_block(ctx, pos, mom, s, 1e6, metrics=True, magnitude=True, sigma=r,
+                                             warm_start={"cold": 0, "galerkin": 1, "learned": 2}[mode])
+            its.extend(x.iterations for x in res)
+            block_its.append(max(x.iterations for x in res))
+            mags.extend(x.magnE for x in res[:2])
+        return n_test / (time.perf_counter() - t), its, block_its, mags
+
+    out = {}
+    ref = None
+    for mode in ("cold", "learned", "galerkin"):
+        rate, its, bits, mags = run(mode)
+        out[mode] = {"placements_per_s": round(rate, 2), "iterations_mean": round(float(np.mean(its)), 2),
+                     "block_iterations_mean": round(float(np.mean(bits)), 2)}
+        if ref is None:
+            ref = mags
+        else:
+            out[mode]["magnE_rel_diff_vs_cold"] = max(
+                float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(mags, ref))
+    out.update({"train_solves": n_train, "test_placements": n_test, "modes": rank, "train_loss": round(loss, 5),
+                "solve_s": round(t_solve, 2), "basis_s": round(t_basis, 2), "network_train_s": round(t_train, 2)})
+    net.free()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +176,9 @@ def main() -> int:
     ap.add_argument("--autotune", type=int, default=1,
                     help="1: pick block width (4/8/16), SpMM tile and Chebyshev degree (2-6) on 32 training "
                          "placements before the sweeps; 0: use --block-k, the automatic tile and degree 2")
+    ap.add_argument("--learned", type=int, default=512,
+                    help="training solves of the learned warm start on random poses and conductivities, "
+                         "evaluated on 256 held-out placements (0: skip)")
     ap.add_argument("--isolated", type=int, default=0,
                     help="1: also time the SpMM and V-cycle in isolated repeated launches (warm caches)")
     ap.add_argument("--kernels-only", action="store_true",
@@ -226,6 +298,16 @@ def main() -> int:
     log(f"in-sweep ({timing['samples']} samples) K={K}: differenced SpMM {spmm_ms * 1e3:.1f} us "
         f"({achieved:.0f} GB/s), V-cycle {vc_ms * 1e3:.1f} us ({vc_achieved:.0f} GB/s), "
         f"iteration {it_ms * 1e3:.1f} us")
+    # dA/dt: FP32 matrix-core products, 20 useful flop per (node, dipole) pair (4 multiply-adds of
+    # the distance product, 6 of the accumulation), timed with HIP events around each block's launch.
+    n_dip = pos.shape[0]
+    dadt_flop = 20.0 * ctx.n_nodes * n_dip
+    dadt_ms = timing["dadt_ms"]
+    dadt_tf = dadt_flop / (dadt_ms * 1e-3) / 1e12 if dadt_ms > 0 else float("nan")
+    dadt_iso = {"mfma": round(ctx.bench_dadt(pos, mom, sites[0], k=8, variant=0, reps=3) / 8, 4),
+                "valu": round(ctx.bench_dadt(pos, mom, sites[0], k=8, variant=1, reps=3) / 8, 4)}
+    log(f"dA/dt in sweep {dadt_ms:.3f} ms per placement ({dadt_tf:.1f} TFLOP/s); isolated per placement "
+        f"{dadt_iso}")
     isolated = None
     if args.isolated:
         isolated = {"spmm_us": round(ctx.bench_spmm(K, 50, mode=1) * 1e3, 2),
@@ -291,6 +373,16 @@ def main() -> int:
             "algorithmic_bytes": vc_bytes,
             "degree": degree,
         },
+        "dadt_roofline": {
+            "bound": "mfma",
+            "achieved": round(dadt_tf, 2),
+            "peak": 157.3,
+            "unit": "TFLOP/s",
+            "frac": round(dadt_tf / 157.3, 4),
+            "avg_ms_per_placement": round(dadt_ms, 4),
+            "flop_per_placement": dadt_flop,
+            "isolated_ms_per_placement": dadt_iso,
+        },
         "iteration_us": round(it_ms * 1e3, 2),
         "cold_in_sweep_us": {"spmm": round(cold_timing["spmm_ms"] * 1e3, 2),
                              "vcycle": round(cold_timing["vcycle_ms"] * 1e3, 2),
@@ -329,6 +421,10 @@ def main() -> int:
                       f"reconstruction) after an untimed oracle setup",
         }
         log(f"cpu baseline {out['cpu_baseline']['value']} placements/s on {ncores} threads; GM err {errs}")
+    if args.learned > 0:
+        out["learned"] = learned_leg(ctx, pos, mom, args.learned, 256)
+        log(f"learned warm start: {out['learned']}")
+
     print(json.dumps(out), flush=True)
     ctx.free()
     return 0

@@ -108,6 +108,10 @@ struct cn_context {
         std::vector<int> tags;
         cn::DeviceBuffer<double> vals, off, rowsum;  // (T, nnz_red), (T, nnz_off), (T, n_red) FP64
     } tissue;
+    // dA/dt timing of the blocks solved while in-solve timing is on (HIP events, outside graphs).
+    hipEvent_t dadt_ev[2] = {nullptr, nullptr};
+    double dadt_ms = 0.0;
+    std::int64_t dadt_placements = 0;
     // Per-placement conductivities of the current block: (K, T) sigma, combined operator.
     cn::DeviceBuffer<double> blk_sigma;
     cn::DeviceBuffer<float> blk_sigma_tag, blk_off, blk_rowsum;
@@ -134,6 +138,8 @@ struct cn_context {
         return {rptr.get(), rcol.get(), rval32.get(), n - 1, n - 1, nnz_red};
     }
     ~cn_context() {
+        for (auto& e : dadt_ev)
+            if (e) (void)hipEventDestroy(e);
         solver.reset();
         if (s) (void)hipStreamDestroy(s);
     }
@@ -780,8 +786,21 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
     c.d_q.ensure(4 * m * K, "corner terms");
     c.d_B.ensure(nr * K, "rhs block");
     const float scale = static_cast<float>(didt) * static_cast<float>(kMu0Over4Pi);
+    if (c.solver->timing_on()) {
+        if (c.dadt_ev[0] == nullptr)
+            for (auto& e : c.dadt_ev) check(hipEventCreate(&e), "dadt event");
+        check(hipEventRecord(c.dadt_ev[0], s), "dadt event");
+    }
     dadt_nbody(c.d_dip.get(), static_cast<int>(n_dip), c.nodes.get(), n, c.d_origins.get(), k, K, scale,
                c.d_dadt_nodes.get(), s);
+    if (c.solver->timing_on()) {
+        check(hipEventRecord(c.dadt_ev[1], s), "dadt event");
+        check(hipEventSynchronize(c.dadt_ev[1]), "dadt event");
+        float ms = 0.f;
+        check(hipEventElapsedTime(&ms, c.dadt_ev[0], c.dadt_ev[1]), "dadt elapsed");
+        c.dadt_ms += ms;
+        c.dadt_placements += k;
+    }
     // Per-placement conductivities: the tissue decomposition of the operator and a (K, n_sigma)
     // table for the RHS; padded columns take the context's table.
     const bool per_sigma = sopt != nullptr && sopt->sigma != nullptr;
@@ -1370,6 +1389,8 @@ int32_t cn_timing_enable(cn_context* c, int32_t on) {
     return guarded([&] {
         cn::require(c != nullptr, "null context");
         c->solver->set_timing(on != 0);
+        c->dadt_ms = 0.0;
+        c->dadt_placements = 0;
     });
 }
 
@@ -1377,6 +1398,46 @@ int32_t cn_timing_read(const cn_context* c, double* out) {
     return guarded([&] {
         cn::require(c != nullptr && out != nullptr, "null argument");
         c->solver->timing(out);
+        out[4] = static_cast<double>(c->dadt_placements);
+        out[5] = c->dadt_placements ? c->dadt_ms / static_cast<double>(c->dadt_placements) : 0.0;
+    });
+}
+
+int32_t cn_bench_dadt(cn_context* c, const double* dip_pos_m, const double* dip_moment, int64_t n_dip,
+                      const double* center_mm, const double* handle_mm, double distance_mm, int32_t k,
+                      int32_t variant, int32_t reps, double* ms) {
+    return guarded([&] {
+        cn::require(c != nullptr && dip_pos_m && dip_moment && center_mm && ms && n_dip >= 1, "bad argument");
+        cn::require(k >= 1 && k <= cn::kMaxBlock && reps >= 1 && (variant == 0 || variant == 1), "bad argument");
+        const hipStream_t s = c->s;
+        std::vector<double> tf(16);
+        std::vector<std::int32_t> degen(1);
+        place_block(*c, center_mm, handle_mm, &distance_mm, 1, tf.data(), degen.data());
+        std::vector<float> dip(static_cast<std::size_t>(10 * n_dip * k));
+        std::vector<double> origins(3 * k);
+        for (int i = 0; i < k; ++i)
+            pack_dipoles(tf.data(), dip_pos_m, dip_moment, n_dip, dip.data() + 10 * n_dip * i, origins.data() + 3 * i);
+        auto ddip = cn::upload(dip.data(), dip.size(), s, "dipoles");
+        auto dorg = cn::upload(origins.data(), origins.size(), s, "origins");
+        const int K = cn::padded_width(k);
+        cn::DeviceBuffer<float> out(3 * c->n * K, "dadt");
+        auto run = [&] {
+            cn::dadt_nbody(ddip.get(), static_cast<int>(n_dip), c->nodes.get(), c->n, dorg.get(), k, K, 1e-7f,
+                           out.get(), s, variant);
+        };
+        run();
+        hipEvent_t e0, e1;
+        cn::check(hipEventCreate(&e0), "event");
+        cn::check(hipEventCreate(&e1), "event");
+        cn::check(hipEventRecord(e0, s), "event");
+        for (int r = 0; r < reps; ++r) run();
+        cn::check(hipEventRecord(e1, s), "event");
+        cn::check(hipEventSynchronize(e1), "event");
+        float t = 0.f;
+        cn::check(hipEventElapsedTime(&t, e0, e1), "elapsed");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms = static_cast<double>(t) / reps;
     });
 }
 

@@ -243,13 +243,16 @@ __global__ void __launch_bounds__(kDadtWaves * 64) dadt_kernel(const float* __re
                 const float v = s_m[8 * (t + 4 * quad + r) + (col & 7)];
                 bm[r] = col < 6 ? v : 0.f;
             }
+            floatx4_t d2[kDadtGroups];
 #pragma unroll
-            for (int g = 0; g < kDadtGroups; ++g) {
-                const floatx4_t d2 =
-                    __builtin_amdgcn_mfma_f32_16x16x4f32(a, bn[g], floatx4_t{rn[g], rn[g], rn[g], rn[g]}, 0, 0, 0);
+            for (int g = 0; g < kDadtGroups; ++g)
+                d2[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bn[g], floatx4_t{rn[g], rn[g], rn[g], rn[g]}, 0, 0, 0);
+            // Consecutive accumulations go to different groups, so no MFMA waits on the one before.
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const float inv = rsqrtf(d2[r]);
+            for (int r = 0; r < 4; ++r) {
+#pragma unroll
+                for (int g = 0; g < kDadtGroups; ++g) {
+                    const float inv = rsqrtf(d2[g][r]);
                     acc[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(inv * inv * inv, bm[r], acc[g], 0, 0, 0);
                 }
             }
@@ -276,6 +279,84 @@ __global__ void __launch_bounds__(kDadtWaves * 64) dadt_kernel(const float* __re
             o[2 * K] = scale * ((w[0] * y - w[1] * x) - w[5]);
         }
         __syncthreads();
+    }
+}
+
+constexpr int kDadtBlock = 256;
+constexpr int kDipTile = 256;
+constexpr int kNodesPerThread = 4;
+
+// The same sum on the vector ALUs alone (dadt.cu:27-104 restated): dipole tiles staged in LDS, 4
+// nodes per thread in registers, one placement per blockIdx.y; kept as the second variant the
+// matrix-core kernel is measured against (cn_bench_dadt).
+__global__ void __launch_bounds__(kDadtBlock) dadt_valu_kernel(const float* __restrict__ dip, int n_dip,
+                                                          const double* __restrict__ nodes_mm, std::int64_t n,
+                                                          const double* __restrict__ origin, float scale, int K,
+                                                          float* __restrict__ out) {
+    __shared__ float s_s[kDipTile * 3];
+    __shared__ float s_mp[kDipTile * 6];
+    __shared__ float s_sn[kDipTile];
+    const int c = blockIdx.y;
+    const float* dp = dip + static_cast<std::int64_t>(c) * n_dip * 10;  // per placement: s(3) mp(6) sn(1)
+    const double ox = origin[3 * c], oy = origin[3 * c + 1], oz = origin[3 * c + 2];
+    const std::int64_t T = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
+    const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    float rx[kNodesPerThread], ry[kNodesPerThread], rz[kNodesPerThread], rn[kNodesPerThread];
+    float wx[kNodesPerThread], wy[kNodesPerThread], wz[kNodesPerThread];
+    float px[kNodesPerThread], py[kNodesPerThread], pz[kNodesPerThread];
+#pragma unroll
+    for (int k = 0; k < kNodesPerThread; ++k) {
+        const std::int64_t node = t + k * T;
+        rx[k] = ry[k] = rz[k] = rn[k] = 0.f;
+        if (node < n) {
+            rx[k] = static_cast<float>(nodes_mm[3 * node] * 1e-3 - ox);
+            ry[k] = static_cast<float>(nodes_mm[3 * node + 1] * 1e-3 - oy);
+            rz[k] = static_cast<float>(nodes_mm[3 * node + 2] * 1e-3 - oz);
+            rn[k] = fmaf(rz[k], rz[k], fmaf(ry[k], ry[k], rx[k] * rx[k]));
+        }
+        wx[k] = wy[k] = wz[k] = px[k] = py[k] = pz[k] = 0.f;
+    }
+    for (int base = 0; base < n_dip; base += kDipTile) {
+        const int tile = min(kDipTile, n_dip - base);
+        __syncthreads();
+        for (int i = threadIdx.x; i < tile; i += kDadtBlock) {
+            const float* d = dp + static_cast<std::int64_t>(base + i) * 10;
+            s_s[3 * i] = d[0];
+            s_s[3 * i + 1] = d[1];
+            s_s[3 * i + 2] = d[2];
+            for (int q = 0; q < 6; ++q) s_mp[6 * i + q] = d[3 + q];
+            s_sn[i] = d[9];
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int j = 0; j < tile; ++j) {
+            const float sx = s_s[3 * j], sy = s_s[3 * j + 1], sz = s_s[3 * j + 2], snj = s_sn[j];
+            const float m0 = s_mp[6 * j], m1 = s_mp[6 * j + 1], m2 = s_mp[6 * j + 2];
+            const float m3 = s_mp[6 * j + 3], m4 = s_mp[6 * j + 4], m5 = s_mp[6 * j + 5];
+#pragma unroll
+            for (int k = 0; k < kNodesPerThread; ++k) {
+                const float rs = fmaf(rz[k], sz, fmaf(ry[k], sy, rx[k] * sx));
+                const float d2 = fmaf(-2.f, rs, rn[k] + snj);
+                const float inv = rsqrtf(d2);
+                const float w = inv * inv * inv;
+                wx[k] = fmaf(w, m0, wx[k]);
+                wy[k] = fmaf(w, m1, wy[k]);
+                wz[k] = fmaf(w, m2, wz[k]);
+                px[k] = fmaf(w, m3, px[k]);
+                py[k] = fmaf(w, m4, py[k]);
+                pz[k] = fmaf(w, m5, pz[k]);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kNodesPerThread; ++k) {
+        const std::int64_t node = t + k * T;
+        if (node < n) {
+            float* o = out + node * 3 * K + c;
+            o[0] = scale * ((wy[k] * rz[k] - wz[k] * ry[k]) - px[k]);
+            o[K] = scale * ((wz[k] * rx[k] - wx[k] * rz[k]) - py[k]);
+            o[2 * K] = scale * ((wx[k] * ry[k] - wy[k] * rx[k]) - pz[k]);
+        }
     }
 }
 
@@ -346,8 +427,14 @@ void place_transforms(const double* centers, const double* handles, const double
 }
 
 void dadt_nbody(const float* dip, int n_dip, const double* nodes_mm, std::int64_t n, const double* origins, int k,
-                int K, float scale, float* out, hipStream_t s) {
+                int K, float scale, float* out, hipStream_t s, int variant) {
     if (n <= 0 || k <= 0) return;
+    if (variant == 1) {
+        const std::int64_t threads = (n + kNodesPerThread - 1) / kNodesPerThread;
+        const unsigned gx = grid_for(threads, kDadtBlock);
+        dadt_valu_kernel<<<dim3(gx, k), kDadtBlock, 0, s>>>(dip, n_dip, nodes_mm, n, origins, scale, K, out);
+        return;
+    }
     const std::int64_t blocks = (n + kDadtNodesPerBlock - 1) / kDadtNodesPerBlock;
     require(blocks <= 0x7fffffff && k <= 65535, "dA/dt launch exceeds the grid limits");
     dadt_kernel<<<dim3(static_cast<unsigned>(blocks), k), kDadtWaves * 64, 0, s>>>(dip, n_dip, nodes_mm, n, origins,

@@ -1104,6 +1104,7 @@ BlockSolver::BlockSolver(const Operator& op, const Hierarchy& h, hipStream_t s) 
 
 BlockSolver::~BlockSolver() {
     drop_graphs();
+    for (auto g : retired_) (void)hipGraphExecDestroy(g);
     if (host_state_) (void)hipHostFree(host_state_);
 }
 
@@ -1113,9 +1114,12 @@ void BlockSolver::set_multi(const MultiOperator* m) {
     drop_graphs();  // captured iterations name the operator's kernels and buffers
 }
 
+// Captured iterations that are replaced are retired, not destroyed, until the solver goes away: a
+// graph exec destroyed and another instantiated in the same process crashes rocprofv3's kernel
+// tracer on this ROCm, and a retired exec costs only its host-side node list.
 void BlockSolver::drop_graphs() {
     for (auto& g : graphs_) {
-        if (g) (void)hipGraphExecDestroy(g);
+        if (g) retired_.push_back(g);
         g = nullptr;
     }
 }
@@ -1325,7 +1329,7 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
         while (kBlockWidths[slot] != K) ++slot;
         if (opt.use_graph && (graphs_[slot] == nullptr || graph_drop_[slot] != opt.drop_tol)) {
             if (graphs_[slot]) {
-                (void)hipGraphExecDestroy(graphs_[slot]);
+                retired_.push_back(graphs_[slot]);
                 graphs_[slot] = nullptr;
             }
             hipGraph_t g = nullptr;

@@ -105,6 +105,7 @@ class BlockSolver {
     // kernel at the end of each iteration (captured with it) adds the spans of every iteration
     // that ran in full. Switching drops the captured iterations; every call resets the sums.
     void set_timing(bool on);
+    bool timing_on() const { return timing_; }
     // out[0] iterations sampled, out[1..3] mean ms of the SpMM, the V-cycle and the iteration.
     void timing(double* out) const;
 
@@ -153,6 +154,7 @@ class BlockSolver {
     void mark(int i, const int* state);
     hipGraphExec_t graphs_[kNumBlockWidths] = {};
     double graph_drop_[kNumBlockWidths] = {};  // the drop tolerance each graph was captured with
+    std::vector<hipGraphExec_t> retired_;
     void drop_graphs();
 };
 

@@ -247,9 +247,10 @@ def _site(direction: np.ndarray, handle_angle: float, distance_mm: float, radius
 
 def cap_sites(
     n: int, *, seed: int = 0, max_polar_deg: float = 60.0, distance_mm: float = 4.0,
-    radius_mm: float = SHELL_RADII_MM[-1],
+    radius_mm: float = SHELL_RADII_MM[-1], distance_range_mm: tuple[float, float] | None = None,
 ) -> list[tuple[np.ndarray, np.ndarray, float]]:
-    """``n`` (center, handle, distance) triples uniform on the upper cap, handle angle U[0, 2pi)."""
+    """``n`` (center, handle, distance) triples uniform on the upper cap, handle angle U[0, 2pi);
+    the distance is ``distance_mm`` or, with ``distance_range_mm``, uniform on that range."""
     rng = np.random.default_rng(seed)
     cmin = math.cos(math.radians(max_polar_deg))
     out = []
@@ -257,8 +258,9 @@ def cap_sites(
         ct = rng.uniform(cmin, 1.0)
         ph = rng.uniform(0.0, 2.0 * math.pi)
         a = rng.uniform(0.0, 2.0 * math.pi)
+        d = distance_mm if distance_range_mm is None else rng.uniform(*distance_range_mm)
         st = math.sqrt(max(0.0, 1.0 - ct * ct))
-        out.append(_site(np.array([st * math.cos(ph), st * math.sin(ph), ct]), a, distance_mm, radius_mm))
+        out.append(_site(np.array([st * math.cos(ph), st * math.sin(ph), ct]), a, d, radius_mm))
     return out
 
 

@@ -318,8 +318,16 @@ int32_t cn_bench_vcycle(cn_context* ctx, int32_t K, int32_t reps, double* ms);
  * on this runtime). Every call resets the sums; switching drops the captured iterations. */
 int32_t cn_timing_enable(cn_context* ctx, int32_t on);
 /* out[0] iterations sampled, out[1] mean ms of the differenced SpMM (earliest block start to
- * latest block end), out[2] of the V-cycle, out[3] of the whole iteration. */
-int32_t cn_timing_read(const cn_context* ctx, double* out4);
+ * latest block end), out[2] of the V-cycle, out[3] of the whole iteration; out[4] placements whose
+ * dA/dt was timed (HIP events around the N-body launch of each block, which runs outside the
+ * graphs), out[5] mean ms of dA/dt per placement. */
+int32_t cn_timing_read(const cn_context* ctx, double* out6);
+/* Average device time (ms) of the dA/dt N-body for k copies of one placement over the context's
+ * nodes: variant 0 is the matrix-core kernel the solves use, 1 the vector-ALU formulation it is
+ * measured against. A measurement aid with no counterpart in the reference. */
+int32_t cn_bench_dadt(cn_context* ctx, const double* dip_pos_m, const double* dip_moment, int64_t n_dip,
+                      const double* center_mm, const double* handle_mm, double distance_mm, int32_t k,
+                      int32_t variant, int32_t reps, double* ms);
 /* Average device time (ms) of a streaming device-to-device copy of `bytes` bytes (a multiple of 16)
  * on the library's stream: the calibration launch for the HBM traffic counters and the practical
  * bandwidth ceiling. No counterpart in the reference (a measurement aid). */
