@@ -179,6 +179,10 @@ def main() -> int:
     ap.add_argument("--learned", type=int, default=512,
                     help="training solves of the learned warm start on random poses and conductivities, "
                          "evaluated on 256 held-out placements (0: skip)")
+    ap.add_argument("--graphs", type=int, default=1,
+                    help="0: launch every iteration's kernels directly instead of replaying its captured graph")
+    ap.add_argument("--fine-sub", type=int, default=-1,
+                    help="--kernels-only: fine-level tile (sub-lanes) to measure at (-1: automatic)")
     ap.add_argument("--isolated", type=int, default=0,
                     help="1: also time the SpMM and V-cycle in isolated repeated launches (warm caches)")
     ap.add_argument("--kernels-only", action="store_true",
@@ -203,12 +207,14 @@ def main() -> int:
     k = args.block_k
     pos, mom = coil.positions_m, coil.moments
     if args.kernels_only:
+        if args.fine_sub >= 0:
+            ctx.set_tuning(fine_sub=args.fine_sub)
         return kernels_only(ctx, coil, sites, k)
 
     def block(i0: int, kk: int, warm: bool = False):
         chunk = [sites[(i0 + j) % len(sites)] for j in range(kk)]
         return fem.solve_placements_block(ctx, pos, mom, chunk, 1e6, magnitude=False, metrics=True,
-                                          warm_start=warm)
+                                          warm_start=warm, use_graph=bool(args.graphs))
 
     def sweep(warm: bool):
         for w in range(args.warmup):
@@ -298,14 +304,13 @@ def main() -> int:
     log(f"in-sweep ({timing['samples']} samples) K={K}: differenced SpMM {spmm_ms * 1e3:.1f} us "
         f"({achieved:.0f} GB/s), V-cycle {vc_ms * 1e3:.1f} us ({vc_achieved:.0f} GB/s), "
         f"iteration {it_ms * 1e3:.1f} us")
-    # dA/dt: FP32 matrix-core products, 20 useful flop per (node, dipole) pair (4 multiply-adds of
-    # the distance product, 6 of the accumulation), timed with HIP events around each block's launch.
+    # dA/dt: FP32 on the vector ALUs, 20 flop per (node, dipole) pair (the distance as r.s, rn + sn
+    # - 2 r.s, rsqrt, cube, six multiply-adds), timed with HIP events around each block's launch.
     n_dip = pos.shape[0]
     dadt_flop = 20.0 * ctx.n_nodes * n_dip
     dadt_ms = timing["dadt_ms"]
     dadt_tf = dadt_flop / (dadt_ms * 1e-3) / 1e12 if dadt_ms > 0 else float("nan")
-    dadt_iso = {"mfma": round(ctx.bench_dadt(pos, mom, sites[0], k=8, variant=0, reps=3) / 8, 4),
-                "valu": round(ctx.bench_dadt(pos, mom, sites[0], k=8, variant=1, reps=3) / 8, 4)}
+    dadt_iso = round(ctx.bench_dadt(pos, mom, sites[0], k=8, reps=3) / 8, 4)
     log(f"dA/dt in sweep {dadt_ms:.3f} ms per placement ({dadt_tf:.1f} TFLOP/s); isolated per placement "
         f"{dadt_iso}")
     isolated = None
@@ -374,7 +379,7 @@ def main() -> int:
             "degree": degree,
         },
         "dadt_roofline": {
-            "bound": "mfma",
+            "bound": "valu",
             "achieved": round(dadt_tf, 2),
             "peak": 157.3,
             "unit": "TFLOP/s",

@@ -1405,10 +1405,10 @@ int32_t cn_timing_read(const cn_context* c, double* out) {
 
 int32_t cn_bench_dadt(cn_context* c, const double* dip_pos_m, const double* dip_moment, int64_t n_dip,
                       const double* center_mm, const double* handle_mm, double distance_mm, int32_t k,
-                      int32_t variant, int32_t reps, double* ms) {
+                      int32_t reps, double* ms) {
     return guarded([&] {
         cn::require(c != nullptr && dip_pos_m && dip_moment && center_mm && ms && n_dip >= 1, "bad argument");
-        cn::require(k >= 1 && k <= cn::kMaxBlock && reps >= 1 && (variant == 0 || variant == 1), "bad argument");
+        cn::require(k >= 1 && k <= cn::kMaxBlock && reps >= 1, "bad argument");
         const hipStream_t s = c->s;
         std::vector<double> tf(16);
         std::vector<std::int32_t> degen(1);
@@ -1423,7 +1423,7 @@ int32_t cn_bench_dadt(cn_context* c, const double* dip_pos_m, const double* dip_
         cn::DeviceBuffer<float> out(3 * c->n * K, "dadt");
         auto run = [&] {
             cn::dadt_nbody(ddip.get(), static_cast<int>(n_dip), c->nodes.get(), c->n, dorg.get(), k, K, 1e-7f,
-                           out.get(), s, variant);
+                           out.get(), s);
         };
         run();
         hipEvent_t e0, e1;

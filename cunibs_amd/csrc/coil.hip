@@ -165,131 +165,17 @@ __global__ void place_frame_kernel(const double* __restrict__ centers, const dou
     o[15] = 1.0;
 }
 
-typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4_t;
-
-constexpr int kDadtWaves = 4;    // wavefronts per block
-constexpr int kDadtGroups = 4;   // 16-node groups per wavefront
-constexpr int kDadtStage = 256;  // dipoles staged in LDS per pass (a multiple of 16)
-constexpr int kDadtNodesPerBlock = kDadtWaves * kDadtGroups * 16;
-
-// dadt.cu:27-104 restated for CDNA matrix cores. dA/dt(r) = scale (W x r - P) with
-// W = sum_j w_j m_j, P = sum_j w_j (m_j x s_j), w_j = |r - s_j|^-3, positions relative to the coil
-// origin (FP64 subtraction, then FP32). Both sums are matrix products over the dipoles, run as
-// FP32 MFMAs (v_mfma_f32_16x16x4f32) per 16 nodes x 16 dipoles:
-//   1. d2 (16 dipoles x 16 nodes) = [-2 s | |s|^2] [r ; 1] + |r|^2, the reference's
-//      rn + sn - 2 r.s form, with |r|^2 as the accumulator input;
-//   2. w = rsqrtf(d2)^3 on the vector ALUs; a lane holds w for one node and four consecutive
-//      dipoles, which is exactly the A operand of four K = 4 steps;
-//   3. [W | P] (16 nodes x 16, six columns used) += w (16 x 4) [m | m x s] (4 x 16), four times.
-// One placement per blockIdx.y; dipoles staged in LDS, padded to a multiple of 16 with a unit
-// distance and zero moment. Output (n, 3, K), the K placements of a node component contiguous.
-__global__ void __launch_bounds__(kDadtWaves * 64) dadt_kernel(const float* __restrict__ dip, int n_dip,
-                                                               const double* __restrict__ nodes_mm, std::int64_t n,
-                                                               const double* __restrict__ origin, float scale, int K,
-                                                               float* __restrict__ out) {
-    __shared__ float s_a[kDadtStage * 4];  // per dipole: -2 sx, -2 sy, -2 sz, |s|^2
-    __shared__ float s_m[kDadtStage * 8];  // per dipole: m (3), m x s (3), two zeros
-    __shared__ float s_o[kDadtWaves][16 * 6];
-    const int c = blockIdx.y;
-    const float* dp = dip + static_cast<std::int64_t>(c) * n_dip * 10;  // per dipole: s(3) mp(6) sn(1)
-    const double ox = origin[3 * c], oy = origin[3 * c + 1], oz = origin[3 * c + 2];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int col = lane & 15, quad = lane >> 4;
-    const std::int64_t base = static_cast<std::int64_t>(blockIdx.x) * kDadtNodesPerBlock +
-                              static_cast<std::int64_t>(wave) * (kDadtGroups * 16);
-    float bn[kDadtGroups], rn[kDadtGroups];
-    floatx4_t acc[kDadtGroups];
-#pragma unroll
-    for (int g = 0; g < kDadtGroups; ++g) {
-        const std::int64_t node = base + g * 16 + col;
-        float x = 0.f, y = 0.f, z = 0.f, q = 1.f;  // a missing node sits at unit distance
-        if (node < n) {
-            x = static_cast<float>(nodes_mm[3 * node] * 1e-3 - ox);
-            y = static_cast<float>(nodes_mm[3 * node + 1] * 1e-3 - oy);
-            z = static_cast<float>(nodes_mm[3 * node + 2] * 1e-3 - oz);
-            q = fmaf(z, z, fmaf(y, y, x * x));
-        }
-        rn[g] = q;
-        bn[g] = quad == 0 ? x : (quad == 1 ? y : (quad == 2 ? z : 1.f));
-        acc[g] = floatx4_t{0.f, 0.f, 0.f, 0.f};
-    }
-    for (int s0 = 0; s0 < n_dip; s0 += kDadtStage) {
-        const int cnt = min(kDadtStage, n_dip - s0);
-        const int cnt16 = (cnt + 15) & ~15;
-        __syncthreads();
-        for (int i = threadIdx.x; i < cnt16; i += kDadtWaves * 64) {
-            if (i < cnt) {
-                const float* d = dp + static_cast<std::int64_t>(s0 + i) * 10;
-                s_a[4 * i] = -2.f * d[0];
-                s_a[4 * i + 1] = -2.f * d[1];
-                s_a[4 * i + 2] = -2.f * d[2];
-                s_a[4 * i + 3] = d[9];
-#pragma unroll
-                for (int q = 0; q < 6; ++q) s_m[8 * i + q] = d[3 + q];
-            } else {
-                s_a[4 * i] = s_a[4 * i + 1] = s_a[4 * i + 2] = 0.f;
-                s_a[4 * i + 3] = 1.f;
-#pragma unroll
-                for (int q = 0; q < 6; ++q) s_m[8 * i + q] = 0.f;
-            }
-            s_m[8 * i + 6] = s_m[8 * i + 7] = 0.f;
-        }
-        __syncthreads();
-        for (int t = 0; t < cnt16; t += 16) {
-            const float a = s_a[4 * (t + col) + quad];
-            float bm[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const float v = s_m[8 * (t + 4 * quad + r) + (col & 7)];
-                bm[r] = col < 6 ? v : 0.f;
-            }
-            floatx4_t d2[kDadtGroups];
-#pragma unroll
-            for (int g = 0; g < kDadtGroups; ++g)
-                d2[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bn[g], floatx4_t{rn[g], rn[g], rn[g], rn[g]}, 0, 0, 0);
-            // Consecutive accumulations go to different groups, so no MFMA waits on the one before.
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-#pragma unroll
-                for (int g = 0; g < kDadtGroups; ++g) {
-                    const float inv = rsqrtf(d2[g][r]);
-                    acc[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(inv * inv * inv, bm[r], acc[g], 0, 0, 0);
-                }
-            }
-        }
-    }
-    // acc[g]: lane (quad, col) holds column col of nodes 4 quad .. 4 quad + 3; turn it through LDS
-    // so that lane i < 16 holds W and P of node i.
-#pragma unroll
-    for (int g = 0; g < kDadtGroups; ++g) {
-        if (col < 6) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) s_o[wave][(4 * quad + r) * 6 + col] = acc[g][r];
-        }
-        __syncthreads();
-        const std::int64_t node = base + g * 16 + lane;
-        if (lane < 16 && node < n) {
-            const float x = static_cast<float>(nodes_mm[3 * node] * 1e-3 - ox);
-            const float y = static_cast<float>(nodes_mm[3 * node + 1] * 1e-3 - oy);
-            const float z = static_cast<float>(nodes_mm[3 * node + 2] * 1e-3 - oz);
-            const float* w = &s_o[wave][lane * 6];
-            float* o = out + node * 3 * K + c;
-            o[0] = scale * ((w[1] * z - w[2] * y) - w[3]);
-            o[K] = scale * ((w[2] * x - w[0] * z) - w[4]);
-            o[2 * K] = scale * ((w[0] * y - w[1] * x) - w[5]);
-        }
-        __syncthreads();
-    }
-}
-
 constexpr int kDadtBlock = 256;
 constexpr int kDipTile = 256;
 constexpr int kNodesPerThread = 4;
 
-// The same sum on the vector ALUs alone (dadt.cu:27-104 restated): dipole tiles staged in LDS, 4
-// nodes per thread in registers, one placement per blockIdx.y; kept as the second variant the
-// matrix-core kernel is measured against (cn_bench_dadt).
-__global__ void __launch_bounds__(kDadtBlock) dadt_valu_kernel(const float* __restrict__ dip, int n_dip,
+// dadt.cu:27-104 restated for CDNA: dipole tiles staged in LDS, 4 nodes per thread in registers,
+// one placement per blockIdx.y. w_j = |r - s_j|^-3 from rn + sn - 2 r.s; rsqrtf, FP32, on the
+// vector ALUs. (A matrix-core version, the distance and accumulation sums as 16x16x4 FP32 MFMAs
+// with rsqrt on the vector ALUs, measured 1.35 ms per placement at 1.0M nodes x 2400 dipoles
+// against 0.89 ms for this kernel, and was dropped.)
+// Output (n, 3, K): the K placements of a node component are contiguous.
+__global__ void __launch_bounds__(kDadtBlock) dadt_kernel(const float* __restrict__ dip, int n_dip,
                                                           const double* __restrict__ nodes_mm, std::int64_t n,
                                                           const double* __restrict__ origin, float scale, int K,
                                                           float* __restrict__ out) {
@@ -427,18 +313,11 @@ void place_transforms(const double* centers, const double* handles, const double
 }
 
 void dadt_nbody(const float* dip, int n_dip, const double* nodes_mm, std::int64_t n, const double* origins, int k,
-                int K, float scale, float* out, hipStream_t s, int variant) {
+                int K, float scale, float* out, hipStream_t s) {
     if (n <= 0 || k <= 0) return;
-    if (variant == 1) {
-        const std::int64_t threads = (n + kNodesPerThread - 1) / kNodesPerThread;
-        const unsigned gx = grid_for(threads, kDadtBlock);
-        dadt_valu_kernel<<<dim3(gx, k), kDadtBlock, 0, s>>>(dip, n_dip, nodes_mm, n, origins, scale, K, out);
-        return;
-    }
-    const std::int64_t blocks = (n + kDadtNodesPerBlock - 1) / kDadtNodesPerBlock;
-    require(blocks <= 0x7fffffff && k <= 65535, "dA/dt launch exceeds the grid limits");
-    dadt_kernel<<<dim3(static_cast<unsigned>(blocks), k), kDadtWaves * 64, 0, s>>>(dip, n_dip, nodes_mm, n, origins,
-                                                                                 scale, K, out);
+    const std::int64_t threads = (n + kNodesPerThread - 1) / kNodesPerThread;
+    const unsigned gx = grid_for(threads, kDadtBlock);
+    dadt_kernel<<<dim3(gx, k), kDadtBlock, 0, s>>>(dip, n_dip, nodes_mm, n, origins, scale, K, out);
 }
 
 void element_pass(const float* nodal, const std::int32_t* tets, const float* g, const float* neg_vc, std::int64_t m,

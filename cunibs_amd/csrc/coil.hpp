@@ -14,9 +14,8 @@ void place_transforms(const double* centers, const double* handles, const double
                       const double* tb, const double* tc, const double* tn, int n_tri, double* out, int* degenerate,
                       int n_cu, hipStream_t s);
 // dip: per placement n_dip x [s(3) m(3) m x s(3) |s|^2] FP32, centred at that placement's origin.
-// variant 0: FP32 matrix cores (the product path); 1: vector ALUs only (for comparison).
 void dadt_nbody(const float* dip, int n_dip, const double* nodes_mm, std::int64_t n, const double* origins, int k,
-                int K, float scale, float* out, hipStream_t s, int variant = 0);
+                int K, float scale, float* out, hipStream_t s);
 // Element average of the nodal dA/dt and the RHS corner terms; sigma_tag (K x n_sigma FP32, with
 // vols and tags) gives each placement its own conductivities instead of neg_vc.
 void element_pass(const float* nodal, const std::int32_t* tets, const float* g, const float* neg_vc, std::int64_t m,

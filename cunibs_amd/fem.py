@@ -485,16 +485,15 @@ class SolverContext:
         return {"samples": int(out[0]), "spmm_ms": float(out[1]), "vcycle_ms": float(out[2]),
                 "iteration_ms": float(out[3]), "dadt_placements": int(out[4]), "dadt_ms": float(out[5])}
 
-    def bench_dadt(self, dip_pos_m, dip_moment, site, k: int = 8, variant: int = 0, reps: int = 5) -> float:
-        """Average ms of the dA/dt N-body for k copies of one placement (variant 0: matrix cores,
-        1: vector ALUs)."""
+    def bench_dadt(self, dip_pos_m, dip_moment, site, k: int = 8, reps: int = 5) -> float:
+        """Average ms of the dA/dt N-body for k copies of one placement (repeated launches)."""
         pos = np.ascontiguousarray(dip_pos_m, np.float64).reshape(-1, 3)
         mom = np.ascontiguousarray(dip_moment, np.float64).reshape(-1, 3)
         c = np.ascontiguousarray(site[0], np.float64).reshape(3)
         h = np.ascontiguousarray(site[1], np.float64).reshape(3)
         ms = ctypes.c_double()
         check(lib().cn_bench_dadt(self.handle, ptr(pos), ptr(mom), pos.shape[0], ptr(c), ptr(h), float(site[2]), k,
-                                  variant, reps, ctypes.byref(ms)), "cn_bench_dadt")
+                                  reps, ctypes.byref(ms)), "cn_bench_dadt")
         return float(ms.value)
 
     def bench_copy(self, nbytes: int, reps: int = 20) -> float:

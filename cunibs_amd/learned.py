@@ -17,27 +17,25 @@ import numpy as np
 from cunibs_amd import _lib
 from cunibs_amd._lib import check, lib, ptr
 
-N_POSE = 4
+N_POSE = 6
 N_SIGMA = 10
-N_INPUTS = N_POSE + N_SIGMA  # (theta, phi, handle angle, distance) + 10 log-conductivities
+N_INPUTS = N_POSE + N_SIGMA  # target direction (x, y), handle direction (3), distance + 10 log sigma
 
 
 def pose_features(site, sigma: Sequence[float] | None = None) -> np.ndarray:
-    """(theta, phi) of the target direction, the handle's angle about it in the local tangent frame,
-    the coil distance, then log sigma of the ten tissues (zeros when the table is not varied)."""
+    """The pose as continuous coordinates: the target direction's (x, y) (its z follows on the upper
+    cap), the handle direction projected on the tangent plane at the target as a global unit vector
+    (3), the coil distance; then log sigma of the ten tissues (zeros when the table is not varied).
+    (theta, phi) and an in-plane handle angle would carry the same information, but angles wrap at
+    +-pi and the tangent frame turns fast near the pole, and the network fits a discontinuous input
+    badly; the pose takes six inputs instead of four, sixteen in all."""
     center, handle, dist = (np.asarray(site[0], np.float64), np.asarray(site[1], np.float64), float(site[2]))
     r = center / np.linalg.norm(center)
-    theta = np.arccos(np.clip(r[2], -1.0, 1.0))
-    phi = np.arctan2(r[1], r[0])
-    up = np.array([0.0, 0.0, 1.0]) if abs(r[2]) < 0.99 else np.array([1.0, 0.0, 0.0])
-    e1 = np.cross(up, r)
-    e1 /= np.linalg.norm(e1)
-    e2 = np.cross(r, e1)
     hd = handle - center
-    psi = np.arctan2(hd @ e2, hd @ e1)
+    hd = hd - (hd @ r) * r
+    hd /= np.linalg.norm(hd)
     logs = np.zeros(N_SIGMA) if sigma is None else np.log(np.asarray(sigma, np.float64))
-    # Angles enter through sin/cos-free raw values; the network sees standardized inputs.
-    return np.concatenate([[theta, phi, psi, dist], logs])
+    return np.concatenate([[r[0], r[1], hd[0], hd[1], hd[2], dist], logs])
 
 
 class WarmStartNetwork:

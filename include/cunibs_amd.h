@@ -323,11 +323,10 @@ int32_t cn_timing_enable(cn_context* ctx, int32_t on);
  * graphs), out[5] mean ms of dA/dt per placement. */
 int32_t cn_timing_read(const cn_context* ctx, double* out6);
 /* Average device time (ms) of the dA/dt N-body for k copies of one placement over the context's
- * nodes: variant 0 is the matrix-core kernel the solves use, 1 the vector-ALU formulation it is
- * measured against. A measurement aid with no counterpart in the reference. */
+ * nodes, repeated launches. A measurement aid with no counterpart in the reference. */
 int32_t cn_bench_dadt(cn_context* ctx, const double* dip_pos_m, const double* dip_moment, int64_t n_dip,
                       const double* center_mm, const double* handle_mm, double distance_mm, int32_t k,
-                      int32_t variant, int32_t reps, double* ms);
+                      int32_t reps, double* ms);
 /* Average device time (ms) of a streaming device-to-device copy of `bytes` bytes (a multiple of 16)
  * on the library's stream: the calibration launch for the HBM traffic counters and the practical
  * bandwidth ceiling. No counterpart in the reference (a measurement aid). */
