@@ -136,7 +136,9 @@ __device__ __forceinline__ void vstore(float* p, const float (&v)[V]) {
 // acc[v] = sum over j in [j, e) step SUB of av[j] * (X[ci[j] * K + c0 + v] (- xi[v] when DIFF)),
 // folded in the plain loop's order. Entries go in batches of U: the U index/value loads, then the
 // U gathers, are all issued before the first use, so a typical row is two memory round trips.
-// Entries per lane per batch: a stiffness row (about 15 entries) in one batch for SUB <= 2.
+// Every load is unconditional: an entry past the row's end reads the row's last entry again and
+// gets weight 0, so no branch splits the batch (a branch per entry serialises the loads).
+// Requires e > j implies e - 1 is a valid entry.
 template <int SUB>
 constexpr int gather_batch() {
     return SUB <= 2 ? 8 : (SUB == 4 ? 4 : 2);
@@ -153,26 +155,18 @@ __device__ __forceinline__ void row_gather_v(const std::int32_t* __restrict__ ci
         float aa[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const bool ok = j + u * SUB < e;
-            cc[u] = ok ? ci[j + u * SUB] : 0;
-            aa[u] = ok ? av[j + u * SUB] : 0.0f;
+            const int jj = min(j + u * SUB, e - 1);
+            cc[u] = ci[jj];
+            aa[u] = av[jj];
         }
         float xx[U][V];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (j + u * SUB < e) {
-                vload<V>(X + static_cast<std::int64_t>(cc[u]) * K + c0, xx[u]);
-            } else {
-#pragma unroll
-                for (int v = 0; v < V; ++v) xx[u][v] = 0.0f;
-            }
-        }
+        for (int u = 0; u < U; ++u) vload<V>(X + static_cast<std::int64_t>(cc[u]) * K + c0, xx[u]);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (j + u * SUB < e) {
+            const float a = (j + u * SUB < e) ? aa[u] : 0.0f;
 #pragma unroll
-                for (int v = 0; v < V; ++v) acc[v] = fmaf(aa[u], DIFF ? xx[u][v] - xi[v] : xx[u][v], acc[v]);
-            }
+            for (int v = 0; v < V; ++v) acc[v] = fmaf(a, DIFF ? xx[u][v] - xi[v] : xx[u][v], acc[v]);
         }
     }
 }
@@ -185,41 +179,30 @@ __device__ __forceinline__ void row_gather_multi(const std::int32_t* __restrict_
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v] = 0.0f;
     for (; j < e; j += U * SUB) {
-        std::int64_t cc[U];
+        int cc[U];
         float aa[U][V], xx[U][V];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (j + u * SUB < e) {
-                cc[u] = ci[j + u * SUB];
-                vload<V>(av + static_cast<std::int64_t>(j + u * SUB) * K + c0, aa[u]);
-            } else {
-                cc[u] = 0;
-#pragma unroll
-                for (int v = 0; v < V; ++v) aa[u][v] = 0.0f;
-            }
+            const int jj = min(j + u * SUB, e - 1);
+            cc[u] = ci[jj];
+            vload<V>(av + static_cast<std::int64_t>(jj) * K + c0, aa[u]);
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (j + u * SUB < e) {
-                vload<V>(X + cc[u] * K + c0, xx[u]);
-            } else {
-#pragma unroll
-                for (int v = 0; v < V; ++v) xx[u][v] = 0.0f;
-            }
-        }
+        for (int u = 0; u < U; ++u) vload<V>(X + static_cast<std::int64_t>(cc[u]) * K + c0, xx[u]);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (j + u * SUB < e) {
+            const bool ok = j + u * SUB < e;
 #pragma unroll
-                for (int v = 0; v < V; ++v) acc[v] = fmaf(aa[u][v], xx[u][v] - xi[v], acc[v]);
-            }
+            for (int v = 0; v < V; ++v) acc[v] = fmaf(ok ? aa[u][v] : 0.0f, xx[u][v] - xi[v], acc[v]);
         }
     }
 }
 
 constexpr int kRowThreads = 256;
 constexpr int kRowWaves = kRowThreads / kWave;
-constexpr int kRowIters = 4;  // row groups each wave walks per block
+// Row groups each wave walks per block. One: a second pass of the loop would have to wait for the
+// first pass's stores (out may alias x_prev, so the compiler orders them), a full memory round trip.
+constexpr int kRowIters = 1;
 
 template <int K, int SUB>
 constexpr std::int64_t rows_per_vblock() {
@@ -247,54 +230,55 @@ __global__ void __launch_bounds__(kRowThreads) rowk_kernel(
     constexpr int G = K / V;
     constexpr int L = G * SUB;
     constexpr int RPW = kWave / L;
+    static_assert(kRowIters == 1, "one row group per wavefront");
     const int lb = logical_block();
-    if (stamps != nullptr && threadIdx.x == 0) stamps[2 * lb] = wall_clock64();
+    // The block's start time stays in a register until the end: a store here would make the first
+    // loads wait for it.
+    const unsigned long long t_start = stamps != nullptr ? wall_clock64() : 0ull;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
     const int l = lane % L, grp = l % G, sub = l / G;
     const int c0 = grp * V;
     const std::int64_t lo = static_cast<std::int64_t>(lb) * rows_per_vblock<K, SUB>();
     const std::int64_t hi = min(lo + rows_per_vblock<K, SUB>(), n);
-    for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW; r0 < hi; r0 += kRowWaves * RPW) {
+    const std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW;
+    if (r0 < hi) {
         const std::int64_t row = r0 + lane / L;
         const bool valid = row < hi;
         const bool lead = valid && sub == 0;
-        const std::int64_t at = row * K + c0;
+        // Lanes past the last row read the last row (no branch around the loads) and gather nothing.
+        const std::int64_t rr = valid ? row : hi - 1;
+        const std::int64_t at = rr * K + c0;
+        // Row pointers first: the gather waits on them, and loads complete in issue order. Both are
+        // loaded unconditionally (a load under a condition becomes a branch and a wait).
+        const int p0 = rp[rr], p1 = rp[rr + 1];
+        const int j0 = p0 + sub;
+        const int e = valid ? p1 : j0;
         float xi[V], acc[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) xi[v] = 0.0f;
-        if constexpr (MODE == kSpmmDiff || MODE == kSpmmDiffMulti) {
-            if (valid) vload<V>(x + at, xi);
-        }
+        if constexpr (MODE == kSpmmDiff || MODE == kSpmmDiffMulti) vload<V>(x + at, xi);
         // The row's own operands go out before the gather, so their latency hides behind it
         // (x_prev is read before the store below either way: out may alias it).
         float bb[V], cur[V], prev[V], sv[V];
         float dg = 0.0f;
 #pragma unroll
         for (int v = 0; v < V; ++v) bb[v] = cur[v] = prev[v] = sv[v] = 0.0f;
-        if (lead) {
-            if constexpr (MODE == kResidual || MODE == kCheby) vload<V>(b + at, bb);
-            if constexpr (MODE == kCheby) {
-                vload<V>(x + at, cur);
-                vload<V>(x_prev + at, prev);
-            }
-            if constexpr (MODE == kCheby || MODE == kSpmmDiff) dg = diag[row];
-            if constexpr (MODE == kSpmmDiffMulti) vload<V>(diag + at, sv);
+        if constexpr (MODE == kResidual || MODE == kCheby) vload<V>(b + at, bb);
+        if constexpr (MODE == kCheby) {
+            vload<V>(x + at, cur);
+            vload<V>(x_prev + at, prev);
         }
-        if (valid) {
-            if constexpr (MODE == kSpmmDiffMulti)
-                row_gather_multi<K, V, SUB>(ci, av, rp[row] + sub, rp[row + 1], x, c0, xi, acc);
-            else
-                row_gather_v<K, V, SUB, MODE == kSpmmDiff>(ci, av, rp[row] + sub, rp[row + 1], x, c0, xi, acc);
-        } else {
-#pragma unroll
-            for (int v = 0; v < V; ++v) acc[v] = 0.0f;
-        }
+        if constexpr (MODE == kCheby || MODE == kSpmmDiff) dg = diag[rr];
+        if constexpr (MODE == kSpmmDiffMulti) vload<V>(diag + at, sv);
+        if constexpr (MODE == kSpmmDiffMulti)
+            row_gather_multi<K, V, SUB>(ci, av, j0, e, x, c0, xi, acc);
+        else
+            row_gather_v<K, V, SUB, MODE == kSpmmDiff>(ci, av, j0, e, x, c0, xi, acc);
 #pragma unroll
         for (int off = G; off < L; off <<= 1) {
 #pragma unroll
             for (int v = 0; v < V; ++v) acc[v] += __shfl_xor(acc[v], off, kWave);
         }
-        if (!lead) continue;
         float o[V];
         if constexpr (MODE == kSpmm) {
 #pragma unroll
@@ -313,11 +297,14 @@ __global__ void __launch_bounds__(kRowThreads) rowk_kernel(
 #pragma unroll
             for (int v = 0; v < V; ++v) o[v] = fmaf(d, bb[v] - acc[v], fmaf(c_prev, prev[v], c_cur * cur[v]));
         }
-        vstore<V>(out + at, o);
+        if (lead) vstore<V>(out + at, o);
     }
     if (stamps != nullptr) {  // the block's end: after its last wavefront's stores
         __syncthreads();
-        if (threadIdx.x == 0) stamps[2 * lb + 1] = wall_clock64();
+        if (threadIdx.x == 0) {
+            stamps[2 * lb] = t_start;
+            stamps[2 * lb + 1] = wall_clock64();
+        }
     }
 }
 
@@ -562,8 +549,9 @@ __global__ void __launch_bounds__(kRowThreads) prolongate_kernel(
     const int c0 = static_cast<int>(t % G) * V;
     const float zero[V] = {};
     float acc[V], cur[V];
-    row_gather_v<K, V, 1, false>(ci, pv, rp[row], rp[row + 1], coarse, c0, zero, acc);
-    vload<V>(fine + row * K + c0, cur);
+    const int j0 = rp[row], e = rp[row + 1];
+    vload<V>(fine + row * K + c0, cur);  // issued before the gather, used after it
+    row_gather_v<K, V, 1, false>(ci, pv, j0, e, coarse, c0, zero, acc);
 #pragma unroll
     for (int v = 0; v < V; ++v) cur[v] += acc[v];
     vstore<V>(fine + row * K + c0, cur);
@@ -1159,8 +1147,8 @@ void BlockSolver::ensure(int K) {
 }
 
 std::size_t BlockSolver::stamp_marks() const {
-    // room for two stamps per block of the finest row kernel at any tile (16 rows per block at least)
-    return 2 * static_cast<std::size_t>(n() / 16 + 64);
+    // room for two stamps per block of the finest row kernel at any tile (4 rows per block at least)
+    return 2 * static_cast<std::size_t>(n() / 4 + 64);
 }
 
 unsigned long long* BlockSolver::stamps() { return timing_ ? stamps_.get() : nullptr; }
