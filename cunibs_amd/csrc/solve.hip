@@ -769,6 +769,46 @@ __device__ void reduce_partials(const double* __restrict__ partials, int nb, int
     __syncthreads();
 }
 
+// The same sum over many blocks: out[e] = sum_b partials[b * stride + e] for e < E, one wavefront
+// per output; lane l adds b = l, l + 64, ... in order (all its loads issued together), then a fixed
+// xor tree over the lanes. Run-to-run deterministic like the one-block form, and a single memory
+// round trip instead of one per block of eight partials.
+constexpr int kReduceThreads = 256;
+
+__global__ void __launch_bounds__(kReduceThreads) reduce_cols_kernel(const double* __restrict__ partials, int nb,
+                                                                     int stride, int E, double* __restrict__ out,
+                                                                     const int* __restrict__ state) {
+    if (finished(state)) return;
+    const int w = static_cast<int>((static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & (kWave - 1);
+    if (w >= E) return;  // whole wavefronts
+    constexpr int kPer = kGramBlocks / kWave;
+    double v[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int b = lane + u * kWave;
+        v[u] = b < nb ? partials[static_cast<std::int64_t>(b) * stride + w] : 0.0;
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) acc += v[u];
+    for (int b = lane + kPer * kWave; b < nb; b += kWave) acc += partials[static_cast<std::int64_t>(b) * stride + w];
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+    if (lane == 0) out[w] = acc;
+}
+
+void reduce_cols(const double* partials, int nb, int E, double* out, const int* state, hipStream_t s) {
+    const unsigned g = static_cast<unsigned>((static_cast<std::int64_t>(E) * kWave + kReduceThreads - 1) / kReduceThreads);
+    reduce_cols_kernel<<<g, kReduceThreads, 0, s>>>(partials, nb, E, E, out, state);
+}
+
+template <int E>
+__device__ void load_reduced(const double* __restrict__ red, int off, double* out_lds) {
+    for (int e = threadIdx.x; e < E; e += blockDim.x) out_lds[e] = red[off + e];
+    __syncthreads();
+}
+
 // G, H -> T (pivoted Cholesky of the unit-diagonal scaling of G), TT = T T^T, a = TT H with the
 // frozen columns zeroed.
 template <int K>
@@ -789,8 +829,8 @@ __global__ void __launch_bounds__(kSmallThreads) coef_kernel(const double* __res
     __shared__ int idx[K];
     __shared__ int rem[K];
     __shared__ int rank_s, stop;
-    reduce_partials<K * K>(partials, nb, 2 * K * K, 0, A);
-    reduce_partials<K * K>(partials, nb, 2 * K * K, K * K, H);
+    load_reduced<K * K>(partials, 0, A);
+    load_reduced<K * K>(partials, K * K, H);
     const int t = threadIdx.x;
     if (independent) {  // K separate systems: each column is its own CG chain
         if (t < K * K && (t / K) != (t % K)) {
@@ -893,7 +933,7 @@ __global__ void __launch_bounds__(kSmallThreads) dmat_kernel(const double* __res
                                                              int* __restrict__ state, int independent) {
     if (finished(state)) return;
     __shared__ double E[K * K];
-    reduce_partials<K * K>(partials, nb, K * K, 0, E);
+    load_reduced<K * K>(partials, 0, E);
     const int t = threadIdx.x;
     if (independent) {
         if (t < K * K && (t / K) != (t % K)) E[t] = 0.0;
@@ -916,7 +956,7 @@ __global__ void __launch_bounds__(kSmallThreads) converge_kernel(const double* _
     if (finished(state)) return;
     __shared__ double v[K];
     __shared__ int all;
-    reduce_partials<K>(partials, nb, K, 0, v);
+    load_reduced<K>(partials, 0, v);
     const int t = threadIdx.x;
     if (t == 0) all = 1;
     __syncthreads();
@@ -1134,6 +1174,7 @@ void BlockSolver::ensure(int K) {
     cb_.reset(nc, "coarse b");
     cx_.reset(nc, "coarse x");
     partials_.reset(static_cast<std::size_t>(kGramBlocks) * 2 * K * K, "gram partials");
+    red_.reset(static_cast<std::size_t>(2) * K * K, "reduced partials");
     G_.reset(K * K, "G");
     T_.reset(K * K, "T");
     TT_.reset(K * K, "TT");
@@ -1235,17 +1276,20 @@ void BlockSolver::iteration_k(const SolveOptions& opt) {
     int* stw = state_.get();
     mark(0, st);
     spmm_block_k<K>(st);
-    coef_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, opt.drop_tol, G_.get(), T_.get(),
+    reduce_cols(partials_.get(), kGramBlocks, 2 * K * K, red_.get(), st, s_);
+    coef_kernel<K><<<1, kSmallThreads, 0, s_>>>(red_.get(), kGramBlocks, opt.drop_tol, G_.get(), T_.get(),
                                                 TT_.get(), AL_.get(), stw, multi_ ? 1 : 0);
     update_xr_kernel<K><<<kGramBlocks, xr_threads<K>(), 0, s_>>>(nn, W_.get(), S_.get(), AL_.get(), X64_.get(), R_.get(),
                                                              partials_.get(), rpb, st);
-    converge_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, target_.get(), norms_.get(), stw);
+    reduce_cols(partials_.get(), kGramBlocks, K, red_.get(), st, s_);
+    converge_kernel<K><<<1, kSmallThreads, 0, s_>>>(red_.get(), kGramBlocks, target_.get(), norms_.get(), stw);
     mark(1, st);
     vcycle_k<K>(R_.get(), Z_.get(), st);
     mark(2, st);
     gram_rows_kernel<K, false><<<kGramBlocks, gram_threads<K, false>(), 0, s_>>>(nn, S_.get(), Z_.get(), nullptr,
                                                                          partials_.get(), rpb, st);
-    dmat_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, TT_.get(), D_.get(), stw,
+    reduce_cols(partials_.get(), kGramBlocks, K * K, red_.get(), st, s_);
+    dmat_kernel<K><<<1, kSmallThreads, 0, s_>>>(red_.get(), kGramBlocks, TT_.get(), D_.get(), stw,
                                                 multi_ ? 1 : 0);
     update_w_kernel<K><<<grid_for(nn * col_groups<K>(), kRowThreads), kRowThreads, 0, s_>>>(nn, Z_.get(), D_.get(),
                                                                                            W_.get(), st);

@@ -138,6 +138,7 @@ class BlockSolver {
     DeviceBuffer<float> cb_, cx_;  // coarsest level
     // reductions and small state
     DeviceBuffer<double> partials_;  // kGramBlocks * 2 * K * K
+    DeviceBuffer<double> red_;       // the partials summed over the blocks (2 K^2)
     DeviceBuffer<double> G_, T_, TT_, AL_, D_, norms_, target_;
     DeviceBuffer<int> state_;  // [0] done, [1] iteration, [2..2+K) frozen, [2+K..2+2K) col_iters
     int* host_state_ = nullptr;  // pinned
