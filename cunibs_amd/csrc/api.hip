@@ -358,6 +358,71 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
 
 // Dipoles of one placement in its coil frame, FP64 as fem/placement.py:294-313, then FP32 pack
 // [s | m | m x s | |s|^2] centred at the coil origin.
+// The direction every dipole moment lies along, if there is one (coil models of planar windings
+// put all their moments on the coil normal): |m_j x d| <= 1e-12 |m_j| for all j. Writes the unit
+// direction of the largest moment.
+bool common_moment_direction(const double* mom, std::int64_t n_dip, double* d) {
+    std::int64_t big = 0;
+    double best = 0.0;
+    for (std::int64_t j = 0; j < n_dip; ++j) {
+        const double q = mom[3 * j] * mom[3 * j] + mom[3 * j + 1] * mom[3 * j + 1] + mom[3 * j + 2] * mom[3 * j + 2];
+        if (q > best) {
+            best = q;
+            big = j;
+        }
+    }
+    if (!(best > 0.0)) return false;
+    const double nb = std::sqrt(best);
+    for (int r = 0; r < 3; ++r) d[r] = mom[3 * big + r] / nb;
+    for (std::int64_t j = 0; j < n_dip; ++j) {
+        const double* m = mom + 3 * j;
+        const double cx = m[1] * d[2] - m[2] * d[1], cy = m[2] * d[0] - m[0] * d[2], cz = m[0] * d[1] - m[1] * d[0];
+        const double mm = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+        if (cx * cx + cy * cy + cz * cz > 1e-24 * mm) return false;
+    }
+    return true;
+}
+
+// Per placement, for the parallel-moment N-body: -2 s (3), |s|^2, mu, mu s (3) in FP32 with s
+// relative to the coil origin (as pack_dipoles), and the rotated direction.
+void pack_dipoles_parallel(const double* tf, const double* pos_m, const double* mom, std::int64_t n_dip,
+                           const double* dir, float* out, double* origin, float* dir_out) {
+    double rot[3][3], trans[3];
+    for (int r = 0; r < 3; ++r) {
+        for (int q = 0; q < 3; ++q) rot[r][q] = tf[4 * r + q];
+        trans[r] = tf[4 * r + 3];
+    }
+    const double o[3] = {trans[0] * 1e-3, trans[1] * 1e-3, trans[2] * 1e-3};
+    for (int r = 0; r < 3; ++r) origin[r] = o[r];
+    for (int r = 0; r < 3; ++r) {
+        double acc = 0.0;
+        for (int q = 0; q < 3; ++q) acc += dir[q] * rot[r][q];
+        dir_out[r] = static_cast<float>(acc);
+    }
+    for (std::int64_t j = 0; j < n_dip; ++j) {
+        double sd[3];
+        for (int r = 0; r < 3; ++r) {
+            double acc = 0.0;
+            for (int q = 0; q < 3; ++q) acc += pos_m[3 * j + q] * 1e3 * rot[r][q];
+            sd[r] = (acc + trans[r]) * 1e-3 - o[r];
+        }
+        const double mu = mom[3 * j] * dir[0] + mom[3 * j + 1] * dir[1] + mom[3 * j + 2] * dir[2];
+        float* p = out + 8 * j;
+        const float sf[3] = {static_cast<float>(sd[0]), static_cast<float>(sd[1]), static_cast<float>(sd[2])};
+        for (int r = 0; r < 3; ++r) {
+            p[r] = -2.0f * sf[r];
+            p[5 + r] = static_cast<float>(mu * sd[r]);
+        }
+        p[3] = sf[0] * sf[0] + sf[1] * sf[1] + sf[2] * sf[2];
+        p[4] = static_cast<float>(mu);
+    }
+}
+
+// dA/dt at the nodes for k placements (frames tf, k x 16) into out (n, 3, K): the parallel-moment
+// kernel when the coil allows it, the general one otherwise.
+void dadt_for_frames(cn_context& c, const double* tf, int k, int K, const double* pos_m, const double* mom,
+                     std::int64_t n_dip, float scale, float* out, hipStream_t s);
+
 void pack_dipoles(const double* tf, const double* pos_m, const double* mom, std::int64_t n_dip, float* out,
                   double* origin) {
     double rot[3][3], trans[3];
@@ -386,6 +451,35 @@ void pack_dipoles(const double* tf, const double* pos_m, const double* mom, std:
             p[6 + r] = static_cast<float>(mxs[r]);
         }
         p[9] = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];  // from the FP32 positions, as the reference
+    }
+}
+
+void dadt_for_frames(cn_context& c, const double* tf, int k, int K, const double* pos_m, const double* mom,
+                     std::int64_t n_dip, float scale, float* out, hipStream_t s) {
+    using namespace cn;
+    std::vector<double> origins(3 * k);
+    double dir[3];
+    if (common_moment_direction(mom, n_dip, dir)) {
+        std::vector<float> dip(static_cast<std::size_t>(8 * n_dip * k)), dirs(3 * k);
+        for (int i = 0; i < k; ++i)
+            pack_dipoles_parallel(tf + 16 * i, pos_m, mom, n_dip, dir, dip.data() + 8 * n_dip * i,
+                                  origins.data() + 3 * i, dirs.data() + 3 * i);
+        c.d_dip.ensure(dip.size() + dirs.size(), "dipoles");
+        c.d_origins.ensure(3 * k, "origins");
+        h2d(c.d_dip.get(), dip.data(), dip.size(), s);
+        h2d(c.d_dip.get() + dip.size(), dirs.data(), dirs.size(), s);
+        h2d(c.d_origins.get(), origins.data(), 3 * k, s);
+        dadt_nbody_parallel(c.d_dip.get(), static_cast<int>(n_dip), c.nodes.get(), c.n, c.d_origins.get(),
+                            c.d_dip.get() + dip.size(), k, K, scale, out, s);
+    } else {
+        std::vector<float> dip(static_cast<std::size_t>(10 * n_dip * k));
+        for (int i = 0; i < k; ++i)
+            pack_dipoles(tf + 16 * i, pos_m, mom, n_dip, dip.data() + 10 * n_dip * i, origins.data() + 3 * i);
+        c.d_dip.ensure(dip.size(), "dipoles");
+        c.d_origins.ensure(3 * k, "origins");
+        h2d(c.d_dip.get(), dip.data(), dip.size(), s);
+        h2d(c.d_origins.get(), origins.data(), 3 * k, s);
+        dadt_nbody(c.d_dip.get(), static_cast<int>(n_dip), c.nodes.get(), c.n, c.d_origins.get(), k, K, scale, out, s);
     }
 }
 
@@ -774,13 +868,6 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
         }
     }
     // 2. dA/dt at nodes, element average, RHS.
-    std::vector<float> dip(static_cast<std::size_t>(10 * n_dip * k));
-    std::vector<double> origins(3 * k);
-    for (int i = 0; i < k; ++i) pack_dipoles(tf.data() + 16 * i, pos_m, mom, n_dip, dip.data() + 10 * n_dip * i, origins.data() + 3 * i);
-    c.d_dip.ensure(dip.size(), "dipoles");
-    c.d_origins.ensure(3 * k, "origins");
-    h2d(c.d_dip.get(), dip.data(), dip.size(), s);
-    h2d(c.d_origins.get(), origins.data(), 3 * k, s);
     c.d_dadt_nodes.ensure(3 * n * K, "dadt nodes");
     c.d_dadt_elm.ensure(3 * m * K, "dadt elements");
     c.d_q.ensure(4 * m * K, "corner terms");
@@ -791,8 +878,7 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
             for (auto& e : c.dadt_ev) check(hipEventCreate(&e), "dadt event");
         check(hipEventRecord(c.dadt_ev[0], s), "dadt event");
     }
-    dadt_nbody(c.d_dip.get(), static_cast<int>(n_dip), c.nodes.get(), n, c.d_origins.get(), k, K, scale,
-               c.d_dadt_nodes.get(), s);
+    dadt_for_frames(c, tf.data(), k, K, pos_m, mom, n_dip, scale, c.d_dadt_nodes.get(), s);
     if (c.solver->timing_on()) {
         check(hipEventRecord(c.dadt_ev[1], s), "dadt event");
         check(hipEventSynchronize(c.dadt_ev[1]), "dadt event");
@@ -1413,17 +1499,12 @@ int32_t cn_bench_dadt(cn_context* c, const double* dip_pos_m, const double* dip_
         std::vector<double> tf(16);
         std::vector<std::int32_t> degen(1);
         place_block(*c, center_mm, handle_mm, &distance_mm, 1, tf.data(), degen.data());
-        std::vector<float> dip(static_cast<std::size_t>(10 * n_dip * k));
-        std::vector<double> origins(3 * k);
-        for (int i = 0; i < k; ++i)
-            pack_dipoles(tf.data(), dip_pos_m, dip_moment, n_dip, dip.data() + 10 * n_dip * i, origins.data() + 3 * i);
-        auto ddip = cn::upload(dip.data(), dip.size(), s, "dipoles");
-        auto dorg = cn::upload(origins.data(), origins.size(), s, "origins");
+        std::vector<double> frames(16 * static_cast<std::size_t>(k));
+        for (int i = 0; i < k; ++i) std::copy(tf.begin(), tf.end(), frames.begin() + 16 * i);
         const int K = cn::padded_width(k);
         cn::DeviceBuffer<float> out(3 * c->n * K, "dadt");
         auto run = [&] {
-            cn::dadt_nbody(ddip.get(), static_cast<int>(n_dip), c->nodes.get(), c->n, dorg.get(), k, K, 1e-7f,
-                           out.get(), s);
+            dadt_for_frames(*c, frames.data(), k, K, dip_pos_m, dip_moment, n_dip, 1e-7f, out.get(), s);
         };
         run();
         hipEvent_t e0, e1;

@@ -16,6 +16,10 @@ void place_transforms(const double* centers, const double* handles, const double
 // dip: per placement n_dip x [s(3) m(3) m x s(3) |s|^2] FP32, centred at that placement's origin.
 void dadt_nbody(const float* dip, int n_dip, const double* nodes_mm, std::int64_t n, const double* origins, int k,
                 int K, float scale, float* out, hipStream_t s);
+// All moments along one direction per placement (dirs: k x 3): dip8 per placement n_dip x
+// [-2 s(3) |s|^2 mu mu s(3)] FP32.
+void dadt_nbody_parallel(const float* dip8, int n_dip, const double* nodes_mm, std::int64_t n, const double* origins,
+                         const float* dirs, int k, int K, float scale, float* out, hipStream_t s);
 // Element average of the nodal dA/dt and the RHS corner terms; sigma_tag (K x n_sigma FP32, with
 // vols and tags) gives each placement its own conductivities instead of neg_vc.
 void element_pass(const float* nodal, const std::int32_t* tets, const float* g, const float* neg_vc, std::int64_t m,

@@ -274,3 +274,22 @@ def test_kernel_dadt_nbody(oracle_mod):
     sn = (s * s).sum(1)
     got = fem.dadt_nbody(s, mp, sn, r, 1e6)
     assert rel(got, oracle_mod.dadt_direct(s, m, r, 1e6)) < 1e-5
+
+
+def test_dadt_general_and_parallel_moment_paths(gctx, coil, oracle_mod, octx):
+    """The block solve uses the four-term N-body when every moment lies along one direction (the
+    synthetic figure-8) and the general six-term one otherwise; both match the FP64 closed form.
+    A 1e-6 tilt of the moments sends the same coil down the general path."""
+    from cunibs_amd import fem, synthetic
+    site = synthetic.SPHERE_SITE
+    rng = np.random.default_rng(4)
+    tilted = coil.moments + 1e-6 * np.abs(coil.moments).max() * rng.standard_normal(coil.moments.shape)
+    out = {}
+    for name, mom in (("parallel", coil.moments), ("general", tilted)):
+        ex = {"dadt_elm": 1}
+        fem.solve_placements_block(gctx, coil.positions_m, mom, [site], 1e6, magnitude=False, metrics=False,
+                                   extra=ex)
+        ref = oracle_mod.solve_site(octx, coil.positions_m, mom, site, 1e6, tol=1e-8)
+        assert rel(ex["dadt_elm"][0], ref["dadt_elm"]) < 1e-5, name
+        out[name] = ex["dadt_elm"][0]
+    assert rel(out["general"], out["parallel"]) < 1e-5
