@@ -155,9 +155,11 @@ __device__ __forceinline__ void row_gather_v(const std::int32_t* __restrict__ ci
         float aa[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
+            // The operator streams through once per pass: non-temporal loads keep it from evicting
+            // the gathered block rows from L2.
             const int jj = min(j + u * SUB, e - 1);
-            cc[u] = ci[jj];
-            aa[u] = av[jj];
+            cc[u] = __builtin_nontemporal_load(ci + jj);
+            aa[u] = __builtin_nontemporal_load(av + jj);
         }
         float xx[U][V];
 #pragma unroll
