@@ -155,11 +155,11 @@ __device__ __forceinline__ void row_gather_v(const std::int32_t* __restrict__ ci
         float aa[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            // The operator streams through once per pass: non-temporal loads keep it from evicting
-            // the gathered block rows from L2.
+            // (Non-temporal loads of the operator were measured: they slow every row kernel by
+            // 20-75 %, so the operator goes through the caches like everything else.)
             const int jj = min(j + u * SUB, e - 1);
-            cc[u] = __builtin_nontemporal_load(ci + jj);
-            aa[u] = __builtin_nontemporal_load(av + jj);
+            cc[u] = ci[jj];
+            aa[u] = av[jj];
         }
         float xx[U][V];
 #pragma unroll
@@ -811,6 +811,13 @@ __device__ void load_reduced(const double* __restrict__ red, int off, double* ou
     __syncthreads();
 }
 
+// Threads of the one-block K x K kernels: one per matrix entry (whole wavefronts), so the barriers
+// of the factorisation span as few wavefronts as the work needs.
+template <int K>
+constexpr int small_threads() {
+    return K * K <= kWave ? kWave : K * K;
+}
+
 // G, H -> T (pivoted Cholesky of the unit-diagonal scaling of G), TT = T T^T, a = TT H with the
 // frozen columns zeroed.
 template <int K>
@@ -1279,19 +1286,19 @@ void BlockSolver::iteration_k(const SolveOptions& opt) {
     mark(0, st);
     spmm_block_k<K>(st);
     reduce_cols(partials_.get(), kGramBlocks, 2 * K * K, red_.get(), st, s_);
-    coef_kernel<K><<<1, kSmallThreads, 0, s_>>>(red_.get(), kGramBlocks, opt.drop_tol, G_.get(), T_.get(),
+    coef_kernel<K><<<1, small_threads<K>(), 0, s_>>>(red_.get(), kGramBlocks, opt.drop_tol, G_.get(), T_.get(),
                                                 TT_.get(), AL_.get(), stw, multi_ ? 1 : 0);
     update_xr_kernel<K><<<kGramBlocks, xr_threads<K>(), 0, s_>>>(nn, W_.get(), S_.get(), AL_.get(), X64_.get(), R_.get(),
                                                              partials_.get(), rpb, st);
     reduce_cols(partials_.get(), kGramBlocks, K, red_.get(), st, s_);
-    converge_kernel<K><<<1, kSmallThreads, 0, s_>>>(red_.get(), kGramBlocks, target_.get(), norms_.get(), stw);
+    converge_kernel<K><<<1, small_threads<K>(), 0, s_>>>(red_.get(), kGramBlocks, target_.get(), norms_.get(), stw);
     mark(1, st);
     vcycle_k<K>(R_.get(), Z_.get(), st);
     mark(2, st);
     gram_rows_kernel<K, false><<<kGramBlocks, gram_threads<K, false>(), 0, s_>>>(nn, S_.get(), Z_.get(), nullptr,
                                                                          partials_.get(), rpb, st);
     reduce_cols(partials_.get(), kGramBlocks, K * K, red_.get(), st, s_);
-    dmat_kernel<K><<<1, kSmallThreads, 0, s_>>>(red_.get(), kGramBlocks, TT_.get(), D_.get(), stw,
+    dmat_kernel<K><<<1, small_threads<K>(), 0, s_>>>(red_.get(), kGramBlocks, TT_.get(), D_.get(), stw,
                                                 multi_ ? 1 : 0);
     update_w_kernel<K><<<grid_for(nn * col_groups<K>(), kRowThreads), kRowThreads, 0, s_>>>(nn, Z_.get(), D_.get(),
                                                                                            W_.get(), st);
