@@ -23,6 +23,11 @@ import numpy as np
 
 METRIC = "placements/s @ rel.res 1e-6, GM |E| err<=1e-3 vs FP64; batched+warm vs plain"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the differenced SpMM, from two rocprofv3 counter passes over
+# `bench.py --kernels-only --block-k 16 --fine-sub 2` (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the
+# factors that make the counters read 512 MiB each way for the streaming-copy launch), keyed by
+# (grid spacing mm, block width). 198.4 MB fetched + 64.2 MB written, 1.07x the algorithmic bytes.
+MEASURED_SPMM_TRAFFIC = {(1.5, 16): 262.6e6}
 
 
 def log(*a):
@@ -363,7 +368,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": MEASURED_SPMM_TRAFFIC.get((args.spacing, K)),
             "avg_us": round(spmm_ms * 1e3, 2),
             "samples": timing["samples"],
             "algorithmic_bytes": spmm_diff_bytes(n_red, nnz, K),

@@ -372,23 +372,44 @@ __global__ void __launch_bounds__((gram_threads<K, TWO>())) gram_rows_kernel(
     double g1[K], g2[K];
 #pragma unroll
     for (int c = 0; c < K; ++c) g1[c] = g2[c] = 0.0;
-    for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW; r0 < hi; r0 += WAVES * RPW) {
-        const std::int64_t row = r0 + rg;
-        if (row < hi) {
-            const double av = static_cast<double>(A[row * K + a]);
+    // Software-pipelined: the next row group's loads are issued before this one's products, so a
+    // wavefront always has a row group in flight. Rows past the range read the last row with
+    // weight 0 (no branch around the loads).
+    constexpr int NB = TWO ? 2 : 1;
+    float ca, cb[NB][K];
+    auto fetch = [&](std::int64_t row, float& av, float (&bv)[NB][K]) {
+        const std::int64_t rr = row < hi ? row : hi - 1;
+        av = row < hi ? A[rr * K + a] : 0.0f;
 #pragma unroll
-            for (int c = 0; c < K; c += V) {
-                float bv[V];
-                vload<V>(B1 + row * K + c, bv);
+        for (int c = 0; c < K; c += V) {
+            float t[V];
+            vload<V>(B1 + rr * K + c, t);
 #pragma unroll
-                for (int v = 0; v < V; ++v) g1[c + v] = fma(av, static_cast<double>(bv[v]), g1[c + v]);
-                if constexpr (TWO) {
-                    vload<V>(B2 + row * K + c, bv);
+            for (int v = 0; v < V; ++v) bv[0][c + v] = t[v];
+            if constexpr (TWO) {
+                vload<V>(B2 + rr * K + c, t);
 #pragma unroll
-                    for (int v = 0; v < V; ++v) g2[c + v] = fma(av, static_cast<double>(bv[v]), g2[c + v]);
-                }
+                for (int v = 0; v < V; ++v) bv[NB - 1][c + v] = t[v];
             }
         }
+    };
+    std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW;
+    if (r0 < hi) fetch(r0 + rg, ca, cb);
+    for (; r0 < hi; r0 += WAVES * RPW) {
+        float na = 0.0f, nb[NB][K];
+        const std::int64_t rn = r0 + WAVES * RPW;
+        if (rn < hi) fetch(rn + rg, na, nb);
+        const double av = static_cast<double>(ca);
+#pragma unroll
+        for (int c = 0; c < K; ++c) {
+            g1[c] = fma(av, static_cast<double>(cb[0][c]), g1[c]);
+            if constexpr (TWO) g2[c] = fma(av, static_cast<double>(cb[NB - 1][c]), g2[c]);
+        }
+        ca = na;
+#pragma unroll
+        for (int q = 0; q < NB; ++q)
+#pragma unroll
+            for (int c = 0; c < K; ++c) cb[q][c] = nb[q][c];
     }
 #pragma unroll
     for (int c = 0; c < K; ++c) {
