@@ -144,10 +144,13 @@ constexpr int gather_batch() {
     return SUB <= 2 ? 8 : (SUB == 4 ? 4 : 2);
 }
 
-template <int K, int V, int SUB, bool DIFF, int U = gather_batch<SUB>()>
+// With GS, the gathered rows are scaled on the fly: X_j -> (gs0 * gscale[j]) X_j (the Jacobi
+// iterate from zero, alpha0 dinv_j b_j, without writing it out).
+template <int K, int V, int SUB, bool DIFF, int U = gather_batch<SUB>(), bool GS = false>
 __device__ __forceinline__ void row_gather_v(const std::int32_t* __restrict__ ci, const float* __restrict__ av, int j,
                                              int e, const float* __restrict__ X, int c0, const float (&xi)[V],
-                                             float (&acc)[V]) {
+                                             float (&acc)[V], const float* __restrict__ gscale = nullptr,
+                                             float gs0 = 0.0f) {
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v] = 0.0f;
     for (; j < e; j += U * SUB) {
@@ -162,13 +165,23 @@ __device__ __forceinline__ void row_gather_v(const std::int32_t* __restrict__ ci
             aa[u] = av[jj];
         }
         float xx[U][V];
+        float gw[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) vload<V>(X + static_cast<std::int64_t>(cc[u]) * K + c0, xx[u]);
+        for (int u = 0; u < U; ++u) {
+            vload<V>(X + static_cast<std::int64_t>(cc[u]) * K + c0, xx[u]);
+            if constexpr (GS) gw[u] = gscale[cc[u]];
+        }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const float a = (j + u * SUB < e) ? aa[u] : 0.0f;
+            if constexpr (GS) {
+                const float w = gs0 * gw[u];
 #pragma unroll
-            for (int v = 0; v < V; ++v) acc[v] = fmaf(a, DIFF ? xx[u][v] - xi[v] : xx[u][v], acc[v]);
+                for (int v = 0; v < V; ++v) acc[v] = fmaf(a, w * xx[u][v], acc[v]);
+            } else {
+#pragma unroll
+                for (int v = 0; v < V; ++v) acc[v] = fmaf(a, DIFF ? xx[u][v] - xi[v] : xx[u][v], acc[v]);
+            }
         }
     }
 }
@@ -211,7 +224,7 @@ constexpr std::int64_t rows_per_vblock() {
     return static_cast<std::int64_t>(kRowWaves) * (kWave / ((K / vec_width<K>()) * SUB)) * kRowIters;
 }
 
-enum RowMode : int { kResidual = 0, kCheby = 1, kSpmm = 2, kSpmmDiff = 3, kSpmmDiffMulti = 4 };
+enum RowMode : int { kResidual = 0, kCheby = 1, kSpmm = 2, kSpmmDiff = 3, kSpmmDiffMulti = 4, kChebyZero = 5 };
 
 // One kernel for every SpMV-shaped pass:
 //   kResidual  out = b - A x
@@ -221,6 +234,9 @@ enum RowMode : int { kResidual = 0, kCheby = 1, kSpmm = 2, kSpmmDiff = 3, kSpmmD
 //              `diag` holding the row sums s_i
 //   kSpmmDiffMulti  the same for K matrices sharing the pattern: values (nnz, K) and row sums
 //              (n, K), column c of x multiplied by matrix c
+//   kChebyZero the first two smoothing steps from a zero guess in one pass: with
+//              x1 = alpha0 dinv b (formed on the fly for the row and its neighbours, never
+//              stored), out = c_cur x1 + alpha dinv (b - A x1); `x` is b and c_prev carries alpha0
 template <int K, int SUB, int MODE>
 __global__ void __launch_bounds__(kRowThreads) rowk_kernel(
     std::int64_t n, const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
@@ -265,15 +281,17 @@ __global__ void __launch_bounds__(kRowThreads) rowk_kernel(
         float dg = 0.0f;
 #pragma unroll
         for (int v = 0; v < V; ++v) bb[v] = cur[v] = prev[v] = sv[v] = 0.0f;
-        if constexpr (MODE == kResidual || MODE == kCheby) vload<V>(b + at, bb);
+        if constexpr (MODE == kResidual || MODE == kCheby || MODE == kChebyZero) vload<V>(b + at, bb);
         if constexpr (MODE == kCheby) {
             vload<V>(x + at, cur);
             vload<V>(x_prev + at, prev);
         }
-        if constexpr (MODE == kCheby || MODE == kSpmmDiff) dg = diag[rr];
+        if constexpr (MODE == kCheby || MODE == kSpmmDiff || MODE == kChebyZero) dg = diag[rr];
         if constexpr (MODE == kSpmmDiffMulti) vload<V>(diag + at, sv);
         if constexpr (MODE == kSpmmDiffMulti)
             row_gather_multi<K, V, SUB>(ci, av, j0, e, x, c0, xi, acc);
+        else if constexpr (MODE == kChebyZero)
+            row_gather_v<K, V, SUB, false, gather_batch<SUB>(), true>(ci, av, j0, e, x, c0, xi, acc, diag, c_prev);
         else
             row_gather_v<K, V, SUB, MODE == kSpmmDiff>(ci, av, j0, e, x, c0, xi, acc);
 #pragma unroll
@@ -294,6 +312,13 @@ __global__ void __launch_bounds__(kRowThreads) rowk_kernel(
         } else if constexpr (MODE == kResidual) {
 #pragma unroll
             for (int v = 0; v < V; ++v) o[v] = bb[v] - acc[v];
+        } else if constexpr (MODE == kChebyZero) {
+            const float d = alpha * dg, w = c_prev * dg;  // c_prev carries alpha0
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const float x1 = w * bb[v];
+                o[v] = fmaf(d, bb[v] - acc[v], fmaf(0.0f, x1, c_cur * x1));
+            }
         } else {
             const float d = alpha * dg;
 #pragma unroll
@@ -1092,6 +1117,14 @@ void run_vcycle(const Hierarchy& h, const std::vector<int>& subs, std::vector<De
         const Level& L = h.levels[li];
         const int sub = subs[li];
         auto target = [&](int st) { return (st + 1 == ch.degree && final_out) ? final_out : ((st % 2) ? other : first); };
+        if (x0 == nullptr && ch.degree == 2) {
+            // Jacobi from zero and the Chebyshev step after it, in one pass (the Jacobi iterate is
+            // only ever read by that step).
+            float* out = target(1);
+            launch_rows<K, kChebyZero>(sub, L.a, L.dinv.get(), bi, bi, nullptr, ch.c_cur[0], ch.alpha0, ch.alpha[0],
+                                       out, state, s);
+            return out;
+        }
         if (x0 == nullptr) {
             jacobi_zero_kernel<K><<<grid_for(L.a.n * K), kBlock, 0, s>>>(L.a.n, L.dinv.get(), bi, ch.alpha0, target(0), state);
         } else {
