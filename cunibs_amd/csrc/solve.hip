@@ -371,92 +371,110 @@ __global__ void __launch_bounds__(kBlock) timing_accumulate_kernel(const unsigne
     }
 }
 
-// G1 = A^T B1 (and G2 = A^T B2), FP64 partials of one logical block: lane a of a row group holds
-// column a of A and accumulates row a of G from the B rows, which every lane of the group reads
-// whole (a broadcast). No cross-lane traffic until the block reduction.
-// Block size of the streaming block-vector kernels: as many waves as the accumulators' registers
-// allow, since each wave walks its rows with one load round trip per step.
+// G1 = A^T B1 (and G2 = A^T B2), FP64 partials of one logical block's row range. The block stages
+// tiles of TR rows of A, B1 (, B2) through LDS with coalesced 16-byte loads, the next tile's loads
+// issued before the current tile's products; thread t owns entries p = t, t + 256, ... of the
+// K x K result(s) and adds the tile's rows to them in row order (FP64 products of the FP32 inputs).
+// Every entry has one owner, so the block partial needs no reduction across threads.
 template <int K, bool TWO>
 constexpr int gram_threads() {
-    return (TWO ? 2 : 1) * K <= 16 ? 1024 : ((TWO ? 2 : 1) * K <= 32 ? 512 : 256);
+    return 256;
 }
 
+constexpr int kGramTileRows = 32;
+
 template <int K, bool TWO>
-__global__ void __launch_bounds__((gram_threads<K, TWO>())) gram_rows_kernel(
+__global__ void __launch_bounds__(256) gram_rows_kernel(
     std::int64_t n, const float* __restrict__ A, const float* __restrict__ B1, const float* __restrict__ B2,
     double* __restrict__ partials, std::int64_t rows_per_block, const int* __restrict__ state) {
     if (finished(state)) return;
-    constexpr int RPW = kWave / K;
-    constexpr int WAVES = gram_threads<K, TWO>() / kWave;
-    constexpr int V = vec_width<K>();
+    constexpr int NB = TWO ? 2 : 1;
+    constexpr int TR = kGramTileRows;
+    constexpr int TILE = TR * K;                 // floats per array per tile
+    constexpr int TILE4 = (TILE + 3) / 4;        // 16-byte chunks per array per tile
+    constexpr int CHUNKS = (1 + NB) * TILE4;     // chunks per tile, all arrays
+    constexpr int PER = (CHUNKS + 255) / 256;    // chunks each thread loads
+    constexpr int KK = K * K;
+    constexpr int OWN = (KK + 255) / 256;        // result entries per thread
+    __shared__ float sA[TILE], sB[NB][TILE];
+    const int t = threadIdx.x;
     const int lb = logical_block();
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
-    const int a = lane % K, rg = lane / K;
     const std::int64_t lo = static_cast<std::int64_t>(lb) * rows_per_block;
     const std::int64_t hi = min(lo + rows_per_block, n);
-    double g1[K], g2[K];
+    double g[NB][OWN];
 #pragma unroll
-    for (int c = 0; c < K; ++c) g1[c] = g2[c] = 0.0;
-    // Software-pipelined: the next row group's loads are issued before this one's products, so a
-    // wavefront always has a row group in flight. Rows past the range read the last row with
-    // weight 0 (no branch around the loads).
-    constexpr int NB = TWO ? 2 : 1;
-    float ca, cb[NB][K];
-    auto fetch = [&](std::int64_t row, float& av, float (&bv)[NB][K]) {
-        const std::int64_t rr = row < hi ? row : hi - 1;
-        av = row < hi ? A[rr * K + a] : 0.0f;
+    for (int q = 0; q < NB; ++q)
 #pragma unroll
-        for (int c = 0; c < K; c += V) {
-            float t[V];
-            vload<V>(B1 + rr * K + c, t);
+        for (int o = 0; o < OWN; ++o) g[q][o] = 0.0;
+    // Chunk c of a tile: array c / TILE4 (0 = A, 1 = B1, 2 = B2), offset 4 (c % TILE4) floats.
+    // Rows past the range load as zeros (the tile is a contiguous TR x K slab of each array).
+    auto fetch = [&](std::int64_t r0, float4 (&buf)[PER]) {
 #pragma unroll
-            for (int v = 0; v < V; ++v) bv[0][c + v] = t[v];
-            if constexpr (TWO) {
-                vload<V>(B2 + rr * K + c, t);
-#pragma unroll
-                for (int v = 0; v < V; ++v) bv[NB - 1][c + v] = t[v];
+        for (int k = 0; k < PER; ++k) {
+            const int c = t + 256 * k;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (c < CHUNKS) {
+                const int arr = c / TILE4, off = 4 * (c % TILE4);
+                const std::int64_t e = r0 * K + off;  // element index of the chunk
+                const float* src = arr == 0 ? A : (arr == 1 ? B1 : B2);
+                if (e + 4 <= hi * K) {
+                    v = *reinterpret_cast<const float4*>(src + e);
+                } else {
+                    float tmp[4] = {0.f, 0.f, 0.f, 0.f};
+                    for (int q = 0; q < 4; ++q)
+                        if (e + q < hi * K) tmp[q] = src[e + q];
+                    v = make_float4(tmp[0], tmp[1], tmp[2], tmp[3]);
+                }
             }
+            buf[k] = v;
         }
     };
-    std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW;
-    if (r0 < hi) fetch(r0 + rg, ca, cb);
-    for (; r0 < hi; r0 += WAVES * RPW) {
-        float na = 0.0f, nb[NB][K];
-        const std::int64_t rn = r0 + WAVES * RPW;
-        if (rn < hi) fetch(rn + rg, na, nb);
-        const double av = static_cast<double>(ca);
+    float4 nxt[PER];
+    if (lo < hi) fetch(lo, nxt);
+    for (std::int64_t r0 = lo; r0 < hi; r0 += TR) {
+        __syncthreads();  // the previous tile's products are done with the LDS copy
 #pragma unroll
-        for (int c = 0; c < K; ++c) {
-            g1[c] = fma(av, static_cast<double>(cb[0][c]), g1[c]);
-            if constexpr (TWO) g2[c] = fma(av, static_cast<double>(cb[NB - 1][c]), g2[c]);
-        }
-        ca = na;
-#pragma unroll
-        for (int q = 0; q < NB; ++q)
-#pragma unroll
-            for (int c = 0; c < K; ++c) cb[q][c] = nb[q][c];
-    }
-#pragma unroll
-    for (int c = 0; c < K; ++c) {
-        g1[c] = wave_rows_sum<K>(g1[c]);
-        if constexpr (TWO) g2[c] = wave_rows_sum<K>(g2[c]);
-    }
-    // Block partial in wave order: lanes a < K of each wave add row a.
-    __shared__ double acc[(TWO ? 2 : 1) * K * K];
-    for (int e = threadIdx.x; e < (TWO ? 2 : 1) * K * K; e += blockDim.x) acc[e] = 0.0;
-    __syncthreads();
-    for (int w = 0; w < WAVES; ++w) {
-        if (wave == w && lane < K) {
-#pragma unroll
-            for (int c = 0; c < K; ++c) {
-                acc[a * K + c] += g1[c];
-                if constexpr (TWO) acc[K * K + a * K + c] += g2[c];
+        for (int k = 0; k < PER; ++k) {
+            const int c = t + 256 * k;
+            if (c < CHUNKS) {
+                const int arr = c / TILE4, off = 4 * (c % TILE4);
+                float* dst = arr == 0 ? sA : sB[arr - 1];
+                if (off + 4 <= TILE) {
+                    *reinterpret_cast<float4*>(dst + off) = nxt[k];
+                } else {
+                    const float tmp[4] = {nxt[k].x, nxt[k].y, nxt[k].z, nxt[k].w};
+                    for (int q = 0; q < 4 && off + q < TILE; ++q) dst[off + q] = tmp[q];
+                }
             }
         }
         __syncthreads();
+        if (r0 + TR < hi) fetch(r0 + TR, nxt);  // in flight during this tile's products
+        const int rows = static_cast<int>(min<std::int64_t>(TR, hi - r0));
+#pragma unroll
+        for (int o = 0; o < OWN; ++o) {
+            const int p = t + 256 * o;
+            if (p < KK) {
+                const int a = p / K, c = p % K;
+                double s1 = g[0][o], s2 = g[NB - 1][o];
+                for (int r = 0; r < rows; ++r) {
+                    const double av = static_cast<double>(sA[r * K + a]);
+                    s1 = fma(av, static_cast<double>(sB[0][r * K + c]), s1);
+                    if constexpr (TWO) s2 = fma(av, static_cast<double>(sB[NB - 1][r * K + c]), s2);
+                }
+                g[0][o] = s1;
+                if constexpr (TWO) g[NB - 1][o] = s2;
+            }
+        }
     }
-    double* out = partials + static_cast<std::int64_t>(lb) * (TWO ? 2 : 1) * K * K;
-    for (int e = threadIdx.x; e < (TWO ? 2 : 1) * K * K; e += blockDim.x) out[e] = acc[e];
+    double* out = partials + static_cast<std::int64_t>(lb) * NB * KK;
+#pragma unroll
+    for (int o = 0; o < OWN; ++o) {
+        const int p = t + 256 * o;
+        if (p < KK) {
+            out[p] = g[0][o];
+            if constexpr (TWO) out[KK + p] = g[NB - 1][o];
+        }
+    }
 }
 
 constexpr int kGramThreads = 1024;
@@ -1049,6 +1067,8 @@ __global__ void column_norms_kernel(const double* __restrict__ partials, int nb,
 }
 
 std::int64_t gram_rows(std::int64_t n) { return (n + kGramBlocks - 1) / kGramBlocks; }
+// The Gram kernel's rows per block: a multiple of 4, so every tile starts on a 16-byte boundary.
+std::int64_t gram_rows4(std::int64_t n) { return (gram_rows(n) + 3) / 4 * 4; }
 
 // Sub-lanes per column group (G = K / V groups per row) for a level's row kernels: enough that
 // each lane walks a few nonzeros, more when the level is too small to fill the machine
@@ -1328,7 +1348,7 @@ void BlockSolver::spmm_block_k(const int* state) {
                                                0.f, 0.f, S_.get(), state, s_, stamps());
     }
     gram_rows_kernel<K, true><<<kGramBlocks, gram_threads<K, true>(), 0, s_>>>(nn, W_.get(), S_.get(), R_.get(),
-                                                                        partials_.get(), gram_rows(nn), state);
+                                                                        partials_.get(), gram_rows4(nn), state);
 }
 
 template <int K>
@@ -1350,7 +1370,7 @@ void BlockSolver::iteration_k(const SolveOptions& opt) {
     vcycle_k<K>(R_.get(), Z_.get(), st);
     mark(2, st);
     gram_rows_kernel<K, false><<<kGramBlocks, gram_threads<K, false>(), 0, s_>>>(nn, S_.get(), Z_.get(), nullptr,
-                                                                         partials_.get(), rpb, st);
+                                                                         partials_.get(), gram_rows4(nn), st);
     reduce_cols(partials_.get(), kGramBlocks, K * K, red_.get(), st, s_);
     dmat_kernel<K><<<1, small_threads<K>(), 0, s_>>>(red_.get(), kGramBlocks, TT_.get(), D_.get(), stw,
                                                 multi_ ? 1 : 0);
@@ -1506,7 +1526,7 @@ void BlockSolver::cross_gram32(const float* A, const float* B, double* out_host)
     ensure(32);
     const std::int64_t nn = n();
     gram_rows_kernel<32, false><<<kGramBlocks, gram_threads<32, false>(), 0, s_>>>(nn, A, B, nullptr, partials_.get(),
-                                                                               gram_rows(nn), nullptr);
+                                                                               gram_rows4(nn), nullptr);
     sum_partials_kernel<32 * 32><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, G_.get());
     d2h(out_host, G_.get(), 32 * 32, s_);
     sync(s_, "cross gram");
