@@ -381,7 +381,7 @@ constexpr int gram_threads() {
     return 256;
 }
 
-constexpr int kGramTileRows = 32;
+constexpr int kGramTileRows = 128;
 
 template <int K, bool TWO>
 __global__ void __launch_bounds__(256) gram_rows_kernel(
