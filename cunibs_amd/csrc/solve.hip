@@ -224,7 +224,9 @@ constexpr std::int64_t rows_per_vblock() {
     return static_cast<std::int64_t>(kRowWaves) * (kWave / ((K / vec_width<K>()) * SUB)) * kRowIters;
 }
 
-enum RowMode : int { kResidual = 0, kCheby = 1, kSpmm = 2, kSpmmDiff = 3, kSpmmDiffMulti = 4, kChebyZero = 5 };
+enum RowMode : int {
+    kResidual = 0, kCheby = 1, kSpmm = 2, kSpmmDiff = 3, kSpmmDiffMulti = 4, kChebyZero = 5, kProlong = 6
+};
 
 // One kernel for every SpMV-shaped pass:
 //   kResidual  out = b - A x
@@ -234,6 +236,8 @@ enum RowMode : int { kResidual = 0, kCheby = 1, kSpmm = 2, kSpmmDiff = 3, kSpmmD
 //              `diag` holding the row sums s_i
 //   kSpmmDiffMulti  the same for K matrices sharing the pattern: values (nnz, K) and row sums
 //              (n, K), column c of x multiplied by matrix c
+//   kProlong   x_prev += P x over the prolongator's rows (x the coarse vector, x_prev the fine one,
+//              updated in place by the row's own lanes)
 //   kChebyZero the first two smoothing steps from a zero guess in one pass: with
 //              x1 = alpha0 dinv b (formed on the fly for the row and its neighbours, never
 //              stored), out = c_cur x1 + alpha dinv (b - A x1); `x` is b and c_prev carries alpha0
@@ -286,6 +290,7 @@ __global__ void __launch_bounds__(kRowThreads) rowk_kernel(
             vload<V>(x + at, cur);
             vload<V>(x_prev + at, prev);
         }
+        if constexpr (MODE == kProlong) vload<V>(x_prev + at, prev);  // out is x_prev
         if constexpr (MODE == kCheby || MODE == kSpmmDiff || MODE == kChebyZero) dg = diag[rr];
         if constexpr (MODE == kSpmmDiffMulti) vload<V>(diag + at, sv);
         if constexpr (MODE == kSpmmDiffMulti)
@@ -303,6 +308,9 @@ __global__ void __launch_bounds__(kRowThreads) rowk_kernel(
         if constexpr (MODE == kSpmm) {
 #pragma unroll
             for (int v = 0; v < V; ++v) o[v] = acc[v];
+        } else if constexpr (MODE == kProlong) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) o[v] = prev[v] + acc[v];
         } else if constexpr (MODE == kSpmmDiff) {
 #pragma unroll
             for (int v = 0; v < V; ++v) o[v] = fmaf(dg, xi[v], acc[v]);
@@ -371,110 +379,71 @@ __global__ void __launch_bounds__(kBlock) timing_accumulate_kernel(const unsigne
     }
 }
 
-// G1 = A^T B1 (and G2 = A^T B2), FP64 partials of one logical block's row range. The block stages
-// tiles of TR rows of A, B1 (, B2) through LDS with coalesced 16-byte loads, the next tile's loads
-// issued before the current tile's products; thread t owns entries p = t, t + 256, ... of the
-// K x K result(s) and adds the tile's rows to them in row order (FP64 products of the FP32 inputs).
-// Every entry has one owner, so the block partial needs no reduction across threads.
+// G1 = A^T B1 (and G2 = A^T B2), FP64 partials of one logical block: lane a of a row group holds
+// column a of A and accumulates row a of G from the B rows, which every lane of the group reads
+// whole (a broadcast). No cross-lane traffic until the block reduction.
+// Block size of the streaming block-vector kernels: as many waves as the accumulators' registers
+// allow, since each wave walks its rows with one load round trip per step.
 template <int K, bool TWO>
 constexpr int gram_threads() {
-    return 256;
+    return (TWO ? 2 : 1) * K <= 16 ? 1024 : ((TWO ? 2 : 1) * K <= 32 ? 512 : 256);
 }
 
-constexpr int kGramTileRows = 128;
-
 template <int K, bool TWO>
-__global__ void __launch_bounds__(256) gram_rows_kernel(
+__global__ void __launch_bounds__((gram_threads<K, TWO>())) gram_rows_kernel(
     std::int64_t n, const float* __restrict__ A, const float* __restrict__ B1, const float* __restrict__ B2,
     double* __restrict__ partials, std::int64_t rows_per_block, const int* __restrict__ state) {
     if (finished(state)) return;
-    constexpr int NB = TWO ? 2 : 1;
-    constexpr int TR = kGramTileRows;
-    constexpr int TILE = TR * K;                 // floats per array per tile
-    constexpr int TILE4 = (TILE + 3) / 4;        // 16-byte chunks per array per tile
-    constexpr int CHUNKS = (1 + NB) * TILE4;     // chunks per tile, all arrays
-    constexpr int PER = (CHUNKS + 255) / 256;    // chunks each thread loads
-    constexpr int KK = K * K;
-    constexpr int OWN = (KK + 255) / 256;        // result entries per thread
-    __shared__ float sA[TILE], sB[NB][TILE];
-    const int t = threadIdx.x;
+    constexpr int RPW = kWave / K;
+    constexpr int WAVES = gram_threads<K, TWO>() / kWave;
+    constexpr int V = vec_width<K>();
     const int lb = logical_block();
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
+    const int a = lane % K, rg = lane / K;
     const std::int64_t lo = static_cast<std::int64_t>(lb) * rows_per_block;
     const std::int64_t hi = min(lo + rows_per_block, n);
-    double g[NB][OWN];
+    double g1[K], g2[K];
 #pragma unroll
-    for (int q = 0; q < NB; ++q)
+    for (int c = 0; c < K; ++c) g1[c] = g2[c] = 0.0;
+    for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW; r0 < hi; r0 += WAVES * RPW) {
+        const std::int64_t row = r0 + rg;
+        if (row < hi) {
+            const double av = static_cast<double>(A[row * K + a]);
 #pragma unroll
-        for (int o = 0; o < OWN; ++o) g[q][o] = 0.0;
-    // Chunk c of a tile: array c / TILE4 (0 = A, 1 = B1, 2 = B2), offset 4 (c % TILE4) floats.
-    // Rows past the range load as zeros (the tile is a contiguous TR x K slab of each array).
-    auto fetch = [&](std::int64_t r0, float4 (&buf)[PER]) {
+            for (int c = 0; c < K; c += V) {
+                float bv[V];
+                vload<V>(B1 + row * K + c, bv);
 #pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int c = t + 256 * k;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (c < CHUNKS) {
-                const int arr = c / TILE4, off = 4 * (c % TILE4);
-                const std::int64_t e = r0 * K + off;  // element index of the chunk
-                const float* src = arr == 0 ? A : (arr == 1 ? B1 : B2);
-                if (e + 4 <= hi * K) {
-                    v = *reinterpret_cast<const float4*>(src + e);
-                } else {
-                    float tmp[4] = {0.f, 0.f, 0.f, 0.f};
-                    for (int q = 0; q < 4; ++q)
-                        if (e + q < hi * K) tmp[q] = src[e + q];
-                    v = make_float4(tmp[0], tmp[1], tmp[2], tmp[3]);
+                for (int v = 0; v < V; ++v) g1[c + v] = fma(av, static_cast<double>(bv[v]), g1[c + v]);
+                if constexpr (TWO) {
+                    vload<V>(B2 + row * K + c, bv);
+#pragma unroll
+                    for (int v = 0; v < V; ++v) g2[c + v] = fma(av, static_cast<double>(bv[v]), g2[c + v]);
                 }
             }
-            buf[k] = v;
         }
-    };
-    float4 nxt[PER];
-    if (lo < hi) fetch(lo, nxt);
-    for (std::int64_t r0 = lo; r0 < hi; r0 += TR) {
-        __syncthreads();  // the previous tile's products are done with the LDS copy
+    }
 #pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int c = t + 256 * k;
-            if (c < CHUNKS) {
-                const int arr = c / TILE4, off = 4 * (c % TILE4);
-                float* dst = arr == 0 ? sA : sB[arr - 1];
-                if (off + 4 <= TILE) {
-                    *reinterpret_cast<float4*>(dst + off) = nxt[k];
-                } else {
-                    const float tmp[4] = {nxt[k].x, nxt[k].y, nxt[k].z, nxt[k].w};
-                    for (int q = 0; q < 4 && off + q < TILE; ++q) dst[off + q] = tmp[q];
-                }
+    for (int c = 0; c < K; ++c) {
+        g1[c] = wave_rows_sum<K>(g1[c]);
+        if constexpr (TWO) g2[c] = wave_rows_sum<K>(g2[c]);
+    }
+    // Block partial in wave order: lanes a < K of each wave add row a.
+    __shared__ double acc[(TWO ? 2 : 1) * K * K];
+    for (int e = threadIdx.x; e < (TWO ? 2 : 1) * K * K; e += blockDim.x) acc[e] = 0.0;
+    __syncthreads();
+    for (int w = 0; w < WAVES; ++w) {
+        if (wave == w && lane < K) {
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                acc[a * K + c] += g1[c];
+                if constexpr (TWO) acc[K * K + a * K + c] += g2[c];
             }
         }
         __syncthreads();
-        if (r0 + TR < hi) fetch(r0 + TR, nxt);  // in flight during this tile's products
-        const int rows = static_cast<int>(min<std::int64_t>(TR, hi - r0));
-#pragma unroll
-        for (int o = 0; o < OWN; ++o) {
-            const int p = t + 256 * o;
-            if (p < KK) {
-                const int a = p / K, c = p % K;
-                double s1 = g[0][o], s2 = g[NB - 1][o];
-                for (int r = 0; r < rows; ++r) {
-                    const double av = static_cast<double>(sA[r * K + a]);
-                    s1 = fma(av, static_cast<double>(sB[0][r * K + c]), s1);
-                    if constexpr (TWO) s2 = fma(av, static_cast<double>(sB[NB - 1][r * K + c]), s2);
-                }
-                g[0][o] = s1;
-                if constexpr (TWO) g[NB - 1][o] = s2;
-            }
-        }
     }
-    double* out = partials + static_cast<std::int64_t>(lb) * NB * KK;
-#pragma unroll
-    for (int o = 0; o < OWN; ++o) {
-        const int p = t + 256 * o;
-        if (p < KK) {
-            out[p] = g[0][o];
-            if constexpr (TWO) out[KK + p] = g[NB - 1][o];
-        }
-    }
+    double* out = partials + static_cast<std::int64_t>(lb) * (TWO ? 2 : 1) * K * K;
+    for (int e = threadIdx.x; e < (TWO ? 2 : 1) * K * K; e += blockDim.x) out[e] = acc[e];
 }
 
 constexpr int kGramThreads = 1024;
@@ -569,58 +538,6 @@ __global__ void jacobi_zero_kernel(std::int64_t n, const float* __restrict__ din
     if (finished(state)) return;
     const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n * K) x[i] = alpha * dinv[i / K] * b[i];
-}
-
-// b_coarse = R r_fine, one wavefront per coarse row (R rows are the widest in the hierarchy).
-template <int K>
-__global__ void __launch_bounds__(kRowThreads) restrict_kernel(
-    std::int64_t n_coarse, const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
-    const float* __restrict__ rv, const float* __restrict__ fine, float* __restrict__ out,
-    const int* __restrict__ state) {
-    if (finished(state)) return;
-    constexpr int V = vec_width<K>();
-    constexpr int G = K / V;
-    constexpr int SUB = kWave / G;
-    const std::int64_t row = static_cast<std::int64_t>(blockIdx.x) * kRowWaves + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & (kWave - 1);
-    const int grp = lane % G, sub = lane / G;
-    const float zero[V] = {};
-    float acc[V];
-    if (row < n_coarse) {
-        row_gather_v<K, V, SUB, false>(ci, rv, rp[row] + sub, rp[row + 1], fine, grp * V, zero, acc);
-    } else {
-#pragma unroll
-        for (int v = 0; v < V; ++v) acc[v] = 0.0f;
-    }
-#pragma unroll
-    for (int off = G; off < kWave; off <<= 1) {
-#pragma unroll
-        for (int v = 0; v < V; ++v) acc[v] += __shfl_xor(acc[v], off, kWave);
-    }
-    if (row < n_coarse && sub == 0) vstore<V>(out + row * K + grp * V, acc);
-}
-
-// x_fine += P x_coarse, one lane per (row, column group).
-template <int K>
-__global__ void __launch_bounds__(kRowThreads) prolongate_kernel(
-    std::int64_t n, const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
-    const float* __restrict__ pv, const float* __restrict__ coarse, float* __restrict__ fine,
-    const int* __restrict__ state) {
-    if (finished(state)) return;
-    constexpr int V = vec_width<K>();
-    constexpr int G = K / V;
-    const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= n * G) return;
-    const std::int64_t row = t / G;
-    const int c0 = static_cast<int>(t % G) * V;
-    const float zero[V] = {};
-    float acc[V], cur[V];
-    const int j0 = rp[row], e = rp[row + 1];
-    vload<V>(fine + row * K + c0, cur);  // issued before the gather, used after it
-    row_gather_v<K, V, 1, false>(ci, pv, j0, e, coarse, c0, zero, acc);
-#pragma unroll
-    for (int v = 0; v < V; ++v) cur[v] += acc[v];
-    vstore<V>(fine + row * K + c0, cur);
 }
 
 // Coarsest level in FP64: x = Ainv b, one wavefront per row of the dense inverse.
@@ -1067,8 +984,6 @@ __global__ void column_norms_kernel(const double* __restrict__ partials, int nb,
 }
 
 std::int64_t gram_rows(std::int64_t n) { return (n + kGramBlocks - 1) / kGramBlocks; }
-// The Gram kernel's rows per block: a multiple of 4, so every tile starts on a 16-byte boundary.
-std::int64_t gram_rows4(std::int64_t n) { return (gram_rows(n) + 3) / 4 * 4; }
 
 // Sub-lanes per column group (G = K / V groups per row) for a level's row kernels: enough that
 // each lane walks a few nonzeros, more when the level is too small to fill the machine
@@ -1125,7 +1040,7 @@ unsigned launch_rows(int sub, const CsrView& a, const float* diag, const float* 
 template <int K>
 void run_vcycle(const Hierarchy& h, const std::vector<int>& subs, std::vector<DeviceBuffer<float>>& lb,
                 std::vector<DeviceBuffer<float>>& lx, std::vector<DeviceBuffer<float>>& lr, float* cb, float* cx,
-                const float* B, float* X, const int* state, hipStream_t s) {
+                const float* B, float* X, const int* state, int n_cu, hipStream_t s) {
     const Cheby& ch = h.cheb;
     const int nl = static_cast<int>(h.levels.size());
     if (nl == 0) {
@@ -1170,8 +1085,9 @@ void run_vcycle(const Hierarchy& h, const std::vector<int>& subs, std::vector<De
         launch_rows<K, kResidual>(subs[i], L.a, L.dinv.get(), bi, lx[i].get(), lx[i].get(), 0.f, 0.f, 0.f, lr[i].get(),
                                   state, s);
         float* next_b = (i + 1 < nl) ? lb[i + 1].get() : cb;
-        restrict_kernel<K><<<grid_for(L.n_coarse, kRowWaves), kRowThreads, 0, s>>>(
-            L.n_coarse, L.r.ptr.get(), L.r.col.get(), L.r.val.get(), lr[i].get(), next_b, state);
+        const CsrView R = L.r.view();
+        launch_rows<K, kSpmm>(choose_sub(R.n, R.nnz, col_groups<K>(), n_cu), R, nullptr, nullptr, lr[i].get(),
+                              nullptr, 0.f, 0.f, 0.f, next_b, state, s);
     }
     coarse_kernel<K><<<grid_for(h.n_coarse(), kRowWaves), kRowThreads, 0, s>>>(static_cast<int>(h.n_coarse()),
                                                                                 h.coarse_inv.get(), cb, cx, state);
@@ -1179,8 +1095,9 @@ void run_vcycle(const Hierarchy& h, const std::vector<int>& subs, std::vector<De
     for (int i = nl - 1; i >= 0; --i) {
         const Level& L = h.levels[i];
         const float* bi = (i == 0) ? B : lb[i].get();
-        prolongate_kernel<K><<<grid_for(L.a.n * col_groups<K>()), kBlock, 0, s>>>(L.a.n, L.p.ptr.get(), L.p.col.get(), L.p.val.get(),
-                                                                    xc, lx[i].get(), state);
+        const CsrView P = L.p.view();
+        launch_rows<K, kProlong>(choose_sub(P.n, P.nnz, col_groups<K>(), n_cu), P, nullptr, nullptr, xc, lx[i].get(),
+                                 0.f, 0.f, 0.f, lx[i].get(), state, s);
         xc = smooth(i, bi, lx[i].get(), lr[i].get(), lx[i].get(), (i == 0) ? X : nullptr);
     }
 }
@@ -1324,7 +1241,7 @@ void BlockSolver::vcycle_k(const float* B, float* X, const int* state) {
     for (std::size_t i = 0; i < h_.levels.size(); ++i)
         subs[i] = choose_sub(h_.levels[i].a.n, h_.levels[i].a.nnz, col_groups<K>(), n_cu_);
     if (!subs.empty() && fine_sub_override_ > 0) subs[0] = std::min(fine_sub_override_, kWave / col_groups<K>());
-    run_vcycle<K>(h_, subs, lb_, lx_, lr_, cb_.get(), cx_.get(), B, X, state, s_);
+    run_vcycle<K>(h_, subs, lb_, lx_, lr_, cb_.get(), cx_.get(), B, X, state, n_cu_, s_);
 }
 
 template <int K>
@@ -1348,7 +1265,7 @@ void BlockSolver::spmm_block_k(const int* state) {
                                                0.f, 0.f, S_.get(), state, s_, stamps());
     }
     gram_rows_kernel<K, true><<<kGramBlocks, gram_threads<K, true>(), 0, s_>>>(nn, W_.get(), S_.get(), R_.get(),
-                                                                        partials_.get(), gram_rows4(nn), state);
+                                                                        partials_.get(), gram_rows(nn), state);
 }
 
 template <int K>
@@ -1370,7 +1287,7 @@ void BlockSolver::iteration_k(const SolveOptions& opt) {
     vcycle_k<K>(R_.get(), Z_.get(), st);
     mark(2, st);
     gram_rows_kernel<K, false><<<kGramBlocks, gram_threads<K, false>(), 0, s_>>>(nn, S_.get(), Z_.get(), nullptr,
-                                                                         partials_.get(), gram_rows4(nn), st);
+                                                                         partials_.get(), rpb, st);
     reduce_cols(partials_.get(), kGramBlocks, K * K, red_.get(), st, s_);
     dmat_kernel<K><<<1, small_threads<K>(), 0, s_>>>(red_.get(), kGramBlocks, TT_.get(), D_.get(), stw,
                                                 multi_ ? 1 : 0);
@@ -1526,7 +1443,7 @@ void BlockSolver::cross_gram32(const float* A, const float* B, double* out_host)
     ensure(32);
     const std::int64_t nn = n();
     gram_rows_kernel<32, false><<<kGramBlocks, gram_threads<32, false>(), 0, s_>>>(nn, A, B, nullptr, partials_.get(),
-                                                                               gram_rows4(nn), nullptr);
+                                                                               gram_rows(nn), nullptr);
     sum_partials_kernel<32 * 32><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, G_.get());
     d2h(out_host, G_.get(), 32 * 32, s_);
     sync(s_, "cross gram");
