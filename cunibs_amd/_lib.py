@@ -126,6 +126,7 @@ _SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "cn_k_spgemm": (_I32, [_I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "cn_k_spmm": (_I32, [_P, _P, _P, _I32]),
     "cn_k_spmm_diff": (_I32, [_P, _P, _P, _I32]),
+    "cn_k_gram": (_I32, [_P, _P, _P, _P, _P, _I32]),
     "cn_k_vcycle": (_I32, [_P, _P, _P, _I32]),
     "cn_k_dadt": (_I32, [_P, _P, _P, _I64, _P, _I64, _F, _F, _P]),
     "cn_k_place": (_I32, [_P, _P, _P, _P, _I32, _P, _P]),

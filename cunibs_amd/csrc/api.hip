@@ -108,6 +108,8 @@ struct cn_context {
         std::vector<int> tags;
         cn::DeviceBuffer<double> vals, off, rowsum;  // (T, nnz_red), (T, nnz_off), (T, n_red) FP64
     } tissue;
+    // Device buffers of the per-block region summary (grow-only, reused by every block).
+    cn::MetricsWorkspace metrics_ws;
     // dA/dt timing of the blocks solved while in-solve timing is on (HIP events, outside graphs).
     hipEvent_t dadt_ev[2] = {nullptr, nullptr};
     double dadt_ms = 0.0;
@@ -806,7 +808,7 @@ void uq_solve(cn_context& c, const double* sigma, int k, const cn_solve_options*
         }
         if (out->metrics)
             region_metrics(c.d_magn.get(), m, k, K, c.region_idx.get(), c.n_region, c.vols32.get(), c.bary.get(),
-                           out->metrics, s);
+                           c.metrics_ws, out->metrics, s);
         for (int i = 0; i < k; ++i) {
             if (out->iterations) out->iterations[i] = st.column_iterations[i];
             if (out->relative_residual) out->relative_residual[i] = st.relative_residual[i];
@@ -1035,7 +1037,7 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
         }
         if (out->metrics)
             region_metrics(c.d_magn.get(), m, k, K, c.region_idx.get(), c.n_region, c.vols32.get(), c.bary.get(),
-                           out->metrics, s);
+                           c.metrics_ws, out->metrics, s);
         for (int i = 0; i < k; ++i) {
             if (out->iterations) out->iterations[i] = st.column_iterations[i];
             if (out->relative_residual) out->relative_residual[i] = st.relative_residual[i];
@@ -1407,6 +1409,20 @@ int32_t cn_k_spmm_diff(cn_context* c, const float* X, float* Y, int32_t K) {
         cn::check(hipGetLastError(), "spmm launch");
         cn::d2h(Y, dy.get(), nk, c->s);
         cn::sync(c->s, "spmm");
+    });
+}
+
+int32_t cn_k_gram(cn_context* c, const float* A, const float* B1, const float* B2, double* out, int32_t K) {
+    return guarded([&] {
+        cn::require(c != nullptr, "null context");
+        cn::require(A != nullptr && B1 != nullptr && out != nullptr, "null operand");
+        cn::require(cn::padded_width(K) == K, "K must be a compiled block width");
+        const int64_t nk = (c->n - 1) * K;
+        auto da = cn::upload(A, nk, c->s, "A");
+        auto db1 = cn::upload(B1, nk, c->s, "B1");
+        cn::DeviceBuffer<float> db2;
+        if (B2 != nullptr) db2 = cn::upload(B2, nk, c->s, "B2");
+        c->solver->gram(da.get(), db1.get(), B2 != nullptr ? db2.get() : nullptr, out, K);
     });
 }
 

@@ -36,8 +36,19 @@ void reconstruct(const double* X, const std::int32_t* tets, const float* g, cons
 void unpack(const float* in, std::int64_t rows, int inner, int k, int K, float* out, hipStream_t s);
 
 constexpr int kMetricsLen = 16;
-// Region summary per placement (metrics.py:209-243) from |E| laid out (m, K).
+// Device buffers of the region summary, kept by the caller across blocks (grow-only).
+struct MetricsWorkspace {
+    DeviceBuffer<float> vals;                  // (k, n_region) gathered |E|
+    DeviceBuffer<std::uint64_t> keys, sorted;  // placement << 32 | bits of |E|
+    DeviceBuffer<std::int32_t> pos, order;     // region position, in sorted order
+    DeviceBuffer<double> w, cw, totals;        // sorted volumes, their prefix sums, scan tile totals
+    DeviceBuffer<double> part, small;          // block partials; per-placement scalars and results
+    DeviceBuffer<unsigned char> scratch;       // radix sort temporary storage
+};
+// Region summary per placement (metrics.py:209-243) from |E| laid out (m, K), all k placements of
+// the block in one pass.
 void region_metrics(const float* magn, std::int64_t m, int k, int K, const std::int32_t* region_idx,
-                    std::int64_t n_region, const float* vols, const double* bary, double* out_host, hipStream_t s);
+                    std::int64_t n_region, const float* vols, const double* bary, MetricsWorkspace& ws,
+                    double* out_host, hipStream_t s);
 
 }  // namespace cn

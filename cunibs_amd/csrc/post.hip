@@ -68,22 +68,32 @@ __global__ void unpack_kernel(const float* __restrict__ in, std::int64_t rows, i
 // ------------------------------------------------------------------------------------------------
 constexpr int kStatBlocks = 256;
 
-__global__ void gather_region_kernel(const float* __restrict__ magn, int K, const std::int32_t* __restrict__ idx,
-                                     std::int64_t nr, int c, float* __restrict__ vals, std::uint32_t* __restrict__ keys) {
+// Every placement of the block at once: thread j reads its element's K-wide |E| row and writes
+// vals[c nr + j], the sort key c << 32 | bits(|E|) (|E| >= 0, so the bits order like the values
+// and the placement index keeps the placements apart) and its region position j.
+__global__ void gather_region_kernel(const float* __restrict__ magn, int K, int k, const std::int32_t* __restrict__ idx,
+                                     std::int64_t nr, float* __restrict__ vals, std::uint64_t* __restrict__ keys,
+                                     std::int32_t* __restrict__ pos) {
     const std::int64_t j = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= nr) return;
-    const float v = magn[static_cast<std::int64_t>(idx[j]) * K + c];
-    vals[j] = v;
-    keys[j] = __float_as_uint(v);  // |E| >= 0: the bit pattern orders like the value
+    const float* row = magn + static_cast<std::int64_t>(idx[j]) * K;
+    for (int c = 0; c < k; ++c) {
+        const float v = row[c];
+        const std::int64_t t = c * nr + j;
+        vals[t] = v;
+        keys[t] = (static_cast<std::uint64_t>(c) << 32) | __float_as_uint(v);
+        pos[t] = static_cast<std::int32_t>(j);
+    }
 }
 
-// Per block: [max, argmax, sum w, sum w m, sum w m bx, by, bz, unused]
-__global__ void stats_kernel(const float* __restrict__ vals, const std::int32_t* __restrict__ idx,
+// Per block of placement blockIdx.y: [max, argmax, sum w, sum w m, sum w m bx, by, bz, unused]
+__global__ void stats_kernel(const float* __restrict__ vals_all, const std::int32_t* __restrict__ idx,
                              const float* __restrict__ vols, const double* __restrict__ bary, std::int64_t nr,
-                             double* __restrict__ out) {
+                             double* __restrict__ out_all) {
     __shared__ double s[6][kBlock];
     __shared__ float smax[kBlock];
     __shared__ std::int64_t sarg[kBlock];
+    const float* vals = vals_all + blockIdx.y * nr;
     const std::int64_t per = (nr + kStatBlocks - 1) / kStatBlocks;
     const std::int64_t lo = blockIdx.x * per, hi = min(lo + per, nr);
     double acc[6] = {0, 0, 0, 0, 0, 0};
@@ -121,7 +131,7 @@ __global__ void stats_kernel(const float* __restrict__ vals, const std::int32_t*
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        double* o = out + 8 * static_cast<std::int64_t>(blockIdx.x);
+        double* o = out_all + 8 * (static_cast<std::int64_t>(blockIdx.y) * kStatBlocks + blockIdx.x);
         o[0] = smax[0];
         o[1] = static_cast<double>(sarg[0]);
         for (int q = 0; q < 5; ++q) o[2 + q] = s[q][0];
@@ -129,10 +139,13 @@ __global__ void stats_kernel(const float* __restrict__ vals, const std::int32_t*
     }
 }
 
-__global__ void variance_kernel(const float* __restrict__ vals, const std::int32_t* __restrict__ idx,
-                                const float* __restrict__ vols, std::int64_t nr, double mean,
-                                double* __restrict__ out) {
+// sum w (m - mean_c)^2 per block of placement blockIdx.y.
+__global__ void variance_kernel(const float* __restrict__ vals_all, const std::int32_t* __restrict__ idx,
+                                const float* __restrict__ vols, std::int64_t nr, const double* __restrict__ means,
+                                double* __restrict__ out_all) {
     __shared__ double s[kBlock];
+    const float* vals = vals_all + blockIdx.y * nr;
+    const double mean = means[blockIdx.y];
     const std::int64_t per = (nr + kStatBlocks - 1) / kStatBlocks;
     const std::int64_t lo = blockIdx.x * per, hi = min(lo + per, nr);
     double acc = 0.0;
@@ -146,28 +159,44 @@ __global__ void variance_kernel(const float* __restrict__ vals, const std::int32
         if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+    if (threadIdx.x == 0) out_all[static_cast<std::int64_t>(blockIdx.y) * kStatBlocks + blockIdx.x] = s[0];
 }
 
+// Barycentre of each placement's peak element (region position args[c]; NaN when < 0).
+__global__ void peak_location_kernel(const double* __restrict__ args, const std::int32_t* __restrict__ idx,
+                                     const double* __restrict__ bary, int k, double* __restrict__ loc) {
+    const int c = threadIdx.x;
+    if (c >= k) return;
+    const double a = args[c];
+    const std::int64_t e = a >= 0.0 ? static_cast<std::int64_t>(idx[static_cast<std::int64_t>(a)]) : -1;
+    for (int d = 0; d < 3; ++d) loc[3 * c + d] = e >= 0 ? bary[3 * e + d] : NAN;
+}
+
+// w[t] = volume of the element at sorted position t (every placement's segment at once).
 __global__ void sorted_weights_kernel(const std::int32_t* __restrict__ order, const std::int32_t* __restrict__ idx,
-                                      const float* __restrict__ vols, std::int64_t nr, double* __restrict__ w) {
-    const std::int64_t j = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j < nr) w[j] = static_cast<double>(vols[idx[order[j]]]);
+                                      const float* __restrict__ vols, std::int64_t total, double* __restrict__ w) {
+    const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < total) w[t] = static_cast<double>(vols[idx[order[t]]]);
 }
 
-// Fixed-order inclusive scan of one 1024-element tile per block: 4 sequential items per thread,
-// a Hillis-Steele pass over the 256 thread totals, then the tile total to `totals`.
+// Fixed-order inclusive scan, per segment (one placement each: `len` elements at stride `stride`),
+// of one 1024-element tile per block: 4 sequential items per thread, a Hillis-Steele pass over
+// the 256 thread totals, then the tile total to `totals` (segment-major, `tiles` per segment).
+// The association depends on the segment length alone.
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kBlock * kScanItems;
 
-__global__ void scan_tiles_kernel(double* __restrict__ a, std::int64_t n, double* __restrict__ totals) {
+__global__ void scan_tiles_kernel(double* __restrict__ a, std::int64_t len, std::int64_t stride, std::int64_t tiles,
+                                  double* __restrict__ totals) {
     __shared__ double s[kBlock];
-    const std::int64_t base = static_cast<std::int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+    const std::int64_t seg = blockIdx.x / tiles, tile = blockIdx.x % tiles;
+    double* const sa = a + seg * stride;
+    const std::int64_t base = tile * kScanTile + threadIdx.x * kScanItems;
     double v[kScanItems];
     double run = 0.0;
     for (int i = 0; i < kScanItems; ++i) {
         const std::int64_t j = base + i;
-        run += (j < n) ? a[j] : 0.0;
+        run += (j < len) ? sa[j] : 0.0;
         v[i] = run;
     }
     s[threadIdx.x] = run;
@@ -181,29 +210,62 @@ __global__ void scan_tiles_kernel(double* __restrict__ a, std::int64_t n, double
     const double before = threadIdx.x > 0 ? s[threadIdx.x - 1] : 0.0;
     for (int i = 0; i < kScanItems; ++i) {
         const std::int64_t j = base + i;
-        if (j < n) a[j] = before + v[i];
+        if (j < len) sa[j] = before + v[i];
     }
     if (threadIdx.x == kBlock - 1) totals[blockIdx.x] = s[kBlock - 1];
 }
 
-__global__ void add_offsets_kernel(double* __restrict__ a, std::int64_t n, const double* __restrict__ scanned_totals) {
-    const std::int64_t j = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j >= n) return;
+__global__ void add_offsets_kernel(double* __restrict__ a, std::int64_t len, std::int64_t stride, std::int64_t nseg,
+                                   std::int64_t tiles, const double* __restrict__ scanned_totals) {
+    const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nseg * len) return;
+    const std::int64_t seg = t / len, j = t % len;
     const std::int64_t tile = j / kScanTile;
-    if (tile > 0) a[j] += scanned_totals[tile - 1];
+    if (tile > 0) a[seg * stride + j] += scanned_totals[seg * tiles + tile - 1];
+}
+
+// Scratch doubles the segmented scan of `len` elements per segment needs for its tile totals.
+std::int64_t scan_scratch(std::int64_t len, std::int64_t nseg) {
+    std::int64_t need = 0;
+    while (len > 0) {
+        const std::int64_t tiles = (len + kScanTile - 1) / kScanTile;
+        need += tiles * nseg;
+        if (tiles <= 1) break;
+        len = tiles;
+    }
+    return need;
+}
+
+void prefix_sum_fixed(double* a, std::int64_t len, std::int64_t stride, std::int64_t nseg, double* scratch,
+                      hipStream_t s) {
+    if (len <= 0 || nseg <= 0) return;
+    const std::int64_t tiles = (len + kScanTile - 1) / kScanTile;
+    double* const totals = scratch;
+    scan_tiles_kernel<<<static_cast<unsigned>(tiles * nseg), kBlock, 0, s>>>(a, len, stride, tiles, totals);
+    if (tiles > 1) {
+        prefix_sum_fixed(totals, tiles, tiles, nseg, scratch + tiles * nseg, s);
+        add_offsets_kernel<<<grid_for(nseg * len), kBlock, 0, s>>>(a, len, stride, nseg, tiles, totals);
+    }
 }
 
 // np.interp(q, pos, v) with pos = (cw - w/2) / cw[-1] (metrics.py:82-95), q in FP64 from the FP32
-// value the reference passes.
-__global__ void quantile_kernel(const std::uint32_t* __restrict__ sorted_keys, const double* __restrict__ w,
-                                const double* __restrict__ cw, std::int64_t nr, const double* __restrict__ qs, int nq,
-                                double* __restrict__ out) {
+// value the reference passes; one block per placement.
+__global__ void quantile_kernel(const std::uint64_t* __restrict__ sorted_all, const double* __restrict__ w_all,
+                                const double* __restrict__ cw_all, std::int64_t nr, const double* __restrict__ qs,
+                                int nq, double* __restrict__ out_all) {
     const int t = threadIdx.x;
     if (t >= nq) return;
+    const std::int64_t off = static_cast<std::int64_t>(blockIdx.x) * nr;
+    const std::uint64_t* sorted_keys = sorted_all + off;
+    const double* w = w_all + off;
+    const double* cw = cw_all + off;
+    double* out = out_all + static_cast<std::int64_t>(blockIdx.x) * nq;
     const double total = cw[nr - 1];
     const double x = qs[t];
     auto pos = [&](std::int64_t j) { return (cw[j] - 0.5 * w[j]) / total; };
-    auto val = [&](std::int64_t j) { return static_cast<double>(__uint_as_float(sorted_keys[j])); };
+    auto val = [&](std::int64_t j) {
+        return static_cast<double>(__uint_as_float(static_cast<std::uint32_t>(sorted_keys[j])));
+    };
     if (nr == 1 || x <= pos(0)) {  // np.interp clamps below the first sample point
         out[t] = val(0);
         return;
@@ -226,33 +288,27 @@ __global__ void quantile_kernel(const std::uint32_t* __restrict__ sorted_keys, c
     out[t] = slope * (x - x0) + f0;
 }
 
-// Volume with |E| >= thr: the first sorted position at or above thr, then total minus prefix.
-__global__ void focality_kernel(const std::uint32_t* __restrict__ sorted_keys, const double* __restrict__ cw,
-                                std::int64_t nr, const float* __restrict__ thr, int nf, double* __restrict__ out) {
+// Volume with |E| >= frac * P99.9 (the anchor percentile rounded to FP32, as the reference's
+// threshold is): the first sorted position at or above it, then total minus prefix. One block per
+// placement; q holds nq quantiles per placement, the anchor last.
+__global__ void focality_kernel(const std::uint64_t* __restrict__ sorted_all, const double* __restrict__ cw_all,
+                                std::int64_t nr, const double* __restrict__ q, int nq, const double* __restrict__ fracs,
+                                int nf, double* __restrict__ out_all) {
     const int t = threadIdx.x;
     if (t >= nf) return;
-    const float th = thr[t];
+    const std::int64_t off = static_cast<std::int64_t>(blockIdx.x) * nr;
+    const std::uint64_t* sorted_keys = sorted_all + off;
+    const double* cw = cw_all + off;
+    const float th = static_cast<float>(fracs[t] * q[static_cast<std::int64_t>(blockIdx.x) * nq + nq - 1]);
     std::int64_t lo = 0, hi = nr;
     while (lo < hi) {
         const std::int64_t mid = lo + (hi - lo) / 2;
-        if (__uint_as_float(sorted_keys[mid]) >= th)
+        if (__uint_as_float(static_cast<std::uint32_t>(sorted_keys[mid])) >= th)
             hi = mid;
         else
             lo = mid + 1;
     }
-    out[t] = cw[nr - 1] - (lo > 0 ? cw[lo - 1] : 0.0);
-}
-
-void prefix_sum_fixed(double* a, std::int64_t n, hipStream_t s) {
-    if (n <= 0) return;
-    const std::int64_t tiles = (n + kScanTile - 1) / kScanTile;
-    DeviceBuffer<double> totals(static_cast<std::size_t>(tiles), "scan totals");
-    scan_tiles_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, s>>>(a, n, totals.get());
-    if (tiles > 1) {
-        prefix_sum_fixed(totals.get(), tiles, s);
-        add_offsets_kernel<<<grid_for(n), kBlock, 0, s>>>(a, n, totals.get());
-    }
-    sync(s, "prefix sum");
+    out_all[static_cast<std::int64_t>(blockIdx.x) * nf + t] = cw[nr - 1] - (lo > 0 ? cw[lo - 1] : 0.0);
 }
 
 }  // namespace
@@ -272,79 +328,97 @@ void unpack(const float* in, std::int64_t rows, int inner, int k, int K, float* 
 }
 
 void region_metrics(const float* magn, std::int64_t m, int k, int K, const std::int32_t* idx, std::int64_t nr,
-                    const float* vols, const double* bary, double* out_host, hipStream_t s) {
+                    const float* vols, const double* bary, MetricsWorkspace& ws, double* out_host, hipStream_t s) {
     (void)m;
     std::memset(out_host, 0, sizeof(double) * kMetricsLen * k);
-    if (nr <= 0) return;
-    DeviceBuffer<float> vals(static_cast<std::size_t>(nr), "region values");
-    DeviceBuffer<std::uint32_t> keys(static_cast<std::size_t>(nr), "region keys"), sorted(static_cast<std::size_t>(nr), "sorted keys");
-    DeviceBuffer<std::int32_t> iota(static_cast<std::size_t>(nr), "iota"), order(static_cast<std::size_t>(nr), "order");
-    DeviceBuffer<double> w(static_cast<std::size_t>(nr), "sorted w"), cw(static_cast<std::size_t>(nr), "cum w");
-    DeviceBuffer<double> part(8 * kStatBlocks, "stat partials"), q_out(8, "quantiles");
-    DeviceBuffer<double> qd(4, "q"), foc(2, "focality");
-    DeviceBuffer<float> thr(2, "thresholds");
+    if (nr <= 0 || k <= 0) return;
+    const std::int64_t total = nr * k;
+    ws.vals.ensure(static_cast<std::size_t>(total), "region values");
+    ws.keys.ensure(static_cast<std::size_t>(total), "region keys");
+    ws.sorted.ensure(static_cast<std::size_t>(total), "sorted keys");
+    ws.pos.ensure(static_cast<std::size_t>(total), "region positions");
+    ws.order.ensure(static_cast<std::size_t>(total), "sorted positions");
+    ws.w.ensure(static_cast<std::size_t>(total), "sorted volumes");
+    ws.cw.ensure(static_cast<std::size_t>(total), "cumulative volumes");
+    ws.totals.ensure(static_cast<std::size_t>(std::max<std::int64_t>(scan_scratch(nr, k), 1)), "scan totals");
+    ws.part.ensure(static_cast<std::size_t>(8) * kStatBlocks * k, "stat partials");
+    // small: qs[4] fracs[2] | means[k] args[k] | q[4k] foc[2k] loc[3k]
+    ws.small.ensure(static_cast<std::size_t>(6 + 11 * k), "metric scalars");
+    double* const qd = ws.small.get();
+    double* const fracs = qd + 4;
+    double* const means = qd + 6;
+    double* const args = means + k;
+    double* const q_out = args + k;
+    double* const foc = q_out + 4 * k;
+    double* const loc = foc + 2 * k;
     // metrics.py:131 passes the percentiles through the values' dtype (FP32), then FP64.
-    const double qs[4] = {static_cast<double>(0.5f), static_cast<double>(0.95f), static_cast<double>(0.99f),
-                          static_cast<double>(0.999f)};
-    h2d(qd.get(), qs, 4, s);
-    iota_i32(iota.get(), nr, s);
-    std::vector<double> hp(8 * kStatBlocks), hv(kStatBlocks);
+    const double consts[6] = {static_cast<double>(0.5f), static_cast<double>(0.95f), static_cast<double>(0.99f),
+                              static_cast<double>(0.999f), 0.5, 0.75};
+    h2d(qd, consts, 6, s);
+    gather_region_kernel<<<grid_for(nr), kBlock, 0, s>>>(magn, K, k, idx, nr, ws.vals.get(), ws.keys.get(),
+                                                         ws.pos.get());
+    stats_kernel<<<dim3(kStatBlocks, k), kBlock, 0, s>>>(ws.vals.get(), idx, vols, bary, nr, ws.part.get());
+    // distribution quantiles and focality: a stable sort by (placement, |E|), the volumes in that
+    // order, their per-placement fixed-order prefix sums
+    int bits = 32;
+    while ((1 << (bits - 32)) < k) ++bits;
+    sort_pairs_u64_ws(ws.keys.get(), ws.sorted.get(), ws.pos.get(), ws.order.get(), total, bits, ws.scratch, s);
+    sorted_weights_kernel<<<grid_for(total), kBlock, 0, s>>>(ws.order.get(), idx, vols, total, ws.w.get());
+    d2d(ws.cw.get(), ws.w.get(), static_cast<std::size_t>(total), s);
+    prefix_sum_fixed(ws.cw.get(), nr, nr, k, ws.totals.get(), s);
+    quantile_kernel<<<k, 64, 0, s>>>(ws.sorted.get(), ws.w.get(), ws.cw.get(), nr, qd, 4, q_out);
+    focality_kernel<<<k, 64, 0, s>>>(ws.sorted.get(), ws.cw.get(), nr, q_out, 4, fracs, 2, foc);
+    std::vector<double> hp(static_cast<std::size_t>(8) * kStatBlocks * k);
+    d2h(hp.data(), ws.part.get(), hp.size(), s);
+    sync(s, "region stats");
+    std::vector<double> mean(k), hargs(k), sw(k), swm(k), mx(k), sb(3 * k);
     for (int c = 0; c < k; ++c) {
-        double* o = out_host + kMetricsLen * c;
-        gather_region_kernel<<<grid_for(nr), kBlock, 0, s>>>(magn, K, idx, nr, c, vals.get(), keys.get());
-        stats_kernel<<<kStatBlocks, kBlock, 0, s>>>(vals.get(), idx, vols, bary, nr, part.get());
-        d2h(hp.data(), part.get(), hp.size(), s);
-        sync(s, "region stats");
-        double mx = -INFINITY, sw = 0, swm = 0, sb[3] = {0, 0, 0};
+        double best = -INFINITY, a = -1.0, w = 0, wm = 0, b[3] = {0, 0, 0};
         std::int64_t arg = -1;
-        for (int b = 0; b < kStatBlocks; ++b) {
-            const double* p = hp.data() + 8 * b;
-            if (p[0] > mx || (p[0] == mx && static_cast<std::int64_t>(p[1]) < arg)) {
-                mx = p[0];
+        for (int blk = 0; blk < kStatBlocks; ++blk) {
+            const double* p = hp.data() + 8 * (static_cast<std::size_t>(c) * kStatBlocks + blk);
+            if (p[0] > best || (p[0] == best && static_cast<std::int64_t>(p[1]) < arg)) {
+                best = p[0];
                 arg = static_cast<std::int64_t>(p[1]);
             }
-            sw += p[2];
-            swm += p[3];
-            for (int d = 0; d < 3; ++d) sb[d] += p[4 + d];
+            w += p[2];
+            wm += p[3];
+            for (int d = 0; d < 3; ++d) b[d] += p[4 + d];
         }
-        const double mean = swm / sw;
-        variance_kernel<<<kStatBlocks, kBlock, 0, s>>>(vals.get(), idx, vols, nr, mean, part.get());
-        d2h(hv.data(), part.get(), hv.size(), s);
-        // peak location (NaN when no value compared greater than -inf, i.e. all NaN)
-        std::int32_t e_peak = -1;
-        if (arg >= 0 && arg < nr) d2h(&e_peak, idx + arg, 1, s);
-        sync(s, "region variance");
+        // peak location is NaN when no value compared greater than -inf (all NaN)
+        a = (arg >= 0 && arg < nr) ? static_cast<double>(arg) : -1.0;
+        mx[c] = best;
+        hargs[c] = a;
+        sw[c] = w;
+        swm[c] = wm;
+        for (int d = 0; d < 3; ++d) sb[3 * c + d] = b[d];
+        mean[c] = wm / w;
+    }
+    h2d(means, mean.data(), k, s);
+    h2d(args, hargs.data(), k, s);
+    variance_kernel<<<dim3(kStatBlocks, k), kBlock, 0, s>>>(ws.vals.get(), idx, vols, nr, means, ws.part.get());
+    peak_location_kernel<<<1, 64, 0, s>>>(args, idx, bary, k, loc);
+    std::vector<double> hv(static_cast<std::size_t>(kStatBlocks) * k), res(9 * static_cast<std::size_t>(k));
+    d2h(hv.data(), ws.part.get(), hv.size(), s);
+    d2h(res.data(), q_out, res.size(), s);
+    sync(s, "region variance");
+    const double* qv = res.data();
+    const double* fv = qv + 4 * k;
+    const double* lv = fv + 2 * k;
+    for (int c = 0; c < k; ++c) {
         double var = 0;
-        for (int b = 0; b < kStatBlocks; ++b) var += hv[b];
-        var /= sw;
-        double loc[3] = {NAN, NAN, NAN};
-        if (e_peak >= 0) d2h(loc, bary + 3 * static_cast<std::int64_t>(e_peak), 3, s);
-        // distribution quantiles
-        sort_pairs_u32(keys.get(), sorted.get(), iota.get(), order.get(), nr, 32, s);
-        sorted_weights_kernel<<<grid_for(nr), kBlock, 0, s>>>(order.get(), idx, vols, nr, w.get());
-        d2d(cw.get(), w.get(), static_cast<std::size_t>(nr), s);
-        prefix_sum_fixed(cw.get(), nr, s);
-        quantile_kernel<<<1, 64, 0, s>>>(sorted.get(), w.get(), cw.get(), nr, qd.get(), 4, q_out.get());
-        double qv[4];
-        d2h(qv, q_out.get(), 4, s);
-        sync(s, "quantiles");
-        const float th[2] = {static_cast<float>(0.5 * qv[3]), static_cast<float>(0.75 * qv[3])};
-        h2d(thr.get(), th, 2, s);
-        focality_kernel<<<1, 64, 0, s>>>(sorted.get(), cw.get(), nr, thr.get(), 2, foc.get());
-        double fv[2];
-        d2h(fv, foc.get(), 2, s);
-        sync(s, "focality");
-        o[0] = mx;
-        o[1] = loc[0];
-        o[2] = loc[1];
-        o[3] = loc[2];
-        for (int d = 0; d < 3; ++d) o[4 + d] = sb[d] / swm;
-        o[7] = sw;
-        o[8] = fv[0];
-        o[9] = fv[1];
-        o[10] = mean;
+        for (int blk = 0; blk < kStatBlocks; ++blk) var += hv[static_cast<std::size_t>(c) * kStatBlocks + blk];
+        var /= sw[c];
+        double* o = out_host + kMetricsLen * c;
+        o[0] = mx[c];
+        for (int d = 0; d < 3; ++d) o[1 + d] = lv[3 * c + d];
+        for (int d = 0; d < 3; ++d) o[4 + d] = sb[3 * c + d] / swm[c];
+        o[7] = sw[c];
+        o[8] = fv[2 * c];
+        o[9] = fv[2 * c + 1];
+        o[10] = mean[c];
         o[11] = std::sqrt(var);
-        for (int q = 0; q < 4; ++q) o[12 + q] = qv[q];
+        for (int qi = 0; qi < 4; ++qi) o[12 + qi] = qv[4 * c + qi];
     }
 }
 

@@ -30,6 +30,20 @@ void sort_pairs_u64(const std::uint64_t* keys_in, std::uint64_t* keys_out,
     sort_pairs_impl(keys_in, keys_out, vals_in, vals_out, n, end_bit, s);
 }
 
+void sort_pairs_u64_ws(const std::uint64_t* keys_in, std::uint64_t* keys_out, const std::int32_t* vals_in,
+                       std::int32_t* vals_out, std::int64_t n, int end_bit, DeviceBuffer<unsigned char>& scratch,
+                       hipStream_t s) {
+    if (n <= 0) return;
+    std::size_t bytes = 0;
+    check(rocprim::radix_sort_pairs(nullptr, bytes, keys_in, keys_out, vals_in, vals_out,
+                                    static_cast<std::size_t>(n), 0, end_bit, s),
+          "radix sort (size)");
+    scratch.ensure(bytes ? bytes : 1, "radix sort scratch");
+    check(rocprim::radix_sort_pairs(scratch.get(), bytes, keys_in, keys_out, vals_in, vals_out,
+                                    static_cast<std::size_t>(n), 0, end_bit, s),
+          "radix sort");
+}
+
 void sort_pairs_u32(const std::uint32_t* keys_in, std::uint32_t* keys_out,
                     const std::int32_t* vals_in, std::int32_t* vals_out, std::int64_t n,
                     int end_bit, hipStream_t s) {

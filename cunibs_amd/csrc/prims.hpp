@@ -7,6 +7,10 @@ namespace cn {
 void sort_pairs_u64(const std::uint64_t* keys_in, std::uint64_t* keys_out,
                     const std::int32_t* vals_in, std::int32_t* vals_out, std::int64_t n,
                     int end_bit, hipStream_t s);
+// The same sort with the temporary storage in a caller-held, grow-only buffer and no stream sync.
+void sort_pairs_u64_ws(const std::uint64_t* keys_in, std::uint64_t* keys_out, const std::int32_t* vals_in,
+                       std::int32_t* vals_out, std::int64_t n, int end_bit, DeviceBuffer<unsigned char>& scratch,
+                       hipStream_t s);
 void sort_pairs_u32(const std::uint32_t* keys_in, std::uint32_t* keys_out,
                     const std::int32_t* vals_in, std::int32_t* vals_out, std::int64_t n,
                     int end_bit, hipStream_t s);

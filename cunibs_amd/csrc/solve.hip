@@ -446,6 +446,99 @@ __global__ void __launch_bounds__((gram_threads<K, TWO>())) gram_rows_kernel(
     for (int e = threadIdx.x; e < (TWO ? 2 : 1) * K * K; e += blockDim.x) out[e] = acc[e];
 }
 
+typedef double doublex4_t __attribute__((ext_vector_type(4)));
+
+// The same partials for K = 16 and 32 on the FP64 matrix cores: v_mfma_f64_16x16x4f64 takes a
+// 16x4 block of A^T and a 4x16 block of B (lane l holds column l % 16 of data row l / 16, one
+// coalesced dword load per operand: 4 rows x 64 B) and accumulates the 16x16 tile in FP64, so
+// the products of the FP32 inputs are exact and each input is converted once. Accumulator
+// register v of lane l holds tile row 4 v + l / 16 of tile column l % 16 (the FP64 layout, unlike
+// the FP32 one's rows 4 (l / 16) + v). U quads of rows per step keep 3 U loads in flight.
+constexpr int kGramMfmaThreads = 1024;
+
+template <int K, bool TWO>
+__global__ void __launch_bounds__(kGramMfmaThreads) gram_mfma_kernel(
+    std::int64_t n, const float* __restrict__ A, const float* __restrict__ B1, const float* __restrict__ B2,
+    double* __restrict__ partials, std::int64_t rows_per_block, const int* __restrict__ state) {
+    static_assert(K == 16 || K == 32, "matrix-core Gram tiles are 16 wide");
+    if (finished(state)) return;
+    constexpr int H = K / 16;          // 16-column halves of a row
+    constexpr int NB = TWO ? 2 : 1;
+    constexpr int U = 8 / H;           // row quads per step
+    constexpr int WAVES = kGramMfmaThreads / kWave;
+    const int lb = logical_block();
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
+    const int col = lane & 15, quad = lane >> 4;
+    const std::int64_t lo = static_cast<std::int64_t>(lb) * rows_per_block;
+    const std::int64_t hi = min(lo + rows_per_block, n);
+    doublex4_t acc[NB][H][H];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int x = 0; x < H; ++x)
+#pragma unroll
+            for (int y = 0; y < H; ++y) acc[b][x][y] = doublex4_t{0.0, 0.0, 0.0, 0.0};
+    for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * (4 * U); r0 < hi; r0 += WAVES * 4 * U) {
+        float av[U][H], bv[NB][U][H];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const std::int64_t row = r0 + 4 * u + quad;
+            const bool ok = row < hi;
+            const std::int64_t at = (ok ? row : hi - 1) * K + col;  // clamped: loads stay unconditional
+#pragma unroll
+            for (int h = 0; h < H; ++h) {
+                av[u][h] = ok ? A[at + 16 * h] : 0.0f;
+                bv[0][u][h] = B1[at + 16 * h];
+                if constexpr (TWO) bv[1][u][h] = B2[at + 16 * h];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int x = 0; x < H; ++x) {
+                const double a = static_cast<double>(av[u][x]);
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+#pragma unroll
+                    for (int y = 0; y < H; ++y)
+                        acc[b][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, static_cast<double>(bv[b][u][y]),
+                                                                            acc[b][x][y], 0, 0, 0);
+            }
+    }
+    // Block partial in wave order.
+    __shared__ double red[NB * K * K];
+    for (int e = threadIdx.x; e < NB * K * K; e += blockDim.x) red[e] = 0.0;
+    __syncthreads();
+    for (int w = 0; w < WAVES; ++w) {
+        if (wave == w) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int x = 0; x < H; ++x)
+#pragma unroll
+                    for (int y = 0; y < H; ++y)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                            red[(b * K + 16 * x + 4 * v + quad) * K + 16 * y + col] += acc[b][x][y][v];
+        }
+        __syncthreads();
+    }
+    double* out = partials + static_cast<std::int64_t>(lb) * NB * K * K;
+    for (int e = threadIdx.x; e < NB * K * K; e += blockDim.x) out[e] = red[e];
+}
+
+// G1 = A^T B1 (and G2 = A^T B2) partials, kGramBlocks logical blocks of `rows_per_block` rows.
+template <int K, bool TWO>
+void launch_gram(std::int64_t n, const float* A, const float* B1, const float* B2, double* partials,
+                 std::int64_t rows_per_block, const int* state, hipStream_t s) {
+    if constexpr (K >= 16)
+        gram_mfma_kernel<K, TWO><<<kGramBlocks, kGramMfmaThreads, 0, s>>>(n, A, B1, B2, partials, rows_per_block,
+                                                                          state);
+    else
+        gram_rows_kernel<K, TWO><<<kGramBlocks, gram_threads<K, TWO>(), 0, s>>>(n, A, B1, B2, partials,
+                                                                              rows_per_block, state);
+}
+
 constexpr int kGramThreads = 1024;
 constexpr int kGramWaves = kGramThreads / kWave;
 
@@ -800,119 +893,119 @@ constexpr int small_threads() {
 }
 
 // G, H -> T (pivoted Cholesky of the unit-diagonal scaling of G), TT = T T^T, a = TT H with the
-// frozen columns zeroed.
+// frozen columns zeroed. One wavefront: the factorisation is left-looking, lane i owns row i and
+// keeps its Schur-complement diagonal in a register, so the pivot is a wave-wide max and each step
+// needs one barrier (a single wavefront's barrier is only an LDS ordering point).
 template <int K>
-__global__ void __launch_bounds__(kSmallThreads) coef_kernel(const double* __restrict__ partials, int nb,
-                                                             double drop_tol, double* __restrict__ G_out,
-                                                             double* __restrict__ T, double* __restrict__ TT,
-                                                             double* __restrict__ AL, int* __restrict__ state,
-                                                             int independent) {
+__global__ void __launch_bounds__(kWave) coef_kernel(const double* __restrict__ partials, int nb,
+                                                     double drop_tol, double* __restrict__ G_out,
+                                                     double* __restrict__ T, double* __restrict__ TT,
+                                                     double* __restrict__ AL, int* __restrict__ state,
+                                                     int independent) {
+    static_assert(K <= kWave, "one lane per row");
     if (finished(state)) return;
-    __shared__ double A[K * K];   // scaled G, then its Schur complements; then Lp
-    __shared__ double H[K * K];
-    __shared__ double Lc[K * K];  // Lc[j * K + i]: column j of L at original row i; then T T^T
+    constexpr int kE = K * K;
+    __shared__ double A[kE];   // G scaled to unit diagonal
+    __shared__ double H[kE];
+    __shared__ double Lc[kE];  // Lc[j * K + i]: column j of L at original row i
+    __shared__ double Tm[kE];  // T, then T T^T
     __shared__ double sc[K];
-    __shared__ double Li[K * K];
-    __shared__ double Tm[K * K];
-    double* const Lp = A;   // A is dead once the factorisation stops
-    double* const TTm = Lc; // Lc is dead once Lp has been gathered from it
     __shared__ int idx[K];
-    __shared__ int rem[K];
-    __shared__ int rank_s, stop;
-    load_reduced<K * K>(partials, 0, A);
-    load_reduced<K * K>(partials, K * K, H);
+    load_reduced<kE>(partials, 0, A);
+    load_reduced<kE>(partials, kE, H);
     const int t = threadIdx.x;
     if (independent) {  // K separate systems: each column is its own CG chain
-        if (t < K * K && (t / K) != (t % K)) {
-            A[t] = 0.0;
-            H[t] = 0.0;
-        }
+        for (int e = t; e < kE; e += kWave)
+            if ((e / K) != (e % K)) {
+                A[e] = 0.0;
+                H[e] = 0.0;
+            }
         __syncthreads();
     }
-    if (t < K * K) G_out[t] = A[t];
+    for (int e = t; e < kE; e += kWave) G_out[e] = A[e];
+    bool rm = false;  // lane t < K: row t is still a pivot candidate
     if (t < K) {
         const double d = A[t * K + t];
-        const bool ok = d > 0.0 && isfinite(d);
-        sc[t] = ok ? 1.0 / sqrt(d) : 0.0;
-        rem[t] = ok ? 1 : 0;
+        rm = d > 0.0 && isfinite(d);
+        sc[t] = rm ? 1.0 / sqrt(d) : 0.0;
     }
     __syncthreads();
-    if (t < K * K) {
-        const int i = t / K, j = t % K;
-        A[t] = A[t] * sc[i] * sc[j];
-        Lc[t] = 0.0;
-        Tm[t] = 0.0;
-    }
-    if (t == 0) {
-        rank_s = 0;
-        stop = 0;
+    for (int e = t; e < kE; e += kWave) {
+        A[e] = A[e] * sc[e / K] * sc[e % K];
+        Tm[e] = 0.0;
     }
     __syncthreads();
+    double d = t < K ? A[t * K + t] : 0.0;  // Schur-complement diagonal of row t
+    int r = 0;
     for (int j = 0; j < K; ++j) {
-        if (t == 0) {
-            int p = -1;
-            double best = 0.0;
-            for (int i = 0; i < K; ++i)
-                if (rem[i] && A[i * K + i] > best) {
-                    best = A[i * K + i];
-                    p = i;
-                }
-            if (p < 0 || !(best > drop_tol)) {
-                stop = 1;
-            } else {
-                idx[j] = p;
-                rem[p] = 0;
-                rank_s = j + 1;
+        // pivot: the largest remaining diagonal, lowest index on ties; every lane ends with it
+        double v = rm ? d : -1.0;
+        int p = t < K ? t : K;
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(v, off, kWave);
+            const int op = __shfl_xor(p, off, kWave);
+            if (ov > v || (ov == v && op < p)) {
+                v = ov;
+                p = op;
             }
         }
-        __syncthreads();
-        if (stop) break;
-        const int p = idx[j];
-        const double ljj = sqrt(A[p * K + p]);
-        if (t < K) Lc[j * K + t] = (t == p) ? ljj : (rem[t] ? A[t * K + p] / ljj : 0.0);
-        __syncthreads();
-        if (t < K * K) {
-            const int i = t / K, m = t % K;
-            if (rem[i] && rem[m]) A[t] -= Lc[j * K + i] * Lc[j * K + m];
+        if (p >= K || !(v > drop_tol)) break;
+        if (t == p) rm = false;
+        const double ljj = sqrt(v);
+        if (t < K) {
+            double c = 0.0;
+            if (t == p) {
+                c = ljj;
+            } else if (rm) {
+                double s = A[t * K + p];
+                for (int k = 0; k < j; ++k) s -= Lc[k * K + t] * Lc[k * K + p];
+                c = s / ljj;
+                d -= c * c;
+            }
+            Lc[j * K + t] = c;
         }
+        if (t == 0) idx[j] = p;
+        r = j + 1;
         __syncthreads();
     }
-    const int r = rank_s;
-    if (t < K * K) {
-        const int a = t / K, b = t % K;
-        Lp[t] = (a < r && b < r && b <= a) ? Lc[b * K + idx[a]] : 0.0;
-        Li[t] = 0.0;
-    }
-    __syncthreads();
-    if (t < r) {  // Li = Lp^-1, thread b solves Lp y = e_b
+    if (t < r) {
+        // column b = t of Lp^-1 by forward substitution, Lp(a, q) = Lc[q * K + idx[a]];
+        // T[idx[a]][b] = sc[idx[a]] Lp^-1(b, a), so lane b writes row idx[b] of T
         const int b = t;
-        Li[b * K + b] = 1.0 / Lp[b * K + b];
-        for (int a = b + 1; a < r; ++a) {
-            double s = 0.0;
-            for (int q = b; q < a; ++q) s += Lp[a * K + q] * Li[q * K + b];
-            Li[a * K + b] = -s / Lp[a * K + a];
+        double y[K];
+#pragma unroll
+        for (int a = 0; a < K; ++a) {
+            y[a] = 0.0;
+            if (a < r && a >= b) {
+                const int ia = idx[a];
+                double s = a == b ? 1.0 : 0.0;
+#pragma unroll
+                for (int q = 0; q < a; ++q)
+                    if (q >= b) s -= Lc[q * K + ia] * y[q];
+                y[a] = s / Lc[a * K + ia];
+            }
         }
+        const int ib = idx[b];
+#pragma unroll
+        for (int a = 0; a < K; ++a)
+            if (a < r && a >= b) Tm[ib * K + a] = sc[ib] * y[a];
     }
     __syncthreads();
-    if (t < K * K) {  // T[idx[a]][b] = sc[idx[a]] * Li[b][a]
-        const int a = t / K, b = t % K;
-        if (a < r && b < r && b >= a) Tm[idx[a] * K + b] = sc[idx[a]] * Li[b * K + a];
-    }
-    __syncthreads();
-    if (t < K * K) {  // TT = T T^T
-        const int i = t / K, j = t % K;
+    for (int e = t; e < kE; e += kWave) {  // TT = T T^T (into Lc, dead now)
+        const int i = e / K, j = e % K;
         double s = 0.0;
         for (int b = 0; b < K; ++b) s += Tm[i * K + b] * Tm[j * K + b];
-        TTm[t] = s;
-        T[t] = Tm[t];
-        TT[t] = s;
+        Lc[e] = s;
+        T[e] = Tm[e];
+        TT[e] = s;
     }
     __syncthreads();
-    if (t < K * K) {  // a = TT H, frozen columns zero
-        const int i = t / K, c = t % K;
+    for (int e = t; e < kE; e += kWave) {  // a = TT H, frozen columns zero
+        const int i = e / K, c = e % K;
         double s = 0.0;
-        for (int b = 0; b < K; ++b) s += TTm[i * K + b] * H[b * K + c];
-        AL[t] = state[2 + c] ? 0.0 : s;
+        for (int b = 0; b < K; ++b) s += Lc[i * K + b] * H[b * K + c];
+        AL[e] = state[2 + c] ? 0.0 : s;
     }
 }
 
@@ -1264,8 +1357,7 @@ void BlockSolver::spmm_block_k(const int* state) {
         spmm_grid_ = launch_rows<K, kSpmmDiff>(fine_sub<K>(), op_.off, op_.rowsum, nullptr, W_.get(), nullptr, 0.f,
                                                0.f, 0.f, S_.get(), state, s_, stamps());
     }
-    gram_rows_kernel<K, true><<<kGramBlocks, gram_threads<K, true>(), 0, s_>>>(nn, W_.get(), S_.get(), R_.get(),
-                                                                        partials_.get(), gram_rows(nn), state);
+    launch_gram<K, true>(nn, W_.get(), S_.get(), R_.get(), partials_.get(), gram_rows(nn), state, s_);
 }
 
 template <int K>
@@ -1277,7 +1369,7 @@ void BlockSolver::iteration_k(const SolveOptions& opt) {
     mark(0, st);
     spmm_block_k<K>(st);
     reduce_cols(partials_.get(), kGramBlocks, 2 * K * K, red_.get(), st, s_);
-    coef_kernel<K><<<1, small_threads<K>(), 0, s_>>>(red_.get(), kGramBlocks, opt.drop_tol, G_.get(), T_.get(),
+    coef_kernel<K><<<1, kWave, 0, s_>>>(red_.get(), kGramBlocks, opt.drop_tol, G_.get(), T_.get(),
                                                 TT_.get(), AL_.get(), stw, multi_ ? 1 : 0);
     update_xr_kernel<K><<<kGramBlocks, xr_threads<K>(), 0, s_>>>(nn, W_.get(), S_.get(), AL_.get(), X64_.get(), R_.get(),
                                                              partials_.get(), rpb, st);
@@ -1286,8 +1378,7 @@ void BlockSolver::iteration_k(const SolveOptions& opt) {
     mark(1, st);
     vcycle_k<K>(R_.get(), Z_.get(), st);
     mark(2, st);
-    gram_rows_kernel<K, false><<<kGramBlocks, gram_threads<K, false>(), 0, s_>>>(nn, S_.get(), Z_.get(), nullptr,
-                                                                         partials_.get(), rpb, st);
+    launch_gram<K, false>(nn, S_.get(), Z_.get(), nullptr, partials_.get(), rpb, st, s_);
     reduce_cols(partials_.get(), kGramBlocks, K * K, red_.get(), st, s_);
     dmat_kernel<K><<<1, small_threads<K>(), 0, s_>>>(red_.get(), kGramBlocks, TT_.get(), D_.get(), stw,
                                                 multi_ ? 1 : 0);
@@ -1442,11 +1533,27 @@ void BlockSolver::spmm_diff(const float* X, float* Y, int K) {
 void BlockSolver::cross_gram32(const float* A, const float* B, double* out_host) {
     ensure(32);
     const std::int64_t nn = n();
-    gram_rows_kernel<32, false><<<kGramBlocks, gram_threads<32, false>(), 0, s_>>>(nn, A, B, nullptr, partials_.get(),
-                                                                               gram_rows(nn), nullptr);
+    launch_gram<32, false>(nn, A, B, nullptr, partials_.get(), gram_rows(nn), nullptr, s_);
     sum_partials_kernel<32 * 32><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, G_.get());
     d2h(out_host, G_.get(), 32 * 32, s_);
     sync(s_, "cross gram");
+}
+
+void BlockSolver::gram(const float* A, const float* B1, const float* B2, double* out_host, int K) {
+    ensure(K);
+    const std::int64_t nn = n();
+    const int E = (B2 != nullptr ? 2 : 1) * K * K;
+    dispatch_k(K, [&](auto kc) {
+        constexpr int KK = decltype(kc)::value;
+        if (B2 != nullptr)
+            launch_gram<KK, true>(nn, A, B1, B2, partials_.get(), gram_rows(nn), nullptr, s_);
+        else
+            launch_gram<KK, false>(nn, A, B1, nullptr, partials_.get(), gram_rows(nn), nullptr, s_);
+    });
+    check(hipGetLastError(), "gram launch");
+    reduce_cols(partials_.get(), kGramBlocks, E, red_.get(), nullptr, s_);
+    d2h(out_host, red_.get(), E, s_);
+    sync(s_, "gram");
 }
 
 void BlockSolver::vcycle(const float* B, float* X, int K) {

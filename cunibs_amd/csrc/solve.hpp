@@ -85,6 +85,9 @@ class BlockSolver {
     void set_multi(const MultiOperator* m);
     // out (32 x 32, FP64, host) = A^T B for (n, 32) FP32 blocks, fixed-order reduction.
     void cross_gram32(const float* A, const float* B, double* out_host);
+    // G1 = A^T B1 and, with B2, G2 = A^T B2 (FP64, the iteration's Gram kernels and reduction)
+    // into out_host (K*K, then K*K more with B2); A, B1, B2 device (n, K).
+    void gram(const float* A, const float* B1, const float* B2, double* out_host, int K);
     // Average device time (ms) over `reps` launches, HIP events on the solver's stream.
     // mode 0: plain SpMM; 1: differenced SpMM (the iteration's S = A W); 2: 1 followed by the Gram
     // pass G = W^T S, H = W^T R. sub > 0 forces the sub-lanes per column group of the fine level.

@@ -278,6 +278,20 @@ class SolverContext:
         check(lib().cn_k_spmm_diff(self.handle, ptr(X), ptr(Y), X.shape[1]), "cn_k_spmm_diff")
         return Y
 
+    def gram(self, A: np.ndarray, B1: np.ndarray, B2: np.ndarray | None = None):
+        """FP64 A^T B1 (and A^T B2) of (n_reduced, K) FP32 blocks, by the solver's Gram kernels."""
+        A = np.ascontiguousarray(A, dtype=np.float32)
+        B1 = np.ascontiguousarray(B1, dtype=np.float32)
+        k = A.shape[1]
+        if B1.shape != A.shape or (B2 is not None and np.shape(B2) != A.shape):
+            raise ValueError("A, B1 and B2 must have the same shape")
+        out = np.empty((1 if B2 is None else 2, k, k), np.float64)
+        if B2 is not None:
+            B2 = np.ascontiguousarray(B2, dtype=np.float32)
+        check(lib().cn_k_gram(self.handle, ptr(A), ptr(B1), ptr(B2), ptr(out), k),
+              "cn_k_gram")
+        return out[0] if B2 is None else (out[0], out[1])
+
     def vcycle(self, B: np.ndarray) -> np.ndarray:
         B = np.ascontiguousarray(B, dtype=np.float32)
         X = np.empty_like(B)

@@ -292,6 +292,12 @@ int32_t cn_k_spmm(cn_context* ctx, const float* X, float* Y, int32_t K);
  * + s_i x_i with s_i the FP64 row sum (no counterpart in the reference: it is what keeps the
  * FP32 recursive residual close to the FP64 true residual). */
 int32_t cn_k_spmm_diff(cn_context* ctx, const float* X, float* Y, int32_t K);
+/* FP64 Gram matrices of the block solver, G1 = A^T B1 and (B2 != NULL) G2 = A^T B2, for
+ * A, B1, B2 (n_reduced, K) FP32: the products behind the search-block A-orthonormalisation,
+ * the full-matrix form of the reference's column-wise dots (amg/block_cg.cu:145-160,371-379).
+ * out: K*K, then K*K more for G2; row a, column c. */
+int32_t cn_k_gram(cn_context* ctx, const float* A, const float* B1, const float* B2, double* out,
+                  int32_t K);
 /* X = M B, one zero-guess V-cycle (vcycle.cu:650-663 apply / apply_block). */
 int32_t cn_k_vcycle(cn_context* ctx, const float* B, float* X, int32_t K);
 /* bindings.cpp:194-206 dadt_nbody: targets r (n,3) metres relative to the coil origin, dipoles

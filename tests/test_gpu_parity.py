@@ -89,6 +89,26 @@ def test_block_spmm_plain_and_differenced(gctx, octx, oracle_mod, K):
     assert rel(gctx.spmm_diff(X), ref) < 1e-6
 
 
+@pytest.mark.parametrize("K", [1, 2, 4, 8, 16, 32])
+def test_block_grams_are_fp64_exact_products(gctx, octx, K):
+    """The iteration's Gram kernels (vector lanes below 16 columns, FP64 matrix cores from 16):
+    A^T B1 and A^T B2 of FP32 blocks against the FP64 product; the products of FP32 inputs are
+    exact in FP64, so each entry is within 1e-12 of its sum of |terms|. B1 != B2 and A != B
+    catch a transposed tile or a swapped operand."""
+    n = octx.reduced.n
+    rng = np.random.default_rng(7 + K)
+    A, B1, B2 = (rng.standard_normal((n, K)).astype(np.float32) for _ in range(3))
+    B1[::3] *= 3.0  # rows of unequal weight, so a dropped or repeated row shows
+    g1, g2 = gctx.gram(A, B1, B2)
+    g = gctx.gram(A, B2)
+    for got, B in ((g1, B1), (g2, B2), (g, B2)):
+        A64, B64 = A.astype(np.float64), B.astype(np.float64)
+        ref = A64.T @ B64
+        bound = 1e-12 * (np.abs(A64).T @ np.abs(B64))
+        assert np.all(np.abs(got - ref) <= bound)
+    assert K == 1 or not np.allclose(g1, g1.T)
+
+
 @pytest.mark.parametrize("K", [1, 4, 8, 32])
 def test_vcycle_against_the_fp64_vcycle(gctx, octx, oracle_mod, K):
     n = octx.reduced.n
@@ -181,6 +201,26 @@ def test_block_of_eight_against_the_oracle(block8, octx, oracle_mod, coil):
         ref = oracle_mod.solve_site(octx, coil.positions_m, coil.moments, sites[i], 1e6, tol=1e-12)
         assert rel(res[i].magnE[gm], ref["magnE"][gm]) < GM_ERR_BAR * 0.1
     assert all(r.relative_residual <= 1e-6 for r in res)
+
+
+def test_block_metrics_match_the_oracle_per_placement(block8, gctx, oracle_mod):
+    """The region summary of a whole block (every placement's |E| sorted together by placement,
+    then value) equals the NumPy restatement run on each placement's own |E|."""
+    from cunibs_amd import fem
+    _, res = block8
+    gm = np.nonzero(gctx.tet_tags == 2)[0]
+    vols = gctx.vols[gm]
+    bary = gctx.tet_barycenters_mm[gm]
+    for r in res:
+        want = oracle_mod.summarize(r.magnE[gm], vols, bary, volume_m3=float(vols.astype(np.float64).sum()))
+        got = fem.metrics_dict(r.metrics)
+        assert got["peak_magnE"] == want["peak_magnE"]
+        np.testing.assert_allclose(got["peak_location_mm"], want["peak_location_mm"], rtol=1e-12)
+        np.testing.assert_allclose(got["center_of_gravity_mm"], want["center_of_gravity_mm"], rtol=1e-6)
+        for key in ("mean", "std", "p50", "p95", "p99", "p99.9"):
+            assert got["distribution"][key] == pytest.approx(want["distribution"][key], rel=1e-6), key
+        for key in ("0.5", "0.75"):
+            assert got["focality_m3"][key] == pytest.approx(want["focality_m3"][key], rel=1e-6), key
 
 
 def test_block_solve_is_bitwise_reproducible(block8, gctx, coil):
