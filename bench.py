@@ -244,6 +244,7 @@ def main() -> int:
     ctx.timing_enable(True)  # reset the in-solve sums
     cold_elapsed, cold_iters, cold_block_iters = sweep(False)
     cold_timing = ctx.timing_read()
+    cold_phases = ctx.timing_phases()
     cold_value = args.steps * k / cold_elapsed
     log(f"batched cold k={k}: {cold_value:.1f} placements/s, {1e3 * cold_elapsed / args.steps:.1f} ms/step, "
         f"iterations mean {np.mean(cold_iters):.1f}")
@@ -264,12 +265,19 @@ def main() -> int:
         ctx.timing_enable(True)
         elapsed, iters, block_iters = sweep(True)
         timing = ctx.timing_read()
+        phases = ctx.timing_phases()
     else:
         elapsed, iters, block_iters = cold_elapsed, cold_iters, cold_block_iters
         timing = cold_timing
+        phases = cold_phases
     value = args.steps * k / elapsed
     log(f"batched{' warm' if args.warm_basis > 0 else ''} k={k}: {value:.1f} placements/s, "
         f"{1e3 * elapsed / args.steps:.1f} ms/step, iterations mean {np.mean(iters):.1f}")
+    # Device time per block by phase (HIP events between the phases of the block solve), and what
+    # the step spent outside them (host work between blocks).
+    block_phases = {p: round(v, 3) for p, v in phases["ms_per_block"].items()}
+    block_phases["between_blocks"] = round(1e3 * elapsed / args.steps - sum(phases["ms_per_block"].values()), 3)
+    log(f"block phases (ms per block of {k}): {block_phases}")
 
     # Plain single-placement GPU PCG on the same sites, at the reference's smoother and the
     # automatic tile (no tuning, no warm start).
@@ -334,6 +342,7 @@ def main() -> int:
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "block_phases_ms": block_phases,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

@@ -110,10 +110,31 @@ struct cn_context {
     } tissue;
     // Device buffers of the per-block region summary (grow-only, reused by every block).
     cn::MetricsWorkspace metrics_ws;
-    // dA/dt timing of the blocks solved while in-solve timing is on (HIP events, outside graphs).
-    hipEvent_t dadt_ev[2] = {nullptr, nullptr};
-    double dadt_ms = 0.0;
-    std::int64_t dadt_placements = 0;
+    // Phase timing of the blocks solved while in-solve timing is on (HIP events on the stream,
+    // outside the graphs): coil frames, dA/dt, element pass + RHS, initial guess, block PCG, field
+    // reconstruction, outputs + region metrics.
+    static constexpr int kPhases = 7;
+    hipEvent_t phase_ev[kPhases + 1] = {};
+    double phase_ms[kPhases] = {};
+    std::int64_t timed_blocks = 0, timed_placements = 0;
+    void mark_phase(int i, hipStream_t st) {
+        if (!solver->timing_on()) return;
+        if (phase_ev[0] == nullptr)
+            for (auto& e : phase_ev) cn::check(hipEventCreate(&e), "phase event");
+        cn::check(hipEventRecord(phase_ev[i], st), "phase event");
+    }
+    // after the block's final sync: add the phase spans
+    void close_phases(int k) {
+        if (!solver->timing_on() || phase_ev[0] == nullptr) return;
+        cn::check(hipEventSynchronize(phase_ev[kPhases]), "phase event");
+        for (int i = 0; i < kPhases; ++i) {
+            float ms = 0.f;
+            cn::check(hipEventElapsedTime(&ms, phase_ev[i], phase_ev[i + 1]), "phase elapsed");
+            phase_ms[i] += ms;
+        }
+        ++timed_blocks;
+        timed_placements += k;
+    }
     // Per-placement conductivities of the current block: (K, T) sigma, combined operator.
     cn::DeviceBuffer<double> blk_sigma;
     cn::DeviceBuffer<float> blk_sigma_tag, blk_off, blk_rowsum;
@@ -140,7 +161,7 @@ struct cn_context {
         return {rptr.get(), rcol.get(), rval32.get(), n - 1, n - 1, nnz_red};
     }
     ~cn_context() {
-        for (auto& e : dadt_ev)
+        for (auto& e : phase_ev)
             if (e) (void)hipEventDestroy(e);
         solver.reset();
         if (s) (void)hipStreamDestroy(s);
@@ -853,6 +874,7 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
     const int K = padded_width(k);
     const std::int64_t n = c.n, m = c.m, nr = n - 1;
 
+    c.mark_phase(0, s);
     // 1. coil frames (place.cu), degenerate handles rejected with the placement's index.
     std::vector<double> tf(16 * k);
     std::vector<std::int32_t> degen(k);
@@ -875,20 +897,9 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
     c.d_q.ensure(4 * m * K, "corner terms");
     c.d_B.ensure(nr * K, "rhs block");
     const float scale = static_cast<float>(didt) * static_cast<float>(kMu0Over4Pi);
-    if (c.solver->timing_on()) {
-        if (c.dadt_ev[0] == nullptr)
-            for (auto& e : c.dadt_ev) check(hipEventCreate(&e), "dadt event");
-        check(hipEventRecord(c.dadt_ev[0], s), "dadt event");
-    }
+    c.mark_phase(1, s);
     dadt_for_frames(c, tf.data(), k, K, pos_m, mom, n_dip, scale, c.d_dadt_nodes.get(), s);
-    if (c.solver->timing_on()) {
-        check(hipEventRecord(c.dadt_ev[1], s), "dadt event");
-        check(hipEventSynchronize(c.dadt_ev[1]), "dadt event");
-        float ms = 0.f;
-        check(hipEventElapsedTime(&ms, c.dadt_ev[0], c.dadt_ev[1]), "dadt elapsed");
-        c.dadt_ms += ms;
-        c.dadt_placements += k;
-    }
+    c.mark_phase(2, s);
     // Per-placement conductivities: the tissue decomposition of the operator and a (K, n_sigma)
     // table for the RHS; padded columns take the context's table.
     const bool per_sigma = sopt != nullptr && sopt->sigma != nullptr;
@@ -933,6 +944,7 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
         rhs_full = c.d_rhs.get();
     }
     node_rhs(c.d_q.get(), c.n2c_ptr.get(), c.n2c_idx.get(), n, k, K, c.ground, c.d_B.get(), rhs_full, s);
+    c.mark_phase(3, s);
     // 3. block PCG.
     SolveOptions opt;
     if (sopt) {
@@ -975,6 +987,7 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
             for (std::int64_t r = 0; r < nr; ++r) out->x0[ci * nr + r] = blk[r * K + ci];
     }
     SolveStats st;
+    c.mark_phase(4, s);
     if (per_sigma) {
         c.solver->set_multi(&c.blk_mop);
         try {
@@ -988,6 +1001,7 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
         st = c.solver->solve(c.d_B.get(), c.d_X.get(), dx0, k, opt);
     }
     if (sopt && sopt->collect) c.warm.append(c.d_X.get(), k, K, s);
+    c.mark_phase(5, s);
     // 4. reconstruction, recovery and metrics.
     const int mode = sopt ? sopt->recovery : CN_RECOVERY_RAW;
     const RecoveryOp* rec = mode == CN_RECOVERY_RAW ? nullptr : ensure_recovery(c, mode);
@@ -1013,6 +1027,7 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
                     c.d_magn.get(), s);
         apply_spr(*rec, c.d_Eraw.get(), m, k, K, c.d_slots.get(), dE, c.d_magn.get(), s);
     }
+    c.mark_phase(6, s);
     if (out) {
         // Placement-fastest device arrays -> placement-major host arrays through one staging buffer.
         auto fetch = [&](float* host, const float* dev, std::int64_t rows, int inner) {
@@ -1049,7 +1064,9 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
             out->round_residual[r] = r < cn::kMaxRounds ? st.round_residual[r] : 0.0;
         }
     }
+    c.mark_phase(7, s);
     sync(s, "block outputs");
+    c.close_phases(k);
     for (int i = 0; i < k; ++i) {
         if (!(st.relative_residual[i] <= opt.tolerance)) {
             char buf[256];
@@ -1491,8 +1508,9 @@ int32_t cn_timing_enable(cn_context* c, int32_t on) {
     return guarded([&] {
         cn::require(c != nullptr, "null context");
         c->solver->set_timing(on != 0);
-        c->dadt_ms = 0.0;
-        c->dadt_placements = 0;
+        for (auto& v : c->phase_ms) v = 0.0;
+        c->timed_blocks = 0;
+        c->timed_placements = 0;
     });
 }
 
@@ -1500,8 +1518,17 @@ int32_t cn_timing_read(const cn_context* c, double* out) {
     return guarded([&] {
         cn::require(c != nullptr && out != nullptr, "null argument");
         c->solver->timing(out);
-        out[4] = static_cast<double>(c->dadt_placements);
-        out[5] = c->dadt_placements ? c->dadt_ms / static_cast<double>(c->dadt_placements) : 0.0;
+        out[4] = static_cast<double>(c->timed_placements);
+        out[5] = c->timed_placements ? c->phase_ms[1] / static_cast<double>(c->timed_placements) : 0.0;
+    });
+}
+
+int32_t cn_timing_phases(const cn_context* c, double* out) {
+    return guarded([&] {
+        cn::require(c != nullptr && out != nullptr, "null argument");
+        for (int i = 0; i < cn_context::kPhases; ++i) out[i] = c->phase_ms[i];
+        out[7] = static_cast<double>(c->timed_blocks);
+        out[8] = static_cast<double>(c->timed_placements);
     });
 }
 

@@ -40,7 +40,7 @@ constexpr int kMetricsLen = 16;
 struct MetricsWorkspace {
     DeviceBuffer<float> vals;                  // (k, n_region) gathered |E|
     DeviceBuffer<std::uint64_t> keys, sorted;  // placement << 32 | bits of |E|
-    DeviceBuffer<std::int32_t> pos, order;     // region position, in sorted order
+    DeviceBuffer<std::int32_t> pos, order;     // volume bits (the sort payload), in sorted order
     DeviceBuffer<double> w, cw, totals;        // sorted volumes, their prefix sums, scan tile totals
     DeviceBuffer<double> part, small;          // block partials; per-placement scalars and results
     DeviceBuffer<unsigned char> scratch;       // radix sort temporary storage

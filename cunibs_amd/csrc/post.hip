@@ -70,19 +70,22 @@ constexpr int kStatBlocks = 256;
 
 // Every placement of the block at once: thread j reads its element's K-wide |E| row and writes
 // vals[c nr + j], the sort key c << 32 | bits(|E|) (|E| >= 0, so the bits order like the values
-// and the placement index keeps the placements apart) and its region position j.
+// and the placement index keeps the placements apart) and, as the sort's payload, the bits of the
+// element's volume.
 __global__ void gather_region_kernel(const float* __restrict__ magn, int K, int k, const std::int32_t* __restrict__ idx,
-                                     std::int64_t nr, float* __restrict__ vals, std::uint64_t* __restrict__ keys,
-                                     std::int32_t* __restrict__ pos) {
+                                     const float* __restrict__ vols, std::int64_t nr, float* __restrict__ vals,
+                                     std::uint64_t* __restrict__ keys, std::int32_t* __restrict__ vol_bits) {
     const std::int64_t j = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= nr) return;
-    const float* row = magn + static_cast<std::int64_t>(idx[j]) * K;
+    const std::int64_t e = idx[j];
+    const float* row = magn + e * K;
+    const std::int32_t vb = __float_as_int(vols[e]);
     for (int c = 0; c < k; ++c) {
         const float v = row[c];
         const std::int64_t t = c * nr + j;
         vals[t] = v;
         keys[t] = (static_cast<std::uint64_t>(c) << 32) | __float_as_uint(v);
-        pos[t] = static_cast<std::int32_t>(j);
+        vol_bits[t] = vb;
     }
 }
 
@@ -172,11 +175,12 @@ __global__ void peak_location_kernel(const double* __restrict__ args, const std:
     for (int d = 0; d < 3; ++d) loc[3 * c + d] = e >= 0 ? bary[3 * e + d] : NAN;
 }
 
-// w[t] = volume of the element at sorted position t (every placement's segment at once).
-__global__ void sorted_weights_kernel(const std::int32_t* __restrict__ order, const std::int32_t* __restrict__ idx,
-                                      const float* __restrict__ vols, std::int64_t total, double* __restrict__ w) {
+// w[t] = cw[t] = volume of the element at sorted position t (every placement's segment at once);
+// cw is then scanned in place.
+__global__ void sorted_weights_kernel(const std::int32_t* __restrict__ vol_bits, std::int64_t total,
+                                      double* __restrict__ w, double* __restrict__ cw) {
     const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t < total) w[t] = static_cast<double>(vols[idx[order[t]]]);
+    if (t < total) w[t] = cw[t] = static_cast<double>(__int_as_float(vol_bits[t]));
 }
 
 // Fixed-order inclusive scan, per segment (one placement each: `len` elements at stride `stride`),
@@ -355,16 +359,15 @@ void region_metrics(const float* magn, std::int64_t m, int k, int K, const std::
     const double consts[6] = {static_cast<double>(0.5f), static_cast<double>(0.95f), static_cast<double>(0.99f),
                               static_cast<double>(0.999f), 0.5, 0.75};
     h2d(qd, consts, 6, s);
-    gather_region_kernel<<<grid_for(nr), kBlock, 0, s>>>(magn, K, k, idx, nr, ws.vals.get(), ws.keys.get(),
+    gather_region_kernel<<<grid_for(nr), kBlock, 0, s>>>(magn, K, k, idx, vols, nr, ws.vals.get(), ws.keys.get(),
                                                          ws.pos.get());
     stats_kernel<<<dim3(kStatBlocks, k), kBlock, 0, s>>>(ws.vals.get(), idx, vols, bary, nr, ws.part.get());
-    // distribution quantiles and focality: a stable sort by (placement, |E|), the volumes in that
-    // order, their per-placement fixed-order prefix sums
+    // distribution quantiles and focality: a stable sort by (placement, |E|) carrying the volumes,
+    // their per-placement fixed-order prefix sums
     int bits = 32;
     while ((1 << (bits - 32)) < k) ++bits;
     sort_pairs_u64_ws(ws.keys.get(), ws.sorted.get(), ws.pos.get(), ws.order.get(), total, bits, ws.scratch, s);
-    sorted_weights_kernel<<<grid_for(total), kBlock, 0, s>>>(ws.order.get(), idx, vols, total, ws.w.get());
-    d2d(ws.cw.get(), ws.w.get(), static_cast<std::size_t>(total), s);
+    sorted_weights_kernel<<<grid_for(total), kBlock, 0, s>>>(ws.order.get(), total, ws.w.get(), ws.cw.get());
     prefix_sum_fixed(ws.cw.get(), nr, nr, k, ws.totals.get(), s);
     quantile_kernel<<<k, 64, 0, s>>>(ws.sorted.get(), ws.w.get(), ws.cw.get(), nr, qd, 4, q_out);
     focality_kernel<<<k, 64, 0, s>>>(ws.sorted.get(), ws.cw.get(), nr, q_out, 4, fracs, 2, foc);

@@ -69,44 +69,80 @@ __global__ void gather_modes_kernel(const unsigned short* __restrict__ modes, st
     out[t] = (j0 + c < R) ? pair_value(modes, r * R + j0 + c, n * R) : 0.0f;
 }
 
-// Partials of W^T B: one thread per basis column s (blockDim = S rounded up to a wave multiple),
-// the block walks its row range; B rows are read whole by every thread (a broadcast).
-template <int K>
-__global__ void __launch_bounds__(512) project_kernel(const unsigned short* __restrict__ modes, std::int64_t n, int S,
-                                                      const float* __restrict__ B, std::int64_t rows_per_block,
-                                                      double* __restrict__ partials) {
-    const int s = threadIdx.x;
+typedef double doublex4_t __attribute__((ext_vector_type(4)));
+
+// Partials of Phi^T B on the FP64 matrix cores: v_mfma_f64_16x16x4f64 with A = a 16-mode x 4-row
+// block of Phi^T (lane l: mode 16 t + l % 16 of row r0 + l / 16, read as its exact FP32 hi + lo
+// value) and B = the 4 rows of the block (lane l: column l % 16), accumulated in FP64 per 16-mode
+// tile t; accumulator register v of lane l holds mode 16 t + 4 v + l / 16, column l % 16. The
+// block's waves fold into its partial in wave order: partials[(block * R16 + mode) * K + c].
+constexpr int kProjectThreads = 512;
+
+__global__ void __launch_bounds__(kProjectThreads) project_kernel(const unsigned short* __restrict__ modes,
+                                                                  std::int64_t n, int R16, const float* __restrict__ B,
+                                                                  int K, std::int64_t rows_per_block,
+                                                                  double* __restrict__ partials) {
+    constexpr int kWaves = kProjectThreads / 64;
+    const int T = R16 / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, quad = lane >> 4;
     const std::int64_t lo = static_cast<std::int64_t>(blockIdx.x) * rows_per_block;
     const std::int64_t hi = min(lo + rows_per_block, n);
-    double acc[K];
+    const std::int64_t half = n * R16;
+    doublex4_t acc[16];
 #pragma unroll
-    for (int c = 0; c < K; ++c) acc[c] = 0.0;
-    if (s < S) {
-        for (std::int64_t r = lo; r < hi; ++r) {
-            const double w = static_cast<double>(pair_value(modes, r * S + s, n * S));
+    for (int t = 0; t < 16; ++t) acc[t] = doublex4_t{0.0, 0.0, 0.0, 0.0};
+    for (std::int64_t r0 = lo + 4 * wave; r0 < hi; r0 += 4 * kWaves) {
+        const std::int64_t row = r0 + quad;
+        const bool ok = row < hi;
+        const std::int64_t rr = ok ? row : hi - 1;  // clamped: the loads stay in bounds
+        const double b = (ok && col < K) ? static_cast<double>(B[rr * K + col]) : 0.0;
+        const unsigned short* m = modes + rr * R16 + col;
 #pragma unroll
-            for (int c = 0; c < K; ++c) acc[c] = fma(w, static_cast<double>(B[r * K + c]), acc[c]);
+        for (int t = 0; t < 16; ++t) {
+            if (t < T) {
+                const double a = static_cast<double>(bf16_value(m[16 * t]) + bf16_value(m[half + 16 * t]));
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+            }
         }
-        double* out = partials + (static_cast<std::int64_t>(blockIdx.x) * S + s) * K;
-#pragma unroll
-        for (int c = 0; c < K; ++c) out[c] = acc[c];
     }
+    __shared__ double red[kMaxWarmModes * 16];
+    for (int e = threadIdx.x; e < R16 * 16; e += blockDim.x) red[e] = 0.0;
+    __syncthreads();
+    for (int w = 0; w < kWaves; ++w) {
+        if (wave == w) {
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                if (t < T) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) red[(16 * t + 4 * v + quad) * 16 + col] += acc[t][v];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double* out = partials + static_cast<std::int64_t>(blockIdx.x) * R16 * K;
+    for (int e = threadIdx.x; e < R16 * K; e += blockDim.x) out[e] = red[(e / K) * 16 + e % K];
 }
 
-// c = P (Phi^T B): the partials summed over blocks in order, then the R x R projector (FP64);
-// c is split into BF16 hi/lo halves laid out for the expansion: coef[(half * R + s) * 16 + col]
-// (R a multiple of 16, columns >= k zero).
-template <int K>
-__global__ void coefficients_kernel(const double* __restrict__ partials, int nb, int R,
-                                    const double* __restrict__ P, int k, unsigned short* __restrict__ coef) {
-    extern __shared__ double ptb[];  // R * K
-    for (int e = threadIdx.x; e < R * K; e += blockDim.x) {
-        double acc = 0.0;
-        for (int b = 0; b < nb; ++b) acc += partials[static_cast<std::int64_t>(b) * R * K + e];
-        ptb[e] = acc;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < R * 16; e += blockDim.x) {
+// out[e] = sum over the nb block partials of entry e (E entries per block): one wavefront per
+// entry, lane j adding blocks j, j + 64, ... in order, then a fixed xor tree.
+__global__ void sum_blocks_kernel(const double* __restrict__ partials, int nb, int E, double* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int e = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (e >= E) return;
+    double acc = 0.0;
+    for (int b = lane; b < nb; b += 64) acc += partials[static_cast<std::int64_t>(b) * E + e];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) out[e] = acc;
+}
+
+// c = P (Phi^T B) from the reduced R x K product, then split into BF16 hi/lo halves laid out for
+// the expansion: coef[(half * R + s) * 16 + col] (R a multiple of 16, columns >= k zero).
+__global__ void coefficients_kernel(const double* __restrict__ ptb, int R, int K, const double* __restrict__ P, int k,
+                                    unsigned short* __restrict__ coef) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < R * 16; e += gridDim.x * blockDim.x) {
         const int sidx = e / 16, c = e % 16;
         double v = 0.0;
         if (c < k) {
@@ -126,36 +162,56 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4_t;
 // x0 (n, K) FP64 = Phi c: one wavefront per 16-row tile, v_mfma_f32_16x16x16bf16_1k over the modes
 // in 16-column steps; hi.hi, hi.lo and lo.hi products accumulate into the same FP32 tile.
 // Operand layout of the 16x16x16 BF16 MFMA (CDNA3/4 ISA): lane l supplies A[l % 16][4 (l / 16) + j]
-// and B[4 (l / 16) + j][l % 16], j = 0..3, and receives D[4 (l / 16) + j][l % 16].
-__global__ void __launch_bounds__(256) expand_kernel(const unsigned short* __restrict__ modes, std::int64_t n, int S,
-                                                     int S16, const unsigned short* __restrict__ coef, int k, int K,
-                                                     double* __restrict__ x0) {
+// and B[4 (l / 16) + j][l % 16], j = 0..3, and receives D[4 (l / 16) + j][l % 16]. Every step's
+// A fragment (4 consecutive modes of one row: one 8-byte load per half) is loaded before the first
+// product; the coefficients sit in LDS transposed, so a lane's B fragment is one 8-byte read.
+constexpr int kExpandThreads = 256;
+
+__global__ void __launch_bounds__(kExpandThreads) expand_kernel(const unsigned short* __restrict__ modes,
+                                                                std::int64_t n, int R16,
+                                                                const unsigned short* __restrict__ coef, int k, int K,
+                                                                double* __restrict__ x0) {
+    __shared__ uint2 ct[2][16][kMaxWarmModes / 4];  // [half][column][mode / 4]: 4 modes' BF16 bits
+    {
+        unsigned short* flat = reinterpret_cast<unsigned short*>(ct);
+        for (int e = threadIdx.x; e < 2 * R16 * 16; e += blockDim.x) {
+            const int h = e / (R16 * 16), r = e % (R16 * 16), s = r / 16, c = r % 16;
+            flat[(h * 16 + c) * kMaxWarmModes + s] = coef[e];
+        }
+    }
+    __syncthreads();
     const int lane = threadIdx.x & 63;
-    const std::int64_t tile = static_cast<std::int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const std::int64_t tile = static_cast<std::int64_t>(blockIdx.x) * (kExpandThreads / 64) + (threadIdx.x >> 6);
     const std::int64_t r0 = tile * 16;
     if (r0 >= n) return;
     const int i = lane & 15, q = lane >> 4;
     const std::int64_t row = r0 + i;
+    const std::int64_t rr = row < n ? row : n - 1;  // clamped: the loads stay in bounds
+    const int T = R16 / 16;
+    const uint2* hrow = reinterpret_cast<const uint2*>(modes + rr * R16);
+    const uint2* lrow = reinterpret_cast<const uint2*>(modes + n * R16 + rr * R16);
+    uint2 ah[16], al[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        if (t < T) {
+            ah[t] = hrow[4 * t + q];
+            al[t] = lrow[4 * t + q];
+        }
+    }
     floatx4_t acc = {0.f, 0.f, 0.f, 0.f};
-    for (int s0 = 0; s0 < S16; s0 += 16) {
-        bf16x4_t a, alo;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int s = s0 + 4 * q + j;
-            const bool ok = row < n && s < S;
-            a[j] = ok ? static_cast<short>(modes[row * S + s]) : static_cast<short>(0);
-            alo[j] = ok ? static_cast<short>(modes[n * S + row * S + s]) : static_cast<short>(0);
+    for (int t = 0; t < 16; ++t) {
+        if (t < T) {
+            const uint2 bh2 = ct[0][i][4 * t + q], bl2 = ct[1][i][4 * t + q];
+            bf16x4_t a, alo, bh, bl;
+            __builtin_memcpy(&a, &ah[t], 8);
+            __builtin_memcpy(&alo, &al[t], 8);
+            __builtin_memcpy(&bh, &bh2, 8);
+            __builtin_memcpy(&bl, &bl2, 8);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo, bh, acc, 0, 0, 0);
         }
-        bf16x4_t bh, bl;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int s = s0 + 4 * q + j;
-            bh[j] = static_cast<short>(coef[s * 16 + i]);
-            bl[j] = static_cast<short>(coef[S16 * 16 + s * 16 + i]);
-        }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bh, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bl, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo, bh, acc, 0, 0, 0);
     }
     // acc[j] = x0[r0 + 4 q + j][i]
     if (i < K) {
@@ -296,31 +352,25 @@ void WarmBasis::mode_chunk(int j0, float* out, hipStream_t s) const {
 }
 
 void WarmBasis::expand(int k, int K, double* x0, hipStream_t s) {
+    require(n_modes % 16 == 0 && n_modes <= kMaxWarmModes, "mode width must be a multiple of 16, at most 256");
     const std::int64_t tiles = (n + 15) / 16;
-    expand_kernel<<<grid_for(tiles, 4), 256, 0, s>>>(modes.get(), n, n_modes, n_modes, coef.get(), k, K, x0);
+    expand_kernel<<<grid_for(tiles, kExpandThreads / 64), kExpandThreads, 0, s>>>(modes.get(), n, n_modes, coef.get(),
+                                                                                 k, K, x0);
 }
 
 void WarmBasis::initial_guess(const float* B, int k, int K, double* x0, hipStream_t s) {
     require(ready, "warm start requested but no modes are set");
+    require(K >= 1 && K <= 16 && k <= K, "warm start supports block widths up to 16");
+    require(n_modes % 16 == 0 && n_modes <= kMaxWarmModes, "mode width must be a multiple of 16, at most 256");
     const int R = n_modes;
     const int nb = kProjectBlocks;
     const std::int64_t rpb = (n + nb - 1) / nb;
-    partials.ensure(static_cast<std::size_t>(nb) * R * K, "warm partials");
+    partials.ensure(static_cast<std::size_t>(nb) * R * K + static_cast<std::size_t>(R) * K, "warm partials");
+    double* const ptb = partials.get() + static_cast<std::size_t>(nb) * R * K;
     coef.ensure(static_cast<std::size_t>(2) * R * 16, "warm coefficients");
-    const int threads = (R + 63) / 64 * 64;
-    auto run = [&](auto kc) {
-        constexpr int KK = decltype(kc)::value;
-        project_kernel<KK><<<nb, threads, 0, s>>>(modes.get(), n, R, B, rpb, partials.get());
-        coefficients_kernel<KK><<<1, 512, sizeof(double) * R * KK, s>>>(partials.get(), nb, R, P.get(), k, coef.get());
-    };
-    switch (K) {
-        case 1: run(std::integral_constant<int, 1>{}); break;
-        case 2: run(std::integral_constant<int, 2>{}); break;
-        case 4: run(std::integral_constant<int, 4>{}); break;
-        case 8: run(std::integral_constant<int, 8>{}); break;
-        case 16: run(std::integral_constant<int, 16>{}); break;
-        default: throw InvalidArgument("warm start supports block widths up to 16");
-    }
+    project_kernel<<<nb, kProjectThreads, 0, s>>>(modes.get(), n, R, B, K, rpb, partials.get());
+    sum_blocks_kernel<<<(R * K + 3) / 4, 256, 0, s>>>(partials.get(), nb, R * K, ptb);
+    coefficients_kernel<<<(R * 16 + 255) / 256, 256, 0, s>>>(ptb, R, K, P.get(), k, coef.get());
     expand(k, K, x0, s);
 }
 

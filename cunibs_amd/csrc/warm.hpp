@@ -8,7 +8,7 @@ namespace cn {
 
 constexpr int kMaxWarmBasis = 512;
 constexpr int kMaxWarmModes = 256;
-constexpr int kProjectBlocks = 1024;
+constexpr int kProjectBlocks = 512;
 
 struct WarmBasis {
     std::int64_t n = 0;  // rows (unknowns of the reduced system)

@@ -499,6 +499,16 @@ class SolverContext:
         return {"samples": int(out[0]), "spmm_ms": float(out[1]), "vcycle_ms": float(out[2]),
                 "iteration_ms": float(out[3]), "dadt_placements": int(out[4]), "dadt_ms": float(out[5])}
 
+    PHASES = ("frames", "dadt", "rhs", "guess", "pcg", "field", "outputs")
+
+    def timing_phases(self) -> dict:
+        """Device ms per block of each phase of the blocks solved while timing is on."""
+        out = np.zeros(9)
+        check(lib().cn_timing_phases(self.handle, ptr(out)), "cn_timing_phases")
+        blocks = int(out[7])
+        per = {p: (float(out[i]) / blocks if blocks else 0.0) for i, p in enumerate(self.PHASES)}
+        return {"blocks": blocks, "placements": int(out[8]), "ms_per_block": per}
+
     def bench_dadt(self, dip_pos_m, dip_moment, site, k: int = 8, reps: int = 5) -> float:
         """Average ms of the dA/dt N-body for k copies of one placement (repeated launches)."""
         pos = np.ascontiguousarray(dip_pos_m, np.float64).reshape(-1, 3)

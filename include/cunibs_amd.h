@@ -328,6 +328,13 @@ int32_t cn_timing_enable(cn_context* ctx, int32_t on);
  * dA/dt was timed (HIP events around the N-body launch of each block, which runs outside the
  * graphs), out[5] mean ms of dA/dt per placement. */
 int32_t cn_timing_read(const cn_context* ctx, double* out6);
+/* Where the blocks solved while timing is on spent their device time, in ms summed over those
+ * blocks (HIP events recorded on the library's stream between the phases of cn_solve_block, so
+ * host work between the phases shows up in the phase it delays): out[0] coil frames, out[1] dA/dt,
+ * out[2] element pass and RHS, out[3] initial guess, out[4] block PCG, out[5] field
+ * reconstruction and recovery, out[6] outputs and region metrics; out[7] blocks, out[8]
+ * placements. A measurement aid with no counterpart in the reference. */
+int32_t cn_timing_phases(const cn_context* ctx, double* out9);
 /* Average device time (ms) of the dA/dt N-body for k copies of one placement over the context's
  * nodes, repeated launches. A measurement aid with no counterpart in the reference. */
 int32_t cn_bench_dadt(cn_context* ctx, const double* dip_pos_m, const double* dip_moment, int64_t n_dip,

@@ -110,6 +110,7 @@ def test_in_solve_timing_leaves_the_answer_bitwise_unchanged(tctx, coil):
     try:
         timed = fem.solve_placements_block(tctx, coil.positions_m, coil.moments, sites, 1e6)
         t = tctx.timing_read()
+        ph = tctx.timing_phases()
     finally:
         tctx.timing_enable(False)
     for a, b in zip(base, timed):
@@ -119,3 +120,8 @@ def test_in_solve_timing_leaves_the_answer_bitwise_unchanged(tctx, coil):
     assert 0 < t["spmm_ms"] < t["iteration_ms"]
     assert 0 < t["vcycle_ms"] < t["iteration_ms"]
     assert t["spmm_ms"] + t["vcycle_ms"] <= t["iteration_ms"]
+    assert ph["blocks"] == 1 and ph["placements"] == 8
+    per = ph["ms_per_block"]
+    assert set(per) == set(fem.SolverContext.PHASES)
+    assert all(v >= 0 for v in per.values()) and per["pcg"] > 0 and per["dadt"] > 0
+    assert t["dadt_ms"] == pytest.approx(per["dadt"] / 8)
