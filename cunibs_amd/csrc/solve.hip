@@ -542,85 +542,6 @@ void launch_gram(std::int64_t n, const float* A, const float* B1, const float* B
 constexpr int kGramThreads = 1024;
 constexpr int kGramWaves = kGramThreads / kWave;
 
-// FP64 true residual R64 = B - A X64 with the FP64 operator, and per-column ||R64||^2 partials.
-template <int K>
-__global__ void __launch_bounds__(kGramThreads) residual64_kernel(
-    std::int64_t n, const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
-    const double* __restrict__ av, const float* __restrict__ B, const double* __restrict__ X,
-    double* __restrict__ R, double* __restrict__ partials, std::int64_t rows_per_block, bool x_zero) {
-    constexpr int L = spmm_lanes<K>();
-    constexpr int SUB = L / K;
-    constexpr int RPW = kWave / L;
-    const int lb = logical_block();
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
-    const int l = lane % L, c = l % K, sub = l / K;
-    const std::int64_t lo = static_cast<std::int64_t>(lb) * rows_per_block;
-    const std::int64_t hi = min(lo + rows_per_block, n);
-    double nrm = 0.0;
-    for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW; r0 < hi; r0 += kGramWaves * RPW) {
-        const std::int64_t row = r0 + lane / L;
-        const bool valid = row < hi;
-        double acc = 0.0;
-        if (valid && !x_zero) {
-            const int e = rp[row + 1];
-            for (int j = rp[row] + sub; j < e; j += SUB)
-                acc = fma(av[j], X[static_cast<std::int64_t>(ci[j]) * K + c], acc);
-        }
-#pragma unroll
-        for (int off = K; off < L; off <<= 1) acc += __shfl_xor(acc, off, kWave);
-        if (valid && sub == 0) {
-            const double r = static_cast<double>(B[row * K + c]) - acc;
-            R[row * K + c] = r;
-            nrm = fma(r, r, nrm);
-        }
-    }
-    nrm = wave_rows_sum<L>(nrm);
-    block_vec_store<K, kGramWaves>(nrm, partials + static_cast<std::int64_t>(lb) * K);
-}
-
-// The FP64 true residual for K matrices A_c = sum_t sigma[c, t] A_t sharing one pattern: the
-// per-tissue FP64 values tv (T, nnz) are combined per entry in tissue order.
-template <int K>
-__global__ void __launch_bounds__(kGramThreads) residual64_multi_kernel(
-    std::int64_t n, const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
-    const double* __restrict__ tv, std::int64_t nnz, int T, const double* __restrict__ sigma,
-    const float* __restrict__ B, const double* __restrict__ X, double* __restrict__ R, double* __restrict__ partials,
-    std::int64_t rows_per_block, bool x_zero) {
-    constexpr int L = spmm_lanes<K>();
-    constexpr int SUB = L / K;
-    constexpr int RPW = kWave / L;
-    const int lb = logical_block();
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
-    const int l = lane % L, c = l % K, sub = l / K;
-    const std::int64_t lo = static_cast<std::int64_t>(lb) * rows_per_block;
-    const std::int64_t hi = min(lo + rows_per_block, n);
-    double sg[kMaxTissues];
-    for (int t = 0; t < kMaxTissues; ++t) sg[t] = t < T ? sigma[c * T + t] : 0.0;
-    double nrm = 0.0;
-    for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW; r0 < hi; r0 += kGramWaves * RPW) {
-        const std::int64_t row = r0 + lane / L;
-        const bool valid = row < hi;
-        double acc = 0.0;
-        if (valid && !x_zero) {
-            const int e = rp[row + 1];
-            for (int j = rp[row] + sub; j < e; j += SUB) {
-                double a = 0.0;
-                for (int t = 0; t < T; ++t) a = fma(sg[t], tv[static_cast<std::int64_t>(t) * nnz + j], a);
-                acc = fma(a, X[static_cast<std::int64_t>(ci[j]) * K + c], acc);
-            }
-        }
-#pragma unroll
-        for (int off = K; off < L; off <<= 1) acc += __shfl_xor(acc, off, kWave);
-        if (valid && sub == 0) {
-            const double r = static_cast<double>(B[row * K + c]) - acc;
-            R[row * K + c] = r;
-            nrm = fma(r, r, nrm);
-        }
-    }
-    nrm = wave_rows_sum<L>(nrm);
-    block_vec_store<K, kGramWaves>(nrm, partials + static_cast<std::int64_t>(lb) * K);
-}
-
 // ---------------------------------------------------------------------------------------------
 // V-cycle kernels (solver/amg/vcycle.cu:124-311), K-wide, FP32. The smoother and residual passes
 // are rowk_kernel; SUB is chosen per level from its shape (wide coarse rows take more lanes).
@@ -687,6 +608,108 @@ __device__ __forceinline__ void block_cols_store(double (&v)[V], int grp, double
         __syncthreads();
     }
     if (threadIdx.x < K) out[threadIdx.x] = acc[threadIdx.x];
+}
+
+// FP64 true residual R64 = B - A X64 with the FP64 operator (MULTI: A_c = sum_t sigma[c, t] A_t
+// from the per-tissue FP64 values tv (T, nnz), combined per entry in tissue order), and per-column
+// ||R64||^2 partials. A lane carries V = 2 consecutive columns (16-byte loads of X64), G = K / V
+// lanes per row; SUB sub-lanes split the row's entries and are folded by a fixed xor tree. Each
+// sub-lane loads a batch of kResBatch entries (indices clamped to the row, values zeroed past it)
+// before their X64 gathers, so the gathers of a batch are in flight together.
+template <int K>
+constexpr int res_vec() {
+    return K >= 2 ? 2 : 1;
+}
+template <int K>
+constexpr int res_sub() {
+    return (K / res_vec<K>()) >= 16 ? 1 : 16 / (K / res_vec<K>());
+}
+constexpr int kResBatch = 4;
+
+template <int K, bool MULTI>
+__global__ void __launch_bounds__(kGramThreads) residual64_kernel(
+    std::int64_t n, const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
+    const double* __restrict__ av, std::int64_t nnz, int T, const double* __restrict__ sigma,
+    const float* __restrict__ B, const double* __restrict__ X, double* __restrict__ R, double* __restrict__ partials,
+    std::int64_t rows_per_block, bool x_zero) {
+    constexpr int V = res_vec<K>();
+    constexpr int G = K / V;
+    constexpr int SUB = res_sub<K>();
+    constexpr int L = G * SUB;
+    constexpr int RPW = kWave / L;
+    const int lb = logical_block();
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
+    const int l = lane % L, grp = l % G, sub = l / G, c0 = grp * V;
+    const std::int64_t lo = static_cast<std::int64_t>(lb) * rows_per_block;
+    const std::int64_t hi = min(lo + rows_per_block, n);
+    double nrm[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) nrm[v] = 0.0;
+    for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW; r0 < hi; r0 += kGramWaves * RPW) {
+        const std::int64_t row = r0 + lane / L;
+        const bool valid = row < hi;
+        double acc[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] = 0.0;
+        if (valid && !x_zero) {
+            const int b = rp[row], e = rp[row + 1];
+            for (int j0 = b + sub; j0 < e; j0 += SUB * kResBatch) {
+                int col[kResBatch];
+                double a[kResBatch][V];
+#pragma unroll
+                for (int u = 0; u < kResBatch; ++u) {
+                    const int j = j0 + SUB * u;
+                    const bool ok = j < e;
+                    const int jj = ok ? j : e - 1;
+                    col[u] = ci[jj];
+                    if constexpr (MULTI) {
+#pragma unroll
+                        for (int v = 0; v < V; ++v) a[u][v] = 0.0;
+                        for (int t = 0; t < T; ++t) {
+                            const double tv = av[static_cast<std::int64_t>(t) * nnz + jj];
+#pragma unroll
+                            for (int v = 0; v < V; ++v) a[u][v] = fma(sigma[(c0 + v) * T + t], tv, a[u][v]);
+                        }
+#pragma unroll
+                        for (int v = 0; v < V; ++v) a[u][v] = ok ? a[u][v] : 0.0;
+                    } else {
+                        const double x = ok ? av[jj] : 0.0;
+#pragma unroll
+                        for (int v = 0; v < V; ++v) a[u][v] = x;
+                    }
+                }
+                double xv[kResBatch][V];
+#pragma unroll
+                for (int u = 0; u < kResBatch; ++u) {
+                    const double* xp = X + static_cast<std::int64_t>(col[u]) * K + c0;
+                    if constexpr (V == 2) {
+                        const double2 t2 = *reinterpret_cast<const double2*>(xp);
+                        xv[u][0] = t2.x;
+                        xv[u][1] = t2.y;
+                    } else {
+                        xv[u][0] = xp[0];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kResBatch; ++u)
+#pragma unroll
+                    for (int v = 0; v < V; ++v) acc[v] = fma(a[u][v], xv[u][v], acc[v]);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int off = G; off < L; off <<= 1) acc[v] += __shfl_xor(acc[v], off, kWave);
+        if (valid && sub == 0) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const double r = static_cast<double>(B[row * K + c0 + v]) - acc[v];
+                R[row * K + c0 + v] = r;
+                nrm[v] = fma(r, r, nrm[v]);
+            }
+        }
+    }
+    block_cols_store<K, V, kGramWaves>(nrm, grp, partials + static_cast<std::int64_t>(lb) * K);
 }
 
 // X64 += W a (FP64), R -= S a (FP32), partials of ||R_c||^2. Frozen columns have a zero column
@@ -1404,12 +1427,13 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
     std::vector<double> bnorm(K), rnorm(K);
     auto true_residual = [&](bool zero) {
         if (multi_)
-            residual64_multi_kernel<K><<<kGramBlocks, kGramThreads, 0, s_>>>(
+            residual64_kernel<K, true><<<kGramBlocks, kGramThreads, 0, s_>>>(
                 nn, op_.a32.ptr, op_.a32.col, multi_->tissue_vals, op_.a32.nnz, multi_->n_tissue, multi_->sigma, B, Xi,
                 R64_.get(), partials_.get(), rpb, zero);
         else
-            residual64_kernel<K><<<kGramBlocks, kGramThreads, 0, s_>>>(nn, op_.a32.ptr, op_.a32.col, op_.val64, B, Xi,
-                                                                       R64_.get(), partials_.get(), rpb, zero);
+            residual64_kernel<K, false><<<kGramBlocks, kGramThreads, 0, s_>>>(
+                nn, op_.a32.ptr, op_.a32.col, op_.val64, op_.a32.nnz, 0, nullptr, B, Xi, R64_.get(), partials_.get(), rpb,
+                zero);
         column_norms_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, norms_.get());
         d2h(rnorm.data(), norms_.get(), K, s_);
         sync(s_, "true residual");
