@@ -73,8 +73,9 @@ def kernels_only(ctx, coil, sites, k: int) -> int:
     """The launches the HBM traffic counters are read on: 20 streaming copies of COPY_BYTES (the
     counter calibration), 20 differenced SpMMs at width k (each after one untimed launch), then one
     cold block of k placements with each recovery (raw, spr_global, harmonic), so that every kernel
-    of the path appears once with the same sizes as in the sweep."""
-    from cunibs_amd import fem
+    of the path appears once with the same sizes as in the sweep; then the sweep's warm-start basis
+    (256 solves) and two warm-started raw blocks, the sweep's steady state."""
+    from cunibs_amd import fem, synthetic
 
     K = k if k in fem.BLOCK_SIZES else min(b for b in fem.BLOCK_SIZES if b >= k)
     n_red, nnz = int(ctx.info.n_reduced), int(ctx.info.nnz_reduced)
@@ -85,6 +86,16 @@ def kernels_only(ctx, coil, sites, k: int) -> int:
         res = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites[:k], 1e6, magnitude=False,
                                          metrics=True, recovery=rec)
         iters[rec] = [r.iterations for r in res]
+    train = synthetic.cap_sites(256, seed=7)
+    ctx.warm_reset(len(train))
+    for i in range(0, len(train), 16):
+        fem.solve_placements_block(ctx, coil.positions_m, coil.moments, train[i:i + 16], 1e6, metrics=False,
+                                   magnitude=False, collect=True)
+    ctx.warm_finalize()
+    for i in range(2):
+        res = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites[k * (i + 3):k * (i + 4)], 1e6,
+                                         magnitude=False, metrics=True, warm_start=True)
+        iters[f"warm{i}"] = [r.iterations for r in res]
     print(json.dumps({
         "copy_bytes_read": COPY_BYTES, "copy_bytes_written": COPY_BYTES, "copy_us": round(copy_ms * 1e3, 2),
         "copy_GBps": round(2 * COPY_BYTES / (copy_ms * 1e-3) / 1e9, 1),
@@ -278,6 +289,10 @@ def main() -> int:
     block_phases = {p: round(v, 3) for p, v in phases["ms_per_block"].items()}
     block_phases["between_blocks"] = round(1e3 * elapsed / args.steps - sum(phases["ms_per_block"].values()), 3)
     log(f"block phases (ms per block of {k}): {block_phases}")
+    guess_ms = None
+    if args.warm_basis > 0:
+        guess_ms = {key: round(v, 4) for key, v in ctx.bench_guess(k, 5).items()}
+        log(f"warm-start guess stages: {guess_ms}")
 
     # Plain single-placement GPU PCG on the same sites, at the reference's smoother and the
     # automatic tile (no tuning, no warm start).
@@ -343,6 +358,7 @@ def main() -> int:
         "warmup": args.warmup,
         "ms_per_step": round(1e3 * elapsed / args.steps, 3),
         "block_phases_ms": block_phases,
+        "guess_stage_ms": guess_ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

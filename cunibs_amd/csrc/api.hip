@@ -142,7 +142,7 @@ struct cn_context {
     struct Uq {
         bool ready = false;
         cn::DeviceBuffer<float> rhs;                  // (T, n_red)
-        cn::DeviceBuffer<float> dadt_elm1, dadt_nodes1;  // the placement's dA/dt, one column
+        cn::DeviceBuffer<float> dadt_nodes1;  // the placement's nodal dA/dt, one column
         cn::DeviceBuffer<double> sigma;               // (K, T)
         cn::DeviceBuffer<float> off_k, rowsum_k;      // (nnz_off, K), (n_red, K)
         cn::DeviceBuffer<double> sum, sumsq;          // |E| moments over every sample solved
@@ -715,13 +715,12 @@ void uq_prepare(cn_context& c, const double* pos_m, const double* mom, std::int6
     auto ddip = upload(dip.data(), dip.size(), s, "dipoles");
     auto dorg = upload(origin.data(), 3, s, "origin");
     u.dadt_nodes1.reset(3 * n, "uq dadt nodes");
-    u.dadt_elm1.reset(3 * m, "uq dadt elements");
     dadt_nbody(ddip.get(), static_cast<int>(n_dip), c.nodes.get(), n, dorg.get(), 1, 1,
                static_cast<float>(didt) * static_cast<float>(kMu0Over4Pi), u.dadt_nodes1.get(), s);
-    DeviceBuffer<float> negvc(m, "tissue neg_vc"), q(4 * m, "corner terms"), elm(3 * m, "elm");
+    DeviceBuffer<float> negvc(m, "tissue neg_vc"), q(4 * m, "corner terms");
     for (int t = 0; t < T; ++t) {
         tissue_neg_vc(c.vols32.get(), c.tags.get(), m, present[t], negvc.get(), s);
-        element_pass(u.dadt_nodes1.get(), c.tets.get(), c.g32.get(), negvc.get(), m, 1, 1, u.dadt_elm1.get(), q.get(), s);
+        element_pass(u.dadt_nodes1.get(), c.tets.get(), c.g32.get(), negvc.get(), m, 1, 1, nullptr, q.get(), s);
         node_rhs(q.get(), c.n2c_ptr.get(), c.n2c_idx.get(), n, 1, 1, c.ground,
                  u.rhs.get() + static_cast<std::int64_t>(t) * nr, nullptr, s);
     }
@@ -800,15 +799,14 @@ void uq_solve(cn_context& c, const double* sigma, int k, const cn_solve_options*
     }
     c.solver->set_multi(nullptr);
     // |E| per sample: dA/dt is the same for every sample.
-    c.d_dadt_elm.ensure(3 * m * K, "dadt elements");
-    broadcast_columns(u.dadt_elm1.get(), 3 * m, K, c.d_dadt_elm.get(), s);
     c.d_magn.ensure(m * K, "magnE");
     float* dE = nullptr;
     if (out && out->E) {
         c.d_E.ensure(3 * m * K, "E");
         dE = c.d_E.get();
     }
-    reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), c.d_dadt_elm.get(), m, k, K, c.ground, dE, c.d_magn.get(), s);
+    reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), u.dadt_nodes1.get(), 1, m, k, K, c.ground, dE, c.d_magn.get(),
+                s);
     accumulate_moments(c.d_magn.get(), m, k, K, u.sum.get(), u.sumsq.get(), s);
     u.count += k;
     if (out) {
@@ -893,7 +891,6 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
     }
     // 2. dA/dt at nodes, element average, RHS.
     c.d_dadt_nodes.ensure(3 * n * K, "dadt nodes");
-    c.d_dadt_elm.ensure(3 * m * K, "dadt elements");
     c.d_q.ensure(4 * m * K, "corner terms");
     c.d_B.ensure(nr * K, "rhs block");
     const float scale = static_cast<float>(didt) * static_cast<float>(kMu0Over4Pi);
@@ -935,7 +932,13 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
         c.blk_mop.n_tissue = T;
         c.blk_mop.sigma = c.blk_sigma.get();
     }
-    element_pass(c.d_dadt_nodes.get(), c.tets.get(), c.g32.get(), c.neg_vc.get(), m, k, K, c.d_dadt_elm.get(),
+    // The element dA/dt is stored only when it is an output; the field pass rebuilds it from the nodes.
+    float* elm_out = nullptr;
+    if (out && out->dadt_elm) {
+        c.d_dadt_elm.ensure(3 * m * K, "dadt elements");
+        elm_out = c.d_dadt_elm.get();
+    }
+    element_pass(c.d_dadt_nodes.get(), c.tets.get(), c.g32.get(), c.neg_vc.get(), m, k, K, elm_out,
                  c.d_q.get(), s, c.vols32.get(), c.tags.get(), per_sigma ? c.blk_sigma_tag.get() : nullptr,
                  per_sigma ? sopt->n_sigma : 0);
     float* rhs_full = nullptr;
@@ -1012,8 +1015,8 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
         dE = c.d_E.get();
     }
     if (rec == nullptr) {
-        reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), c.d_dadt_elm.get(), m, k, K, c.ground, dE, c.d_magn.get(),
-                    s);
+        reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), c.d_dadt_nodes.get(), K, m, k, K, c.ground, dE,
+                    c.d_magn.get(), s);
     } else if (rec->mode == kRecoveryHarmonic) {
         // The potential and the exact nodal dA/dt feed the fits; |E| and E describe the result.
         c.d_slots.ensure(3 * rec->n_slots * K, "recovered slots");
@@ -1023,8 +1026,8 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
         // The raw field feeds the patch fits; |E| and E then describe the recovered field.
         c.d_Eraw.ensure(3 * m * K, "raw E");
         c.d_slots.ensure(3 * rec->n_slots * K, "recovered slots");
-        reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), c.d_dadt_elm.get(), m, k, K, c.ground, c.d_Eraw.get(),
-                    c.d_magn.get(), s);
+        reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), c.d_dadt_nodes.get(), K, m, k, K, c.ground,
+                    c.d_Eraw.get(), c.d_magn.get(), s);
         apply_spr(*rec, c.d_Eraw.get(), m, k, K, c.d_slots.get(), dE, c.d_magn.get(), s);
     }
     c.mark_phase(6, s);
@@ -1501,6 +1504,17 @@ int32_t cn_bench_vcycle(cn_context* c, int32_t K, int32_t reps, double* ms) {
     return guarded([&] {
         cn::require(c != nullptr && reps >= 1, "bad argument");
         *ms = c->solver->bench_vcycle(K, reps);
+    });
+}
+
+int32_t cn_bench_guess(cn_context* c, int32_t K, int32_t reps, double* ms3) {
+    return guarded([&] {
+        cn::require(c != nullptr && ms3 != nullptr && reps >= 1, "bad argument");
+        cn::require(K >= 1 && K <= 16 && cn::padded_width(K) == K, "K must be a compiled block width up to 16");
+        cn::require(c->d_B.size() >= static_cast<std::size_t>((c->n - 1) * K),
+                    "solve a block of this width first (its right-hand sides are the projected block)");
+        c->d_x0.ensure((c->n - 1) * K, "x0");
+        c->warm.bench(c->d_B.get(), K, K, c->d_x0.get(), reps, ms3, c->s);
     });
 }
 

@@ -341,7 +341,8 @@ __global__ void element_pass_kernel(const float* __restrict__ nodal, const std::
             d[a] = 0.25f * (((v0 + v1) + v2) + v3);
         }
     }
-    for (int a = 0; a < 3; ++a) elm[(e * 3 + a) * K + c] = d[a];
+    if (elm != nullptr)
+        for (int a = 0; a < 3; ++a) elm[(e * 3 + a) * K + c] = d[a];
     const float w = sigma_tag != nullptr ? -(vols[e] * sigma_tag[c * n_sigma + tags[e]]) : neg_vc[e];
     for (int i = 0; i < 4; ++i) {
         const float* gc = g + (4 * e + i) * 3;

@@ -30,8 +30,8 @@ void node_rhs(const float* q, const std::int32_t* ptr, const std::int32_t* idx, 
 
 // post.hip
 void scatter_potential(const double* X, std::int64_t n, int k, int K, int ground, double* v, hipStream_t s);
-void reconstruct(const double* X, const std::int32_t* tets, const float* g, const float* elm, std::int64_t m, int k,
-                 int K, int ground, float* E, float* magn, hipStream_t s);
+void reconstruct(const double* X, const std::int32_t* tets, const float* g, const float* nodal, int Kn,
+                 std::int64_t m, int k, int K, int ground, float* E, float* magn, hipStream_t s);
 // (rows, inner, K) -> (k, rows, inner)
 void unpack(const float* in, std::int64_t rows, int inner, int k, int K, float* out, hipStream_t s);
 

@@ -23,10 +23,13 @@ __global__ void scatter_kernel(const double* __restrict__ X, std::int64_t n, int
 }
 
 // reconstruct.cu:36-78: E = -sum_i v_i g_i - dadt_elm with the gradient in FP64, |E| per element.
-// Threads over (element, placement), placements fastest; E (m, 3, K), |E| (m, K).
+// Threads over (element, placement), placements fastest; E (m, 3, K), |E| (m, K). The element
+// dA/dt is the corner average of the nodal dA/dt (n, 3, Kn), formed here exactly as the element
+// pass forms it (dadt_element.cu); Kn = 1 serves every column from one placement.
 __global__ void reconstruct_kernel(const double* __restrict__ X, const std::int32_t* __restrict__ tets,
-                                   const float* __restrict__ g, const float* __restrict__ elm, std::int64_t m, int k,
-                                   int K, int ground, float* __restrict__ E, float* __restrict__ magn) {
+                                   const float* __restrict__ g, const float* __restrict__ nodal, int Kn,
+                                   std::int64_t m, int k, int K, int ground, float* __restrict__ E,
+                                   float* __restrict__ magn) {
     const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= m * K) return;
     const std::int64_t e = t / K;
@@ -42,9 +45,17 @@ __global__ void reconstruct_kernel(const double* __restrict__ X, const std::int3
         gy = fma(vi, static_cast<double>(ge[3 * i + 1]), gy);
         gz = fma(vi, static_cast<double>(ge[3 * i + 2]), gz);
     }
-    const double ex = -gx - static_cast<double>(elm[(e * 3) * K + c]);
-    const double ey = -gy - static_cast<double>(elm[(e * 3 + 1) * K + c]);
-    const double ez = -gz - static_cast<double>(elm[(e * 3 + 2) * K + c]);
+    const int cn = Kn == 1 ? 0 : c;
+    const std::int64_t stride = 3 * static_cast<std::int64_t>(Kn);
+    float d[3];
+    for (int a = 0; a < 3; ++a) {
+        const float v0 = nodal[tt.x * stride + a * Kn + cn], v1 = nodal[tt.y * stride + a * Kn + cn];
+        const float v2 = nodal[tt.z * stride + a * Kn + cn], v3 = nodal[tt.w * stride + a * Kn + cn];
+        d[a] = 0.25f * (((v0 + v1) + v2) + v3);
+    }
+    const double ex = -gx - static_cast<double>(d[0]);
+    const double ey = -gy - static_cast<double>(d[1]);
+    const double ez = -gz - static_cast<double>(d[2]);
     if (E != nullptr) {
         E[(e * 3) * K + c] = static_cast<float>(ex);
         E[(e * 3 + 1) * K + c] = static_cast<float>(ey);
@@ -321,10 +332,11 @@ void scatter_potential(const double* X, std::int64_t n, int k, int K, int ground
     if (const unsigned g = grid_for(n * k)) scatter_kernel<<<g, kBlock, 0, s>>>(X, n, k, K, ground, v);
 }
 
-void reconstruct(const double* X, const std::int32_t* tets, const float* g, const float* elm, std::int64_t m, int k,
-                 int K, int ground, float* E, float* magn, hipStream_t s) {
+void reconstruct(const double* X, const std::int32_t* tets, const float* g, const float* nodal, int Kn,
+                 std::int64_t m, int k, int K, int ground, float* E, float* magn, hipStream_t s) {
+    require(Kn == 1 || Kn == K, "nodal dA/dt must have one column or the block's width");
     if (const unsigned gr = grid_for(m * K))
-        reconstruct_kernel<<<gr, kBlock, 0, s>>>(X, tets, g, elm, m, k, K, ground, E, magn);
+        reconstruct_kernel<<<gr, kBlock, 0, s>>>(X, tets, g, nodal, Kn, m, k, K, ground, E, magn);
 }
 
 void unpack(const float* in, std::int64_t rows, int inner, int k, int K, float* out, hipStream_t s) {

@@ -1490,8 +1490,13 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
         // Bounded by the launches the host issues, not only by the device counter.
         int launched = total_iters;
         bool done = false;
+        const int hint = round == 0 ? last_first_round_[slot] : 0;
+        int reads = 0;
         while (launched < opt.max_iters) {
-            const int batch = std::min(opt.check_every, opt.max_iters - launched);
+            int want = opt.check_every;
+            if (hint > 0) want = reads == 0 ? std::max(1, hint - 2) : (reads <= 4 ? 1 : opt.check_every);
+            const int batch = std::min(want, opt.max_iters - launched);
+            ++reads;
             for (int b = 0; b < batch; ++b) {
                 if (opt.use_graph)
                     check(hipGraphLaunch(graphs_[slot], s_), "graph launch");
@@ -1508,6 +1513,7 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
             }
         }
         if (!done) total_iters = launched;
+        if (round == 0) last_first_round_[slot] = total_iters - host_state[1];
         for (int c = 0; c < K; ++c)
             if (host_state[2 + c] == 0 && host_state_[2 + K + c] > 0) col_iters[c] = host_state_[2 + K + c];
         stats.vcycles += total_iters - host_state[1] + 1;

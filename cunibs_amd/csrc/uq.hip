@@ -48,11 +48,6 @@ __global__ void combine_kernel(const T_IN* __restrict__ in, std::int64_t count, 
     out[t] = static_cast<float>(acc);
 }
 
-__global__ void broadcast_kernel(const float* __restrict__ in, std::int64_t count, int K, float* __restrict__ out) {
-    const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t < count * K) out[t] = in[t / K];
-}
-
 __global__ void moments_kernel(const float* __restrict__ x, std::int64_t m, int k, int K, double* __restrict__ sum,
                                double* __restrict__ sumsq) {
     const std::int64_t e = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -188,10 +183,6 @@ void combine_tissues(const double* in, std::int64_t count, int T, const double* 
 void combine_tissues(const float* in, std::int64_t count, int T, const double* sigma, int k, int K, float* out,
                      hipStream_t s) {
     if (const unsigned g = grid_for(count * K)) combine_kernel<float><<<g, kBlock, 0, s>>>(in, count, T, sigma, k, K, out);
-}
-
-void broadcast_columns(const float* in, std::int64_t rows_inner, int K, float* out, hipStream_t s) {
-    if (const unsigned g = grid_for(rows_inner * K)) broadcast_kernel<<<g, kBlock, 0, s>>>(in, rows_inner, K, out);
 }
 
 void accumulate_moments(const float* x, std::int64_t m, int k, int K, double* sum, double* sumsq, hipStream_t s) {

@@ -16,8 +16,6 @@ void combine_tissues(const double* in, std::int64_t count, int T, const double* 
                      hipStream_t s);
 void combine_tissues(const float* in, std::int64_t count, int T, const double* sigma, int k, int K, float* out,
                      hipStream_t s);
-// (rows, inner, 1) -> (rows, inner, K), every column a copy.
-void broadcast_columns(const float* in, std::int64_t rows_inner, int K, float* out, hipStream_t s);
 // sum[e] += x[e, c], sumsq[e] += x[e, c]^2 over c < k, FP64, columns in order.
 void accumulate_moments(const float* x, std::int64_t m, int k, int K, double* sum, double* sumsq, hipStream_t s);
 

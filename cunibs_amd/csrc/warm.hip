@@ -71,39 +71,63 @@ __global__ void gather_modes_kernel(const unsigned short* __restrict__ modes, st
 
 typedef double doublex4_t __attribute__((ext_vector_type(4)));
 
-// Partials of Phi^T B on the FP64 matrix cores: v_mfma_f64_16x16x4f64 with A = a 16-mode x 4-row
-// block of Phi^T (lane l: mode 16 t + l % 16 of row r0 + l / 16, read as its exact FP32 hi + lo
-// value) and B = the 4 rows of the block (lane l: column l % 16), accumulated in FP64 per 16-mode
-// tile t; accumulator register v of lane l holds mode 16 t + 4 v + l / 16, column l % 16. The
-// block's waves fold into its partial in wave order: partials[(block * R16 + mode) * K + c].
+// n ushorts from p (n a compile-time count, p aligned to 2 n bytes up to 16) into v.
+template <int N>
+__device__ __forceinline__ void load_u16(const unsigned short* __restrict__ p, unsigned short (&v)[N]) {
+    if constexpr (N % 8 == 0) {
+#pragma unroll
+        for (int i = 0; i < N / 8; ++i) {
+            const uint4 w = reinterpret_cast<const uint4*>(p)[i];
+            __builtin_memcpy(&v[8 * i], &w, 16);
+        }
+    } else if constexpr (N % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < N / 4; ++i) {
+            const uint2 w = reinterpret_cast<const uint2*>(p)[i];
+            __builtin_memcpy(&v[4 * i], &w, 8);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = p[i];
+    }
+}
+
+// Partials of Phi^T B on the FP64 matrix cores, T = R16 / 16 tiles of 16 modes: v_mfma_f64_16x16x4f64
+// with A = 16 modes x 4 rows of Phi^T and B = the 4 rows of the block (lane l: column l % 16 of
+// row r0 + l / 16), accumulated in FP64 per tile. Tile t, tile row i is mode i T + t, so lane l
+// reads the T consecutive modes i T .. i T + T - 1 (i = l % 16) of its row in 16-byte loads, and
+// each value enters as its exact FP32 hi + lo sum. Accumulator register v of lane l holds tile row
+// 4 v + l / 16, column l % 16. The block's waves fold into its partial in wave order:
+// partials[(block * R16 + mode) * K + c].
 constexpr int kProjectThreads = 512;
 
+template <int T>
 __global__ void __launch_bounds__(kProjectThreads) project_kernel(const unsigned short* __restrict__ modes,
-                                                                  std::int64_t n, int R16, const float* __restrict__ B,
-                                                                  int K, std::int64_t rows_per_block,
+                                                                  std::int64_t n, const float* __restrict__ B, int K,
+                                                                  std::int64_t rows_per_block,
                                                                   double* __restrict__ partials) {
+    constexpr int R16 = 16 * T;
     constexpr int kWaves = kProjectThreads / 64;
-    const int T = R16 / 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, quad = lane >> 4;
     const std::int64_t lo = static_cast<std::int64_t>(blockIdx.x) * rows_per_block;
     const std::int64_t hi = min(lo + rows_per_block, n);
     const std::int64_t half = n * R16;
-    doublex4_t acc[16];
+    doublex4_t acc[T];
 #pragma unroll
-    for (int t = 0; t < 16; ++t) acc[t] = doublex4_t{0.0, 0.0, 0.0, 0.0};
+    for (int t = 0; t < T; ++t) acc[t] = doublex4_t{0.0, 0.0, 0.0, 0.0};
     for (std::int64_t r0 = lo + 4 * wave; r0 < hi; r0 += 4 * kWaves) {
         const std::int64_t row = r0 + quad;
         const bool ok = row < hi;
         const std::int64_t rr = ok ? row : hi - 1;  // clamped: the loads stay in bounds
         const double b = (ok && col < K) ? static_cast<double>(B[rr * K + col]) : 0.0;
-        const unsigned short* m = modes + rr * R16 + col;
+        unsigned short mh[T], ml[T];
+        load_u16<T>(modes + rr * R16 + col * T, mh);
+        load_u16<T>(modes + half + rr * R16 + col * T, ml);
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            if (t < T) {
-                const double a = static_cast<double>(bf16_value(m[16 * t]) + bf16_value(m[half + 16 * t]));
-                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
-            }
+        for (int t = 0; t < T; ++t) {
+            const double a = static_cast<double>(bf16_value(mh[t]) + bf16_value(ml[t]));
+            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
         }
     }
     __shared__ double red[kMaxWarmModes * 16];
@@ -112,12 +136,9 @@ __global__ void __launch_bounds__(kProjectThreads) project_kernel(const unsigned
     for (int w = 0; w < kWaves; ++w) {
         if (wave == w) {
 #pragma unroll
-            for (int t = 0; t < 16; ++t) {
-                if (t < T) {
+            for (int t = 0; t < T; ++t)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) red[(16 * t + 4 * v + quad) * 16 + col] += acc[t][v];
-                }
-            }
+                for (int v = 0; v < 4; ++v) red[((4 * v + quad) * T + t) * 16 + col] += acc[t][v];
         }
         __syncthreads();
     }
@@ -160,17 +181,19 @@ typedef __attribute__((__vector_size__(4 * sizeof(short)))) short bf16x4_t;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4_t;
 
 // x0 (n, K) FP64 = Phi c: one wavefront per 16-row tile, v_mfma_f32_16x16x16bf16_1k over the modes
-// in 16-column steps; hi.hi, hi.lo and lo.hi products accumulate into the same FP32 tile.
+// in T steps of 16; hi.hi, hi.lo and lo.hi products accumulate into the same FP32 tile.
 // Operand layout of the 16x16x16 BF16 MFMA (CDNA3/4 ISA): lane l supplies A[l % 16][4 (l / 16) + j]
-// and B[4 (l / 16) + j][l % 16], j = 0..3, and receives D[4 (l / 16) + j][l % 16]. Every step's
-// A fragment (4 consecutive modes of one row: one 8-byte load per half) is loaded before the first
-// product; the coefficients sit in LDS transposed, so a lane's B fragment is one 8-byte read.
+// and B[4 (l / 16) + j][l % 16], j = 0..3, and receives D[4 (l / 16) + j][l % 16]. At step t the
+// product index 4 q + j (q = l / 16) stands for mode 4 T q + 4 t + j, so lane l reads the 4 T
+// consecutive modes 4 T q .. of its row (16-byte loads) before the first product; the
+// coefficients sit in LDS transposed, so a lane's B fragment is one 8-byte read.
 constexpr int kExpandThreads = 256;
 
+template <int T>
 __global__ void __launch_bounds__(kExpandThreads) expand_kernel(const unsigned short* __restrict__ modes,
-                                                                std::int64_t n, int R16,
-                                                                const unsigned short* __restrict__ coef, int k, int K,
-                                                                double* __restrict__ x0) {
+                                                                std::int64_t n, const unsigned short* __restrict__ coef,
+                                                                int k, int K, double* __restrict__ x0) {
+    constexpr int R16 = 16 * T;
     __shared__ uint2 ct[2][16][kMaxWarmModes / 4];  // [half][column][mode / 4]: 4 modes' BF16 bits
     {
         unsigned short* flat = reinterpret_cast<unsigned short*>(ct);
@@ -187,31 +210,21 @@ __global__ void __launch_bounds__(kExpandThreads) expand_kernel(const unsigned s
     const int i = lane & 15, q = lane >> 4;
     const std::int64_t row = r0 + i;
     const std::int64_t rr = row < n ? row : n - 1;  // clamped: the loads stay in bounds
-    const int T = R16 / 16;
-    const uint2* hrow = reinterpret_cast<const uint2*>(modes + rr * R16);
-    const uint2* lrow = reinterpret_cast<const uint2*>(modes + n * R16 + rr * R16);
-    uint2 ah[16], al[16];
-#pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        if (t < T) {
-            ah[t] = hrow[4 * t + q];
-            al[t] = lrow[4 * t + q];
-        }
-    }
+    unsigned short ah[4 * T], al[4 * T];
+    load_u16<4 * T>(modes + rr * R16 + 4 * T * q, ah);
+    load_u16<4 * T>(modes + n * R16 + rr * R16 + 4 * T * q, al);
     floatx4_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        if (t < T) {
-            const uint2 bh2 = ct[0][i][4 * t + q], bl2 = ct[1][i][4 * t + q];
-            bf16x4_t a, alo, bh, bl;
-            __builtin_memcpy(&a, &ah[t], 8);
-            __builtin_memcpy(&alo, &al[t], 8);
-            __builtin_memcpy(&bh, &bh2, 8);
-            __builtin_memcpy(&bl, &bl2, 8);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bh, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bl, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo, bh, acc, 0, 0, 0);
-        }
+    for (int t = 0; t < T; ++t) {
+        const uint2 bh2 = ct[0][i][T * q + t], bl2 = ct[1][i][T * q + t];
+        bf16x4_t a, alo, bh, bl;
+        __builtin_memcpy(&a, &ah[4 * t], 8);
+        __builtin_memcpy(&alo, &al[4 * t], 8);
+        __builtin_memcpy(&bh, &bh2, 8);
+        __builtin_memcpy(&bl, &bl2, 8);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo, bh, acc, 0, 0, 0);
     }
     // acc[j] = x0[r0 + 4 q + j][i]
     if (i < K) {
@@ -220,6 +233,30 @@ __global__ void __launch_bounds__(kExpandThreads) expand_kernel(const unsigned s
             const std::int64_t r = r0 + 4 * q + j;
             if (r < n) x0[r * K + i] = (i < k) ? static_cast<double>(acc[j]) : 0.0;
         }
+    }
+}
+
+// Calls f with the tile count T = R16 / 16 as a compile-time constant.
+template <typename F>
+void dispatch_tiles(int R16, F&& f) {
+    switch (R16 / 16) {
+        case 1: f(std::integral_constant<int, 1>{}); break;
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 3: f(std::integral_constant<int, 3>{}); break;
+        case 4: f(std::integral_constant<int, 4>{}); break;
+        case 5: f(std::integral_constant<int, 5>{}); break;
+        case 6: f(std::integral_constant<int, 6>{}); break;
+        case 7: f(std::integral_constant<int, 7>{}); break;
+        case 8: f(std::integral_constant<int, 8>{}); break;
+        case 9: f(std::integral_constant<int, 9>{}); break;
+        case 10: f(std::integral_constant<int, 10>{}); break;
+        case 11: f(std::integral_constant<int, 11>{}); break;
+        case 12: f(std::integral_constant<int, 12>{}); break;
+        case 13: f(std::integral_constant<int, 13>{}); break;
+        case 14: f(std::integral_constant<int, 14>{}); break;
+        case 15: f(std::integral_constant<int, 15>{}); break;
+        case 16: f(std::integral_constant<int, 16>{}); break;
+        default: throw InvalidArgument("mode width must be a multiple of 16 in [16, 256]");
     }
 }
 
@@ -354,24 +391,64 @@ void WarmBasis::mode_chunk(int j0, float* out, hipStream_t s) const {
 void WarmBasis::expand(int k, int K, double* x0, hipStream_t s) {
     require(n_modes % 16 == 0 && n_modes <= kMaxWarmModes, "mode width must be a multiple of 16, at most 256");
     const std::int64_t tiles = (n + 15) / 16;
-    expand_kernel<<<grid_for(tiles, kExpandThreads / 64), kExpandThreads, 0, s>>>(modes.get(), n, n_modes, coef.get(),
-                                                                                 k, K, x0);
+    dispatch_tiles(n_modes, [&](auto tc) {
+        expand_kernel<decltype(tc)::value><<<grid_for(tiles, kExpandThreads / 64), kExpandThreads, 0, s>>>(
+            modes.get(), n, coef.get(), k, K, x0);
+    });
 }
 
-void WarmBasis::initial_guess(const float* B, int k, int K, double* x0, hipStream_t s) {
+void WarmBasis::project(const float* B, int K, hipStream_t s) {
     require(ready, "warm start requested but no modes are set");
-    require(K >= 1 && K <= 16 && k <= K, "warm start supports block widths up to 16");
+    require(K >= 1 && K <= 16, "warm start supports block widths up to 16");
     require(n_modes % 16 == 0 && n_modes <= kMaxWarmModes, "mode width must be a multiple of 16, at most 256");
     const int R = n_modes;
     const int nb = kProjectBlocks;
     const std::int64_t rpb = (n + nb - 1) / nb;
     partials.ensure(static_cast<std::size_t>(nb) * R * K + static_cast<std::size_t>(R) * K, "warm partials");
+    dispatch_tiles(R, [&](auto tc) {
+        project_kernel<decltype(tc)::value><<<nb, kProjectThreads, 0, s>>>(modes.get(), n, B, K, rpb, partials.get());
+    });
+}
+
+void WarmBasis::coefficients(int k, int K, hipStream_t s) {
+    require(k >= 1 && k <= K, "k must be in [1, K]");
+    const int R = n_modes;
+    const int nb = kProjectBlocks;
     double* const ptb = partials.get() + static_cast<std::size_t>(nb) * R * K;
     coef.ensure(static_cast<std::size_t>(2) * R * 16, "warm coefficients");
-    project_kernel<<<nb, kProjectThreads, 0, s>>>(modes.get(), n, R, B, K, rpb, partials.get());
     sum_blocks_kernel<<<(R * K + 3) / 4, 256, 0, s>>>(partials.get(), nb, R * K, ptb);
     coefficients_kernel<<<(R * 16 + 255) / 256, 256, 0, s>>>(ptb, R, K, P.get(), k, coef.get());
+}
+
+void WarmBasis::initial_guess(const float* B, int k, int K, double* x0, hipStream_t s) {
+    project(B, K, s);
+    coefficients(k, K, s);
     expand(k, K, x0, s);
+}
+
+void WarmBasis::bench(const float* B, int k, int K, double* x0, int reps, double* ms3, hipStream_t s) {
+    require(reps >= 1, "reps must be positive");
+    initial_guess(B, k, K, x0, s);  // untimed: buffers and caches
+    hipEvent_t ev[4];
+    for (auto& e : ev) check(hipEventCreate(&e), "event");
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int r = 0; r < reps; ++r) {
+        check(hipEventRecord(ev[0], s), "event record");
+        project(B, K, s);
+        check(hipEventRecord(ev[1], s), "event record");
+        coefficients(k, K, s);
+        check(hipEventRecord(ev[2], s), "event record");
+        expand(k, K, x0, s);
+        check(hipEventRecord(ev[3], s), "event record");
+        check(hipEventSynchronize(ev[3]), "event sync");
+        for (int i = 0; i < 3; ++i) {
+            float ms = 0.f;
+            check(hipEventElapsedTime(&ms, ev[i], ev[i + 1]), "elapsed");
+            acc[i] += ms;
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (int i = 0; i < 3; ++i) ms3[i] = acc[i] / reps;
 }
 
 void WarmBasis::guess_from_coefficients(const double* c_host, int k, int K, double* x0, hipStream_t s) {

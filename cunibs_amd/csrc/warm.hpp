@@ -33,8 +33,13 @@ struct WarmBasis {
     void mode_chunk(int j0, float* out, hipStream_t s) const;
     // P (R x R) FP64 host: the inverse of Phi^T A Phi.
     void set_projector(const double* P_host, int R, hipStream_t s);
-    // x0 (n, K) FP64 = Phi P Phi^T B for the (n, K) FP32 block B (columns >= k of x0 are zero).
+    // x0 (n, K) FP64 = Phi P Phi^T B for the (n, K) FP32 block B (columns >= k of x0 are zero):
+    // project (the block partials of Phi^T B), coefficients (c = P Phi^T B), expand (x0 = Phi c).
     void initial_guess(const float* B, int k, int K, double* x0, hipStream_t s);
+    void project(const float* B, int K, hipStream_t s);
+    void coefficients(int k, int K, hipStream_t s);
+    // Average ms of the three stages of initial_guess over `reps` timed runs (HIP events).
+    void bench(const float* B, int k, int K, double* x0, int reps, double* ms3, hipStream_t s);
     // x0 = Phi c for caller-supplied coefficients c (k x R) FP64 host.
     void guess_from_coefficients(const double* c_host, int k, int K, double* x0, hipStream_t s);
     // x0 = Phi c from the coefficients already in `coef`.

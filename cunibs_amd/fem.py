@@ -499,6 +499,13 @@ class SolverContext:
         return {"samples": int(out[0]), "spmm_ms": float(out[1]), "vcycle_ms": float(out[2]),
                 "iteration_ms": float(out[3]), "dadt_placements": int(out[4]), "dadt_ms": float(out[5])}
 
+    def bench_guess(self, K: int = 16, reps: int = 5) -> dict:
+        """Average ms of the warm-start guess's projection, coefficients and expansion at width K,
+        on the right-hand sides of the last block solved at that width."""
+        ms = np.zeros(3)
+        check(lib().cn_bench_guess(self.handle, K, reps, ptr(ms)), "cn_bench_guess")
+        return {"project_ms": float(ms[0]), "coefficients_ms": float(ms[1]), "expand_ms": float(ms[2])}
+
     PHASES = ("frames", "dadt", "rhs", "guess", "pcg", "field", "outputs")
 
     def timing_phases(self) -> dict:

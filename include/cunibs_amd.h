@@ -335,6 +335,11 @@ int32_t cn_timing_read(const cn_context* ctx, double* out6);
  * reconstruction and recovery, out[6] outputs and region metrics; out[7] blocks, out[8]
  * placements. A measurement aid with no counterpart in the reference. */
 int32_t cn_timing_phases(const cn_context* ctx, double* out9);
+/* Average device time (ms) of the warm-start guess for a block of width K (<= 16) on the right-hand
+ * sides of the last block solved at that width: ms3[0] the projection's block partials of
+ * Phi^T B, ms3[1] their reduction and c = P Phi^T B, ms3[2] the expansion x0 = Phi c. Needs modes
+ * (cn_warm_set_modes). A measurement aid with no counterpart in the reference. */
+int32_t cn_bench_guess(cn_context* ctx, int32_t K, int32_t reps, double* ms3);
 /* Average device time (ms) of the dA/dt N-body for k copies of one placement over the context's
  * nodes, repeated launches. A measurement aid with no counterpart in the reference. */
 int32_t cn_bench_dadt(cn_context* ctx, const double* dip_pos_m, const double* dip_moment, int64_t n_dip,
