@@ -170,11 +170,17 @@ constexpr int kDipTile = 256;
 constexpr int kNodesPerThread = 4;
 
 // dadt.cu:27-104 restated for CDNA: dipole tiles staged in LDS, 4 nodes per thread in registers,
-// one placement per blockIdx.y. w_j = |r - s_j|^-3 from rn + sn - 2 r.s; rsqrtf, FP32, on the
-// vector ALUs. (A matrix-core version, the distance and accumulation sums as 16x16x4 FP32 MFMAs
-// with rsqrt on the vector ALUs, measured 1.35 ms per placement at 1.0M nodes x 2400 dipoles
-// against 0.89 ms for this kernel, and was dropped.)
+// one placement per blockIdx.y. w_j = |r - s_j|^-3 from rn + sn - 2 r.s, FP32, on the vector ALUs.
+// The reciprocal square root is the bare v_rsq_f32 (rsq_normal): rsqrtf wraps the same instruction
+// in a denormal rescale (compare, two multiplies, two selects per pair), and a squared coil-to-node
+// distance is never denormal, so the results are the same bits with a third fewer vector
+// instructions per pair. (Matrix-core versions, with the distances as 16x16x4 FP32 MFMAs and the
+// sums either as MFMAs or on the vector ALUs, measured 0.82-1.35 ms per placement at 1.0M nodes x
+// 2400 dipoles against 0.78 ms for this kernel with rsqrtf, and were dropped.)
 // Output (n, 3, K): the K placements of a node component are contiguous.
+
+// v_rsq_f32 without the denormal rescale; for normal positive inputs it equals rsqrtf bit for bit.
+__device__ __forceinline__ float rsq_normal(float x) { return __builtin_amdgcn_rsqf(x); }
 __global__ void __launch_bounds__(kDadtBlock) dadt_kernel(const float* __restrict__ dip, int n_dip,
                                                           const double* __restrict__ nodes_mm, std::int64_t n,
                                                           const double* __restrict__ origin, float scale, int K,
@@ -223,7 +229,7 @@ __global__ void __launch_bounds__(kDadtBlock) dadt_kernel(const float* __restric
             for (int k = 0; k < kNodesPerThread; ++k) {
                 const float rs = fmaf(rz[k], sz, fmaf(ry[k], sy, rx[k] * sx));
                 const float d2 = fmaf(-2.f, rs, rn[k] + snj);
-                const float inv = rsqrtf(d2);
+                const float inv = rsq_normal(d2);
                 const float w = inv * inv * inv;
                 wx[k] = fmaf(w, m0, wx[k]);
                 wy[k] = fmaf(w, m1, wy[k]);
@@ -292,7 +298,7 @@ __global__ void __launch_bounds__(kDadtBlock) dadt_parallel_kernel(const float* 
 #pragma unroll
             for (int k = 0; k < kNodesPerThread; ++k) {
                 const float q = fmaf(rz[k], a.z, fmaf(ry[k], a.y, fmaf(rx[k], a.x, a.w)));
-                const float inv = rsqrtf(q + rn[k]);
+                const float inv = rsq_normal(q + rn[k]);
                 const float w = inv * inv * inv;
                 s0[k] = fmaf(w, m.x, s0[k]);
                 sx[k] = fmaf(w, m.y, sx[k]);
