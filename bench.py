@@ -250,7 +250,8 @@ def main() -> int:
 
         tune = autotune.autotune(ctx, pos, mom, synthetic.cap_sites(32, seed=13), 1e6, block_ks=(4, 8, 16))
         k = tune.block_k
-        log(f"autotune {tune.seconds:.2f}s: block_k {k}, degree {tune.degree}, fine_sub {tune.fine_sub}; "
+        log(f"autotune {tune.seconds:.2f}s: block_k {k}, degree {tune.degree}, fine_sub {tune.fine_sub}, "
+            f"coarse_sub {tune.coarse_sub} (coarse tile us {tune.coarse_tile_us}); "
             f"tile us {tune.tile_us}, degree rates {tune.degree_rates}, width rates {tune.width_rates}")
     ctx.timing_enable(True)  # reset the in-solve sums
     cold_elapsed, cold_iters, cold_block_iters = sweep(False)
@@ -300,7 +301,7 @@ def main() -> int:
     plain = float("nan")
     if args.plain > 0:
         tuned = ctx.tuning
-        ctx.set_tuning(degree=2, lower_ratio=12.0, fine_sub=0)
+        ctx.set_tuning(degree=2, lower_ratio=12.0, fine_sub=0, coarse_sub=0)
         plain_iters = []
         block(0, 1)
         t0 = time.perf_counter()

@@ -55,7 +55,7 @@ class Info(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("degree", _I32), ("lower_ratio", _F), ("fine_sub", _I32)]
+    _fields_ = [("degree", _I32), ("lower_ratio", _F), ("fine_sub", _I32), ("coarse_sub", _I32)]
 
 
 class SolveOptions(ctypes.Structure):

@@ -6,7 +6,8 @@ Chebyshev degree 2 with ratio 12 (fem/solve.py:153-168) and the V-cycle's thread
 
 * the tile (sub-lanes per 4-column group of the fine level's row kernels) changes only the speed
   of one iteration, so it is chosen from kernel timings: the differenced SpMM with its Gram pass
-  plus one V-cycle, HIP-event timed on the library's stream;
+  plus one V-cycle, HIP-event timed on the library's stream; the coarser levels' tile likewise,
+  from V-cycle timings;
 * the degree trades smoother cost against iterations, and the block width trades the operator
   stream's reuse and the faster convergence of wide blocks against their K^2 Gram work, so both
   are chosen by timing whole blocks over the same placements.
@@ -29,15 +30,18 @@ class TuningResult:
     block_k: int
     degree: int
     fine_sub: int
+    coarse_sub: int = 0
     seconds: float = 0.0
     tile_us: dict = field(default_factory=dict)          # fine_sub -> us per iteration's fine work
+    coarse_tile_us: dict = field(default_factory=dict)   # coarse_sub -> us per V-cycle
     degree_rates: dict = field(default_factory=dict)     # degree -> placements/s
     width_rates: dict = field(default_factory=dict)      # block_k -> placements/s
 
     def as_dict(self) -> dict:
         return {"block_k": self.block_k, "degree": self.degree, "fine_sub": self.fine_sub,
-                "seconds": round(self.seconds, 3),
+                "coarse_sub": self.coarse_sub, "seconds": round(self.seconds, 3),
                 "tile_us": {str(k): round(v, 2) for k, v in self.tile_us.items()},
+                "coarse_tile_us": {str(k): round(v, 2) for k, v in self.coarse_tile_us.items()},
                 "degree_rates": {str(k): round(v, 2) for k, v in self.degree_rates.items()},
                 "width_rates": {str(k): round(v, 2) for k, v in self.width_rates.items()}}
 
@@ -56,9 +60,10 @@ def _rate(ctx, pos, mom, sites, didt, k: int, warm_start: bool) -> float:
 
 def autotune(ctx: fem.SolverContext, dip_pos_m, dip_moment, sites: Sequence, didt: float = 1e6, *,
              block_ks: Sequence[int] = (4, 8, 16, 32), degrees: Sequence[int] = (2, 3, 4, 5, 6),
-             subs: Sequence[int] = (0, 1, 2, 4, 8), base_k: int = 8, warm_start: bool = False) -> TuningResult:
-    """Tune on `sites` (their count should be a multiple of every width, e.g. 32): the tile at
-    width `base_k`, then the degree at `base_k`, then the width. Widths above what a warm start
+             subs: Sequence[int] = (0, 1, 2, 4, 8), coarse_subs: Sequence[int] = (0, 2, 4, 8, 16),
+             base_k: int = 8, warm_start: bool = False) -> TuningResult:
+    """Tune on `sites` (their count should be a multiple of every width, e.g. 32): the fine and the
+    coarse tiles at width `base_k`, then the degree at `base_k`, then the width. Widths above what a warm start
     supports (16) are skipped when `warm_start` is set."""
     t_start = time.perf_counter()
     widths = [k for k in block_ks if 1 <= k <= fem.MAX_BLOCK and k <= len(sites) and not (warm_start and k > 16)]
@@ -78,6 +83,11 @@ def autotune(ctx: fem.SolverContext, dip_pos_m, dip_moment, sites: Sequence, did
             continue  # more sub-lanes than a wavefront holds at this width
     res.fine_sub = min(res.tile_us, key=res.tile_us.get)
     ctx.set_tuning(fine_sub=res.fine_sub)
+    for sub in coarse_subs:
+        ctx.set_tuning(coarse_sub=sub)
+        res.coarse_tile_us[sub] = 1e3 * ctx.bench_vcycle(K, 10)
+    res.coarse_sub = min(res.coarse_tile_us, key=res.coarse_tile_us.get)
+    ctx.set_tuning(coarse_sub=res.coarse_sub)
 
     # 2. degree: whole blocks at the base width
     for d in degrees:

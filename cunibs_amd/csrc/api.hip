@@ -1157,6 +1157,7 @@ int32_t cn_context_get_tuning(const cn_context* c, cn_tuning* o) {
         o->degree = c->hier.cheb.degree;
         o->lower_ratio = c->opts.lower_ratio;
         o->fine_sub = c->solver->fine_sub_setting();
+        o->coarse_sub = c->solver->coarse_sub_setting();
     });
 }
 
@@ -1167,11 +1168,14 @@ int32_t cn_context_set_tuning(cn_context* c, const cn_tuning* t) {
         cn::require(t->lower_ratio > 1.0f, "lower_ratio must exceed 1");
         cn::require(t->fine_sub >= 0 && t->fine_sub <= 64 && (t->fine_sub & (t->fine_sub - 1)) == 0,
                     "fine_sub must be 0 or a power of two <= 64");
+        cn::require(t->coarse_sub >= 0 && t->coarse_sub <= 64 && (t->coarse_sub & (t->coarse_sub - 1)) == 0,
+                    "coarse_sub must be 0 or a power of two <= 64");
         cn::check(hipStreamSynchronize(c->s), "sync before retuning");
         c->hier.cheb = cn::chebyshev(t->degree, t->lower_ratio);
         c->opts.degree = t->degree;
         c->opts.lower_ratio = t->lower_ratio;
         c->solver->set_fine_sub(t->fine_sub);
+        c->solver->set_coarse_sub(t->coarse_sub);
         c->solver->invalidate();
     });
 }

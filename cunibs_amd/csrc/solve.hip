@@ -1349,6 +1349,12 @@ void BlockSolver::set_fine_sub(int sub) {
     drop_graphs();
 }
 
+void BlockSolver::set_coarse_sub(int sub) {
+    require(sub >= 0 && sub <= kWave && (sub & (sub - 1)) == 0, "sub must be 0 or a power of two <= 64");
+    coarse_sub_ = sub;
+    drop_graphs();
+}
+
 template <int K>
 void BlockSolver::vcycle_k(const float* B, float* X, const int* state) {
     // Lanes per column of each level's row kernels: a property of the level and the width alone,
@@ -1357,6 +1363,8 @@ void BlockSolver::vcycle_k(const float* B, float* X, const int* state) {
     for (std::size_t i = 0; i < h_.levels.size(); ++i)
         subs[i] = choose_sub(h_.levels[i].a.n, h_.levels[i].a.nnz, col_groups<K>(), n_cu_);
     if (!subs.empty() && fine_sub_override_ > 0) subs[0] = std::min(fine_sub_override_, kWave / col_groups<K>());
+    if (coarse_sub_ > 0)
+        for (std::size_t i = 1; i < subs.size(); ++i) subs[i] = std::min(coarse_sub_, kWave / col_groups<K>());
     run_vcycle<K>(h_, subs, lb_, lx_, lr_, cb_.get(), cx_.get(), B, X, state, n_cu_, s_);
 }
 

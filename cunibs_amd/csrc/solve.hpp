@@ -101,6 +101,8 @@ class BlockSolver {
     // the level's shape. Drops the captured iterations.
     void set_fine_sub(int sub);
     int fine_sub_setting() const { return fine_sub_set_; }
+    void set_coarse_sub(int sub);
+    int coarse_sub_setting() const { return coarse_sub_; }
     // Forget every captured iteration (after a change to the hierarchy's smoother).
     void invalidate() { drop_graphs(); }
     // In-solve timing on the device wall clock: with it on, the differenced SpMM's blocks stamp
@@ -132,6 +134,7 @@ class BlockSolver {
     int n_cu_ = 256;
     int fine_sub_set_ = 0;       // set_fine_sub's value (0: automatic)
     int fine_sub_override_ = 0;  // the fine level's sub-lane count in force (bench_spmm may force one)
+    int coarse_sub_ = 0;         // sub-lanes of the coarser levels' row kernels (0: automatic)
     const MultiOperator* multi_ = nullptr;
     // block vectors (n, K)
     DeviceBuffer<float> R_, Z_, W_, S_;

@@ -463,18 +463,21 @@ class SolverContext:
 
     @property
     def tuning(self) -> dict:
-        """Smoother degree, Chebyshev ratio and fine-level SpMM tile (sub-lanes, 0 = automatic)."""
+        """Smoother degree, Chebyshev ratio, fine-level SpMM tile and the coarser levels' tile
+        (sub-lanes, 0 = automatic)."""
         t = _lib.Tuning()
         check(lib().cn_context_get_tuning(self.handle, ctypes.byref(t)), "cn_context_get_tuning")
-        return {"degree": int(t.degree), "lower_ratio": float(t.lower_ratio), "fine_sub": int(t.fine_sub)}
+        return {"degree": int(t.degree), "lower_ratio": float(t.lower_ratio), "fine_sub": int(t.fine_sub),
+                "coarse_sub": int(t.coarse_sub)}
 
     def set_tuning(self, degree: int | None = None, lower_ratio: float | None = None,
-                   fine_sub: int | None = None) -> None:
+                   fine_sub: int | None = None, coarse_sub: int | None = None) -> None:
         """Change the settings of `tuning` between solves (None keeps the current value)."""
         cur = self.tuning
         t = _lib.Tuning(cur["degree"] if degree is None else int(degree),
                         cur["lower_ratio"] if lower_ratio is None else float(lower_ratio),
-                        cur["fine_sub"] if fine_sub is None else int(fine_sub))
+                        cur["fine_sub"] if fine_sub is None else int(fine_sub),
+                        cur["coarse_sub"] if coarse_sub is None else int(coarse_sub))
         check(lib().cn_context_set_tuning(self.handle, ctypes.byref(t)), "cn_context_set_tuning")
 
     def bench_spmm(self, K: int, reps: int = 20, mode: int = 1, sub: int = 0) -> float:

@@ -84,6 +84,8 @@ typedef struct {
     int32_t fine_sub;   /* lanes per 4-column group of the fine level's row kernels (the SpMM
                            tile): 0 = chosen from the level's shape, else a power of two <= 64,
                            clamped to one wavefront per row at wide blocks */
+    int32_t coarse_sub; /* the same for the smoothing and residual kernels of every coarser
+                           level (0 = chosen from each level's shape) */
 } cn_tuning;
 
 int32_t cn_context_get_tuning(const cn_context* ctx, cn_tuning* out);

@@ -35,7 +35,7 @@ def tctx(head4):
 
 def test_default_tuning_is_the_reference_setting(tctx):
     t = tctx.tuning
-    assert t == {"degree": 2, "lower_ratio": pytest.approx(12.0), "fine_sub": 0}
+    assert t == {"degree": 2, "lower_ratio": pytest.approx(12.0), "fine_sub": 0, "coarse_sub": 0}
 
 
 @pytest.mark.parametrize("degree", [3, 6])
@@ -69,6 +69,19 @@ def test_products_at_any_tile_match_the_oracle(tctx, octx, oracle_mod, sub):
         tctx.set_tuning(fine_sub=0)
 
 
+@pytest.mark.parametrize("sub", [1, 4, 16])
+def test_vcycle_at_any_coarse_tile_matches_the_oracle(tctx, octx, oracle_mod, sub):
+    """The coarser levels' tile changes only how the row kernels split a row's entries."""
+    X = np.random.default_rng(50 + sub).standard_normal((octx.reduced.n, 8)).astype(np.float32)
+    vref = np.stack([oracle_mod.vcycle_apply(octx.hier, X[:, c].astype(np.float64)) for c in range(8)], 1)
+    tctx.set_tuning(coarse_sub=sub)
+    try:
+        assert tctx.tuning["coarse_sub"] == sub
+        assert rel(tctx.vcycle(X), vref) < 1e-5
+    finally:
+        tctx.set_tuning(coarse_sub=0)
+
+
 def test_bad_tuning_is_rejected(tctx):
     with pytest.raises(ValueError):
         tctx.set_tuning(degree=7)
@@ -76,6 +89,8 @@ def test_bad_tuning_is_rejected(tctx):
         tctx.set_tuning(fine_sub=3)
     with pytest.raises(ValueError):
         tctx.set_tuning(lower_ratio=1.0)
+    with pytest.raises(ValueError):
+        tctx.set_tuning(coarse_sub=6)
     assert tctx.tuning["degree"] == 2
 
 
@@ -85,9 +100,11 @@ def test_autotuned_context_gives_the_same_field(tctx, octx, oracle_mod, coil):
     from cunibs_amd import autotune, fem, synthetic
     sites = synthetic.cap_sites(8, seed=11)
     res = autotune.autotune(tctx, coil.positions_m, coil.moments, sites, block_ks=(2, 4, 8), degrees=(2, 3),
-                            subs=(0, 2), base_k=4)
+                            subs=(0, 2), coarse_subs=(0, 8), base_k=4)
     try:
         assert res.block_k in (2, 4, 8) and res.degree in (2, 3) and res.fine_sub in (0, 2)
+        assert res.coarse_sub in (0, 8) and set(res.coarse_tile_us) == {0, 8}
+        assert tctx.tuning["coarse_sub"] == res.coarse_sub
         assert set(res.width_rates) == {2, 4, 8} and set(res.degree_rates) == {2, 3}
         assert all(v > 0 for v in res.width_rates.values())
         assert tctx.tuning["degree"] == res.degree and tctx.tuning["fine_sub"] == res.fine_sub
@@ -97,7 +114,7 @@ def test_autotuned_context_gives_the_same_field(tctx, octx, oracle_mod, coil):
         assert rel(out[0].magnE[gm], ref["magnE"][gm]) < 1e-4
         assert all(r.relative_residual <= 1e-6 for r in out)
     finally:
-        tctx.set_tuning(degree=2, fine_sub=0)
+        tctx.set_tuning(degree=2, fine_sub=0, coarse_sub=0)
 
 
 def test_in_solve_timing_leaves_the_answer_bitwise_unchanged(tctx, coil):
