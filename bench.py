@@ -248,9 +248,11 @@ def main() -> int:
     if args.autotune:
         from cunibs_amd import autotune
 
-        tune = autotune.autotune(ctx, pos, mom, synthetic.cap_sites(32, seed=13), 1e6, block_ks=(4, 8, 16))
+        tune = autotune.autotune(ctx, pos, mom, synthetic.cap_sites(32, seed=13), 1e6, block_ks=(4, 8, 16),
+                                 ratios=(8.0, 12.0, 20.0, 30.0))
         k = tune.block_k
-        log(f"autotune {tune.seconds:.2f}s: block_k {k}, degree {tune.degree}, fine_sub {tune.fine_sub}, "
+        log(f"autotune {tune.seconds:.2f}s: block_k {k}, degree {tune.degree}, ratio {tune.lower_ratio} "
+            f"(rates {tune.ratio_rates}), fine_sub {tune.fine_sub}, "
             f"coarse_sub {tune.coarse_sub} (coarse tile us {tune.coarse_tile_us}); "
             f"tile us {tune.tile_us}, degree rates {tune.degree_rates}, width rates {tune.width_rates}")
     ctx.timing_enable(True)  # reset the in-solve sums

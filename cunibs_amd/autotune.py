@@ -34,15 +34,18 @@ class TuningResult:
     seconds: float = 0.0
     tile_us: dict = field(default_factory=dict)          # fine_sub -> us per iteration's fine work
     coarse_tile_us: dict = field(default_factory=dict)   # coarse_sub -> us per V-cycle
+    lower_ratio: float = 12.0
     degree_rates: dict = field(default_factory=dict)     # degree -> placements/s
+    ratio_rates: dict = field(default_factory=dict)      # Chebyshev lower ratio -> placements/s
     width_rates: dict = field(default_factory=dict)      # block_k -> placements/s
 
     def as_dict(self) -> dict:
         return {"block_k": self.block_k, "degree": self.degree, "fine_sub": self.fine_sub,
-                "coarse_sub": self.coarse_sub, "seconds": round(self.seconds, 3),
+                "coarse_sub": self.coarse_sub, "lower_ratio": self.lower_ratio, "seconds": round(self.seconds, 3),
                 "tile_us": {str(k): round(v, 2) for k, v in self.tile_us.items()},
                 "coarse_tile_us": {str(k): round(v, 2) for k, v in self.coarse_tile_us.items()},
                 "degree_rates": {str(k): round(v, 2) for k, v in self.degree_rates.items()},
+                "ratio_rates": {str(k): round(v, 2) for k, v in self.ratio_rates.items()},
                 "width_rates": {str(k): round(v, 2) for k, v in self.width_rates.items()}}
 
 
@@ -59,11 +62,12 @@ def _rate(ctx, pos, mom, sites, didt, k: int, warm_start: bool) -> float:
 
 
 def autotune(ctx: fem.SolverContext, dip_pos_m, dip_moment, sites: Sequence, didt: float = 1e6, *,
-             block_ks: Sequence[int] = (4, 8, 16, 32), degrees: Sequence[int] = (2, 3, 4, 5, 6),
+             block_ks: Sequence[int] = (4, 8, 16, 32), degrees: Sequence[int] = (1, 2, 3, 4, 5, 6),
              subs: Sequence[int] = (0, 1, 2, 4, 8), coarse_subs: Sequence[int] = (0, 2, 4, 8, 16),
-             base_k: int = 8, warm_start: bool = False) -> TuningResult:
+             ratios: Sequence[float] = (12.0,), base_k: int = 8, warm_start: bool = False) -> TuningResult:
     """Tune on `sites` (their count should be a multiple of every width, e.g. 32): the fine and the
-    coarse tiles at width `base_k`, then the degree at `base_k`, then the width. Widths above what a warm start
+    coarse tiles at width `base_k`, then the degree and the Chebyshev ratio at `base_k`, then the
+    width. Widths above what a warm start
     supports (16) are skipped when `warm_start` is set."""
     t_start = time.perf_counter()
     widths = [k for k in block_ks if 1 <= k <= fem.MAX_BLOCK and k <= len(sites) and not (warm_start and k > 16)]
@@ -95,6 +99,20 @@ def autotune(ctx: fem.SolverContext, dip_pos_m, dip_moment, sites: Sequence, did
         res.degree_rates[d] = _rate(ctx, dip_pos_m, dip_moment, sites, didt, base_k, warm_start)
     res.degree = max(res.degree_rates, key=res.degree_rates.get)
     ctx.set_tuning(degree=res.degree)
+    base_ratio = ctx.tuning["lower_ratio"]
+    for r in ratios:
+        if abs(r - base_ratio) < 1e-6:
+            res.ratio_rates[r] = res.degree_rates[res.degree]
+            continue
+        ctx.set_tuning(lower_ratio=r)
+        res.ratio_rates[r] = _rate(ctx, dip_pos_m, dip_moment, sites, didt, base_k, warm_start)
+    if res.ratio_rates:
+        res.lower_ratio = max(res.ratio_rates, key=res.ratio_rates.get)
+        if res.ratio_rates[res.lower_ratio] > res.degree_rates[res.degree]:
+            res.degree_rates[res.degree] = res.ratio_rates[res.lower_ratio]
+    else:
+        res.lower_ratio = base_ratio
+    ctx.set_tuning(lower_ratio=res.lower_ratio)
 
     # 3. width: whole blocks over the same placements
     for k in widths:

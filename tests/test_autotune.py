@@ -38,7 +38,7 @@ def test_default_tuning_is_the_reference_setting(tctx):
     assert t == {"degree": 2, "lower_ratio": pytest.approx(12.0), "fine_sub": 0, "coarse_sub": 0}
 
 
-@pytest.mark.parametrize("degree", [3, 6])
+@pytest.mark.parametrize("degree", [1, 3, 6])
 def test_vcycle_at_another_degree_matches_the_oracle(tctx, octx, oracle_mod, degree):
     h = dataclasses.replace(octx.hier, cheb=oracle_mod.chebyshev_coefficients(degree, 12.0))
     B = np.random.default_rng(degree).standard_normal((octx.reduced.n, 8)).astype(np.float32)
