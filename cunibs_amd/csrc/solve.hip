@@ -141,7 +141,7 @@ __device__ __forceinline__ void vstore(float* p, const float (&v)[V]) {
 // Requires e > j implies e - 1 is a valid entry.
 template <int SUB>
 constexpr int gather_batch() {
-    return SUB <= 2 ? 8 : (SUB == 4 ? 4 : 2);
+    return SUB == 4 ? 4 : 8;
 }
 
 // With GS, the gathered rows are scaled on the fly: X_j -> (gs0 * gscale[j]) X_j (the Jacobi
@@ -559,29 +559,33 @@ template <int K>
 __global__ void __launch_bounds__(kRowThreads) coarse_kernel(int n, const double* __restrict__ inv,
                                                              const float* __restrict__ b, float* __restrict__ x,
                                                              const int* __restrict__ state) {
+    // One wavefront per row; lane l takes column l % K of the entries j = l / K (mod 64 / K) of the
+    // row, in FP64, then the 64 / K partial sums of a column fold by a fixed xor tree.
+    constexpr int C = K < kWave ? K : kWave;
+    constexpr int Q = kWave / C;
     if (finished(state)) return;
     const int row = static_cast<int>(blockIdx.x) * kRowWaves + static_cast<int>(threadIdx.x >> 6);
     const int lane = threadIdx.x & (kWave - 1);
-    double acc[K];
+    const int c = lane % C, q = lane / C;
+    double acc[K / C];
 #pragma unroll
-    for (int c = 0; c < K; ++c) acc[c] = 0.0;
+    for (int u = 0; u < K / C; ++u) acc[u] = 0.0;
     if (row < n) {
         const double* a = inv + static_cast<std::int64_t>(row) * n;
-        for (int j = lane; j < n; j += kWave) {
+        for (int j = q; j < n; j += Q) {
             const double v = a[j];
 #pragma unroll
-            for (int c = 0; c < K; ++c) acc[c] = fma(v, static_cast<double>(b[static_cast<std::int64_t>(j) * K + c]), acc[c]);
+            for (int u = 0; u < K / C; ++u)
+                acc[u] = fma(v, static_cast<double>(b[static_cast<std::int64_t>(j) * K + u * C + c]), acc[u]);
         }
     }
 #pragma unroll
-    for (int c = 0; c < K; ++c)
-        for (int off = kWave / 2; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, kWave);
-    if (row < n && lane < K) {
-        double v = 0.0;
+    for (int u = 0; u < K / C; ++u)
 #pragma unroll
-        for (int c = 0; c < K; ++c)
-            if (c == lane) v = acc[c];
-        x[static_cast<std::int64_t>(row) * K + lane] = static_cast<float>(v);
+        for (int off = C; off < kWave; off <<= 1) acc[u] += __shfl_xor(acc[u], off, kWave);
+    if (row < n && q == 0) {
+#pragma unroll
+        for (int u = 0; u < K / C; ++u) x[static_cast<std::int64_t>(row) * K + u * C + c] = static_cast<float>(acc[u]);
     }
 }
 
