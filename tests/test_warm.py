@@ -82,3 +82,18 @@ def test_warm_start_keeps_the_answer_and_saves_iterations(trained, coil):
     again_c = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, train[:4], 1e6)
     again_w = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, train[:4], 1e6, warm_start=True)
     assert max(r.iterations for r in again_w) <= min(r.iterations for r in again_c) - 5
+
+
+def test_guess_stage_timing(trained, coil):
+    """cn_bench_guess times the projection, the coefficients and the expansion of the last
+    block's right-hand sides, and leaves the warm-started answer unchanged."""
+    from cunibs_amd import fem, synthetic
+    ctx, _, _, _ = trained
+    sites = synthetic.cap_sites(4, seed=14)
+    before = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites, 1e6, warm_start=True)
+    t = ctx.bench_guess(4, reps=3)
+    assert set(t) == {"project_ms", "coefficients_ms", "expand_ms"}
+    assert all(v > 0 for v in t.values())
+    after = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites, 1e6, warm_start=True)
+    for a, b in zip(before, after):
+        np.testing.assert_array_equal(a.magnE, b.magnE)
