@@ -69,6 +69,7 @@ struct cn_context {
     cn::DeviceBuffer<std::int32_t> tets, tags, skin, node_perm, tet_perm;
     cn::DeviceBuffer<float> g32, vols32, neg_vc;
     cn::DeviceBuffer<std::int32_t> n2c_ptr, n2c_idx;
+    cn::DeviceBuffer<std::int32_t> corner_pos;  // position of corner 4 e + i in node2corner order
     // full stiffness (optional) and the reduced operator
     cn::DeviceBuffer<std::int32_t> indptr, indices;
     cn::DeviceBuffer<double> data;
@@ -290,6 +291,8 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
     c.n2c_ptr.reset(n + 1, "n2c ptr");
     c.n2c_idx.reset(4 * m, "n2c idx");
     node2corner(c.tets.get(), m, 4, n, c.n2c_ptr.get(), c.n2c_idx.get(), s);
+    c.corner_pos.reset(4 * m, "corner positions");
+    corner_positions(c.n2c_idx.get(), 4 * m, c.corner_pos.get(), s);
     DeviceBuffer<std::int32_t> indptr, indices;
     const std::int64_t nnz = stiffness_pattern(c.tets.get(), c.n2c_ptr.get(), c.n2c_idx.get(), n, indptr, indices, s);
     DeviceBuffer<double> data(std::max<std::int64_t>(nnz, 1), "stiffness");
@@ -720,9 +723,10 @@ void uq_prepare(cn_context& c, const double* pos_m, const double* mom, std::int6
     DeviceBuffer<float> negvc(m, "tissue neg_vc"), q(4 * m, "corner terms");
     for (int t = 0; t < T; ++t) {
         tissue_neg_vc(c.vols32.get(), c.tags.get(), m, present[t], negvc.get(), s);
-        element_pass(u.dadt_nodes1.get(), c.tets.get(), c.g32.get(), negvc.get(), m, 1, 1, nullptr, q.get(), s);
-        node_rhs(q.get(), c.n2c_ptr.get(), c.n2c_idx.get(), n, 1, 1, c.ground,
-                 u.rhs.get() + static_cast<std::int64_t>(t) * nr, nullptr, s);
+        element_pass(u.dadt_nodes1.get(), c.tets.get(), c.g32.get(), negvc.get(), m, 1, 1, nullptr, q.get(),
+                     c.corner_pos.get(), s);
+        node_rhs(q.get(), c.n2c_ptr.get(), n, 1, 1, c.ground, u.rhs.get() + static_cast<std::int64_t>(t) * nr, nullptr,
+                 s);
     }
     u.sum.reset(m, "|E| sum");
     u.sumsq.reset(m, "|E| sum of squares");
@@ -939,14 +943,14 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
         elm_out = c.d_dadt_elm.get();
     }
     element_pass(c.d_dadt_nodes.get(), c.tets.get(), c.g32.get(), c.neg_vc.get(), m, k, K, elm_out,
-                 c.d_q.get(), s, c.vols32.get(), c.tags.get(), per_sigma ? c.blk_sigma_tag.get() : nullptr,
-                 per_sigma ? sopt->n_sigma : 0);
+                 c.d_q.get(), c.corner_pos.get(), s, c.vols32.get(), c.tags.get(),
+                 per_sigma ? c.blk_sigma_tag.get() : nullptr, per_sigma ? sopt->n_sigma : 0);
     float* rhs_full = nullptr;
     if (out && out->rhs) {
         c.d_rhs.ensure(n * k, "rhs full");
         rhs_full = c.d_rhs.get();
     }
-    node_rhs(c.d_q.get(), c.n2c_ptr.get(), c.n2c_idx.get(), n, k, K, c.ground, c.d_B.get(), rhs_full, s);
+    node_rhs(c.d_q.get(), c.n2c_ptr.get(), n, k, K, c.ground, c.d_B.get(), rhs_full, s);
     c.mark_phase(3, s);
     // 3. block PCG.
     SolveOptions opt;

@@ -331,8 +331,9 @@ __global__ void __launch_bounds__(kDadtBlock) dadt_parallel_kernel(const float* 
 __global__ void element_pass_kernel(const float* __restrict__ nodal, const std::int32_t* __restrict__ tets,
                                     const float* __restrict__ g, const float* __restrict__ neg_vc, std::int64_t m,
                                     int k, int K, float* __restrict__ elm, float* __restrict__ q,
-                                    const float* __restrict__ vols, const std::int32_t* __restrict__ tags,
-                                    const float* __restrict__ sigma_tag, int n_sigma) {
+                                    const std::int32_t* __restrict__ corner_pos, const float* __restrict__ vols,
+                                    const std::int32_t* __restrict__ tags, const float* __restrict__ sigma_tag,
+                                    int n_sigma) {
     const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= m * K) return;
     const std::int64_t e = t / K;
@@ -353,23 +354,29 @@ __global__ void element_pass_kernel(const float* __restrict__ nodal, const std::
     for (int i = 0; i < 4; ++i) {
         const float* gc = g + (4 * e + i) * 3;
         const float w0 = gc[0] * w, w1 = gc[1] * w, w2 = gc[2] * w;
-        q[(4 * e + i) * K + c] = fmaf(d[2], w2, fmaf(d[1], w1, d[0] * w0));
+        q[static_cast<std::int64_t>(corner_pos[4 * e + i]) * K + c] = fmaf(d[2], w2, fmaf(d[1], w1, d[0] * w0));
     }
 }
 
 // Node gather (rhs.cu:84-104): b[node] = sum of its corners' q in node2corner order, written to
-// the solver block with the ground row dropped, and optionally to (k, n) full-node storage.
-__global__ void node_rhs_kernel(const float* __restrict__ q, const std::int32_t* __restrict__ ptr,
-                                const std::int32_t* __restrict__ idx, std::int64_t n, int k, int K, int ground,
-                                float* __restrict__ B, float* __restrict__ b_full) {
+// the solver block with the ground row dropped, and optionally to (k, n) full-node storage. The
+// element pass stored the corner terms in that order, so a node's terms are one contiguous run.
+__global__ void node_rhs_kernel(const float* __restrict__ q, const std::int32_t* __restrict__ ptr, std::int64_t n,
+                                int k, int K, int ground, float* __restrict__ B, float* __restrict__ b_full) {
     const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= n * K) return;
     const std::int64_t node = t / K;
     const int c = static_cast<int>(t % K);
     float acc = 0.f;
-    for (int p = ptr[node]; p < ptr[node + 1]; ++p) acc += q[static_cast<std::int64_t>(idx[p]) * K + c];
+    for (int p = ptr[node]; p < ptr[node + 1]; ++p) acc += q[static_cast<std::int64_t>(p) * K + c];
     if (b_full != nullptr && c < k) b_full[c * n + node] = acc;
     if (node != ground) B[(node - (node > ground)) * K + c] = acc;
+}
+
+__global__ void corner_pos_kernel(const std::int32_t* __restrict__ idx, std::int64_t count,
+                                  std::int32_t* __restrict__ corner_pos) {
+    const std::int64_t p = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p < count) corner_pos[idx[p]] = static_cast<std::int32_t>(p);
 }
 
 }  // namespace
@@ -405,16 +412,20 @@ void dadt_nbody_parallel(const float* dip8, int n_dip, const double* nodes_mm, s
 }
 
 void element_pass(const float* nodal, const std::int32_t* tets, const float* g, const float* neg_vc, std::int64_t m,
-                  int k, int K, float* elm, float* q, hipStream_t s, const float* vols, const std::int32_t* tags,
-                  const float* sigma_tag, int n_sigma) {
+                  int k, int K, float* elm, float* q, const std::int32_t* corner_pos, hipStream_t s, const float* vols,
+                  const std::int32_t* tags, const float* sigma_tag, int n_sigma) {
     if (const unsigned gr = grid_for(m * K))
-        element_pass_kernel<<<gr, kBlock, 0, s>>>(nodal, tets, g, neg_vc, m, k, K, elm, q, vols, tags, sigma_tag,
-                                                  n_sigma);
+        element_pass_kernel<<<gr, kBlock, 0, s>>>(nodal, tets, g, neg_vc, m, k, K, elm, q, corner_pos, vols, tags,
+                                                  sigma_tag, n_sigma);
 }
 
-void node_rhs(const float* q, const std::int32_t* ptr, const std::int32_t* idx, std::int64_t n, int k, int K,
-              int ground, float* B, float* b_full, hipStream_t s) {
-    if (const unsigned gr = grid_for(n * K)) node_rhs_kernel<<<gr, kBlock, 0, s>>>(q, ptr, idx, n, k, K, ground, B, b_full);
+void corner_positions(const std::int32_t* idx, std::int64_t count, std::int32_t* corner_pos, hipStream_t s) {
+    if (const unsigned gr = grid_for(count)) corner_pos_kernel<<<gr, kBlock, 0, s>>>(idx, count, corner_pos);
+}
+
+void node_rhs(const float* q, const std::int32_t* ptr, std::int64_t n, int k, int K, int ground, float* B,
+              float* b_full, hipStream_t s) {
+    if (const unsigned gr = grid_for(n * K)) node_rhs_kernel<<<gr, kBlock, 0, s>>>(q, ptr, n, k, K, ground, B, b_full);
 }
 
 }  // namespace cn

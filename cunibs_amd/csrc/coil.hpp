@@ -21,12 +21,17 @@ void dadt_nbody(const float* dip, int n_dip, const double* nodes_mm, std::int64_
 void dadt_nbody_parallel(const float* dip8, int n_dip, const double* nodes_mm, std::int64_t n, const double* origins,
                          const float* dirs, int k, int K, float scale, float* out, hipStream_t s);
 // Element average of the nodal dA/dt and the RHS corner terms; sigma_tag (K x n_sigma FP32, with
-// vols and tags) gives each placement its own conductivities instead of neg_vc.
+// vols and tags) gives each placement its own conductivities instead of neg_vc. The corner term of
+// corner 4 e + i is stored at row corner_pos[4 e + i] of q: its position in node2corner order.
 void element_pass(const float* nodal, const std::int32_t* tets, const float* g, const float* neg_vc, std::int64_t m,
-                  int k, int K, float* elm, float* q, hipStream_t s, const float* vols = nullptr,
-                  const std::int32_t* tags = nullptr, const float* sigma_tag = nullptr, int n_sigma = 0);
-void node_rhs(const float* q, const std::int32_t* ptr, const std::int32_t* idx, std::int64_t n, int k, int K,
-              int ground, float* B, float* b_full, hipStream_t s);
+                  int k, int K, float* elm, float* q, const std::int32_t* corner_pos, hipStream_t s,
+                  const float* vols = nullptr, const std::int32_t* tags = nullptr, const float* sigma_tag = nullptr,
+                  int n_sigma = 0);
+// corner_pos[idx[p]] = p for the 4m corners of a node2corner list.
+void corner_positions(const std::int32_t* idx, std::int64_t count, std::int32_t* corner_pos, hipStream_t s);
+// b[node] = sum of the node's corner terms, q in node2corner order (rows ptr[node] .. ptr[node+1]).
+void node_rhs(const float* q, const std::int32_t* ptr, std::int64_t n, int k, int K, int ground, float* B,
+              float* b_full, hipStream_t s);
 
 // post.hip
 void scatter_potential(const double* X, std::int64_t n, int k, int K, int ground, double* v, hipStream_t s);
