@@ -716,29 +716,25 @@ __global__ void __launch_bounds__(kGramThreads) residual64_kernel(
     block_cols_store<K, V, kGramWaves>(nrm, grp, partials + static_cast<std::int64_t>(lb) * K);
 }
 
-// X64 += W a (FP64), R -= S a (FP32), partials of ||R_c||^2. Frozen columns have a zero column
-// in `a`, which leaves X and R exactly unchanged. A lane owns V columns of one row and reads the
-// row's W and S whole (the other lanes of the row read the same cache lines).
+// R -= S a (FP32), partials of ||R_c||^2; the solution update X += W a is deferred to the end of
+// the window (flush_x_kernel). Frozen columns have a zero column in `a`, which leaves R exactly
+// unchanged. A lane owns V columns of one row and reads the row's S whole (the other lanes of the
+// row read the same cache lines).
 template <int K>
 constexpr int xr_threads() {
     return K <= 16 ? 1024 : 512;
 }
 
 template <int K>
-__global__ void __launch_bounds__(xr_threads<K>()) update_xr_kernel(
-    std::int64_t n, const float* __restrict__ W, const float* __restrict__ S, const double* __restrict__ a,
-    double* __restrict__ X, float* __restrict__ R, double* __restrict__ partials, std::int64_t rows_per_block,
-    const int* __restrict__ state) {
+__global__ void __launch_bounds__(xr_threads<K>()) update_r_kernel(
+    std::int64_t n, const float* __restrict__ S, const double* __restrict__ a, float* __restrict__ R,
+    double* __restrict__ partials, std::int64_t rows_per_block, const int* __restrict__ state) {
     if (finished(state)) return;
     constexpr int V = vec_width<K>();
     constexpr int G = K / V;
     constexpr int RPW = kWave / G;
-    __shared__ double al[K * K];
     __shared__ float alf[K * K];
-    for (int e = threadIdx.x; e < K * K; e += blockDim.x) {
-        al[e] = a[e];
-        alf[e] = static_cast<float>(a[e]);
-    }
+    for (int e = threadIdx.x; e < K * K; e += blockDim.x) alf[e] = static_cast<float>(a[e]);
     __syncthreads();
     const int lb = logical_block();
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
@@ -752,25 +748,17 @@ __global__ void __launch_bounds__(xr_threads<K>()) update_xr_kernel(
     for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW; r0 < hi; r0 += WAVES * RPW) {
         const std::int64_t row = r0 + lane / G;
         if (row >= hi) continue;
-        double dx[V];
         float dr[V];
 #pragma unroll
-        for (int v = 0; v < V; ++v) {
-            dx[v] = 0.0;
-            dr[v] = 0.0f;
-        }
+        for (int v = 0; v < V; ++v) dr[v] = 0.0f;
 #pragma unroll
         for (int b0 = 0; b0 < K; b0 += V) {
-            float wb[V], sb[V];
-            vload<V>(W + row * K + b0, wb);
+            float sb[V];
             vload<V>(S + row * K + b0, sb);
 #pragma unroll
             for (int u = 0; u < V; ++u) {
 #pragma unroll
-                for (int v = 0; v < V; ++v) {
-                    dx[v] = fma(static_cast<double>(wb[u]), al[(b0 + u) * K + c0 + v], dx[v]);
-                    dr[v] = fmaf(sb[u], alf[(b0 + u) * K + c0 + v], dr[v]);
-                }
+                for (int v = 0; v < V; ++v) dr[v] = fmaf(sb[u], alf[(b0 + u) * K + c0 + v], dr[v]);
             }
         }
         const std::int64_t at = row * K + c0;
@@ -778,7 +766,6 @@ __global__ void __launch_bounds__(xr_threads<K>()) update_xr_kernel(
         vload<V>(R + at, rr);
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-            X[at + v] += dx[v];
             rr[v] -= dr[v];
             nrm[v] = fma(static_cast<double>(rr[v]), static_cast<double>(rr[v]), nrm[v]);
         }
@@ -787,12 +774,64 @@ __global__ void __launch_bounds__(xr_threads<K>()) update_xr_kernel(
     block_cols_store<K, V, WAVES>(nrm, grp, partials + static_cast<std::int64_t>(lb) * K);
 }
 
-// W <- Z - W D, in place: every lane of a row loads the row's W before any of them stores (the
-// stores depend on the loads, and a row's lanes share one wavefront).
+// X64 += sum over the window's slots j < L whose iteration ran of W_j a_j (FP64), in slot order.
+// Runs whether or not the solve has finished: the iterations before the finish are still owed.
+// Launched in the window's last iteration before its W update.
+template <int K>
+__global__ void __launch_bounds__(kRowThreads) flush_x_kernel(std::int64_t n, const float* __restrict__ W0,
+                                                              const float* __restrict__ W1,
+                                                              const float* __restrict__ W2,
+                                                              const float* __restrict__ W3,
+                                                              const double* __restrict__ AL,
+                                                              const int* __restrict__ ran, int L,
+                                                              double* __restrict__ X) {
+    static_assert(kLag == 4, "four window slots");
+    constexpr int V = vec_width<K>();
+    constexpr int G = K / V;
+    __shared__ double al[kLag * K * K];
+    __shared__ int use[kLag];
+    for (int e = threadIdx.x; e < kLag * K * K; e += blockDim.x) al[e] = AL[e];
+    if (threadIdx.x < kLag) use[threadIdx.x] = (static_cast<int>(threadIdx.x) < L) && ran[threadIdx.x] != 0;
+    __syncthreads();
+    const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * kRowThreads + threadIdx.x;
+    const std::int64_t row = t / G;
+    if (row >= n) return;
+    const int c0 = static_cast<int>(t % G) * V;
+    const float* const ws[kLag] = {W0, W1, W2, W3};
+    double dx[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) dx[v] = 0.0;
+#pragma unroll
+    for (int j = 0; j < kLag; ++j) {
+        if (!use[j]) continue;
+        const double* a = al + j * K * K;
+#pragma unroll
+        for (int b0 = 0; b0 < K; b0 += V) {
+            float wb[V];
+            vload<V>(ws[j] + row * K + b0, wb);
+#pragma unroll
+            for (int u = 0; u < V; ++u) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) dx[v] = fma(static_cast<double>(wb[u]), a[(b0 + u) * K + c0 + v], dx[v]);
+            }
+        }
+    }
+    const std::int64_t at = row * K + c0;
+#pragma unroll
+    for (int v = 0; v < V; ++v) X[at + v] += dx[v];
+}
+
+__global__ void clear_ran_kernel(int* __restrict__ ran) {
+    if (threadIdx.x < kLag) ran[threadIdx.x] = 0;
+}
+
+// W_out <- Z - W D (the next slot of the window's ring, or W itself): every lane of a row loads
+// the row's W before any of them stores (the stores depend on the loads, and a row's lanes share
+// one wavefront), so W_out may be W.
 template <int K>
 __global__ void __launch_bounds__(kRowThreads) update_w_kernel(std::int64_t n, const float* __restrict__ Z,
-                                                               const double* __restrict__ D, float* W,
-                                                               const int* __restrict__ state) {
+                                                               const double* __restrict__ D, const float* W,
+                                                               float* W_out, const int* __restrict__ state) {
     if (finished(state)) return;
     constexpr int V = vec_width<K>();
     constexpr int G = K / V;
@@ -819,12 +858,17 @@ __global__ void __launch_bounds__(kRowThreads) update_w_kernel(std::int64_t n, c
             for (int v = 0; v < V; ++v) acc[v] = fmaf(-wb[u], dm[(b0 + u) * K + c0 + v], acc[v]);
         }
     }
-    if (valid) vstore<V>(W + row * K + c0, acc);
+    if (valid) vstore<V>(W_out + row * K + c0, acc);
 }
 
 __global__ void cast_f64_f32_kernel(std::int64_t count, const double* __restrict__ in, float* __restrict__ out) {
     const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < count) out[i] = static_cast<float>(in[i]);
+}
+
+__global__ void zero_f32_kernel(std::int64_t count, float* __restrict__ out) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = 0.0f;
 }
 
 __global__ void zero_f64_kernel(std::int64_t count, double* __restrict__ out) {
@@ -928,9 +972,10 @@ __global__ void __launch_bounds__(kWave) coef_kernel(const double* __restrict__ 
                                                      double drop_tol, double* __restrict__ G_out,
                                                      double* __restrict__ T, double* __restrict__ TT,
                                                      double* __restrict__ AL, int* __restrict__ state,
-                                                     int independent) {
+                                                     int independent, int* __restrict__ ran) {
     static_assert(K <= kWave, "one lane per row");
     if (finished(state)) return;
+    if (threadIdx.x == 0) *ran = 1;  // this window slot's coefficients are to be applied
     constexpr int kE = K * K;
     __shared__ double A[kE];   // G scaled to unit diagonal
     __shared__ double H[kE];
@@ -1266,10 +1311,11 @@ void BlockSolver::set_multi(const MultiOperator* m) {
 // graph exec destroyed and another instantiated in the same process crashes rocprofv3's kernel
 // tracer on this ROCm, and a retired exec costs only its host-side node list.
 void BlockSolver::drop_graphs() {
-    for (auto& g : graphs_) {
-        if (g) retired_.push_back(g);
-        g = nullptr;
-    }
+    for (auto* set : {graphs_, graphs1_})
+        for (int i = 0; i < kNumBlockWidths; ++i) {
+            if (set[i]) retired_.push_back(set[i]);
+            set[i] = nullptr;
+        }
 }
 
 void BlockSolver::ensure(int K) {
@@ -1278,6 +1324,12 @@ void BlockSolver::ensure(int K) {
     R_.reset(nk, "R");
     Z_.reset(nk, "Z");
     W_.reset(nk, "W");
+    for (auto& w : Wx_) {
+        w.reset(nk, "W ring");
+        // finite before any window reads it
+        zero_f32_kernel<<<grid_for(static_cast<std::int64_t>(nk)), kBlock, 0, s_>>>(static_cast<std::int64_t>(nk),
+                                                                                   w.get());
+    }
     S_.reset(nk, "S");
     R64_.reset(nk, "R64");
     X64_.reset(nk, "X64");
@@ -1298,7 +1350,8 @@ void BlockSolver::ensure(int K) {
     G_.reset(K * K, "G");
     T_.reset(K * K, "T");
     TT_.reset(K * K, "TT");
-    AL_.reset(K * K, "alpha");
+    AL_.reset(static_cast<std::size_t>(kLag) * K * K, "alpha");
+    ran_.reset(kLag, "window slots run");
     D_.reset(K * K, "D");
     norms_.reset(K, "norms");
     target_.reset(K, "targets");
@@ -1380,34 +1433,37 @@ int BlockSolver::fine_sub() const {
 
 // S = A W (differenced), then the partials of G = W^T S and H = W^T R.
 template <int K>
-void BlockSolver::spmm_block_k(const int* state) {
+void BlockSolver::spmm_block_k(const int* state, const float* W) {
     const std::int64_t nn = n();
     if (multi_) {
         require(multi_->width == K, "multi-matrix values were prepared for another block width");
         CsrView off = op_.off;
         off.val = multi_->off_vals;
-        spmm_grid_ = launch_rows<K, kSpmmDiffMulti>(fine_sub<K>(), off, multi_->rowsum, nullptr, W_.get(), nullptr,
-                                                    0.f, 0.f, 0.f, S_.get(), state, s_, stamps());
+        spmm_grid_ = launch_rows<K, kSpmmDiffMulti>(fine_sub<K>(), off, multi_->rowsum, nullptr, W, nullptr, 0.f,
+                                                    0.f, 0.f, S_.get(), state, s_, stamps());
     } else {
-        spmm_grid_ = launch_rows<K, kSpmmDiff>(fine_sub<K>(), op_.off, op_.rowsum, nullptr, W_.get(), nullptr, 0.f,
-                                               0.f, 0.f, S_.get(), state, s_, stamps());
+        spmm_grid_ = launch_rows<K, kSpmmDiff>(fine_sub<K>(), op_.off, op_.rowsum, nullptr, W, nullptr, 0.f, 0.f,
+                                               0.f, S_.get(), state, s_, stamps());
     }
-    launch_gram<K, true>(nn, W_.get(), S_.get(), R_.get(), partials_.get(), gram_rows(nn), state, s_);
+    launch_gram<K, true>(nn, W, S_.get(), R_.get(), partials_.get(), gram_rows(nn), state, s_);
 }
 
 template <int K>
-void BlockSolver::iteration_k(const SolveOptions& opt) {
+void BlockSolver::iteration_k(const SolveOptions& opt, int j, int L) {
     const std::int64_t nn = n();
     const std::int64_t rpb = gram_rows(nn);
     const int* st = state_.get();
     int* stw = state_.get();
+    const bool last = j == L - 1;
+    const float* w_in = wslot(j);
+    float* w_out = last ? wslot(0) : wslot(j + 1);
+    double* al = AL_.get() + static_cast<std::int64_t>(j) * K * K;
     mark(0, st);
-    spmm_block_k<K>(st);
+    spmm_block_k<K>(st, w_in);
     reduce_cols(partials_.get(), kGramBlocks, 2 * K * K, red_.get(), st, s_);
-    coef_kernel<K><<<1, kWave, 0, s_>>>(red_.get(), kGramBlocks, opt.drop_tol, G_.get(), T_.get(),
-                                                TT_.get(), AL_.get(), stw, multi_ ? 1 : 0);
-    update_xr_kernel<K><<<kGramBlocks, xr_threads<K>(), 0, s_>>>(nn, W_.get(), S_.get(), AL_.get(), X64_.get(), R_.get(),
-                                                             partials_.get(), rpb, st);
+    coef_kernel<K><<<1, kWave, 0, s_>>>(red_.get(), kGramBlocks, opt.drop_tol, G_.get(), T_.get(), TT_.get(), al, stw,
+                                        multi_ ? 1 : 0, ran_.get() + j);
+    update_r_kernel<K><<<kGramBlocks, xr_threads<K>(), 0, s_>>>(nn, S_.get(), al, R_.get(), partials_.get(), rpb, st);
     reduce_cols(partials_.get(), kGramBlocks, K, red_.get(), st, s_);
     converge_kernel<K><<<1, small_threads<K>(), 0, s_>>>(red_.get(), kGramBlocks, target_.get(), norms_.get(), stw);
     mark(1, st);
@@ -1417,12 +1473,21 @@ void BlockSolver::iteration_k(const SolveOptions& opt) {
     reduce_cols(partials_.get(), kGramBlocks, K * K, red_.get(), st, s_);
     dmat_kernel<K><<<1, small_threads<K>(), 0, s_>>>(red_.get(), kGramBlocks, TT_.get(), D_.get(), stw,
                                                 multi_ ? 1 : 0);
+    if (last)  // before this iteration's W update, which may overwrite slot 0
+        flush_x_kernel<K><<<grid_for(nn * col_groups<K>(), kRowThreads), kRowThreads, 0, s_>>>(
+            nn, wslot(0), wslot(1), wslot(2), wslot(3), AL_.get(), ran_.get(), L, X64_.get());
     update_w_kernel<K><<<grid_for(nn * col_groups<K>(), kRowThreads), kRowThreads, 0, s_>>>(nn, Z_.get(), D_.get(),
-                                                                                           W_.get(), st);
+                                                                                           w_in, w_out, st);
     mark(3, st);
     if (timing_)
         timing_accumulate_kernel<<<1, kBlock, 0, s_>>>(stamps_.get(), static_cast<int>(spmm_grid_),
                                                        stamps_.get() + stamp_marks(), t_sums_.get());
+}
+
+template <int K>
+void BlockSolver::window_k(const SolveOptions& opt, int L) {
+    clear_ran_kernel<<<1, kWave, 0, s_>>>(ran_.get());
+    for (int j = 0; j < L; ++j) iteration_k<K>(opt, j, L);
 }
 
 template <int K>
@@ -1486,17 +1551,21 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
         d2d(W_.get(), Z_.get(), static_cast<std::size_t>(nk), s_);
         int slot = 0;
         while (kBlockWidths[slot] != K) ++slot;
-        if (opt.use_graph && (graphs_[slot] == nullptr || graph_drop_[slot] != opt.drop_tol)) {
-            if (graphs_[slot]) {
-                retired_.push_back(graphs_[slot]);
-                graphs_[slot] = nullptr;
+        if (opt.use_graph &&
+            (graphs_[slot] == nullptr || graphs1_[slot] == nullptr || graph_drop_[slot] != opt.drop_tol)) {
+            for (auto* gs : {&graphs_[slot], &graphs1_[slot]}) {
+                if (*gs) retired_.push_back(*gs);
+                *gs = nullptr;
             }
-            hipGraph_t g = nullptr;
-            check(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal), "begin capture");
-            iteration_k<K>(opt);
-            check(hipStreamEndCapture(s_, &g), "end capture");
-            check(hipGraphInstantiate(&graphs_[slot], g, nullptr, nullptr, 0), "graph instantiate");
-            (void)hipGraphDestroy(g);
+            for (const int L : {kLag, 1}) {
+                hipGraph_t g = nullptr;
+                check(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal), "begin capture");
+                window_k<K>(opt, L);
+                check(hipStreamEndCapture(s_, &g), "end capture");
+                check(hipGraphInstantiate(L == 1 ? &graphs1_[slot] : &graphs_[slot], g, nullptr, nullptr, 0),
+                      "graph instantiate");
+                (void)hipGraphDestroy(g);
+            }
             graph_drop_[slot] = opt.drop_tol;
         }
         // Bounded by the launches the host issues, not only by the device counter.
@@ -1509,11 +1578,14 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
             if (hint > 0) want = reads == 0 ? std::max(1, hint - 2) : (reads <= 4 ? 1 : opt.check_every);
             const int batch = std::min(want, opt.max_iters - launched);
             ++reads;
-            for (int b = 0; b < batch; ++b) {
+            // windows of kLag iterations, then single-iteration windows for the remainder
+            for (int b = 0; b < batch;) {
+                const int L = batch - b >= kLag ? kLag : 1;
                 if (opt.use_graph)
-                    check(hipGraphLaunch(graphs_[slot], s_), "graph launch");
+                    check(hipGraphLaunch(L == 1 ? graphs1_[slot] : graphs_[slot], s_), "graph launch");
                 else
-                    iteration_k<K>(opt);
+                    window_k<K>(opt, L);
+                b += L;
             }
             launched += batch;
             d2h(host_state_, state_.get(), 2 + 2 * K, s_);
@@ -1620,7 +1692,7 @@ double BlockSolver::bench_spmm(int K, int reps, int mode, int sub) {
     if (sub > 0) fine_sub_override_ = sub;
     auto launch = [&] {
         if (mode == 2)
-            dispatch_k(K, [&](auto kc) { spmm_block_k<decltype(kc)::value>(nullptr); });
+            dispatch_k(K, [&](auto kc) { spmm_block_k<decltype(kc)::value>(nullptr, W_.get()); });
         else if (mode == 1)
             spmm_diff(W_.get(), S_.get(), K);
         else

@@ -17,6 +17,10 @@ constexpr int kMaxRounds = 8;
 // reduction depends on the row count alone.
 constexpr int kGramBlocks = 512;
 
+// Iterations whose solution updates X += W a are deferred and applied together: X (FP64) is read
+// and written once per window instead of once per iteration.
+constexpr int kLag = 4;
+
 int padded_width(int k);  // smallest compiled width >= k
 
 struct SolveOptions {
@@ -120,9 +124,15 @@ class BlockSolver {
     template <int K>
     void vcycle_k(const float* B, float* X, const int* state);
     template <int K>
-    void iteration_k(const SolveOptions& opt);
+    // Iteration j of a window of L: reads W slot j and writes slot j + 1, except the last, which
+    // first applies the window's deferred updates X += W_j a_j and then writes slot 0 (in place
+    // for L = 1), so every window starts from slot 0.
+    void iteration_k(const SolveOptions& opt, int j, int L);
+    // L (1..kLag) iterations.
     template <int K>
-    void spmm_block_k(const int* state);
+    void window_k(const SolveOptions& opt, int L);
+    template <int K>
+    void spmm_block_k(const int* state, const float* W);
     template <int K>
     int fine_sub() const;
     void ensure(int K);
@@ -137,7 +147,11 @@ class BlockSolver {
     int coarse_sub_ = 0;         // sub-lanes of the coarser levels' row kernels (0: automatic)
     const MultiOperator* multi_ = nullptr;
     // block vectors (n, K)
+    // W_ is slot 0 of the search-block ring (kLag slots), so the deferred solution update still
+    // has every W of its window.
     DeviceBuffer<float> R_, Z_, W_, S_;
+    DeviceBuffer<float> Wx_[kLag - 1];
+    float* wslot(int j) const { return j == 0 ? W_.get() : Wx_[j - 1].get(); }
     DeviceBuffer<double> R64_, X64_;
     // per-level V-cycle vectors
     std::vector<DeviceBuffer<float>> lb_, lx_, lr_;
@@ -145,7 +159,8 @@ class BlockSolver {
     // reductions and small state
     DeviceBuffer<double> partials_;  // kGramBlocks * 2 * K * K
     DeviceBuffer<double> red_;       // the partials summed over the blocks (2 K^2)
-    DeviceBuffer<double> G_, T_, TT_, AL_, D_, norms_, target_;
+    DeviceBuffer<double> G_, T_, TT_, AL_, D_, norms_, target_;  // AL_: kLag K x K coefficient slots
+    DeviceBuffer<int> ran_;  // per window slot: the iteration ran (its coefficients are to be applied)
     DeviceBuffer<int> state_;  // [0] done, [1] iteration, [2..2+K) frozen, [2+K..2+2K) col_iters
     int* host_state_ = nullptr;  // pinned
     // One captured iteration per block width, kept for the solver's lifetime: every buffer the
@@ -159,7 +174,8 @@ class BlockSolver {
     std::size_t stamp_marks() const;
     unsigned long long* stamps();
     void mark(int i, const int* state);
-    hipGraphExec_t graphs_[kNumBlockWidths] = {};
+    hipGraphExec_t graphs_[kNumBlockWidths] = {};   // a window of kLag iterations
+    hipGraphExec_t graphs1_[kNumBlockWidths] = {};  // a window of one iteration
     double graph_drop_[kNumBlockWidths] = {};  // the drop tolerance each graph was captured with
     // Inner iterations the last solve at each width took in its first round: the next solve runs
     // that many minus two before its first convergence read, then reads after every iteration
