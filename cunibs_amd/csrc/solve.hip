@@ -1575,7 +1575,8 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
         int reads = 0;
         while (launched < opt.max_iters) {
             int want = opt.check_every;
-            if (hint > 0) want = reads == 0 ? std::max(1, hint - 2) : (reads <= 4 ? 1 : opt.check_every);
+            // the first read after whole windows short of the hint, then after every iteration
+            if (hint > 0) want = reads == 0 ? std::max(1, (hint - 1) / kLag * kLag) : (reads <= 4 ? 1 : opt.check_every);
             const int batch = std::min(want, opt.max_iters - launched);
             ++reads;
             // windows of kLag iterations, then single-iteration windows for the remainder

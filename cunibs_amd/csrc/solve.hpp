@@ -178,9 +178,9 @@ class BlockSolver {
     hipGraphExec_t graphs1_[kNumBlockWidths] = {};  // a window of one iteration
     double graph_drop_[kNumBlockWidths] = {};  // the drop tolerance each graph was captured with
     // Inner iterations the last solve at each width took in its first round: the next solve runs
-    // that many minus two before its first convergence read, then reads after every iteration
-    // for a few, so a sweep of similar blocks neither waits on early reads nor replays finished
-    // iterations (each a graph of early-exiting launches).
+    // the whole windows short of that count before its first convergence read, then reads after
+    // every iteration for a few, so a sweep of similar blocks neither waits on early reads nor
+    // replays finished iterations (each a graph of early-exiting launches).
     int last_first_round_[kNumBlockWidths] = {};
     std::vector<hipGraphExec_t> retired_;
     void drop_graphs();
