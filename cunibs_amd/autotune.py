@@ -121,3 +121,20 @@ def autotune(ctx: fem.SolverContext, dip_pos_m, dip_moment, sites: Sequence, did
     res.block_k = max(res.width_rates, key=res.width_rates.get)
     res.seconds = time.perf_counter() - t_start
     return res
+
+
+def tune_overlap(ctx: fem.SolverContext, dip_pos_m, dip_moment, sites: Sequence, didt: float, block_k: int,
+                 candidates: Sequence[int] = (0, 256, 512, 1024, 2048), warm_start: bool = False) -> dict:
+    """The workgroup cap of the dA/dt that `fem.solve_sweep` overlaps with the previous block's
+    solve: each candidate times a pipelined sweep of `sites` (after one untimed sweep of two
+    blocks); the context is left at the fastest. Returns {"coil_blocks": best, "rates": {cap: placements/s}}."""
+    rates = {}
+    for cap in candidates:
+        ctx.set_tuning(coil_blocks=cap)
+        fem.solve_sweep(ctx, dip_pos_m, dip_moment, list(sites[:2 * block_k]), didt, block_k, warm_start=warm_start)
+        t0 = time.perf_counter()
+        fem.solve_sweep(ctx, dip_pos_m, dip_moment, list(sites), didt, block_k, warm_start=warm_start)
+        rates[cap] = len(sites) / (time.perf_counter() - t0)
+    best = max(rates, key=rates.get)
+    ctx.set_tuning(coil_blocks=best)
+    return {"coil_blocks": best, "rates": rates}

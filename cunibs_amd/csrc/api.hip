@@ -92,6 +92,12 @@ struct cn_context {
     cn::DeviceBuffer<std::int32_t> d_degen;
     cn::DeviceBuffer<float> d_dip, d_dadt_nodes, d_dadt_elm, d_q, d_B, d_E, d_magn, d_rhs, d_stage;
     cn::DeviceBuffer<double> d_X, d_v, d_x0;
+    // Pipelined sweeps (cn_solve_sweep): the next block's coil stage (dA/dt, element pass, RHS)
+    // runs on s2 into the second buffer set while the current block solves on s.
+    hipStream_t s2 = nullptr;
+    int coil_blocks = 0;  // grid cap of the overlapped dA/dt (0: the full grid)
+    hipEvent_t coil_done[2] = {nullptr, nullptr};
+    cn::DeviceBuffer<float> d_dadt_nodes2, d_B2;
     // recovery operators, built on first use (index: CN_RECOVERY_*), and their block buffers
     std::unique_ptr<cn::RecoveryOp> recovery[4];
     cn::DeviceBuffer<float> d_slots, d_Eraw;
@@ -164,7 +170,10 @@ struct cn_context {
     ~cn_context() {
         for (auto& e : phase_ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto& e : coil_done)
+            if (e) (void)hipEventDestroy(e);
         solver.reset();
+        if (s2) (void)hipStreamDestroy(s2);
         if (s) (void)hipStreamDestroy(s);
     }
 };
@@ -447,7 +456,7 @@ void pack_dipoles_parallel(const double* tf, const double* pos_m, const double* 
 // dA/dt at the nodes for k placements (frames tf, k x 16) into out (n, 3, K): the parallel-moment
 // kernel when the coil allows it, the general one otherwise.
 void dadt_for_frames(cn_context& c, const double* tf, int k, int K, const double* pos_m, const double* mom,
-                     std::int64_t n_dip, float scale, float* out, hipStream_t s);
+                     std::int64_t n_dip, float scale, float* out, hipStream_t s, int max_blocks = 0);
 
 void pack_dipoles(const double* tf, const double* pos_m, const double* mom, std::int64_t n_dip, float* out,
                   double* origin) {
@@ -481,7 +490,7 @@ void pack_dipoles(const double* tf, const double* pos_m, const double* mom, std:
 }
 
 void dadt_for_frames(cn_context& c, const double* tf, int k, int K, const double* pos_m, const double* mom,
-                     std::int64_t n_dip, float scale, float* out, hipStream_t s) {
+                     std::int64_t n_dip, float scale, float* out, hipStream_t s, int max_blocks) {
     using namespace cn;
     std::vector<double> origins(3 * k);
     double dir[3];
@@ -496,7 +505,7 @@ void dadt_for_frames(cn_context& c, const double* tf, int k, int K, const double
         h2d(c.d_dip.get() + dip.size(), dirs.data(), dirs.size(), s);
         h2d(c.d_origins.get(), origins.data(), 3 * k, s);
         dadt_nbody_parallel(c.d_dip.get(), static_cast<int>(n_dip), c.nodes.get(), c.n, c.d_origins.get(),
-                            c.d_dip.get() + dip.size(), k, K, scale, out, s);
+                            c.d_dip.get() + dip.size(), k, K, scale, out, s, max_blocks);
     } else {
         std::vector<float> dip(static_cast<std::size_t>(10 * n_dip * k));
         for (int i = 0; i < k; ++i)
@@ -1087,6 +1096,127 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
     }
 }
 
+// A sweep of n placements in blocks of block_k, pipelined over two streams: block b + 1's coil
+// stage (dA/dt at the nodes, the element pass and the node RHS, into buffer set (b + 1) % 2) runs on
+// s2 while block b solves on s (initial guess, block PCG, field, region metrics). The two overlap
+// on the device: the N-body keeps the vector ALUs busy while the solve streams the operator. The
+// frames of every placement are found first, in one pass. Each block's numerics are those of
+// cn_solve_block for the same block.
+void solve_sweep(cn_context& c, const double* pos_m, const double* mom, std::int64_t n_dip, const double* centers,
+                 const double* handles, const double* dists, int n_place, int block_k, double didt,
+                 const cn_solve_options* sopt, double* metrics_out, std::int32_t* iters_out, double* resid_out,
+                 std::int32_t* block_iters_out) {
+    using namespace cn;
+    require(n_place >= 1, "no placements");
+    require(block_k >= 1 && block_k <= kMaxBlock, "block_k must be in [1, 32]");
+    require(n_dip >= 1, "the coil needs at least one dipole");
+    require(std::isfinite(didt), "didt must be finite");
+    require(sopt == nullptr || (sopt->sigma == nullptr && sopt->collect == 0 && sopt->warm_start != 2),
+            "a sweep takes the context's conductivities, no collection and no supplied coefficients");
+    const std::int64_t n = c.n, m = c.m, nr = n - 1;
+    const int K = padded_width(block_k);
+    // frames of every placement (place.cu), degenerate handles rejected with the placement's index
+    std::vector<double> tf(static_cast<std::size_t>(16) * n_place);
+    std::vector<std::int32_t> degen(n_place);
+    place_block(c, centers, handles, dists, n_place, tf.data(), degen.data());
+    if (handles != nullptr)
+        for (int i = 0; i < n_place; ++i)
+            if (degen[i])
+                throw InvalidArgument("Degenerate coil placement at index " + std::to_string(i) + " of " +
+                                      std::to_string(n_place) +
+                                      ": handle_mm lies on the outward normal through the scalp projection of "
+                                      "center_mm, so the coil's rotation about the normal is undefined.");
+    if (c.s2 == nullptr) check(hipStreamCreateWithFlags(&c.s2, hipStreamNonBlocking), "create stream");
+    for (auto& e : c.coil_done)
+        if (e == nullptr) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "coil event");
+    c.d_dadt_nodes.ensure(3 * n * K, "dadt nodes");
+    c.d_dadt_nodes2.ensure(3 * n * K, "dadt nodes");
+    c.d_B.ensure(nr * K, "rhs block");
+    c.d_B2.ensure(nr * K, "rhs block");
+    c.d_q.ensure(4 * m * K, "corner terms");
+    float* const nodal[2] = {c.d_dadt_nodes.get(), c.d_dadt_nodes2.get()};
+    float* const rhs[2] = {c.d_B.get(), c.d_B2.get()};
+    const float scale = static_cast<float>(didt) * static_cast<float>(kMu0Over4Pi);
+    const int n_blocks = (n_place + block_k - 1) / block_k;
+    auto coil_stage = [&](int b) {
+        const int first = b * block_k, k = std::min(block_k, n_place - first), buf = b % 2;
+        dadt_for_frames(c, tf.data() + 16 * static_cast<std::size_t>(first), k, K, pos_m, mom, n_dip, scale, nodal[buf],
+                        c.s2, c.coil_blocks);
+        element_pass(nodal[buf], c.tets.get(), c.g32.get(), c.neg_vc.get(), m, k, K, nullptr, c.d_q.get(),
+                     c.corner_pos.get(), c.s2);
+        node_rhs(c.d_q.get(), c.n2c_ptr.get(), n, k, K, c.ground, rhs[buf], nullptr, c.s2);
+        check(hipEventRecord(c.coil_done[buf], c.s2), "coil event");
+    };
+    SolveOptions opt;
+    if (sopt) {
+        opt.tolerance = sopt->tolerance;
+        opt.max_iters = sopt->max_iters;
+        opt.max_refine = sopt->max_refine;
+        opt.drop_tol = sopt->drop_tol;
+        opt.use_graph = sopt->use_graph != 0;
+        opt.check_every = std::max(1, sopt->check_every);
+    }
+    const bool warm = sopt && sopt->warm_start == 1;
+    const int mode = sopt ? sopt->recovery : CN_RECOVERY_RAW;
+    const RecoveryOp* rec = mode == CN_RECOVERY_RAW ? nullptr : ensure_recovery(c, mode);
+    c.d_X.ensure(nr * K, "solution");
+    c.d_magn.ensure(m * K, "magnE");
+    const hipStream_t s = c.s;
+    try {
+        coil_stage(0);
+        for (int b = 0; b < n_blocks; ++b) {
+            if (b + 1 < n_blocks) coil_stage(b + 1);
+            const int first = b * block_k, k = std::min(block_k, n_place - first), buf = b % 2;
+            check(hipStreamWaitEvent(s, c.coil_done[buf], 0), "wait for the coil stage");
+            const double* dx0 = nullptr;
+            if (warm) {
+                c.d_x0.ensure(nr * K, "x0");
+                c.warm.initial_guess(rhs[buf], k, K, c.d_x0.get(), s);
+                dx0 = c.d_x0.get();
+            }
+            const SolveStats st = c.solver->solve(rhs[buf], c.d_X.get(), dx0, k, opt);
+            if (rec == nullptr) {
+                reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), nodal[buf], K, m, k, K, c.ground, nullptr,
+                            c.d_magn.get(), s);
+            } else if (rec->mode == kRecoveryHarmonic) {
+                c.d_slots.ensure(3 * rec->n_slots * K, "recovered slots");
+                apply_harmonic(*rec, c.d_X.get(), c.ground, nodal[buf], m, k, K, c.d_slots.get(), nullptr,
+                               c.d_magn.get(), s);
+            } else {
+                c.d_Eraw.ensure(3 * m * K, "raw E");
+                c.d_slots.ensure(3 * rec->n_slots * K, "recovered slots");
+                reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), nodal[buf], K, m, k, K, c.ground, c.d_Eraw.get(),
+                            c.d_magn.get(), s);
+                apply_spr(*rec, c.d_Eraw.get(), m, k, K, c.d_slots.get(), nullptr, c.d_magn.get(), s);
+            }
+            if (metrics_out)
+                region_metrics(c.d_magn.get(), m, k, K, c.region_idx.get(), c.n_region, c.vols32.get(), c.bary.get(),
+                               c.metrics_ws, metrics_out + static_cast<std::size_t>(kMetricsLen) * first, s);
+            sync(s, "block outputs");
+            for (int i = 0; i < k; ++i) {
+                if (iters_out) iters_out[first + i] = st.column_iterations[i];
+                if (resid_out) resid_out[first + i] = st.relative_residual[i];
+            }
+            if (block_iters_out) block_iters_out[b] = st.iterations;
+            for (int i = 0; i < k; ++i) {
+                if (!(st.relative_residual[i] <= opt.tolerance)) {
+                    char msg[256];
+                    std::snprintf(msg, sizeof msg,
+                                  "PCG did not converge (placement %d of the sweep): %d iterations, relative residual "
+                                  "%.3e > tolerance %.3e",
+                                  first + i, st.column_iterations[i] ? st.column_iterations[i] : st.iterations,
+                                  st.relative_residual[i], opt.tolerance);
+                    throw NotConverged(msg);
+                }
+            }
+        }
+    } catch (...) {
+        (void)hipStreamSynchronize(c.s2);  // the next coil stage may still be writing its buffers
+        (void)hipStreamSynchronize(s);
+        throw;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1162,6 +1292,7 @@ int32_t cn_context_get_tuning(const cn_context* c, cn_tuning* o) {
         o->lower_ratio = c->opts.lower_ratio;
         o->fine_sub = c->solver->fine_sub_setting();
         o->coarse_sub = c->solver->coarse_sub_setting();
+        o->coil_blocks = c->coil_blocks;
     });
 }
 
@@ -1174,7 +1305,9 @@ int32_t cn_context_set_tuning(cn_context* c, const cn_tuning* t) {
                     "fine_sub must be 0 or a power of two <= 64");
         cn::require(t->coarse_sub >= 0 && t->coarse_sub <= 64 && (t->coarse_sub & (t->coarse_sub - 1)) == 0,
                     "coarse_sub must be 0 or a power of two <= 64");
+        cn::require(t->coil_blocks >= 0, "coil_blocks must be >= 0");
         cn::check(hipStreamSynchronize(c->s), "sync before retuning");
+        c->coil_blocks = t->coil_blocks;
         c->hier.cheb = cn::chebyshev(t->degree, t->lower_ratio);
         c->opts.degree = t->degree;
         c->opts.lower_ratio = t->lower_ratio;
@@ -1296,6 +1429,17 @@ int32_t cn_context_skin_normals(const cn_context* c, double* out) {
         cn::require(c != nullptr && out != nullptr, "null argument");
         cn::d2h(out, c->skin_n.get(), 3 * c->n_skin, c->s);
         cn::sync(c->s, "skin normals readback");
+    });
+}
+
+int32_t cn_solve_sweep(cn_context* c, const double* dip_pos_m, const double* dip_moment, int64_t n_dip,
+                       const double* centers_mm, const double* handles_mm, const double* distances_mm,
+                       int32_t n_place, int32_t block_k, double didt, const cn_solve_options* opt, double* metrics,
+                       int32_t* iterations, double* relative_residual, int32_t* block_iterations) {
+    return guarded([&] {
+        cn::require(c != nullptr && dip_pos_m && dip_moment && centers_mm && distances_mm, "null argument");
+        solve_sweep(*c, dip_pos_m, dip_moment, n_dip, centers_mm, handles_mm, distances_mm, n_place, block_k, didt,
+                    opt, metrics, iterations, relative_residual, block_iterations);
     });
 }
 

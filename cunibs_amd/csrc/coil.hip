@@ -267,8 +267,14 @@ __global__ void __launch_bounds__(kDadtBlock) dadt_parallel_kernel(const float* 
     const int c = blockIdx.y;
     const float* dp = dip + static_cast<std::int64_t>(c) * n_dip * 8;
     const double ox = origin[3 * c], oy = origin[3 * c + 1], oz = origin[3 * c + 2];
-    const std::int64_t T = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
-    const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    // A thread's nodes are t, t + NT, t + 2 NT, t + 3 NT; a grid smaller than NT threads walks the
+    // slots in block-uniform steps (so every thread meets the same barriers).
+    const std::int64_t NT = (n + kNodesPerThread - 1) / kNodesPerThread;
+    const float dx = dirs[3 * c], dy = dirs[3 * c + 1], dz = dirs[3 * c + 2];
+    for (std::int64_t slot0 = static_cast<std::int64_t>(blockIdx.x) * blockDim.x; slot0 < NT;
+         slot0 += static_cast<std::int64_t>(gridDim.x) * blockDim.x) {
+    const std::int64_t t = slot0 + threadIdx.x;
+    const std::int64_t T = NT;
     float rx[kNodesPerThread], ry[kNodesPerThread], rz[kNodesPerThread], rn[kNodesPerThread];
     float s0[kNodesPerThread], sx[kNodesPerThread], sy[kNodesPerThread], sz[kNodesPerThread];
 #pragma unroll
@@ -307,7 +313,6 @@ __global__ void __launch_bounds__(kDadtBlock) dadt_parallel_kernel(const float* 
             }
         }
     }
-    const float dx = dirs[3 * c], dy = dirs[3 * c + 1], dz = dirs[3 * c + 2];
 #pragma unroll
     for (int k = 0; k < kNodesPerThread; ++k) {
         const std::int64_t node = t + k * T;
@@ -318,6 +323,7 @@ __global__ void __launch_bounds__(kDadtBlock) dadt_parallel_kernel(const float* 
             o[K] = scale * (dz * fx - dx * fz);
             o[2 * K] = scale * (dx * fy - dy * fx);
         }
+    }
     }
 }
 
@@ -404,10 +410,11 @@ void dadt_nbody(const float* dip, int n_dip, const double* nodes_mm, std::int64_
 }
 
 void dadt_nbody_parallel(const float* dip8, int n_dip, const double* nodes_mm, std::int64_t n, const double* origins,
-                         const float* dirs, int k, int K, float scale, float* out, hipStream_t s) {
+                         const float* dirs, int k, int K, float scale, float* out, hipStream_t s, int max_blocks) {
     if (n <= 0 || k <= 0) return;
     const std::int64_t threads = (n + kNodesPerThread - 1) / kNodesPerThread;
-    const unsigned gx = grid_for(threads, kDadtBlock);
+    unsigned gx = grid_for(threads, kDadtBlock);
+    if (max_blocks > 0) gx = std::max(1u, std::min(gx, static_cast<unsigned>((max_blocks + k - 1) / k)));
     dadt_parallel_kernel<<<dim3(gx, k), kDadtBlock, 0, s>>>(dip8, n_dip, nodes_mm, n, origins, dirs, scale, K, out);
 }
 

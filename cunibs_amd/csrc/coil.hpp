@@ -18,8 +18,10 @@ void dadt_nbody(const float* dip, int n_dip, const double* nodes_mm, std::int64_
                 int K, float scale, float* out, hipStream_t s);
 // All moments along one direction per placement (dirs: k x 3): dip8 per placement n_dip x
 // [-2 s(3) |s|^2 mu mu s(3)] FP32.
+// max_blocks > 0 caps the grid (all k placements together) and the blocks walk the nodes: a
+// smaller footprint for running beside other work.
 void dadt_nbody_parallel(const float* dip8, int n_dip, const double* nodes_mm, std::int64_t n, const double* origins,
-                         const float* dirs, int k, int K, float scale, float* out, hipStream_t s);
+                         const float* dirs, int k, int K, float scale, float* out, hipStream_t s, int max_blocks = 0);
 // Element average of the nodal dA/dt and the RHS corner terms; sigma_tag (K x n_sigma FP32, with
 // vols and tags) gives each placement its own conductivities instead of neg_vc. The corner term of
 // corner 4 e + i is stored at row corner_pos[4 e + i] of q: its position in node2corner order.

@@ -1311,11 +1311,14 @@ void BlockSolver::set_multi(const MultiOperator* m) {
 // graph exec destroyed and another instantiated in the same process crashes rocprofv3's kernel
 // tracer on this ROCm, and a retired exec costs only its host-side node list.
 void BlockSolver::drop_graphs() {
-    for (auto* set : {graphs_, graphs1_})
-        for (int i = 0; i < kNumBlockWidths; ++i) {
-            if (set[i]) retired_.push_back(set[i]);
-            set[i] = nullptr;
+    for (int i = 0; i < kNumBlockWidths; ++i) {
+        if (graphs_[i]) retired_.push_back(graphs_[i]);
+        graphs_[i] = nullptr;
+        for (auto& g : graphs1_[i]) {
+            if (g) retired_.push_back(g);
+            g = nullptr;
         }
+    }
 }
 
 void BlockSolver::ensure(int K) {
@@ -1449,12 +1452,12 @@ void BlockSolver::spmm_block_k(const int* state, const float* W) {
 }
 
 template <int K>
-void BlockSolver::iteration_k(const SolveOptions& opt, int j, int L) {
+void BlockSolver::iteration_k(const SolveOptions& opt, int j) {
     const std::int64_t nn = n();
     const std::int64_t rpb = gram_rows(nn);
     const int* st = state_.get();
     int* stw = state_.get();
-    const bool last = j == L - 1;
+    const bool last = j == kLag - 1;
     const float* w_in = wslot(j);
     float* w_out = last ? wslot(0) : wslot(j + 1);
     double* al = AL_.get() + static_cast<std::int64_t>(j) * K * K;
@@ -1473,9 +1476,9 @@ void BlockSolver::iteration_k(const SolveOptions& opt, int j, int L) {
     reduce_cols(partials_.get(), kGramBlocks, K * K, red_.get(), st, s_);
     dmat_kernel<K><<<1, small_threads<K>(), 0, s_>>>(red_.get(), kGramBlocks, TT_.get(), D_.get(), stw,
                                                 multi_ ? 1 : 0);
-    if (last)  // before this iteration's W update, which may overwrite slot 0
+    if (last)  // before this iteration's W update, which overwrites slot 0
         flush_x_kernel<K><<<grid_for(nn * col_groups<K>(), kRowThreads), kRowThreads, 0, s_>>>(
-            nn, wslot(0), wslot(1), wslot(2), wslot(3), AL_.get(), ran_.get(), L, X64_.get());
+            nn, wslot(0), wslot(1), wslot(2), wslot(3), AL_.get(), ran_.get(), kLag, X64_.get());
     update_w_kernel<K><<<grid_for(nn * col_groups<K>(), kRowThreads), kRowThreads, 0, s_>>>(nn, Z_.get(), D_.get(),
                                                                                            w_in, w_out, st);
     mark(3, st);
@@ -1485,9 +1488,16 @@ void BlockSolver::iteration_k(const SolveOptions& opt, int j, int L) {
 }
 
 template <int K>
-void BlockSolver::window_k(const SolveOptions& opt, int L) {
-    clear_ran_kernel<<<1, kWave, 0, s_>>>(ran_.get());
-    for (int j = 0; j < L; ++j) iteration_k<K>(opt, j, L);
+void BlockSolver::window_k(const SolveOptions& opt, int j0, int L) {
+    if (j0 == 0) clear_ran_kernel<<<1, kWave, 0, s_>>>(ran_.get());
+    for (int j = j0; j < j0 + L; ++j) iteration_k<K>(opt, j);
+}
+
+template <int K>
+void BlockSolver::flush_pending_k(int L) {
+    const std::int64_t nn = n();
+    flush_x_kernel<K><<<grid_for(nn * col_groups<K>(), kRowThreads), kRowThreads, 0, s_>>>(
+        nn, wslot(0), wslot(1), wslot(2), wslot(3), AL_.get(), ran_.get(), L, X64_.get());
 }
 
 template <int K>
@@ -1551,19 +1561,20 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
         d2d(W_.get(), Z_.get(), static_cast<std::size_t>(nk), s_);
         int slot = 0;
         while (kBlockWidths[slot] != K) ++slot;
-        if (opt.use_graph &&
-            (graphs_[slot] == nullptr || graphs1_[slot] == nullptr || graph_drop_[slot] != opt.drop_tol)) {
-            for (auto* gs : {&graphs_[slot], &graphs1_[slot]}) {
-                if (*gs) retired_.push_back(*gs);
-                *gs = nullptr;
-            }
-            for (const int L : {kLag, 1}) {
+        if (opt.use_graph && (graphs_[slot] == nullptr || graph_drop_[slot] != opt.drop_tol)) {
+            // a whole window (j0 = 0, L = kLag), then one iteration at each ring position
+            for (int v = -1; v < kLag; ++v) {
+                hipGraphExec_t& gx = v < 0 ? graphs_[slot] : graphs1_[slot][v];
+                if (gx) retired_.push_back(gx);
+                gx = nullptr;
                 hipGraph_t g = nullptr;
                 check(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal), "begin capture");
-                window_k<K>(opt, L);
+                if (v < 0)
+                    window_k<K>(opt, 0, kLag);
+                else
+                    window_k<K>(opt, v, 1);
                 check(hipStreamEndCapture(s_, &g), "end capture");
-                check(hipGraphInstantiate(L == 1 ? &graphs1_[slot] : &graphs_[slot], g, nullptr, nullptr, 0),
-                      "graph instantiate");
+                check(hipGraphInstantiate(&gx, g, nullptr, nullptr, 0), "graph instantiate");
                 (void)hipGraphDestroy(g);
             }
             graph_drop_[slot] = opt.drop_tol;
@@ -1573,20 +1584,23 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
         bool done = false;
         const int hint = round == 0 ? last_first_round_[slot] : 0;
         int reads = 0;
+        int pos = 0;  // ring position of the next iteration: the round's iteration count mod kLag
         while (launched < opt.max_iters) {
             int want = opt.check_every;
             // the first read after whole windows short of the hint, then after every iteration
             if (hint > 0) want = reads == 0 ? std::max(1, (hint - 1) / kLag * kLag) : (reads <= 4 ? 1 : opt.check_every);
             const int batch = std::min(want, opt.max_iters - launched);
             ++reads;
-            // windows of kLag iterations, then single-iteration windows for the remainder
+            // whole windows where the ring position is 0 and four or more remain, else one
+            // iteration at the current position
             for (int b = 0; b < batch;) {
-                const int L = batch - b >= kLag ? kLag : 1;
+                const int L = (pos == 0 && batch - b >= kLag) ? kLag : 1;
                 if (opt.use_graph)
-                    check(hipGraphLaunch(L == 1 ? graphs1_[slot] : graphs_[slot], s_), "graph launch");
+                    check(hipGraphLaunch(L == kLag ? graphs_[slot] : graphs1_[slot][pos], s_), "graph launch");
                 else
-                    window_k<K>(opt, L);
+                    window_k<K>(opt, pos, L);
                 b += L;
+                pos = (pos + L) % kLag;
             }
             launched += batch;
             d2h(host_state_, state_.get(), 2 + 2 * K, s_);
@@ -1597,6 +1611,7 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
                 break;
             }
         }
+        if (pos != 0) flush_pending_k<K>(pos);  // the round ended inside a window
         if (!done) total_iters = launched;
         if (round == 0) last_first_round_[slot] = total_iters - host_state[1];
         for (int c = 0; c < K; ++c)

@@ -124,13 +124,17 @@ class BlockSolver {
     template <int K>
     void vcycle_k(const float* B, float* X, const int* state);
     template <int K>
-    // Iteration j of a window of L: reads W slot j and writes slot j + 1, except the last, which
-    // first applies the window's deferred updates X += W_j a_j and then writes slot 0 (in place
-    // for L = 1), so every window starts from slot 0.
-    void iteration_k(const SolveOptions& opt, int j, int L);
-    // L (1..kLag) iterations.
+    // Iteration at ring position j (the round's iteration count mod kLag): reads W slot j and
+    // writes slot j + 1, except at the last position, which first applies the window's deferred
+    // updates X += W_j a_j and then writes slot 0. Windows are thus aligned to the round's own
+    // iteration count, whatever batches the host launches them in.
+    void iteration_k(const SolveOptions& opt, int j);
+    // the iterations at positions j0 .. j0 + L - 1 (clearing the window's slots when j0 = 0)
     template <int K>
-    void window_k(const SolveOptions& opt, int L);
+    void window_k(const SolveOptions& opt, int j0, int L);
+    // X += the deferred updates of a window cut short after L iterations (the round ended)
+    template <int K>
+    void flush_pending_k(int L);
     template <int K>
     void spmm_block_k(const int* state, const float* W);
     template <int K>
@@ -174,8 +178,8 @@ class BlockSolver {
     std::size_t stamp_marks() const;
     unsigned long long* stamps();
     void mark(int i, const int* state);
-    hipGraphExec_t graphs_[kNumBlockWidths] = {};   // a window of kLag iterations
-    hipGraphExec_t graphs1_[kNumBlockWidths] = {};  // a window of one iteration
+    hipGraphExec_t graphs_[kNumBlockWidths] = {};          // a whole window of kLag iterations
+    hipGraphExec_t graphs1_[kNumBlockWidths][kLag] = {};  // one iteration at each ring position
     double graph_drop_[kNumBlockWidths] = {};  // the drop tolerance each graph was captured with
     // Inner iterations the last solve at each width took in its first round: the next solve runs
     // the whole windows short of that count before its first convergence read, then reads after

@@ -468,16 +468,18 @@ class SolverContext:
         t = _lib.Tuning()
         check(lib().cn_context_get_tuning(self.handle, ctypes.byref(t)), "cn_context_get_tuning")
         return {"degree": int(t.degree), "lower_ratio": float(t.lower_ratio), "fine_sub": int(t.fine_sub),
-                "coarse_sub": int(t.coarse_sub)}
+                "coarse_sub": int(t.coarse_sub), "coil_blocks": int(t.coil_blocks)}
 
     def set_tuning(self, degree: int | None = None, lower_ratio: float | None = None,
-                   fine_sub: int | None = None, coarse_sub: int | None = None) -> None:
+                   fine_sub: int | None = None, coarse_sub: int | None = None,
+                   coil_blocks: int | None = None) -> None:
         """Change the settings of `tuning` between solves (None keeps the current value)."""
         cur = self.tuning
         t = _lib.Tuning(cur["degree"] if degree is None else int(degree),
                         cur["lower_ratio"] if lower_ratio is None else float(lower_ratio),
                         cur["fine_sub"] if fine_sub is None else int(fine_sub),
-                        cur["coarse_sub"] if coarse_sub is None else int(coarse_sub))
+                        cur["coarse_sub"] if coarse_sub is None else int(coarse_sub),
+                        cur["coil_blocks"] if coil_blocks is None else int(coil_blocks))
         check(lib().cn_context_set_tuning(self.handle, ctypes.byref(t)), "cn_context_set_tuning")
 
     def bench_spmm(self, K: int, reps: int = 20, mode: int = 1, sub: int = 0) -> float:
@@ -535,6 +537,56 @@ class SolverContext:
         ms = ctypes.c_double()
         check(lib().cn_bench_copy(self.handle, int(nbytes), reps, ctypes.byref(ms)), "cn_bench_copy")
         return float(ms.value)
+
+
+def solve_sweep(
+    ctx: SolverContext,
+    dip_pos_m: np.ndarray,
+    dip_moment: np.ndarray,
+    sites: Sequence[tuple[np.ndarray, np.ndarray, float]],
+    didt: float,
+    block_k: int,
+    *,
+    tolerance: float = DEFAULT_TOLERANCE,
+    max_iters: int = DEFAULT_MAX_ITERS,
+    use_graph: bool = True,
+    recovery: str = "raw",
+    warm_start: bool = False,
+) -> dict:
+    """Every placement of `sites` in blocks of `block_k`, summaries only: the next block's dA/dt
+    and right-hand side are computed while the current block solves. Returns "metrics"
+    (n, METRICS_LEN), "iterations" and "relative_residual" (n) and "block_iterations" (per block),
+    each block's numbers those of `solve_placements_block` on the same placements."""
+    _blocks()
+    n_place = len(sites)
+    if n_place < 1:
+        raise ValueError("solve_sweep: no placements")
+    if block_k < 1 or block_k > MAX_BLOCK:
+        raise ValueError(f"solve_sweep: block_k={block_k} must be in [1, {MAX_BLOCK}]")
+    if recovery not in _lib.RECOVERY_CODES:
+        raise ValueError(f"unknown recovery {recovery!r}")
+    pos = np.ascontiguousarray(dip_pos_m, np.float64).reshape(-1, 3)
+    mom = np.ascontiguousarray(dip_moment, np.float64).reshape(-1, 3)
+    if pos.shape != mom.shape:
+        raise ValueError("dipole positions and moments must have the same shape")
+    centers = np.ascontiguousarray([np.asarray(s[0], np.float64).reshape(3) for s in sites])
+    handles = np.ascontiguousarray([np.asarray(s[1], np.float64).reshape(3) for s in sites])
+    dists = np.ascontiguousarray([float(s[2]) for s in sites], np.float64)
+    n_blocks = (n_place + block_k - 1) // block_k
+    met = np.empty((n_place, _lib.METRICS_LEN), np.float64)
+    it = np.empty(n_place, np.int32)
+    rel = np.empty(n_place, np.float64)
+    bit = np.empty(n_blocks, np.int32)
+    opt = _lib.default_solve_options()
+    opt.recovery = _lib.RECOVERY_CODES[recovery]
+    opt.warm_start = int(bool(warm_start))
+    opt.tolerance = float(tolerance)
+    opt.max_iters = int(max_iters)
+    opt.use_graph = int(use_graph)
+    check(lib().cn_solve_sweep(ctx.handle, ptr(pos), ptr(mom), pos.shape[0], ptr(centers), ptr(handles), ptr(dists),
+                               n_place, int(block_k), float(didt), ctypes.byref(opt), ptr(met), ptr(it), ptr(rel),
+                               ptr(bit)), "cn_solve_sweep")
+    return {"metrics": met, "iterations": it, "relative_residual": rel, "block_iterations": bit}
 
 
 def solve_placements_block(

@@ -86,6 +86,9 @@ typedef struct {
                            clamped to one wavefront per row at wide blocks */
     int32_t coarse_sub; /* the same for the smoothing and residual kernels of every coarser
                            level (0 = chosen from each level's shape) */
+    int32_t coil_blocks; /* cn_solve_sweep: workgroups of the next block's dA/dt while it overlaps
+                            the current block's solve (0 = the full grid); fewer leave the solve
+                            more of the device */
 } cn_tuning;
 
 int32_t cn_context_get_tuning(const cn_context* ctx, cn_tuning* out);
@@ -262,6 +265,19 @@ int32_t cn_solve_block(cn_context* ctx, const double* dip_pos_m, const double* d
                        int64_t n_dip, const double* centers_mm, const double* handles_mm,
                        const double* distances_mm, int32_t k, double didt,
                        const cn_solve_options* opt, const double* x0, cn_block_outputs* out);
+
+/* A sweep of n_place placements in blocks of block_k (the loop of Subject.iter_simulate,
+ * simulation.py, over fem/solve.py:865-979 per block), returning only the per-placement summaries:
+ * metrics (n_place, CN_METRICS_LEN), iterations and relative_residual (n_place), block_iterations
+ * (one per block); any may be NULL. Every block's numerics are cn_solve_block's for the same
+ * placements; the library pipelines them, computing the next block's dA/dt and right-hand side on
+ * a second stream while the current block solves. opt->sigma, opt->collect and warm_start = 2 are
+ * not accepted here (CN_EINVAL); warm_start = 1 uses the context's modes. */
+int32_t cn_solve_sweep(cn_context* ctx, const double* dip_pos_m, const double* dip_moment, int64_t n_dip,
+                       const double* centers_mm, const double* handles_mm, const double* distances_mm,
+                       int32_t n_place, int32_t block_k, double didt, const cn_solve_options* opt,
+                       double* metrics, int32_t* iterations, double* relative_residual,
+                       int32_t* block_iterations);
 
 /* ---- kernel-level entry points (reference binding each one stands in for) ---- */
 

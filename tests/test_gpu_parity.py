@@ -333,3 +333,27 @@ def test_dadt_general_and_parallel_moment_paths(gctx, coil, oracle_mod, octx):
         assert rel(ex["dadt_elm"][0], ref["dadt_elm"]) < 1e-5, name
         out[name] = ex["dadt_elm"][0]
     assert rel(out["general"], out["parallel"]) < 1e-5
+
+
+@pytest.mark.parametrize("warm", [False, True])
+def test_pipelined_sweep_equals_blockwise_solves(gctx, coil, warm):
+    """cn_solve_sweep overlaps the next block's dA/dt and RHS with the current block's solve; its
+    summaries equal those of solving the same blocks one at a time, bit for bit (a ragged last
+    block included)."""
+    from cunibs_amd import fem, synthetic
+    sites = synthetic.cap_sites(11, seed=17)
+    if warm:
+        gctx.warm_reset(16)
+        fem.solve_placements_block(gctx, coil.positions_m, coil.moments, synthetic.cap_sites(16, seed=18), 1e6,
+                                   metrics=False, magnitude=False, collect=True)
+        gctx.warm_finalize()
+    sweep = fem.solve_sweep(gctx, coil.positions_m, coil.moments, sites, 1e6, block_k=4, warm_start=warm)
+    ref = []
+    for i in range(0, len(sites), 4):
+        ref.extend(fem.solve_placements_block(gctx, coil.positions_m, coil.moments, sites[i:i + 4], 1e6,
+                                              magnitude=False, warm_start=warm))
+    np.testing.assert_array_equal(sweep["metrics"], np.stack([r.metrics for r in ref]))
+    np.testing.assert_array_equal(sweep["iterations"], [r.iterations for r in ref])
+    np.testing.assert_array_equal(sweep["relative_residual"], [r.relative_residual for r in ref])
+    assert len(sweep["block_iterations"]) == 3
+    assert np.all(sweep["relative_residual"] <= 1e-6)
