@@ -273,57 +273,57 @@ __global__ void __launch_bounds__(kDadtBlock) dadt_parallel_kernel(const float* 
     const float dx = dirs[3 * c], dy = dirs[3 * c + 1], dz = dirs[3 * c + 2];
     for (std::int64_t slot0 = static_cast<std::int64_t>(blockIdx.x) * blockDim.x; slot0 < NT;
          slot0 += static_cast<std::int64_t>(gridDim.x) * blockDim.x) {
-    const std::int64_t t = slot0 + threadIdx.x;
-    const std::int64_t T = NT;
-    float rx[kNodesPerThread], ry[kNodesPerThread], rz[kNodesPerThread], rn[kNodesPerThread];
-    float s0[kNodesPerThread], sx[kNodesPerThread], sy[kNodesPerThread], sz[kNodesPerThread];
+        const std::int64_t t = slot0 + threadIdx.x;
+        const std::int64_t T = NT;
+        float rx[kNodesPerThread], ry[kNodesPerThread], rz[kNodesPerThread], rn[kNodesPerThread];
+        float s0[kNodesPerThread], sx[kNodesPerThread], sy[kNodesPerThread], sz[kNodesPerThread];
 #pragma unroll
-    for (int k = 0; k < kNodesPerThread; ++k) {
-        const std::int64_t node = t + k * T;
-        rx[k] = ry[k] = rz[k] = rn[k] = 0.f;
-        if (node < n) {
-            rx[k] = static_cast<float>(nodes_mm[3 * node] * 1e-3 - ox);
-            ry[k] = static_cast<float>(nodes_mm[3 * node + 1] * 1e-3 - oy);
-            rz[k] = static_cast<float>(nodes_mm[3 * node + 2] * 1e-3 - oz);
-            rn[k] = fmaf(rz[k], rz[k], fmaf(ry[k], ry[k], rx[k] * rx[k]));
+        for (int k = 0; k < kNodesPerThread; ++k) {
+            const std::int64_t node = t + k * T;
+            rx[k] = ry[k] = rz[k] = rn[k] = 0.f;
+            if (node < n) {
+                rx[k] = static_cast<float>(nodes_mm[3 * node] * 1e-3 - ox);
+                ry[k] = static_cast<float>(nodes_mm[3 * node + 1] * 1e-3 - oy);
+                rz[k] = static_cast<float>(nodes_mm[3 * node + 2] * 1e-3 - oz);
+                rn[k] = fmaf(rz[k], rz[k], fmaf(ry[k], ry[k], rx[k] * rx[k]));
+            }
+            s0[k] = sx[k] = sy[k] = sz[k] = 0.f;
         }
-        s0[k] = sx[k] = sy[k] = sz[k] = 0.f;
-    }
-    for (int base = 0; base < n_dip; base += kDipTile) {
-        const int tile = min(kDipTile, n_dip - base);
-        __syncthreads();
-        for (int i = threadIdx.x; i < tile; i += kDadtBlock) {
-            const float* d = dp + static_cast<std::int64_t>(base + i) * 8;
-            s_a[i] = make_float4(d[0], d[1], d[2], d[3]);
-            s_m[i] = make_float4(d[4], d[5], d[6], d[7]);
-        }
-        __syncthreads();
+        for (int base = 0; base < n_dip; base += kDipTile) {
+            const int tile = min(kDipTile, n_dip - base);
+            __syncthreads();
+            for (int i = threadIdx.x; i < tile; i += kDadtBlock) {
+                const float* d = dp + static_cast<std::int64_t>(base + i) * 8;
+                s_a[i] = make_float4(d[0], d[1], d[2], d[3]);
+                s_m[i] = make_float4(d[4], d[5], d[6], d[7]);
+            }
+            __syncthreads();
 #pragma unroll 4
-        for (int j = 0; j < tile; ++j) {
-            const float4 a = s_a[j], m = s_m[j];
+            for (int j = 0; j < tile; ++j) {
+                const float4 a = s_a[j], m = s_m[j];
 #pragma unroll
-            for (int k = 0; k < kNodesPerThread; ++k) {
-                const float q = fmaf(rz[k], a.z, fmaf(ry[k], a.y, fmaf(rx[k], a.x, a.w)));
-                const float inv = rsq_normal(q + rn[k]);
-                const float w = inv * inv * inv;
-                s0[k] = fmaf(w, m.x, s0[k]);
-                sx[k] = fmaf(w, m.y, sx[k]);
-                sy[k] = fmaf(w, m.z, sy[k]);
-                sz[k] = fmaf(w, m.w, sz[k]);
+                for (int k = 0; k < kNodesPerThread; ++k) {
+                    const float q = fmaf(rz[k], a.z, fmaf(ry[k], a.y, fmaf(rx[k], a.x, a.w)));
+                    const float inv = rsq_normal(q + rn[k]);
+                    const float w = inv * inv * inv;
+                    s0[k] = fmaf(w, m.x, s0[k]);
+                    sx[k] = fmaf(w, m.y, sx[k]);
+                    sy[k] = fmaf(w, m.z, sy[k]);
+                    sz[k] = fmaf(w, m.w, sz[k]);
+                }
             }
         }
-    }
 #pragma unroll
-    for (int k = 0; k < kNodesPerThread; ++k) {
-        const std::int64_t node = t + k * T;
-        if (node < n) {
-            const float fx = s0[k] * rx[k] - sx[k], fy = s0[k] * ry[k] - sy[k], fz = s0[k] * rz[k] - sz[k];
-            float* o = out + node * 3 * K + c;
-            o[0] = scale * (dy * fz - dz * fy);
-            o[K] = scale * (dz * fx - dx * fz);
-            o[2 * K] = scale * (dx * fy - dy * fx);
+        for (int k = 0; k < kNodesPerThread; ++k) {
+            const std::int64_t node = t + k * T;
+            if (node < n) {
+                const float fx = s0[k] * rx[k] - sx[k], fy = s0[k] * ry[k] - sy[k], fz = s0[k] * rz[k] - sz[k];
+                float* o = out + node * 3 * K + c;
+                o[0] = scale * (dy * fz - dz * fy);
+                o[K] = scale * (dz * fx - dx * fz);
+                o[2 * K] = scale * (dx * fy - dy * fx);
+            }
         }
-    }
     }
 }
 
