@@ -167,9 +167,9 @@ __global__ void place_frame_kernel(const double* __restrict__ centers, const dou
 
 constexpr int kDadtBlock = 256;
 constexpr int kDipTile = 256;
-constexpr int kNodesPerThread = 4;
+constexpr int kNodesPerThread = 8;
 
-// dadt.cu:27-104 restated for CDNA: dipole tiles staged in LDS, 4 nodes per thread in registers,
+// dadt.cu:27-104 restated for CDNA: dipole tiles staged in LDS, 8 nodes per thread in registers,
 // one placement per blockIdx.y. w_j = |r - s_j|^-3 from rn + sn - 2 r.s, FP32, on the vector ALUs.
 // The reciprocal square root is the bare v_rsq_f32 (rsq_normal): rsqrtf wraps the same instruction
 // in a denormal rescale (compare, two multiplies, two selects per pair), and a squared coil-to-node
@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(kDadtBlock) dadt_parallel_kernel(const float* 
     const int c = blockIdx.y;
     const float* dp = dip + static_cast<std::int64_t>(c) * n_dip * 8;
     const double ox = origin[3 * c], oy = origin[3 * c + 1], oz = origin[3 * c + 2];
-    // A thread's nodes are t, t + NT, t + 2 NT, t + 3 NT; a grid smaller than NT threads walks the
+    // A thread's nodes are t, t + NT, t + 2 NT, ...; a grid smaller than NT threads walks the
     // slots in block-uniform steps (so every thread meets the same barriers).
     const std::int64_t NT = (n + kNodesPerThread - 1) / kNodesPerThread;
     const float dx = dirs[3 * c], dy = dirs[3 * c + 1], dz = dirs[3 * c + 2];
