@@ -298,7 +298,7 @@ def main() -> int:
         log(f"warm basis: {args.warm_basis} training solves, rank {rank}, {train_s:.2f}s")
         if args.autotune:
             overlap = autotune.tune_overlap(ctx, pos, mom, synthetic.cap_sites(8 * k, seed=19), 1e6, k,
-                                            candidates=(256, 384, 512, 768, 1024, 0), warm_start=True)
+                                            candidates=(128, 256, 512, 768, 1024, 0), warm_start=True)
             log(f"overlap (warm): coil_blocks {overlap['coil_blocks']}, rates "
                 f"{ {c: round(r, 1) for c, r in overlap['rates'].items()} }")
         ctx.timing_enable(True)

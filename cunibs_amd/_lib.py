@@ -95,7 +95,7 @@ _SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "cn_context_level": (_I32, [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "cn_context_skin_normals": (_I32, [_P, _P]),
     "cn_solve_block": (_I32, [_P, _P, _P, _I64, _P, _P, _P, _I32, _D, _P, _P, _P]),
-    "cn_solve_sweep": (_I32, [_P, _P, _P, _I64, _P, _P, _P, _I32, _I32, _D, _P, _P, _P, _P, _P]),
+    "cn_solve_sweep": (_I32, [_P, _P, _P, _I64, _P, _P, _P, _I32, _I32, _D, _P, _P, _P, _P, _P, _P]),
     "cn_recovery_build": (_I32, [_P, _I32, _P, _P]),
     "cn_warm_reset": (_I32, [_P, _I32]),
     "cn_warm_count": (_I32, [_P, _P]),

@@ -1104,8 +1104,8 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
 // cn_solve_block for the same block.
 void solve_sweep(cn_context& c, const double* pos_m, const double* mom, std::int64_t n_dip, const double* centers,
                  const double* handles, const double* dists, int n_place, int block_k, double didt,
-                 const cn_solve_options* sopt, double* metrics_out, std::int32_t* iters_out, double* resid_out,
-                 std::int32_t* block_iters_out) {
+                 const cn_solve_options* sopt, double* transforms_out, double* metrics_out, std::int32_t* iters_out,
+                 double* resid_out, std::int32_t* block_iters_out) {
     using namespace cn;
     require(n_place >= 1, "no placements");
     require(block_k >= 1 && block_k <= kMaxBlock, "block_k must be in [1, 32]");
@@ -1126,6 +1126,7 @@ void solve_sweep(cn_context& c, const double* pos_m, const double* mom, std::int
                                       std::to_string(n_place) +
                                       ": handle_mm lies on the outward normal through the scalp projection of "
                                       "center_mm, so the coil's rotation about the normal is undefined.");
+    if (transforms_out) std::memcpy(transforms_out, tf.data(), sizeof(double) * tf.size());
     if (c.s2 == nullptr) check(hipStreamCreateWithFlags(&c.s2, hipStreamNonBlocking), "create stream");
     for (auto& e : c.coil_done)
         if (e == nullptr) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "coil event");
@@ -1138,13 +1139,16 @@ void solve_sweep(cn_context& c, const double* pos_m, const double* mom, std::int
     float* const rhs[2] = {c.d_B.get(), c.d_B2.get()};
     const float scale = static_cast<float>(didt) * static_cast<float>(kMu0Over4Pi);
     const int n_blocks = (n_place + block_k - 1) / block_k;
+    // Each block is laid out at its own padded width (a ragged last block may be narrower), as
+    // the solver takes it.
     auto coil_stage = [&](int b) {
         const int first = b * block_k, k = std::min(block_k, n_place - first), buf = b % 2;
-        dadt_for_frames(c, tf.data() + 16 * static_cast<std::size_t>(first), k, K, pos_m, mom, n_dip, scale, nodal[buf],
-                        c.s2, c.coil_blocks);
-        element_pass(nodal[buf], c.tets.get(), c.g32.get(), c.neg_vc.get(), m, k, K, nullptr, c.d_q.get(),
+        const int Kb = padded_width(k);
+        dadt_for_frames(c, tf.data() + 16 * static_cast<std::size_t>(first), k, Kb, pos_m, mom, n_dip, scale,
+                        nodal[buf], c.s2, c.coil_blocks);
+        element_pass(nodal[buf], c.tets.get(), c.g32.get(), c.neg_vc.get(), m, k, Kb, nullptr, c.d_q.get(),
                      c.corner_pos.get(), c.s2);
-        node_rhs(c.d_q.get(), c.n2c_ptr.get(), n, k, K, c.ground, rhs[buf], nullptr, c.s2);
+        node_rhs(c.d_q.get(), c.n2c_ptr.get(), n, k, Kb, c.ground, rhs[buf], nullptr, c.s2);
         check(hipEventRecord(c.coil_done[buf], c.s2), "coil event");
     };
     SolveOptions opt;
@@ -1167,30 +1171,31 @@ void solve_sweep(cn_context& c, const double* pos_m, const double* mom, std::int
         for (int b = 0; b < n_blocks; ++b) {
             if (b + 1 < n_blocks) coil_stage(b + 1);
             const int first = b * block_k, k = std::min(block_k, n_place - first), buf = b % 2;
+            const int Kb = padded_width(k);
             check(hipStreamWaitEvent(s, c.coil_done[buf], 0), "wait for the coil stage");
             const double* dx0 = nullptr;
             if (warm) {
                 c.d_x0.ensure(nr * K, "x0");
-                c.warm.initial_guess(rhs[buf], k, K, c.d_x0.get(), s);
+                c.warm.initial_guess(rhs[buf], k, Kb, c.d_x0.get(), s);
                 dx0 = c.d_x0.get();
             }
             const SolveStats st = c.solver->solve(rhs[buf], c.d_X.get(), dx0, k, opt);
             if (rec == nullptr) {
-                reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), nodal[buf], K, m, k, K, c.ground, nullptr,
+                reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), nodal[buf], Kb, m, k, Kb, c.ground, nullptr,
                             c.d_magn.get(), s);
             } else if (rec->mode == kRecoveryHarmonic) {
                 c.d_slots.ensure(3 * rec->n_slots * K, "recovered slots");
-                apply_harmonic(*rec, c.d_X.get(), c.ground, nodal[buf], m, k, K, c.d_slots.get(), nullptr,
+                apply_harmonic(*rec, c.d_X.get(), c.ground, nodal[buf], m, k, Kb, c.d_slots.get(), nullptr,
                                c.d_magn.get(), s);
             } else {
                 c.d_Eraw.ensure(3 * m * K, "raw E");
                 c.d_slots.ensure(3 * rec->n_slots * K, "recovered slots");
-                reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), nodal[buf], K, m, k, K, c.ground, c.d_Eraw.get(),
-                            c.d_magn.get(), s);
-                apply_spr(*rec, c.d_Eraw.get(), m, k, K, c.d_slots.get(), nullptr, c.d_magn.get(), s);
+                reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), nodal[buf], Kb, m, k, Kb, c.ground,
+                            c.d_Eraw.get(), c.d_magn.get(), s);
+                apply_spr(*rec, c.d_Eraw.get(), m, k, Kb, c.d_slots.get(), nullptr, c.d_magn.get(), s);
             }
             if (metrics_out)
-                region_metrics(c.d_magn.get(), m, k, K, c.region_idx.get(), c.n_region, c.vols32.get(), c.bary.get(),
+                region_metrics(c.d_magn.get(), m, k, Kb, c.region_idx.get(), c.n_region, c.vols32.get(), c.bary.get(),
                                c.metrics_ws, metrics_out + static_cast<std::size_t>(kMetricsLen) * first, s);
             sync(s, "block outputs");
             for (int i = 0; i < k; ++i) {
@@ -1434,12 +1439,13 @@ int32_t cn_context_skin_normals(const cn_context* c, double* out) {
 
 int32_t cn_solve_sweep(cn_context* c, const double* dip_pos_m, const double* dip_moment, int64_t n_dip,
                        const double* centers_mm, const double* handles_mm, const double* distances_mm,
-                       int32_t n_place, int32_t block_k, double didt, const cn_solve_options* opt, double* metrics,
-                       int32_t* iterations, double* relative_residual, int32_t* block_iterations) {
+                       int32_t n_place, int32_t block_k, double didt, const cn_solve_options* opt,
+                       double* transforms, double* metrics, int32_t* iterations, double* relative_residual,
+                       int32_t* block_iterations) {
     return guarded([&] {
         cn::require(c != nullptr && dip_pos_m && dip_moment && centers_mm && distances_mm, "null argument");
         solve_sweep(*c, dip_pos_m, dip_moment, n_dip, centers_mm, handles_mm, distances_mm, n_place, block_k, didt,
-                    opt, metrics, iterations, relative_residual, block_iterations);
+                    opt, transforms, metrics, iterations, relative_residual, block_iterations);
     });
 }
 

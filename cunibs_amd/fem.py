@@ -554,9 +554,10 @@ def solve_sweep(
     warm_start: bool = False,
 ) -> dict:
     """Every placement of `sites` in blocks of `block_k`, summaries only: the next block's dA/dt
-    and right-hand side are computed while the current block solves. Returns "metrics"
-    (n, METRICS_LEN), "iterations" and "relative_residual" (n) and "block_iterations" (per block),
-    each block's numbers those of `solve_placements_block` on the same placements."""
+    and right-hand side are computed while the current block solves. Returns "transforms"
+    (n, 4, 4), "metrics" (n, METRICS_LEN), "iterations" and "relative_residual" (n) and
+    "block_iterations" (per block), each block's numbers those of `solve_placements_block` on the
+    same placements."""
     _blocks()
     n_place = len(sites)
     if n_place < 1:
@@ -573,6 +574,7 @@ def solve_sweep(
     handles = np.ascontiguousarray([np.asarray(s[1], np.float64).reshape(3) for s in sites])
     dists = np.ascontiguousarray([float(s[2]) for s in sites], np.float64)
     n_blocks = (n_place + block_k - 1) // block_k
+    tf = np.empty((n_place, 4, 4), np.float64)
     met = np.empty((n_place, _lib.METRICS_LEN), np.float64)
     it = np.empty(n_place, np.int32)
     rel = np.empty(n_place, np.float64)
@@ -584,9 +586,9 @@ def solve_sweep(
     opt.max_iters = int(max_iters)
     opt.use_graph = int(use_graph)
     check(lib().cn_solve_sweep(ctx.handle, ptr(pos), ptr(mom), pos.shape[0], ptr(centers), ptr(handles), ptr(dists),
-                               n_place, int(block_k), float(didt), ctypes.byref(opt), ptr(met), ptr(it), ptr(rel),
-                               ptr(bit)), "cn_solve_sweep")
-    return {"metrics": met, "iterations": it, "relative_residual": rel, "block_iterations": bit}
+                               n_place, int(block_k), float(didt), ctypes.byref(opt), ptr(tf), ptr(met), ptr(it),
+                               ptr(rel), ptr(bit)), "cn_solve_sweep")
+    return {"transforms": tf, "metrics": met, "iterations": it, "relative_residual": rel, "block_iterations": bit}
 
 
 def solve_placements_block(

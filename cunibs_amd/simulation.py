@@ -27,6 +27,8 @@ _DEFAULT_REGION = "gray_matter"
 _SUMMARY_FOCALITY_FRACS = (0.5, 0.75)
 _SUMMARY_PERCENTILES = (50.0, 95.0, 99.0, 99.9)
 FOCALITY_ANCHOR_PERCENTILE = 99.9  # metrics.py
+# Blocks per pipelined run when iter_simulate streams summaries only.
+SWEEP_BLOCKS = 8
 
 
 def _as_point(value) -> np.ndarray:
@@ -321,6 +323,22 @@ class Subject:
                      nodal: bool = False) -> Iterator[FieldResult]:
         ctx = self.context
         chunk_k = fem.MAX_BLOCK if block_k is None else max(1, min(fem.MAX_BLOCK, int(block_k)))
+        if not (magnitude or vectors or potential or nodal) and len(sites) > chunk_k:
+            # Summaries only: the pipelined sweep (the next block's dA/dt and RHS overlap the
+            # current block's solve), in runs of SWEEP_BLOCKS blocks so results still stream.
+            run = SWEEP_BLOCKS * chunk_k
+            for start in range(0, len(sites), run):
+                part = sites[start:start + run]
+                args = [(s.center_mm, s.handle_mm, s.distance_mm) for s in part]
+                out = fem.solve_sweep(ctx, coil.positions_m, coil.moments, args, didt, chunk_k, recovery=recovery)
+                for i, site in enumerate(part):
+                    yield FieldResult(
+                        summary=fem.metrics_dict(out["metrics"][i]), magnE=None, E=None, v=None,
+                        transform=out["transforms"][i], placement=site, coil_name=coil.name, didt=didt, vols=None,
+                        tet_tags=None, barycenters_mm=None, recovery=recovery, iterations=int(out["iterations"][i]),
+                        relative_residual=float(out["relative_residual"][i]), n_nodes=None, E_slots=None,
+                        slot_node=None, slot_tag=None, tet_nodes=None)
+            return
         host = self._host_metric_inputs() if (magnitude or nodal) else (None, None, None)
         slot_node, slot_tag = self._slot_maps(recovery) if nodal else (None, None)
         tets = ctx.tet_nodes if nodal and slot_tag is None else None

@@ -268,15 +268,15 @@ int32_t cn_solve_block(cn_context* ctx, const double* dip_pos_m, const double* d
 
 /* A sweep of n_place placements in blocks of block_k (the loop of Subject.iter_simulate,
  * simulation.py, over fem/solve.py:865-979 per block), returning only the per-placement summaries:
- * metrics (n_place, CN_METRICS_LEN), iterations and relative_residual (n_place), block_iterations
- * (one per block); any may be NULL. Every block's numerics are cn_solve_block's for the same
+ * transforms (n_place, 4, 4), metrics (n_place, CN_METRICS_LEN), iterations and relative_residual
+ * (n_place), block_iterations (one per block); any may be NULL. Every block's numerics are cn_solve_block's for the same
  * placements; the library pipelines them, computing the next block's dA/dt and right-hand side on
  * a second stream while the current block solves. opt->sigma, opt->collect and warm_start = 2 are
  * not accepted here (CN_EINVAL); warm_start = 1 uses the context's modes. */
 int32_t cn_solve_sweep(cn_context* ctx, const double* dip_pos_m, const double* dip_moment, int64_t n_dip,
                        const double* centers_mm, const double* handles_mm, const double* distances_mm,
                        int32_t n_place, int32_t block_k, double didt, const cn_solve_options* opt,
-                       double* metrics, int32_t* iterations, double* relative_residual,
+                       double* transforms, double* metrics, int32_t* iterations, double* relative_residual,
                        int32_t* block_iterations);
 
 /* ---- kernel-level entry points (reference binding each one stands in for) ---- */

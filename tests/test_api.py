@@ -129,3 +129,32 @@ def test_subject_end_to_end_matches_the_block_solver(head4, coil):
         assert nf.shape == (subj.context.n_nodes, 3) and np.isfinite(nf).any() and np.isnan(nf).any()
         with pytest.raises(ValueError):
             h.nodal_field("all")
+
+
+@pytest.mark.gpu
+def test_summary_only_streaming_uses_the_pipelined_sweep(head4, coil):
+    """iter_simulate without field outputs streams through the pipelined sweep; each result's
+    summary, frame, iterations and residual equal a block-by-block solve of the same placements."""
+    from cunibs_amd import fem, synthetic
+    from cunibs_amd.mesh import HeadMesh
+    sites = synthetic.cap_sites(5, seed=23)
+    placements = [Placement(c, h, d) for c, h, d in sites]
+    c = Coil(coil.positions_m, coil.moments, name="fig8")
+    with Subject(HeadMesh(head4.nodes_mm, head4.tet_nodes, head4.tet_tags, head4.skin_tris)) as subj:
+        res = list(subj.iter_simulate(c, placements, block_k=2, recovery="raw"))
+        assert [r.placement for r in res] == placements
+        ref = []
+        for i in range(0, 5, 2):
+            ref.extend(fem.solve_placements_block(subj.context, c.positions_m, c.moments, sites[i:i + 2], 1e6,
+                                                  magnitude=False))
+        for a, b in zip(res, ref):
+            assert a.magnE is None
+            want = fem.metrics_dict(b.metrics)
+            assert a.summary.keys() == want.keys()
+            for key, value in want.items():
+                if isinstance(value, dict):
+                    assert a.summary[key] == value, key
+                else:
+                    np.testing.assert_array_equal(a.summary[key], value, err_msg=key)
+            np.testing.assert_array_equal(a.transform, b.transform)
+            assert a.iterations == b.iterations and a.relative_residual == b.relative_residual
