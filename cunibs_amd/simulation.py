@@ -93,18 +93,17 @@ class Coil:
 
     @classmethod
     def load(cls, source: str | Path) -> Coil:
-        """A SimNIBS ``.ccd`` dipole file (header line ``# key=value; ...``, a count line, a column
-        line, then ``x y z mx my mz`` rows), or an ``.h5`` file when h5py is installed
-        (coil.py:66-98)."""
+        """An ``.h5`` coil (one ``dipoles`` dataset, n x 6 ``x y z mx my mz``, with the ``.ccd``
+        header's keys as attributes; coil.py:66-83), read by :mod:`cunibs_amd.h5lite` so h5py is
+        not needed, or a SimNIBS ``.ccd`` dipole file (header line ``# key=value; ...``, a count
+        line, a column line, then the rows; coil.py:85-98). ``dIdtmax`` is stated in A/us and
+        exposed in A/s; the raw attribute stays in ``metadata``."""
         path = Path(source)
         if path.suffix == ".h5":
-            try:
-                import h5py
-            except ImportError as exc:
-                raise ImportError("reading .h5 coils needs h5py, which is not installed") from exc
-            with h5py.File(path, "r") as f:
-                data = np.asarray(f["dipoles"], dtype=np.float64)
-                attrs = {k: (v.decode() if isinstance(v, bytes) else v) for k, v in f["dipoles"].attrs.items()}
+            from . import h5lite
+            dset = h5lite.open_file(path)["dipoles"]
+            data = np.asarray(dset.read(), dtype=np.float64).reshape(-1, 6)
+            attrs = {k: _decode_attr(v) for k, v in dset.attrs.items()}
         else:
             with path.open() as f:
                 meta_line = f.readline().strip().lstrip("#").strip()
@@ -114,6 +113,17 @@ class Coil:
         didt = attrs.get("dIdtmax")
         return cls(data[:, 0:3], data[:, 3:6], name=str(attrs.get("coilname", path.stem)),
                    didt_max=None if didt is None else float(didt) * 1e6, metadata=attrs)
+
+
+def _decode_attr(value: object) -> str | int | float:
+    """An attribute as str, int or float (coil.py:38-45)."""
+    if isinstance(value, bytes):
+        return value.decode()
+    if isinstance(value, np.generic):
+        value = value.item()
+    if isinstance(value, str | int | float):
+        return value
+    return str(value)
 
 
 def _region_mask(tags: np.ndarray, region: str) -> np.ndarray:

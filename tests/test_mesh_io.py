@@ -1,8 +1,8 @@
 """Gmsh 2.2 reader and the reference's real-head patch pins (CPU).
 
 The patch (tests/data/head_patch_r25.msh.gz in the reference: 8404 nodes, 41139 tets, 1125 skin
-triangles) is read from the reference tree when it is present and the tests skip otherwise; it is
-not copied into this repository."""
+triangles, a 25 mm ball cut from a CHARM head) and its manifest are data files of the reference's
+tests, kept as fixtures under tests/golden/."""
 
 from __future__ import annotations
 
@@ -13,9 +13,10 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-PATCH = Path("/root/reference/tests/data/head_patch_r25.msh.gz")
-PATCH_JSON = Path("/root/reference/tests/data/head_patch_r25.json")
-needs_patch = pytest.mark.skipif(not PATCH.exists(), reason="reference patch mesh not present")
+GOLDEN = Path(__file__).resolve().parent / "golden"
+PATCH = GOLDEN / "head_patch_r25.msh.gz"
+PATCH_JSON = GOLDEN / "head_patch_r25.json"
+needs_patch = pytest.mark.skipif(not PATCH.exists(), reason="patch fixture missing")
 
 
 @needs_patch

@@ -1,0 +1,124 @@
+"""End-to-end pins on the reference's real-head patch with the bundled MagStim D70 coil.
+
+The reference pins (tests/test_realmesh.py:25-27, 116-141): at the topmost scalp node of the
+patch, handle +50 mm along y, 4 mm off the scalp, dI/dt 1e6 A/s, gray-matter peak |E| 0.556405
+V/m at (11.794, -1.125, 77.349) mm with raw recovery and 0.484461 V/m with harmonic recovery, both
+to rel 2e-3 (location to 2 mm). The CPU tests hold the FP64 oracle to those numbers; the `gpu` tests
+hold the HIP path to the same numbers through the public Subject API. Fixtures: tests/golden/."""
+
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+PATCH = GOLDEN / "head_patch_r25.msh.gz"
+D70 = GOLDEN / "MagStim_D70.h5"
+
+PATCH_PEAK_MAGNE = 0.556405
+PATCH_PEAK_MAGNE_HARMONIC = 0.484461
+PATCH_PEAK_LOCATION_MM = np.array([11.794, -1.125, 77.349])
+
+
+@pytest.fixture(scope="module")
+def patch_mesh():
+    from cunibs_amd.mesh import load_mesh
+    return load_mesh(PATCH)
+
+
+@pytest.fixture(scope="module")
+def d70():
+    from cunibs_amd.simulation import Coil
+    return Coil.load(D70)
+
+
+@pytest.fixture(scope="module")
+def patch_site(patch_mesh):
+    """tests/conftest.py:337-349: the topmost scalp node, handle +50 mm in y, 4 mm offset."""
+    skin = np.unique(patch_mesh.skin_tris)
+    coords = patch_mesh.nodes_mm[skin]
+    top = np.ascontiguousarray(coords[np.argmax(coords[:, 2])])
+    return top, top + np.array([0.0, 50.0, 0.0]), 4.0
+
+
+def test_d70_loads_without_h5py(d70):
+    """tests/test_coil.py:54-65: the bundled D70 reads back, dIdtmax kept raw and exposed in A/s."""
+    assert d70.name == "D70"
+    assert d70.positions_m.shape == (964, 3) and d70.moments.shape == (964, 3)
+    assert d70.positions_m.dtype == np.float64
+    assert d70.metadata["dIdtmax"] == "114.73"
+    assert d70.metadata["num_elements"] == 964 and isinstance(d70.metadata["num_elements"], int)
+    assert d70.didt_max == pytest.approx(114.73e6)
+    assert np.abs(d70.positions_m).max() < 1.0 and np.abs(d70.moments).max() > 0.0
+
+
+@pytest.fixture(scope="module")
+def oracle_patch(oracle_mod, patch_mesh, d70, patch_site):
+    ctx = oracle_mod.build_context(patch_mesh.nodes_mm, patch_mesh.tet_nodes, patch_mesh.tet_tags,
+                                   patch_mesh.skin_tris)
+    r = oracle_mod.solve_site(ctx, d70.positions_m, d70.moments, patch_site, 1e6, tol=1e-10)
+    return ctx, r
+
+
+def test_oracle_raw_peak_matches_the_reference_pin(oracle_mod, oracle_patch):
+    ctx, r = oracle_patch
+    s = oracle_mod.gm_summary(ctx, r["magnE"])
+    assert s["peak_magnE"] == pytest.approx(PATCH_PEAK_MAGNE, rel=2e-3)
+    np.testing.assert_allclose(s["peak_location_mm"], PATCH_PEAK_LOCATION_MM, atol=2.0)
+    # The oracle lands far inside the reference's tolerance: 6 digits and 1 micrometre.
+    assert s["peak_magnE"] == pytest.approx(PATCH_PEAK_MAGNE, rel=2e-6)
+    np.testing.assert_allclose(s["peak_location_mm"], PATCH_PEAK_LOCATION_MM, atol=1e-3)
+
+
+def test_oracle_harmonic_peak_matches_the_reference_pin(oracle_mod, oracle_patch, d70):
+    ctx, r = oracle_patch
+    op = oracle_mod.harmonic_operator(ctx.nodes_mm, ctx.tet_nodes, ctx.tet_tags)
+    dadt = oracle_mod.dadt_at_nodes(d70.positions_m, d70.moments, r["transform"], 1e6, ctx.nodes_mm)
+    _, _, mag = oracle_mod.harmonic_apply(op, r["v"], dadt)
+    s = oracle_mod.gm_summary(ctx, mag)
+    assert s["peak_magnE"] == pytest.approx(PATCH_PEAK_MAGNE_HARMONIC, rel=2e-6)
+
+
+@pytest.fixture(scope="module")
+def patch_subject(patch_mesh):
+    from cunibs_amd.simulation import Subject
+    s = Subject(patch_mesh)
+    yield s
+    s.free()
+
+
+@pytest.mark.gpu
+def test_gpu_raw_peak_matches_the_reference_pin(patch_subject, d70, patch_site):
+    """tests/test_realmesh.py:116-133."""
+    from cunibs_amd.simulation import Placement
+    r = patch_subject.simulate(d70, Placement(*patch_site), 1e6, magnitude=True, vectors=True,
+                               potential=True, recovery="raw")
+    assert r.peak_magnE() == pytest.approx(PATCH_PEAK_MAGNE, rel=2e-3)
+    np.testing.assert_allclose(r.peak_location_mm(), PATCH_PEAK_LOCATION_MM, atol=2.0)
+    assert r.coil_name == d70.name
+    assert r.relative_residual <= 1e-6 and 0 < r.iterations < 200
+
+
+@pytest.mark.gpu
+def test_gpu_harmonic_peak_matches_the_reference_pin(patch_subject, d70, patch_site):
+    """tests/test_realmesh.py:136-141: the default recovery is harmonic."""
+    from cunibs_amd.simulation import Placement
+    r = patch_subject.simulate(d70, Placement(*patch_site), 1e6, magnitude=True)
+    assert r.recovery == "harmonic"
+    assert r.peak_magnE() == pytest.approx(PATCH_PEAK_MAGNE_HARMONIC, rel=2e-3)
+
+
+@pytest.mark.gpu
+def test_gpu_patch_field_matches_the_oracle(patch_subject, oracle_mod, oracle_patch, d70, patch_site):
+    """Element |E| in gray matter against the FP64 oracle on the same placement, to the 1e-3 bar."""
+    from cunibs_amd.simulation import Placement
+    ctx, ref = oracle_patch
+    r = patch_subject.simulate(d70, Placement(*patch_site), 1e6, magnitude=True, recovery="raw")
+    # Results are indexed in the Morton order, which the oracle's context shares.
+    np.testing.assert_array_equal(patch_subject.mesh.tet_tags, ctx.tet_tags)
+    gm = ctx.tet_tags == 2
+    ours = np.asarray(r.magnE)[gm]
+    theirs = ref["magnE"][gm]
+    assert np.abs(ours - theirs).max() <= 1e-3 * np.abs(theirs).max()
