@@ -119,7 +119,7 @@ def learned_leg(ctx, pos, mom, n_train: int, n_test: int, log_sd: float = 0.2) -
 
     def draws(n, seed):
         sites = synthetic.cap_sites(n, seed=seed, distance_range_mm=(2.0, 8.0))
-        sig = uq.sample_conductivities(tags, n, log_sd, seed=seed + 1)
+        sig = uq.draw_lognormal(tags, n, log_sd, seed=seed + 1)
         rows = fem.conductivity_rows([dict(zip(tags, r)) for r in sig])
         return sites, rows
 

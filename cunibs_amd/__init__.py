@@ -15,6 +15,8 @@ _EXPORTS = {
     "RECOVERY_MODES": "cunibs_amd.simulation",
     "HeadMesh": "cunibs_amd.mesh",
     "load_mesh": "cunibs_amd.mesh",
+    "ConductivityUQConfig": "cunibs_amd.uq",
+    "ConductivityUQResult": "cunibs_amd.uq",
 }
 
 __all__ = sorted(_EXPORTS)

@@ -746,6 +746,8 @@ void uq_prepare(cn_context& c, const double* pos_m, const double* mom, std::int6
     u.ready = true;
 }
 
+const cn::RecoveryOp* ensure_recovery(cn_context& c, int mode);
+
 // k <= 16 conductivity samples (sigma: k x T, T in the order of the tags uq_prepare found) of the
 // prepared placement, as K independent CG chains under the nominal preconditioner.
 void uq_solve(cn_context& c, const double* sigma, int k, const cn_solve_options* sopt, cn_block_outputs* out) {
@@ -811,15 +813,30 @@ void uq_solve(cn_context& c, const double* sigma, int k, const cn_solve_options*
         throw;
     }
     c.solver->set_multi(nullptr);
-    // |E| per sample: dA/dt is the same for every sample.
+    // |E| per sample (dA/dt is the same for every sample), recovered before any statistic is
+    // taken (uq/conductivity/run.py:340-352).
+    const int mode = sopt ? sopt->recovery : CN_RECOVERY_RAW;
+    const RecoveryOp* rec = mode == CN_RECOVERY_RAW ? nullptr : ensure_recovery(c, mode);
     c.d_magn.ensure(m * K, "magnE");
     float* dE = nullptr;
     if (out && out->E) {
         c.d_E.ensure(3 * m * K, "E");
         dE = c.d_E.get();
     }
-    reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), u.dadt_nodes1.get(), 1, m, k, K, c.ground, dE, c.d_magn.get(),
-                s);
+    if (rec == nullptr) {
+        reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), u.dadt_nodes1.get(), 1, m, k, K, c.ground, dE,
+                    c.d_magn.get(), s);
+    } else if (rec->mode == kRecoveryHarmonic) {
+        c.d_slots.ensure(3 * rec->n_slots * K, "recovered slots");
+        apply_harmonic(*rec, c.d_X.get(), c.ground, u.dadt_nodes1.get(), 1, m, k, K, c.d_slots.get(), dE,
+                       c.d_magn.get(), s);
+    } else {
+        c.d_Eraw.ensure(3 * m * K, "raw E");
+        c.d_slots.ensure(3 * rec->n_slots * K, "recovered slots");
+        reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), u.dadt_nodes1.get(), 1, m, k, K, c.ground, c.d_Eraw.get(),
+                    c.d_magn.get(), s);
+        apply_spr(*rec, c.d_Eraw.get(), m, k, K, c.d_slots.get(), dE, c.d_magn.get(), s);
+    }
     accumulate_moments(c.d_magn.get(), m, k, K, u.sum.get(), u.sumsq.get(), s);
     u.count += k;
     if (out) {
@@ -1033,8 +1050,8 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
     } else if (rec->mode == kRecoveryHarmonic) {
         // The potential and the exact nodal dA/dt feed the fits; |E| and E describe the result.
         c.d_slots.ensure(3 * rec->n_slots * K, "recovered slots");
-        apply_harmonic(*rec, c.d_X.get(), c.ground, c.d_dadt_nodes.get(), m, k, K, c.d_slots.get(), dE, c.d_magn.get(),
-                       s);
+        apply_harmonic(*rec, c.d_X.get(), c.ground, c.d_dadt_nodes.get(), K, m, k, K, c.d_slots.get(), dE,
+                       c.d_magn.get(), s);
     } else {
         // The raw field feeds the patch fits; |E| and E then describe the recovered field.
         c.d_Eraw.ensure(3 * m * K, "raw E");
@@ -1185,7 +1202,7 @@ void solve_sweep(cn_context& c, const double* pos_m, const double* mom, std::int
                             c.d_magn.get(), s);
             } else if (rec->mode == kRecoveryHarmonic) {
                 c.d_slots.ensure(3 * rec->n_slots * K, "recovered slots");
-                apply_harmonic(*rec, c.d_X.get(), c.ground, nodal[buf], m, k, Kb, c.d_slots.get(), nullptr,
+                apply_harmonic(*rec, c.d_X.get(), c.ground, nodal[buf], Kb, m, k, Kb, c.d_slots.get(), nullptr,
                                c.d_magn.get(), s);
             } else {
                 c.d_Eraw.ensure(3 * m * K, "raw E");

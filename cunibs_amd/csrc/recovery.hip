@@ -562,7 +562,7 @@ __global__ void __launch_bounds__(kFitWaves * kWave) hpr_fit_kernel(
 // order; v is the reduced potential with the ground node's zero restored.
 __global__ void hpr_grad_kernel(const double* __restrict__ X, int ground, const float* __restrict__ w3,
                                 const std::int32_t* __restrict__ pptr, const std::int32_t* __restrict__ pidx,
-                                const std::int32_t* __restrict__ slot_node, const float* __restrict__ dadt,
+                                const std::int32_t* __restrict__ slot_node, const float* __restrict__ dadt, int Kn,
                                 std::int64_t n_slots, int K, float* __restrict__ slots) {
     const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= n_slots * K) return;
@@ -577,11 +577,11 @@ __global__ void hpr_grad_kernel(const double* __restrict__ X, int ground, const 
         a2 = fma(static_cast<double>(w3[3 * static_cast<std::int64_t>(p) + 2]), vm, a2);
     }
     const std::int64_t node = slot_node[s];
-    const float* da = dadt + node * 3 * K + c;
+    const float* da = dadt + node * 3 * Kn + (Kn == 1 ? 0 : c);
     float* dst = slots + s * 3 * K + c;
     dst[0] = static_cast<float>(-a0 - static_cast<double>(da[0]));
-    dst[K] = static_cast<float>(-a1 - static_cast<double>(da[K]));
-    dst[2 * K] = static_cast<float>(-a2 - static_cast<double>(da[2 * K]));
+    dst[K] = static_cast<float>(-a1 - static_cast<double>(da[Kn]));
+    dst[2 * K] = static_cast<float>(-a2 - static_cast<double>(da[2 * Kn]));
 }
 
 }  // namespace
@@ -648,11 +648,12 @@ void build_harmonic(RecoveryOp& op, const double* nodes, const std::int32_t* tet
     }
 }
 
-void apply_harmonic(const RecoveryOp& op, const double* X, int ground, const float* dadt_nodes, std::int64_t m, int k,
-                    int K, float* slots, float* E_out, float* magn, hipStream_t s) {
+void apply_harmonic(const RecoveryOp& op, const double* X, int ground, const float* dadt_nodes, int Kn, std::int64_t m,
+                    int k, int K, float* slots, float* E_out, float* magn, hipStream_t s) {
+    require(Kn == 1 || Kn == K, "dA/dt column stride must be 1 or the block width");
     if (const unsigned g = grid_for(op.n_slots * K))
         hpr_grad_kernel<<<g, kBlock, 0, s>>>(X, ground, op.w3.get(), op.pptr.get(), op.pidx.get(), op.slot_node.get(),
-                                             dadt_nodes, op.n_slots, K, slots);
+                                             dadt_nodes, Kn, op.n_slots, K, slots);
     if (const unsigned g = grid_for(m * K))
         recover_elements_kernel<<<g, kBlock, 0, s>>>(slots, op.soc_view, m, k, K, E_out, magn);
 }

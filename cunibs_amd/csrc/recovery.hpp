@@ -44,8 +44,9 @@ void tissue_slots(RecoveryOp& op, const std::int32_t* tets, const std::int32_t* 
 void build_harmonic(RecoveryOp& op, const double* nodes, const std::int32_t* tets, const std::int32_t* tags,
                     std::int64_t n, std::int64_t m, int max_tag, hipStream_t s);
 // X (n-1, K) FP64 reduced potential with `ground` removed; dadt_nodes (n, 3, K) FP32.
-void apply_harmonic(const RecoveryOp& op, const double* X, int ground, const float* dadt_nodes, std::int64_t m, int k,
-                    int K, float* slots, float* E_out, float* magn, hipStream_t s);
+// dadt_nodes is (n, 3, Kn): Kn = K gives each column its own dA/dt, Kn = 1 shares one.
+void apply_harmonic(const RecoveryOp& op, const double* X, int ground, const float* dadt_nodes, int Kn, std::int64_t m,
+                    int k, int K, float* slots, float* E_out, float* magn, hipStream_t s);
 // E (m, 3, K) -> slots (n_slots, 3, K), E_out (m, 3, K) (may be null), magn (m, K).
 void apply_spr(const RecoveryOp& op, const float* E, std::int64_t m, int k, int K, float* slots, float* E_out,
                float* magn, hipStream_t s);

@@ -380,10 +380,14 @@ class SolverContext:
 
     def uq_solve(self, sigma: np.ndarray, *, magnitude: bool = True, vectors: bool = False, potential: bool = False,
                  metrics: bool = True, tolerance: float = DEFAULT_TOLERANCE,
-                 max_iters: int = DEFAULT_MAX_ITERS, initial_guess: bool = True) -> list[PlacementResult]:
+                 max_iters: int = DEFAULT_MAX_ITERS, initial_guess: bool = True,
+                 recovery: str = "raw") -> list[PlacementResult]:
         """k <= 16 conductivity samples (k x n_tissue, columns in uq_tags order). With
         ``initial_guess`` each draw starts from the nominal solution plus its Galerkin correction
-        on the sensitivity span (uq/conductivity/run.py:154-200), else from zero."""
+        on the sensitivity span (uq/conductivity/run.py:154-200), else from zero. ``recovery``
+        post-processes each draw's field before |E|, the metrics and the moments are taken."""
+        if recovery not in _lib.RECOVERY_CODES:
+            raise ValueError(f"unknown recovery {recovery!r}")
         sg = np.ascontiguousarray(sigma, np.float64)
         if sg.ndim != 2 or sg.shape[1] != len(self.uq_tags):
             raise ValueError("sigma must be (k, n_tissue) in the order uq_prepare returned")
@@ -402,6 +406,7 @@ class SolverContext:
         opt.tolerance = float(tolerance)
         opt.max_iters = int(max_iters)
         opt.warm_start = 1 if initial_guess else 0
+        opt.recovery = _lib.RECOVERY_CODES[recovery]
         check(lib().cn_uq_solve(self.handle, ptr(sg), k, ctypes.byref(opt), ctypes.byref(out)), "cn_uq_solve")
         return [PlacementResult(transform=tf[i], magnE=None if mag is None else mag[i],
                                 E=None if E is None else E[i], v=None if v is None else v[i],

@@ -10,7 +10,7 @@ asked for come back. There is no CPU fallback.
 from __future__ import annotations
 
 import threading
-from collections.abc import Iterator, Sequence
+from collections.abc import Iterator, Mapping, Sequence
 from dataclasses import dataclass, field
 from pathlib import Path
 
@@ -392,4 +392,40 @@ class Subject:
                             "a sequence.")
         (result,) = self.iter_simulate(coil, [placement], didt, magnitude=magnitude, vectors=vectors,
                                        potential=potential, nodal=nodal, recovery=recovery, block_k=block_k)
+        return result
+
+    def iter_simulate_conductivity_uq(self, coil: Coil, placements: Sequence[Placement], config, didt: float = 1e6,
+                                      *, moments: bool = False, record_rois: Mapping | None = None,
+                                      recovery: str = "harmonic"):
+        """A conductivity Monte Carlo per placement, in the order given (simulation.py:593-632).
+        Every result carries its gray-matter summary and the per-draw peak |E|, focality and peak
+        location; ``moments`` keeps the per-element mean/std/cov of |E|; ``record_rois`` maps a
+        name to an ROI (an object with ``elem_idx`` and ``weights``, or an ``(elem_idx, weights)``
+        pair) whose weighted mean |E| each draw records."""
+        from cunibs_amd import uq
+        sites = _sites(placements, "iter_simulate_conductivity_uq")
+        recovery = validate_recovery(recovery)
+        if not isinstance(config, uq.ConductivityUQConfig):
+            raise TypeError("config must be a ConductivityUQConfig")
+        rois = dict(record_rois or {})
+
+        def run():
+            with self._lock:
+                ctx = self.context
+                host = self._host_metric_inputs()
+                for site in sites:
+                    yield uq.conductivity_uq(ctx, coil, site, config, didt, moments=moments, record_rois=rois,
+                                             recovery=recovery, host_inputs=host)
+
+        return run()
+
+    def simulate_conductivity_uq(self, coil: Coil, placement: Placement, config, didt: float = 1e6, *,
+                                 moments: bool = False, record_rois: Mapping | None = None,
+                                 recovery: str = "harmonic"):
+        """One placement's conductivity Monte Carlo (simulation.py:634-664)."""
+        if not isinstance(placement, Placement):
+            raise TypeError("simulate_conductivity_uq() takes a single Placement. Use "
+                            "Subject.iter_simulate_conductivity_uq(...) to stream a sequence.")
+        (result,) = self.iter_simulate_conductivity_uq(coil, [placement], config, didt, moments=moments,
+                                                       record_rois=record_rois, recovery=recovery)
         return result

@@ -1,19 +1,209 @@
-"""Conductivity uncertainty for one placement (the reference's uq/conductivity, config.py:56-72 and
-run.py:203-387): tissue conductivities drawn log-normal around the table values, solved in blocks
-of up to 16 samples that share the operator's pattern (one launch per kernel for the block), the
-GM peak |E| per sample and the moments of |E| over all samples."""
+"""Conductivity uncertainty for one placement (the reference's uq/conductivity, config.py:1-72,
+run.py:203-387, result.py:1-256): tissue conductivities drawn around the table values, solved in
+blocks of up to 16 samples that share the operator's pattern (one launch per kernel for the block),
+each draw's field recovered before any statistic, the GM peak |E|, focality and peak location per
+draw, and the per-element moments of |E| over all draws.
+
+``ConductivityUQConfig``, ``sample_conductivities``, ``ConductivityUQResult`` (with ``summary``,
+``tissue_sensitivity``) and ``Subject.simulate_conductivity_uq`` mirror the reference's public
+surface. The draws come from NumPy's seeded generator, so they are not the reference's CuPy draws
+for the same seed; their distribution is the same."""
 
 from __future__ import annotations
 
-from dataclasses import dataclass
+from collections.abc import Mapping
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
 from cunibs_amd import fem
 
+# Lognormal coefficients of variation per tissue tag (config.py:12-25): the reference's
+# illustrative defaults, overridable per study through ConductivityUQConfig.tissue_cov.
+DEFAULT_TISSUE_COV: dict[int, float] = {
+    1: 0.15, 2: 0.15, 3: 0.05, 4: 0.35, 5: 0.20, 6: 0.20, 7: 0.35, 8: 0.35, 9: 0.15, 10: 0.20,
+}
 
-def sample_conductivities(tags, n: int, log_sd: float = 0.2, seed: int = 0, table: dict | None = None,
-                          mean_preserving: bool = True) -> np.ndarray:
+
+@dataclass(frozen=True)
+class ConductivityUQConfig:
+    """Monte Carlo settings (config.py:28-53): each perturbed tissue's conductivity is an
+    independent draw with the given coefficient of variation, log-normal with mean equal to the
+    nominal value (or normal, clamped to 1e-6 x nominal). The preconditioner stays the nominal one
+    for every draw; a draw that misses tolerance raises."""
+
+    n_samples: int = 500
+    seed: int = 0
+    tissue_cov: dict[int, float] = field(default_factory=dict)
+    perturbed_tags: tuple[int, ...] | None = None
+    distribution: str = "lognormal"
+
+    def cov_for(self, tag: int) -> float:
+        if tag in self.tissue_cov:
+            return self.tissue_cov[tag]
+        return DEFAULT_TISSUE_COV[tag]
+
+
+def sample_conductivities(config: ConductivityUQConfig, perturbed_tags) -> np.ndarray:
+    """(n_samples, len(perturbed_tags)) conductivities (config.py:56-72): log-normal
+    sigma_0 exp(s z - s^2/2) with s = sqrt(log(1 + cov^2)), so E[sigma] = sigma_0; or normal
+    sigma_0 (1 + cov z) clamped to 1e-6 sigma_0."""
+    if config.distribution not in ("lognormal", "normal"):
+        raise ValueError(f"unknown distribution {config.distribution!r}")
+    if config.n_samples < 1:
+        raise ValueError("n_samples must be at least 1")
+    tags = [int(t) for t in perturbed_tags]
+    nominal = np.array([fem.TISSUE_CONDUCTIVITY[t] for t in tags], np.float64)
+    covs = np.array([config.cov_for(t) for t in tags], np.float64)
+    z = np.random.default_rng(config.seed).standard_normal((config.n_samples, len(tags)))
+    if config.distribution == "lognormal":
+        s = np.sqrt(np.log1p(covs * covs))
+        return nominal * np.exp(z * s - 0.5 * s * s)
+    return np.maximum(nominal * (1.0 + covs * z), 1e-6 * nominal)
+
+
+@dataclass(frozen=True)
+class ConductivityUQSummary:
+    """result.py:64-76: metrics of the mean field over a region, and the largest per-element
+    coefficient of variation of |E| there."""
+
+    mean_field: dict
+    max_local_cov: float
+
+
+def _tissue_sensitivity(result, output: str) -> dict[int, float]:
+    """result.py:31-61: regress log of the per-draw output on the log conductivity draws; each
+    tag's share is beta_j^2 Var(log sigma_j) / Var(log y). Not a Sobol index."""
+    if output == "peak":
+        y = result.peak_samples
+    elif output == "focality":
+        y = result.focality_samples
+    elif output in result.roi_samples:
+        y = result.roi_samples[output]
+    else:
+        raise ValueError(f"Unknown output {output!r}; expected 'peak', 'focality', or a recorded ROI name "
+                         f"(available: {['peak', 'focality', *result.roi_samples]}).")
+    x = np.log(np.asarray(result.sigma_samples, np.float64))
+    logy = np.log(np.asarray(y, np.float64))
+    xc = x - x.mean(0)
+    yc = logy - logy.mean()
+    var_y = float(yc.var())
+    if var_y == 0.0:
+        return {int(t): 0.0 for t in result.perturbed_tags}
+    beta, *_ = np.linalg.lstsq(xc, yc, rcond=None)
+    contrib = beta ** 2 * xc.var(0) / var_y
+    return {int(t): float(c) for t, c in zip(result.perturbed_tags, contrib)}
+
+
+@dataclass
+class ConductivityUQResult:
+    """result.py:78-256: per-element moments of |E| over the draws (None unless ``moments``), the
+    draws themselves and the per-draw gray-matter scalars, on the host."""
+
+    mean_magnE: np.ndarray | None
+    std_magnE: np.ndarray | None
+    cov_magnE: np.ndarray | None
+    n_samples: int
+    perturbed_tags: tuple[int, ...]
+    sigma_samples: np.ndarray
+    vols: np.ndarray | None
+    tet_tags: np.ndarray | None
+    barycenters_mm: np.ndarray | None
+    placement: object
+    coil_name: str
+    didt: float
+    peak_samples: np.ndarray
+    focality_samples: np.ndarray
+    peak_location_samples: np.ndarray
+    roi_samples: dict
+    summary: ConductivityUQSummary | None = None
+    recovery: str = "raw"
+    iterations: np.ndarray | None = None
+
+    def compute_summary(self, region: str = "gray_matter") -> ConductivityUQSummary:
+        from cunibs_amd.simulation import _region_mask, compute_metrics
+        if self.mean_magnE is None or self.cov_magnE is None:
+            raise ValueError(f"Metrics for region {region!r} need the per-element moment arrays, which "
+                             "this result did not retain. Re-run with moments=True.")
+        mask = _region_mask(self.tet_tags, region)
+        return ConductivityUQSummary(
+            mean_field=compute_metrics(self.mean_magnE, self.vols, self.barycenters_mm, self.tet_tags, region=region),
+            max_local_cov=float(np.max(self.cov_magnE[mask])) if mask.any() else 0.0)
+
+    def _summary_for(self, region: str) -> ConductivityUQSummary:
+        if self.summary is not None and region == self.summary.mean_field["region"]:
+            return self.summary
+        return self.compute_summary(region)
+
+    def peak_mean_magnE(self, region: str = "gray_matter") -> float:
+        return self._summary_for(region).mean_field["peak_magnE"]
+
+    def max_local_cov(self, region: str = "gray_matter") -> float:
+        return self._summary_for(region).max_local_cov
+
+    def tissue_sensitivity(self, output: str = "peak") -> dict[int, float]:
+        return _tissue_sensitivity(self, output)
+
+    def to_numpy(self) -> ConductivityUQResult:
+        return self
+
+
+def _roi_arrays(target) -> tuple[np.ndarray, np.ndarray]:
+    if isinstance(target, tuple):
+        idx, w = target
+    else:
+        idx, w = target.elem_idx, target.weights
+    return np.asarray(idx, np.int64), np.asarray(w, np.float64)
+
+
+def conductivity_uq(ctx: fem.SolverContext, coil, placement, config: ConductivityUQConfig, didt: float, *,
+                    moments: bool = False, record_rois: Mapping = {}, recovery: str = "harmonic",
+                    host_inputs=None, block: int = 16) -> ConductivityUQResult:
+    """One placement's Monte Carlo (run.py:203-387) on the GPU: every draw solved under the
+    nominal preconditioner from the nominal solution plus its sensitivity correction, recovered,
+    summarised on the device; moments accumulated on the device over all draws."""
+    site = (placement.center_mm, placement.handle_mm, placement.distance_mm)
+    tags = ctx.uq_prepare(coil.positions_m, coil.moments, site, didt)
+    perturbed = tuple(sorted(int(t) for t in config.perturbed_tags)) if config.perturbed_tags is not None \
+        else tuple(tags)
+    missing = [t for t in perturbed if t not in tags]
+    if missing:
+        raise ValueError(f"perturbed tags {missing} are not in the mesh (tags {tags})")
+    draws = sample_conductivities(config, perturbed)
+    sigma = np.tile(np.array([fem.TISSUE_CONDUCTIVITY[t] for t in tags], np.float64), (config.n_samples, 1))
+    for j, t in enumerate(perturbed):
+        sigma[:, tags.index(t)] = draws[:, j]
+    probes = {name: _roi_arrays(t) for name, t in record_rois.items()}
+    n = config.n_samples
+    peak, foc, loc, its = np.empty(n), np.empty(n), np.empty((n, 3)), np.empty(n, np.int32)
+    roi = {name: np.empty(n) for name in probes}
+    for i in range(0, n, block):
+        res = ctx.uq_solve(sigma[i:i + block], magnitude=bool(probes), metrics=True, recovery=recovery)
+        for j, r in enumerate(res):
+            peak[i + j] = r.metrics[0]
+            loc[i + j] = r.metrics[1:4]
+            foc[i + j] = r.metrics[8]
+            its[i + j] = r.iterations
+            for name, (idx, w) in probes.items():
+                roi[name][i + j] = float((r.magnE[idx].astype(np.float64) * w).sum())
+    mean, var, _ = ctx.uq_moments()
+    std = np.sqrt(np.clip(var, 0.0, None))
+    cov = np.where(mean > 1e-12 * float(mean.max()), std / np.where(mean > 0, mean, 1.0), 0.0)
+    vols, tet_tags, bary = host_inputs if host_inputs is not None else (None, None, None)
+    result = ConductivityUQResult(
+        mean_magnE=mean.astype(np.float32), std_magnE=std.astype(np.float32), cov_magnE=cov.astype(np.float32),
+        n_samples=n, perturbed_tags=perturbed, sigma_samples=draws, vols=vols, tet_tags=tet_tags,
+        barycenters_mm=bary, placement=placement, coil_name=coil.name, didt=float(didt), peak_samples=peak,
+        focality_samples=foc, peak_location_samples=loc, roi_samples=roi, recovery=recovery, iterations=its)
+    if vols is not None:
+        result.summary = result.compute_summary()
+    if not moments:
+        result = replace(result, mean_magnE=None, std_magnE=None, cov_magnE=None)
+    return result
+
+
+def draw_lognormal(tags, n: int, log_sd: float = 0.2, seed: int = 0, table: dict | None = None,
+                   mean_preserving: bool = True) -> np.ndarray:
     """(n, len(tags)) draws sigma = sigma_0 exp(s z - s^2/2) (mean sigma_0, the reference's
     parameterisation, config.py:63-69) or sigma_0 exp(s z) (median sigma_0), z ~ N(0, 1), seeded."""
     table = fem.TISSUE_CONDUCTIVITY if table is None else table
@@ -49,7 +239,7 @@ def run_conductivity_uq(ctx: fem.SolverContext, dip_pos_m, dip_moment, site, did
                         initial_guess: bool = True) -> UQResult:
     tags = ctx.uq_prepare(dip_pos_m, dip_moment, site, didt)
     if sigma is None:
-        sigma = sample_conductivities(tags, n_samples, log_sd, seed)
+        sigma = draw_lognormal(tags, n_samples, log_sd, seed)
     peaks, iters, foc, loc = [], [], [], []
     for i in range(0, sigma.shape[0], block):
         res = ctx.uq_solve(sigma[i:i + block], magnitude=False, initial_guess=initial_guess)

@@ -8,17 +8,17 @@ import pytest
 
 
 def test_sampling_is_seeded_lognormal_with_the_table_mean():
-    from cunibs_amd.uq import sample_conductivities
+    from cunibs_amd.uq import draw_lognormal
     tags = [1, 2, 3, 4, 5]
-    a = sample_conductivities(tags, 20000, 0.2, seed=3)
-    b = sample_conductivities(tags, 20000, 0.2, seed=3)
+    a = draw_lognormal(tags, 20000, 0.2, seed=3)
+    b = draw_lognormal(tags, 20000, 0.2, seed=3)
     np.testing.assert_array_equal(a, b)
     s0 = np.array([0.126, 0.275, 1.654, 0.010, 0.465])
     np.testing.assert_allclose(a.mean(axis=0), s0, rtol=0.01)  # mean-preserving form
     np.testing.assert_allclose(np.log(a).std(axis=0), 0.2, rtol=0.03)
-    med = sample_conductivities(tags, 20000, 0.2, seed=4, mean_preserving=False)
+    med = draw_lognormal(tags, 20000, 0.2, seed=4, mean_preserving=False)
     np.testing.assert_allclose(np.median(med, axis=0), s0, rtol=0.02)
-    assert not np.array_equal(a, sample_conductivities(tags, 20000, 0.2, seed=5))
+    assert not np.array_equal(a, draw_lognormal(tags, 20000, 0.2, seed=5))
 
 
 @pytest.mark.gpu
@@ -29,7 +29,7 @@ def test_uq_samples_match_per_sample_fp64_solves(head4, coil, oracle_mod):
         site = synthetic.SPHERE_SITE
         tags = ctx.uq_prepare(coil.positions_m, coil.moments, site, 1e6)
         assert tags == [1, 2, 3, 4, 5]
-        sigma = uq.sample_conductivities(tags, 5, 0.2, seed=11)
+        sigma = uq.draw_lognormal(tags, 5, 0.2, seed=11)
         sigma[0] = [fem.TISSUE_CONDUCTIVITY[t] for t in tags]  # the nominal table
         res = ctx.uq_solve(sigma)
         nominal = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, [site], 1e6)[0]
@@ -94,3 +94,104 @@ def test_per_placement_conductivities_match_fp64_solves(head4, coil, oracle_mod)
             fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites, 1e6, sigma=bad)
     finally:
         ctx.free()
+
+
+def test_reference_sampling_law_per_tissue_cov():
+    """config.py:56-72: log-normal with E[sigma] = sigma_0 and the tissue's coefficient of
+    variation; the normal law clamps at 1e-6 sigma_0; overrides replace the default table."""
+    from cunibs_amd import fem
+    from cunibs_amd.uq import DEFAULT_TISSUE_COV, ConductivityUQConfig, sample_conductivities
+    tags = (1, 2, 3, 5)
+    cfg = ConductivityUQConfig(n_samples=40000, seed=2)
+    a = sample_conductivities(cfg, tags)
+    assert a.shape == (40000, 4) and (a > 0).all()
+    np.testing.assert_array_equal(a, sample_conductivities(cfg, tags))
+    s0 = np.array([fem.TISSUE_CONDUCTIVITY[t] for t in tags])
+    cov = np.array([DEFAULT_TISSUE_COV[t] for t in tags])
+    np.testing.assert_allclose(a.mean(axis=0), s0, rtol=0.01)
+    np.testing.assert_allclose(a.std(axis=0) / a.mean(axis=0), cov, rtol=0.05)
+    # log sigma is normal with variance log(1 + cov^2)
+    np.testing.assert_allclose(np.log(a).var(axis=0), np.log1p(cov ** 2), rtol=0.05)
+    fixed = sample_conductivities(ConductivityUQConfig(n_samples=8, tissue_cov={t: 0.0 for t in tags}), tags)
+    np.testing.assert_array_equal(fixed, np.tile(s0, (8, 1)))
+    wide = sample_conductivities(ConductivityUQConfig(n_samples=20000, seed=1, distribution="normal",
+                                                      tissue_cov={1: 3.0, 2: 0.1, 3: 0.1, 5: 0.1}), tags)
+    assert wide[:, 0].min() == pytest.approx(1e-6 * s0[0])  # the clamp floor is reached
+    with pytest.raises(KeyError):
+        ConductivityUQConfig().cov_for(42)
+
+
+def test_tissue_sensitivity_recovers_a_log_linear_response():
+    """result.py:31-61: for y = c prod sigma_j^b_j the regression recovers b exactly, so the shares
+    are b_j^2 Var(log sigma_j) / Var(log y) (summing to one up to the draws' sample correlation);
+    an output that ignores a tissue gets zero share."""
+    from cunibs_amd.uq import ConductivityUQConfig, ConductivityUQResult, sample_conductivities
+    tags = (1, 2, 3)
+    sig = sample_conductivities(ConductivityUQConfig(n_samples=400, seed=5), tags)
+    b = np.array([2.0, 0.0, -1.0])
+    y = 0.7 * np.exp(np.log(sig) @ b)
+    r = ConductivityUQResult(None, None, None, 400, tags, sig, None, None, None, None, "c", 1e6, y, y * 2.0,
+                             np.zeros((400, 3)), {"roi": y ** 0.5})
+    share = r.tissue_sensitivity("peak")
+    lx = np.log(sig)
+    expect = b ** 2 * lx.var(axis=0) / np.log(y).var()
+    np.testing.assert_allclose([share[t] for t in tags], expect, rtol=1e-9, atol=1e-12)
+    assert sum(share.values()) == pytest.approx(1.0, abs=0.1)
+    assert share[2] == pytest.approx(0.0, abs=1e-12)
+    assert r.tissue_sensitivity("roi") == pytest.approx(share)  # a power of y has the same shares
+    with pytest.raises(ValueError, match="Unknown output"):
+        r.tissue_sensitivity("nope")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("recovery", ["raw", "harmonic"])
+def test_subject_uq_with_fixed_conductivities_reproduces_simulate(head4, coil, recovery):
+    """With every coefficient of variation zero each draw is the nominal model, so the Monte Carlo
+    mean is the deterministic field of Subject.simulate under the same recovery, its spread is
+    zero, and every draw's peak is the deterministic peak."""
+    from cunibs_amd import synthetic
+    from cunibs_amd.simulation import Placement, Subject
+    from cunibs_amd.uq import ConductivityUQConfig
+    subj = Subject(head4)
+    try:
+        place = Placement(*synthetic.SPHERE_SITE)
+        det = subj.simulate(coil, place, 1e6, magnitude=True, recovery=recovery)
+        cfg = ConductivityUQConfig(n_samples=6, tissue_cov={t: 0.0 for t in range(1, 11)})
+        r = subj.simulate_conductivity_uq(coil, place, cfg, 1e6, moments=True, recovery=recovery)
+        assert r.recovery == recovery and r.n_samples == 6 and r.perturbed_tags == (1, 2, 3, 4, 5)
+        assert r.sigma_samples.shape == (6, 5)
+        ref = det.magnE.astype(np.float64)
+        assert np.linalg.norm(r.mean_magnE - ref) <= 2e-5 * np.linalg.norm(ref)
+        assert float(np.max(r.std_magnE)) <= 1e-5 * float(ref.max())
+        np.testing.assert_allclose(r.peak_samples, det.peak_magnE(), rtol=2e-5)
+        assert r.peak_mean_magnE() == pytest.approx(det.peak_magnE(), rel=2e-5)
+        assert r.max_local_cov() <= 1e-4
+        lean = subj.simulate_conductivity_uq(coil, place, cfg, 1e6, recovery=recovery)
+        assert lean.mean_magnE is None and lean.summary is not None
+    finally:
+        subj.free()
+
+
+@pytest.mark.gpu
+def test_subject_uq_draws_spread_and_attribute_to_tissues(head4, coil):
+    """Perturbing gray matter alone: the draws spread the peak, the ROI records each draw's mean
+    |E| over the region it names, and the whole log-variance of the peak goes to that tissue."""
+    from cunibs_amd import synthetic
+    from cunibs_amd.simulation import Placement, Subject
+    from cunibs_amd.uq import ConductivityUQConfig
+    subj = Subject(head4)
+    try:
+        place = Placement(*synthetic.SPHERE_SITE)
+        gm = np.nonzero(subj.mesh.tet_tags == 2)[0]
+        w = np.full(gm.size, 1.0 / gm.size)
+        cfg = ConductivityUQConfig(n_samples=24, seed=3, perturbed_tags=(2,), tissue_cov={2: 0.3})
+        r = subj.simulate_conductivity_uq(coil, place, cfg, 1e6, moments=True, recovery="raw",
+                                          record_rois={"gm_mean": (gm, w)})
+        assert r.sigma_samples.shape == (24, 1) and r.perturbed_tags == (2,)
+        assert np.std(r.peak_samples) > 1e-3 * np.mean(r.peak_samples)
+        assert r.roi_samples["gm_mean"].shape == (24,) and (r.roi_samples["gm_mean"] > 0).all()
+        share = r.tissue_sensitivity("peak")
+        assert set(share) == {2} and 0.5 < share[2] <= 1.0 + 1e-9
+        assert r.iterations is not None and (r.iterations > 0).all()
+    finally:
+        subj.free()
