@@ -13,11 +13,18 @@ defines for files that h5py writes with its default settings, plus the newer obj
 * attributes of those types, scalar or simple dataspaces.
 
 Anything else raises ``NotImplementedError`` naming the structure, never a silent misread.
+
+``write_file`` writes the matching subset: superblock version 0, version-1 object headers,
+symbol-table groups, contiguous datasets of little-endian integers and IEEE floats, and
+attributes of those types or of variable-length UTF-8 strings (one global heap collection). Each
+structure is encoded as the HDF5 library encodes it in files h5py writes with its defaults (the
+datatype, dataspace, fill-value and attribute encodings follow such a file byte for byte).
 """
 from __future__ import annotations
 
 import zlib
 from pathlib import Path
+from typing import NamedTuple
 
 import numpy as np
 
@@ -423,3 +430,189 @@ def open_file(path: str | Path) -> Node:
     p = Path(path)
     r = _Reader(p.read_bytes(), str(p))
     return Node(r, r.root, "/")
+
+
+# ------------------------------------------------------------------------------------------------
+# Writer
+# ------------------------------------------------------------------------------------------------
+
+_ATTRS = "__attrs__"
+_FREE_NULL = 1  # a local heap's "no free block" offset, as the HDF5 library stores it
+_INTERNAL_K = 16
+_VLEN_UTF8 = bytes.fromhex("1901010010000000100000000100000000000800")
+_DEFLATE_PIPELINE = bytes.fromhex("010100000000000001000800010001006465666c617465000400000000000000")
+_CHUNK_K = 32  # the B-tree K of chunk indexes (a version-0 superblock does not store it)
+
+
+class DatasetSpec(NamedTuple):
+    """A dataset to write together with its attributes."""
+
+    data: np.ndarray
+    attrs: dict
+
+
+def _pad8(b: bytes) -> bytes:
+    return b + bytes(-len(b) % 8)
+
+
+def _u(v: int, width: int) -> bytes:
+    return int(v).to_bytes(width, "little")
+
+
+def _numeric_type(dt: np.dtype) -> bytes:
+    dt = np.dtype(dt)
+    if dt.kind in "iu":
+        return bytes([0x10, 0x08 if dt.kind == "i" else 0x00, 0, 0]) + _u(dt.itemsize, 4) + _u(0, 2) + \
+            _u(8 * dt.itemsize, 2)
+    if dt == np.float64:
+        return bytes.fromhex("11203f00") + _u(8, 4) + _u(0, 2) + _u(64, 2) + bytes([52, 11, 0, 52]) + _u(1023, 4)
+    if dt == np.float32:
+        return bytes.fromhex("11201f00") + _u(4, 4) + _u(0, 2) + _u(32, 2) + bytes([23, 8, 0, 23]) + _u(127, 4)
+    raise NotImplementedError(f"writing dtype {dt}")
+
+
+def _dataspace(shape: tuple[int, ...]) -> bytes:
+    return bytes([1, len(shape), 0, 0]) + bytes(4) + b"".join(_u(d, 8) for d in shape)
+
+
+def _as_array(value) -> np.ndarray:
+    a = np.asarray(value)
+    if a.dtype == bool:
+        a = a.astype(np.uint8)
+    if a.dtype.kind in "iuf":
+        return a.astype(a.dtype.newbyteorder("<"), order="C")  # keeps 0-d values 0-d
+    raise NotImplementedError(f"writing values of dtype {a.dtype}")
+
+
+class _Writer:
+    def __init__(self, tree: dict, compress: bool) -> None:
+        self.buf = bytearray(96)  # the superblock, filled in last
+        self.compress = compress
+        self.strings: dict[str, int] = {}
+        self._collect(tree)
+        self.leaf_k = max(4, (self._widest(tree) + 1) // 2)
+        self.gcol = self._global_heap() if self.strings else 0
+        root, btree, heap = self._group(tree)
+        eof = len(self.buf)
+        sb = _SIGNATURE + bytes([0, 0, 0, 0, 0, 8, 8, 0]) + _u(self.leaf_k, 2) + _u(_INTERNAL_K, 2) + _u(0, 4)
+        sb += _u(0, 8) + _u(_UNDEFINED, 8) + _u(eof, 8) + _u(_UNDEFINED, 8)
+        sb += _u(0, 8) + _u(root, 8) + _u(1, 4) + _u(0, 4) + _u(btree, 8) + _u(heap, 8)
+        assert len(sb) == 96
+        self.buf[:96] = sb
+
+    def _collect(self, node) -> None:
+        if isinstance(node, (dict, DatasetSpec)):
+            attrs = node.attrs if isinstance(node, DatasetSpec) else node.get(_ATTRS, {})
+            for k, v in attrs.items():
+                for item in ([v] if isinstance(v, str) else (v if isinstance(v, (list, tuple)) and v
+                                                              and all(isinstance(x, str) for x in v) else [])):
+                    self.strings.setdefault(item, len(self.strings) + 1)
+        if isinstance(node, dict):
+            for k, v in node.items():
+                if k != _ATTRS:
+                    self._collect(v)
+
+    def _widest(self, node) -> int:
+        if not isinstance(node, dict):
+            return 0
+        kids = [v for k, v in node.items() if k != _ATTRS]
+        return max([len(kids)] + [self._widest(v) for v in kids])
+
+    def put(self, data: bytes) -> int:
+        addr = len(self.buf)
+        self.buf += _pad8(bytes(data))
+        return addr
+
+    def _global_heap(self) -> int:
+        objs = b""
+        for text, idx in self.strings.items():
+            raw = text.encode()
+            objs += _u(idx, 2) + _u(0, 2) + bytes(4) + _u(len(raw), 8) + _pad8(raw)
+        size = max(4096, 16 + len(objs) + 16)
+        free = size - 16 - len(objs)
+        body = b"GCOL" + bytes([1, 0, 0, 0]) + _u(size, 8) + objs + _u(0, 2) + _u(0, 2) + bytes(4) + _u(free, 8)
+        return self.put(body + bytes(size - len(body)))
+
+    def _attribute(self, name: str, value) -> bytes:
+        if isinstance(value, str) or (isinstance(value, (list, tuple)) and value
+                                      and all(isinstance(x, str) for x in value)):
+            items = [value] if isinstance(value, str) else list(value)
+            dtype, shape = _VLEN_UTF8, (() if isinstance(value, str) else (len(items),))
+            data = b"".join(_u(len(t.encode()), 4) + _u(self.gcol, 8) + _u(self.strings[t], 4) for t in items)
+        else:
+            a = _as_array(value)
+            dtype, shape, data = _numeric_type(a.dtype), a.shape, a.tobytes()
+        nm = name.encode() + b"\0"
+        space = _dataspace(shape)
+        return bytes([1, 0]) + _u(len(nm), 2) + _u(len(dtype), 2) + _u(len(space), 2) + _pad8(nm) + \
+            _pad8(dtype) + _pad8(space) + data
+
+    def _header(self, messages: list[tuple[int, int, bytes]]) -> int:
+        body = b""
+        for mtype, flags, data in messages:
+            data = _pad8(data)
+            body += _u(mtype, 2) + _u(len(data), 2) + bytes([flags, 0, 0, 0]) + data
+        return self.put(bytes([1, 0]) + _u(len(messages), 2) + _u(1, 4) + _u(len(body), 4) + bytes(4) + body)
+
+    def _attr_messages(self, attrs: dict) -> list[tuple[int, int, bytes]]:
+        return [(0x0C, 0, self._attribute(k, v)) for k, v in attrs.items()]
+
+    def _dataset(self, value, attrs: dict | None = None) -> int:
+        a = _as_array(value)
+        raw = a.tobytes()
+        msgs = [(0x01, 0, _dataspace(a.shape)), (0x03, 1, _numeric_type(a.dtype))]
+        if self.compress and a.ndim >= 1 and a.size > 0:
+            # One chunk spanning the array, deflated, indexed by a one-entry v1 B-tree.
+            packed = zlib.compress(raw, 4)
+            data = self.put(packed)
+            key = lambda offs: _u(len(packed), 4) + _u(0, 4) + b"".join(_u(o, 8) for o in offs)
+            node = b"TREE" + bytes([1, 0]) + _u(1, 2) + _u(_UNDEFINED, 8) + _u(_UNDEFINED, 8)
+            node += key([0] * (a.ndim + 1)) + _u(data, 8) + key(list(a.shape) + [0])
+            full = 24 + (2 * _CHUNK_K + 1) * (8 + 8 * (a.ndim + 1)) + 2 * _CHUNK_K * 8
+            btree = self.put(node + bytes(full - len(node)))
+            layout = bytes([3, 2, a.ndim + 1]) + _u(btree, 8) + b"".join(_u(d, 4) for d in a.shape) + \
+                _u(a.dtype.itemsize, 4)
+            msgs += [(0x05, 1, bytes([2, 3, 2, 1]) + _u(0, 4)), (0x0B, 1, _DEFLATE_PIPELINE), (0x08, 0, layout)]
+        else:
+            addr = self.put(raw) if raw else _UNDEFINED
+            msgs += [(0x05, 1, bytes([2, 2, 2, 1]) + _u(0, 4)),
+                     (0x08, 0, bytes([3, 1]) + _u(addr, 8) + _u(len(raw), 8))]
+        return self._header(msgs + self._attr_messages(attrs or {}))
+
+    def _group(self, node: dict) -> tuple[int, int, int]:
+        names = sorted((k for k in node if k != _ATTRS), key=lambda k: k.encode())
+        children = {}
+        for k in names:
+            v = node[k]
+            if isinstance(v, dict):
+                children[k] = self._group(v)[0]
+            elif isinstance(v, DatasetSpec):
+                children[k] = self._dataset(v.data, v.attrs)
+            else:
+                children[k] = self._dataset(v)
+        heap_data, offsets = bytearray(8), {}
+        for k in names:
+            offsets[k] = len(heap_data)
+            heap_data += _pad8(k.encode() + b"\0")
+        seg = self.put(bytes(heap_data))
+        heap = self.put(b"HEAP" + bytes(4) + _u(len(heap_data), 8) + _u(_FREE_NULL, 8) + _u(seg, 8))
+        entry = lambda k: _u(offsets[k], 8) + _u(children[k], 8) + _u(0, 4) + _u(0, 4) + bytes(16)
+        snod = b"SNOD" + bytes([1, 0]) + _u(len(names), 2) + b"".join(entry(k) for k in names)
+        snod_addr = self.put(snod + bytes(8 + 2 * self.leaf_k * 40 - len(snod)))
+        last = offsets[names[-1]] if names else 0
+        tree = b"TREE" + bytes([0, 0]) + _u(1, 2) + _u(_UNDEFINED, 8) + _u(_UNDEFINED, 8)
+        tree += _u(0, 8) + _u(snod_addr, 8) + _u(last, 8)
+        full = 24 + (2 * _INTERNAL_K + 1) * 8 + 2 * _INTERNAL_K * 8
+        btree = self.put(tree + bytes(full - len(tree)))
+        msgs = [(0x11, 0, _u(btree, 8) + _u(heap, 8))] + self._attr_messages(node.get(_ATTRS, {}))
+        return self._header(msgs), btree, heap
+
+
+def write_file(path: str | Path, tree: dict, compression: str | None = None) -> None:
+    """Write an HDF5 file. ``tree`` maps member names to arrays (datasets) or to nested dicts
+    (groups) or to ``DatasetSpec(array, attrs)``; the key ``"__attrs__"`` of a dict holds that
+    group's attributes (numbers, arrays of numbers, strings or lists of strings).
+    ``compression="gzip"`` stores each non-empty array as one deflated chunk."""
+    if compression not in (None, "gzip"):
+        raise ValueError(f"unsupported compression {compression!r}")
+    Path(path).write_bytes(bytes(_Writer(tree, compression == "gzip").buf))

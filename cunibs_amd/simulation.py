@@ -260,6 +260,81 @@ class FieldResult:
         return compute_metrics(magnE, self.vols, self.barycenters_mm, self.tet_tags, region,
                                focality_fracs=(frac,))["focality_m3"][key]
 
+    def save(self, path: str | Path) -> None:
+        """Write the result as an HDF5 file of the reference's format version 1
+        (simulation.py:805-834): the retained arrays as datasets (absent ones are left out), the
+        transform, a ``summary`` group, and the placement, coil, dI/dt and recovery as attributes."""
+        from . import h5lite
+        tree: dict = {name: getattr(self, name) for name in _SAVED_ARRAYS if getattr(self, name) is not None}
+        tree["transform"] = np.asarray(self.transform, np.float64)
+        tree["summary"] = _metrics_tree(self.summary)
+        attrs = {"format_version": np.int64(_FORMAT_VERSION), "recovery": self.recovery}
+        if self.n_nodes is not None:
+            attrs["n_nodes"] = np.int64(self.n_nodes)
+        attrs.update({"coil_name": self.coil_name, "didt": float(self.didt),
+                      "placement_center_mm": self.placement.center_mm,
+                      "placement_handle_mm": self.placement.handle_mm,
+                      "placement_distance_mm": float(self.placement.distance_mm)})
+        tree["__attrs__"] = attrs
+        h5lite.write_file(path, tree, compression="gzip")
+
+    @classmethod
+    def load(cls, path: str | Path) -> FieldResult:
+        """Read a result saved by ``save`` or by the reference (simulation.py:836-866)."""
+        from . import h5lite
+        f = h5lite.open_file(path)
+        stored = int(f.attrs.get("format_version", 0))
+        if stored != _FORMAT_VERSION:
+            raise ValueError(f"{path}: file format version {stored} is not readable by this version "
+                             f"(expected {_FORMAT_VERSION}).")
+        opt = {name: (f[name].read() if name in f else None) for name in _SAVED_ARRAYS}
+        a = f.attrs
+        return cls(summary=_read_metrics_tree(f["summary"]), transform=f["transform"].read(),
+                   placement=Placement(a["placement_center_mm"], a["placement_handle_mm"],
+                                       float(a["placement_distance_mm"])),
+                   coil_name=str(_decode_attr(a["coil_name"])), didt=float(a["didt"]),
+                   recovery=str(_decode_attr(a.get("recovery", "raw"))),
+                   n_nodes=int(a["n_nodes"]) if "n_nodes" in a else None, **opt)
+
+
+_FORMAT_VERSION = 1
+_SAVED_ARRAYS = ("magnE", "E", "v", "vols", "tet_tags", "barycenters_mm", "E_slots", "slot_node", "slot_tag",
+                 "tet_nodes")
+
+
+def _metrics_tree(m: dict) -> dict:
+    """A FieldMetrics dict as the ``summary`` group (simulation.py:56-65)."""
+    return {"__attrs__": {"region": m["region"], "peak_magnE": float(m["peak_magnE"]),
+                          "region_volume_m3": float(m["region_volume_m3"])},
+            "peak_location_mm": np.asarray(m["peak_location_mm"], np.float64),
+            "center_of_gravity_mm": np.asarray(m["center_of_gravity_mm"], np.float64),
+            "focality_m3": {"__attrs__": {k: float(v) for k, v in m["focality_m3"].items()}},
+            "distribution": {"__attrs__": {k: float(v) for k, v in m["distribution"].items()}}}
+
+
+def _read_metrics_tree(g) -> dict:
+    """simulation.py:68-77."""
+    return {"region": str(_decode_attr(g.attrs["region"])), "peak_magnE": float(g.attrs["peak_magnE"]),
+            "peak_location_mm": g["peak_location_mm"].read(),
+            "center_of_gravity_mm": g["center_of_gravity_mm"].read(),
+            "region_volume_m3": float(g.attrs["region_volume_m3"]),
+            "focality_m3": {k: float(v) for k, v in g["focality_m3"].attrs.items()},
+            "distribution": {k: float(v) for k, v in g["distribution"].attrs.items()}}
+
+
+def encode_ccd(ccd_file: str | Path, output_h5: str | Path) -> None:
+    """A SimNIBS ``.ccd`` dipole file as the ``.h5`` coil ``Coil.load`` reads (coil.py:85-98): the
+    rows as the ``dipoles`` dataset, the header's ``key=value`` pairs and ``num_elements`` as its
+    attributes, gzip-compressed."""
+    from . import h5lite
+    with Path(ccd_file).open() as f:
+        meta_line = f.readline().strip().lstrip("#").strip()
+        count = int(f.readline().strip())
+        data = np.loadtxt(f, dtype=np.float64, skiprows=1, max_rows=count).reshape(-1, 6)
+    attrs: dict = {k.strip(): v.strip() for k, v in (p.split("=", 1) for p in meta_line.split(";") if "=" in p)}
+    attrs["num_elements"] = np.int64(count)
+    h5lite.write_file(output_h5, {"dipoles": h5lite.DatasetSpec(data, attrs)}, compression="gzip")
+
 
 def _sites(placements, method: str) -> list[Placement]:
     if isinstance(placements, Placement):

@@ -122,3 +122,18 @@ def test_gpu_patch_field_matches_the_oracle(patch_subject, oracle_mod, oracle_pa
     ours = np.asarray(r.magnE)[gm]
     theirs = ref["magnE"][gm]
     assert np.abs(ours - theirs).max() <= 1e-3 * np.abs(theirs).max()
+
+
+@pytest.mark.gpu
+def test_gpu_field_result_round_trip(tmp_path, patch_subject, d70, patch_site):
+    """tests/test_realmesh.py:262-278: save/load at 41k elements, compressed."""
+    from cunibs_amd.simulation import FieldResult, Placement
+    r = patch_subject.simulate(d70, Placement(*patch_site), 1e6, magnitude=True, vectors=True, potential=True)
+    path = tmp_path / "patch.h5"
+    r.save(path)
+    loaded = FieldResult.load(path)
+    for name in ("E", "magnE", "v", "transform", "vols", "tet_tags", "barycenters_mm"):
+        np.testing.assert_array_equal(getattr(loaded, name), getattr(r, name), err_msg=name)
+    assert loaded.summary["peak_magnE"] == r.summary["peak_magnE"]
+    assert loaded.recovery == "harmonic" and loaded.coil_name == "D70"
+    assert path.stat().st_size < 3_000_000
