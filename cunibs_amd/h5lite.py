@@ -597,11 +597,11 @@ class _Writer:
         seg = self.put(bytes(heap_data))
         heap = self.put(b"HEAP" + bytes(4) + _u(len(heap_data), 8) + _u(_FREE_NULL, 8) + _u(seg, 8))
         entry = lambda k: _u(offsets[k], 8) + _u(children[k], 8) + _u(0, 4) + _u(0, 4) + bytes(16)
-        snod = b"SNOD" + bytes([1, 0]) + _u(len(names), 2) + b"".join(entry(k) for k in names)
-        snod_addr = self.put(snod + bytes(8 + 2 * self.leaf_k * 40 - len(snod)))
-        last = offsets[names[-1]] if names else 0
-        tree = b"TREE" + bytes([0, 0]) + _u(1, 2) + _u(_UNDEFINED, 8) + _u(_UNDEFINED, 8)
-        tree += _u(0, 8) + _u(snod_addr, 8) + _u(last, 8)
+        tree = b"TREE" + bytes([0, 0]) + _u(1 if names else 0, 2) + _u(_UNDEFINED, 8) + _u(_UNDEFINED, 8)
+        if names:  # an empty group's B-tree has no entries and no symbol node
+            snod = b"SNOD" + bytes([1, 0]) + _u(len(names), 2) + b"".join(entry(k) for k in names)
+            snod_addr = self.put(snod + bytes(8 + 2 * self.leaf_k * 40 - len(snod)))
+            tree += _u(0, 8) + _u(snod_addr, 8) + _u(offsets[names[-1]], 8)
         full = 24 + (2 * _INTERNAL_K + 1) * 8 + 2 * _INTERNAL_K * 8
         btree = self.put(tree + bytes(full - len(tree)))
         msgs = [(0x11, 0, _u(btree, 8) + _u(heap, 8))] + self._attr_messages(node.get(_ATTRS, {}))

@@ -147,6 +147,56 @@ class ConductivityUQResult:
     def to_numpy(self) -> ConductivityUQResult:
         return self
 
+    def save(self, path) -> None:
+        """An HDF5 file in the reference's layout (result.py:182-223): the retained moment arrays,
+        the draws, the summary, the per-draw scalars and ROI records, the run's attributes."""
+        from cunibs_amd import h5lite
+        from cunibs_amd.simulation import _metrics_tree
+        if self.summary is None:
+            raise ValueError("This result has no summary to save; call compute_summary() first "
+                             "(results from Subject already carry one).")
+        tree: dict = {name: getattr(self, name) for name in _UQ_ARRAYS if getattr(self, name) is not None}
+        tree["sigma_samples"] = np.asarray(self.sigma_samples, np.float64)
+        tree["summary"] = {"mean_field": _metrics_tree(self.summary.mean_field),
+                           "__attrs__": {"max_local_cov": float(self.summary.max_local_cov)}}
+        tree["roi_samples"] = {k: np.asarray(v, np.float64) for k, v in self.roi_samples.items()}
+        for name in ("peak_samples", "focality_samples", "peak_location_samples"):
+            tree[name] = np.asarray(getattr(self, name), np.float64)
+        tree["__attrs__"] = {
+            "format_version": np.int64(1), "recovery": self.recovery, "n_samples": np.int64(self.n_samples),
+            "perturbed_tags": np.asarray(self.perturbed_tags, np.int32), "coil_name": self.coil_name,
+            "didt": float(self.didt), "placement_center_mm": self.placement.center_mm,
+            "placement_handle_mm": self.placement.handle_mm,
+            "placement_distance_mm": float(self.placement.distance_mm)}
+        h5lite.write_file(path, tree, compression="gzip")
+
+    @classmethod
+    def load(cls, path) -> ConductivityUQResult:
+        """result.py:225-256."""
+        from cunibs_amd import h5lite
+        from cunibs_amd.simulation import Placement, _decode_attr, _read_metrics_tree
+        f = h5lite.open_file(path)
+        stored = int(f.attrs.get("format_version", 0))
+        if stored != 1:
+            raise ValueError(f"{path}: file format version {stored} is not readable by this version (expected 1).")
+        a = f.attrs
+        opt = {name: (f[name].read() if name in f else None) for name in _UQ_ARRAYS}
+        roi = f["roi_samples"]
+        return cls(n_samples=int(a["n_samples"]), perturbed_tags=tuple(int(t) for t in np.atleast_1d(a["perturbed_tags"])),
+                   sigma_samples=f["sigma_samples"].read(),
+                   placement=Placement(a["placement_center_mm"], a["placement_handle_mm"],
+                                       float(a["placement_distance_mm"])),
+                   coil_name=str(_decode_attr(a["coil_name"])), didt=float(a["didt"]),
+                   recovery=str(_decode_attr(a.get("recovery", "raw"))),
+                   summary=ConductivityUQSummary(_read_metrics_tree(f["summary"]["mean_field"]),
+                                                 float(f["summary"].attrs["max_local_cov"])),
+                   roi_samples={k: roi[k].read() for k in roi.keys()},
+                   peak_samples=f["peak_samples"].read(), focality_samples=f["focality_samples"].read(),
+                   peak_location_samples=f["peak_location_samples"].read(), **opt)
+
+
+_UQ_ARRAYS = ("mean_magnE", "std_magnE", "cov_magnE", "vols", "tet_tags", "barycenters_mm")
+
 
 def _roi_arrays(target) -> tuple[np.ndarray, np.ndarray]:
     if isinstance(target, tuple):
@@ -197,8 +247,9 @@ def conductivity_uq(ctx: fem.SolverContext, coil, placement, config: Conductivit
         focality_samples=foc, peak_location_samples=loc, roi_samples=roi, recovery=recovery, iterations=its)
     if vols is not None:
         result.summary = result.compute_summary()
-    if not moments:
-        result = replace(result, mean_magnE=None, std_magnE=None, cov_magnE=None)
+    if not moments:  # the per-element arrays come as a set (simulation.py:572-581)
+        result = replace(result, mean_magnE=None, std_magnE=None, cov_magnE=None, vols=None, tet_tags=None,
+                         barycenters_mm=None)
     return result
 
 

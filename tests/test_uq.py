@@ -195,3 +195,40 @@ def test_subject_uq_draws_spread_and_attribute_to_tissues(head4, coil):
         assert r.iterations is not None and (r.iterations > 0).all()
     finally:
         subj.free()
+
+
+@pytest.mark.parametrize("moments", [True, False])
+def test_uq_result_save_load_round_trip(tmp_path, moments):
+    """result.py:182-256 there: the draws, per-draw scalars, ROI records and summary come back;
+    moment arrays the run did not keep load as None."""
+    from cunibs_amd.simulation import Placement
+    from cunibs_amd.uq import ConductivityUQResult, ConductivityUQSummary
+    rng = np.random.default_rng(1)
+    m, n = 30, 7
+    summary = ConductivityUQSummary(
+        {"region": "gray_matter", "peak_magnE": 0.4, "peak_location_mm": np.array([1.0, 2.0, 3.0]),
+         "center_of_gravity_mm": np.array([0.0, 1.0, 2.0]), "region_volume_m3": 2e-6,
+         "focality_m3": {"0.5": 1e-7, "0.75": 5e-8},
+         "distribution": {"mean": 0.1, "std": 0.02, "p50": 0.1, "p95": 0.2, "p99": 0.3, "p99.9": 0.35}}, 0.12)
+    mom = (lambda a: a) if moments else (lambda a: None)
+    r = ConductivityUQResult(
+        mom(rng.random(m).astype(np.float32)), mom(rng.random(m).astype(np.float32)),
+        mom(rng.random(m).astype(np.float32)), n, (1, 2), rng.random((n, 2)), mom(rng.random(m).astype(np.float32)),
+        mom(rng.integers(1, 4, m).astype(np.int32)), mom(rng.random((m, 3))),
+        Placement([0.0, 0.0, 90.0], [0.0, 50.0, 90.0], 4.0), "D70", 1e6, rng.random(n), rng.random(n),
+        rng.random((n, 3)), {} if moments else {"m1": rng.random(n)}, summary, "harmonic")
+    r.save(tmp_path / "uq.h5")
+    b = ConductivityUQResult.load(tmp_path / "uq.h5")
+    for name in ("mean_magnE", "std_magnE", "cov_magnE", "vols", "tet_tags", "barycenters_mm"):
+        if getattr(r, name) is None:
+            assert getattr(b, name) is None
+        else:
+            np.testing.assert_array_equal(getattr(b, name), getattr(r, name))
+    for name in ("sigma_samples", "peak_samples", "focality_samples", "peak_location_samples"):
+        np.testing.assert_array_equal(getattr(b, name), getattr(r, name))
+    assert set(b.roi_samples) == set(r.roi_samples)
+    for k in r.roi_samples:
+        np.testing.assert_array_equal(b.roi_samples[k], r.roi_samples[k])
+    assert (b.n_samples, b.perturbed_tags, b.coil_name, b.didt, b.recovery) == (n, (1, 2), "D70", 1e6, "harmonic")
+    assert b.summary.max_local_cov == 0.12 and b.summary.mean_field["peak_magnE"] == 0.4
+    assert b.tissue_sensitivity("peak") == pytest.approx(r.tissue_sensitivity("peak"))
