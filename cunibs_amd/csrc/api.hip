@@ -784,6 +784,7 @@ void uq_solve(cn_context& c, const double* sigma, int k, const cn_solve_options*
         opt.drop_tol = sopt->drop_tol;
         opt.use_graph = sopt->use_graph != 0;
         opt.check_every = std::max(1, sopt->check_every);
+        opt.independent = sopt->independent != 0;
     }
     c.d_X.ensure(nr * K, "solution");
     const double* dx0 = nullptr;
@@ -987,6 +988,7 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
         opt.drop_tol = sopt->drop_tol;
         opt.use_graph = sopt->use_graph != 0;
         opt.check_every = std::max(1, sopt->check_every);
+        opt.independent = sopt->independent != 0;
     }
     c.d_X.ensure(nr * K, "solution");
     const double* dx0 = nullptr;
@@ -1176,6 +1178,7 @@ void solve_sweep(cn_context& c, const double* pos_m, const double* mom, std::int
         opt.drop_tol = sopt->drop_tol;
         opt.use_graph = sopt->use_graph != 0;
         opt.check_every = std::max(1, sopt->check_every);
+        opt.independent = sopt->independent != 0;
     }
     const bool warm = sopt && sopt->warm_start == 1;
     const int mode = sopt ? sopt->recovery : CN_RECOVERY_RAW;
@@ -1278,6 +1281,7 @@ void cn_default_solve_options(cn_solve_options* o) {
     o->drop_tol = 1e-6;
     o->use_graph = 1;
     o->check_every = 4;
+    o->independent = 0;
 }
 
 int32_t cn_context_create(const double* nodes_mm, int64_t n_nodes, const int32_t* tet_nodes, const int32_t* tet_tags,

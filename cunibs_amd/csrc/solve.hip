@@ -1465,7 +1465,7 @@ void BlockSolver::iteration_k(const SolveOptions& opt, int j) {
     spmm_block_k<K>(st, w_in);
     reduce_cols(partials_.get(), kGramBlocks, 2 * K * K, red_.get(), st, s_);
     coef_kernel<K><<<1, kWave, 0, s_>>>(red_.get(), kGramBlocks, opt.drop_tol, G_.get(), T_.get(), TT_.get(), al, stw,
-                                        multi_ ? 1 : 0, ran_.get() + j);
+                                        (multi_ || opt.independent) ? 1 : 0, ran_.get() + j);
     update_r_kernel<K><<<kGramBlocks, xr_threads<K>(), 0, s_>>>(nn, S_.get(), al, R_.get(), partials_.get(), rpb, st);
     reduce_cols(partials_.get(), kGramBlocks, K, red_.get(), st, s_);
     converge_kernel<K><<<1, small_threads<K>(), 0, s_>>>(red_.get(), kGramBlocks, target_.get(), norms_.get(), stw);
@@ -1475,7 +1475,7 @@ void BlockSolver::iteration_k(const SolveOptions& opt, int j) {
     launch_gram<K, false>(nn, S_.get(), Z_.get(), nullptr, partials_.get(), rpb, st, s_);
     reduce_cols(partials_.get(), kGramBlocks, K * K, red_.get(), st, s_);
     dmat_kernel<K><<<1, small_threads<K>(), 0, s_>>>(red_.get(), kGramBlocks, TT_.get(), D_.get(), stw,
-                                                multi_ ? 1 : 0);
+                                                (multi_ || opt.independent) ? 1 : 0);
     if (last)  // before this iteration's W update, which overwrites slot 0
         flush_x_kernel<K><<<grid_for(nn * col_groups<K>(), kRowThreads), kRowThreads, 0, s_>>>(
             nn, wslot(0), wslot(1), wslot(2), wslot(3), AL_.get(), ran_.get(), kLag, X64_.get());
@@ -1561,7 +1561,8 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
         d2d(W_.get(), Z_.get(), static_cast<std::size_t>(nk), s_);
         int slot = 0;
         while (kBlockWidths[slot] != K) ++slot;
-        if (opt.use_graph && (graphs_[slot] == nullptr || graph_drop_[slot] != opt.drop_tol)) {
+        const int diag = (multi_ || opt.independent) ? 1 : 0;
+        if (opt.use_graph && (graphs_[slot] == nullptr || graph_drop_[slot] != opt.drop_tol || graph_diag_[slot] != diag)) {
             // a whole window (j0 = 0, L = kLag), then one iteration at each ring position
             for (int v = -1; v < kLag; ++v) {
                 hipGraphExec_t& gx = v < 0 ? graphs_[slot] : graphs1_[slot][v];
@@ -1578,6 +1579,7 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
                 (void)hipGraphDestroy(g);
             }
             graph_drop_[slot] = opt.drop_tol;
+            graph_diag_[slot] = diag;
         }
         // Bounded by the launches the host issues, not only by the device counter.
         int launched = total_iters;

@@ -31,6 +31,7 @@ struct SolveOptions {
     double drop_tol = 1e-6;     // pivoted Cholesky: relative Schur pivot below which a direction drops
     int check_every = 4;        // graph replays between host convergence reads
     bool use_graph = true;
+    bool independent = false;   // k independent CG chains (diagonal Grams) instead of block CG
 };
 
 struct SolveStats {
@@ -181,6 +182,7 @@ class BlockSolver {
     hipGraphExec_t graphs_[kNumBlockWidths] = {};          // a whole window of kLag iterations
     hipGraphExec_t graphs1_[kNumBlockWidths][kLag] = {};  // one iteration at each ring position
     double graph_drop_[kNumBlockWidths] = {};  // the drop tolerance each graph was captured with
+    int graph_diag_[kNumBlockWidths] = {};     // and whether its Grams were diagonal
     // Inner iterations the last solve at each width took in its first round: the next solve runs
     // the whole windows short of that count before its first convergence read, then reads after
     // every iteration for a few, so a sweep of similar blocks neither waits on early reads nor

@@ -617,8 +617,11 @@ def solve_placements_block(
     warm_start: bool = False,
     collect: bool = False,
     sigma: np.ndarray | None = None,
+    independent: bool = False,
 ) -> list[PlacementResult]:
-    """Solve up to MAX_BLOCK placements as one block (fem/solve.py:865-979). ``recovery`` selects
+    """Solve up to MAX_BLOCK placements as one block (fem/solve.py:865-979). ``independent`` runs
+    the block as k independent CG chains in lockstep, as the reference's solver does, instead of
+    the true block method. ``recovery`` selects
     the field E, |E| and the metrics describe ("raw", "spr_global", "spr_tissue", "harmonic");
     ``slots`` also returns the recovered field on its slots. ``sigma`` (k x n_tags, S/m by tag,
     see `conductivity_rows`) gives every placement its own conductivities; the block then runs as
@@ -673,6 +676,7 @@ def solve_placements_block(
     opt.tolerance = float(tolerance)
     opt.max_iters = int(max_iters)
     opt.use_graph = int(use_graph)
+    opt.independent = int(independent)
     sig = None
     if sigma is not None:
         sig = np.ascontiguousarray(sigma, np.float64)

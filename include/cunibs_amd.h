@@ -151,6 +151,11 @@ typedef struct {
      * present in the mesh. */
     const double* sigma;
     int32_t n_sigma;
+    /* 1: the block runs as k independent CG chains in lockstep (the Gram matrices restricted to
+     * their diagonals, each column with its own step lengths and its own freeze), the reference's
+     * block solver (solver/amg/solver.cpp:84-198, fem/solve.py:508-552); 0 (default): the true
+     * block method, whose shared Krylov space converges in fewer iterations. */
+    int32_t independent;
 } cn_solve_options;
 
 void cn_default_solve_options(cn_solve_options* out);
