@@ -49,9 +49,10 @@ def _solve(ctx, coil, sites, independent):
 @pytest.mark.parametrize("k", [2, 3, 4, 5, 8])
 def test_block_width_does_not_reach_the_answer(patch_ctx, patch_sites, k):
     """tests/test_block_solve.py:194-209 and :215-232 there: solved k at a time in independent mode,
-    every placement's |E|, iteration count and residual equal its solve alone. Up to a padded width
-    of 4 every kernel sums each column in the same order as at width 1, so the equality is bitwise;
-    the width-8 kernels use more lanes per row and agree within the solver's tolerance."""
+    every placement's |E|, potential and iteration count equal its solve alone. Up to a padded
+    width of 4 every solver kernel sums each column in the same order as at width 1, so the field
+    is bitwise equal (the reported residual norm to 1e-9); the width-8 kernels use more lanes per
+    row and agree within the solver's tolerance."""
     from cunibs_amd.simulation import Coil
     ctx, _ = patch_ctx
     coil = Coil.load(GOLDEN / "MagStim_D70.h5")
@@ -68,7 +69,9 @@ def test_block_width_does_not_reach_the_answer(patch_ctx, patch_sites, k):
         for i, (g, r) in enumerate(zip(got, ref)):
             np.testing.assert_array_equal(g.magnE, r.magnE, err_msg=f"k={k}, placement {i}")
             np.testing.assert_array_equal(g.v, r.v, err_msg=f"k={k}, placement {i}")
-            assert (g.iterations, g.relative_residual) == (r.iterations, r.relative_residual)
+            assert g.iterations == r.iterations
+            # the reported FP64 residual norm sums its rows in a width-dependent order
+            assert g.relative_residual == pytest.approx(r.relative_residual, rel=1e-9)
 
 
 @pytest.mark.gpu
