@@ -206,7 +206,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the placement sweep shards "
-              "over GPUs by placement, see DESIGN.md", file=sys.stderr)
+              "over GPUs by placement (each GPU a contiguous share of the placements)", file=sys.stderr)
         return 2
 
     from cunibs_amd import fem, synthetic

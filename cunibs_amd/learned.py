@@ -2,7 +2,7 @@
 maps a placement's pose and the tissue conductivities to the Galerkin coefficients of its
 solution in the reduced basis of solver-generated solutions; the solve then starts from the
 basis expansion of the prediction (cn_warm_set_coefficients + warm_start = 2) and still iterates
-to the same tolerance, so the plain method remains the fallback and the judge.
+to the same tolerance, so the plain method remains the fallback and the arbiter of accuracy.
 
 No reference counterpart: cuNIBS has no learned component.
 """
