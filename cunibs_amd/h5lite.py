@@ -422,7 +422,11 @@ class Node:
 
     @property
     def shape(self) -> tuple[int, ...]:
-        return tuple(self.read().shape)
+        """From the dataspace message alone; the data is not read."""
+        if not self.is_dataset:
+            raise TypeError(f"{self.name} is a group")
+        space = self._r.dataspace(next(b for t, b in self._msgs if t == 0x01))
+        return (0,) if space is None else space
 
 
 def open_file(path: str | Path) -> Node:
