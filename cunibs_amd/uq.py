@@ -72,27 +72,30 @@ class ConductivityUQSummary:
 
 
 def _tissue_sensitivity(result, output: str) -> dict[int, float]:
-    """result.py:31-61: regress log of the per-draw output on the log conductivity draws; each
-    tag's share is beta_j^2 Var(log sigma_j) / Var(log y). Not a Sobol index."""
-    if output == "peak":
-        y = result.peak_samples
-    elif output == "focality":
-        y = result.focality_samples
-    elif output in result.roi_samples:
-        y = result.roi_samples[output]
-    else:
-        raise ValueError(f"Unknown output {output!r}; expected 'peak', 'focality', or a recorded ROI name "
-                         f"(available: {['peak', 'focality', *result.roi_samples]}).")
-    x = np.log(np.asarray(result.sigma_samples, np.float64))
-    logy = np.log(np.asarray(y, np.float64))
-    xc = x - x.mean(0)
-    yc = logy - logy.mean()
-    var_y = float(yc.var())
-    if var_y == 0.0:
-        return {int(t): 0.0 for t in result.perturbed_tags}
-    beta, *_ = np.linalg.lstsq(xc, yc, rcond=None)
-    contrib = beta ** 2 * xc.var(0) / var_y
-    return {int(t): float(c) for t, c in zip(result.perturbed_tags, contrib)}
+    """First-order variance shares of a log-linear surrogate (the quantity of result.py:31-61).
+
+    Model: log y = a + sum_j b_j u_j with u_j = log sigma_j, fitted by least squares over the
+    draws. Under that model (and uncorrelated draws) Var(log y) = sum_j b_j^2 Var(u_j), so tissue
+    j's share of the output's log-variance is b_j^2 Var(u_j) / Var(log y). The slopes solve the
+    normal equations Cov(u) b = Cov(u, log y); a pseudo-inverse keeps a tissue whose draws do not
+    vary (zero variance) at slope zero. Not a Sobol index."""
+    named = {"peak": result.peak_samples, "focality": result.focality_samples, **result.roi_samples}
+    if output not in named:
+        raise ValueError(f"Unknown output {output!r}: choose one of {sorted(named)}")
+    u = np.log(np.asarray(result.sigma_samples, np.float64))
+    v = np.log(np.asarray(named[output], np.float64))
+    n = v.shape[0]
+    du = u - u.sum(axis=0) / n
+    dv = v - v.sum() / n
+    var_v = float(dv @ dv) / n
+    tags = [int(t) for t in result.perturbed_tags]
+    if var_v == 0.0:
+        return dict.fromkeys(tags, 0.0)
+    cov_uu = du.T @ du / n
+    cov_uv = du.T @ dv / n
+    slopes = np.linalg.pinv(cov_uu, rcond=1e-12, hermitian=True) @ cov_uv
+    shares = slopes * slopes * np.diag(cov_uu) / var_v
+    return {t: float(s) for t, s in zip(tags, shares)}
 
 
 @dataclass

@@ -143,6 +143,32 @@ def test_tissue_sensitivity_recovers_a_log_linear_response():
         r.tissue_sensitivity("nope")
 
 
+def test_tissue_sensitivity_on_an_orthogonal_design():
+    """Closed form: on a two-level full factorial design the centred log-conductivity columns are
+    exactly orthogonal, so a log-linear response with slopes b has shares b_j^2 s_j^2 /
+    sum_k b_k^2 s_k^2 (half-ranges s_j), summing to one; a tissue held fixed gets share 0 and a
+    constant output gives all zeros."""
+    import itertools
+    from cunibs_amd.uq import ConductivityUQResult
+    tags = (1, 2, 3, 5)
+    half = np.array([0.1, 0.3, 0.2, 0.0])  # tag 5 never varies
+    levels = np.array(list(itertools.product((-1.0, 1.0), repeat=3)))
+    logs = np.concatenate([levels * half[:3], np.zeros((8, 1))], axis=1) + np.log([0.1, 0.3, 1.6, 0.5])
+    b = np.array([1.5, -0.5, 2.0, 7.0])
+    y = np.exp(0.3 + (logs - logs.mean(0)) @ b)
+
+    def make(out):
+        return ConductivityUQResult(None, None, None, 8, tags, np.exp(logs), None, None, None, None, "c", 1e6,
+                                    out, np.ones(8), np.zeros((8, 3)), {})
+
+    share = make(y).tissue_sensitivity("peak")
+    w = b[:3] ** 2 * half[:3] ** 2
+    np.testing.assert_allclose([share[t] for t in tags[:3]], w / w.sum(), rtol=1e-12)
+    assert share[5] == 0.0
+    assert sum(share.values()) == pytest.approx(1.0, rel=1e-12)
+    assert make(y).tissue_sensitivity("focality") == dict.fromkeys(tags, 0.0)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("recovery", ["raw", "harmonic"])
 def test_subject_uq_with_fixed_conductivities_reproduces_simulate(head4, coil, recovery):
