@@ -107,6 +107,36 @@ def kernels_only(ctx, coil, sites, k: int) -> int:
     return 0
 
 
+def sweep_only(ctx, coil, sites, args) -> int:
+    """The pipelined cold sweep alone (warmup, then steps x block_k placements), with the library's
+    in-sweep device-clock timing: a kernel trace of this command averages the differenced SpMM
+    over exactly the launches the bench's in-sweep roofline averages."""
+    from cunibs_amd import fem
+
+    k = args.block_k
+    ctx.set_tuning(degree=2, lower_ratio=8.0, fine_sub=max(args.fine_sub, 0), coil_blocks=args.coil_blocks)
+    pos, mom = coil.positions_m, coil.moments
+    run = lambda n: fem.solve_sweep(ctx, pos, mom, [sites[j % len(sites)] for j in range(n * k)], 1e6, k)
+    ctx.timing_enable(True)
+    run(args.warmup)
+    ctx.timing_enable(True)  # reset the in-solve sums
+    t0 = time.perf_counter()
+    res = run(args.steps)
+    elapsed = time.perf_counter() - t0
+    timing = ctx.timing_read()
+    n_red, nnz = int(ctx.info.n_reduced), int(ctx.info.nnz_reduced)
+    K = k if k in fem.BLOCK_SIZES else min(b for b in fem.BLOCK_SIZES if b >= k)
+    print(json.dumps({
+        "sweep_placements_per_s": round(args.steps * k / elapsed, 2), "block_k": k, "tuning": ctx.tuning,
+        "iterations_mean": round(float(np.mean(res["iterations"])), 2),
+        "spmm_us": round(timing["spmm_ms"] * 1e3, 2), "spmm_samples": timing["samples"],
+        "spmm_algorithmic_bytes": spmm_diff_bytes(n_red, nnz, K),
+        "vcycle_us": round(timing["vcycle_ms"] * 1e3, 2), "iteration_us": round(timing["iteration_ms"] * 1e3, 2),
+    }), flush=True)
+    ctx.free()
+    return 0
+
+
 def learned_leg(ctx, pos, mom, n_train: int, n_test: int, log_sd: float = 0.2) -> dict:
     """The learned warm start on poses and conductivities (distance U[2, 8] mm, the five tissues of
     the head log-normal around the table): n_train solver-generated solutions, each with its own
@@ -203,6 +233,11 @@ def main() -> int:
                     help="1: also time the SpMM and V-cycle in isolated repeated launches (warm caches)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the roofline launches (copy calibration + differenced SpMM), for counter runs")
+    ap.add_argument("--sweep-only", action="store_true",
+                    help="only the pipelined cold sweep at --block-k, --fine-sub and --coil-blocks (no tuning, no "
+                         "blocks solved alone), so that a kernel trace's averages are the in-sweep ones")
+    ap.add_argument("--coil-blocks", type=int, default=256,
+                    help="--sweep-only: grid cap of the overlapped dA/dt")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the placement sweep shards "
@@ -226,6 +261,8 @@ def main() -> int:
         if args.fine_sub >= 0:
             ctx.set_tuning(fine_sub=args.fine_sub)
         return kernels_only(ctx, coil, sites, k)
+    if args.sweep_only:
+        return sweep_only(ctx, coil, sites, args)
 
     def block(i0: int, kk: int, warm: bool = False):
         chunk = [sites[(i0 + j) % len(sites)] for j in range(kk)]

@@ -92,6 +92,9 @@ struct cn_context {
     cn::DeviceBuffer<std::int32_t> d_degen;
     cn::DeviceBuffer<float> d_dip, d_dadt_nodes, d_dadt_elm, d_q, d_B, d_E, d_magn, d_rhs, d_stage;
     cn::DeviceBuffer<double> d_X, d_v, d_x0;
+    // Pinned sources of the host-to-device copies a block issues without waiting on them: the
+    // packed dipoles (coil stage, either stream) and the block's own small tables.
+    cn::PinnedStage stage_coil, stage_blk;
     // Pipelined sweeps (cn_solve_sweep): the next block's coil stage (dA/dt, element pass, RHS)
     // runs on s2 into the second buffer set while the current block solves on s.
     hipStream_t s2 = nullptr;
@@ -492,30 +495,34 @@ void pack_dipoles(const double* tf, const double* pos_m, const double* mom, std:
 void dadt_for_frames(cn_context& c, const double* tf, int k, int K, const double* pos_m, const double* mom,
                      std::int64_t n_dip, float scale, float* out, hipStream_t s, int max_blocks) {
     using namespace cn;
-    std::vector<double> origins(3 * k);
+    // The packed dipoles (and the moment directions) and the origins are built in the pinned
+    // stage and copied from there: this function returns before the copies have run.
     double dir[3];
-    if (common_moment_direction(mom, n_dip, dir)) {
-        std::vector<float> dip(static_cast<std::size_t>(8 * n_dip * k)), dirs(3 * k);
-        for (int i = 0; i < k; ++i)
-            pack_dipoles_parallel(tf + 16 * i, pos_m, mom, n_dip, dir, dip.data() + 8 * n_dip * i,
-                                  origins.data() + 3 * i, dirs.data() + 3 * i);
-        c.d_dip.ensure(dip.size() + dirs.size(), "dipoles");
-        c.d_origins.ensure(3 * k, "origins");
-        h2d(c.d_dip.get(), dip.data(), dip.size(), s);
-        h2d(c.d_dip.get() + dip.size(), dirs.data(), dirs.size(), s);
-        h2d(c.d_origins.get(), origins.data(), 3 * k, s);
-        dadt_nbody_parallel(c.d_dip.get(), static_cast<int>(n_dip), c.nodes.get(), c.n, c.d_origins.get(),
-                            c.d_dip.get() + dip.size(), k, K, scale, out, s, max_blocks);
-    } else {
-        std::vector<float> dip(static_cast<std::size_t>(10 * n_dip * k));
-        for (int i = 0; i < k; ++i)
-            pack_dipoles(tf + 16 * i, pos_m, mom, n_dip, dip.data() + 10 * n_dip * i, origins.data() + 3 * i);
-        c.d_dip.ensure(dip.size(), "dipoles");
-        c.d_origins.ensure(3 * k, "origins");
-        h2d(c.d_dip.get(), dip.data(), dip.size(), s);
-        h2d(c.d_origins.get(), origins.data(), 3 * k, s);
-        dadt_nbody(c.d_dip.get(), static_cast<int>(n_dip), c.nodes.get(), c.n, c.d_origins.get(), k, K, scale, out, s);
+    const bool parallel = common_moment_direction(mom, n_dip, dir);
+    const std::size_t n_pack = static_cast<std::size_t>((parallel ? 8 : 10) * n_dip * k);
+    const std::size_t n_dirs = parallel ? 3 * static_cast<std::size_t>(k) : 0;
+    const std::size_t f_bytes = (n_pack + n_dirs) * sizeof(float);
+    const std::size_t o_off = (f_bytes + 15) / 16 * 16;
+    char* host = static_cast<char*>(c.stage_coil.fill(o_off + 3 * sizeof(double) * k));
+    float* dip = reinterpret_cast<float*>(host);
+    double* origins = reinterpret_cast<double*>(host + o_off);
+    for (int i = 0; i < k; ++i) {
+        if (parallel)
+            pack_dipoles_parallel(tf + 16 * i, pos_m, mom, n_dip, dir, dip + 8 * n_dip * i, origins + 3 * i,
+                                  dip + n_pack + 3 * i);
+        else
+            pack_dipoles(tf + 16 * i, pos_m, mom, n_dip, dip + 10 * n_dip * i, origins + 3 * i);
     }
+    c.d_dip.ensure(n_pack + n_dirs, "dipoles");
+    c.d_origins.ensure(3 * k, "origins");
+    c.stage_coil.send(c.d_dip.get(), 0, f_bytes, s);
+    c.stage_coil.send(c.d_origins.get(), o_off, 3 * sizeof(double) * k, s);
+    c.stage_coil.done(s);
+    if (parallel)
+        dadt_nbody_parallel(c.d_dip.get(), static_cast<int>(n_dip), c.nodes.get(), c.n, c.d_origins.get(),
+                            c.d_dip.get() + n_pack, k, K, scale, out, s, max_blocks);
+    else
+        dadt_nbody(c.d_dip.get(), static_cast<int>(n_dip), c.nodes.get(), c.n, c.d_origins.get(), k, K, scale, out, s);
 }
 
 void place_block(cn_context& c, const double* centers, const double* handles, const double* dists, int k,
@@ -666,7 +673,7 @@ void uq_initial_guess(cn_context& c) {
     std::vector<double> sig0(T);
     for (int t = 0; t < T; ++t) sig0[t] = c.lut[c.tissue.tags[t]];
     DeviceBuffer<double> dsig(T, "nominal sigma");
-    h2d(dsig.get(), sig0.data(), T, s);
+    c.stage_blk.put(dsig.get(), sig0.data(), T, s);
     c.d_B.ensure(nr * 1, "rhs block");
     combine_tissues(u.rhs.get(), nr, T, dsig.get(), 1, 1, c.d_B.get(), s);
     c.d_X.ensure(nr * 1, "solution");
@@ -724,8 +731,11 @@ void uq_prepare(cn_context& c, const double* pos_m, const double* mom, std::int6
     std::vector<float> dip(static_cast<std::size_t>(10 * n_dip));
     std::vector<double> origin(3);
     pack_dipoles(tf.data(), pos_m, mom, n_dip, dip.data(), origin.data());
-    auto ddip = upload(dip.data(), dip.size(), s, "dipoles");
-    auto dorg = upload(origin.data(), 3, s, "origin");
+    DeviceBuffer<float> ddip(dip.size(), "dipoles");
+    DeviceBuffer<double> dorg(3, "origin");
+    h2d(ddip.get(), dip.data(), dip.size(), s);
+    h2d(dorg.get(), origin.data(), 3, s);
+    sync(s, "dipole upload");  // the sources are function-local
     u.dadt_nodes1.reset(3 * n, "uq dadt nodes");
     dadt_nbody(ddip.get(), static_cast<int>(n_dip), c.nodes.get(), n, dorg.get(), 1, 1,
                static_cast<float>(didt) * static_cast<float>(kMu0Over4Pi), u.dadt_nodes1.get(), s);
@@ -763,7 +773,7 @@ void uq_solve(cn_context& c, const double* sigma, int k, const cn_solve_options*
     std::vector<double> sg(static_cast<std::size_t>(K) * T, 0.0);
     std::copy(sigma, sigma + static_cast<std::size_t>(k) * T, sg.begin());
     u.sigma.ensure(sg.size(), "uq sigma");
-    h2d(u.sigma.get(), sg.data(), sg.size(), s);
+    c.stage_blk.put(u.sigma.get(), sg.data(), sg.size(), s);
     u.off_k.ensure(static_cast<std::size_t>(std::max<std::int64_t>(nnz_off, 1)) * K, "sample off-diagonals");
     u.rowsum_k.ensure(static_cast<std::size_t>(nr) * K, "sample row sums");
     c.d_B.ensure(nr * K, "rhs block");
@@ -798,7 +808,7 @@ void uq_solve(cn_context& c, const double* sigma, int k, const cn_solve_options*
                 coef[q * K + col] = v;
             }
         DeviceBuffer<double> dcoef(coef.size(), "guess coefficients");
-        h2d(dcoef.get(), coef.data(), coef.size(), s);
+        c.stage_blk.put(dcoef.get(), coef.data(), coef.size(), s);
         c.d_x0.ensure(nr * K, "initial guess");
         expand_guess(u.x_nom.get(), u.guess_rank > 0 ? u.S.get() : u.x_nom.get(), nr, u.guess_rank > 0 ? T : 0,
                      u.S_width, dcoef.get(), k, K, c.d_x0.get(), s);
@@ -949,8 +959,13 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
         }
         c.blk_sigma.ensure(sg.size(), "block sigma");
         c.blk_sigma_tag.ensure(tab.size(), "block sigma by tag");
-        h2d(c.blk_sigma.get(), sg.data(), sg.size(), s);
-        h2d(c.blk_sigma_tag.get(), tab.data(), tab.size(), s);
+        const std::size_t sg_bytes = sg.size() * sizeof(double);
+        char* host = static_cast<char*>(c.stage_blk.fill(sg_bytes + tab.size() * sizeof(float)));
+        std::memcpy(host, sg.data(), sg_bytes);
+        std::memcpy(host + sg_bytes, tab.data(), tab.size() * sizeof(float));
+        c.stage_blk.send(c.blk_sigma.get(), 0, sg_bytes, s);
+        c.stage_blk.send(c.blk_sigma_tag.get(), sg_bytes, tab.size() * sizeof(float), s);
+        c.stage_blk.done(s);
         const std::int64_t nnz_off = c.nnz_red - nr;
         c.blk_off.ensure(static_cast<std::size_t>(std::max<std::int64_t>(nnz_off, 1)) * K, "block off-diagonals");
         c.blk_rowsum.ensure(static_cast<std::size_t>(nr) * K, "block row sums");
@@ -994,11 +1009,13 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
     const double* dx0 = nullptr;
     if (x0 != nullptr) {
         // (k, n_red) placement-major host guess -> (n_red, K) row-major device block
-        std::vector<double> blk(static_cast<std::size_t>(nr * K), 0.0);
+        double* blk = static_cast<double*>(c.stage_blk.fill(sizeof(double) * nr * K));
+        std::fill(blk, blk + nr * K, 0.0);
         for (int ci = 0; ci < k; ++ci)
             for (std::int64_t r = 0; r < nr; ++r) blk[r * K + ci] = x0[ci * nr + r];
         c.d_x0.ensure(nr * K, "x0");
-        h2d(c.d_x0.get(), blk.data(), blk.size(), s);
+        c.stage_blk.send(c.d_x0.get(), 0, sizeof(double) * nr * K, s);
+        c.stage_blk.done(s);
         dx0 = c.d_x0.get();
     } else if (sopt && sopt->warm_start == 1) {
         c.d_x0.ensure(nr * K, "x0");

@@ -114,4 +114,56 @@ DeviceBuffer<T> upload(const T* src, std::size_t count, hipStream_t s, const cha
 
 inline void sync(hipStream_t s, const char* what) { check(hipStreamSynchronize(s), what); }
 
+// Pinned host staging for host-to-device copies issued without a wait after them. A copy from
+// pageable memory may be carried out by the runtime after hipMemcpyAsync has returned, and a
+// function-local vector is freed (a large one unmapped) when the function returns; staging the
+// bytes in pinned memory that the stage owns makes the source outlive the copy. One use: fill()
+// waits until every copy of the previous use has completed (an event recorded after the last
+// of them) and returns the pinned bytes; send() queues copies out of them; done() records the
+// event. Grow-only.
+class PinnedStage {
+  public:
+    PinnedStage() = default;
+    PinnedStage(const PinnedStage&) = delete;
+    PinnedStage& operator=(const PinnedStage&) = delete;
+    ~PinnedStage() {
+        if (ev_) {
+            (void)hipEventSynchronize(ev_);
+            (void)hipEventDestroy(ev_);
+        }
+        if (p_) (void)hipHostFree(p_);
+    }
+    void* fill(std::size_t bytes) {
+        if (ev_) check(hipEventSynchronize(ev_), "staging wait");
+        if (bytes > cap_) {
+            if (p_) check(hipHostFree(p_), "staging free");
+            p_ = nullptr;
+            cap_ = 0;
+            check(hipHostMalloc(&p_, bytes), "pinned staging");
+            cap_ = bytes;
+        }
+        return p_;
+    }
+    void send(void* dst, std::size_t offset, std::size_t bytes, hipStream_t s) {
+        if (bytes) check(hipMemcpyAsync(dst, static_cast<char*>(p_) + offset, bytes, hipMemcpyHostToDevice, s), "staged copy");
+    }
+    void done(hipStream_t s) {
+        if (!ev_) check(hipEventCreateWithFlags(&ev_, hipEventDisableTiming), "staging event");
+        check(hipEventRecord(ev_, s), "staging event");
+    }
+    // One array: stage `count` values of `src` and copy them to `dst` on `s`.
+    template <typename T>
+    void put(T* dst, const T* src, std::size_t count, hipStream_t s) {
+        if (!count) return;
+        std::memcpy(fill(count * sizeof(T)), src, count * sizeof(T));
+        send(dst, 0, count * sizeof(T), s);
+        done(s);
+    }
+
+  private:
+    void* p_ = nullptr;
+    std::size_t cap_ = 0;
+    hipEvent_t ev_ = nullptr;
+};
+
 }  // namespace cn

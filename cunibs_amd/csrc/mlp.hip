@@ -255,10 +255,13 @@ void Mlp::ensure_batch(int B) {
 
 // X (B, n_in) host-padded to (B, kInPad) on upload.
 void Mlp::upload_inputs(const float* X, int B) {
-    std::vector<float> pad(static_cast<std::size_t>(B) * kInPad, 0.0f);
+    const std::size_t count = static_cast<std::size_t>(B) * kInPad;
+    float* pad = static_cast<float*>(stage_.fill(count * sizeof(float)));
+    std::fill(pad, pad + count, 0.0f);
     for (int r = 0; r < B; ++r)
         for (int c = 0; c < n_in_; ++c) pad[static_cast<std::size_t>(r) * kInPad + c] = X[static_cast<std::size_t>(r) * n_in_ + c];
-    h2d(x_.get(), pad.data(), pad.size(), s_);
+    stage_.send(x_.get(), 0, count * sizeof(float), s_);
+    stage_.done(s_);
 }
 
 double Mlp::forward_backward(int B) {

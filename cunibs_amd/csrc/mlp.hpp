@@ -37,6 +37,7 @@ class Mlp {
     DeviceBuffer<unsigned short> w16_;
     DeviceBuffer<float> x_, z1_, h1_, z2_, h2_, y_, t_, dy_, dh_, dz_;
     DeviceBuffer<double> loss_;
+    PinnedStage stage_;  // padded inputs, copied without a wait
     DeviceBuffer<unsigned short> x16_, h16a_, h16b_;
 };
 

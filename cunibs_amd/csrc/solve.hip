@@ -1554,8 +1554,15 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
             tgt[c] = active ? opt.inner_margin * opt.tolerance * bnorm[c] : 0.0;
             host_state[2 + c] = active ? 0 : 1;
         }
-        h2d(target_.get(), tgt.data(), K, s_);
-        h2d(state_.get(), host_state.data(), host_state.size(), s_);
+        {
+            const std::size_t tb = sizeof(double) * K, sb = sizeof(int) * host_state.size();
+            char* h = static_cast<char*>(stage_.fill(tb + sb));
+            std::memcpy(h, tgt.data(), tb);
+            std::memcpy(h + tb, host_state.data(), sb);
+            stage_.send(target_.get(), 0, tb, s_);
+            stage_.send(state_.get(), tb, sb, s_);
+            stage_.done(s_);
+        }
         cast_f64_f32_kernel<<<grid_for(nk), kBlock, 0, s_>>>(nk, R64_.get(), R_.get());
         vcycle_k<K>(R_.get(), Z_.get(), nullptr);
         d2d(W_.get(), Z_.get(), static_cast<std::size_t>(nk), s_);

@@ -168,6 +168,7 @@ class BlockSolver {
     DeviceBuffer<int> ran_;  // per window slot: the iteration ran (its coefficients are to be applied)
     DeviceBuffer<int> state_;  // [0] done, [1] iteration, [2..2+K) frozen, [2+K..2+2K) col_iters
     int* host_state_ = nullptr;  // pinned
+    PinnedStage stage_;          // the round's targets and initial state, copied without a wait
     // One captured iteration per block width, kept for the solver's lifetime: every buffer the
     // iteration names is solver-owned and reallocated only when the capacity grows (which drops
     // them all).

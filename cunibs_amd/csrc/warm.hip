@@ -464,7 +464,7 @@ void WarmBasis::guess_from_coefficients(const double* c_host, int k, int K, doub
             hl[static_cast<std::size_t>(R + j) * 16 + c] = host_bf16(v - host_bf16_value(hi));
         }
     coef.ensure(hl.size(), "warm coefficients");
-    h2d(coef.get(), hl.data(), hl.size(), s);
+    stage.put(coef.get(), hl.data(), hl.size(), s);
     expand(k, K, x0, s);
 }
 

@@ -21,6 +21,7 @@ struct WarmBasis {
     DeviceBuffer<unsigned short> modes;  // (2, n, n_modes) BF16 hi then lo halves, row-major
     DeviceBuffer<double> P, partials;    // P: (n_modes, n_modes), identity-padded
     DeviceBuffer<unsigned short> coef;
+    PinnedStage stage;  // supplied coefficients, copied without a wait
 
     void reset(std::int64_t n_rows, int capacity);
     // Append columns 0..k-1 of the (n, K) row-major FP64 block X; returns how many fitted.
