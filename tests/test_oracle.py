@@ -251,6 +251,57 @@ def test_chebyshev_coefficients_give_the_chebyshev_error_polynomial(oracle_mod):
         np.testing.assert_allclose(err, want, atol=2e-6)
 
 
+def _chebyshev_error_matrix(a, dinv, degree, ratio):
+    """T_d(D^-1 A) for the scaled Chebyshev polynomial T_d((theta - x)/delta) / T_d(theta/delta) on
+    [1/ratio, 1], from the eigenvectors of the symmetric D^-1/2 A D^-1/2 (independent of the
+    smoother's three-term recurrence)."""
+    lo, hi = 1.0 / ratio, 1.0
+    theta, delta = 0.5 * (hi + lo), 0.5 * (hi - lo)
+    d = np.sqrt(dinv)
+    lam, vec = np.linalg.eigh(d[:, None] * a * d[None, :])
+    t = np.polynomial.chebyshev.Chebyshev.basis(degree)
+    f = t((theta - lam) / delta) / t(theta / delta)
+    return (vec * f) @ vec.T * d[:, None] / d[None, :]
+
+
+def _vcycle_closed_form(h, level, degree, ratio, coarse=True):
+    """Dense V-cycle operator from its definition: error propagation E (I - P M_c P^T A) E with E
+    the smoother's error polynomial, so M = (I - E (I - P M_c P^T A) E) A^-1, recursing down to the
+    exact coarsest inverse."""
+    if level == len(h.levels):
+        return np.linalg.inv(h.coarse.dense().astype(np.float64))
+    lv = h.levels[level]
+    a = lv.a.dense().astype(np.float64)
+    e = _chebyshev_error_matrix(a, lv.dinv.astype(np.float64), degree, ratio)
+    p = lv.p.dense().astype(np.float64)
+    corr = p @ _vcycle_closed_form(h, level + 1, degree, ratio) @ p.T @ a if coarse else 0.0
+    eye = np.eye(a.shape[0])
+    return (eye - e @ (eye - corr) @ e) @ np.linalg.inv(a)
+
+
+def test_vcycle_equals_its_dense_closed_form(oracle_mod, monkeypatch):
+    """solver/amg/vcycle.cu:560-648 composed from dense matrices on a two-level hierarchy (a
+    1,728-node jittered two-tissue box, coarse rows floor lowered to 16 so that two smoothed levels
+    form): the oracle's V-cycle applied to every unit vector equals the closed form to the FP32
+    rounding of the Chebyshev coefficients. One smoothing step more or the coarse correction left
+    out moves the operator by far more than the tolerance."""
+    nodes, tets = kuhn_grid(11, 11, 11, spacing=5.0, jitter=0.1, seed=3)
+    tags = np.ones(len(tets), np.int32)
+    tags[: len(tets) // 3] = 2
+    from conftest import boundary_tris
+    ctx = oracle_mod.build_context(nodes, tets, tags, boundary_tris(tets, nodes), hierarchy=False)
+    monkeypatch.setattr(oracle_mod, "MIN_COARSE_ROWS", 16)
+    h = oracle_mod.build_hierarchy(ctx.reduced)
+    assert len(h.levels) == 2 and h.coarse.n >= 16
+    n = h.levels[0].a.n
+    got = np.stack([oracle_mod.vcycle_apply(h, e) for e in np.eye(n)], axis=1)
+    want = _vcycle_closed_form(h, 0, 2, 12.0)
+    scale = np.abs(want).max()
+    assert np.abs(got - want).max() <= 2e-6 * scale
+    assert np.abs(got - _vcycle_closed_form(h, 0, 3, 12.0)).max() > 1e-2 * scale
+    assert np.abs(got - _vcycle_closed_form(h, 0, 2, 12.0, coarse=False)).max() > 1e-2 * scale
+
+
 def test_vcycle_is_symmetric_and_positive(head8_ctx, oracle_mod):
     """A symmetric V-cycle (same polynomial smoother before and after, R = P^T) is an SPD
     preconditioner: u^T M v = v^T M u and v^T M v > 0."""

@@ -81,6 +81,81 @@ def test_oracle_harmonic_peak_matches_the_reference_pin(oracle_mod, oracle_patch
     assert s["peak_magnE"] == pytest.approx(PATCH_PEAK_MAGNE_HARMONIC, rel=2e-6)
 
 
+def _smoothed_normals(nodes_mm, tris, rounds=2):
+    """The skin smoothing of solver/fem/skin.cu as the reference's tests describe it
+    (tests/test_solver_kernels.py:732-754), restated with a node-by-face incidence matrix C:
+    face vectors f start as the unnormalised cross products (twice the area along the normal);
+    each round spreads them to the nodes (u = C f) and gathers back normalised sums
+    (f = C^T u / |C^T u|); the last node vectors are normalised, and each face takes the
+    normalised sum of its three."""
+    import scipy.sparse as sp
+    tris = np.asarray(tris)
+    used, local = np.unique(tris.ravel(), return_inverse=True)
+    n_tri = tris.shape[0]
+    inc = sp.csr_matrix((np.ones(3 * n_tri), (local, np.repeat(np.arange(n_tri), 3))), shape=(used.size, n_tri))
+    p = np.asarray(nodes_mm, np.float64)[tris]
+    f = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    unit = lambda v: v / np.linalg.norm(v, axis=1, keepdims=True)
+    for _ in range(rounds):
+        u = inc @ f
+        f = unit(inc.T @ u)
+    return unit(inc.T @ unit(u))
+
+
+def test_oracle_skin_normals_match_the_smoothing_definition(oracle_mod, patch_mesh):
+    """On the patch's scalp (tests/test_solver_kernels.py:757-770): the oracle's normals equal the
+    restated definition to a few ulp; one smoothing round instead of two moves them by more than
+    1e-3, so the tolerance separates a wrong round count, corner or weighting."""
+    got = oracle_mod.skin_normals(patch_mesh.nodes_mm, patch_mesh.skin_tris)
+    want = _smoothed_normals(patch_mesh.nodes_mm, patch_mesh.skin_tris)
+    np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-15)
+    assert np.abs(_smoothed_normals(patch_mesh.nodes_mm, patch_mesh.skin_tris, rounds=1) - want).max() > 1e-3
+
+
+@pytest.fixture(scope="module")
+def patch_sites8(patch_mesh):
+    """tests/test_block_solve.py:46-54: eight scalp placements spread over the patch's top, each
+    handle 40 mm out along its own azimuth, 4 mm off the scalp."""
+    cen = patch_mesh.nodes_mm[patch_mesh.skin_tris].mean(axis=1)
+    high = cen[np.argsort(cen[:, 2])[-400:]]
+    chosen = high[::max(len(high) // 8, 1)][:8]
+    ang = np.linspace(0.0, 2.0 * np.pi, 8, endpoint=False)
+    return [(c, c + 40.0 * np.array([np.cos(a), np.sin(a), 0.0]), 4.0) for c, a in zip(chosen, ang)]
+
+
+def _two_rhs(n):
+    """tests/test_block_solve.py:289-293: a two-point source (+1 at n // 3, -1 at 2n // 3) and one
+    period of a sine over the reduced unknowns in the solver's order."""
+    pts = np.zeros(n)
+    pts[n // 3], pts[2 * n // 3] = 1.0, -1.0
+    return pts, np.sin(np.arange(n) * (2.0 * np.pi / n))
+
+
+def test_oracle_pcg_takes_the_reference_iteration_counts(oracle_mod, oracle_patch, d70, patch_sites8):
+    """The reference's counts at tolerance 1e-6, Chebyshev degree 2, ratio 12, two SIZE_4 rounds:
+    every patch placement converges in 11 iterations (tests/test_block_solve.py:284), the sine in
+    12 and the two-point source in 10 (:286-288). These pin the whole V-cycle composition (smoother
+    polynomial, restriction, coarse correction, prolongation). Mistakes planted in the oracle's
+    V-cycle, on the CPU, move them: one smoothing step fewer or more (degree 1 or 3), or the coarse
+    correction left out."""
+    import copy
+    ctx, _ = oracle_patch
+    for site in patch_sites8:
+        r = oracle_mod.solve_site(ctx, d70.positions_m, d70.moments, site, 1e6, tol=1e-6)
+        assert r["iterations"] == 11 and r["relative_residual"] <= 1e-6
+    pts, sine = _two_rhs(ctx.reduced.n)
+
+    def counts(h):
+        return tuple(oracle_mod.pcg(h, ctx.reduced, ctx.reduced_val64, b, 1e-6)[1] for b in (pts, sine))
+
+    assert counts(ctx.hier) == (10, 12)
+    assert counts(oracle_mod.build_hierarchy(ctx.reduced, degree=1)) == (16, 18)
+    assert counts(oracle_mod.build_hierarchy(ctx.reduced, degree=3)) == (9, 10)
+    no_coarse = copy.copy(ctx.hier)
+    no_coarse.coarse_l = ctx.hier.coarse_l * 1e8  # coarse solve returns ~0: correction dropped
+    assert min(counts(no_coarse)) > 60
+
+
 @pytest.fixture(scope="module")
 def patch_subject(patch_mesh):
     from cunibs_amd.simulation import Subject
