@@ -1714,6 +1714,20 @@ int32_t cn_bench_guess(cn_context* c, int32_t K, int32_t reps, double* ms3) {
     });
 }
 
+int32_t cn_debug_fail_captures(cn_context* c, int32_t n) {
+    return guarded([&] {
+        cn::require(c != nullptr && n >= 0, "bad argument");
+        c->solver->fail_next_captures(n);
+    });
+}
+
+int32_t cn_graph_execs(const cn_context* c, int32_t* out) {
+    return guarded([&] {
+        cn::require(c != nullptr && out != nullptr, "bad argument");
+        *out = c->solver->graph_execs();
+    });
+}
+
 int32_t cn_timing_enable(cn_context* c, int32_t on) {
     return guarded([&] {
         cn::require(c != nullptr, "null context");

@@ -1303,21 +1303,99 @@ BlockSolver::~BlockSolver() {
 
 void BlockSolver::set_multi(const MultiOperator* m) {
     if (m) require(m->n_tissue >= 1 && m->n_tissue <= kMaxTissues, "tissue count must be in [1, 16]");
-    multi_ = m;
-    drop_graphs();  // captured iterations name the operator's kernels and buffers
+    multi_ = m;  // the solve picks (or captures) the graph set of this mode
 }
 
-// Captured iterations that are replaced are retired, not destroyed, until the solver goes away: a
-// graph exec destroyed and another instantiated in the same process crashes rocprofv3's kernel
-// tracer on this ROCm, and a retired exec costs only its host-side node list.
-void BlockSolver::drop_graphs() {
-    for (int i = 0; i < kNumBlockWidths; ++i) {
-        if (graphs_[i]) retired_.push_back(graphs_[i]);
-        graphs_[i] = nullptr;
-        for (auto& g : graphs1_[i]) {
-            if (g) retired_.push_back(g);
-            g = nullptr;
+static bool same_layout(const MultiOperator& a, const MultiOperator& b) {
+    return a.width == b.width && a.off_vals == b.off_vals && a.rowsum == b.rowsum &&
+           a.tissue_vals == b.tissue_vals && a.n_tissue == b.n_tissue && a.sigma == b.sigma;
+}
+
+BlockSolver::GraphSet& BlockSolver::graph_set() {
+    ++use_clock_;
+    GraphSet* pick = nullptr;
+    for (auto& g : sets_)
+        if (g.in_use && g.multi == (multi_ != nullptr) && (!multi_ || same_layout(g.key, *multi_))) pick = &g;
+    if (pick == nullptr) {
+        pick = &sets_[0];
+        for (auto& g : sets_) {
+            if (!g.in_use) {
+                pick = &g;
+                break;
+            }
+            if (g.last_use < pick->last_use) pick = &g;
         }
+        for (int i = 0; i < kNumBlockWidths; ++i) {
+            retire(pick->win[i]);
+            for (auto& x : pick->one[i]) retire(x);
+        }
+        *pick = GraphSet{};
+        pick->in_use = true;
+        pick->multi = multi_ != nullptr;
+        if (multi_) pick->key = *multi_;
+    }
+    pick->last_use = use_clock_;
+    return *pick;
+}
+
+void BlockSolver::retire(hipGraphExec_t& g) {
+    if (g == nullptr) return;
+    retired_.push_back(g);
+    g = nullptr;
+    if (retired_.size() > kMaxRetired) {
+        (void)hipGraphExecDestroy(retired_.front());
+        retired_.erase(retired_.begin());
+    }
+}
+
+int BlockSolver::graph_execs() const {
+    int n = static_cast<int>(retired_.size());
+    for (const auto& g : sets_)
+        for (int i = 0; i < kNumBlockWidths; ++i) {
+            n += g.win[i] != nullptr;
+            for (auto x : g.one[i]) n += x != nullptr;
+        }
+    return n;
+}
+
+template <typename F>
+bool BlockSolver::capture(hipGraphExec_t& gx, F&& body) {
+    gx = nullptr;
+    if (fail_captures_ > 0) {
+        --fail_captures_;
+        return false;
+    }
+    if (hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    body();
+    hipGraph_t g = nullptr;
+    const hipError_t ended = hipStreamEndCapture(s_, &g);
+    if (ended != hipSuccess || g == nullptr) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return false;
+    }
+    const hipError_t made = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (made != hipSuccess) {
+        gx = nullptr;
+        (void)hipGetLastError();
+        return false;
+    }
+    return true;
+}
+
+// Replaced executables are retired rather than destroyed at once (a short queue, the oldest
+// destroyed beyond its bound).
+void BlockSolver::drop_graphs() {
+    for (auto& g : sets_) {
+        for (int i = 0; i < kNumBlockWidths; ++i) {
+            retire(g.win[i]);
+            for (auto& x : g.one[i]) retire(x);
+        }
+        g = GraphSet{};
     }
 }
 
@@ -1536,6 +1614,7 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
     for (int c = 0; c < k; ++c) require(bnorm[c] > 0.0, "zero right-hand-side column " + std::to_string(c));
 
     std::vector<int> host_state(2 + 2 * kMaxBlock, 0);
+    bool use_graph = opt.use_graph;  // cleared for the rest of this solve if a capture fails
     int total_iters = 0;
     std::vector<int> col_iters(K, 0);
     for (int round = 0; round < opt.max_refine; ++round) {
@@ -1569,24 +1648,32 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
         int slot = 0;
         while (kBlockWidths[slot] != K) ++slot;
         const int diag = (multi_ || opt.independent) ? 1 : 0;
-        if (opt.use_graph && (graphs_[slot] == nullptr || graph_drop_[slot] != opt.drop_tol || graph_diag_[slot] != diag)) {
-            // a whole window (j0 = 0, L = kLag), then one iteration at each ring position
-            for (int v = -1; v < kLag; ++v) {
-                hipGraphExec_t& gx = v < 0 ? graphs_[slot] : graphs1_[slot][v];
-                if (gx) retired_.push_back(gx);
-                gx = nullptr;
-                hipGraph_t g = nullptr;
-                check(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal), "begin capture");
-                if (v < 0)
-                    window_k<K>(opt, 0, kLag);
-                else
-                    window_k<K>(opt, v, 1);
-                check(hipStreamEndCapture(s_, &g), "end capture");
-                check(hipGraphInstantiate(&gx, g, nullptr, nullptr, 0), "graph instantiate");
-                (void)hipGraphDestroy(g);
+        hipGraphExec_t win = nullptr, one[kLag] = {};
+        if (use_graph) {
+            GraphSet& gs = graph_set();
+            if (gs.win[slot] == nullptr || gs.drop[slot] != opt.drop_tol || gs.diag[slot] != diag) {
+                // a whole window (j0 = 0, L = kLag), then one iteration at each ring position
+                bool ok = true;
+                for (int v = -1; v < kLag && ok; ++v) {
+                    hipGraphExec_t& gx = v < 0 ? gs.win[slot] : gs.one[slot][v];
+                    retire(gx);
+                    ok = capture(gx, [&] { v < 0 ? window_k<K>(opt, 0, kLag) : window_k<K>(opt, v, 1); });
+                }
+                if (ok) {
+                    gs.drop[slot] = opt.drop_tol;
+                    gs.diag[slot] = diag;
+                } else {
+                    // A failed capture records and runs nothing: this solve launches its iterations
+                    // directly; the next solve tries to capture again.
+                    retire(gs.win[slot]);
+                    for (auto& x : gs.one[slot]) retire(x);
+                    use_graph = false;
+                }
             }
-            graph_drop_[slot] = opt.drop_tol;
-            graph_diag_[slot] = diag;
+            if (use_graph) {
+                win = gs.win[slot];
+                for (int v = 0; v < kLag; ++v) one[v] = gs.one[slot][v];
+            }
         }
         // Bounded by the launches the host issues, not only by the device counter.
         int launched = total_iters;
@@ -1604,8 +1691,8 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
             // iteration at the current position
             for (int b = 0; b < batch;) {
                 const int L = (pos == 0 && batch - b >= kLag) ? kLag : 1;
-                if (opt.use_graph)
-                    check(hipGraphLaunch(L == kLag ? graphs_[slot] : graphs1_[slot][pos], s_), "graph launch");
+                if (use_graph)
+                    check(hipGraphLaunch(L == kLag ? win : one[pos], s_), "graph launch");
                 else
                     window_k<K>(opt, pos, L);
                 b += L;

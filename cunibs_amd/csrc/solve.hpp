@@ -115,6 +115,11 @@ class BlockSolver {
     // kernel at the end of each iteration (captured with it) adds the spans of every iteration
     // that ran in full. Switching drops the captured iterations; every call resets the sums.
     void set_timing(bool on);
+    // Failure handling test hook: the next n graph captures report failure, so those solves run
+    // their iterations as direct launches.
+    void fail_next_captures(int n) { fail_captures_ = n; }
+    // Captured executables alive (current and retired), for the tests of the graph cache.
+    int graph_execs() const;
     bool timing_on() const { return timing_; }
     // out[0] iterations sampled, out[1..3] mean ms of the SpMM, the V-cycle and the iteration.
     void timing(double* out) const;
@@ -180,15 +185,36 @@ class BlockSolver {
     std::size_t stamp_marks() const;
     unsigned long long* stamps();
     void mark(int i, const int* state);
-    hipGraphExec_t graphs_[kNumBlockWidths] = {};          // a whole window of kLag iterations
-    hipGraphExec_t graphs1_[kNumBlockWidths][kLag] = {};  // one iteration at each ring position
-    double graph_drop_[kNumBlockWidths] = {};  // the drop tolerance each graph was captured with
-    int graph_diag_[kNumBlockWidths] = {};     // and whether its Grams were diagonal
+    // Captured iterations, one set per operator mode: the single operator, or one multi-matrix
+    // layout (its value buffers and width), so that alternating between them captures nothing
+    // again. Per width: a whole window of kLag iterations and one iteration at each ring position.
+    struct GraphSet {
+        bool in_use = false;
+        bool multi = false;
+        MultiOperator key{};
+        unsigned long long last_use = 0;
+        hipGraphExec_t win[kNumBlockWidths] = {};
+        hipGraphExec_t one[kNumBlockWidths][kLag] = {};
+        double drop[kNumBlockWidths] = {};  // the drop tolerance each width was captured with
+        int diag[kNumBlockWidths] = {};     // and whether its Grams were diagonal
+    };
+    static constexpr int kGraphSets = 4;
+    GraphSet sets_[kGraphSets];
+    unsigned long long use_clock_ = 0;
+    GraphSet& graph_set();  // the current mode's set; a new mode takes the least recently used one
+    void retire(hipGraphExec_t& g);
+    // Capture `body` into gx; false (nothing recorded, nothing run) when the capture or the
+    // instantiation fails, or when the failure hook asks for it.
+    template <typename F>
+    bool capture(hipGraphExec_t& gx, F&& body);
+    int fail_captures_ = 0;
     // Inner iterations the last solve at each width took in its first round: the next solve runs
     // the whole windows short of that count before its first convergence read, then reads after
     // every iteration for a few, so a sweep of similar blocks neither waits on early reads nor
     // replays finished iterations (each a graph of early-exiting launches).
     int last_first_round_[kNumBlockWidths] = {};
+    // Replaced executables wait here and the oldest are destroyed beyond kMaxRetired.
+    static constexpr std::size_t kMaxRetired = 64;
     std::vector<hipGraphExec_t> retired_;
     void drop_graphs();
 };

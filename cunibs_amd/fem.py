@@ -543,6 +543,17 @@ class SolverContext:
         check(lib().cn_bench_copy(self.handle, int(nbytes), reps, ctypes.byref(ms)), "cn_bench_copy")
         return float(ms.value)
 
+    def debug_fail_captures(self, n: int) -> None:
+        """Test hook: the next `n` captures of the block iteration report failure, so the solves
+        that need them launch their iterations directly."""
+        check(lib().cn_debug_fail_captures(self.handle, int(n)), "cn_debug_fail_captures")
+
+    def graph_execs(self) -> int:
+        """Captured block-iteration executables alive in the solver (in use and retired)."""
+        out = ctypes.c_int32()
+        check(lib().cn_graph_execs(self.handle, ctypes.byref(out)), "cn_graph_execs")
+        return int(out.value)
+
 
 def solve_sweep(
     ctx: SolverContext,

@@ -372,6 +372,14 @@ int32_t cn_bench_dadt(cn_context* ctx, const double* dip_pos_m, const double* di
  * on the library's stream: the calibration launch for the HBM traffic counters and the practical
  * bandwidth ceiling. No counterpart in the reference (a measurement aid). */
 int32_t cn_bench_copy(cn_context* ctx, int64_t bytes, int32_t reps, double* ms);
+/* Graph-capture failure handling (the reference's GraphCache falls back to direct execution when a
+ * capture fails, solver/amg/solver.hpp:88-122): the next `n` captures of the block iteration
+ * report failure, so the solves that needed them launch their iterations directly (and the next
+ * solve tries to capture again). A test hook; the results are those of the captured path. */
+int32_t cn_debug_fail_captures(cn_context* ctx, int32_t n);
+/* Captured block-iteration executables the context holds (in use and retired): stays bounded
+ * however many blocks of whichever operator mode are solved. A test hook. */
+int32_t cn_graph_execs(const cn_context* ctx, int32_t* out);
 
 #ifdef __cplusplus
 }

@@ -357,3 +357,44 @@ def test_pipelined_sweep_equals_blockwise_solves(gctx, coil, warm):
     np.testing.assert_array_equal(sweep["relative_residual"], [r.relative_residual for r in ref])
     assert len(sweep["block_iterations"]) == 3
     assert np.all(sweep["relative_residual"] <= 1e-6)
+
+
+def test_graph_replays_equal_direct_launches(block8, gctx, coil):
+    """The captured iteration and the same kernels launched directly give the same bits."""
+    from cunibs_amd import fem
+    sites, res = block8
+    direct = fem.solve_placements_block(gctx, coil.positions_m, coil.moments, sites, 1e6, use_graph=False)
+    for a, b in zip(res, direct):
+        np.testing.assert_array_equal(a.magnE, b.magnE)
+        assert a.iterations == b.iterations
+
+
+def test_failed_capture_falls_back_to_direct_launches(block8, gctx, coil):
+    """solver/amg/solver.hpp:88-122: a capture that fails (forced from the host) leaves the solve
+    to launch its iterations directly, with the captured path's results; the next solve captures
+    again."""
+    from cunibs_amd import fem
+    sites, res = block8
+    gctx.set_tuning(fine_sub=gctx.tuning["fine_sub"])  # drop the captured iterations
+    gctx.debug_fail_captures(1)
+    fell_back = fem.solve_placements_block(gctx, coil.positions_m, coil.moments, sites, 1e6)
+    before = gctx.graph_execs()
+    again = fem.solve_placements_block(gctx, coil.positions_m, coil.moments, sites, 1e6)
+    assert gctx.graph_execs() > before  # captured this time
+    for a, b, c in zip(res, fell_back, again):
+        np.testing.assert_array_equal(a.magnE, b.magnE)
+        np.testing.assert_array_equal(a.magnE, c.magnE)
+
+
+def test_graph_cache_is_bounded_across_operator_modes(gctx, coil):
+    """Blocks with per-placement conductivities (multi-matrix mode) alternating with nominal
+    blocks reuse one captured set per mode: after the first of each, no block captures again."""
+    from cunibs_amd import fem, synthetic
+    sites = synthetic.cap_sites(8, seed=23)
+    rows = fem.conductivity_rows([{2: 0.3 + 0.02 * i} for i in range(4)])
+    for rep in range(3):
+        fem.solve_placements_block(gctx, coil.positions_m, coil.moments, sites[:4], 1e6, sigma=rows)
+        fem.solve_placements_block(gctx, coil.positions_m, coil.moments, sites[4:], 1e6)
+        if rep == 0:
+            count = gctx.graph_execs()
+    assert gctx.graph_execs() == count
