@@ -212,3 +212,21 @@ def test_gpu_field_result_round_trip(tmp_path, patch_subject, d70, patch_site):
     assert loaded.summary["peak_magnE"] == r.summary["peak_magnE"]
     assert loaded.recovery == "harmonic" and loaded.coil_name == "D70"
     assert path.stat().st_size < 3_000_000
+
+
+@pytest.mark.gpu
+def test_gpu_iteration_counts_match_the_reference(patch_mesh, d70, patch_sites8):
+    """tests/test_block_solve.py:284: at the reference's smoother (degree 2, ratio 12) every patch
+    placement converges in 11 iterations on the HIP path too, alone and as the reference's
+    independent chains in a block of eight."""
+    from cunibs_amd import fem
+    ctx = fem.SolverContext(patch_mesh.nodes_mm, patch_mesh.tet_nodes, patch_mesh.tet_tags, patch_mesh.skin_tris)
+    try:
+        ctx.set_tuning(degree=2, lower_ratio=12.0)
+        alone = [fem.solve_placements_block(ctx, d70.positions_m, d70.moments, [s], 1e6)[0] for s in patch_sites8]
+        chains = fem.solve_placements_block(ctx, d70.positions_m, d70.moments, patch_sites8, 1e6, independent=True)
+        assert [r.iterations for r in alone] == [11] * 8
+        assert [r.iterations for r in chains] == [11] * 8
+        assert all(r.relative_residual <= 1e-6 for r in alone + chains)
+    finally:
+        ctx.free()
