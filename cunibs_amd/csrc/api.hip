@@ -1714,6 +1714,11 @@ int32_t cn_bench_guess(cn_context* c, int32_t K, int32_t reps, double* ms3) {
     });
 }
 
+int32_t cn_k_gram_noalloc(int64_t n, int32_t K, const float* A, const float* B1, const float* B2, double* out,
+                          void* stream) {
+    return guarded([&] { cn::gram_noalloc(n, K, A, B1, B2, out, static_cast<hipStream_t>(stream)); });
+}
+
 int32_t cn_debug_fail_captures(cn_context* c, int32_t n) {
     return guarded([&] {
         cn::require(c != nullptr && n >= 0, "bad argument");

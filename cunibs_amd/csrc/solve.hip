@@ -25,6 +25,7 @@
 // the captured iteration graph several times between convergence reads.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "solve.hpp"
 
@@ -1878,6 +1879,59 @@ double BlockSolver::bench_vcycle(int K, int reps) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return static_cast<double>(ms) / reps;
+}
+
+// ----------------------------------------------------------------------------------------------
+// Gram entry point that allocates nothing
+// ----------------------------------------------------------------------------------------------
+
+// Partials of gram_noalloc: kGramBlocks chunks x 2 K^2 entries at the largest width (8 MiB). Each
+// device holds its own copy for the life of the process; every call on a device shares it and
+// overwrites all the partials it reads (nothing has to start at zero), so calls that overlap on
+// one device race.
+constexpr std::size_t kNoAllocScratch = static_cast<std::size_t>(kGramBlocks) * 2 * kMaxBlock * kMaxBlock;
+static __device__ __align__(256) double scratch[kNoAllocScratch];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_lock;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (the symbol's host-side name is not a
+// device address), looked up once per device.
+double* scratch_on_current_device() {
+    int dev = 0;
+    check(hipGetDevice(&dev), "current device");
+    require(dev >= 0 && dev < kMaxDevices, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> hold(g_scratch_lock);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "scratch symbol address");
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+void gram_noalloc(std::int64_t n, int K, const float* A, const float* B1, const float* B2, double* out,
+                  hipStream_t s) {
+    require(n >= 1, "n must be at least 1");
+    require(K >= 1 && K <= kMaxBlock && padded_width(K) == K, "K must be one of 1, 2, 4, 8, 16, 32");
+    require(A != nullptr && B1 != nullptr && out != nullptr, "null operand");
+    const int E = (B2 != nullptr ? 2 : 1) * K * K;
+    require(static_cast<std::size_t>(kGramBlocks) * E <= kNoAllocScratch, "Gram partials exceed the scratch");
+    double* const part = scratch_on_current_device();
+    const std::int64_t rpb = gram_rows(n);
+    dispatch_k(K, [&](auto kc) {
+        constexpr int KK = decltype(kc)::value;
+        if (B2 != nullptr)
+            launch_gram<KK, true>(n, A, B1, B2, part, rpb, nullptr, s);
+        else
+            launch_gram<KK, false>(n, A, B1, nullptr, part, rpb, nullptr, s);
+    });
+    check(hipGetLastError(), "gram launch");
+    reduce_cols(part, kGramBlocks, E, out, nullptr, s);
+    check(hipGetLastError(), "gram reduction launch");
 }
 
 }  // namespace cn

@@ -23,6 +23,12 @@ constexpr int kLag = 4;
 
 int padded_width(int k);  // smallest compiled width >= k
 
+// G1 = A^T B1 and, with B2, G2 = A^T B2 for (n, K) FP32 device blocks, into out (device, K*K then
+// K*K more with B2), the iteration's Gram kernels and fixed-order reduction, launched on s without
+// a wait. Allocates nothing: the partials live in a file-scope device array (one per device).
+void gram_noalloc(std::int64_t n, int K, const float* A, const float* B1, const float* B2, double* out,
+                  hipStream_t s);
+
 struct SolveOptions {
     double tolerance = 1e-6;    // ||b - A x|| / ||b|| per column (FP64 true residual)
     int max_iters = 2000;       // inner iterations over all refinement rounds

@@ -372,6 +372,17 @@ int32_t cn_bench_dadt(cn_context* ctx, const double* dip_pos_m, const double* di
  * on the library's stream: the calibration launch for the HBM traffic counters and the practical
  * bandwidth ceiling. No counterpart in the reference (a measurement aid). */
 int32_t cn_bench_copy(cn_context* ctx, int64_t bytes, int32_t reps, double* ms);
+/* G1 = A^T B1 and, with B2 non-null, G2 = A^T B2 for (n, K) row-major FP32 blocks A, B1, B2 in
+ * device memory, written to `out` in device memory (K*K FP64 values, then K*K more with B2): the
+ * kernels and the fixed-order reduction of cn_k_gram, launched on `stream` (a hipStream_t; null
+ * for the default stream) without waiting for them. No counterpart in the reference: it is for
+ * callers that cannot allocate (code under stream capture, tight serving loops). It calls no
+ * hipMalloc, hipFree or allocating library routine: its partial sums live in a file-scope device
+ * array of 512 x 2 x 32 x 32 doubles (8 MiB) that each GPU it runs on holds until the process
+ * ends. K must be a compiled width (1, 2, 4, 8, 16, 32); any n >= 1. Every call on one GPU shares
+ * that array, so calls that overlap on one GPU race. */
+int32_t cn_k_gram_noalloc(int64_t n, int32_t K, const float* A, const float* B1, const float* B2, double* out,
+                          void* stream);
 /* Graph-capture failure handling (the reference's GraphCache falls back to direct execution when a
  * capture fails, solver/amg/solver.hpp:88-122): the next `n` captures of the block iteration
  * report failure, so the solves that needed them launch their iterations directly (and the next
