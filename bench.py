@@ -70,38 +70,41 @@ COPY_BYTES = 512 << 20  # calibration copy: 512 MiB read + 512 MiB written per l
 
 
 def kernels_only(ctx, coil, sites, k: int) -> int:
-    """The launches the HBM traffic counters are read on: 20 streaming copies of COPY_BYTES (the
-    counter calibration), 20 differenced SpMMs at width k (each after one untimed launch), then one
-    cold block of k placements with each recovery (raw, spr_global, harmonic), so that every kernel
-    of the path appears once with the same sizes as in the sweep; then the sweep's warm-start basis
-    (256 solves) and two warm-started raw blocks, the sweep's steady state."""
-    from cunibs_amd import fem, synthetic
+    """Launches for kernel traces and counter passes, with no block solve (the block PCG is not
+    traced): the context build above (assembly, pattern, AMG setup), 20 streaming copies of
+    COPY_BYTES (the counter calibration), 20 differenced SpMMs at width k on the sliced ELL copy and
+    20 on the CSR arrays (each set after one untimed launch), 20 SpMMs with their Gram pass, 10
+    V-cycles, the recovery operator builds, the dA/dt N-body for 8 placements (3 times) and the
+    block stages around the solve (coil stage, field with harmonic and with raw recovery, region
+    metrics; 5 times each)."""
+    from cunibs_amd import fem
 
     K = k if k in fem.BLOCK_SIZES else min(b for b in fem.BLOCK_SIZES if b >= k)
     n_red, nnz = int(ctx.info.n_reduced), int(ctx.info.nnz_reduced)
+    pos, mom = coil.positions_m, coil.moments
     copy_ms = ctx.bench_copy(COPY_BYTES, 20)
     spmm_ms = ctx.bench_spmm(K, 20, mode=1)
-    iters = {}
-    for rec in ("raw", "spr_global", "harmonic"):
-        res = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites[:k], 1e6, magnitude=False,
-                                         metrics=True, recovery=rec)
-        iters[rec] = [r.iterations for r in res]
-    train = synthetic.cap_sites(256, seed=7)
-    ctx.warm_reset(len(train))
-    for i in range(0, len(train), 16):
-        fem.solve_placements_block(ctx, coil.positions_m, coil.moments, train[i:i + 16], 1e6, metrics=False,
-                                   magnitude=False, collect=True)
-    ctx.warm_finalize()
-    for i in range(2):
-        res = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites[k * (i + 3):k * (i + 4)], 1e6,
-                                         magnitude=False, metrics=True, warm_start=True)
-        iters[f"warm{i}"] = [r.iterations for r in res]
+    ctx.set_tuning(layout=1)
+    spmm_csr_ms = ctx.bench_spmm(K, 20, mode=1)
+    ctx.set_tuning(layout=0)
+    gram_ms = ctx.bench_spmm(K, 20, mode=2)
+    vc_ms = ctx.bench_vcycle(K, 10)
+    t0 = time.perf_counter()
+    ctx.recovery_build("harmonic")
+    ctx.recovery_build("spr_global")
+    rec_s = time.perf_counter() - t0
+    dadt_ms = ctx.bench_dadt(pos, mom, sites[0], k=8, reps=3)
+    stages_h = ctx.bench_stages(pos, mom, sites[0], k=K, recovery="harmonic", reps=5)
+    stages_raw = ctx.bench_stages(pos, mom, sites[0], k=K, recovery="raw", reps=5)
     print(json.dumps({
         "copy_bytes_read": COPY_BYTES, "copy_bytes_written": COPY_BYTES, "copy_us": round(copy_ms * 1e3, 2),
         "copy_GBps": round(2 * COPY_BYTES / (copy_ms * 1e-3) / 1e9, 1),
         "spmm_K": K, "n_reduced": n_red, "nnz_reduced": nnz,
         "spmm_algorithmic_bytes": spmm_diff_bytes(n_red, nnz, K), "spmm_us": round(spmm_ms * 1e3, 2),
-        "block_iterations": iters,
+        "spmm_csr_us": round(spmm_csr_ms * 1e3, 2), "spmm_with_gram_us": round(gram_ms * 1e3, 2),
+        "vcycle_us": round(vc_ms * 1e3, 2), "vcycle_algorithmic_bytes": vcycle_bytes(ctx.info, K, ctx.tuning["degree"]),
+        "recovery_build_s": round(rec_s, 3), "dadt_ms_per_8": round(dadt_ms, 3),
+        "stages_harmonic_ms": stages_h, "stages_raw_ms": stages_raw,
     }), flush=True)
     ctx.free()
     return 0
@@ -206,12 +209,146 @@ def learned_leg(ctx, pos, mom, n_train: int, n_test: int, log_sd: float = 0.2) -
     return out
 
 
+def cpu_host() -> dict:
+    """The host the CPU baseline runs on: logical CPUs and the model name (/proc/cpuinfo, as lscpu
+    reports it)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
+
+
+def config3_leg(pos, mom, tuning: dict, block_k: int, recovery: str) -> dict:
+    """The 256-placement sweep spread over three perturbed heads (2.0 / 1.5 / 1.25 mm grids, 86 / 85
+    / 85 placements of the 16 x 16 cap grid, each head's share taking every third grid site, so
+    every head ends on a partial block): per head the mesh size, the context setup and the
+    pipelined sweep at the tuning found on the 1.5 mm head; the whole sweep's placements/s with
+    and without the three setups."""
+    from cunibs_amd import fem, synthetic
+
+    grid = synthetic.grid_sites()
+    heads, total_solve, total_setup = [], 0.0, 0.0
+    for j, spacing in enumerate((2.0, 1.5, 1.25)):
+        sites = grid[j::3]
+        head = synthetic.shell_head(spacing, perturb_rms_mm=2.0, seed=1)
+        t0 = time.perf_counter()
+        ctx = fem.SolverContext(head.nodes_mm, head.tet_nodes, head.tet_tags, head.skin_tris)
+        ctx.set_tuning(**tuning)
+        setup = time.perf_counter() - t0
+        fem.solve_sweep(ctx, pos, mom, sites[:block_k], 1e6, block_k, recovery=recovery)  # captures
+        t0 = time.perf_counter()
+        res = fem.solve_sweep(ctx, pos, mom, sites, 1e6, block_k, recovery=recovery)
+        solve = time.perf_counter() - t0
+        if not np.all(res["relative_residual"] <= 1e-6):
+            raise RuntimeError(f"a placement of the {spacing} mm sweep missed the tolerance")
+        heads.append({"spacing_mm": spacing, "n_nodes": ctx.n_nodes, "n_tets": ctx.n_tets,
+                      "placements": len(sites), "setup_s": round(setup, 3), "sweep_s": round(solve, 3),
+                      "placements_per_s": round(len(sites) / solve, 2),
+                      "iterations_mean": round(float(np.mean(res["iterations"])), 2)})
+        total_solve += solve
+        total_setup += setup
+        ctx.free()
+        log(f"config 3 head {spacing} mm: {heads[-1]}")
+    n = sum(h["placements"] for h in heads)
+    return {"heads": heads, "placements": n, "placements_per_s": round(n / total_solve, 2),
+            "placements_per_s_incl_setup": round(n / (total_solve + total_setup), 2), "recovery": recovery,
+            "block_k": block_k}
+
+
+def config4_leg(ctx, head, pos, mom, n_draws: int, oracle_draws: int) -> dict:
+    """Conductivity uncertainty at one placement: n_draws draws of every tissue's conductivity
+    (log-normal, log-sd 0.2, mean-preserving), 16 per launch as independent chains under the nominal
+    preconditioner, each starting from the nominal solution's Galerkin correction; the GM peak |E|
+    mean and sd over the draws, and, for `oracle_draws` of them, the peak against the FP64 oracle
+    solved at that draw's conductivities."""
+    from cunibs_amd import fem, synthetic, uq
+
+    site = synthetic.grid_sites()[8]
+    t0 = time.perf_counter()
+    tags = ctx.uq_prepare(pos, mom, site, 1e6)
+    prep = time.perf_counter() - t0
+    sigma = uq.draw_lognormal(tags, n_draws, 0.2, seed=12)
+    ctx.uq_solve(sigma[:16], magnitude=False)  # captures
+    peaks, its = [], []
+    t0 = time.perf_counter()
+    for i in range(0, n_draws, 16):
+        res = ctx.uq_solve(sigma[i:i + 16], magnitude=False)
+        peaks.extend(float(r.metrics[0]) for r in res)
+        its.extend(r.iterations for r in res)
+    solve = time.perf_counter() - t0
+    out = {"draws": n_draws, "per_launch": 16, "draws_per_s": round(n_draws / solve, 2),
+           "prepare_s": round(prep, 3), "iterations_mean": round(float(np.mean(its)), 2),
+           "gm_peak_mean": float(np.mean(peaks)), "gm_peak_sd": float(np.std(peaks, ddof=1)),
+           "matched_preconditioners": int(ctx.info.matched_preconditioners)}
+    if oracle_draws:
+        import oracle
+
+        errs = []
+        for i in range(oracle_draws):
+            table = dict(fem.TISSUE_CONDUCTIVITY)
+            table.update({t: float(v) for t, v in zip(tags, sigma[i])})
+            octx = oracle.build_context(head.nodes_mm, head.tet_nodes, head.tet_tags, head.skin_tris,
+                                        conductivity=table)
+            ref = oracle.solve_site(octx, pos, mom, site, 1e6, tol=1e-12)
+            gm = octx.tet_tags == fem.GM_TAG
+            errs.append(abs(peaks[i] - float(ref["magnE"][gm].max())) / float(ref["magnE"][gm].max()))
+        out["gm_peak_rel_err_vs_fp64"] = max(errs)
+        out["oracle_draws"] = oracle_draws
+    return out
+
+
+def cpu_baseline_leg(head, pos, mom, sites, sample: int) -> dict:
+    """The FP64 CPU oracle on this host: `sample` placements of the sweep on min(nproc, 16)
+    threads and one on a single thread (dA/dt, RHS, AMG-PCG to 1e-6, reconstruction, after an
+    untimed setup), and the 4 mm sphere configuration end to end on the same threads (assembly,
+    AMG setup and one placement to 1e-10)."""
+    import oracle
+    from cunibs_amd import synthetic
+
+    host = cpu_host()
+    threads = min(host["nproc"] or 1, 16)
+    oracle.set_threads(threads)
+    t0 = time.time()
+    octx = oracle.build_context(head.nodes_mm, head.tet_nodes, head.tet_tags, head.skin_tris)
+    setup = time.time() - t0
+    t0 = time.time()
+    for i in range(sample):
+        oracle.solve_site(octx, pos, mom, sites[i], 1e6, tol=1e-6)
+    multi = sample / (time.time() - t0)
+    oracle.set_threads(1)
+    t0 = time.time()
+    oracle.solve_site(octx, pos, mom, sites[sample], 1e6, tol=1e-6)
+    single = 1.0 / (time.time() - t0)
+    oracle.set_threads(threads)
+    sphere = synthetic.shell_head(4.0)
+    t0 = time.time()
+    sctx = oracle.build_context(sphere.nodes_mm, sphere.tet_nodes, sphere.tet_tags, sphere.skin_tris)
+    t1 = time.time()
+    oracle.solve_site(sctx, pos, mom, synthetic.SPHERE_SITE, 1e6, tol=1e-10)
+    t2 = time.time()
+    return {
+        "value": round(multi, 4), "unit": "placements/s", "cores": threads, "kind": "port",
+        "sample": f"{sample} placements of the 1.5 mm sweep (dA/dt, RHS, FP64 AMG-PCG to 1e-6, reconstruction) "
+                  f"after an untimed oracle setup ({setup:.1f} s)",
+        "nproc": host["nproc"], "model": host["model"],
+        "single_thread_placements_per_s": round(single, 4),
+        "config1_end_to_end_s": {"setup": round(t1 - t0, 3), "solve_1e-10": round(t2 - t1, 3),
+                                 "total": round(t2 - t0, 3), "n_nodes": int(sphere.nodes_mm.shape[0])},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=32)
+    ap.add_argument("--steps", type=int, default=16, help="blocks of the timed sweep (steps x block_k placements)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--block-k", type=int, default=8)
+    ap.add_argument("--block-k", type=int, default=16)
     ap.add_argument("--spacing", type=float, default=1.5)
     ap.add_argument("--plain", type=int, default=16, help="placements timed at k=1")
     ap.add_argument("--accuracy", type=int, default=2, help="placements checked against the FP64 oracle")
@@ -225,10 +362,12 @@ def main() -> int:
     ap.add_argument("--learned", type=int, default=512,
                     help="training solves of the learned warm start on random poses and conductivities, "
                          "evaluated on 256 held-out placements (0: skip)")
+    ap.add_argument("--config3", type=int, default=1, help="1: the 256-placement sweep over three heads")
+    ap.add_argument("--uq-draws", type=int, default=512, help="conductivity draws of the UQ leg (0: skip)")
     ap.add_argument("--graphs", type=int, default=1,
                     help="0: launch every iteration's kernels directly instead of replaying its captured graph")
     ap.add_argument("--fine-sub", type=int, default=-1,
-                    help="--kernels-only: fine-level tile (sub-lanes) to measure at (-1: automatic)")
+                    help="--kernels-only / --sweep-only: fine-level tile (sub-lanes) to measure at (-1: automatic)")
     ap.add_argument("--isolated", type=int, default=0,
                     help="1: also time the SpMM and V-cycle in isolated repeated launches (warm caches)")
     ap.add_argument("--kernels-only", action="store_true",
@@ -264,39 +403,31 @@ def main() -> int:
     if args.sweep_only:
         return sweep_only(ctx, coil, sites, args)
 
-    def block(i0: int, kk: int, warm: bool = False):
+    def block(i0: int, kk: int, warm: bool = False, recovery: str = "raw"):
         chunk = [sites[(i0 + j) % len(sites)] for j in range(kk)]
         return fem.solve_placements_block(ctx, pos, mom, chunk, 1e6, magnitude=False, metrics=True,
-                                          warm_start=warm, use_graph=bool(args.graphs))
+                                          warm_start=warm, use_graph=bool(args.graphs), recovery=recovery)
 
-    def sweep(warm: bool):
-        """The timed sweep: steps x k placements through the pipelined sweep entry point (the next
-        block's dA/dt and RHS overlap the current block's solve), after warmup blocks."""
+    def sweep(warm: bool, recovery: str):
+        """The timed sweep: steps x k placements of the 16 x 16 grid through the pipelined sweep entry
+        point (the next block's dA/dt and RHS overlap the current block's solve), after warmup
+        blocks."""
         if args.warmup:
             fem.solve_sweep(ctx, pos, mom, [sites[j % len(sites)] for j in range(args.warmup * k)], 1e6, k,
-                            warm_start=warm, use_graph=bool(args.graphs))
+                            warm_start=warm, use_graph=bool(args.graphs), recovery=recovery)
         chunk = [sites[j % len(sites)] for j in range(args.steps * k)]
+        ctx.timing_enable(True)  # the in-solve sums cover this sweep alone
         t_start = time.perf_counter()
-        res = fem.solve_sweep(ctx, pos, mom, chunk, 1e6, k, warm_start=warm, use_graph=bool(args.graphs))
+        res = fem.solve_sweep(ctx, pos, mom, chunk, 1e6, k, warm_start=warm, use_graph=bool(args.graphs),
+                              recovery=recovery)
         elapsed = time.perf_counter() - t_start
+        timing = ctx.timing_read()
         its = [int(i) for i in res["iterations"]]
         block_its = [max(its[b:b + k]) for b in range(0, len(its), k)]
         if not np.all(res["relative_residual"] <= 1e-6):
             raise RuntimeError("a placement of the sweep missed the tolerance")
-        return elapsed, its, block_its
-
-    def phase_run(warm: bool, n_blocks: int = 4) -> dict:
-        """Device time per block by phase, measured on blocks solved one at a time (HIP events
-        between the phases of the block solve; the pipelined sweep overlaps them)."""
-        ctx.timing_enable(True)
-        for b in range(n_blocks):
-            block(b * k, k, warm)
-        ph = ctx.timing_phases()
-        nonlocal unshared_timing
-        unshared_timing = ctx.timing_read()
-        return {p: round(v, 3) for p, v in ph["ms_per_block"].items()}
-
-    unshared_timing = None
+        return {"elapsed": elapsed, "iters": its, "block_iters": block_its, "timing": timing,
+                "rate": args.steps * k / elapsed}
 
     ctx.timing_enable(True)
     tune = None
@@ -317,14 +448,15 @@ def main() -> int:
         overlap = autotune.tune_overlap(ctx, pos, mom, synthetic.cap_sites(8 * k, seed=19), 1e6, k)
         log(f"overlap: coil_blocks {overlap['coil_blocks']}, rates "
             f"{ {c: round(r, 1) for c, r in overlap['rates'].items()} }")
-    ctx.timing_enable(True)  # reset the in-solve sums
-    cold_elapsed, cold_iters, cold_block_iters = sweep(False)
-    cold_timing = ctx.timing_read()
-    cold_value = args.steps * k / cold_elapsed
-    log(f"batched cold k={k}: {cold_value:.1f} placements/s, {1e3 * cold_elapsed / args.steps:.1f} ms/step, "
-        f"iterations mean {np.mean(cold_iters):.1f}")
+    # Cold sweeps at the reference's default recovery (harmonic) and at raw.
+    cold = sweep(False, "harmonic")
+    cold_raw = sweep(False, "raw")
+    log(f"batched cold k={k}: harmonic {cold['rate']:.1f} placements/s ({1e3 * cold['elapsed'] / args.steps:.1f} "
+        f"ms/step), raw {cold_raw['rate']:.1f}; iterations mean {np.mean(cold['iters']):.1f}")
+    warm = warm_raw = None
     train_s = 0.0
     rank = 0
+    cold_tuning = ctx.tuning
     if args.warm_basis > 0:
         # Warm-start basis: solutions of training placements drawn uniformly on the same cap
         # (a different seed from any sweep), solved once, then the Galerkin projector.
@@ -338,43 +470,50 @@ def main() -> int:
         train_s = time.perf_counter() - t0
         log(f"warm basis: {args.warm_basis} training solves, rank {rank}, {train_s:.2f}s")
         if args.autotune:
-            overlap = autotune.tune_overlap(ctx, pos, mom, synthetic.cap_sites(8 * k, seed=19), 1e6, k,
-                                            candidates=(128, 256, 512, 768, 1024, 0), warm_start=True)
-            log(f"overlap (warm): coil_blocks {overlap['coil_blocks']}, rates "
-                f"{ {c: round(r, 1) for c, r in overlap['rates'].items()} }")
-        ctx.timing_enable(True)
-        elapsed, iters, block_iters = sweep(True)
-        timing = ctx.timing_read()
+            overlap_warm = autotune.tune_overlap(ctx, pos, mom, synthetic.cap_sites(8 * k, seed=19), 1e6, k,
+                                                 candidates=(128, 256, 512, 768, 1024, 0), warm_start=True)
+            log(f"overlap (warm): coil_blocks {overlap_warm['coil_blocks']}")
+        warm = sweep(True, "harmonic")
+        warm_raw = sweep(True, "raw")
+        log(f"batched warm k={k}: harmonic {warm['rate']:.1f} placements/s, raw {warm_raw['rate']:.1f}; "
+            f"iterations mean {np.mean(warm['iters']):.1f}")
+        ctx.set_tuning(**cold_tuning)
+    n_sweep = args.steps * k
+    warm_incl = n_sweep / (warm["elapsed"] + train_s) if warm else None
+    # The headline: the sweep at the drop-in default recovery, by the method that is fastest for a
+    # sweep of this size once the warm start's snapshot solves and mode build are charged to it.
+    if warm and warm_incl > cold["rate"]:
+        value, method, head_run = warm_incl, "warm-started, snapshot build included", warm
     else:
-        elapsed, iters, block_iters = cold_elapsed, cold_iters, cold_block_iters
-        timing = cold_timing
-    value = args.steps * k / elapsed
-    log(f"batched{' warm' if args.warm_basis > 0 else ''} k={k}: {value:.1f} placements/s, "
-        f"{1e3 * elapsed / args.steps:.1f} ms/step, iterations mean {np.mean(iters):.1f}")
-    # Device time per block by phase, on blocks solved one at a time; their sum against the
-    # pipelined step time shows what the overlap of the coil stage with the solve buys.
-    block_phases = phase_run(args.warm_basis > 0)
+        value, method, head_run = cold["rate"], "cold (the warm start's snapshot build does not pay off at this size)", cold
+    breakeven = None
+    if warm and warm["rate"] > cold["rate"]:
+        breakeven = int(np.ceil(train_s / (1.0 / cold["rate"] - 1.0 / warm["rate"])))
+    timing = cold["timing"]
+    # Device time per block by phase, on blocks solved one at a time.
+    ctx.timing_enable(True)
+    for b in range(4):
+        block(b * k, k, False, "harmonic")
+    ph = ctx.timing_phases()
+    unshared_timing = ctx.timing_read()
+    block_phases = {p: round(v, 3) for p, v in ph["ms_per_block"].items()}
     block_phases["sum"] = round(sum(block_phases.values()), 3)
     log(f"block phases, blocks one at a time (ms per block of {k}): {block_phases}; pipelined step "
-        f"{1e3 * elapsed / args.steps:.1f} ms")
+        f"{1e3 * cold['elapsed'] / args.steps:.1f} ms")
     dadt_phase_ms = block_phases["dadt"] / k
-    guess_ms = None
-    if args.warm_basis > 0:
-        guess_ms = {key: round(v, 4) for key, v in ctx.bench_guess(k, 5).items()}
-        log(f"warm-start guess stages: {guess_ms}")
 
     # Plain single-placement GPU PCG on the same sites, at the reference's smoother and the
-    # automatic tile (no tuning, no warm start).
+    # automatic tile (no tuning, no warm start), harmonic recovery as the headline.
     plain_iters = [0]
     plain = float("nan")
     if args.plain > 0:
         tuned = ctx.tuning
         ctx.set_tuning(degree=2, lower_ratio=12.0, fine_sub=0, coarse_sub=0)
         plain_iters = []
-        block(0, 1)
+        block(0, 1, recovery="harmonic")
         t0 = time.perf_counter()
         for i in range(args.plain):
-            res = block(i, 1)
+            res = block(i, 1, recovery="harmonic")
             plain_iters.append(res[0].iterations)
         plain = args.plain / (time.perf_counter() - t0)
         log(f"plain k=1: {plain:.1f} placements/s, iterations mean {np.mean(plain_iters):.1f}")
@@ -385,9 +524,8 @@ def main() -> int:
         ex: dict = {}
         fem.solve_placements_block(ctx, pos, mom, sites[:kk], 1e6, magnitude=False, metrics=False, extra=ex)
         rounds[kk] = {"iterations": ex["round_iterations"], "true_residual": ex["round_residual"]}
-        log(f"rounds k={kk}: {rounds[kk]}")
 
-    # Rooflines from the timed sweep itself: device wall-clock stamps inside the captured
+    # Rooflines from the timed cold sweep itself: device wall-clock stamps inside the captured
     # iteration (the differenced SpMM's own blocks, and marks around the V-cycle), averaged over
     # every iteration of the sweep.
     n_red = int(ctx.info.n_reduced)
@@ -402,15 +540,11 @@ def main() -> int:
     log(f"in-sweep ({timing['samples']} samples) K={K}: differenced SpMM {spmm_ms * 1e3:.1f} us "
         f"({achieved:.0f} GB/s), V-cycle {vc_ms * 1e3:.1f} us ({vc_achieved:.0f} GB/s), "
         f"iteration {it_ms * 1e3:.1f} us")
-    # dA/dt: FP32 on the vector ALUs, 20 flop per (node, dipole) pair (the distance as r.s, rn + sn
-    # - 2 r.s, rsqrt, cube, six multiply-adds), timed with HIP events around each block's launch.
     n_dip = pos.shape[0]
     dadt_flop = 20.0 * ctx.n_nodes * n_dip
     dadt_ms = dadt_phase_ms
     dadt_tf = dadt_flop / (dadt_ms * 1e-3) / 1e12 if dadt_ms > 0 else float("nan")
     dadt_iso = round(ctx.bench_dadt(pos, mom, sites[0], k=8, reps=3) / 8, 4)
-    log(f"dA/dt {dadt_ms:.3f} ms per placement (blocks one at a time) ({dadt_tf:.1f} TFLOP/s); isolated per placement "
-        f"{dadt_iso}")
     isolated = None
     if args.isolated:
         isolated = {"spmm_us": round(ctx.bench_spmm(K, 50, mode=1) * 1e3, 2),
@@ -426,34 +560,40 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
-        "block_phases_ms": block_phases,
-        "guess_stage_ms": guess_ms,
+        "ms_per_step": round(1e3 * head_run["elapsed"] / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32 (fp64 Gram/refinement)",
         "data": "synthetic (seeded perturbed 5-shell head, figure-8 coil)",
         "config": {
-            "workload": f"{args.spacing} mm perturbed 5-shell head, 16x16 cap-grid sweep, block_k={k}",
+            "workload": f"{args.spacing} mm perturbed 5-shell head, {n_sweep} placements of the 16x16 cap grid, "
+                        f"block_k={k}, harmonic recovery",
             "n_nodes": ctx.n_nodes, "n_tets": ctx.n_tets, "nnz_reduced": nnz,
-            "hierarchy": ctx.hierarchy_sizes, "block_k": k, "tolerance": 1e-6, "recovery": "raw",
-            "warm_start": (f"Galerkin projection on {args.warm_basis} solver-generated solutions (FP32 snapshot "
-                           f"basis, A-orthonormal modes as BF16 hi+lo pairs, MFMA expansion)")
-            if args.warm_basis > 0 else "none",
+            "hierarchy": ctx.hierarchy_sizes, "block_k": k, "tolerance": 1e-6, "recovery": "harmonic",
+            "method": method,
         },
-        "batched_cold_placements_per_s": round(cold_value, 3),
-        "cold_iterations_mean": round(float(np.mean(cold_iters)), 2),
-        "cold_block_iterations_mean": round(float(np.mean(cold_block_iters)), 2),
-        "block_iterations_mean": round(float(np.mean(block_iters)), 2),
-        "warm_basis_rank": rank,
-        "warm_training_s": round(train_s, 3),
-        "value_incl_training": round(args.steps * k / (elapsed + train_s), 3),
+        "batched_cold_placements_per_s": round(cold["rate"], 3),
+        "batched_cold_raw_placements_per_s": round(cold_raw["rate"], 3),
+        "cold_iterations_mean": round(float(np.mean(cold["iters"])), 2),
+        "cold_block_iterations_mean": round(float(np.mean(cold["block_iters"])), 2),
+        "warm": None if not warm else {
+            "placements_per_s_excl_build": round(warm["rate"], 3),
+            "raw_placements_per_s_excl_build": round(warm_raw["rate"], 3),
+            "placements_per_s_incl_build": round(warm_incl, 3),
+            "snapshot_build_s": round(train_s, 3),
+            "basis": f"Galerkin projection on {args.warm_basis} solver-generated solutions (FP32 snapshots, "
+                     f"A-orthonormal modes as BF16 hi+lo pairs, MFMA expansion)",
+            "rank": rank,
+            "iterations_mean": round(float(np.mean(warm["iters"])), 2),
+            "block_iterations_mean": round(float(np.mean(warm["block_iters"])), 2),
+            "break_even_placements": breakeven,
+        },
         "plain_placements_per_s": round(plain, 3),
         "speedup_vs_plain": round(value / plain, 3),
-        "iterations_mean": round(float(np.mean(iters)), 2),
         "plain_iterations_mean": round(float(np.mean(plain_iters)), 2),
         "setup_s": round(setup_s, 3),
+        "block_phases_ms": block_phases,
         "autotune": tune.as_dict() if tune else None,
         "overlap": ({"coil_blocks": overlap["coil_blocks"],
                      "rates": {str(c): round(r, 2) for c, r in overlap["rates"].items()}} if overlap else None),
@@ -469,9 +609,9 @@ def main() -> int:
             "avg_us": round(spmm_ms * 1e3, 2),
             "samples": timing["samples"],
             "algorithmic_bytes": spmm_diff_bytes(n_red, nnz, K),
-            "note": "timed inside the pipelined sweep, sharing the device with the next block's dA/dt",
+            "layout": "sliced ELL" if ctx.tuning.get("layout", 0) == 0 and ctx.info.ell_width_off else "CSR",
+            "note": "timed inside the headline's pipelined cold sweep, sharing the device with the next block's dA/dt",
         },
-        # the same kernel over the blocks solved one at a time (nothing else on the device)
         "roofline_unshared": {
             "achieved": round(unshared_achieved, 1),
             "frac": round(unshared_achieved / HBM_PEAK_GBS, 4),
@@ -499,49 +639,40 @@ def main() -> int:
             "isolated_ms_per_placement": dadt_iso,
         },
         "iteration_us": round(it_ms * 1e3, 2),
-        "cold_in_sweep_us": {"spmm": round(cold_timing["spmm_ms"] * 1e3, 2),
-                             "vcycle": round(cold_timing["vcycle_ms"] * 1e3, 2),
-                             "iteration": round(cold_timing["iteration_ms"] * 1e3, 2)},
         "isolated_us": isolated,
     }
 
     if not args.no_cpu:
         import oracle
 
-        ncores = min(os.cpu_count() or 1, 16)
-        oracle.set_threads(ncores)
-        t0 = time.time()
+        oracle.set_threads(min(os.cpu_count() or 1, 16))
         octx = oracle.build_context(head.nodes_mm, head.tet_nodes, head.tet_tags, head.skin_tris)
-        log(f"oracle context {time.time() - t0:.1f}s")
-        # Accuracy: GM |E| of sample placements against the FP64 oracle solved to 1e-12.
+        # Accuracy: GM |E| of sample placements against the FP64 oracle solved to 1e-12 (raw
+        # field: the oracle's recovery is checked separately).
         errs = []
         gm = octx.tet_tags == fem.GM_TAG
         for i in range(args.accuracy):
-            ref = oracle.solve_site(octx, pos, mom, sites[(i * 37) % len(sites)], 1e6, tol=1e-12)
-            g = fem.solve_placements_block(ctx, pos, mom, [sites[(i * 37) % len(sites)]], 1e6,
-                                           warm_start=args.warm_basis > 0)[0]
+            site = sites[(i * 37) % len(sites)]
+            ref = oracle.solve_site(octx, pos, mom, site, 1e6, tol=1e-12)
+            g = fem.solve_placements_block(ctx, pos, mom, [site], 1e6)[0]
             errs.append(float(np.linalg.norm(g.magnE[gm] - ref["magnE"][gm]) / np.linalg.norm(ref["magnE"][gm])))
         out["gm_magnE_rel_err_vs_fp64"] = max(errs) if errs else None
-        # CPU baseline: the oracle's FP64 AMG-PCG path to 1e-6, per placement.
-        t0 = time.time()
-        for i in range(args.cpu_sample):
-            oracle.solve_site(octx, pos, mom, sites[i], 1e6, tol=1e-6)
-        cpu_s = time.time() - t0
-        out["cpu_baseline"] = {
-            "value": round(args.cpu_sample / cpu_s, 4),
-            "unit": "placements/s",
-            "cores": ncores,
-            "kind": "port",
-            "sample": f"{args.cpu_sample} placements of the same sweep (dA/dt, RHS, FP64 AMG-PCG to 1e-6, "
-                      f"reconstruction) after an untimed oracle setup",
-        }
-        log(f"cpu baseline {out['cpu_baseline']['value']} placements/s on {ncores} threads; GM err {errs}")
+        del octx
+        out["cpu_baseline"] = cpu_baseline_leg(head, pos, mom, sites, args.cpu_sample)
+        log(f"cpu baseline {out['cpu_baseline']}; GM err {errs}")
+    tuning = ctx.tuning
+    if args.uq_draws > 0:
+        out["conductivity_uq"] = config4_leg(ctx, head, pos, mom, args.uq_draws, 0 if args.no_cpu else 2)
+        log(f"conductivity UQ: {out['conductivity_uq']}")
     if args.learned > 0:
         out["learned"] = learned_leg(ctx, pos, mom, args.learned, 256)
         log(f"learned warm start: {out['learned']}")
+    ctx.free()
+    if args.config3:
+        out["three_head_sweep"] = config3_leg(pos, mom, tuning, k, "harmonic")
+        log(f"config 3: {out['three_head_sweep']}")
 
     print(json.dumps(out), flush=True)
-    ctx.free()
     return 0
 
 

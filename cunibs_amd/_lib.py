@@ -50,13 +50,14 @@ class Info(ctypes.Structure):
         ("ground_node", _I32), ("n_levels", _I32),
         ("level_rows", _I64 * (MAX_LEVELS + 1)), ("level_nnz", _I64 * (MAX_LEVELS + 1)),
         ("n_region", _I64), ("region_volume_m3", _D), ("n_cu", _I32),
-        ("level_p_nnz", _I64 * MAX_LEVELS),
+        ("level_p_nnz", _I64 * MAX_LEVELS), ("ell_width_full", _I32), ("ell_width_off", _I32),
+        ("preconditioner_rebuilds", _I32), ("matched_preconditioners", _I32),
     ]
 
 
 class Tuning(ctypes.Structure):
     _fields_ = [("degree", _I32), ("lower_ratio", _F), ("fine_sub", _I32), ("coarse_sub", _I32),
-                ("coil_blocks", _I32)]
+                ("coil_blocks", _I32), ("layout", _I32)]
 
 
 class SolveOptions(ctypes.Structure):
@@ -140,6 +141,7 @@ _SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "cn_timing_phases": (_I32, [_P, _P]),
     "cn_bench_guess": (_I32, [_P, _I32, _I32, _P]),
     "cn_bench_dadt": (_I32, [_P, _P, _P, _I64, _P, _P, _D, _I32, _I32, _P]),
+    "cn_bench_stages": (_I32, [_P, _P, _P, _I64, _P, _P, _D, _I32, _I32, _I32, _P]),
     "cn_k_gram_noalloc": (_I32, [_I64, _I32, _P, _P, _P, _P, _P]),
     "cn_debug_fail_captures": (_I32, [_P, _I32]),
     "cn_graph_execs": (_I32, [_P, _P]),

@@ -735,6 +735,44 @@ Cheby chebyshev(int degree, float lower_ratio) {
 }
 
 // fem/solve.py:224-256 (aggregation_levels) and :259-284 (build_native_vcycle).
+Hierarchy build_hierarchy_like(const CsrView& a0, const Hierarchy& like, const AmgParams& prm, hipStream_t s) {
+    if (like.levels.empty()) {  // the operator itself is the coarsest level
+        AmgParams direct = prm;
+        direct.max_levels = 0;
+        Hierarchy h = build_hierarchy(a0, direct, s);
+        h.cheb = like.cheb;
+        return h;
+    }
+    Hierarchy h;
+    h.cheb = like.cheb;
+    CsrView a = a0;
+    CsrF a_next;
+    for (std::size_t lv = 0; lv < like.levels.size(); ++lv) {
+        const Level& ref = like.levels[lv];
+        require(a.n == ref.a.n && a.nnz == ref.a.nnz, "the operator's pattern differs from the hierarchy's");
+        Level L;
+        if (lv == 0) {
+            L.a = a;
+        } else {
+            L.a_store = std::move(a_next);
+            L.a = L.a_store.view();
+        }
+        L.dinv.reset(static_cast<std::size_t>(L.a.n), "dinv");
+        l1_dinv(L.a, L.dinv.get(), s);
+        L.p = smoothed_prolongator(L.a, ref.agg.get(), ref.n_coarse, L.dinv.get(), s);
+        L.r = transpose(L.p.view(), s);
+        L.n_coarse = ref.n_coarse;
+        CsrF c = spgemm(L.r.view(), spgemm(L.a, L.p.view(), s).view(), s);
+        sync(s, "galerkin");
+        h.levels.push_back(std::move(L));
+        a_next = std::move(c);
+        a = a_next.view();
+    }
+    h.coarse = std::move(a_next);
+    coarse_inverse(h.coarse.view(), h.coarse_inv, s);
+    return h;
+}
+
 Hierarchy build_hierarchy(const CsrView& a0, const AmgParams& prm, hipStream_t s) {
     Hierarchy h;
     h.cheb = chebyshev(prm.degree, prm.lower_ratio);

@@ -92,6 +92,24 @@ struct cn_context {
     cn::DeviceBuffer<std::int32_t> d_degen;
     cn::DeviceBuffer<float> d_dip, d_dadt_nodes, d_dadt_elm, d_q, d_B, d_E, d_magn, d_rhs, d_stage;
     cn::DeviceBuffer<double> d_X, d_v, d_x0;
+    // Sliced ELL copies of the finest level's operator (with its diagonal, for the V-cycle) and of
+    // the differenced off-diagonal operator (the block iteration's SpMM); width 0 when not built.
+    cn::DeviceBuffer<int2> ell_full, ell_off;
+    int ell_full_w = 0, ell_off_w = 0;
+    // Missed-tolerance recovery counts: nominal hierarchy rebuilds, and preconditioners built for
+    // one column's own operator (per-placement conductivities, conductivity draws).
+    int precond_rebuilds = 0, matched_preconditioners = 0;
+    int layout = 0;  // cn_tuning.layout
+    void set_layout(int l) {
+        layout = l;
+        const int max_k = l == 0 ? cn::kEllAutoMaxK : (l == 2 ? cn::kMaxBlock : 0);
+        if (!hier.levels.empty()) {
+            hier.levels[0].a.ell = max_k > 0 && ell_full_w > 0 ? ell_full.get() : nullptr;
+            hier.levels[0].a.ell_w = ell_full_w;
+            hier.levels[0].a.ell_max_k = max_k;
+        }
+        solver->set_off_ell(max_k > 0 && ell_off_w > 0 ? ell_off.get() : nullptr, ell_off_w, max_k);
+    }
     // Pinned sources of the host-to-device copies a block issues without waiting on them: the
     // packed dipoles (coil stage, either stream) and the block's own small tables.
     cn::PinnedStage stage_coil, stage_blk;
@@ -246,6 +264,31 @@ __global__ void product_kernel(const double* __restrict__ a, const double* __res
     if (i < n) o[i] = a[i] * b[i];
 }
 
+// Column c of an (n, K) row-major block into a vector, and back.
+template <typename T>
+__global__ void column_get_kernel(const T* __restrict__ blk, std::int64_t n, int K, int c, T* __restrict__ v) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = blk[i * K + c];
+}
+
+template <typename T>
+__global__ void column_put_kernel(const T* __restrict__ v, std::int64_t n, int K, int c, T* __restrict__ blk) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) blk[i * K + c] = v[i];
+}
+
+// out64[i] = sum_t sigma[t] in[t * count + i] (FP64, tissues in order), out32 the same rounded.
+__global__ void combine64_kernel(const double* __restrict__ in, std::int64_t count, int T,
+                                 const double* __restrict__ sigma, double* __restrict__ out64,
+                                 float* __restrict__ out32) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    double acc = 0.0;
+    for (int q = 0; q < T; ++q) acc = fma(sigma[q], in[q * count + i], acc);
+    out64[i] = acc;
+    out32[i] = static_cast<float>(acc);
+}
+
 void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const std::int32_t* tet_nodes,
                    const std::int32_t* tet_tags, std::int64_t m, const std::int32_t* skin_tris, std::int64_t n_skin) {
     using namespace cn;
@@ -366,6 +409,10 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
     op.rowsum = c.rowsum.get();
     op.val64 = c.rval64.get();
     c.solver = std::make_unique<BlockSolver>(op, c.hier, s);
+    c.ell_full_w = build_ell(c.reduced_view(), c.ell_full, s);
+    c.ell_off_w = build_ell(op.off, c.ell_off, s);
+    sync(s, "ell copies");
+    c.set_layout(0);
 
     // Skin surface for the placement scan.
     c.skin_a.reset(std::max<std::int64_t>(3 * n_skin, 1), "skin a");
@@ -758,6 +805,96 @@ void uq_prepare(cn_context& c, const double* pos_m, const double* mom, std::int6
 
 const cn::RecoveryOp* ensure_recovery(cn_context& c, int mode);
 
+cn::AmgParams amg_params(const cn_context& c) {
+    cn::AmgParams prm;
+    prm.min_coarse_rows = c.opts.min_coarse_rows;
+    prm.max_levels = c.opts.max_levels;
+    prm.rounds = c.opts.rounds;
+    prm.degree = c.hier.cheb.degree;
+    prm.lower_ratio = c.opts.lower_ratio;
+    return prm;
+}
+
+// A preconditioner matched to one column's operator K(sigma) = sum_t sigma_t K_t: its FP64, FP32
+// and differenced values on the reduced pattern, the nominal hierarchy's aggregates with the
+// prolongators, Galerkin products and coarse inverse recomputed from it, and a solver over them.
+struct MatchedSolve {
+    cn::DeviceBuffer<double> v64;
+    cn::DeviceBuffer<float> v32, off32, rs32;
+    cn::Hierarchy h;
+    std::unique_ptr<cn::BlockSolver> solver;
+};
+
+std::unique_ptr<MatchedSolve> matched_solve(cn_context& c, const double* sigma_row) {
+    using namespace cn;
+    const hipStream_t s = c.s;
+    const std::int64_t nr = c.n - 1, nnz_off = c.nnz_red - nr;
+    const int T = static_cast<int>(c.tissue.tags.size());
+    auto ms = std::make_unique<MatchedSolve>();
+    ms->v64.reset(c.nnz_red, "matched values");
+    ms->v32.reset(c.nnz_red, "matched values");
+    ms->off32.reset(std::max<std::int64_t>(nnz_off, 1), "matched off-diagonals");
+    ms->rs32.reset(nr, "matched row sums");
+    combine64_kernel<<<grid_for(c.nnz_red), kBlock, 0, s>>>(c.tissue.vals.get(), c.nnz_red, T, sigma_row,
+                                                            ms->v64.get(), ms->v32.get());
+    combine_tissues(c.tissue.off.get(), nnz_off, T, sigma_row, 1, 1, ms->off32.get(), s);
+    combine_tissues(c.tissue.rowsum.get(), nr, T, sigma_row, 1, 1, ms->rs32.get(), s);
+    const CsrView a32{c.rptr.get(), c.rcol.get(), ms->v32.get(), nr, nr, c.nnz_red};
+    ms->h = build_hierarchy_like(a32, c.hier, amg_params(c), s);
+    Operator op;
+    op.a32 = a32;
+    op.off = {c.optr.get(), c.ocol.get(), ms->off32.get(), nr, nr, nnz_off};
+    op.rowsum = ms->rs32.get();
+    op.val64 = ms->v64.get();
+    ms->solver = std::make_unique<BlockSolver>(op, ms->h, s);
+    sync(s, "matched preconditioner");
+    return ms;
+}
+
+// Columns of a block solve that missed tolerance are solved again one at a time from their own
+// iterate (fem/solve.py:508-552): under a rebuilt nominal hierarchy (sigma == nullptr), or, when
+// the columns carry their own conductivities ((K, T) device `sigma`), each under a preconditioner
+// built for its own operator and then discarded (uq/conductivity/run.py:333-342). The block's X
+// and the columns' statistics are updated in place; a column that misses again stays missed.
+void retry_missed(cn_context& c, cn::SolveStats& st, const cn::SolveOptions& opt, const float* B, int k, int K,
+                  const double* sigma) {
+    using namespace cn;
+    std::vector<int> missed;
+    for (int i = 0; i < k; ++i)
+        if (!(st.relative_residual[i] <= opt.tolerance)) missed.push_back(i);
+    if (missed.empty()) return;
+    const hipStream_t s = c.s;
+    const std::int64_t nr = c.n - 1;
+    if (sigma == nullptr) {
+        // Rebuild the V-cycle from the operator's current values (fem/solve.py:429-442).
+        sync(s, "before the rebuild");
+        c.hier = build_hierarchy(c.reduced_view(), amg_params(c), s);
+        c.set_layout(c.layout);
+        c.solver->invalidate();
+        ++c.precond_rebuilds;
+    }
+    DeviceBuffer<float> b1(nr, "retry rhs");
+    DeviceBuffer<double> x0(nr, "retry guess"), x1(nr, "retry solution");
+    const int T = static_cast<int>(c.tissue.tags.size());
+    for (int col : missed) {
+        column_get_kernel<float><<<grid_for(nr), kBlock, 0, s>>>(B, nr, K, col, b1.get());
+        column_get_kernel<double><<<grid_for(nr), kBlock, 0, s>>>(c.d_X.get(), nr, K, col, x0.get());
+        SolveStats r;
+        if (sigma != nullptr) {
+            auto ms = matched_solve(c, sigma + static_cast<std::int64_t>(col) * T);
+            ++c.matched_preconditioners;
+            r = ms->solver->solve(b1.get(), x1.get(), x0.get(), 1, opt);
+        } else {
+            r = c.solver->solve(b1.get(), x1.get(), x0.get(), 1, opt);
+        }
+        column_put_kernel<double><<<grid_for(nr), kBlock, 0, s>>>(x1.get(), nr, K, col, c.d_X.get());
+        st.relative_residual[col] = r.relative_residual[0];
+        st.column_iterations[col] = (st.column_iterations[col] ? st.column_iterations[col] : st.iterations) +
+                                    r.iterations;
+    }
+    sync(s, "retried columns");
+}
+
 // k <= 16 conductivity samples (sigma: k x T, T in the order of the tags uq_prepare found) of the
 // prepared placement, as K independent CG chains under the nominal preconditioner.
 void uq_solve(cn_context& c, const double* sigma, int k, const cn_solve_options* sopt, cn_block_outputs* out) {
@@ -824,6 +961,7 @@ void uq_solve(cn_context& c, const double* sigma, int k, const cn_solve_options*
         throw;
     }
     c.solver->set_multi(nullptr);
+    retry_missed(c, st, opt, c.d_B.get(), k, K, u.sigma.get());
     // |E| per sample (dA/dt is the same for every sample), recovered before any statistic is
     // taken (uq/conductivity/run.py:340-352).
     const int mode = sopt ? sopt->recovery : CN_RECOVERY_RAW;
@@ -1049,8 +1187,10 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
             throw;
         }
         c.solver->set_multi(nullptr);
+        retry_missed(c, st, opt, c.d_B.get(), k, K, c.blk_sigma.get());
     } else {
         st = c.solver->solve(c.d_B.get(), c.d_X.get(), dx0, k, opt);
+        retry_missed(c, st, opt, c.d_B.get(), k, K, nullptr);
     }
     if (sopt && sopt->collect) c.warm.append(c.d_X.get(), k, K, s);
     c.mark_phase(5, s);
@@ -1216,7 +1356,8 @@ void solve_sweep(cn_context& c, const double* pos_m, const double* mom, std::int
                 c.warm.initial_guess(rhs[buf], k, Kb, c.d_x0.get(), s);
                 dx0 = c.d_x0.get();
             }
-            const SolveStats st = c.solver->solve(rhs[buf], c.d_X.get(), dx0, k, opt);
+            SolveStats st = c.solver->solve(rhs[buf], c.d_X.get(), dx0, k, opt);
+            retry_missed(c, st, opt, rhs[buf], k, Kb, nullptr);
             if (rec == nullptr) {
                 reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), nodal[buf], Kb, m, k, Kb, c.ground, nullptr,
                             c.d_magn.get(), s);
@@ -1336,6 +1477,7 @@ int32_t cn_context_get_tuning(const cn_context* c, cn_tuning* o) {
         o->fine_sub = c->solver->fine_sub_setting();
         o->coarse_sub = c->solver->coarse_sub_setting();
         o->coil_blocks = c->coil_blocks;
+        o->layout = c->layout;
     });
 }
 
@@ -1349,8 +1491,10 @@ int32_t cn_context_set_tuning(cn_context* c, const cn_tuning* t) {
         cn::require(t->coarse_sub >= 0 && t->coarse_sub <= 64 && (t->coarse_sub & (t->coarse_sub - 1)) == 0,
                     "coarse_sub must be 0 or a power of two <= 64");
         cn::require(t->coil_blocks >= 0, "coil_blocks must be >= 0");
+        cn::require(t->layout >= 0 && t->layout <= 2, "layout must be 0 (automatic), 1 (CSR) or 2 (sliced ELL)");
         cn::check(hipStreamSynchronize(c->s), "sync before retuning");
         c->coil_blocks = t->coil_blocks;
+        c->set_layout(t->layout);
         c->hier.cheb = cn::chebyshev(t->degree, t->lower_ratio);
         c->opts.degree = t->degree;
         c->opts.lower_ratio = t->lower_ratio;
@@ -1381,6 +1525,10 @@ int32_t cn_context_info(const cn_context* c, cn_info* o) {
         o->level_nnz[o->n_levels] = c->hier.coarse.nnz;
         o->n_region = c->n_region;
         o->region_volume_m3 = c->region_volume;
+        o->ell_width_full = c->ell_full_w;
+        o->ell_width_off = c->ell_off_w;
+        o->preconditioner_rebuilds = c->precond_rebuilds;
+        o->matched_preconditioners = c->matched_preconditioners;
         o->n_cu = c->n_cu;
     });
 }
@@ -1711,6 +1859,74 @@ int32_t cn_bench_guess(cn_context* c, int32_t K, int32_t reps, double* ms3) {
                     "solve a block of this width first (its right-hand sides are the projected block)");
         c->d_x0.ensure((c->n - 1) * K, "x0");
         c->warm.bench(c->d_B.get(), K, K, c->d_x0.get(), reps, ms3, c->s);
+    });
+}
+
+int32_t cn_bench_stages(cn_context* c, const double* dip_pos_m, const double* dip_moment, int64_t n_dip,
+                        const double* center_mm, const double* handle_mm, double distance_mm, int32_t k,
+                        int32_t recovery, int32_t reps, double* ms3) {
+    return guarded([&] {
+        using namespace cn;
+        require(c != nullptr && dip_pos_m && dip_moment && center_mm && ms3 && n_dip >= 1, "bad argument");
+        require(k >= 1 && k <= kMaxBlock && reps >= 1, "bad argument");
+        const hipStream_t s = c->s;
+        const int K = padded_width(k);
+        const std::int64_t n = c->n, m = c->m, nr = n - 1;
+        std::vector<double> tf(16);
+        std::vector<std::int32_t> degen(1);
+        place_block(*c, center_mm, handle_mm, &distance_mm, 1, tf.data(), degen.data());
+        std::vector<double> frames(16 * static_cast<std::size_t>(k));
+        for (int i = 0; i < k; ++i) std::copy(tf.begin(), tf.end(), frames.begin() + 16 * i);
+        c->d_dadt_nodes.ensure(3 * n * K, "dadt nodes");
+        c->d_q.ensure(4 * m * K, "corner terms");
+        c->d_B.ensure(nr * K, "rhs block");
+        c->d_X.ensure(nr * K, "solution");
+        c->d_magn.ensure(m * K, "magnE");
+        check(hipMemsetAsync(c->d_X.get(), 0, sizeof(double) * nr * K, s), "potential clear");
+        const RecoveryOp* rec = recovery == CN_RECOVERY_RAW ? nullptr : ensure_recovery(*c, recovery);
+        if (rec) c->d_slots.ensure(3 * rec->n_slots * K, "recovered slots");
+        if (rec && rec->mode != kRecoveryHarmonic) c->d_Eraw.ensure(3 * m * K, "raw E");
+        std::vector<double> met(static_cast<std::size_t>(kMetricsLen) * k);
+        auto coil = [&] {
+            dadt_for_frames(*c, frames.data(), k, K, dip_pos_m, dip_moment, n_dip, 0.1f, c->d_dadt_nodes.get(), s);
+            element_pass(c->d_dadt_nodes.get(), c->tets.get(), c->g32.get(), c->neg_vc.get(), m, k, K, nullptr,
+                         c->d_q.get(), c->corner_pos.get(), s, c->vols32.get(), c->tags.get(), nullptr, 0);
+            node_rhs(c->d_q.get(), c->n2c_ptr.get(), n, k, K, c->ground, c->d_B.get(), nullptr, s);
+        };
+        auto field = [&] {
+            if (rec == nullptr)
+                reconstruct(c->d_X.get(), c->tets.get(), c->g32.get(), c->d_dadt_nodes.get(), K, m, k, K, c->ground,
+                            nullptr, c->d_magn.get(), s);
+            else if (rec->mode == kRecoveryHarmonic)
+                apply_harmonic(*rec, c->d_X.get(), c->ground, c->d_dadt_nodes.get(), K, m, k, K, c->d_slots.get(),
+                               nullptr, c->d_magn.get(), s);
+            else {
+                reconstruct(c->d_X.get(), c->tets.get(), c->g32.get(), c->d_dadt_nodes.get(), K, m, k, K, c->ground,
+                            c->d_Eraw.get(), c->d_magn.get(), s);
+                apply_spr(*rec, c->d_Eraw.get(), m, k, K, c->d_slots.get(), nullptr, c->d_magn.get(), s);
+            }
+        };
+        auto metrics = [&] {
+            region_metrics(c->d_magn.get(), m, k, K, c->region_idx.get(), c->n_region, c->vols32.get(), c->bary.get(),
+                           c->metrics_ws, met.data(), s);
+        };
+        hipEvent_t e[2];
+        for (auto& x : e) check(hipEventCreate(&x), "event");
+        auto timed = [&](auto&& f) {
+            f();  // untimed first run
+            sync(s, "stage warmup");
+            check(hipEventRecord(e[0], s), "event");
+            for (int r = 0; r < reps; ++r) f();
+            check(hipEventRecord(e[1], s), "event");
+            check(hipEventSynchronize(e[1]), "event");
+            float t = 0.f;
+            check(hipEventElapsedTime(&t, e[0], e[1]), "elapsed");
+            return static_cast<double>(t) / reps;
+        };
+        ms3[0] = timed(coil);
+        ms3[1] = timed(field);
+        ms3[2] = timed(metrics);
+        for (auto& x : e) (void)hipEventDestroy(x);
     });
 }
 

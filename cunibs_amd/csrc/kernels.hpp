@@ -18,7 +18,22 @@ struct CsrView {
     std::int64_t n = 0;       // rows
     std::int64_t n_cols = 0;  // columns
     std::int64_t nnz = 0;
+    // Optional copy of the same rows in sliced ELL form: slices of kEllSlice consecutive rows,
+    // each row padded to ell_w slots, slot j of the slice's rows contiguous, (column, value bits)
+    // pairs; a padding slot names the row itself with value 0. Entry (row r, slot j) is at
+    // (r / kEllSlice) * kEllSlice * ell_w + j * kEllSlice + r % kEllSlice.
+    const int2* ell = nullptr;
+    int ell_w = 0;
+    int ell_max_k = 0;  // the row kernels take the ELL copy for block widths up to this
 };
+
+constexpr int kEllSlice = 64;
+// Widest block for which the sliced ELL copy is the default: at 1, 2 and 4 columns the row
+// kernels are bound by their index and value loads, which the ELL copy issues without waiting on
+// row pointers (the 1.5 mm head's plain solve: SpMM 34 -> 23 us, V-cycle 252 -> 209 us); at 8 and
+// 16 the gathers of whole W rows dominate and the CSR arrays measured as fast or faster.
+constexpr int kEllAutoMaxK = 4;
+constexpr int kEllMaxWidth = 24;  // rows longer than this keep the CSR kernels
 
 // Owning CSR.
 struct CsrF {
@@ -118,5 +133,9 @@ void coarse_inverse(const CsrView& a, DeviceBuffer<double>& inv, hipStream_t s);
 std::pair<DeviceBuffer<std::int32_t>, std::int64_t> aggregate_level(const CsrView& a, const AmgParams& prm,
                                                                      hipStream_t s);
 Hierarchy build_hierarchy(const CsrView& a, const AmgParams& prm, hipStream_t s);
+// The same levels as `like` (its aggregates and coarse sizes) for another operator on the same
+// pattern: l1 diagonals, smoothed prolongators, Galerkin products and the coarse inverse recomputed
+// from `a`, no aggregation pass. The result keeps no aggregates.
+Hierarchy build_hierarchy_like(const CsrView& a, const Hierarchy& like, const AmgParams& prm, hipStream_t s);
 
 }  // namespace cn

@@ -56,6 +56,25 @@ __device__ __forceinline__ bool finished(const int* state) { return state != nul
 
 unsigned xcd_grid(std::int64_t blocks) { return static_cast<unsigned>((blocks + 7) / 8 * 8); }
 
+// One thread per ELL slot: slot j of row r takes the row's j-th CSR entry, or (r, 0) past its end.
+__global__ void ell_fill_kernel(std::int64_t n, const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
+                                const float* __restrict__ av, int w, int2* __restrict__ out) {
+    const std::int64_t slots = (n + kEllSlice - 1) / kEllSlice * kEllSlice * w;
+    for (std::int64_t t = blockIdx.x * static_cast<std::int64_t>(blockDim.x) + threadIdx.x; t < slots;
+         t += static_cast<std::int64_t>(gridDim.x) * blockDim.x) {
+        const std::int64_t slice = t / (static_cast<std::int64_t>(kEllSlice) * w);
+        const std::int64_t rem = t - slice * kEllSlice * w;
+        const int j = static_cast<int>(rem / kEllSlice);
+        const std::int64_t r = slice * kEllSlice + rem % kEllSlice;
+        int2 pv = make_int2(static_cast<int>(r < n ? r : 0), __float_as_int(0.0f));
+        if (r < n) {
+            const int p = rp[r] + j;
+            if (p < rp[r + 1]) pv = make_int2(ci[p], __float_as_int(av[p]));
+        }
+        out[t] = pv;
+    }
+}
+
 // Sum over the row groups of a wave that share a lane-in-row index (xor over L, 2L, ..., 32).
 template <int L, typename T>
 __device__ __forceinline__ T wave_rows_sum(T v) {
@@ -214,6 +233,50 @@ __device__ __forceinline__ void row_gather_multi(const std::int32_t* __restrict_
     }
 }
 
+// row_gather_v over a sliced ELL row: slot j of the row at pr[base + j * kEllSlice], slots
+// [j, e) step SUB in batches of U. The slots follow the CSR entries in order and a padding slot
+// (the row itself, value 0) adds an exact zero, so the sums are those of the CSR form. No row
+// pointers: the slot loads depend on nothing, and a wavefront's rows read one contiguous run per
+// slot.
+template <int K, int V, int SUB, bool DIFF, int U = gather_batch<SUB>(), bool GS = false>
+__device__ __forceinline__ void ell_gather_v(const int2* __restrict__ pr, std::int64_t base, int j, int e,
+                                             const float* __restrict__ X, int c0, const float (&xi)[V],
+                                             float (&acc)[V], const float* __restrict__ gscale = nullptr,
+                                             float gs0 = 0.0f) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = 0.0f;
+    for (; j < e; j += U * SUB) {
+        int cc[U];
+        float aa[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int jj = min(j + u * SUB, e - 1);
+            const int2 pv = pr[base + static_cast<std::int64_t>(jj) * kEllSlice];
+            cc[u] = pv.x;
+            aa[u] = __int_as_float(pv.y);
+        }
+        float xx[U][V];
+        float gw[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            vload<V>(X + static_cast<std::int64_t>(cc[u]) * K + c0, xx[u]);
+            if constexpr (GS) gw[u] = gscale[cc[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const float a = (j + u * SUB < e) ? aa[u] : 0.0f;
+            if constexpr (GS) {
+                const float w = gs0 * gw[u];
+#pragma unroll
+                for (int v = 0; v < V; ++v) acc[v] = fmaf(a, w * xx[u][v], acc[v]);
+            } else {
+#pragma unroll
+                for (int v = 0; v < V; ++v) acc[v] = fmaf(a, DIFF ? xx[u][v] - xi[v] : xx[u][v], acc[v]);
+            }
+        }
+    }
+}
+
 constexpr int kRowThreads = 256;
 constexpr int kRowWaves = kRowThreads / kWave;
 // Row groups each wave walks per block. One: a second pass of the loop would have to wait for the
@@ -242,12 +305,14 @@ enum RowMode : int {
 //   kChebyZero the first two smoothing steps from a zero guess in one pass: with
 //              x1 = alpha0 dinv b (formed on the fly for the row and its neighbours, never
 //              stored), out = c_cur x1 + alpha dinv (b - A x1); `x` is b and c_prev carries alpha0
-template <int K, int SUB, int MODE>
+// ELL: the rows come from the sliced ELL copy (ci holds its pairs, ell_w slots a row; rp and av
+// unused), the same sums as the CSR form.
+template <int K, int SUB, int MODE, bool ELL = false>
 __global__ void __launch_bounds__(kRowThreads) rowk_kernel(
     std::int64_t n, const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
     const float* __restrict__ av, const float* __restrict__ diag, const float* __restrict__ b,
     const float* __restrict__ x, const float* x_prev, float c_cur, float c_prev, float alpha, float* out,
-    const int* __restrict__ state, unsigned long long* __restrict__ stamps) {
+    const int* __restrict__ state, unsigned long long* __restrict__ stamps, int ell_w) {
     if (finished(state)) return;
     constexpr int V = vec_width<K>();
     constexpr int G = K / V;
@@ -273,9 +338,18 @@ __global__ void __launch_bounds__(kRowThreads) rowk_kernel(
         const std::int64_t at = rr * K + c0;
         // Row pointers first: the gather waits on them, and loads complete in issue order. Both are
         // loaded unconditionally (a load under a condition becomes a branch and a wait).
-        const int p0 = rp[rr], p1 = rp[rr + 1];
-        const int j0 = p0 + sub;
-        const int e = valid ? p1 : j0;
+        int j0, e;
+        std::int64_t ebase = 0;
+        if constexpr (ELL) {
+            j0 = sub;
+            e = valid ? ell_w : j0;
+            ebase = (rr / kEllSlice) * kEllSlice * static_cast<std::int64_t>(ell_w) + rr % kEllSlice;
+        } else {
+            const int p0 = rp[rr], p1 = rp[rr + 1];
+            j0 = p0 + sub;
+            e = valid ? p1 : j0;
+        }
+        const int2* const pr = reinterpret_cast<const int2*>(ci);
         float xi[V], acc[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) xi[v] = 0.0f;
@@ -294,12 +368,20 @@ __global__ void __launch_bounds__(kRowThreads) rowk_kernel(
         if constexpr (MODE == kProlong) vload<V>(x_prev + at, prev);  // out is x_prev
         if constexpr (MODE == kCheby || MODE == kSpmmDiff || MODE == kChebyZero) dg = diag[rr];
         if constexpr (MODE == kSpmmDiffMulti) vload<V>(diag + at, sv);
-        if constexpr (MODE == kSpmmDiffMulti)
+        if constexpr (MODE == kSpmmDiffMulti) {
+            static_assert(!ELL, "the multi-matrix values follow the CSR entries");
             row_gather_multi<K, V, SUB>(ci, av, j0, e, x, c0, xi, acc);
-        else if constexpr (MODE == kChebyZero)
-            row_gather_v<K, V, SUB, false, gather_batch<SUB>(), true>(ci, av, j0, e, x, c0, xi, acc, diag, c_prev);
-        else
-            row_gather_v<K, V, SUB, MODE == kSpmmDiff>(ci, av, j0, e, x, c0, xi, acc);
+        } else if constexpr (MODE == kChebyZero) {
+            if constexpr (ELL)
+                ell_gather_v<K, V, SUB, false, gather_batch<SUB>(), true>(pr, ebase, j0, e, x, c0, xi, acc, diag, c_prev);
+            else
+                row_gather_v<K, V, SUB, false, gather_batch<SUB>(), true>(ci, av, j0, e, x, c0, xi, acc, diag, c_prev);
+        } else {
+            if constexpr (ELL)
+                ell_gather_v<K, V, SUB, MODE == kSpmmDiff>(pr, ebase, j0, e, x, c0, xi, acc);
+            else
+                row_gather_v<K, V, SUB, MODE == kSpmmDiff>(ci, av, j0, e, x, c0, xi, acc);
+        }
 #pragma unroll
         for (int off = G; off < L; off <<= 1) {
 #pragma unroll
@@ -1183,9 +1265,17 @@ unsigned launch_rows(int sub, const CsrView& a, const float* diag, const float* 
         if constexpr (col_groups<K>() * SUB <= kWave) {
             const unsigned g = xcd_grid((a.n + rows_per_vblock<K, SUB>() - 1) / rows_per_vblock<K, SUB>());
             grid = g;
-            if (g)
-                rowk_kernel<K, SUB, MODE><<<g, kRowThreads, 0, s>>>(a.n, a.ptr, a.col, a.val, diag, b, x, xp, cc, cp,
-                                                                    al, out, state, stamps);
+            if (g == 0) return;
+            if constexpr (MODE != kSpmmDiffMulti) {
+                if (a.ell != nullptr && K <= a.ell_max_k) {
+                    rowk_kernel<K, SUB, MODE, true><<<g, kRowThreads, 0, s>>>(
+                        a.n, nullptr, reinterpret_cast<const std::int32_t*>(a.ell), nullptr, diag, b, x, xp, cc, cp, al,
+                        out, state, stamps, a.ell_w);
+                    return;
+                }
+            }
+            rowk_kernel<K, SUB, MODE><<<g, kRowThreads, 0, s>>>(a.n, a.ptr, a.col, a.val, diag, b, x, xp, cc, cp, al,
+                                                                out, state, stamps, 0);
         } else {
             throw InvalidArgument("sub-lanes per column group exceed a wavefront");
         }
@@ -1521,6 +1611,7 @@ void BlockSolver::spmm_block_k(const int* state, const float* W) {
         require(multi_->width == K, "multi-matrix values were prepared for another block width");
         CsrView off = op_.off;
         off.val = multi_->off_vals;
+        off.ell = nullptr;  // per-column values follow the CSR entries
         spmm_grid_ = launch_rows<K, kSpmmDiffMulti>(fine_sub<K>(), off, multi_->rowsum, nullptr, W, nullptr, 0.f,
                                                     0.f, 0.f, S_.get(), state, s_, stamps());
     } else {
@@ -1879,6 +1970,23 @@ double BlockSolver::bench_vcycle(int K, int reps) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return static_cast<double>(ms) / reps;
+}
+
+int build_ell(const CsrView& a, DeviceBuffer<int2>& pairs, hipStream_t s) {
+    pairs.release();
+    if (a.n <= 0) return 0;
+    std::vector<std::int32_t> rp(static_cast<std::size_t>(a.n + 1));
+    d2h(rp.data(), a.ptr, rp.size(), s);
+    sync(s, "row pointers");
+    int w = 0;
+    for (std::int64_t r = 0; r < a.n; ++r) w = std::max(w, rp[r + 1] - rp[r]);
+    if (w == 0 || w > kEllMaxWidth) return 0;
+    const std::int64_t slots = (a.n + kEllSlice - 1) / kEllSlice * kEllSlice * w;
+    pairs.reset(static_cast<std::size_t>(slots), "ell pairs");
+    ell_fill_kernel<<<static_cast<unsigned>(std::min<std::int64_t>((slots + kBlock - 1) / kBlock, 65536)), kBlock, 0, s>>>(
+        a.n, a.ptr, a.col, a.val, w, pairs.get());
+    check(hipGetLastError(), "ell fill");
+    return w;
 }
 
 // ----------------------------------------------------------------------------------------------

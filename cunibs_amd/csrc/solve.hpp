@@ -23,6 +23,10 @@ constexpr int kLag = 4;
 
 int padded_width(int k);  // smallest compiled width >= k
 
+// The sliced ELL copy of `a` (see CsrView::ell) when its longest row has at most kEllMaxWidth
+// entries: fills `pairs` and returns the width; else 0 and `pairs` stays empty.
+int build_ell(const CsrView& a, DeviceBuffer<int2>& pairs, hipStream_t s);
+
 // G1 = A^T B1 and, with B2, G2 = A^T B2 for (n, K) FP32 device blocks, into out (device, K*K then
 // K*K more with B2), the iteration's Gram kernels and fixed-order reduction, launched on s without
 // a wait. Allocates nothing: the partials live in a file-scope device array (one per device).
@@ -113,6 +117,14 @@ class BlockSolver {
     void set_fine_sub(int sub);
     int fine_sub_setting() const { return fine_sub_set_; }
     void set_coarse_sub(int sub);
+    // The differenced operator's sliced ELL copy for the block iteration's SpMM (nullptr: CSR).
+    // Drops the captured iterations.
+    void set_off_ell(const int2* ell, int w, int max_k) {
+        op_.off.ell = ell;
+        op_.off.ell_w = w;
+        op_.off.ell_max_k = max_k;
+        drop_graphs();
+    }
     int coarse_sub_setting() const { return coarse_sub_; }
     // Forget every captured iteration (after a change to the hierarchy's smoother).
     void invalidate() { drop_graphs(); }

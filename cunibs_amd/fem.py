@@ -24,6 +24,10 @@ DEFAULT_MAX_ITERS = 2000  # fem/solve.py:57
 # this module loads the HIP library; there is no CPU fallback.
 BLOCK_SIZES: tuple[int, ...] = _lib.block_sizes()
 MAX_BLOCK: int = _lib.max_block()
+# Block width of the drop-in front end when the caller names none: the width the autotuner picks
+# on the 1.5 mm head (throughput falls at 32, where the wider Gram and coefficient work outgrow
+# the shared operator reads).
+DEFAULT_BLOCK: int = 16
 MU0_OVER_4PI = 1e-7  # fem/placement.py:16
 
 TISSUE_CONDUCTIVITY: dict[int, float] = {
@@ -473,18 +477,22 @@ class SolverContext:
         t = _lib.Tuning()
         check(lib().cn_context_get_tuning(self.handle, ctypes.byref(t)), "cn_context_get_tuning")
         return {"degree": int(t.degree), "lower_ratio": float(t.lower_ratio), "fine_sub": int(t.fine_sub),
-                "coarse_sub": int(t.coarse_sub), "coil_blocks": int(t.coil_blocks)}
+                "coarse_sub": int(t.coarse_sub), "coil_blocks": int(t.coil_blocks), "layout": int(t.layout)}
 
     def set_tuning(self, degree: int | None = None, lower_ratio: float | None = None,
                    fine_sub: int | None = None, coarse_sub: int | None = None,
-                   coil_blocks: int | None = None) -> None:
-        """Change the settings of `tuning` between solves (None keeps the current value)."""
+                   coil_blocks: int | None = None, layout: int | None = None) -> None:
+        """Change the settings of `tuning` between solves (None keeps the current value).
+        ``layout`` 0 runs the finest level's row kernels on the sliced ELL copy (when its rows
+        fit) for blocks of up to 4 columns and on the CSR arrays above, 1 always on the CSR
+        arrays, 2 always on the ELL copy; results are the same."""
         cur = self.tuning
         t = _lib.Tuning(cur["degree"] if degree is None else int(degree),
                         cur["lower_ratio"] if lower_ratio is None else float(lower_ratio),
                         cur["fine_sub"] if fine_sub is None else int(fine_sub),
                         cur["coarse_sub"] if coarse_sub is None else int(coarse_sub),
-                        cur["coil_blocks"] if coil_blocks is None else int(coil_blocks))
+                        cur["coil_blocks"] if coil_blocks is None else int(coil_blocks),
+                        cur["layout"] if layout is None else int(layout))
         check(lib().cn_context_set_tuning(self.handle, ctypes.byref(t)), "cn_context_set_tuning")
 
     def bench_spmm(self, K: int, reps: int = 20, mode: int = 1, sub: int = 0) -> float:
@@ -542,6 +550,20 @@ class SolverContext:
         ms = ctypes.c_double()
         check(lib().cn_bench_copy(self.handle, int(nbytes), reps, ctypes.byref(ms)), "cn_bench_copy")
         return float(ms.value)
+
+    def bench_stages(self, dip_pos_m, dip_moment, site, k: int = 16, recovery: str = "harmonic",
+                     reps: int = 5) -> dict:
+        """Average ms of the block stages around the solve for k copies of one placement: the coil
+        stage (dA/dt, element pass, RHS), the field on a zero potential with `recovery`, and the
+        region metrics (the block PCG does not run)."""
+        pos = np.ascontiguousarray(dip_pos_m, np.float64).reshape(-1, 3)
+        mom = np.ascontiguousarray(dip_moment, np.float64).reshape(-1, 3)
+        c = np.ascontiguousarray(site[0], np.float64).reshape(3)
+        h = np.ascontiguousarray(site[1], np.float64).reshape(3)
+        ms = np.zeros(3)
+        check(lib().cn_bench_stages(self.handle, ptr(pos), ptr(mom), pos.shape[0], ptr(c), ptr(h), float(site[2]),
+                                    int(k), _lib.RECOVERY_CODES[recovery], int(reps), ptr(ms)), "cn_bench_stages")
+        return {"coil_ms": float(ms[0]), "field_ms": float(ms[1]), "metrics_ms": float(ms[2])}
 
     def debug_fail_captures(self, n: int) -> None:
         """Test hook: the next `n` captures of the block iteration report failure, so the solves

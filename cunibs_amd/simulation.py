@@ -407,7 +407,7 @@ class Subject:
                      magnitude: bool, vectors: bool, potential: bool, recovery: str,
                      nodal: bool = False) -> Iterator[FieldResult]:
         ctx = self.context
-        chunk_k = fem.MAX_BLOCK if block_k is None else max(1, min(fem.MAX_BLOCK, int(block_k)))
+        chunk_k = fem.DEFAULT_BLOCK if block_k is None else max(1, min(fem.MAX_BLOCK, int(block_k)))
         if not (magnitude or vectors or potential or nodal) and len(sites) > chunk_k:
             # Summaries only: the pipelined sweep (the next block's dA/dt and RHS overlap the
             # current block's solve), in runs of SWEEP_BLOCKS blocks so results still stream.
@@ -447,7 +447,7 @@ class Subject:
                       nodal: bool = False, recovery: str = "harmonic",
                       block_k: int | None = None) -> Iterator[FieldResult]:
         """Stream one FieldResult per placement, in order (simulation.py:454-509). Placements are
-        solved in blocks of ``block_k`` (default MAX_BLOCK, clamped to [1, MAX_BLOCK]) that share
+        solved in blocks of ``block_k`` (default fem.DEFAULT_BLOCK = 16, clamped to [1, MAX_BLOCK]) that share
         one read of the operator and hierarchy per iteration. Arguments are checked here, before
         the first result is requested."""
         sites = _sites(placements, "iter_simulate")

@@ -89,6 +89,10 @@ typedef struct {
     int32_t coil_blocks; /* cn_solve_sweep: workgroups of the next block's dA/dt while it overlaps
                             the current block's solve (0 = the full grid); fewer leave the solve
                             more of the device */
+    int32_t layout;     /* finest level's operator in the row kernels: 0 = the sliced ELL copy
+                           (built when every row has at most 24 entries) for blocks of up to 4
+                           columns and CSR above, 1 = CSR, 2 = the ELL copy at every width; the
+                           sums are the same */
 } cn_tuning;
 
 int32_t cn_context_get_tuning(const cn_context* ctx, cn_tuning* out);
@@ -107,6 +111,12 @@ typedef struct {
     double region_volume_m3;
     int32_t n_cu;
     int64_t level_p_nnz[CN_MAX_LEVELS]; /* nonzeros of each smoothed level's prolongator */
+    int32_t ell_width_full; /* slots per row of the finest operator's sliced ELL copy (0: none) */
+    int32_t ell_width_off;  /* the same for the differenced off-diagonal operator */
+    int32_t preconditioner_rebuilds; /* block solves whose missed columns were retried singly under a
+                                        rebuilt hierarchy (fem/solve.py:508-552) */
+    int32_t matched_preconditioners; /* throwaway preconditioners built for one column's own
+                                        conductivities after it missed (uq/conductivity/run.py:333-342) */
 } cn_info;
 
 int32_t cn_context_info(const cn_context* ctx, cn_info* out);
@@ -372,6 +382,14 @@ int32_t cn_bench_dadt(cn_context* ctx, const double* dip_pos_m, const double* di
  * on the library's stream: the calibration launch for the HBM traffic counters and the practical
  * bandwidth ceiling. No counterpart in the reference (a measurement aid). */
 int32_t cn_bench_copy(cn_context* ctx, int64_t bytes, int32_t reps, double* ms);
+/* Average device time (ms) of the block stages around the solve, k copies of one placement,
+ * repeated launches (HIP events): ms3[0] the coil stage (dA/dt N-body, element pass, node RHS),
+ * ms3[1] the field on a zero potential (reconstruction, and the recovery named by `recovery`,
+ * a CN_RECOVERY_* code), ms3[2] the region metrics (one host sync inside). The block PCG does not
+ * run. A measurement aid with no counterpart in the reference. */
+int32_t cn_bench_stages(cn_context* ctx, const double* dip_pos_m, const double* dip_moment, int64_t n_dip,
+                        const double* center_mm, const double* handle_mm, double distance_mm, int32_t k,
+                        int32_t recovery, int32_t reps, double* ms3);
 /* G1 = A^T B1 and, with B2 non-null, G2 = A^T B2 for (n, K) row-major FP32 blocks A, B1, B2 in
  * device memory, written to `out` in device memory (K*K FP64 values, then K*K more with B2): the
  * kernels and the fixed-order reduction of cn_k_gram, launched on `stream` (a hipStream_t; null

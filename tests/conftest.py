@@ -51,6 +51,22 @@ def boundary_tris(tets: np.ndarray, nodes: np.ndarray) -> np.ndarray:
     return _boundary_faces(tets, nodes)
 
 
+def true_residuals(ctx, res, rhs):
+    """||b - A v|| / ||b|| per placement in FP64 on the host: b the solver's FP32 right-hand side
+    (the `rhs` extra output) with the ground row removed, A the FP64 reduced operator, v the
+    returned potential."""
+    import scipy.sparse as sp
+    rp, rc, rv = ctx.reduced()
+    a = sp.csr_matrix((rv, rc, rp), shape=(ctx.n_nodes - 1,) * 2)
+    keep = np.ones(ctx.n_nodes, bool)
+    keep[ctx.ground_node] = False
+    out = []
+    for r, b in zip(res, rhs):
+        bb = b[keep].astype(np.float64)
+        out.append(float(np.linalg.norm(bb - a @ r.v[keep]) / np.linalg.norm(bb)))
+    return out
+
+
 @pytest.fixture(scope="session")
 def oracle_mod():
     import oracle

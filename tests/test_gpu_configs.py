@@ -21,6 +21,8 @@ import os
 import numpy as np
 import pytest
 
+from conftest import true_residuals
+
 pytestmark = pytest.mark.gpu
 
 GM_BAR = 1e-3
@@ -58,21 +60,6 @@ def octx15(head15, oracle_mod):
     return oracle_mod.build_context(head15.nodes_mm, head15.tet_nodes, head15.tet_tags, head15.skin_tris)
 
 
-def _true_residuals(ctx, res, rhs):
-    """||b - A v|| / ||b|| per placement in FP64 on the host: b the solver's FP32 right-hand side
-    with the ground row removed, A the FP64 reduced operator, v the returned potential."""
-    import scipy.sparse as sp
-    rp, rc, rv = ctx.reduced()
-    a = sp.csr_matrix((rv, rc, rp), shape=(ctx.n_nodes - 1,) * 2)
-    keep = np.ones(ctx.n_nodes, bool)
-    keep[ctx.ground_node] = False
-    out = []
-    for r, b in zip(res, rhs):
-        bb = b[keep].astype(np.float64)
-        out.append(float(np.linalg.norm(bb - a @ r.v[keep]) / np.linalg.norm(bb)))
-    return out
-
-
 def test_config2_block_of_eight_at_1p5mm(ctx15, octx15, oracle_mod, coil):
     from cunibs_amd import fem, synthetic
     assert ctx15.n_nodes > 1_000_000 and ctx15.n_tets > 5_800_000
@@ -80,7 +67,7 @@ def test_config2_block_of_eight_at_1p5mm(ctx15, octx15, oracle_mod, coil):
     ex = {"rhs": 1}
     res = fem.solve_placements_block(ctx15, coil.positions_m, coil.moments, sites, 1e6, potential=True, extra=ex)
     assert all(r.relative_residual <= 1e-6 for r in res)
-    assert max(_true_residuals(ctx15, res, ex["rhs"])) <= 1e-6
+    assert max(true_residuals(ctx15, res, ex["rhs"])) <= 1e-6
     gm = octx15.tet_tags == fem.GM_TAG
     np.testing.assert_array_equal(ctx15.tet_tags, octx15.tet_tags)
     for i in (0, 5):
@@ -105,7 +92,7 @@ def test_config3_heads_ragged_sweep_and_linearity(spacing, coil, oracle_mod):
             rhs.extend(ex["rhs"])
         np.testing.assert_array_equal(sweep["metrics"], np.stack([r.metrics for r in ref]))
         np.testing.assert_array_equal(sweep["iterations"], [r.iterations for r in ref])
-        assert max(_true_residuals(ctx, ref, rhs)) <= 1e-6
+        assert max(true_residuals(ctx, ref, rhs)) <= 1e-6
         double = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites[8:], 2e6)
         for a, b in zip(ref[8:], double):
             np.testing.assert_array_equal(2.0 * a.magnE, b.magnE)

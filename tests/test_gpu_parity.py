@@ -398,3 +398,111 @@ def test_graph_cache_is_bounded_across_operator_modes(gctx, coil):
         if rep == 0:
             count = gctx.graph_execs()
     assert gctx.graph_execs() == count
+
+
+def test_ell_layout_gives_the_csr_bits(block8, gctx, coil):
+    """The finest level's sliced ELL copy and the CSR arrays give the same bits, at widths 1, 4 and
+    16 and through a whole block solve: the slots follow the CSR entries in order and padding adds
+    exact zeros. Kuhn rows fit (15 slots with the diagonal, 14 without)."""
+    from cunibs_amd import fem
+    assert gctx.info.ell_width_full == 15 and gctx.info.ell_width_off == 14
+    assert gctx.tuning["layout"] == 0
+    sites, res = block8
+    rng = np.random.default_rng(8)
+    Xs = [rng.standard_normal((gctx.n_nodes - 1, K)).astype(np.float32) for K in (1, 4, 16)]
+    out = {}
+    for layout in (2, 1, 0):  # ELL at every width, CSR, automatic
+        gctx.set_tuning(layout=layout)
+        out[layout] = ([(gctx.spmm_diff(X), gctx.vcycle(X)) for X in Xs],
+                       fem.solve_placements_block(gctx, coil.positions_m, coil.moments, sites, 1e6))
+    for layout in (2, 1):
+        for (a1, a2), (b1, b2) in zip(out[layout][0], out[0][0]):
+            np.testing.assert_array_equal(a1, b1)
+            np.testing.assert_array_equal(a2, b2)
+        for a, b in zip(out[layout][1], res):
+            np.testing.assert_array_equal(a.magnE, b.magnE)
+            assert a.iterations == b.iterations
+
+
+def test_missed_columns_are_retried_singly_after_a_rebuild(gctx, coil):
+    """fem/solve.py:508-552: a block whose columns miss tolerance (forced from the host with an
+    iteration cap below what the block needs) rebuilds the hierarchy once and solves each missed
+    column again on its own from its iterate; the answers meet the tolerance and agree with the
+    uncapped block."""
+    from cunibs_amd import fem, synthetic
+    sites = synthetic.cap_sites(4, seed=31)
+    full = fem.solve_placements_block(gctx, coil.positions_m, coil.moments, sites, 1e6)
+    need = max(r.iterations for r in full)
+    before = gctx.info.preconditioner_rebuilds
+    capped = fem.solve_placements_block(gctx, coil.positions_m, coil.moments, sites, 1e6, max_iters=need - 4)
+    assert gctx.info.preconditioner_rebuilds == before + 1
+    for a, b in zip(full, capped):
+        assert b.relative_residual <= 1e-6
+        assert rel(b.magnE, a.magnE) < 2e-5
+    assert max(r.iterations for r in capped) > need - 4
+
+
+def test_unreachable_tolerance_rebuilds_then_raises(gctx, coil):
+    """tests/test_block_solve.py:330-354: an unreachable tolerance rebuilds the hierarchy once, then
+    raises, naming a column of the block."""
+    from cunibs_amd import fem, synthetic
+    before = gctx.info.preconditioner_rebuilds
+    with pytest.raises(fem.SolverConvergenceError) as err:
+        fem.solve_placements_block(gctx, coil.positions_m, coil.moments, synthetic.cap_sites(2, seed=32), 1e6,
+                                   tolerance=0.0, max_iters=25)
+    assert gctx.info.preconditioner_rebuilds == before + 1
+    assert err.value.column in (0, 1)
+
+
+def test_a_draw_that_misses_gets_its_own_preconditioner(head4, coil):
+    """uq/conductivity/run.py:333-342: a conductivity draw that misses tolerance under the nominal
+    preconditioner (forced with an iteration cap) is solved again from its iterate under a
+    preconditioner built for its own operator (the nominal aggregates, its own prolongators and
+    Galerkin products), which is then discarded; the answers agree with the uncapped draws."""
+    from cunibs_amd import fem, synthetic, uq
+    ctx = fem.SolverContext(head4.nodes_mm, head4.tet_nodes, head4.tet_tags, head4.skin_tris)
+    try:
+        tags = ctx.uq_prepare(coil.positions_m, coil.moments, synthetic.SPHERE_SITE, 1e6)
+        sigma = uq.draw_lognormal(tags, 6, 0.5, seed=33)
+        full = ctx.uq_solve(sigma)
+        need = max(r.iterations for r in full)
+        capped = ctx.uq_solve(sigma, max_iters=need - 3)
+        assert ctx.info.matched_preconditioners >= 1
+        for a, b in zip(full, capped):
+            assert b.relative_residual <= 1e-6
+            assert rel(b.magnE, a.magnE) < 2e-5
+    finally:
+        ctx.free()
+
+
+@pytest.mark.parametrize("k", [13, 29])
+def test_wide_ragged_blocks_against_the_oracle(gctx, octx, oracle_mod, coil, k):
+    """Blocks padded to widths 16 and 32 (the drop-in default width and the widest): every
+    placement's FP64 true residual, recomputed on the host, within the tolerance; two placements'
+    GM |E| within the 1e-3 bar of the FP64 oracle."""
+    from cunibs_amd import fem, synthetic
+    from conftest import true_residuals
+    sites = synthetic.cap_sites(k, seed=40 + k)
+    ex = {"rhs": 1}
+    res = fem.solve_placements_block(gctx, coil.positions_m, coil.moments, sites, 1e6, potential=True, extra=ex)
+    assert max(true_residuals(gctx, res, ex["rhs"])) <= 1e-6
+    gm = octx.tet_tags == 2
+    for i in (0, k - 1):
+        ref = oracle_mod.solve_site(octx, coil.positions_m, coil.moments, sites[i], 1e6, tol=1e-12)
+        assert rel(res[i].magnE[gm], ref["magnE"][gm]) < GM_ERR_BAR
+
+
+def test_degenerate_and_unknown_tetrahedra_are_rejected():
+    """fem/solve.py:702-706 and fem/assembly.py:15-42: a tetrahedron of zero volume, and a tissue
+    tag the conductivity table does not hold, are rejected with ValueError naming the problem."""
+    from cunibs_amd import fem
+    from conftest import CUBE_NODES, CUBE_TETS, boundary_tris
+    skin = boundary_tris(CUBE_TETS, CUBE_NODES)
+    flat = CUBE_TETS.copy()
+    flat[2] = [0, 1, 2, 3]  # four coplanar corners of the bottom face
+    with pytest.raises(ValueError, match="volume"):
+        fem.SolverContext(CUBE_NODES, flat, np.ones(6, np.int32), skin)
+    tags = np.ones(6, np.int32)
+    tags[4] = 42
+    with pytest.raises(ValueError, match="42|tag"):
+        fem.SolverContext(CUBE_NODES, CUBE_TETS, tags, skin)
