@@ -434,7 +434,7 @@ def main() -> int:
     if args.autotune:
         from cunibs_amd import autotune
 
-        tune = autotune.autotune(ctx, pos, mom, synthetic.cap_sites(32, seed=13), 1e6, block_ks=(4, 8, 16),
+        tune = autotune.autotune(ctx, pos, mom, synthetic.cap_sites(32, seed=13), 1e6, block_ks=(4, 8, 16, 32),
                                  ratios=(8.0, 12.0, 20.0, 30.0))
         k = tune.block_k
         log(f"autotune {tune.seconds:.2f}s: block_k {k}, degree {tune.degree}, ratio {tune.lower_ratio} "

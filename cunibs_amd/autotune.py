@@ -67,10 +67,9 @@ def autotune(ctx: fem.SolverContext, dip_pos_m, dip_moment, sites: Sequence, did
              ratios: Sequence[float] = (12.0,), base_k: int = 8, warm_start: bool = False) -> TuningResult:
     """Tune on `sites` (their count should be a multiple of every width, e.g. 32): the fine and the
     coarse tiles at width `base_k`, then the degree and the Chebyshev ratio at `base_k`, then the
-    width. Widths above what a warm start
-    supports (16) are skipped when `warm_start` is set."""
+    width."""
     t_start = time.perf_counter()
-    widths = [k for k in block_ks if 1 <= k <= fem.MAX_BLOCK and k <= len(sites) and not (warm_start and k > 16)]
+    widths = [k for k in block_ks if 1 <= k <= fem.MAX_BLOCK and k <= len(sites)]
     if not widths:
         raise ValueError("no candidate block width fits the placements given")
     if base_k not in widths:

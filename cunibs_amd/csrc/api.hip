@@ -2147,7 +2147,7 @@ int32_t cn_warm_set_coefficients(cn_context* c, const double* coef, int32_t k, i
     return guarded([&] {
         cn::require(c != nullptr && coef != nullptr, "null argument");
         cn::require(c->warm.ready && R == c->warm.used_modes, "coefficients must match the current modes");
-        cn::require(k >= 1 && k <= 16, "k must be in [1, 16]");
+        cn::require(k >= 1 && k <= cn::kMaxBlock, "k must be in [1, 32]");
         c->warm_coef.assign(coef, coef + static_cast<std::size_t>(k) * R);
         c->warm_coef_k = k;
     });

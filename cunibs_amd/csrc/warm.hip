@@ -101,10 +101,11 @@ __device__ __forceinline__ void load_u16(const unsigned short* __restrict__ p, u
 // partials[(block * R16 + mode) * K + c].
 constexpr int kProjectThreads = 512;
 
+// Columns c0 .. c0 + Kh - 1 (Kh <= 16) of a block whose rows are ldb values apart.
 template <int T>
 __global__ void __launch_bounds__(kProjectThreads) project_kernel(const unsigned short* __restrict__ modes,
-                                                                  std::int64_t n, const float* __restrict__ B, int K,
-                                                                  std::int64_t rows_per_block,
+                                                                  std::int64_t n, const float* __restrict__ B, int ldb,
+                                                                  int c0, int K, std::int64_t rows_per_block,
                                                                   double* __restrict__ partials) {
     constexpr int R16 = 16 * T;
     constexpr int kWaves = kProjectThreads / 64;
@@ -120,7 +121,7 @@ __global__ void __launch_bounds__(kProjectThreads) project_kernel(const unsigned
         const std::int64_t row = r0 + quad;
         const bool ok = row < hi;
         const std::int64_t rr = ok ? row : hi - 1;  // clamped: the loads stay in bounds
-        const double b = (ok && col < K) ? static_cast<double>(B[rr * K + col]) : 0.0;
+        const double b = (ok && col < K) ? static_cast<double>(B[rr * ldb + c0 + col]) : 0.0;
         unsigned short mh[T], ml[T];
         load_u16<T>(modes + rr * R16 + col * T, mh);
         load_u16<T>(modes + half + rr * R16 + col * T, ml);
@@ -189,10 +190,11 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4_t;
 // coefficients sit in LDS transposed, so a lane's B fragment is one 8-byte read.
 constexpr int kExpandThreads = 256;
 
+// Writes columns c0 .. c0 + K - 1 (K <= 16) of an x0 whose rows are ldx values apart.
 template <int T>
 __global__ void __launch_bounds__(kExpandThreads) expand_kernel(const unsigned short* __restrict__ modes,
                                                                 std::int64_t n, const unsigned short* __restrict__ coef,
-                                                                int k, int K, double* __restrict__ x0) {
+                                                                int k, int K, int ldx, int c0, double* __restrict__ x0) {
     constexpr int R16 = 16 * T;
     __shared__ uint2 ct[2][16][kMaxWarmModes / 4];  // [half][column][mode / 4]: 4 modes' BF16 bits
     {
@@ -231,7 +233,7 @@ __global__ void __launch_bounds__(kExpandThreads) expand_kernel(const unsigned s
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const std::int64_t r = r0 + 4 * q + j;
-            if (r < n) x0[r * K + i] = (i < k) ? static_cast<double>(acc[j]) : 0.0;
+            if (r < n) x0[r * ldx + c0 + i] = (i < k) ? static_cast<double>(acc[j]) : 0.0;
         }
     }
 }
@@ -388,30 +390,32 @@ void WarmBasis::mode_chunk(int j0, float* out, hipStream_t s) const {
     if (const unsigned g = grid_for(n * 32)) gather_modes_kernel<<<g, kBlock, 0, s>>>(modes.get(), n, n_modes, j0, out);
 }
 
-void WarmBasis::expand(int k, int K, double* x0, hipStream_t s) {
+void WarmBasis::expand(int k, int K, double* x0, hipStream_t s, int ldx, int c0) {
     require(n_modes % 16 == 0 && n_modes <= kMaxWarmModes, "mode width must be a multiple of 16, at most 256");
+    require(K >= 1 && K <= 16, "the expansion writes at most 16 columns at a time");
     const std::int64_t tiles = (n + 15) / 16;
     dispatch_tiles(n_modes, [&](auto tc) {
         expand_kernel<decltype(tc)::value><<<grid_for(tiles, kExpandThreads / 64), kExpandThreads, 0, s>>>(
-            modes.get(), n, coef.get(), k, K, x0);
+            modes.get(), n, coef.get(), k, K, ldx > 0 ? ldx : K, c0, x0);
     });
 }
 
-void WarmBasis::project(const float* B, int K, hipStream_t s) {
+void WarmBasis::project(const float* B, int K, hipStream_t s, int ldb, int c0) {
     require(ready, "warm start requested but no modes are set");
-    require(K >= 1 && K <= 16, "warm start supports block widths up to 16");
+    require(K >= 1 && K <= 16, "the projection reads at most 16 columns at a time");
     require(n_modes % 16 == 0 && n_modes <= kMaxWarmModes, "mode width must be a multiple of 16, at most 256");
     const int R = n_modes;
     const int nb = kProjectBlocks;
     const std::int64_t rpb = (n + nb - 1) / nb;
     partials.ensure(static_cast<std::size_t>(nb) * R * K + static_cast<std::size_t>(R) * K, "warm partials");
     dispatch_tiles(R, [&](auto tc) {
-        project_kernel<decltype(tc)::value><<<nb, kProjectThreads, 0, s>>>(modes.get(), n, B, K, rpb, partials.get());
+        project_kernel<decltype(tc)::value><<<nb, kProjectThreads, 0, s>>>(modes.get(), n, B, ldb > 0 ? ldb : K, c0,
+                                                                           K, rpb, partials.get());
     });
 }
 
 void WarmBasis::coefficients(int k, int K, hipStream_t s) {
-    require(k >= 1 && k <= K, "k must be in [1, K]");
+    require(k >= 0 && k <= K, "k must be in [0, K]");
     const int R = n_modes;
     const int nb = kProjectBlocks;
     double* const ptb = partials.get() + static_cast<std::size_t>(nb) * R * K;
@@ -421,13 +425,19 @@ void WarmBasis::coefficients(int k, int K, hipStream_t s) {
 }
 
 void WarmBasis::initial_guess(const float* B, int k, int K, double* x0, hipStream_t s) {
-    project(B, K, s);
-    coefficients(k, K, s);
-    expand(k, K, x0, s);
+    require(K >= 1 && K <= 32, "warm start supports block widths up to 32");
+    // Blocks wider than 16 go through in halves of 16 columns (the MFMA tiles are 16 wide).
+    for (int c0 = 0; c0 < K; c0 += 16) {
+        const int Kh = std::min(16, K - c0), kh = std::max(0, std::min(k - c0, Kh));
+        project(B, Kh, s, K, c0);
+        coefficients(kh, Kh, s);
+        expand(kh, Kh, x0, s, K, c0);
+    }
 }
 
 void WarmBasis::bench(const float* B, int k, int K, double* x0, int reps, double* ms3, hipStream_t s) {
     require(reps >= 1, "reps must be positive");
+    require(K >= 1 && K <= 16, "the stage timing covers blocks of up to 16 columns");
     initial_guess(B, k, K, x0, s);  // untimed: buffers and caches
     hipEvent_t ev[4];
     for (auto& e : ev) check(hipEventCreate(&e), "event");
@@ -453,19 +463,22 @@ void WarmBasis::bench(const float* B, int k, int K, double* x0, int reps, double
 
 void WarmBasis::guess_from_coefficients(const double* c_host, int k, int K, double* x0, hipStream_t s) {
     require(ready, "warm start requested but no modes are set");
-    require(k <= 16 && K <= 16, "supplied-coefficient warm start supports block widths up to 16");
+    require(k <= K && K <= 32, "supplied-coefficient warm start supports block widths up to 32");
     const int R = n_modes;
-    std::vector<unsigned short> hl(static_cast<std::size_t>(2) * R * 16, 0);
-    for (int c = 0; c < k; ++c)
-        for (int j = 0; j < used_modes; ++j) {
-            const float v = static_cast<float>(c_host[static_cast<std::size_t>(c) * used_modes + j]);
-            const unsigned short hi = host_bf16(v);
-            hl[static_cast<std::size_t>(j) * 16 + c] = hi;
-            hl[static_cast<std::size_t>(R + j) * 16 + c] = host_bf16(v - host_bf16_value(hi));
-        }
-    coef.ensure(hl.size(), "warm coefficients");
-    stage.put(coef.get(), hl.data(), hl.size(), s);
-    expand(k, K, x0, s);
+    for (int c0 = 0; c0 < K; c0 += 16) {
+        const int Kh = std::min(16, K - c0), kh = std::max(0, std::min(k - c0, Kh));
+        std::vector<unsigned short> hl(static_cast<std::size_t>(2) * R * 16, 0);
+        for (int c = 0; c < kh; ++c)
+            for (int j = 0; j < used_modes; ++j) {
+                const float v = static_cast<float>(c_host[static_cast<std::size_t>(c0 + c) * used_modes + j]);
+                const unsigned short hi = host_bf16(v);
+                hl[static_cast<std::size_t>(j) * 16 + c] = hi;
+                hl[static_cast<std::size_t>(R + j) * 16 + c] = host_bf16(v - host_bf16_value(hi));
+            }
+        coef.ensure(hl.size(), "warm coefficients");
+        stage.put(coef.get(), hl.data(), hl.size(), s);  // waits for the previous half's copy
+        expand(kh, Kh, x0, s, K, c0);
+    }
 }
 
 }  // namespace cn

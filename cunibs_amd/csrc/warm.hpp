@@ -37,14 +37,16 @@ struct WarmBasis {
     // x0 (n, K) FP64 = Phi P Phi^T B for the (n, K) FP32 block B (columns >= k of x0 are zero):
     // project (the block partials of Phi^T B), coefficients (c = P Phi^T B), expand (x0 = Phi c).
     void initial_guess(const float* B, int k, int K, double* x0, hipStream_t s);
-    void project(const float* B, int K, hipStream_t s);
+    // Columns c0 .. c0 + K - 1 (K <= 16) of B, rows ldb apart (0: K).
+    void project(const float* B, int K, hipStream_t s, int ldb = 0, int c0 = 0);
     void coefficients(int k, int K, hipStream_t s);
     // Average ms of the three stages of initial_guess over `reps` timed runs (HIP events).
     void bench(const float* B, int k, int K, double* x0, int reps, double* ms3, hipStream_t s);
     // x0 = Phi c for caller-supplied coefficients c (k x R) FP64 host.
     void guess_from_coefficients(const double* c_host, int k, int K, double* x0, hipStream_t s);
-    // x0 = Phi c from the coefficients already in `coef`.
-    void expand(int k, int K, double* x0, hipStream_t s);
+    // x0 = Phi c from the coefficients already in `coef`: columns c0 .. c0 + K - 1 (K <= 16) of an
+    // x0 whose rows are ldx apart (0: K).
+    void expand(int k, int K, double* x0, hipStream_t s, int ldx = 0, int c0 = 0);
 };
 
 }  // namespace cn

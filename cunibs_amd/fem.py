@@ -127,9 +127,7 @@ class SolverContext:
                                       ptr(skin), skin.shape[0], ctypes.byref(opts), ctypes.byref(handle)),
               "cn_context_create")
         self._h = handle
-        info = _lib.Info()
-        check(lib().cn_context_info(self._h, ctypes.byref(info)), "cn_context_info")
-        self.info = info
+        info = self.info
         self.n_nodes = int(info.n_nodes)
         self.n_tets = int(info.n_tets)
         self.n_skin = int(info.n_skin)
@@ -142,6 +140,13 @@ class SolverContext:
         self._geom: tuple[np.ndarray, ...] | None = None
 
     # ---- lifetime ----
+    @property
+    def info(self) -> _lib.Info:
+        """The context's sizes and counters (cn_context_info), read afresh on every access."""
+        info = _lib.Info()
+        check(lib().cn_context_info(self.handle, ctypes.byref(info)), "cn_context_info")
+        return info
+
     def free(self) -> None:
         if getattr(self, "_h", None):
             lib().cn_context_destroy(self._h)

@@ -43,19 +43,21 @@ def test_modes_are_a_orthonormal_up_to_bf16(trained):
     np.testing.assert_allclose(ctx.warm_state["P"] @ g, np.eye(rank), atol=1e-6)
 
 
-def test_initial_guess_is_the_galerkin_projection(trained, coil):
+@pytest.mark.parametrize("k", [4, 20])
+def test_initial_guess_is_the_galerkin_projection(trained, coil, k):
     """x0 = Phi P Phi^T b, evaluated on the host from the device's modes, and the MFMA expansion of
-    the coefficients agrees with it."""
+    the coefficients agrees with it; a block of 20 pads to 32 and goes through in two halves of 16
+    columns."""
     from cunibs_amd import fem, synthetic
     ctx, _, _, rank = trained
     keep = np.arange(ctx.n_nodes) != ctx.ground_node
     phi = ctx.warm_modes()[:, :rank].astype(np.float64)
     p = ctx.warm_state["P"]
-    sites = synthetic.cap_sites(4, seed=12)
+    sites = synthetic.cap_sites(k, seed=12)
     extra = {"x0": 1, "rhs": 1}
     fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites, 1e6, warm_start=True, extra=extra)
     coefs = []
-    for i in range(4):
+    for i in range(k):
         b = extra["rhs"][i][keep].astype(np.float64)
         c = p @ (phi.T @ b)
         coefs.append(c)
