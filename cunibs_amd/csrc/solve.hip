@@ -1920,11 +1920,23 @@ double BlockSolver::bench_spmm(int K, int reps, int mode, int sub) {
 // Streaming copy out[i] = in[i] over float4s: the calibration kernel of the HBM traffic counters
 // (FETCH_SIZE / WRITE_SIZE of a launch that moves a known number of bytes) and the practical
 // bandwidth ceiling beside the published peak.
+// Each thread moves U float4s, all U loads issued before the first store; the grid covers the
+// array once (no loop), so every wave has U loads in flight.
+template <int U>
 __global__ void __launch_bounds__(kBlock) stream_copy_kernel(std::int64_t n4, const float4* __restrict__ in,
                                                              float4* __restrict__ out) {
-    const std::int64_t stride = static_cast<std::int64_t>(gridDim.x) * blockDim.x;
-    for (std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n4; i += stride)
-        out[i] = in[i];
+    const std::int64_t base = (static_cast<std::int64_t>(blockIdx.x) * U) * blockDim.x + threadIdx.x;
+    float4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const std::int64_t i = base + static_cast<std::int64_t>(u) * blockDim.x;
+        if (i < n4) v[u] = in[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const std::int64_t i = base + static_cast<std::int64_t>(u) * blockDim.x;
+        if (i < n4) out[i] = v[u];
+    }
 }
 
 double BlockSolver::bench_copy(std::int64_t bytes, int reps) {
@@ -1932,10 +1944,12 @@ double BlockSolver::bench_copy(std::int64_t bytes, int reps) {
     const std::int64_t n4 = bytes / 16;
     DeviceBuffer<float> a(bytes / 4), b(bytes / 4);
     fill_pattern_kernel<<<grid_for(bytes / 4), kBlock, 0, s_>>>(bytes / 4, a.get());
-    const int grid = static_cast<int>(std::min<std::int64_t>((n4 + kBlock - 1) / kBlock, 8192));
+    constexpr int kCopyU = 4;
+    const std::int64_t grid = (n4 + static_cast<std::int64_t>(kBlock) * kCopyU - 1) / (static_cast<std::int64_t>(kBlock) * kCopyU);
+    require(grid <= 0x7fffffff, "copy too large");
     auto launch = [&] {
-        stream_copy_kernel<<<grid, kBlock, 0, s_>>>(n4, reinterpret_cast<const float4*>(a.get()),
-                                                    reinterpret_cast<float4*>(b.get()));
+        stream_copy_kernel<kCopyU><<<static_cast<unsigned>(grid), kBlock, 0, s_>>>(
+            n4, reinterpret_cast<const float4*>(a.get()), reinterpret_cast<float4*>(b.get()));
     };
     launch();
     hipEvent_t e0, e1;
