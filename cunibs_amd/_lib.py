@@ -143,6 +143,7 @@ _SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "cn_bench_dadt": (_I32, [_P, _P, _P, _I64, _P, _P, _D, _I32, _I32, _P]),
     "cn_bench_stages": (_I32, [_P, _P, _P, _I64, _P, _P, _D, _I32, _I32, _I32, _P]),
     "cn_k_gram_noalloc": (_I32, [_I64, _I32, _P, _P, _P, _P, _P]),
+    "cn_context_setup_times": (_I32, [_P, _P]),
     "cn_debug_fail_captures": (_I32, [_P, _I32]),
     "cn_graph_execs": (_I32, [_P, _P]),
 }

@@ -2,6 +2,7 @@
 // entry points. No exception crosses the boundary; each call returns a status code and leaves
 // its message in a thread-local buffer.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -99,6 +100,7 @@ struct cn_context {
     // Missed-tolerance recovery counts: nominal hierarchy rebuilds, and preconditioners built for
     // one column's own operator (per-placement conductivities, conductivity draws).
     int precond_rebuilds = 0, matched_preconditioners = 0;
+    double setup_ms[8] = {};  // host wall time of the setup phases (cn_context_setup_times)
     int layout = 0;  // cn_tuning.layout
     void set_layout(int l) {
         layout = l;
@@ -292,6 +294,7 @@ __global__ void combine64_kernel(const double* __restrict__ in, std::int64_t cou
 void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const std::int32_t* tet_nodes,
                    const std::int32_t* tet_tags, std::int64_t m, const std::int32_t* skin_tris, std::int64_t n_skin) {
     using namespace cn;
+    const auto t_entry = std::chrono::steady_clock::now();
     require(n >= 2, "the mesh needs at least two nodes");
     require(m >= 1, "the mesh needs at least one tetrahedron");
     require(n < (1ll << 31) && 4 * m < (1ll << 31), "mesh too large for int32 indexing");
@@ -302,6 +305,16 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
     for (std::int64_t i = 0; i < 3 * n_skin; ++i)
         require(skin_tris[i] >= 0 && skin_tris[i] < n, "skin triangle references a node outside the mesh");
     const hipStream_t s = c.s;
+    auto t_phase = t_entry;
+    int phase = 0;
+    // Host wall time of each setup phase, ending at a wait on the stream (so it covers the phase's
+    // device work): cn_context_setup_times.
+    auto lap = [&] {
+        sync(s, "setup phase");
+        const auto now = std::chrono::steady_clock::now();
+        c.setup_ms[phase++] = std::chrono::duration<double, std::milli>(now - t_phase).count();
+        t_phase = now;
+    };
     c.n = n;
     c.m = m;
     c.n_skin = n_skin;
@@ -325,6 +338,7 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
     tets_in.release();
     tags_in.release();
 
+    lap();  // 0: checks, upload, Morton order
     DeviceBuffer<double> g64(12 * m, "g64"), vols64(m, "vols64"), cond(m, "cond");
     p1_gradients(c.nodes.get(), c.tets.get(), m, g64.get(), vols64.get(), s);
     const int bad = count_bad_volumes(vols64.get(), m, s);
@@ -343,6 +357,7 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
     c.opts.conductivity = nullptr;  // the caller's table need not outlive this call; keep a copy
     c.lut = lut;
 
+    lap();  // 1: gradients, volume and conductivity checks
     c.n2c_ptr.reset(n + 1, "n2c ptr");
     c.n2c_idx.reset(4 * m, "n2c idx");
     node2corner(c.tets.get(), m, 4, n, c.n2c_ptr.get(), c.n2c_idx.get(), s);
@@ -353,6 +368,7 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
     DeviceBuffer<double> data(std::max<std::int64_t>(nnz, 1), "stiffness");
     DeviceBuffer<double> scale(m, "vol*sigma");  // fem/assembly.py:131, FP64 product
     product_kernel<<<grid_for(m), kBlock, 0, s>>>(vols64.get(), cond.get(), m, scale.get());
+    lap();  // 2: node2corner and the stiffness pattern
     stiffness_values(g64.get(), scale.get(), c.tets.get(), c.n2c_ptr.get(), c.n2c_idx.get(), n, indptr.get(),
                      indices.get(), data.get(), s);
     c.nnz_full = nnz;
@@ -387,7 +403,9 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
     prm.rounds = c.opts.rounds;
     prm.degree = c.opts.degree;
     prm.lower_ratio = c.opts.lower_ratio;
+    lap();  // 3: stiffness values, grounding, resident arrays
     c.hier = build_hierarchy(c.reduced_view(), prm, s);
+    lap();  // 4: AMG hierarchy
     // Differenced form: every reduced row keeps its diagonal (a node lies in its own elements), so
     // dropping it shifts row r's start by exactly r.
     const std::int64_t nr = n - 1;
@@ -412,6 +430,7 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
     c.ell_full_w = build_ell(c.reduced_view(), c.ell_full, s);
     c.ell_off_w = build_ell(op.off, c.ell_off, s);
     sync(s, "ell copies");
+    lap();  // 5: differenced operator, solver, ELL copies
     c.set_layout(0);
 
     // Skin surface for the placement scan.
@@ -439,6 +458,7 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
     region_volume_kernel<<<1, kBlock, 0, s>>>(c.region_idx.get(), n_reg, c.vols32.get(), vol.get());
     d2h(&c.region_volume, vol.get(), 1, s);
     sync(s, "context build");
+    lap();  // 6: skin surface, region slice
 }
 
 // Dipoles of one placement in its coil frame, FP64 as fem/placement.py:294-313, then FP32 pack
@@ -1933,6 +1953,13 @@ int32_t cn_bench_stages(cn_context* c, const double* dip_pos_m, const double* di
 int32_t cn_k_gram_noalloc(int64_t n, int32_t K, const float* A, const float* B1, const float* B2, double* out,
                           void* stream) {
     return guarded([&] { cn::gram_noalloc(n, K, A, B1, B2, out, static_cast<hipStream_t>(stream)); });
+}
+
+int32_t cn_context_setup_times(const cn_context* c, double* ms7) {
+    return guarded([&] {
+        cn::require(c != nullptr && ms7 != nullptr, "null argument");
+        for (int i = 0; i < 7; ++i) ms7[i] = c->setup_ms[i];
+    });
 }
 
 int32_t cn_debug_fail_captures(cn_context* c, int32_t n) {

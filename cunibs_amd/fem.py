@@ -570,6 +570,14 @@ class SolverContext:
                                     int(k), _lib.RECOVERY_CODES[recovery], int(reps), ptr(ms)), "cn_bench_stages")
         return {"coil_ms": float(ms[0]), "field_ms": float(ms[1]), "metrics_ms": float(ms[2])}
 
+    def setup_times(self) -> dict:
+        """Host wall time (ms) of the context build's phases."""
+        ms = np.zeros(7)
+        check(lib().cn_context_setup_times(self.handle, ptr(ms)), "cn_context_setup_times")
+        names = ("upload_order", "gradients_checks", "pattern", "stiffness_grounding", "hierarchy",
+                 "solver_ell", "skin_region")
+        return {k: round(float(v), 3) for k, v in zip(names, ms)}
+
     def debug_fail_captures(self, n: int) -> None:
         """Test hook: the next `n` captures of the block iteration report failure, so the solves
         that need them launch their iterations directly."""
