@@ -100,7 +100,7 @@ struct cn_context {
     // Missed-tolerance recovery counts: nominal hierarchy rebuilds, and preconditioners built for
     // one column's own operator (per-placement conductivities, conductivity draws).
     int precond_rebuilds = 0, matched_preconditioners = 0;
-    double setup_ms[8] = {};  // host wall time of the setup phases (cn_context_setup_times)
+    double setup_ms[10] = {};  // host wall time of the setup phases and the last recovery build
     int layout = 0;  // cn_tuning.layout
     void set_layout(int l) {
         layout = l;
@@ -298,12 +298,15 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
     require(n >= 2, "the mesh needs at least two nodes");
     require(m >= 1, "the mesh needs at least one tetrahedron");
     require(n < (1ll << 31) && 4 * m < (1ll << 31), "mesh too large for int32 indexing");
-    for (std::int64_t i = 0; i < 3 * n; ++i)
-        require(std::isfinite(nodes_mm[i]), "node coordinates must be finite");
-    for (std::int64_t i = 0; i < 4 * m; ++i)
-        require(tet_nodes[i] >= 0 && tet_nodes[i] < n, "tetrahedron references a node outside the mesh");
-    for (std::int64_t i = 0; i < 3 * n_skin; ++i)
-        require(skin_tris[i] >= 0 && skin_tris[i] < n, "skin triangle references a node outside the mesh");
+    // Branch-free scans (each folds its whole array, then one check).
+    bool finite = true, tets_ok = true, skin_ok = true;
+    for (std::int64_t i = 0; i < 3 * n; ++i) finite &= std::isfinite(nodes_mm[i]);
+    const auto nn = static_cast<std::uint32_t>(n);
+    for (std::int64_t i = 0; i < 4 * m; ++i) tets_ok &= static_cast<std::uint32_t>(tet_nodes[i]) < nn;
+    for (std::int64_t i = 0; i < 3 * n_skin; ++i) skin_ok &= static_cast<std::uint32_t>(skin_tris[i]) < nn;
+    require(finite, "node coordinates must be finite");
+    require(tets_ok, "tetrahedron references a node outside the mesh");
+    require(skin_ok, "skin triangle references a node outside the mesh");
     const hipStream_t s = c.s;
     auto t_phase = t_entry;
     int phase = 0;
@@ -320,11 +323,13 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
     c.n_skin = n_skin;
     c.n_cu = device_cu_count();
 
+    lap();  // 0: argument checks
     DeviceBuffer<double> nodes_in = upload(nodes_mm, 3 * n, s, "nodes");
     DeviceBuffer<std::int32_t> tets_in = upload(tet_nodes, 4 * m, s, "tets");
     DeviceBuffer<std::int32_t> tags_in = upload(tet_tags, m, s, "tags");
     c.skin.reset(std::max<std::int64_t>(3 * n_skin, 1), "skin");
     h2d(c.skin.get(), skin_tris, 3 * n_skin, s);
+    lap();  // 1: uploads
     c.node_perm.reset(n, "node perm");
     c.tet_perm.reset(m, "tet perm");
     DeviceBuffer<std::int32_t> inverse(n, "inverse perm");
@@ -338,7 +343,7 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
     tets_in.release();
     tags_in.release();
 
-    lap();  // 0: checks, upload, Morton order
+    lap();  // 2: Morton order
     DeviceBuffer<double> g64(12 * m, "g64"), vols64(m, "vols64"), cond(m, "cond");
     p1_gradients(c.nodes.get(), c.tets.get(), m, g64.get(), vols64.get(), s);
     const int bad = count_bad_volumes(vols64.get(), m, s);
@@ -357,7 +362,7 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
     c.opts.conductivity = nullptr;  // the caller's table need not outlive this call; keep a copy
     c.lut = lut;
 
-    lap();  // 1: gradients, volume and conductivity checks
+    lap();  // 3: gradients, volume and conductivity checks
     c.n2c_ptr.reset(n + 1, "n2c ptr");
     c.n2c_idx.reset(4 * m, "n2c idx");
     node2corner(c.tets.get(), m, 4, n, c.n2c_ptr.get(), c.n2c_idx.get(), s);
@@ -368,7 +373,7 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
     DeviceBuffer<double> data(std::max<std::int64_t>(nnz, 1), "stiffness");
     DeviceBuffer<double> scale(m, "vol*sigma");  // fem/assembly.py:131, FP64 product
     product_kernel<<<grid_for(m), kBlock, 0, s>>>(vols64.get(), cond.get(), m, scale.get());
-    lap();  // 2: node2corner and the stiffness pattern
+    lap();  // 4: node2corner and the stiffness pattern
     stiffness_values(g64.get(), scale.get(), c.tets.get(), c.n2c_ptr.get(), c.n2c_idx.get(), n, indptr.get(),
                      indices.get(), data.get(), s);
     c.nnz_full = nnz;
@@ -403,9 +408,9 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
     prm.rounds = c.opts.rounds;
     prm.degree = c.opts.degree;
     prm.lower_ratio = c.opts.lower_ratio;
-    lap();  // 3: stiffness values, grounding, resident arrays
+    lap();  // 5: stiffness values, grounding, resident arrays
     c.hier = build_hierarchy(c.reduced_view(), prm, s);
-    lap();  // 4: AMG hierarchy
+    lap();  // 6: AMG hierarchy
     // Differenced form: every reduced row keeps its diagonal (a node lies in its own elements), so
     // dropping it shifts row r's start by exactly r.
     const std::int64_t nr = n - 1;
@@ -430,7 +435,7 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
     c.ell_full_w = build_ell(c.reduced_view(), c.ell_full, s);
     c.ell_off_w = build_ell(op.off, c.ell_off, s);
     sync(s, "ell copies");
-    lap();  // 5: differenced operator, solver, ELL copies
+    lap();  // 7: differenced operator, solver, ELL copies
     c.set_layout(0);
 
     // Skin surface for the placement scan.
@@ -458,7 +463,7 @@ void build_context(cn_context& c, const double* nodes_mm, std::int64_t n, const 
     region_volume_kernel<<<1, kBlock, 0, s>>>(c.region_idx.get(), n_reg, c.vols32.get(), vol.get());
     d2h(&c.region_volume, vol.get(), 1, s);
     sync(s, "context build");
-    lap();  // 6: skin surface, region slice
+    lap();  // 8: skin surface, region slice
 }
 
 // Dipoles of one placement in its coil frame, FP64 as fem/placement.py:294-313, then FP32 pack
@@ -1045,12 +1050,15 @@ const cn::RecoveryOp* ensure_recovery(cn_context& c, int mode) {
     require(mode >= CN_RECOVERY_RAW && mode <= CN_RECOVERY_HARMONIC, "unknown recovery mode");
     require(mode != CN_RECOVERY_RAW, "the raw mode has no recovery operator");
     if (!c.recovery[mode]) {
+        const auto t0 = std::chrono::steady_clock::now();
         auto op = std::make_unique<RecoveryOp>();
         if (mode == CN_RECOVERY_HARMONIC)
             build_harmonic(*op, c.nodes.get(), c.tets.get(), c.tags.get(), c.n, c.m, c.max_tag, c.s);
         else
             build_spr(*op, mode, c.nodes.get(), c.tets.get(), c.tags.get(), c.vols32.get(), c.n2c_ptr.get(),
                       c.n2c_idx.get(), c.n, c.m, c.max_tag, c.s);
+        sync(c.s, "recovery operator");
+        c.setup_ms[9] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         c.recovery[mode] = std::move(op);
     }
     return c.recovery[mode].get();
@@ -1955,10 +1963,10 @@ int32_t cn_k_gram_noalloc(int64_t n, int32_t K, const float* A, const float* B1,
     return guarded([&] { cn::gram_noalloc(n, K, A, B1, B2, out, static_cast<hipStream_t>(stream)); });
 }
 
-int32_t cn_context_setup_times(const cn_context* c, double* ms7) {
+int32_t cn_context_setup_times(const cn_context* c, double* ms10) {
     return guarded([&] {
-        cn::require(c != nullptr && ms7 != nullptr, "null argument");
-        for (int i = 0; i < 7; ++i) ms7[i] = c->setup_ms[i];
+        cn::require(c != nullptr && ms10 != nullptr, "null argument");
+        for (int i = 0; i < 10; ++i) ms10[i] = c->setup_ms[i];
     });
 }
 

@@ -10,6 +10,13 @@
 
 namespace cn {
 
+// Minimum over the lanes of a wavefront, returned to every lane.
+__device__ __forceinline__ int wave_min_i32(int v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
 // Non-owning CSR (int32 indices, FP32 values) handed to kernels.
 struct CsrView {
     const std::int32_t* ptr = nullptr;

@@ -181,87 +181,90 @@ __global__ void count_keys_kernel(const std::int32_t* __restrict__ keys, std::in
 }
 
 // Row i of the stiffness pattern: the sorted distinct nodes of the tetrahedra in node i's
-// node2corner segment (pattern.cu:65-87). One thread per row; each distinct value is found as the
-// smallest candidate above the previous one, so no buffer is needed and the result is the sorted
-// set whatever the candidate order.
-__device__ __forceinline__ int next_above(const std::int32_t* __restrict__ tets,
-                                          const std::int32_t* __restrict__ idx, int b, int e, int last) {
-    int best = INT_MAX;
-    for (int p = b; p < e; ++p) {
-        const int4 t = reinterpret_cast<const int4*>(tets)[idx[p] >> 2];
-        if (t.x > last && t.x < best) best = t.x;
-        if (t.y > last && t.y < best) best = t.y;
-        if (t.z > last && t.z < best) best = t.z;
-        if (t.w > last && t.w < best) best = t.w;
-    }
+// node2corner segment (pattern.cu:65-87). One wavefront per row: lane l holds the four nodes of
+// the row's tetrahedra l, l + 64, ... (the first in registers, any further ones re-read), and
+// each step takes the wavefront minimum over the lanes' smallest node above the last one taken.
+// The row comes out sorted and distinct in as many steps as it has entries, each tetrahedron is
+// read once, and the entries are written 64 at a time by the lanes that took them.
+__device__ __forceinline__ int min_above(int4 t, int last, int best) {
+    if (t.x > last && t.x < best) best = t.x;
+    if (t.y > last && t.y < best) best = t.y;
+    if (t.z > last && t.z < best) best = t.z;
+    if (t.w > last && t.w < best) best = t.w;
     return best;
 }
 
-__global__ void pattern_count_kernel(const std::int32_t* __restrict__ tets, const std::int32_t* __restrict__ ptr,
-                                     const std::int32_t* __restrict__ idx, std::int64_t n,
-                                     std::int32_t* __restrict__ counts) {
-    const std::int64_t row = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (row >= n) return;
+template <bool kFill>
+__global__ void __launch_bounds__(kBlock) pattern_wave_kernel(const std::int32_t* __restrict__ tets,
+                                                              const std::int32_t* __restrict__ ptr,
+                                                              const std::int32_t* __restrict__ idx, std::int64_t n,
+                                                              std::int32_t* __restrict__ counts_or_indptr,
+                                                              std::int32_t* __restrict__ indices) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const std::int64_t row = static_cast<std::int64_t>(blockIdx.x) * (kBlock / kWave) +
+                             __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    if (row >= n) return;  // wave-uniform
+    const int4* t4 = reinterpret_cast<const int4*>(tets);
     const int b = ptr[row], e = ptr[row + 1];
-    int k = 0, last = -1;
+    const int4 none = {INT_MAX, INT_MAX, INT_MAX, INT_MAX};
+    const int4 own = b + lane < e ? t4[idx[b + lane] >> 2] : none;
+    const bool wide = e - b > kWave;
+    const int at = kFill ? counts_or_indptr[row] : 0;
+    int k = 0, last = -1, held = 0;
     while (true) {
-        const int v = next_above(tets, idx, b, e, last);
-        if (v == INT_MAX) break;
-        ++k;
-        last = v;
-    }
-    counts[row] = k;
-}
-
-__global__ void pattern_fill_kernel(const std::int32_t* __restrict__ tets, const std::int32_t* __restrict__ ptr,
-                                    const std::int32_t* __restrict__ idx, std::int64_t n,
-                                    const std::int32_t* __restrict__ indptr, std::int32_t* __restrict__ indices) {
-    const std::int64_t row = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (row >= n) return;
-    const int b = ptr[row], e = ptr[row + 1];
-    int at = indptr[row], last = -1;
-    while (true) {
-        const int v = next_above(tets, idx, b, e, last);
-        if (v == INT_MAX) break;
-        indices[at++] = v;
-        last = v;
-    }
-}
-
-__device__ __forceinline__ int find_col(const std::int32_t* __restrict__ indices, int lo, int hi, int col) {
-    while (lo < hi) {
-        const int mid = lo + ((hi - lo) >> 1);
-        if (indices[mid] < col)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
-// stiffness.cu:32-66: one thread per row, node2corner order then j_loc ascending; FP64.
-__global__ void stiffness_kernel(const double* __restrict__ g, const double* __restrict__ scale,
-                                 const std::int32_t* __restrict__ tets, const std::int32_t* __restrict__ ptr,
-                                 const std::int32_t* __restrict__ idx, std::int64_t n,
-                                 const std::int32_t* __restrict__ indptr,
-                                 const std::int32_t* __restrict__ indices, double* __restrict__ data) {
-    const std::int64_t row = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (row >= n) return;
-    const int rb = indptr[row], re = indptr[row + 1];
-    for (int p = rb; p < re; ++p) data[p] = 0.0;
-    for (int p = ptr[row]; p < ptr[row + 1]; ++p) {
-        const std::int64_t corner = idx[p];
-        const std::int64_t e = corner >> 2;
-        const double s = scale[e];
-        const double gi0 = g[3 * corner], gi1 = g[3 * corner + 1], gi2 = g[3 * corner + 2];
-        const int4 t = reinterpret_cast<const int4*>(tets)[e];
-        const int cols[4] = {t.x, t.y, t.z, t.w};
-        for (int j = 0; j < 4; ++j) {
-            const double* gj = g + 3 * (4 * e + j);
-            const double dot = fma(gi2, gj[2], fma(gi1, gj[1], gi0 * gj[0]));
-            const int at = find_col(indices, rb, re, cols[j]);
-            data[at] = fma(s, dot, data[at]);
+        int best = min_above(own, last, INT_MAX);
+        if (wide)
+            for (int p = b + kWave + lane; p < e; p += kWave) best = min_above(t4[idx[p] >> 2], last, best);
+        best = wave_min_i32(best);
+        if (best == INT_MAX) break;
+        if (kFill) {
+            if (lane == (k & (kWave - 1))) held = best;
+            if ((k & (kWave - 1)) == kWave - 1) indices[at + (k & ~(kWave - 1)) + lane] = held;
         }
+        ++k;
+        last = best;
+    }
+    if (kFill) {
+        if (lane < (k & (kWave - 1))) indices[at + (k & ~(kWave - 1)) + lane] = held;
+    } else if (lane == 0) {
+        counts_or_indptr[row] = k;
+    }
+}
+
+// stiffness.cu:32-66: entry (i, j) sums vol*sigma (grad_i . grad_j) over the corners of node i in
+// node2corner order, then local column ascending, in FP64. One wavefront per row: lane l owns the
+// row's entries l, l + 64, ..., every lane walks the row's corners in that order (the same
+// addresses across the wavefront) and folds each contribution into the entry it owns, so every
+// entry sees the fused multiply-adds of a serial walk in the same order, and the row is written
+// coalesced without read-modify-write.
+__global__ void __launch_bounds__(kBlock) stiffness_wave_kernel(
+    const double* __restrict__ g, const double* __restrict__ scale, const std::int32_t* __restrict__ tets,
+    const std::int32_t* __restrict__ ptr, const std::int32_t* __restrict__ idx, std::int64_t n,
+    const std::int32_t* __restrict__ indptr, const std::int32_t* __restrict__ indices, double* __restrict__ data) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const std::int64_t row = static_cast<std::int64_t>(blockIdx.x) * (kBlock / kWave) +
+                             __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    if (row >= n) return;  // wave-uniform
+    const int rb = indptr[row], re = indptr[row + 1];
+    const int cb = ptr[row], ce = ptr[row + 1];
+    for (int base = rb; base < re; base += kWave) {
+        const int mine = base + lane < re ? indices[base + lane] : -1;
+        double acc = 0.0;
+        for (int p = cb; p < ce; ++p) {
+            const std::int64_t corner = idx[p];
+            const std::int64_t e = corner >> 2;
+            const double s = scale[e];
+            const double gi0 = g[3 * corner], gi1 = g[3 * corner + 1], gi2 = g[3 * corner + 2];
+            const int4 t = reinterpret_cast<const int4*>(tets)[e];
+            const int cols[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const double* gj = g + 3 * (4 * e + j);
+                const double dot = fma(gi2, gj[2], fma(gi1, gj[1], gi0 * gj[0]));
+                if (cols[j] == mine) acc = fma(s, dot, acc);
+            }
+        }
+        if (base + lane < re) data[base + lane] = acc;
     }
 }
 
@@ -537,23 +540,23 @@ std::int64_t stiffness_pattern(const std::int32_t* tets, const std::int32_t* ptr
                                DeviceBuffer<std::int32_t>& indices, hipStream_t s) {
     DeviceBuffer<std::int32_t> counts(static_cast<std::size_t>(n + 1), "pattern counts");
     check(hipMemsetAsync(counts.get() + n, 0, sizeof(std::int32_t), s), "memset");
-    if (const unsigned g = grid_for(n)) pattern_count_kernel<<<g, kBlock, 0, s>>>(tets, ptr, idx, n, counts.get());
+    const unsigned rows = grid_for(n, kBlock / kWave);
+    if (rows) pattern_wave_kernel<false><<<rows, kBlock, 0, s>>>(tets, ptr, idx, n, counts.get(), nullptr);
     indptr.reset(static_cast<std::size_t>(n + 1), "indptr");
     exclusive_scan_i32(counts.get(), indptr.get(), n + 1, s);
     std::int32_t nnz = 0;
     d2h(&nnz, indptr.get() + n, 1, s);
     sync(s, "pattern nnz");
     indices.reset(static_cast<std::size_t>(std::max(nnz, 1)), "indices");
-    if (const unsigned g = grid_for(n))
-        pattern_fill_kernel<<<g, kBlock, 0, s>>>(tets, ptr, idx, n, indptr.get(), indices.get());
+    if (rows) pattern_wave_kernel<true><<<rows, kBlock, 0, s>>>(tets, ptr, idx, n, indptr.get(), indices.get());
     return nnz;
 }
 
 void stiffness_values(const double* g, const double* scale, const std::int32_t* tets,
                       const std::int32_t* ptr, const std::int32_t* idx, std::int64_t n,
                       const std::int32_t* indptr, const std::int32_t* indices, double* data, hipStream_t s) {
-    if (const unsigned gr = grid_for(n))
-        stiffness_kernel<<<gr, kBlock, 0, s>>>(g, scale, tets, ptr, idx, n, indptr, indices, data);
+    if (const unsigned gr = grid_for(n, kBlock / kWave))
+        stiffness_wave_kernel<<<gr, kBlock, 0, s>>>(g, scale, tets, ptr, idx, n, indptr, indices, data);
 }
 
 int lowest_z_node(const double* nodes, std::int64_t n, hipStream_t s) {

@@ -584,6 +584,80 @@ __global__ void hpr_grad_kernel(const double* __restrict__ X, int ground, const 
     dst[2 * K] = static_cast<float>(-a2 - static_cast<double>(da[2 * Kn]));
 }
 
+// Slot of (node, tag), slots being ordered by node then tag; -1 if there is none.
+__device__ int find_slot(const std::int32_t* __restrict__ slot_node, const std::int32_t* __restrict__ slot_tag,
+                         std::int64_t n_slots, int node, int tag) {
+    std::int64_t lo = 0, hi = n_slots;
+    while (lo < hi) {
+        const std::int64_t mid = lo + ((hi - lo) >> 1);
+        if (slot_node[mid] < node)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    for (; lo < n_slots && slot_node[lo] == node; ++lo)
+        if (slot_tag[lo] == tag) return static_cast<int>(lo);
+    return -1;
+}
+
+static_assert(kMinPatchNodes <= kWave, "a short first ring must fit the lanes of one wavefront");
+
+// Patch of each slot (fem/recovery.py:231-293): its first ring (the sorted distinct nodes of its
+// own tetrahedra) when that has at least kMinPatchNodes nodes, else the union of the first rings
+// of the same-tissue slots of its ring's nodes (the slot is its own neighbour), sorted and
+// distinct. One wavefront per slot. The union is a merge of sorted rings: lane j keeps a cursor
+// into the ring of the slot's j-th ring node, and each step takes the wavefront minimum of the
+// lanes' heads and advances the lanes that held it, one distinct node per step.
+template <bool kFill>
+__global__ void __launch_bounds__(kBlock) patch_kernel(
+    const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ri,
+    const std::int32_t* __restrict__ slot_node, const std::int32_t* __restrict__ slot_tag, std::int64_t n_slots,
+    std::int32_t* __restrict__ counts_or_ptr, std::int32_t* __restrict__ pidx, int* __restrict__ missing) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const std::int64_t slot = static_cast<std::int64_t>(blockIdx.x) * (kBlock / kWave) +
+                              __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    if (slot >= n_slots) return;  // wave-uniform
+    const int b = rp[slot], len = rp[slot + 1] - b;
+    const int at = kFill ? counts_or_ptr[slot] : 0;
+    if (len >= kMinPatchNodes) {
+        if (kFill)
+            for (int p = lane; p < len; p += kWave) pidx[at + p] = ri[b + p];
+        else if (lane == 0)
+            counts_or_ptr[slot] = len;
+        return;
+    }
+    int cur = 0, end = 0, head = INT_MAX;
+    if (lane < len) {
+        const int nb = find_slot(slot_node, slot_tag, n_slots, ri[b + lane], slot_tag[slot]);
+        if (nb < 0) {
+            if (!kFill) atomicOr(missing, 1);
+        } else {
+            cur = rp[nb];
+            end = rp[nb + 1];
+            head = cur < end ? ri[cur] : INT_MAX;
+        }
+    }
+    int k = 0, held = 0;
+    while (true) {
+        const int v = wave_min_i32(head);
+        if (v == INT_MAX) break;
+        if (kFill) {
+            if (lane == (k & (kWave - 1))) held = v;
+            if ((k & (kWave - 1)) == kWave - 1) pidx[at + (k & ~(kWave - 1)) + lane] = held;
+        }
+        ++k;
+        if (head == v) {
+            ++cur;
+            head = cur < end ? ri[cur] : INT_MAX;
+        }
+    }
+    if (kFill) {
+        if (lane < (k & (kWave - 1))) pidx[at + (k & ~(kWave - 1)) + lane] = held;
+    } else if (lane == 0) {
+        counts_or_ptr[slot] = k;
+    }
+}
+
 }  // namespace
 
 void build_harmonic(RecoveryOp& op, const double* nodes, const std::int32_t* tets, const std::int32_t* tags,
@@ -593,46 +667,29 @@ void build_harmonic(RecoveryOp& op, const double* nodes, const std::int32_t* tet
     const std::int64_t ns = op.n_slots;
     // First ring of each slot: the distinct nodes of its own tetrahedra, sorted.
     DeviceBuffer<std::int32_t> r1_ptr, r1_idx;
-    const std::int64_t nnz1 = stiffness_pattern(tets, op.ptr.get(), op.idx.get(), ns, r1_ptr, r1_idx, s);
-    std::vector<std::int32_t> rp(static_cast<std::size_t>(ns + 1)), ri(static_cast<std::size_t>(nnz1));
-    std::vector<std::int32_t> sn(static_cast<std::size_t>(ns)), st(static_cast<std::size_t>(ns));
-    d2h(rp.data(), r1_ptr.get(), ns + 1, s);
-    d2h(ri.data(), r1_idx.get(), nnz1, s);
-    d2h(sn.data(), op.slot_node.get(), ns, s);
-    d2h(st.data(), op.slot_tag.get(), ns, s);
-    sync(s, "first rings");
-    // Slots below kMinPatchNodes grow to the union of their same-tissue neighbours' first rings
-    // (the slot is its own neighbour), sorted and deduplicated.
-    std::vector<std::uint64_t> key(static_cast<std::size_t>(ns));
-    for (std::int64_t i = 0; i < ns; ++i) key[i] = static_cast<std::uint64_t>(sn[i]) * op.stride + st[i];
-    std::vector<std::int32_t> pp(static_cast<std::size_t>(ns + 1), 0), pi;
-    pi.reserve(static_cast<std::size_t>(nnz1) + static_cast<std::size_t>(ns));
-    std::vector<std::int32_t> cand;
-    for (std::int64_t i = 0; i < ns; ++i) {
-        const int b = rp[i], e = rp[i + 1];
-        if (e - b >= kMinPatchNodes) {
-            pi.insert(pi.end(), ri.begin() + b, ri.begin() + e);
-        } else {
-            cand.clear();
-            for (int j = b; j < e; ++j) {
-                const std::uint64_t want = static_cast<std::uint64_t>(ri[j]) * op.stride + st[i];
-                const auto it = std::lower_bound(key.begin(), key.end(), want);
-                if (it == key.end() || *it != want) throw std::runtime_error("harmonic patch: missing neighbour slot");
-                const std::int64_t nb = it - key.begin();
-                cand.insert(cand.end(), ri.begin() + rp[nb], ri.begin() + rp[nb + 1]);
-            }
-            std::sort(cand.begin(), cand.end());
-            cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
-            pi.insert(pi.end(), cand.begin(), cand.end());
-        }
-        pp[i + 1] = static_cast<std::int32_t>(pi.size());
-    }
-    op.nnz_patch = static_cast<std::int64_t>(pi.size());
+    stiffness_pattern(tets, op.ptr.get(), op.idx.get(), ns, r1_ptr, r1_idx, s);
+    DeviceBuffer<std::int32_t> counts(static_cast<std::size_t>(ns + 1), "patch sizes");
+    DeviceBuffer<int> missing(1, "missing neighbour slot");
+    check(hipMemsetAsync(counts.get() + ns, 0, sizeof(std::int32_t), s), "memset");
+    check(hipMemsetAsync(missing.get(), 0, sizeof(int), s), "memset");
+    const unsigned g_slots = grid_for(ns, kBlock / kWave);
+    if (g_slots)
+        patch_kernel<false><<<g_slots, kBlock, 0, s>>>(r1_ptr.get(), r1_idx.get(), op.slot_node.get(),
+                                                       op.slot_tag.get(), ns, counts.get(), nullptr, missing.get());
     op.pptr.reset(static_cast<std::size_t>(ns + 1), "patch ptr");
-    op.pidx.reset(std::max<std::size_t>(pi.size(), 1), "patch nodes");
-    h2d(op.pptr.get(), pp.data(), pp.size(), s);
-    h2d(op.pidx.get(), pi.data(), pi.size(), s);
-    op.w3.reset(std::max<std::size_t>(3 * pi.size(), 1), "harmonic weights");
+    exclusive_scan_i32(counts.get(), op.pptr.get(), ns + 1, s);
+    std::int32_t nnz = 0;
+    int hmiss = 0;
+    d2h(&nnz, op.pptr.get() + ns, 1, s);
+    d2h(&hmiss, missing.get(), 1, s);
+    sync(s, "patch sizes");
+    if (hmiss) throw std::runtime_error("harmonic patch: missing neighbour slot");
+    op.nnz_patch = nnz;
+    op.pidx.reset(static_cast<std::size_t>(std::max(nnz, 1)), "patch nodes");
+    if (g_slots)
+        patch_kernel<true><<<g_slots, kBlock, 0, s>>>(r1_ptr.get(), r1_idx.get(), op.slot_node.get(),
+                                                      op.slot_tag.get(), ns, op.pptr.get(), op.pidx.get(), nullptr);
+    op.w3.reset(static_cast<std::size_t>(3 * std::max(nnz, 1)), "harmonic weights");
     op.status.reset(static_cast<std::size_t>(std::max<std::int64_t>(ns, 1)), "harmonic status");
     if (const unsigned g = grid_for(ns, kFitWaves))
         hpr_fit_kernel<<<g, kFitWaves * kWave, 0, s>>>(nodes, op.pptr.get(), op.pidx.get(), op.slot_node.get(), ns,

@@ -572,10 +572,10 @@ class SolverContext:
 
     def setup_times(self) -> dict:
         """Host wall time (ms) of the context build's phases."""
-        ms = np.zeros(7)
+        ms = np.zeros(10)
         check(lib().cn_context_setup_times(self.handle, ptr(ms)), "cn_context_setup_times")
-        names = ("upload_order", "gradients_checks", "pattern", "stiffness_grounding", "hierarchy",
-                 "solver_ell", "skin_region")
+        names = ("checks", "upload", "order", "gradients_checks", "pattern", "stiffness_grounding", "hierarchy",
+                 "solver_ell", "skin_region", "last_recovery_build")
         return {k: round(float(v), 3) for k, v in zip(names, ms)}
 
     def debug_fail_captures(self, n: int) -> None:

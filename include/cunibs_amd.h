@@ -121,11 +121,12 @@ typedef struct {
 
 int32_t cn_context_info(const cn_context* ctx, cn_info* out);
 /* Host wall time (ms) of the context build's phases, each ending at a wait on the library's
- * stream: ms7[0] argument checks, upload and the Morton order; [1] gradients, volume and
- * conductivity checks; [2] node2corner and the stiffness pattern; [3] stiffness values,
- * grounding, resident arrays; [4] AMG hierarchy; [5] differenced operator, solver and ELL copies;
- * [6] skin surface and region slice. A measurement aid with no counterpart in the reference. */
-int32_t cn_context_setup_times(const cn_context* ctx, double* ms7);
+ * stream: ms10[0] argument checks; [1] uploads; [2] the Morton order; [3] gradients, volume and
+ * conductivity checks; [4] node2corner and the stiffness pattern; [5] stiffness values,
+ * grounding, resident arrays; [6] AMG hierarchy; [7] differenced operator, solver and ELL copies;
+ * [8] skin surface and region slice; [9] the most recent recovery operator build (0 before the
+ * first). A measurement aid with no counterpart in the reference. */
+int32_t cn_context_setup_times(const cn_context* ctx, double* ms10);
 
 /* The reordered mesh every result is indexed by (fem/solve.py:645-662), and the permutations
  * from the input order. Any pointer may be NULL. */
