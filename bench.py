@@ -227,8 +227,8 @@ def cpu_host() -> dict:
 def config3_leg(pos, mom, tuning: dict, block_k: int, recovery: str) -> dict:
     """The 256-placement sweep spread over three perturbed heads (2.0 / 1.5 / 1.25 mm grids, 86 / 85
     / 85 placements of the 16 x 16 cap grid, each head's share taking every third grid site, so
-    every head ends on a partial block): per head the mesh size, the context setup and the
-    pipelined sweep at the tuning found on the 1.5 mm head; the whole sweep's placements/s with
+    every head ends on a partial block): per head the mesh size, the context setup (with the
+    recovery operator) and the pipelined sweep at the tuning found on the 1.5 mm head; the whole sweep's placements/s with
     and without the three setups."""
     from cunibs_amd import fem, synthetic
 
@@ -240,6 +240,8 @@ def config3_leg(pos, mom, tuning: dict, block_k: int, recovery: str) -> dict:
         t0 = time.perf_counter()
         ctx = fem.SolverContext(head.nodes_mm, head.tet_nodes, head.tet_tags, head.skin_tris)
         ctx.set_tuning(**tuning)
+        if recovery != "raw":
+            ctx.recovery_build(recovery)
         setup = time.perf_counter() - t0
         fem.solve_sweep(ctx, pos, mom, sites[:block_k], 1e6, block_k, recovery=recovery)  # captures
         t0 = time.perf_counter()
@@ -393,6 +395,7 @@ def main() -> int:
     t0 = time.time()
     ctx = fem.SolverContext(head.nodes_mm, head.tet_nodes, head.tet_tags, head.skin_tris)
     setup_s = time.time() - t0
+    setup_phases = ctx.setup_times()
     log(f"context {setup_s:.2f}s hierarchy {ctx.hierarchy_sizes}")
     k = args.block_k
     pos, mom = coil.positions_m, coil.moments
@@ -402,6 +405,9 @@ def main() -> int:
         return kernels_only(ctx, coil, sites, k)
     if args.sweep_only:
         return sweep_only(ctx, coil, sites, args)
+    t0 = time.time()
+    ctx.recovery_build("harmonic")
+    harmonic_build_s = time.time() - t0
 
     def block(i0: int, kk: int, warm: bool = False, recovery: str = "raw"):
         chunk = [sites[(i0 + j) % len(sites)] for j in range(kk)]
@@ -593,6 +599,8 @@ def main() -> int:
         "speedup_vs_plain": round(value / plain, 3),
         "plain_iterations_mean": round(float(np.mean(plain_iters)), 2),
         "setup_s": round(setup_s, 3),
+        "setup_phases_ms": setup_phases,
+        "harmonic_build_s": round(harmonic_build_s, 3),
         "block_phases_ms": block_phases,
         "autotune": tune.as_dict() if tune else None,
         "overlap": ({"coil_blocks": overlap["coil_blocks"],

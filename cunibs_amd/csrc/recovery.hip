@@ -419,12 +419,13 @@ constexpr int min_patch() {
 
 // One rung of the fit for one slot, one wavefront. Normal-matrix entries (lower triangle) are
 // dealt round the lanes and summed over the patch in order from an LDS-staged chunk of basis
-// values; lane 0 factors and solves for rows 1..3 of the inverse; the lanes then write the
-// weights of their patch entries and the amplification sums are folded by a fixed xor tree.
+// values; the lanes factor the matrix together and three of them solve for rows 1..3 of the
+// inverse; the lanes then write the weights of their patch entries and the amplification sums are
+// folded by a fixed xor tree.
 template <int N>
 __device__ bool fit_rung(const double* __restrict__ nodes, const std::int32_t* __restrict__ pidx, int begin, int end,
                          const double (&xn)[3], double h, float* __restrict__ w3, double* q, double* a, double* z,
-                         int* flag, int lane) {
+                         int lane) {
     if (end - begin < min_patch<N>()) return false;
     constexpr int kEntries = N * (N + 1) / 2;
     constexpr int kPer = (kEntries + kWave - 1) / kWave;
@@ -458,43 +459,48 @@ __device__ bool fit_rung(const double* __restrict__ nodes, const std::int32_t* _
     for (int e = 0; e < kPer; ++e)
         if (lane + kWave * e < kEntries) a[ri[e] * N + ci[e]] = acc[e];
     wave_sync();
-    if (lane == 0) {
-        int ok = 1;
-        const double floor_pivot = 1e-12 * a[0];
-        for (int j = 0; j < N && ok; ++j) {
-            double d = a[j * N + j];
-            for (int k = 0; k < j; ++k) d = fma(-a[j * N + k], a[j * N + k], d);
-            if (!(d > floor_pivot)) {
-                ok = 0;
-                break;
-            }
-            const double ljj = sqrt(d);
-            a[j * N + j] = ljj;
-            for (int i = j + 1; i < N; ++i) {
-                double v = a[i * N + j];
-                for (int k = 0; k < j; ++k) v = fma(-a[i * N + k], a[j * N + k], v);
-                a[i * N + j] = v / ljj;
-            }
+    // Cholesky factor L (lower triangle of a), right-looking across the lanes: step j takes the
+    // pivot, scales column j, then folds column j into the trailing triangle. Entry (i, k) thus
+    // receives fma(-l_ij, l_kj, .) for j = 0, 1, ... in order and is divided by its pivot last,
+    // the operations of the serial left-looking factorization in the same order. A pivot not above
+    // 1e-12 A_00 rejects the rung.
+    const double floor_pivot = 1e-12 * a[0];
+    for (int j = 0; j < N; ++j) {
+        const double d = a[j * N + j];
+        if (!(d > floor_pivot)) return false;  // wave-uniform: every lane read the same pivot
+        const double ljj = sqrt(d);
+        wave_sync();
+        if (lane == 0) a[j * N + j] = ljj;
+        if (lane > j && lane < N) a[lane * N + j] = a[lane * N + j] / ljj;
+        wave_sync();
+        const int t = N - 1 - j;  // trailing rows j+1 .. N-1: t (t + 1) / 2 entries
+        for (int q = lane; q < t * (t + 1) / 2; q += kWave) {
+            int r = 0;
+            while ((r + 1) * (r + 2) / 2 <= q) ++r;
+            const int i = j + 1 + r, k = j + 1 + (q - r * (r + 1) / 2);
+            a[i * N + k] = fma(-a[i * N + j], a[k * N + j], a[i * N + k]);
         }
-        if (ok) {
-            for (int c = 0; c < 3; ++c) {
-                double y[N];
-                for (int i = 0; i < N; ++i) {
-                    double v = (i == c + 1) ? 1.0 : 0.0;
-                    for (int k = 0; k < i; ++k) v = fma(-a[i * N + k], y[k], v);
-                    y[i] = v / a[i * N + i];
-                }
-                for (int i = N - 1; i >= 0; --i) {
-                    double v = y[i];
-                    for (int k = i + 1; k < N; ++k) v = fma(-a[k * N + i], z[c * N + k], v);
-                    z[c * N + i] = v / a[i * N + i];
-                }
-            }
+        wave_sync();
+    }
+    // Rows 1..3 of the inverse (z_c = A^-1 e_{c+1}): lane c solves L y = e_{c+1}, then L^T z = y
+    // in place in its row of z.
+    if (lane < 3) {
+        const int c = lane;
+        double* zc = z + c * N;
+#pragma unroll 1
+        for (int i = 0; i < N; ++i) {
+            double v = (i == c + 1) ? 1.0 : 0.0;
+            for (int k = 0; k < i; ++k) v = fma(-a[i * N + k], zc[k], v);
+            zc[i] = v / a[i * N + i];
         }
-        *flag = ok;
+#pragma unroll 1
+        for (int i = N - 1; i >= 0; --i) {
+            double v = zc[i];
+            for (int k = i + 1; k < N; ++k) v = fma(-a[k * N + i], zc[k], v);
+            zc[i] = v / a[i * N + i];
+        }
     }
     wave_sync();
-    if (!*flag) return false;
     double sum[3] = {0.0, 0.0, 0.0};
     for (int p = begin + lane; p < end; p += kWave) {
         const std::int64_t mm = pidx[p];
@@ -516,14 +522,15 @@ __device__ bool fit_rung(const double* __restrict__ nodes, const std::int32_t* _
     return fmax(sum[0], fmax(sum[1], sum[2])) <= kMaxGradientAmplification;
 }
 
-__global__ void __launch_bounds__(kFitWaves * kWave) hpr_fit_kernel(
+// Three waves per SIMD: the LDS of three blocks fills a CU, and the register budget that leaves
+// holds the kernel without spills.
+__global__ void __launch_bounds__(kFitWaves * kWave) __attribute__((amdgpu_waves_per_eu(3))) hpr_fit_kernel(
     const double* __restrict__ nodes, const std::int32_t* __restrict__ pptr, const std::int32_t* __restrict__ pidx,
     const std::int32_t* __restrict__ slot_node, std::int64_t n_slots, float* __restrict__ w3,
     std::int32_t* __restrict__ status) {
     __shared__ double q_s[kFitWaves][kFitChunk * kMaxTerms];
     __shared__ double a_s[kFitWaves][kMaxTerms * kMaxTerms];
     __shared__ double z_s[kFitWaves][3 * kMaxTerms];
-    __shared__ int flag_s[kFitWaves];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
     const std::int64_t s = static_cast<std::int64_t>(blockIdx.x) * kFitWaves + wave;
     if (s >= n_slots) return;  // wave-uniform: no block barrier follows
@@ -543,12 +550,11 @@ __global__ void __launch_bounds__(kFitWaves * kWave) hpr_fit_kernel(
         double* q = q_s[wave];
         double* a = a_s[wave];
         double* z = z_s[wave];
-        int* flag = &flag_s[wave];
-        if (fit_rung<16>(nodes, pidx, begin, end, xn, h, w3, q, a, z, flag, lane))
+        if (fit_rung<16>(nodes, pidx, begin, end, xn, h, w3, q, a, z, lane))
             rung = 0;
-        else if (fit_rung<9>(nodes, pidx, begin, end, xn, h, w3, q, a, z, flag, lane))
+        else if (fit_rung<9>(nodes, pidx, begin, end, xn, h, w3, q, a, z, lane))
             rung = 1;
-        else if (fit_rung<4>(nodes, pidx, begin, end, xn, h, w3, q, a, z, flag, lane))
+        else if (fit_rung<4>(nodes, pidx, begin, end, xn, h, w3, q, a, z, lane))
             rung = 2;
     }
     if (rung == 3) {
