@@ -140,12 +140,17 @@ def sweep_only(ctx, coil, sites, args) -> int:
     return 0
 
 
-def learned_leg(ctx, pos, mom, n_train: int, n_test: int, log_sd: float = 0.2) -> dict:
+def learned_leg(ctx, pos, mom, n_train: int, n_test: int, log_sd: float = 0.2, n_diag: int = 32) -> dict:
     """The learned warm start on poses and conductivities (distance U[2, 8] mm, the five tissues of
     the head log-normal around the table): n_train solver-generated solutions, each with its own
     conductivities, form the snapshot basis and the training set; the network maps pose and log
     conductivities to the solution's coefficients in the 256 leading A-orthonormal modes; on
-    n_test held-out placements it warm-starts PCG, which iterates to the same tolerance."""
+    n_test held-out placements it warm-starts PCG, which iterates to the same tolerance. Two
+    networks are trained, on energy-scaled and on per-mode standardized targets. For the first
+    n_diag held-out placements the energy-norm error ||x - x0||_A / ||x||_A of each initial guess
+    is measured exactly (x^T b from the solution and right-hand side, Phi^T b from the device):
+    the Galerkin guess is the best the modes allow, and a network predicting Galerkin
+    coefficients cannot do better."""
     from cunibs_amd import fem, learned, synthetic, uq
 
     tags = [1, 2, 3, 4, 5]
@@ -166,46 +171,85 @@ def learned_leg(ctx, pos, mom, n_train: int, n_test: int, log_sd: float = 0.2) -
     rank = ctx.warm_finalize(max_modes=256)
     t_basis = time.perf_counter() - t0 - t_solve
     feats = lambda sites, rows: np.stack([learned.pose_features(s, r[1:11]) for s, r in zip(sites, rows)])
-    net = learned.WarmStartNetwork(n_out=(rank + 15) // 16 * 16, hidden=512, seed=0)
-    T = np.zeros((n_train, net.n_out), np.float32)
+    T = np.zeros((n_train, (rank + 15) // 16 * 16), np.float32)
     T[:, :rank] = ctx.warm_targets()[:, :rank]
-    t1 = time.perf_counter()
-    loss = net.fit(feats(tr_sites, tr_rows), T, epochs=300)
-    net.modes = rank
-    t_train = time.perf_counter() - t1
+    X_tr = feats(tr_sites, tr_rows)
+    nets, fits = {}, {}
+    for scale in ("energy", "per_output"):
+        net = learned.WarmStartNetwork(n_out=T.shape[1], hidden=512, seed=0)
+        t1 = time.perf_counter()
+        loss = net.fit(X_tr, T, epochs=300, target_scale=scale)
+        net.modes = rank
+        pred = net.predict(X_tr)[:, :rank]
+        fits[scale] = {"train_loss": round(loss, 5), "network_train_s": round(time.perf_counter() - t1, 2),
+                       "train_coef_rel_err": round(float(np.linalg.norm(pred - T[:, :rank])
+                                                         / np.linalg.norm(T[:, :rank])), 4)}
+        nets[scale] = net
 
     te_sites, te_rows = draws(n_test, 37)
     X = feats(te_sites, te_rows)
+    preds = {s: nets[s].predict(X)[:, :rank].astype(np.float64) for s in nets}
+
+    def block(mode, i, **kw):
+        s, r = te_sites[i:i + 16], te_rows[i:i + 16]
+        if mode.startswith("learned"):
+            ctx.warm_set_coefficients(preds["per_output" if mode == "learned_per_output" else "energy"][i:i + 16])
+        return fem.solve_placements_block(ctx, pos, mom, s, 1e6, metrics=True, magnitude=True, sigma=r,
+                                          warm_start={"cold": 0, "galerkin": 1}.get(mode, 2), **kw)
 
     def run(mode):
-        its, block_its, mags = [], [], []
+        """Timed: the held-out placements in blocks of 16 (after one untimed block that captures the
+        mode's graphs). Untimed: the guess errors of the first n_diag placements."""
+        block(mode, 0)
+        its, block_its, mags, errs = [], [], [], []
         t = time.perf_counter()
         for i in range(0, n_test, 16):
-            s, r = te_sites[i:i + 16], te_rows[i:i + 16]
-            if mode == "learned":
-                c = net.predict(X[i:i + 16])[:, :rank].astype(np.float64)
-                ctx.warm_set_coefficients(c)
-            res = fem.solve_placements_block(ctx, pos, mom, s, 1e6, metrics=True, magnitude=True, sigma=r,
-                                             warm_start={"cold": 0, "galerkin": 1, "learned": 2}[mode])
+            res = block(mode, i)
             its.extend(x.iterations for x in res)
             block_its.append(max(x.iterations for x in res))
             mags.extend(x.magnE for x in res[:2])
-        return n_test / (time.perf_counter() - t), its, block_its, mags
+        rate = n_test / (time.perf_counter() - t)
+        keep = np.ones(ctx.n_nodes, bool)
+        keep[ctx.ground_node] = False
+        for i in range(0, n_diag if mode != "cold" else 0, 16):
+            extra = {"rhs": 1}
+            res = block(mode, i, potential=True, extra=extra)
+            energy = np.array([float(x.v[keep] @ b[keep].astype(np.float64)) for x, b in zip(res, extra["rhs"])])
+            coef = None if mode == "galerkin" else \
+                preds["per_output" if mode == "learned_per_output" else "energy"][i:i + len(res)]
+            errs.extend(ctx.warm_guess_errors(len(res), energy, coef).tolist())
+        return rate, its, block_its, mags, errs
+
+    # Galerkin coefficients P Phi^T b of every held-out placement (untimed), against which the
+    # networks' held-out predictions are scored.
+    c_gal = []
+    for i in range(0, n_test, 16):
+        block("galerkin", i)
+        c_gal.append(ctx.warm_project_last(len(te_sites[i:i + 16])) @ ctx.warm_state["P"].T)
+    c_gal = np.concatenate(c_gal)
+    for scale in nets:
+        fits[scale]["test_coef_rel_err"] = round(float(np.linalg.norm(preds[scale] - c_gal) / np.linalg.norm(c_gal)), 4)
 
     out = {}
     ref = None
-    for mode in ("cold", "learned", "galerkin"):
-        rate, its, bits, mags = run(mode)
+    for mode in ("cold", "galerkin", "learned", "learned_per_output"):
+        rate, its, bits, mags, errs = run(mode)
         out[mode] = {"placements_per_s": round(rate, 2), "iterations_mean": round(float(np.mean(its)), 2),
                      "block_iterations_mean": round(float(np.mean(bits)), 2)}
+        if errs:
+            out[mode]["guess_energy_err_median"] = round(float(np.median(errs)), 4)
+            out[mode]["guess_energy_err_max"] = round(float(np.max(errs)), 4)
+        if mode.startswith("learned"):
+            out[mode].update(fits["per_output" if mode == "learned_per_output" else "energy"])
         if ref is None:
             ref = mags
         else:
             out[mode]["magnE_rel_diff_vs_cold"] = max(
                 float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(mags, ref))
-    out.update({"train_solves": n_train, "test_placements": n_test, "modes": rank, "train_loss": round(loss, 5),
-                "solve_s": round(t_solve, 2), "basis_s": round(t_basis, 2), "network_train_s": round(t_train, 2)})
-    net.free()
+    out.update({"train_solves": n_train, "test_placements": n_test, "modes": rank, "diagnosed_placements": n_diag,
+                "solve_s": round(t_solve, 2), "basis_s": round(t_basis, 2)})
+    for net in nets.values():
+        net.free()
     return out
 
 
@@ -361,7 +405,7 @@ def main() -> int:
     ap.add_argument("--autotune", type=int, default=1,
                     help="1: pick block width (4/8/16), SpMM tile and Chebyshev degree (2-6) on 32 training "
                          "placements before the sweeps; 0: use --block-k, the automatic tile and degree 2")
-    ap.add_argument("--learned", type=int, default=512,
+    ap.add_argument("--learned", type=int, default=2048,
                     help="training solves of the learned warm start on random poses and conductivities, "
                          "evaluated on 256 held-out placements (0: skip)")
     ap.add_argument("--config3", type=int, default=1, help="1: the 256-placement sweep over three heads")

@@ -105,6 +105,7 @@ _SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "cn_warm_mode_grams": (_I32, [_P, _P, _P]),
     "cn_warm_set_projector": (_I32, [_P, _P, _I32]),
     "cn_warm_set_coefficients": (_I32, [_P, _P, _I32, _I32]),
+    "cn_warm_project_last": (_I32, [_P, _I32, _P]),
     "cn_warm_modes": (_I32, [_P, _P, _P]),
     "cn_uq_prepare": (_I32, [_P, _P, _P, _I64, _P, _P, _D, _D, _P, _P]),
     "cn_uq_solve": (_I32, [_P, _P, _I32, _P, _P]),

@@ -2111,13 +2111,16 @@ int32_t cn_warm_gram(cn_context* c, double* G) {
         }
         cn::DeviceBuffer<float> aw(static_cast<std::size_t>(nr) * 32, "A W chunk");
         std::vector<double> tile(32 * 32);
+        // G is symmetric: tiles i <= j are computed, the others mirrored.
         for (int j = 0; j < chunks; ++j) {
             c->solver->spmm_diff(w[j].get(), aw.get(), 32);
-            for (int i = 0; i < chunks; ++i) {
+            for (int i = 0; i <= j; ++i) {
                 c->solver->cross_gram32(w[i].get(), aw.get(), tile.data());
                 for (int a = 0; a < 32 && 32 * i + a < S; ++a)
-                    for (int b = 0; b < 32 && 32 * j + b < S; ++b)
+                    for (int b = 0; b < 32 && 32 * j + b < S; ++b) {
                         G[static_cast<std::int64_t>(32 * i + a) * S + 32 * j + b] = tile[a * 32 + b];
+                        G[static_cast<std::int64_t>(32 * j + b) * S + 32 * i + a] = tile[a * 32 + b];
+                    }
             }
         }
     });
@@ -2185,6 +2188,18 @@ int32_t cn_warm_set_coefficients(cn_context* c, const double* coef, int32_t k, i
         cn::require(k >= 1 && k <= cn::kMaxBlock, "k must be in [1, 32]");
         c->warm_coef.assign(coef, coef + static_cast<std::size_t>(k) * R);
         c->warm_coef_k = k;
+    });
+}
+
+int32_t cn_warm_project_last(cn_context* c, int32_t k, double* out) {
+    return guarded([&] {
+        cn::require(c != nullptr && out != nullptr, "null argument");
+        cn::require(c->warm.ready, "warm-start modes not set");
+        cn::require(k >= 1 && k <= cn::kMaxBlock, "k must be in [1, 32]");
+        const int K = cn::padded_width(k);
+        cn::require(c->d_B.size() >= static_cast<std::size_t>((c->n - 1) * K),
+                    "no block of that width has been solved");
+        c->warm.project_host(c->d_B.get(), k, K, out, c->s);
     });
 }
 

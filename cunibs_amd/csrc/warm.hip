@@ -307,7 +307,7 @@ __global__ void __launch_bounds__(256) modes_kernel(const float* __restrict__ ba
 }  // namespace
 
 void WarmBasis::reset(std::int64_t n_rows, int capacity) {
-    require(capacity >= 1 && capacity <= kMaxWarmBasis, "warm-start capacity must be in [1, 512]");
+    require(capacity >= 1 && capacity <= kMaxWarmBasis, "warm-start capacity must be in [1, 2048]");
     n = n_rows;
     cap = capacity;
     count = 0;
@@ -422,6 +422,25 @@ void WarmBasis::coefficients(int k, int K, hipStream_t s) {
     coef.ensure(static_cast<std::size_t>(2) * R * 16, "warm coefficients");
     sum_blocks_kernel<<<(R * K + 3) / 4, 256, 0, s>>>(partials.get(), nb, R * K, ptb);
     coefficients_kernel<<<(R * 16 + 255) / 256, 256, 0, s>>>(ptb, R, K, P.get(), k, coef.get());
+}
+
+void WarmBasis::project_host(const float* B, int k, int K, double* out, hipStream_t s) {
+    require(K >= 1 && K <= 32 && k >= 1 && k <= K, "k must be in [1, K], K at most 32");
+    const int R = n_modes;
+    const int nb = kProjectBlocks;
+    std::vector<double> h;
+    for (int c0 = 0; c0 < k; c0 += 16) {
+        const int Kh = std::min(16, K - c0);
+        project(B, Kh, s, K, c0);
+        double* const ptb = partials.get() + static_cast<std::size_t>(nb) * R * Kh;
+        sum_blocks_kernel<<<(R * Kh + 3) / 4, 256, 0, s>>>(partials.get(), nb, R * Kh, ptb);
+        h.resize(static_cast<std::size_t>(R) * Kh);
+        d2h(h.data(), ptb, h.size(), s);
+        sync(s, "projection readback");
+        for (int c = 0; c < Kh && c0 + c < k; ++c)
+            for (int j = 0; j < used_modes; ++j)
+                out[static_cast<std::size_t>(c0 + c) * used_modes + j] = h[static_cast<std::size_t>(j) * Kh + c];
+    }
 }
 
 void WarmBasis::initial_guess(const float* B, int k, int K, double* x0, hipStream_t s) {

@@ -6,7 +6,7 @@
 
 namespace cn {
 
-constexpr int kMaxWarmBasis = 512;
+constexpr int kMaxWarmBasis = 2048;
 constexpr int kMaxWarmModes = 256;
 constexpr int kProjectBlocks = 512;
 
@@ -40,6 +40,9 @@ struct WarmBasis {
     // Columns c0 .. c0 + K - 1 (K <= 16) of B, rows ldb apart (0: K).
     void project(const float* B, int K, hipStream_t s, int ldb = 0, int c0 = 0);
     void coefficients(int k, int K, hipStream_t s);
+    // Phi^T B for the first k columns of the (n, K) FP32 block B (K <= 32), into host out (k x R,
+    // row-major), before the projector P.
+    void project_host(const float* B, int k, int K, double* out, hipStream_t s);
     // Average ms of the three stages of initial_guess over `reps` timed runs (HIP events).
     void bench(const float* B, int k, int K, double* x0, int reps, double* ms3, hipStream_t s);
     // x0 = Phi c for caller-supplied coefficients c (k x R) FP64 host.

@@ -363,6 +363,25 @@ class SolverContext:
         check(lib().cn_warm_set_coefficients(self.handle, ptr(c), c.shape[0], c.shape[1]),
               "cn_warm_set_coefficients")
 
+    def warm_project_last(self, k: int) -> np.ndarray:
+        """Phi^T b for the first k placements of the most recent block solve, (k, R)."""
+        r = self.warm_state["modes"]
+        out = np.empty((k, r), np.float64)
+        check(lib().cn_warm_project_last(self.handle, int(k), ptr(out)), "cn_warm_project_last")
+        return out
+
+    def warm_guess_errors(self, k: int, energy: np.ndarray, coefficients: np.ndarray | None = None) -> np.ndarray:
+        """Energy-norm relative error ||x - x0||_A / ||x||_A of the initial guesses of the most
+        recent block's first k placements, given ||x||_A^2 = x^T b per placement: the Galerkin
+        guess Phi P Phi^T b, or Phi c for supplied coefficients c (k, R). With g = Phi^T b and
+        G = Phi^T A Phi, ||x - Phi c||_A^2 = x^T b - 2 c^T g + c^T G c."""
+        st = self.warm_state
+        g = self.warm_project_last(k)
+        c = g @ st["P"].T if coefficients is None else np.asarray(coefficients, np.float64)
+        gm = 0.5 * (st["G_modes"] + st["G_modes"].T)
+        err2 = energy - 2.0 * np.einsum("ij,ij->i", c, g) + np.einsum("ij,jk,ik->i", c, gm, c)
+        return np.sqrt(np.maximum(err2, 0.0) / energy)
+
     def warm_modes(self) -> np.ndarray:
         """The modes (n_reduced, width): hi + lo of their BF16 pairs, in FP64."""
         w = ctypes.c_int32()

@@ -91,15 +91,25 @@ class WarmStartNetwork:
         return np.ascontiguousarray((np.asarray(X, np.float32) - self.mean) / self.scale, np.float32)
 
     def fit(self, X: np.ndarray, T: np.ndarray, epochs: int = 200, batch: int = 256, lr: float = 1e-3,
-            seed: int = 0) -> float:
-        """Standardize inputs and targets on the training set, then train with Adam."""
+            seed: int = 0, target_scale: str = "energy") -> float:
+        """Standardize inputs on the training set, scale the targets, then train with Adam.
+        target_scale "energy" divides every target by one number (their RMS), so the squared loss
+        is the energy-norm error of the expansion in A-orthonormal modes over the mean solution
+        energy; "per_output" standardizes each coefficient, weighting the trailing modes, which
+        carry little of the solution, as much as the leading ones."""
+        if target_scale not in ("energy", "per_output"):
+            raise ValueError("target_scale must be 'energy' or 'per_output'")
         X = np.asarray(X, np.float32)
         T = np.asarray(T, np.float32)
         self.mean = X.mean(axis=0)
         sd = X.std(axis=0)
         self.scale = np.where(sd > 0, sd, 1.0).astype(np.float32)
-        osd = T.std(axis=0)
-        self.out_scale = np.where(osd > 0, osd, 1.0).astype(np.float32)
+        if target_scale == "per_output":
+            osd = T.std(axis=0)
+            self.out_scale = np.where(osd > 0, osd, 1.0).astype(np.float32)
+        else:
+            rms = float(np.sqrt(np.mean(np.square(T, dtype=np.float64))))
+            self.out_scale = np.full(T.shape[1], rms if rms > 0 else 1.0, np.float32)
         xs = self._inputs(X)
         ts = np.ascontiguousarray(T / self.out_scale, np.float32)
         loss = ctypes.c_double()

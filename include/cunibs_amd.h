@@ -224,7 +224,7 @@ int32_t cn_recovery_patches(const cn_context* ctx, int64_t* nnz, int32_t* n_unde
 
 /* Reduced-basis warm start. No reference counterpart (its nearest analogue is the Galerkin
  * initial guess of uq/conductivity/run.py:154-200). Solves with opt->collect = 1 append their
- * solutions (FP32) to a snapshot basis W of up to 512 columns. cn_warm_gram returns
+ * solutions (FP32) to a snapshot basis W of up to 2048 columns. cn_warm_gram returns
  * G = W^T A W (count x count); the caller hands back M = V Lambda^-1/2 of its leading R <= 256
  * eigenpairs, and cn_warm_set_modes forms the modes Phi = W M (BF16 MFMA GEMM). cn_warm_mode_grams
  * returns Phi^T A Phi (R x R) and Phi^T A W (R x count); cn_warm_set_projector takes
@@ -238,6 +238,10 @@ int32_t cn_warm_set_modes(cn_context* ctx, const double* M, int32_t count, int32
 int32_t cn_warm_mode_grams(cn_context* ctx, double* G_modes, double* G_cross);
 int32_t cn_warm_set_projector(cn_context* ctx, const double* P, int32_t R);
 int32_t cn_warm_set_coefficients(cn_context* ctx, const double* c, int32_t k, int32_t R);
+/* Phi^T b for the first k columns of the right-hand side of the most recent block solve (k x R
+ * FP64, row-major): with P and Phi^T A Phi it gives the energy-norm error of each warm start's
+ * initial guess. A measurement aid with no counterpart in the reference. */
+int32_t cn_warm_project_last(cn_context* ctx, int32_t k, double* out);
 /* The modes as BF16 hi + lo pairs: out holds (2, n_reduced, width) BF16 bits, the high halves
  * then the low halves, each row-major; width (a multiple of 16 >= R) is written to *width; call
  * with out = NULL to read it. A mode value is hi + lo. */

@@ -99,3 +99,29 @@ def test_guess_stage_timing(trained, coil):
     after = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites, 1e6, warm_start=True)
     for a, b in zip(before, after):
         np.testing.assert_array_equal(a.magnE, b.magnE)
+
+
+def test_guess_energy_errors_match_a_direct_evaluation(trained, coil):
+    """cn_warm_project_last gives Phi^T b of the last block; from it, P and Phi^T A Phi the
+    energy-norm error of the Galerkin guess and of supplied coefficients follows without the
+    guess itself. Checked against ||x - x0||_A / ||x||_A evaluated on the host with the FP64
+    reduced operator, the solution and the device's own x0."""
+    import scipy.sparse as sp
+    from cunibs_amd import fem, synthetic
+    ctx, _, _, rank = trained
+    keep = np.arange(ctx.n_nodes) != ctx.ground_node
+    rp, rc, rv = ctx.reduced()
+    a = sp.csr_matrix((rv, rc, rp), shape=(ctx.n_nodes - 1,) * 2)
+    sites = synthetic.cap_sites(6, seed=15)
+    for mode, coef in (("galerkin", None), ("supplied", np.random.default_rng(3).normal(size=(6, rank)))):
+        if coef is not None:
+            ctx.warm_set_coefficients(coef)
+        extra = {"x0": 1, "rhs": 1}
+        res = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites, 1e6, potential=True,
+                                         warm_start=1 if coef is None else 2, extra=extra)
+        energy = np.array([r.v[keep] @ b[keep].astype(np.float64) for r, b in zip(res, extra["rhs"])])
+        got = ctx.warm_guess_errors(6, energy, coef)
+        for i, r in enumerate(res):
+            d = r.v[keep] - extra["x0"][i]
+            want = np.sqrt(d @ (a @ d) / (r.v[keep] @ (a @ r.v[keep])))
+            assert abs(got[i] - want) <= 2e-3 * want + 1e-4, (mode, i, got[i], want)
