@@ -89,6 +89,9 @@ def kernels_only(ctx, coil, sites, k: int) -> int:
     ctx.set_tuning(layout=0)
     gram_ms = ctx.bench_spmm(K, 20, mode=2)
     vc_ms = ctx.bench_vcycle(K, 10)
+    ctx.set_tuning(dense_tail=0)
+    vc_lbl_ms = ctx.bench_vcycle(K, 10)
+    ctx.set_tuning(dense_tail=1)
     t0 = time.perf_counter()
     ctx.recovery_build("harmonic")
     ctx.recovery_build("spr_global")
@@ -102,7 +105,8 @@ def kernels_only(ctx, coil, sites, k: int) -> int:
         "spmm_K": K, "n_reduced": n_red, "nnz_reduced": nnz,
         "spmm_algorithmic_bytes": spmm_diff_bytes(n_red, nnz, K), "spmm_us": round(spmm_ms * 1e3, 2),
         "spmm_csr_us": round(spmm_csr_ms * 1e3, 2), "spmm_with_gram_us": round(gram_ms * 1e3, 2),
-        "vcycle_us": round(vc_ms * 1e3, 2), "vcycle_algorithmic_bytes": vcycle_bytes(ctx.info, K, ctx.tuning["degree"]),
+        "vcycle_us": round(vc_ms * 1e3, 2), "vcycle_level_by_level_us": round(vc_lbl_ms * 1e3, 2),
+        "dense_tail_level": int(ctx.info.dense_tail_level), "vcycle_algorithmic_bytes": vcycle_bytes(ctx.info, K, ctx.tuning["degree"]),
         "recovery_build_s": round(rec_s, 3), "dadt_ms_per_8": round(dadt_ms, 3),
         "stages_harmonic_ms": stages_h, "stages_raw_ms": stages_raw,
     }), flush=True)
@@ -251,6 +255,13 @@ def learned_leg(ctx, pos, mom, n_train: int, n_test: int, log_sd: float = 0.2, n
     for net in nets.values():
         net.free()
     return out
+
+
+def spmm_layout(ctx, K: int) -> str:
+    """The form of the finest differenced operator the block iteration's SpMM reads at width K."""
+    layout = ctx.tuning.get("layout", 0)
+    ell = ctx.info.ell_width_off > 0 and (layout == 2 or (layout == 0 and K <= 4))
+    return "sliced ELL" if ell else "CSR"
 
 
 def cpu_host() -> dict:
@@ -661,7 +672,7 @@ def main() -> int:
             "avg_us": round(spmm_ms * 1e3, 2),
             "samples": timing["samples"],
             "algorithmic_bytes": spmm_diff_bytes(n_red, nnz, K),
-            "layout": "sliced ELL" if ctx.tuning.get("layout", 0) == 0 and ctx.info.ell_width_off else "CSR",
+            "layout": spmm_layout(ctx, K),
             "note": "timed inside the headline's pipelined cold sweep, sharing the device with the next block's dA/dt",
         },
         "roofline_unshared": {

@@ -51,13 +51,13 @@ class Info(ctypes.Structure):
         ("level_rows", _I64 * (MAX_LEVELS + 1)), ("level_nnz", _I64 * (MAX_LEVELS + 1)),
         ("n_region", _I64), ("region_volume_m3", _D), ("n_cu", _I32),
         ("level_p_nnz", _I64 * MAX_LEVELS), ("ell_width_full", _I32), ("ell_width_off", _I32),
-        ("preconditioner_rebuilds", _I32), ("matched_preconditioners", _I32),
+        ("preconditioner_rebuilds", _I32), ("matched_preconditioners", _I32), ("dense_tail_level", _I32),
     ]
 
 
 class Tuning(ctypes.Structure):
     _fields_ = [("degree", _I32), ("lower_ratio", _F), ("fine_sub", _I32), ("coarse_sub", _I32),
-                ("coil_blocks", _I32), ("layout", _I32)]
+                ("coil_blocks", _I32), ("layout", _I32), ("dense_tail", _I32)]
 
 
 class SolveOptions(ctypes.Structure):

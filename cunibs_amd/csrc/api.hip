@@ -1506,6 +1506,7 @@ int32_t cn_context_get_tuning(const cn_context* c, cn_tuning* o) {
         o->coarse_sub = c->solver->coarse_sub_setting();
         o->coil_blocks = c->coil_blocks;
         o->layout = c->layout;
+        o->dense_tail = c->solver->dense_tail_setting() ? 1 : 0;
     });
 }
 
@@ -1520,6 +1521,7 @@ int32_t cn_context_set_tuning(cn_context* c, const cn_tuning* t) {
                     "coarse_sub must be 0 or a power of two <= 64");
         cn::require(t->coil_blocks >= 0, "coil_blocks must be >= 0");
         cn::require(t->layout >= 0 && t->layout <= 2, "layout must be 0 (automatic), 1 (CSR) or 2 (sliced ELL)");
+        cn::require(t->dense_tail == 0 || t->dense_tail == 1, "dense_tail must be 0 or 1");
         cn::check(hipStreamSynchronize(c->s), "sync before retuning");
         c->coil_blocks = t->coil_blocks;
         c->set_layout(t->layout);
@@ -1528,6 +1530,7 @@ int32_t cn_context_set_tuning(cn_context* c, const cn_tuning* t) {
         c->opts.lower_ratio = t->lower_ratio;
         c->solver->set_fine_sub(t->fine_sub);
         c->solver->set_coarse_sub(t->coarse_sub);
+        c->solver->set_dense_tail(t->dense_tail != 0);
         c->solver->invalidate();
     });
 }
@@ -1557,6 +1560,15 @@ int32_t cn_context_info(const cn_context* c, cn_info* o) {
         o->ell_width_off = c->ell_off_w;
         o->preconditioner_rebuilds = c->precond_rebuilds;
         o->matched_preconditioners = c->matched_preconditioners;
+        o->dense_tail_level = -1;
+        if (c->solver->dense_tail_setting()) {
+            for (std::size_t i = 0; i < c->hier.levels.size(); ++i) {
+                if (c->hier.levels[i].a.n <= cn::kDenseTailMaxRows) {
+                    o->dense_tail_level = static_cast<int32_t>(i);
+                    break;
+                }
+            }
+        }
         o->n_cu = c->n_cu;
     });
 }

@@ -672,6 +672,95 @@ __global__ void __launch_bounds__(kRowThreads) coarse_kernel(int n, const double
     }
 }
 
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+// Dense tail: x = M b for the (n, n) FP32 operator M of the V-cycle from a level down, on the FP32
+// matrix cores. M is stored in 16 x 4 tiles in the operand order of v_mfma_f32_16x16x4f32 (lane l
+// holds row l % 16, column l / 16 of a tile): tile (row tile rt, column quad q) is the 64 floats at
+// (rt * nq + q) * 64, rows and columns past n zero. A block computes one 16-row tile: each of its
+// wavefronts accumulates a contiguous share of the column quads into 16 x 16 FP32 tiles (one per 16
+// columns of b), one coalesced 256-byte load of M and one of b per product, the loads of kTailBatch
+// products issued before the first of them; the shares fold in wavefront order, so the sums are
+// fixed. Accumulator v of lane l is tile row 4 (l / 16) + v, column l % 16.
+constexpr int kTailThreads = 1024;
+constexpr int kTailBatch = 16;
+
+template <int K>
+__global__ void __launch_bounds__(kTailThreads) dense_tail_kernel(int n, int nq, const float* __restrict__ Mt,
+                                                                  const float* __restrict__ b, float* __restrict__ x,
+                                                                  const int* __restrict__ state) {
+    constexpr int H = K > 16 ? K / 16 : 1;  // 16-column tiles of b
+    constexpr int WAVES = kTailThreads / kWave;
+    if (finished(state)) return;
+    __shared__ float part[WAVES][H][4][kWave];
+    const int rt = static_cast<int>(blockIdx.x);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
+    const int col = lane & 15, quad = lane >> 4;
+    const int per = (nq + WAVES - 1) / WAVES;
+    const int q0 = wave * per, q1 = min(q0 + per, nq);
+    floatx4_t acc[H];
+#pragma unroll
+    for (int h = 0; h < H; ++h) acc[h] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+    const float* a = Mt + static_cast<std::int64_t>(rt) * nq * kWave + lane;
+    for (int q = q0; q < q1; q += kTailBatch) {
+        float av[kTailBatch], bv[kTailBatch][H];
+#pragma unroll
+        for (int u = 0; u < kTailBatch; ++u) {
+            // Past the share's end: the last quad again, with M taken as zero (loads stay unconditional).
+            const int qq = min(q + u, q1 - 1);
+            av[u] = a[static_cast<std::int64_t>(qq) * kWave];
+            const int k = min(4 * qq + quad, n - 1);  // rows of b past n meet zero columns of M
+#pragma unroll
+            for (int h = 0; h < H; ++h) {
+                const int j = 16 * h + col;
+                bv[u][h] = (j < K) ? b[static_cast<std::int64_t>(k) * K + j] : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kTailBatch; ++u) {
+            const float m = (q + u < q1) ? av[u] : 0.0f;
+#pragma unroll
+            for (int h = 0; h < H; ++h) acc[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(m, bv[u][h], acc[h], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < H; ++h)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) part[wave][h][v][lane] = acc[h][v];
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+            const int j = 16 * h + col;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                float s = part[0][h][v][lane];
+                for (int w = 1; w < WAVES; ++w) s += part[w][h][v][lane];
+                const int row = 16 * rt + 4 * quad + v;
+                if (row < n && j < K) x[static_cast<std::int64_t>(row) * K + j] = s;
+            }
+        }
+    }
+}
+
+// Column block [c0, c0 + K) of the identity at n rows, (n, K) row-major.
+template <int K>
+__global__ void identity_cols_kernel(int n, int c0, float* __restrict__ e) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < static_cast<std::int64_t>(n) * K) e[i] = (i / K == c0 + i % K) ? 1.0f : 0.0f;
+}
+
+// Columns c0 .. c0 + K - 1 (those below n) of the tiled M from the (n, K) block X.
+template <int K>
+__global__ void scatter_cols_kernel(int n, int nq, int c0, const float* __restrict__ X, float* __restrict__ Mt) {
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= static_cast<std::int64_t>(n) * K) return;
+    const int r = static_cast<int>(i / K);
+    const int c = c0 + static_cast<int>(i % K);
+    if (c < n)
+        Mt[(static_cast<std::int64_t>(r / 16) * nq + c / 4) * kWave + (c % 4) * 16 + r % 16] = X[i];
+}
+
 // ---------------------------------------------------------------------------------------------
 // Block-vector kernels of the PCG iteration.
 // ---------------------------------------------------------------------------------------------
@@ -1292,22 +1381,39 @@ unsigned launch_rows(int sub, const CsrView& a, const float* diag, const float* 
     return grid;
 }
 
-// One zero-guess V-cycle (solver/amg/vcycle.cu:560-648). `state` lets captured replays skip it.
+// One zero-guess V-cycle (solver/amg/vcycle.cu:560-648) from level `first` (B and X have that
+// level's rows). With a dense tail at level t >= first, the cycle below level t - 1 is one product
+// with the tail's operator: the restriction into level t, x_t = M b_t, the prolongation from it.
+// `state` lets captured replays skip it.
 template <int K>
 void run_vcycle(const Hierarchy& h, const std::vector<int>& subs, std::vector<DeviceBuffer<float>>& lb,
                 std::vector<DeviceBuffer<float>>& lx, std::vector<DeviceBuffer<float>>& lr, float* cb, float* cx,
-                const float* B, float* X, const int* state, int n_cu, hipStream_t s) {
+                const float* B, float* X, const int* state, int n_cu, hipStream_t s, int first = 0,
+                const DenseTail* tail = nullptr) {
     const Cheby& ch = h.cheb;
     const int nl = static_cast<int>(h.levels.size());
-    if (nl == 0) {
+    const bool dense = tail != nullptr && tail->level >= first && tail->level < nl;
+    const int stop = dense ? tail->level : nl;  // levels first .. stop - 1 smooth sparsely
+    auto apply_tail = [&](const float* bt, float* xt) {
+        const unsigned g = static_cast<unsigned>((tail->n + 15) / 16);
+        dense_tail_kernel<K><<<g, kTailThreads, 0, s>>>(tail->n, tail->nq, tail->m.get(), bt, xt, state);
+    };
+    if (dense && stop == first) {
+        apply_tail(B, X);
+        return;
+    }
+    if (nl == first) {
         coarse_kernel<K><<<grid_for(h.n_coarse(), kRowWaves), kRowThreads, 0, s>>>(
             static_cast<int>(h.n_coarse()), h.coarse_inv.get(), B, X, state);
         return;
     }
-    auto smooth = [&](int li, const float* bi, const float* x0, float* first, float* other, float* final_out) -> float* {
+    auto smooth = [&](int li, const float* bi, const float* x0, float* first_buf, float* other,
+                      float* final_out) -> float* {
         const Level& L = h.levels[li];
         const int sub = subs[li];
-        auto target = [&](int st) { return (st + 1 == ch.degree && final_out) ? final_out : ((st % 2) ? other : first); };
+        auto target = [&](int st) {
+            return (st + 1 == ch.degree && final_out) ? final_out : ((st % 2) ? other : first_buf);
+        };
         if (x0 == nullptr && ch.degree == 2) {
             // Jacobi from zero and the Chebyshev step after it, in one pass (the Jacobi iterate is
             // only ever read by that step).
@@ -1334,9 +1440,9 @@ void run_vcycle(const Hierarchy& h, const std::vector<int>& subs, std::vector<De
         return cur;
     };
     const bool odd = (ch.degree % 2) != 0;
-    for (int i = 0; i < nl; ++i) {
+    for (int i = first; i < stop; ++i) {
         const Level& L = h.levels[i];
-        const float* bi = (i == 0) ? B : lb[i].get();
+        const float* bi = (i == first) ? B : lb[i].get();
         smooth(i, bi, nullptr, odd ? lx[i].get() : lr[i].get(), odd ? lr[i].get() : lx[i].get(), nullptr);
         launch_rows<K, kResidual>(subs[i], L.a, L.dinv.get(), bi, lx[i].get(), lx[i].get(), 0.f, 0.f, 0.f, lr[i].get(),
                                   state, s);
@@ -1345,16 +1451,22 @@ void run_vcycle(const Hierarchy& h, const std::vector<int>& subs, std::vector<De
         launch_rows<K, kSpmm>(choose_sub(R.n, R.nnz, col_groups<K>(), n_cu), R, nullptr, nullptr, lr[i].get(),
                               nullptr, 0.f, 0.f, 0.f, next_b, state, s);
     }
-    coarse_kernel<K><<<grid_for(h.n_coarse(), kRowWaves), kRowThreads, 0, s>>>(static_cast<int>(h.n_coarse()),
-                                                                                h.coarse_inv.get(), cb, cx, state);
-    const float* xc = cx;
-    for (int i = nl - 1; i >= 0; --i) {
+    const float* xc;
+    if (dense) {
+        apply_tail(lb[stop].get(), lx[stop].get());
+        xc = lx[stop].get();
+    } else {
+        coarse_kernel<K><<<grid_for(h.n_coarse(), kRowWaves), kRowThreads, 0, s>>>(static_cast<int>(h.n_coarse()),
+                                                                                    h.coarse_inv.get(), cb, cx, state);
+        xc = cx;
+    }
+    for (int i = stop - 1; i >= first; --i) {
         const Level& L = h.levels[i];
-        const float* bi = (i == 0) ? B : lb[i].get();
+        const float* bi = (i == first) ? B : lb[i].get();
         const CsrView P = L.p.view();
         launch_rows<K, kProlong>(choose_sub(P.n, P.nnz, col_groups<K>(), n_cu), P, nullptr, nullptr, xc, lx[i].get(),
                                  0.f, 0.f, 0.f, lx[i].get(), state, s);
-        xc = smooth(i, bi, lx[i].get(), lr[i].get(), lx[i].get(), (i == 0) ? X : nullptr);
+        xc = smooth(i, bi, lx[i].get(), lr[i].get(), lx[i].get(), (i == first) ? X : nullptr);
     }
 }
 
@@ -1481,6 +1593,7 @@ bool BlockSolver::capture(hipGraphExec_t& gx, F&& body) {
 // Replaced executables are retired rather than destroyed at once (a short queue, the oldest
 // destroyed beyond its bound).
 void BlockSolver::drop_graphs() {
+    tail_ready_ = false;  // whatever dropped the graphs may have changed the V-cycle
     for (auto& g : sets_) {
         for (int i = 0; i < kNumBlockWidths; ++i) {
             retire(g.win[i]);
@@ -1585,6 +1698,53 @@ void BlockSolver::set_coarse_sub(int sub) {
 }
 
 template <int K>
+void BlockSolver::build_tail_k(int t) {
+    const int nt = static_cast<int>(h_.levels[t].a.n);
+    const int nq = (nt + 3) / 4;
+    const std::size_t size = static_cast<std::size_t>((nt + 15) / 16) * nq * kWave;
+    std::vector<int> subs(h_.levels.size());
+    for (std::size_t i = 0; i < h_.levels.size(); ++i)
+        subs[i] = choose_sub(h_.levels[i].a.n, h_.levels[i].a.nnz, col_groups<K>(), n_cu_);
+    if (coarse_sub_ > 0)
+        for (std::size_t i = 1; i < subs.size(); ++i) subs[i] = std::min(coarse_sub_, kWave / col_groups<K>());
+    tail_.m.reset(size, "dense tail");
+    check(hipMemsetAsync(tail_.m.get(), 0, sizeof(float) * size, s_), "dense tail clear");
+    DeviceBuffer<float> e(static_cast<std::size_t>(nt) * K, "identity block"), xo(static_cast<std::size_t>(nt) * K,
+                                                                                   "tail columns");
+    const unsigned g = grid_for(static_cast<std::int64_t>(nt) * K);
+    for (int c0 = 0; c0 < nt; c0 += K) {
+        identity_cols_kernel<K><<<g, kBlock, 0, s_>>>(nt, c0, e.get());
+        run_vcycle<K>(h_, subs, lb_, lx_, lr_, cb_.get(), cx_.get(), e.get(), xo.get(), nullptr, n_cu_, s_, t,
+                      nullptr);
+        scatter_cols_kernel<K><<<g, kBlock, 0, s_>>>(nt, nq, c0, xo.get(), tail_.m.get());
+    }
+    sync(s_, "dense tail");  // e and xo are freed on return
+    tail_.level = t;
+    tail_.n = nt;
+    tail_.nq = nq;
+}
+
+void BlockSolver::ensure_tail() {
+    if (tail_ready_) return;
+    tail_.level = -1;
+    if (tail_on_) {
+        int t = -1;
+        for (std::size_t i = 0; i < h_.levels.size(); ++i) {
+            if (h_.levels[i].a.n <= kDenseTailMaxRows) {
+                t = static_cast<int>(i);
+                break;
+            }
+        }
+        if (t >= 0) {
+            if (cap_k_ < 1) ensure(1);
+            // Formed at the widest allocated width up to 16: the columns of M do not depend on it.
+            dispatch_k(std::min(cap_k_, 16), [&](auto kc) { build_tail_k<decltype(kc)::value>(t); });
+        }
+    }
+    tail_ready_ = true;
+}
+
+template <int K>
 void BlockSolver::vcycle_k(const float* B, float* X, const int* state) {
     // Lanes per column of each level's row kernels: a property of the level and the width alone,
     // unless a fine-level count is in force (clamped to what one wavefront holds at this width).
@@ -1594,7 +1754,8 @@ void BlockSolver::vcycle_k(const float* B, float* X, const int* state) {
     if (!subs.empty() && fine_sub_override_ > 0) subs[0] = std::min(fine_sub_override_, kWave / col_groups<K>());
     if (coarse_sub_ > 0)
         for (std::size_t i = 1; i < subs.size(); ++i) subs[i] = std::min(coarse_sub_, kWave / col_groups<K>());
-    run_vcycle<K>(h_, subs, lb_, lx_, lr_, cb_.get(), cx_.get(), B, X, state, n_cu_, s_);
+    run_vcycle<K>(h_, subs, lb_, lx_, lr_, cb_.get(), cx_.get(), B, X, state, n_cu_, s_, 0,
+                  (tail_ready_ && tail_.level >= 0) ? &tail_ : nullptr);
 }
 
 template <int K>
@@ -1673,6 +1834,7 @@ void BlockSolver::flush_pending_k(int L) {
 template <int K>
 SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int k, const SolveOptions& opt) {
     ensure(K);
+    ensure_tail();
     const std::int64_t nn = n();
     const std::int64_t nk = nn * K;
     const std::int64_t rpb = gram_rows(nn);
@@ -1876,6 +2038,7 @@ void BlockSolver::gram(const float* A, const float* B1, const float* B2, double*
 
 void BlockSolver::vcycle(const float* B, float* X, int K) {
     ensure(K);
+    ensure_tail();
     dispatch_k(K, [&](auto kc) { vcycle_k<decltype(kc)::value>(B, X, nullptr); });
 }
 

@@ -33,6 +33,18 @@ int build_ell(const CsrView& a, DeviceBuffer<int2>& pairs, hipStream_t s);
 void gram_noalloc(std::int64_t n, int K, const float* A, const float* B1, const float* B2, double* out,
                   hipStream_t s);
 
+// The zero-guess V-cycle from one level down (that level's smoothing, every coarser level and the
+// coarse solve) as one dense FP32 operator M (n x n), formed by running that part of the cycle on
+// the identity's columns: the solve applies it as one matrix-core product. M is held in the
+// product's 16 x 4 operand tiles (nq = ceil(n / 4) column quads per 16-row tile, zero past n).
+// Levels of up to kDenseTailMaxRows rows qualify (M at most 64 MB); the first one does.
+struct DenseTail {
+    int level = -1;  // -1: none
+    int n = 0, nq = 0;
+    DeviceBuffer<float> m;
+};
+constexpr int kDenseTailMaxRows = 4096;
+
 struct SolveOptions {
     double tolerance = 1e-6;    // ||b - A x|| / ||b|| per column (FP64 true residual)
     int max_iters = 2000;       // inner iterations over all refinement rounds
@@ -117,6 +129,18 @@ class BlockSolver {
     void set_fine_sub(int sub);
     int fine_sub_setting() const { return fine_sub_set_; }
     void set_coarse_sub(int sub);
+    // Apply the V-cycle's bottom levels as one dense product (see DenseTail); on by default.
+    // Drops the captured iterations.
+    void set_dense_tail(bool on) {
+        tail_on_ = on;
+        drop_graphs();
+    }
+    bool dense_tail_setting() const { return tail_on_; }
+    // The level the dense tail starts at (-1: none), forming it first if it is due.
+    int dense_tail_level() {
+        ensure_tail();
+        return tail_.level;
+    }
     // The differenced operator's sliced ELL copy for the block iteration's SpMM (nullptr: CSR).
     // Drops the captured iterations.
     void set_off_ell(const int2* ell, int w, int max_k) {
@@ -164,6 +188,11 @@ class BlockSolver {
     template <int K>
     int fine_sub() const;
     void ensure(int K);
+    // Form the dense tail if it is on and not current (never during a capture: solve entry points
+    // call it before any launch).
+    void ensure_tail();
+    template <int K>
+    void build_tail_k(int t);
 
     Operator op_;
     const Hierarchy& h_;
@@ -174,6 +203,9 @@ class BlockSolver {
     int fine_sub_override_ = 0;  // the fine level's sub-lane count in force (bench_spmm may force one)
     int coarse_sub_ = 0;         // sub-lanes of the coarser levels' row kernels (0: automatic)
     const MultiOperator* multi_ = nullptr;
+    DenseTail tail_;
+    bool tail_on_ = true;
+    bool tail_ready_ = false;  // tail_ matches the hierarchy (cleared with the captured iterations)
     // block vectors (n, K)
     // W_ is slot 0 of the search-block ring (kLag slots), so the deferred solution update still
     // has every W of its window.

@@ -501,22 +501,26 @@ class SolverContext:
         t = _lib.Tuning()
         check(lib().cn_context_get_tuning(self.handle, ctypes.byref(t)), "cn_context_get_tuning")
         return {"degree": int(t.degree), "lower_ratio": float(t.lower_ratio), "fine_sub": int(t.fine_sub),
-                "coarse_sub": int(t.coarse_sub), "coil_blocks": int(t.coil_blocks), "layout": int(t.layout)}
+                "coarse_sub": int(t.coarse_sub), "coil_blocks": int(t.coil_blocks), "layout": int(t.layout),
+                "dense_tail": int(t.dense_tail)}
 
     def set_tuning(self, degree: int | None = None, lower_ratio: float | None = None,
                    fine_sub: int | None = None, coarse_sub: int | None = None,
-                   coil_blocks: int | None = None, layout: int | None = None) -> None:
+                   coil_blocks: int | None = None, layout: int | None = None,
+                   dense_tail: int | None = None) -> None:
         """Change the settings of `tuning` between solves (None keeps the current value).
         ``layout`` 0 runs the finest level's row kernels on the sliced ELL copy (when its rows
         fit) for blocks of up to 4 columns and on the CSR arrays above, 1 always on the CSR
-        arrays, 2 always on the ELL copy; results are the same."""
+        arrays, 2 always on the ELL copy; results are the same. ``dense_tail`` 1 applies the
+        V-cycle's bottom levels as one dense product formed at the first solve, 0 level by level."""
         cur = self.tuning
         t = _lib.Tuning(cur["degree"] if degree is None else int(degree),
                         cur["lower_ratio"] if lower_ratio is None else float(lower_ratio),
                         cur["fine_sub"] if fine_sub is None else int(fine_sub),
                         cur["coarse_sub"] if coarse_sub is None else int(coarse_sub),
                         cur["coil_blocks"] if coil_blocks is None else int(coil_blocks),
-                        cur["layout"] if layout is None else int(layout))
+                        cur["layout"] if layout is None else int(layout),
+                        cur["dense_tail"] if dense_tail is None else int(dense_tail))
         check(lib().cn_context_set_tuning(self.handle, ctypes.byref(t)), "cn_context_set_tuning")
 
     def bench_spmm(self, K: int, reps: int = 20, mode: int = 1, sub: int = 0) -> float:

@@ -93,6 +93,10 @@ typedef struct {
                            (built when every row has at most 24 entries) for blocks of up to 4
                            columns and CSR above, 1 = CSR, 2 = the ELL copy at every width; the
                            sums are the same */
+    int32_t dense_tail; /* 1 (default): the V-cycle below the first level of at most 4096 rows
+                           (that level's smoothing, the coarser levels and the coarse solve) is
+                           formed once as a dense FP32 operator and applied as one product;
+                           0: level by level */
 } cn_tuning;
 
 int32_t cn_context_get_tuning(const cn_context* ctx, cn_tuning* out);
@@ -117,6 +121,8 @@ typedef struct {
                                         rebuilt hierarchy (fem/solve.py:508-552) */
     int32_t matched_preconditioners; /* throwaway preconditioners built for one column's own
                                         conductivities after it missed (uq/conductivity/run.py:333-342) */
+    int32_t dense_tail_level; /* the level from which the V-cycle is applied as one dense product
+                                 (cn_tuning.dense_tail), -1 when none */
 } cn_info;
 
 int32_t cn_context_info(const cn_context* ctx, cn_info* out);

@@ -36,7 +36,7 @@ def tctx(head4):
 def test_default_tuning_is_the_reference_setting(tctx):
     t = tctx.tuning
     assert t == {"degree": 2, "lower_ratio": pytest.approx(12.0), "fine_sub": 0, "coarse_sub": 0, "coil_blocks": 0,
-                 "layout": 0}
+                 "layout": 0, "dense_tail": 1}
 
 
 @pytest.mark.parametrize("degree", [1, 3, 6])

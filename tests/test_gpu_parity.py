@@ -506,3 +506,28 @@ def test_degenerate_and_unknown_tetrahedra_are_rejected():
     tags[4] = 42
     with pytest.raises(ValueError, match="42|tag"):
         fem.SolverContext(CUBE_NODES, CUBE_TETS, tags, skin)
+
+
+def test_dense_tail_matches_the_level_by_level_vcycle(block8, gctx, coil):
+    """The V-cycle's bottom levels applied as one dense FP32 product (formed by running them on the
+    identity's columns) agree with the level-by-level cycle to FP32 rounding, and a block solve
+    with it converges alike."""
+    from cunibs_amd import fem
+    sites, res = block8
+    assert gctx.tuning["dense_tail"] == 1
+    t = gctx.info.dense_tail_level
+    assert 0 <= t < gctx.info.n_levels and gctx.info.level_rows[t] <= 4096
+    assert t == 0 or gctx.info.level_rows[t - 1] > 4096
+    X = np.random.default_rng(21).standard_normal((gctx.n_nodes - 1, 16)).astype(np.float32)
+    with_tail = gctx.vcycle(X)
+    gctx.set_tuning(dense_tail=0)
+    try:
+        assert gctx.info.dense_tail_level == -1
+        level_by_level = gctx.vcycle(X)
+        plain = fem.solve_placements_block(gctx, coil.positions_m, coil.moments, sites, 1e6)
+    finally:
+        gctx.set_tuning(dense_tail=1)
+    assert rel(with_tail, level_by_level) < 2e-6
+    for a, b in zip(res, plain):
+        assert abs(a.iterations - b.iterations) <= 1
+        assert rel(a.magnE, b.magnE) < 2e-5
