@@ -566,6 +566,7 @@ __global__ void __launch_bounds__(kFitWaves * kWave) __attribute__((amdgpu_waves
 
 // E_slots[s, d, c] = -sum_p w3[p, d] v[pidx[p], c] - dA/dt[node, d, c], FP64 accumulation in patch
 // order; v is the reduced potential with the ground node's zero restored.
+constexpr int kGradBatch = 8;
 __global__ void hpr_grad_kernel(const double* __restrict__ X, int ground, const float* __restrict__ w3,
                                 const std::int32_t* __restrict__ pptr, const std::int32_t* __restrict__ pidx,
                                 const std::int32_t* __restrict__ slot_node, const float* __restrict__ dadt, int Kn,
@@ -575,12 +576,36 @@ __global__ void hpr_grad_kernel(const double* __restrict__ X, int ground, const 
     const std::int64_t s = t / K;
     const int c = static_cast<int>(t % K);
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int p = pptr[s]; p < pptr[s + 1]; ++p) {
-        const int mm = pidx[p];
-        const double vm = (mm == ground) ? 0.0 : X[static_cast<std::int64_t>(mm - (mm > ground)) * K + c];
-        a0 = fma(static_cast<double>(w3[3 * static_cast<std::int64_t>(p)]), vm, a0);
-        a1 = fma(static_cast<double>(w3[3 * static_cast<std::int64_t>(p) + 1]), vm, a1);
-        a2 = fma(static_cast<double>(w3[3 * static_cast<std::int64_t>(p) + 2]), vm, a2);
+    const int b = pptr[s], e = pptr[s + 1];
+    // Entries go in batches: the batch's indices and weights, then its potentials, are all loaded
+    // before the first use (an entry past the patch's end reads the last entry again and is not
+    // added), so a patch costs a few memory round trips instead of one per entry.
+    for (int p = b; p < e; p += kGradBatch) {
+        int mm[kGradBatch];
+        float w[kGradBatch][3];
+#pragma unroll
+        for (int u = 0; u < kGradBatch; ++u) {
+            const std::int64_t pp = min(p + u, e - 1);
+            mm[u] = pidx[pp];
+            w[u][0] = w3[3 * pp];
+            w[u][1] = w3[3 * pp + 1];
+            w[u][2] = w3[3 * pp + 2];
+        }
+        double vm[kGradBatch];
+#pragma unroll
+        for (int u = 0; u < kGradBatch; ++u) {
+            const int r = mm[u] - (mm[u] > ground);
+            const double v = X[static_cast<std::int64_t>(mm[u] == ground ? 0 : r) * K + c];
+            vm[u] = (mm[u] == ground) ? 0.0 : v;
+        }
+#pragma unroll
+        for (int u = 0; u < kGradBatch; ++u) {
+            if (p + u < e) {
+                a0 = fma(static_cast<double>(w[u][0]), vm[u], a0);
+                a1 = fma(static_cast<double>(w[u][1]), vm[u], a1);
+                a2 = fma(static_cast<double>(w[u][2]), vm[u], a2);
+            }
+        }
     }
     const std::int64_t node = slot_node[s];
     const float* da = dadt + node * 3 * Kn + (Kn == 1 ? 0 : c);
