@@ -637,38 +637,34 @@ __global__ void jacobi_zero_kernel(std::int64_t n, const float* __restrict__ din
     if (i < n * K) x[i] = alpha * dinv[i / K] * b[i];
 }
 
-// Coarsest level in FP64: x = Ainv b, one wavefront per row of the dense inverse.
+// Coarsest level in FP64: x = Ainv b, one workgroup per row of the dense inverse. Lane l of
+// wavefront w takes column l % K of the entries j = Q w + l / K (mod 4 Q, Q = 64 / K) of the row
+// in FP64; a column's partial sums fold by a fixed xor tree within each wavefront, then over the
+// wavefronts in order.
 template <int K>
 __global__ void __launch_bounds__(kRowThreads) coarse_kernel(int n, const double* __restrict__ inv,
                                                              const float* __restrict__ b, float* __restrict__ x,
                                                              const int* __restrict__ state) {
-    // One wavefront per row; lane l takes column l % K of the entries j = l / K (mod 64 / K) of the
-    // row, in FP64, then the 64 / K partial sums of a column fold by a fixed xor tree.
-    constexpr int C = K < kWave ? K : kWave;
-    constexpr int Q = kWave / C;
+    static_assert(K <= kWave, "a row's columns fit a wavefront");
+    constexpr int Q = kWave / K;
     if (finished(state)) return;
-    const int row = static_cast<int>(blockIdx.x) * kRowWaves + static_cast<int>(threadIdx.x >> 6);
-    const int lane = threadIdx.x & (kWave - 1);
-    const int c = lane % C, q = lane / C;
-    double acc[K / C];
+    __shared__ double part[kRowWaves][K];
+    const int row = static_cast<int>(blockIdx.x);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
+    const int c = lane % K, q = lane / K;
+    const double* a = inv + static_cast<std::int64_t>(row) * n;
+    double acc = 0.0;
+    for (int j = wave * Q + q; j < n; j += kRowWaves * Q)
+        acc = fma(a[j], static_cast<double>(b[static_cast<std::int64_t>(j) * K + c]), acc);
 #pragma unroll
-    for (int u = 0; u < K / C; ++u) acc[u] = 0.0;
-    if (row < n) {
-        const double* a = inv + static_cast<std::int64_t>(row) * n;
-        for (int j = q; j < n; j += Q) {
-            const double v = a[j];
+    for (int off = K; off < kWave; off <<= 1) acc += __shfl_xor(acc, off, kWave);
+    if (q == 0) part[wave][c] = acc;
+    __syncthreads();
+    if (threadIdx.x < K) {
+        double s = part[0][threadIdx.x];
 #pragma unroll
-            for (int u = 0; u < K / C; ++u)
-                acc[u] = fma(v, static_cast<double>(b[static_cast<std::int64_t>(j) * K + u * C + c]), acc[u]);
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < K / C; ++u)
-#pragma unroll
-        for (int off = C; off < kWave; off <<= 1) acc[u] += __shfl_xor(acc[u], off, kWave);
-    if (row < n && q == 0) {
-#pragma unroll
-        for (int u = 0; u < K / C; ++u) x[static_cast<std::int64_t>(row) * K + u * C + c] = static_cast<float>(acc[u]);
+        for (int w = 1; w < kRowWaves; ++w) s += part[w][threadIdx.x];
+        x[static_cast<std::int64_t>(row) * K + threadIdx.x] = static_cast<float>(s);
     }
 }
 
@@ -1403,7 +1399,7 @@ void run_vcycle(const Hierarchy& h, const std::vector<int>& subs, std::vector<De
         return;
     }
     if (nl == first) {
-        coarse_kernel<K><<<grid_for(h.n_coarse(), kRowWaves), kRowThreads, 0, s>>>(
+        coarse_kernel<K><<<static_cast<unsigned>(h.n_coarse()), kRowThreads, 0, s>>>(
             static_cast<int>(h.n_coarse()), h.coarse_inv.get(), B, X, state);
         return;
     }
@@ -1456,8 +1452,8 @@ void run_vcycle(const Hierarchy& h, const std::vector<int>& subs, std::vector<De
         apply_tail(lb[stop].get(), lx[stop].get());
         xc = lx[stop].get();
     } else {
-        coarse_kernel<K><<<grid_for(h.n_coarse(), kRowWaves), kRowThreads, 0, s>>>(static_cast<int>(h.n_coarse()),
-                                                                                    h.coarse_inv.get(), cb, cx, state);
+        coarse_kernel<K><<<static_cast<unsigned>(h.n_coarse()), kRowThreads, 0, s>>>(
+            static_cast<int>(h.n_coarse()), h.coarse_inv.get(), cb, cx, state);
         xc = cx;
     }
     for (int i = stop - 1; i >= first; --i) {
