@@ -68,6 +68,33 @@ __device__ __forceinline__ void count_block(int pred, int* counter) {
     if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicAdd(counter, __popcll(m));
 }
 
+// Matching passes run under device-side control: every pass's kernels return at once when `stop`
+// is set, and a one-thread decision after each pass sets it by the rules a host loop would apply
+// to the pass's count of unmatched rows, so the passes are queued back to back with no readback.
+struct AggLoop {
+    int remaining;  // unmatched rows counted by the current pass
+    int previous;   // the count after the previous pass (n before the first)
+    int stop;
+};
+
+__global__ void agg_loop_init_kernel(AggLoop* __restrict__ c, int n) {
+    c->remaining = 0;
+    c->previous = n;
+    c->stop = 0;
+}
+
+// Stop when no row is left unmatched or a pass matched none; in the second round also when fewer
+// than 5 % of the rows are left (aggregate.cu's loop exits).
+__global__ void agg_loop_decide_kernel(AggLoop* __restrict__ c, int n, int second_round) {
+    if (c->stop) return;
+    const int u = c->remaining;
+    bool stop = u == 0 || u == c->previous;
+    if (second_round) stop = stop || static_cast<float>(u) / static_cast<float>(n) < 0.05f;
+    c->stop = stop ? 1 : 0;
+    c->previous = u;
+    c->remaining = 0;
+}
+
 __global__ void agg_diag_kernel(const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
                                 const float* __restrict__ v, std::int64_t n, float* __restrict__ diag) {
     const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -115,7 +142,9 @@ __global__ void agg_find_strongest_nomerge_kernel(const std::int32_t* __restrict
                                                   const std::int32_t* __restrict__ ci,
                                                   const float* __restrict__ w, std::int64_t n,
                                                   const std::int32_t* __restrict__ partner,
-                                                  std::int32_t* __restrict__ strongest) {
+                                                  std::int32_t* __restrict__ strongest,
+                                                  const AggLoop* __restrict__ ctl) {
+    if (ctl->stop) return;
     const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= n || partner[t] != -1) return;
     Best b{0.0f, -1};
@@ -129,7 +158,9 @@ __global__ void agg_find_strongest_nomerge_kernel(const std::int32_t* __restrict
 
 __global__ void agg_match_edges_kernel(std::int64_t n, std::int32_t* __restrict__ partner,
                                        std::int32_t* __restrict__ agg, const std::int32_t* __restrict__ strongest,
-                                       int* __restrict__ remaining) {
+                                       AggLoop* __restrict__ ctl) {
+    if (ctl->stop) return;
+    int* const remaining = &ctl->remaining;
     const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     int unassigned = 0;
     if (t < n && partner[t] == -1) {
@@ -159,7 +190,9 @@ __global__ void agg_find_strongest_store_kernel(const std::int32_t* __restrict__
                                                 const std::int32_t* __restrict__ aggregated,
                                                 const std::int32_t* __restrict__ agg,
                                                 const std::int32_t* __restrict__ partner,
-                                                std::int32_t* __restrict__ strongest, float* __restrict__ wsn) {
+                                                std::int32_t* __restrict__ strongest, float* __restrict__ wsn,
+                                                const AggLoop* __restrict__ ctl) {
+    if (ctl->stop) return;
     const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= n || aggregated[t] != -1) return;
     const int p = partner[t];
@@ -178,7 +211,9 @@ __global__ void agg_find_strongest_store_kernel(const std::int32_t* __restrict__
 
 __global__ void agg_agree_on_proposal_kernel(std::int64_t n, std::int32_t* __restrict__ aggregated,
                                              std::int32_t* strongest, const float* __restrict__ wsn,
-                                             const std::int32_t* __restrict__ partner) {
+                                             const std::int32_t* __restrict__ partner,
+                                             const AggLoop* __restrict__ ctl) {
+    if (ctl->stop) return;
     const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= n || aggregated[t] != -1) return;
     const int p = partner[t];
@@ -195,7 +230,9 @@ __global__ void agg_agree_on_proposal_kernel(std::int64_t n, std::int32_t* __res
 __global__ void agg_match_aggregates_kernel(std::int64_t n, std::int32_t* __restrict__ agg,
                                             std::int32_t* __restrict__ aggregated,
                                             const std::int32_t* __restrict__ strongest,
-                                            int* __restrict__ remaining) {
+                                            AggLoop* __restrict__ ctl) {
+    if (ctl->stop) return;
+    int* const remaining = &ctl->remaining;
     const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     int unassigned = 0;
     if (t < n && aggregated[t] == -1) {
@@ -493,38 +530,34 @@ std::int64_t select_size4(const CsrView& a, std::int32_t* agg, hipStream_t s) {
     DeviceBuffer<float> diag(n, "agg diag"), w(std::max<std::int64_t>(nnz, 1), "agg w"), wsn(n, "agg wsn");
     DeviceBuffer<std::int32_t> partner(n, "partner"), strongest(n, "strongest"), aggregated(n, "aggregated"),
         cand(n, "cand"), used(n + 1, "used"), excl(n + 1, "excl");
-    DeviceBuffer<int> counter(1, "counter");
+    DeviceBuffer<AggLoop> ctl(1, "matching control");
+    const int ni = static_cast<int>(n);
     agg_diag_kernel<<<g, kBlock, 0, s>>>(a.ptr, a.col, a.val, n, diag.get());
     if (nnz > 0) agg_edge_weight_kernel<<<g, kBlock, 0, s>>>(a.ptr, a.col, a.val, diag.get(), n, w.get());
     agg_init_kernel<<<g, kBlock, 0, s>>>(n, agg, partner.get(), strongest.get());
-    std::int64_t unassigned = n;
+    // Round 1: match rows in pairs along mutual strongest edges, at most 16 passes.
+    agg_loop_init_kernel<<<1, 1, 0, s>>>(ctl.get(), ni);
     for (int iter = 0; iter <= 15; ++iter) {
-        check(hipMemsetAsync(counter.get(), 0, sizeof(int), s), "memset counter");
-        agg_find_strongest_nomerge_kernel<<<g, kBlock, 0, s>>>(a.ptr, a.col, w.get(), n, partner.get(), strongest.get());
-        agg_match_edges_kernel<<<g, kBlock, 0, s>>>(n, partner.get(), agg, strongest.get(), counter.get());
-        const std::int64_t previous = unassigned;
-        unassigned = read_int(counter.get(), s);
-        if (unassigned == 0 || unassigned == previous) break;
+        agg_find_strongest_nomerge_kernel<<<g, kBlock, 0, s>>>(a.ptr, a.col, w.get(), n, partner.get(),
+                                                               strongest.get(), ctl.get());
+        agg_match_edges_kernel<<<g, kBlock, 0, s>>>(n, partner.get(), agg, strongest.get(), ctl.get());
+        agg_loop_decide_kernel<<<1, 1, 0, s>>>(ctl.get(), ni, 0);
     }
     agg_round2_init_kernel<<<g, kBlock, 0, s>>>(n, partner.get(), wsn.get(), aggregated.get());
-    unassigned = n;
+    // Round 2: match the pairs (and singletons) into aggregates of up to four, at most 16 passes.
+    agg_loop_init_kernel<<<1, 1, 0, s>>>(ctl.get(), ni);
     for (int iter = 0; iter <= 15; ++iter) {
-        check(hipMemsetAsync(counter.get(), 0, sizeof(int), s), "memset counter");
         agg_find_strongest_store_kernel<<<g, kBlock, 0, s>>>(a.ptr, a.col, w.get(), n, aggregated.get(), agg,
-                                                             partner.get(), strongest.get(), wsn.get());
+                                                             partner.get(), strongest.get(), wsn.get(), ctl.get());
         agg_agree_on_proposal_kernel<<<g, kBlock, 0, s>>>(n, aggregated.get(), strongest.get(), wsn.get(),
-                                                          partner.get());
-        agg_match_aggregates_kernel<<<g, kBlock, 0, s>>>(n, agg, aggregated.get(), strongest.get(), counter.get());
-        const std::int64_t previous = unassigned;
-        unassigned = read_int(counter.get(), s);
-        if (unassigned == 0 || unassigned == previous ||
-            static_cast<float>(unassigned) / static_cast<float>(n) < 0.05f)
-            break;
+                                                          partner.get(), ctl.get());
+        agg_match_aggregates_kernel<<<g, kBlock, 0, s>>>(n, agg, aggregated.get(), strongest.get(), ctl.get());
+        agg_loop_decide_kernel<<<1, 1, 0, s>>>(ctl.get(), ni, 1);
     }
-    if (unassigned != 0) {
-        agg_merge_existing_kernel<<<g, kBlock, 0, s>>>(a.ptr, a.col, w.get(), n, agg, aggregated.get(), cand.get());
-        agg_join_existing_kernel<<<g, kBlock, 0, s>>>(n, agg, aggregated.get(), cand.get());
-    }
+    // Rows still unmatched join their strongest aggregated neighbour's aggregate (both kernels
+    // touch only unmatched rows, so with none left they change nothing).
+    agg_merge_existing_kernel<<<g, kBlock, 0, s>>>(a.ptr, a.col, w.get(), n, agg, aggregated.get(), cand.get());
+    agg_join_existing_kernel<<<g, kBlock, 0, s>>>(n, agg, aggregated.get(), cand.get());
     check(hipMemsetAsync(used.get(), 0, sizeof(std::int32_t) * (n + 1), s), "memset used");
     agg_mark_used_kernel<<<g, kBlock, 0, s>>>(n, agg, used.get());
     exclusive_scan_i32(used.get(), excl.get(), n + 1, s);
@@ -555,15 +588,12 @@ CsrF spgemm(const CsrView& a, const CsrView& b, hipStream_t s) {
     }
     DeviceBuffer<std::int32_t> cand(static_cast<std::size_t>(a.n), "spgemm candidates");
     candidates_kernel<<<grid_for(a.n), kBlock, 0, s>>>(a.ptr, a.col, b.ptr, a.n, cand.get());
-    std::vector<std::int32_t> hc(static_cast<std::size_t>(a.n));
-    d2h(hc.data(), cand.get(), hc.size(), s);
-    sync(s, "spgemm candidates");
     // Counting table: keys only, sized from the candidate count (duplicates included) but capped
     // at 64 KB; a row whose distinct columns would not fit raises the overflow flag.
     constexpr int kMaxCountCap = 16384;
     constexpr int kMaxFillCap = 4096;  // 16 B per slot: 64 KB
-    std::int64_t widest = 1;
-    for (auto v : hc) widest = std::max<std::int64_t>(widest, std::min<std::int64_t>(v, b.n_cols));
+    const std::int64_t widest =
+        std::max<std::int64_t>(1, std::min<std::int64_t>(max_i32(cand.get(), a.n, s), b.n_cols));
     int cap = 64;
     while (cap < 2 * widest && cap < kMaxCountCap) cap <<= 1;
     DeviceBuffer<std::int32_t> counts(static_cast<std::size_t>(a.n + 1), "spgemm counts");
@@ -575,11 +605,9 @@ CsrF spgemm(const CsrView& a, const CsrView& b, hipStream_t s) {
     check(hipGetLastError(), "spgemm count launch");
     int hov = 0;
     d2h(&hov, overflow.get(), 1, s);
-    d2h(hc.data(), counts.get(), hc.size(), s);
     sync(s, "spgemm count");
     require(hov == 0, "sparse product: a row has more distinct columns than the LDS table holds");
-    std::int64_t widest_u = 1;
-    for (auto v : hc) widest_u = std::max<std::int64_t>(widest_u, v);
+    const std::int64_t widest_u = std::max<std::int64_t>(1, max_i32(counts.get(), a.n, s));
     cap = 64;
     while (cap < 2 * widest_u) cap <<= 1;
     require(cap <= kMaxFillCap, "sparse product: a row has more distinct columns than the LDS table holds");

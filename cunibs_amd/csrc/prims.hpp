@@ -17,5 +17,7 @@ void sort_pairs_u32(const std::uint32_t* keys_in, std::uint32_t* keys_out,
 // out[i] = sum(in[0..i)); n elements.
 void exclusive_scan_i32(const std::int32_t* in, std::int32_t* out, std::int64_t n, hipStream_t s);
 void iota_i32(std::int32_t* out, std::int64_t n, hipStream_t s);
+// max(in[0..n)) read back to the host (INT_MIN for n = 0); waits on s.
+int max_i32(const std::int32_t* in, std::int64_t n, hipStream_t s);
 
 }  // namespace cn
