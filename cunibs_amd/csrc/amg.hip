@@ -62,10 +62,21 @@ __device__ __forceinline__ void best_take(Best& acc, float w, int col) {
     }
 }
 
-// Adds the block's count of set predicates to *counter (integer, order-free).
+// Adds the block's count of set predicates to *counter (integer, order-free): each wavefront's
+// count by ballot, the wavefronts' counts summed in LDS, one atomic per block (a pass over a
+// million rows otherwise queues thousands of atomics on one address). Every thread of the block
+// must call it.
 __device__ __forceinline__ void count_block(int pred, int* counter) {
+    __shared__ int wave_counts[kBlock / kWave];
     const unsigned long long m = __ballot(pred);
-    if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicAdd(counter, __popcll(m));
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & (kWave - 1)) == 0) wave_counts[wave] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int c = 0;
+        for (int w = 0; w < static_cast<int>(blockDim.x) / kWave; ++w) c += wave_counts[w];
+        if (c) atomicAdd(counter, c);
+    }
 }
 
 // Matching passes run under device-side control: every pass's kernels return at once when `stop`

@@ -482,17 +482,25 @@ __device__ bool fit_rung(const double* __restrict__ nodes, const std::int32_t* _
         }
         wave_sync();
     }
-    // Rows 1..3 of the inverse (z_c = A^-1 e_{c+1}): lane c solves L y = e_{c+1}, then L^T z = y
-    // in place in its row of z.
+    // Rows 1..3 of the inverse (z_c = A^-1 e_{c+1}): L y = e_{c+1}, then L^T z = y in place in row c
+    // of z. The forward solve runs by columns across the lanes (lane 16 c + r holds row r of
+    // right-hand side c and folds in y_i, i ascending, as each is found: the multiply-adds of the
+    // row-by-row form in the same order); the backward solve runs on lane c.
+    {
+        static_assert(3 * 16 <= kWave, "three right-hand sides of up to 16 rows fit a wavefront");
+        const int c = lane / 16, r = lane % 16;
+        const bool mine = c < 3 && r < N;
+        double v = (mine && r == c + 1) ? 1.0 : 0.0;
+        for (int i = 0; i < N; ++i) {
+            if (mine && r == i) z[c * N + i] = v / a[i * N + i];
+            wave_sync();
+            if (mine && r > i) v = fma(-a[r * N + i], z[c * N + i], v);
+        }
+        wave_sync();
+    }
     if (lane < 3) {
         const int c = lane;
         double* zc = z + c * N;
-#pragma unroll 1
-        for (int i = 0; i < N; ++i) {
-            double v = (i == c + 1) ? 1.0 : 0.0;
-            for (int k = 0; k < i; ++k) v = fma(-a[i * N + k], zc[k], v);
-            zc[i] = v / a[i * N + i];
-        }
 #pragma unroll 1
         for (int i = N - 1; i >= 0; --i) {
             double v = zc[i];
@@ -638,7 +646,12 @@ static_assert(kMinPatchNodes <= kWave, "a short first ring must fit the lanes of
 // of the same-tissue slots of its ring's nodes (the slot is its own neighbour), sorted and
 // distinct. One wavefront per slot. The union is a merge of sorted rings: lane j keeps a cursor
 // into the ring of the slot's j-th ring node, and each step takes the wavefront minimum of the
-// lanes' heads and advances the lanes that held it, one distinct node per step.
+// lanes' heads and advances the lanes that held it, one distinct node per step. The rings are first
+// copied into the wavefront's LDS stage (each lane its own ring, at an offset from a prefix sum of
+// the ring sizes, all loads independent), so the merge steps wait on LDS, not on global memory;
+// rings that do not fit the stage are merged from global memory.
+constexpr int kPatchStage = 1024;  // ints of staged rings per wavefront
+
 template <bool kFill>
 __global__ void __launch_bounds__(kBlock) patch_kernel(
     const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ri,
@@ -657,7 +670,9 @@ __global__ void __launch_bounds__(kBlock) patch_kernel(
             counts_or_ptr[slot] = len;
         return;
     }
-    int cur = 0, end = 0, head = INT_MAX;
+    __shared__ int stage[kBlock / kWave][kPatchStage];
+    int* const st = stage[threadIdx.x / kWave];
+    int cur = 0, end = 0;
     if (lane < len) {
         const int nb = find_slot(slot_node, slot_tag, n_slots, ri[b + lane], slot_tag[slot]);
         if (nb < 0) {
@@ -665,9 +680,28 @@ __global__ void __launch_bounds__(kBlock) patch_kernel(
         } else {
             cur = rp[nb];
             end = rp[nb + 1];
-            head = cur < end ? ri[cur] : INT_MAX;
         }
     }
+    // Offsets of the rings in the stage: an exclusive prefix sum of their sizes over the lanes.
+    const int size = end - cur;
+    int incl = size;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int t = __shfl_up(incl, d, kWave);
+        if (lane >= d) incl += t;
+    }
+    const int total = __shfl(incl, kWave - 1, kWave);
+    const bool staged = total <= kPatchStage;  // wave-uniform
+    const int* src = ri;
+    if (staged) {
+        const int off = incl - size;
+        for (int q = 0; q < size; ++q) st[off + q] = ri[cur + q];
+        wave_sync();
+        src = st;
+        end = off + size;
+        cur = off;
+    }
+    int head = cur < end ? src[cur] : INT_MAX;
     int k = 0, held = 0;
     while (true) {
         const int v = wave_min_i32(head);
@@ -679,7 +713,7 @@ __global__ void __launch_bounds__(kBlock) patch_kernel(
         ++k;
         if (head == v) {
             ++cur;
-            head = cur < end ? ri[cur] : INT_MAX;
+            head = cur < end ? src[cur] : INT_MAX;
         }
     }
     if (kFill) {
