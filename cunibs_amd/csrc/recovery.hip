@@ -623,19 +623,19 @@ __global__ void hpr_grad_kernel(const double* __restrict__ X, int ground, const 
     dst[2 * K] = static_cast<float>(-a2 - static_cast<double>(da[2 * Kn]));
 }
 
-// Slot of (node, tag), slots being ordered by node then tag; -1 if there is none.
-__device__ int find_slot(const std::int32_t* __restrict__ slot_node, const std::int32_t* __restrict__ slot_tag,
-                         std::int64_t n_slots, int node, int tag) {
-    std::int64_t lo = 0, hi = n_slots;
-    while (lo < hi) {
-        const std::int64_t mid = lo + ((hi - lo) >> 1);
-        if (slot_node[mid] < node)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    for (; lo < n_slots && slot_node[lo] == node; ++lo)
-        if (slot_tag[lo] == tag) return static_cast<int>(lo);
+// Slots per node (slots are ordered by node then tag, so node v's slots are the range
+// [node_slots[v], node_slots[v + 1]) of the exclusive scan of these counts).
+__global__ void slots_per_node_kernel(const std::int32_t* __restrict__ slot_node, std::int64_t n_slots,
+                                      std::int32_t* __restrict__ counts) {
+    const std::int64_t s = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s < n_slots) atomicAdd(&counts[slot_node[s]], 1);
+}
+
+// Slot of (node, tag); -1 if there is none.
+__device__ int find_slot(const std::int32_t* __restrict__ node_slots, const std::int32_t* __restrict__ slot_tag,
+                         int node, int tag) {
+    for (int s = node_slots[node]; s < node_slots[node + 1]; ++s)
+        if (slot_tag[s] == tag) return s;
     return -1;
 }
 
@@ -655,7 +655,7 @@ constexpr int kPatchStage = 1024;  // ints of staged rings per wavefront
 template <bool kFill>
 __global__ void __launch_bounds__(kBlock) patch_kernel(
     const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ri,
-    const std::int32_t* __restrict__ slot_node, const std::int32_t* __restrict__ slot_tag, std::int64_t n_slots,
+    const std::int32_t* __restrict__ node_slots, const std::int32_t* __restrict__ slot_tag, std::int64_t n_slots,
     std::int32_t* __restrict__ counts_or_ptr, std::int32_t* __restrict__ pidx, int* __restrict__ missing) {
     const int lane = threadIdx.x & (kWave - 1);
     const std::int64_t slot = static_cast<std::int64_t>(blockIdx.x) * (kBlock / kWave) +
@@ -674,7 +674,7 @@ __global__ void __launch_bounds__(kBlock) patch_kernel(
     int* const st = stage[threadIdx.x / kWave];
     int cur = 0, end = 0;
     if (lane < len) {
-        const int nb = find_slot(slot_node, slot_tag, n_slots, ri[b + lane], slot_tag[slot]);
+        const int nb = find_slot(node_slots, slot_tag, ri[b + lane], slot_tag[slot]);
         if (nb < 0) {
             if (!kFill) atomicOr(missing, 1);
         } else {
@@ -733,13 +733,18 @@ void build_harmonic(RecoveryOp& op, const double* nodes, const std::int32_t* tet
     // First ring of each slot: the distinct nodes of its own tetrahedra, sorted.
     DeviceBuffer<std::int32_t> r1_ptr, r1_idx;
     stiffness_pattern(tets, op.ptr.get(), op.idx.get(), ns, r1_ptr, r1_idx, s);
+    DeviceBuffer<std::int32_t> per_node(static_cast<std::size_t>(n + 1), "slots per node"),
+        node_slots(static_cast<std::size_t>(n + 1), "node slot ranges");
+    check(hipMemsetAsync(per_node.get(), 0, sizeof(std::int32_t) * (n + 1), s), "memset");
+    if (const unsigned g = grid_for(ns)) slots_per_node_kernel<<<g, kBlock, 0, s>>>(op.slot_node.get(), ns, per_node.get());
+    exclusive_scan_i32(per_node.get(), node_slots.get(), n + 1, s);
     DeviceBuffer<std::int32_t> counts(static_cast<std::size_t>(ns + 1), "patch sizes");
     DeviceBuffer<int> missing(1, "missing neighbour slot");
     check(hipMemsetAsync(counts.get() + ns, 0, sizeof(std::int32_t), s), "memset");
     check(hipMemsetAsync(missing.get(), 0, sizeof(int), s), "memset");
     const unsigned g_slots = grid_for(ns, kBlock / kWave);
     if (g_slots)
-        patch_kernel<false><<<g_slots, kBlock, 0, s>>>(r1_ptr.get(), r1_idx.get(), op.slot_node.get(),
+        patch_kernel<false><<<g_slots, kBlock, 0, s>>>(r1_ptr.get(), r1_idx.get(), node_slots.get(),
                                                        op.slot_tag.get(), ns, counts.get(), nullptr, missing.get());
     op.pptr.reset(static_cast<std::size_t>(ns + 1), "patch ptr");
     exclusive_scan_i32(counts.get(), op.pptr.get(), ns + 1, s);
@@ -752,7 +757,7 @@ void build_harmonic(RecoveryOp& op, const double* nodes, const std::int32_t* tet
     op.nnz_patch = nnz;
     op.pidx.reset(static_cast<std::size_t>(std::max(nnz, 1)), "patch nodes");
     if (g_slots)
-        patch_kernel<true><<<g_slots, kBlock, 0, s>>>(r1_ptr.get(), r1_idx.get(), op.slot_node.get(),
+        patch_kernel<true><<<g_slots, kBlock, 0, s>>>(r1_ptr.get(), r1_idx.get(), node_slots.get(),
                                                       op.slot_tag.get(), ns, op.pptr.get(), op.pidx.get(), nullptr);
     op.w3.reset(static_cast<std::size_t>(3 * std::max(nnz, 1)), "harmonic weights");
     op.status.reset(static_cast<std::size_t>(std::max<std::int64_t>(ns, 1)), "harmonic status");
