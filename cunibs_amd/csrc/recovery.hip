@@ -575,29 +575,26 @@ __global__ void __launch_bounds__(kFitWaves * kWave) __attribute__((amdgpu_waves
 // E_slots[s, d, c] = -sum_p w3[p, d] v[pidx[p], c] - dA/dt[node, d, c], FP64 accumulation in patch
 // order; v is the reduced potential with the ground node's zero restored.
 constexpr int kGradBatch = 8;
-__global__ void hpr_grad_kernel(const double* __restrict__ X, int ground, const float* __restrict__ w3,
-                                const std::int32_t* __restrict__ pptr, const std::int32_t* __restrict__ pidx,
-                                const std::int32_t* __restrict__ slot_node, const float* __restrict__ dadt, int Kn,
-                                std::int64_t n_slots, int K, float* __restrict__ slots) {
-    const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= n_slots * K) return;
-    const std::int64_t s = t / K;
-    const int c = static_cast<int>(t % K);
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    const int b = pptr[s], e = pptr[s + 1];
-    // Entries go in batches: the batch's indices and weights, then its potentials, are all loaded
-    // before the first use (an entry past the patch's end reads the last entry again and is not
-    // added), so a patch costs a few memory round trips instead of one per entry.
+constexpr int kGradStage = 2048;  // patch entries a block stages in LDS (32 KB with their weights)
+
+// a0..a2 += sum over the patch entries [b, e) of w3[p, d] v[pidx[p], c], in order. Entries go in
+// batches: the batch's indices and weights, then its potentials, are all loaded before the first
+// use (an entry past the patch's end reads the last entry again and is not added), so a patch
+// costs a few memory round trips instead of one per entry. `idx` and `w` are either the global
+// arrays (base 0) or the block's LDS copy of its entries from `base` on.
+__device__ __forceinline__ void hpr_accumulate(const double* __restrict__ X, int ground, int K, int c, int b, int e,
+                                               int base, const std::int32_t* idx, const float* w, double& a0,
+                                               double& a1, double& a2) {
     for (int p = b; p < e; p += kGradBatch) {
         int mm[kGradBatch];
-        float w[kGradBatch][3];
+        float wv[kGradBatch][3];
 #pragma unroll
         for (int u = 0; u < kGradBatch; ++u) {
-            const std::int64_t pp = min(p + u, e - 1);
-            mm[u] = pidx[pp];
-            w[u][0] = w3[3 * pp];
-            w[u][1] = w3[3 * pp + 1];
-            w[u][2] = w3[3 * pp + 2];
+            const std::int64_t pp = min(p + u, e - 1) - base;
+            mm[u] = idx[pp];
+            wv[u][0] = w[3 * pp];
+            wv[u][1] = w[3 * pp + 1];
+            wv[u][2] = w[3 * pp + 2];
         }
         double vm[kGradBatch];
 #pragma unroll
@@ -609,12 +606,42 @@ __global__ void hpr_grad_kernel(const double* __restrict__ X, int ground, const 
 #pragma unroll
         for (int u = 0; u < kGradBatch; ++u) {
             if (p + u < e) {
-                a0 = fma(static_cast<double>(w[u][0]), vm[u], a0);
-                a1 = fma(static_cast<double>(w[u][1]), vm[u], a1);
-                a2 = fma(static_cast<double>(w[u][2]), vm[u], a2);
+                a0 = fma(static_cast<double>(wv[u][0]), vm[u], a0);
+                a1 = fma(static_cast<double>(wv[u][1]), vm[u], a1);
+                a2 = fma(static_cast<double>(wv[u][2]), vm[u], a2);
             }
         }
     }
+}
+
+// One thread per (slot, column); a block takes kBlock / K consecutive slots, whose patches are one
+// contiguous run of entries. The block copies that run's indices and weights into LDS once
+// (coalesced) when it fits, instead of each of a slot's K threads loading every entry.
+__global__ void __launch_bounds__(kBlock) hpr_grad_kernel(
+    const double* __restrict__ X, int ground, const float* __restrict__ w3, const std::int32_t* __restrict__ pptr,
+    const std::int32_t* __restrict__ pidx, const std::int32_t* __restrict__ slot_node, const float* __restrict__ dadt,
+    int Kn, std::int64_t n_slots, int K, float* __restrict__ slots) {
+    __shared__ std::int32_t sidx[kGradStage];
+    __shared__ float sw[3 * kGradStage];
+    const int per = kBlock / K;  // K divides kBlock
+    const std::int64_t s0 = static_cast<std::int64_t>(blockIdx.x) * per;
+    const std::int64_t s1 = min(s0 + per, n_slots);
+    const int pb = pptr[s0], pe = pptr[s1];
+    const bool staged = pe - pb <= kGradStage;  // block-uniform
+    if (staged) {
+        for (int i = threadIdx.x; i < pe - pb; i += kBlock) sidx[i] = pidx[pb + i];
+        for (int i = threadIdx.x; i < 3 * (pe - pb); i += kBlock) sw[i] = w3[3 * static_cast<std::int64_t>(pb) + i];
+    }
+    __syncthreads();
+    const std::int64_t s = s0 + threadIdx.x / K;
+    const int c = static_cast<int>(threadIdx.x % K);
+    if (s >= n_slots) return;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    const int b = pptr[s], e = pptr[s + 1];
+    if (staged)
+        hpr_accumulate(X, ground, K, c, b, e, pb, sidx, sw, a0, a1, a2);
+    else
+        hpr_accumulate(X, ground, K, c, b, e, 0, pidx, w3, a0, a1, a2);
     const std::int64_t node = slot_node[s];
     const float* da = dadt + node * 3 * Kn + (Kn == 1 ? 0 : c);
     float* dst = slots + s * 3 * K + c;
