@@ -531,3 +531,17 @@ def test_dense_tail_matches_the_level_by_level_vcycle(block8, gctx, coil):
     for a, b in zip(res, plain):
         assert abs(a.iterations - b.iterations) <= 1
         assert rel(a.magnE, b.magnE) < 2e-5
+
+
+@pytest.mark.parametrize("K", [1, 4, 32])
+def test_dense_tail_at_other_widths(gctx, K):
+    """The dense tail's matrix-core product at widths below 16 (columns past K masked) and at 32
+    (two 16-column tiles) agrees with the level-by-level cycle."""
+    X = np.random.default_rng(40 + K).standard_normal((gctx.n_nodes - 1, K)).astype(np.float32)
+    with_tail = gctx.vcycle(X)
+    gctx.set_tuning(dense_tail=0)
+    try:
+        level_by_level = gctx.vcycle(X)
+    finally:
+        gctx.set_tuning(dense_tail=1)
+    assert rel(with_tail, level_by_level) < 2e-6

@@ -108,12 +108,14 @@ def test_gpu_harmonic_operator_matches_the_oracle(gctx8, oracle_mod):
 
 
 @pytest.mark.gpu
-def test_gpu_harmonic_field_matches_the_oracle(gctx8, oracle_mod, coil):
+@pytest.mark.parametrize("k", [3, 16])
+def test_gpu_harmonic_field_matches_the_oracle(gctx8, oracle_mod, coil, k):
     """The block solve with harmonic recovery against the oracle's apply on the GPU's own
-    potential and nodal dA/dt."""
+    potential and nodal dA/dt; at k = 3 each workgroup's patch entries exceed its LDS stage and
+    are read from global memory, at k = 16 they are staged."""
     from cunibs_amd import fem, synthetic
     ctx = gctx8
-    sites = synthetic.cap_sites(3, seed=6)
+    sites = synthetic.cap_sites(k, seed=6)
     extra = {"dadt_nodes": 1}
     res = fem.solve_placements_block(ctx, coil.positions_m, coil.moments, sites, 1e6, potential=True,
                                      vectors=True, recovery="harmonic", slots=True, extra=extra)
