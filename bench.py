@@ -123,7 +123,8 @@ def sweep_only(ctx, coil, sites, args) -> int:
     k = args.block_k
     ctx.set_tuning(degree=2, lower_ratio=8.0, fine_sub=max(args.fine_sub, 0), coil_blocks=args.coil_blocks)
     pos, mom = coil.positions_m, coil.moments
-    run = lambda n: fem.solve_sweep(ctx, pos, mom, [sites[j % len(sites)] for j in range(n * k)], 1e6, k)
+    run = lambda n: fem.solve_sweep(ctx, pos, mom, [sites[j % len(sites)] for j in range(n * k)], 1e6, k,
+                                    recovery=args.recovery)
     ctx.timing_enable(True)
     run(args.warmup)
     ctx.timing_enable(True)  # reset the in-solve sums
@@ -134,7 +135,8 @@ def sweep_only(ctx, coil, sites, args) -> int:
     n_red, nnz = int(ctx.info.n_reduced), int(ctx.info.nnz_reduced)
     K = k if k in fem.BLOCK_SIZES else min(b for b in fem.BLOCK_SIZES if b >= k)
     print(json.dumps({
-        "sweep_placements_per_s": round(args.steps * k / elapsed, 2), "block_k": k, "tuning": ctx.tuning,
+        "sweep_placements_per_s": round(args.steps * k / elapsed, 2), "block_k": k, "recovery": args.recovery,
+        "tuning": ctx.tuning,
         "iterations_mean": round(float(np.mean(res["iterations"])), 2),
         "spmm_us": round(timing["spmm_ms"] * 1e3, 2), "spmm_samples": timing["samples"],
         "spmm_algorithmic_bytes": spmm_diff_bytes(n_red, nnz, K),
@@ -432,6 +434,8 @@ def main() -> int:
     ap.add_argument("--sweep-only", action="store_true",
                     help="only the pipelined cold sweep at --block-k, --fine-sub and --coil-blocks (no tuning, no "
                          "blocks solved alone), so that a kernel trace's averages are the in-sweep ones")
+    ap.add_argument("--recovery", default="harmonic", choices=("raw", "spr_global", "spr_tissue", "harmonic"),
+                    help="--sweep-only: the field recovery of the sweep (the headline's is harmonic)")
     ap.add_argument("--coil-blocks", type=int, default=256,
                     help="--sweep-only: grid cap of the overlapped dA/dt")
     args = ap.parse_args()

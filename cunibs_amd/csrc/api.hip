@@ -115,12 +115,13 @@ struct cn_context {
     // Pinned sources of the host-to-device copies a block issues without waiting on them: the
     // packed dipoles (coil stage, either stream) and the block's own small tables.
     cn::PinnedStage stage_coil, stage_blk;
-    // Pipelined sweeps (cn_solve_sweep): the next block's coil stage (dA/dt, element pass, RHS)
-    // runs on s2 into the second buffer set while the current block solves on s.
+    // Pipelined sweeps (cn_solve_sweep): the coil stages (dA/dt, element pass, RHS) of the next
+    // two blocks run on s2 into their own buffer sets while the current block solves, recovers its
+    // field and reduces its metrics on s.
     hipStream_t s2 = nullptr;
     int coil_blocks = 0;  // grid cap of the overlapped dA/dt (0: the full grid)
-    hipEvent_t coil_done[2] = {nullptr, nullptr};
-    cn::DeviceBuffer<float> d_dadt_nodes2, d_B2;
+    hipEvent_t coil_done[3] = {nullptr, nullptr, nullptr};
+    cn::DeviceBuffer<float> d_dadt_nodes2, d_B2, d_dadt_nodes3, d_B3;
     // recovery operators, built on first use (index: CN_RECOVERY_*), and their block buffers
     std::unique_ptr<cn::RecoveryOp> recovery[4];
     cn::DeviceBuffer<float> d_slots, d_Eraw;
@@ -1336,17 +1337,21 @@ void solve_sweep(cn_context& c, const double* pos_m, const double* mom, std::int
         if (e == nullptr) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "coil event");
     c.d_dadt_nodes.ensure(3 * n * K, "dadt nodes");
     c.d_dadt_nodes2.ensure(3 * n * K, "dadt nodes");
+    c.d_dadt_nodes3.ensure(3 * n * K, "dadt nodes");
     c.d_B.ensure(nr * K, "rhs block");
     c.d_B2.ensure(nr * K, "rhs block");
+    c.d_B3.ensure(nr * K, "rhs block");
     c.d_q.ensure(4 * m * K, "corner terms");
-    float* const nodal[2] = {c.d_dadt_nodes.get(), c.d_dadt_nodes2.get()};
-    float* const rhs[2] = {c.d_B.get(), c.d_B2.get()};
+    // Block b uses buffer set b % 3: the coil stage of block b + 2 fills its set while block b
+    // still reads its own (the nodal dA/dt in the field recovery) and block b + 1's waits.
+    float* const nodal[3] = {c.d_dadt_nodes.get(), c.d_dadt_nodes2.get(), c.d_dadt_nodes3.get()};
+    float* const rhs[3] = {c.d_B.get(), c.d_B2.get(), c.d_B3.get()};
     const float scale = static_cast<float>(didt) * static_cast<float>(kMu0Over4Pi);
     const int n_blocks = (n_place + block_k - 1) / block_k;
     // Each block is laid out at its own padded width (a ragged last block may be narrower), as
     // the solver takes it.
     auto coil_stage = [&](int b) {
-        const int first = b * block_k, k = std::min(block_k, n_place - first), buf = b % 2;
+        const int first = b * block_k, k = std::min(block_k, n_place - first), buf = b % 3;
         const int Kb = padded_width(k);
         dadt_for_frames(c, tf.data() + 16 * static_cast<std::size_t>(first), k, Kb, pos_m, mom, n_dip, scale,
                         nodal[buf], c.s2, c.coil_blocks);
@@ -1372,10 +1377,13 @@ void solve_sweep(cn_context& c, const double* pos_m, const double* mom, std::int
     c.d_magn.ensure(m * K, "magnE");
     const hipStream_t s = c.s;
     try {
+        // Two coil stages ahead: block b + 2's is queued once block b - 1's outputs are read back
+        // (the end of the previous pass), which frees its buffer set.
         coil_stage(0);
+        if (n_blocks > 1) coil_stage(1);
         for (int b = 0; b < n_blocks; ++b) {
-            if (b + 1 < n_blocks) coil_stage(b + 1);
-            const int first = b * block_k, k = std::min(block_k, n_place - first), buf = b % 2;
+            if (b + 2 < n_blocks) coil_stage(b + 2);
+            const int first = b * block_k, k = std::min(block_k, n_place - first), buf = b % 3;
             const int Kb = padded_width(k);
             check(hipStreamWaitEvent(s, c.coil_done[buf], 0), "wait for the coil stage");
             const double* dx0 = nullptr;
