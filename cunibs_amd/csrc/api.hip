@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <future>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -122,6 +123,11 @@ struct cn_context {
     int coil_blocks = 0;  // grid cap of the overlapped dA/dt (0: the full grid)
     hipEvent_t coil_done[3] = {nullptr, nullptr, nullptr};
     cn::DeviceBuffer<float> d_dadt_nodes2, d_B2, d_dadt_nodes3, d_B3;
+    // ... and the field recovery and metrics of each solved block run on s3 (a worker thread drives
+    // them) while the next block solves; the solution is double-buffered for that.
+    hipStream_t s3 = nullptr;
+    hipEvent_t solve_done[2] = {nullptr, nullptr};
+    cn::DeviceBuffer<double> d_X2;
     // recovery operators, built on first use (index: CN_RECOVERY_*), and their block buffers
     std::unique_ptr<cn::RecoveryOp> recovery[4];
     cn::DeviceBuffer<float> d_slots, d_Eraw;
@@ -196,7 +202,10 @@ struct cn_context {
             if (e) (void)hipEventDestroy(e);
         for (auto& e : coil_done)
             if (e) (void)hipEventDestroy(e);
+        for (auto& e : solve_done)
+            if (e) (void)hipEventDestroy(e);
         solver.reset();
+        if (s3) (void)hipStreamDestroy(s3);
         if (s2) (void)hipStreamDestroy(s2);
         if (s) (void)hipStreamDestroy(s);
     }
@@ -883,7 +892,7 @@ std::unique_ptr<MatchedSolve> matched_solve(cn_context& c, const double* sigma_r
 // built for its own operator and then discarded (uq/conductivity/run.py:333-342). The block's X
 // and the columns' statistics are updated in place; a column that misses again stays missed.
 void retry_missed(cn_context& c, cn::SolveStats& st, const cn::SolveOptions& opt, const float* B, int k, int K,
-                  const double* sigma) {
+                  const double* sigma, double* X) {
     using namespace cn;
     std::vector<int> missed;
     for (int i = 0; i < k; ++i)
@@ -904,7 +913,7 @@ void retry_missed(cn_context& c, cn::SolveStats& st, const cn::SolveOptions& opt
     const int T = static_cast<int>(c.tissue.tags.size());
     for (int col : missed) {
         column_get_kernel<float><<<grid_for(nr), kBlock, 0, s>>>(B, nr, K, col, b1.get());
-        column_get_kernel<double><<<grid_for(nr), kBlock, 0, s>>>(c.d_X.get(), nr, K, col, x0.get());
+        column_get_kernel<double><<<grid_for(nr), kBlock, 0, s>>>(X, nr, K, col, x0.get());
         SolveStats r;
         if (sigma != nullptr) {
             auto ms = matched_solve(c, sigma + static_cast<std::int64_t>(col) * T);
@@ -913,7 +922,7 @@ void retry_missed(cn_context& c, cn::SolveStats& st, const cn::SolveOptions& opt
         } else {
             r = c.solver->solve(b1.get(), x1.get(), x0.get(), 1, opt);
         }
-        column_put_kernel<double><<<grid_for(nr), kBlock, 0, s>>>(x1.get(), nr, K, col, c.d_X.get());
+        column_put_kernel<double><<<grid_for(nr), kBlock, 0, s>>>(x1.get(), nr, K, col, X);
         st.relative_residual[col] = r.relative_residual[0];
         st.column_iterations[col] = (st.column_iterations[col] ? st.column_iterations[col] : st.iterations) +
                                     r.iterations;
@@ -987,7 +996,7 @@ void uq_solve(cn_context& c, const double* sigma, int k, const cn_solve_options*
         throw;
     }
     c.solver->set_multi(nullptr);
-    retry_missed(c, st, opt, c.d_B.get(), k, K, u.sigma.get());
+    retry_missed(c, st, opt, c.d_B.get(), k, K, u.sigma.get(), c.d_X.get());
     // |E| per sample (dA/dt is the same for every sample), recovered before any statistic is
     // taken (uq/conductivity/run.py:340-352).
     const int mode = sopt ? sopt->recovery : CN_RECOVERY_RAW;
@@ -1216,10 +1225,10 @@ void solve_block(cn_context& c, const double* pos_m, const double* mom, std::int
             throw;
         }
         c.solver->set_multi(nullptr);
-        retry_missed(c, st, opt, c.d_B.get(), k, K, c.blk_sigma.get());
+        retry_missed(c, st, opt, c.d_B.get(), k, K, c.blk_sigma.get(), c.d_X.get());
     } else {
         st = c.solver->solve(c.d_B.get(), c.d_X.get(), dx0, k, opt);
-        retry_missed(c, st, opt, c.d_B.get(), k, K, nullptr);
+        retry_missed(c, st, opt, c.d_B.get(), k, K, nullptr, c.d_X.get());
     }
     if (sopt && sopt->collect) c.warm.append(c.d_X.get(), k, K, s);
     c.mark_phase(5, s);
@@ -1374,15 +1383,50 @@ void solve_sweep(cn_context& c, const double* pos_m, const double* mom, std::int
     const int mode = sopt ? sopt->recovery : CN_RECOVERY_RAW;
     const RecoveryOp* rec = mode == CN_RECOVERY_RAW ? nullptr : ensure_recovery(c, mode);
     c.d_X.ensure(nr * K, "solution");
+    c.d_X2.ensure(nr * K, "solution");
+    double* const sol[2] = {c.d_X.get(), c.d_X2.get()};
+    // The block outputs run on s3, driven by a worker thread, while the next block solves: every
+    // buffer they write is theirs alone, allocated here before any of them starts.
     c.d_magn.ensure(m * K, "magnE");
-    const hipStream_t s = c.s;
+    if (rec != nullptr) c.d_slots.ensure(3 * rec->n_slots * K, "recovered slots");
+    if (rec != nullptr && rec->mode != kRecoveryHarmonic) c.d_Eraw.ensure(3 * m * K, "raw E");
+    if (c.s3 == nullptr) check(hipStreamCreateWithFlags(&c.s3, hipStreamNonBlocking), "create stream");
+    for (auto& e : c.solve_done)
+        if (e == nullptr) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "solve event");
+    const hipStream_t s = c.s, s3 = c.s3;
+    // Block b's outputs: its field from its solution and nodal dA/dt, then its region metrics, on
+    // s3 after its solve; the worker waits for them before it returns.
+    int device = 0;
+    check(hipGetDevice(&device), "device");
+    auto outputs = [&c, rec, m, K, s3, metrics_out, &nodal, &sol, device](int b, int first, int k, int Kb, int buf) {
+        check(hipSetDevice(device), "device");  // the worker thread works on the context's device
+        check(hipStreamWaitEvent(s3, c.solve_done[b % 2], 0), "wait for the solve");
+        const double* X = sol[b % 2];
+        if (rec == nullptr) {
+            reconstruct(X, c.tets.get(), c.g32.get(), nodal[buf], Kb, m, k, Kb, c.ground, nullptr, c.d_magn.get(), s3);
+        } else if (rec->mode == kRecoveryHarmonic) {
+            apply_harmonic(*rec, X, c.ground, nodal[buf], Kb, m, k, Kb, c.d_slots.get(), nullptr, c.d_magn.get(), s3);
+        } else {
+            reconstruct(X, c.tets.get(), c.g32.get(), nodal[buf], Kb, m, k, Kb, c.ground, c.d_Eraw.get(),
+                        c.d_magn.get(), s3);
+            apply_spr(*rec, c.d_Eraw.get(), m, k, Kb, c.d_slots.get(), nullptr, c.d_magn.get(), s3);
+        }
+        if (metrics_out)
+            region_metrics(c.d_magn.get(), m, k, Kb, c.region_idx.get(), c.n_region, c.vols32.get(), c.bary.get(),
+                           c.metrics_ws, metrics_out + static_cast<std::size_t>(kMetricsLen) * first, s3);
+        sync(s3, "block outputs");
+    };
+    std::future<void> pending;  // the previous block's outputs
+    auto join = [&pending] {
+        if (pending.valid()) pending.get();  // rethrows the worker's error
+    };
     try {
-        // Two coil stages ahead: block b + 2's is queued once block b - 1's outputs are read back
-        // (the end of the previous pass), which frees its buffer set.
+        // Two coil stages ahead. Block b + 2's is queued after block b has solved and block b - 1's
+        // outputs are done, which frees its buffer set; block b's solve writes the solution buffer
+        // block b - 2's outputs read, done by then too.
         coil_stage(0);
         if (n_blocks > 1) coil_stage(1);
         for (int b = 0; b < n_blocks; ++b) {
-            if (b + 2 < n_blocks) coil_stage(b + 2);
             const int first = b * block_k, k = std::min(block_k, n_place - first), buf = b % 3;
             const int Kb = padded_width(k);
             check(hipStreamWaitEvent(s, c.coil_done[buf], 0), "wait for the coil stage");
@@ -1392,26 +1436,12 @@ void solve_sweep(cn_context& c, const double* pos_m, const double* mom, std::int
                 c.warm.initial_guess(rhs[buf], k, Kb, c.d_x0.get(), s);
                 dx0 = c.d_x0.get();
             }
-            SolveStats st = c.solver->solve(rhs[buf], c.d_X.get(), dx0, k, opt);
-            retry_missed(c, st, opt, rhs[buf], k, Kb, nullptr);
-            if (rec == nullptr) {
-                reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), nodal[buf], Kb, m, k, Kb, c.ground, nullptr,
-                            c.d_magn.get(), s);
-            } else if (rec->mode == kRecoveryHarmonic) {
-                c.d_slots.ensure(3 * rec->n_slots * K, "recovered slots");
-                apply_harmonic(*rec, c.d_X.get(), c.ground, nodal[buf], Kb, m, k, Kb, c.d_slots.get(), nullptr,
-                               c.d_magn.get(), s);
-            } else {
-                c.d_Eraw.ensure(3 * m * K, "raw E");
-                c.d_slots.ensure(3 * rec->n_slots * K, "recovered slots");
-                reconstruct(c.d_X.get(), c.tets.get(), c.g32.get(), nodal[buf], Kb, m, k, Kb, c.ground,
-                            c.d_Eraw.get(), c.d_magn.get(), s);
-                apply_spr(*rec, c.d_Eraw.get(), m, k, Kb, c.d_slots.get(), nullptr, c.d_magn.get(), s);
-            }
-            if (metrics_out)
-                region_metrics(c.d_magn.get(), m, k, Kb, c.region_idx.get(), c.n_region, c.vols32.get(), c.bary.get(),
-                               c.metrics_ws, metrics_out + static_cast<std::size_t>(kMetricsLen) * first, s);
-            sync(s, "block outputs");
+            SolveStats st = c.solver->solve(rhs[buf], sol[b % 2], dx0, k, opt);
+            retry_missed(c, st, opt, rhs[buf], k, Kb, nullptr, sol[b % 2]);
+            check(hipEventRecord(c.solve_done[b % 2], s), "solve event");
+            join();
+            if (b + 2 < n_blocks) coil_stage(b + 2);
+            pending = std::async(std::launch::async, outputs, b, first, k, Kb, buf);
             for (int i = 0; i < k; ++i) {
                 if (iters_out) iters_out[first + i] = st.column_iterations[i];
                 if (resid_out) resid_out[first + i] = st.relative_residual[i];
@@ -1429,9 +1459,12 @@ void solve_sweep(cn_context& c, const double* pos_m, const double* mom, std::int
                 }
             }
         }
+        join();
     } catch (...) {
-        (void)hipStreamSynchronize(c.s2);  // the next coil stage may still be writing its buffers
+        if (pending.valid()) pending.wait();  // the outputs in flight read this sweep's buffers
+        (void)hipStreamSynchronize(c.s2);     // the next coil stages may still be writing theirs
         (void)hipStreamSynchronize(s);
+        (void)hipStreamSynchronize(s3);
         throw;
     }
 }
