@@ -1358,12 +1358,13 @@ void solve_sweep(cn_context& c, const double* pos_m, const double* mom, std::int
     const float scale = static_cast<float>(didt) * static_cast<float>(kMu0Over4Pi);
     const int n_blocks = (n_place + block_k - 1) / block_k;
     // Each block is laid out at its own padded width (a ragged last block may be narrower), as
-    // the solver takes it.
+    // the solver takes it. The first block's N-body overlaps nothing (the first solve waits for
+    // it), so it takes the full grid; later ones take the capped grid beside a solve.
     auto coil_stage = [&](int b) {
         const int first = b * block_k, k = std::min(block_k, n_place - first), buf = b % 3;
         const int Kb = padded_width(k);
         dadt_for_frames(c, tf.data() + 16 * static_cast<std::size_t>(first), k, Kb, pos_m, mom, n_dip, scale,
-                        nodal[buf], c.s2, c.coil_blocks);
+                        nodal[buf], c.s2, b == 0 ? 0 : c.coil_blocks);
         element_pass(nodal[buf], c.tets.get(), c.g32.get(), c.neg_vc.get(), m, k, Kb, nullptr, c.d_q.get(),
                      c.corner_pos.get(), c.s2);
         node_rhs(c.d_q.get(), c.n2c_ptr.get(), n, k, Kb, c.ground, rhs[buf], nullptr, c.s2);
