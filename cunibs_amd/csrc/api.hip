@@ -1399,7 +1399,7 @@ void solve_sweep(cn_context& c, const double* pos_m, const double* mom, std::int
     // s3 after its solve; the worker waits for them before it returns.
     int device = 0;
     check(hipGetDevice(&device), "device");
-    auto outputs = [&c, rec, m, K, s3, metrics_out, &nodal, &sol, device](int b, int first, int k, int Kb, int buf) {
+    auto outputs = [&c, rec, m, s3, metrics_out, &nodal, &sol, device](int b, int first, int k, int Kb, int buf) {
         check(hipSetDevice(device), "device");  // the worker thread works on the context's device
         check(hipStreamWaitEvent(s3, c.solve_done[b % 2], 0), "wait for the solve");
         const double* X = sol[b % 2];

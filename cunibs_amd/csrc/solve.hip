@@ -2,19 +2,20 @@
 // same systems as k lockstep scalar chains; this is a true block method instead).
 //
 // Precision. The operator, the V-cycle and every (n, K) block vector are FP32. FP64 holds the
-// solution X, the K x K Gram matrices and their pivoted Cholesky, the block coefficients, and the
+// solution X, the K x K Gram matrices and their pivoted inverses, the block coefficients, and the
 // true residual B - A X that an outer iterative-refinement loop recomputes with the FP64 operator.
 // Convergence is judged on that true residual, per column, against ||b|| (solver.cpp:237-246).
 //
 // Iteration, written on the pair (W, S = A W) so that no FP32 rounding of a derived search block
 // enters the recursive residual:
 //     S = A W                         (differenced SpMM, fused with G = W^T S and H = W^T R)
-//     T: pivoted Cholesky of G, T^T G T = I on the retained directions (P = W T is A-orthonormal)
-//     a = T T^T H                     (K x K; zero in the columns that have frozen)
+//     TT = G^-1 on the retained directions (diagonal pivoting, pivots below drop_tol dropped)
+//     a = TT H                        (K x K; zero in the columns that have frozen)
 //     X += W a,  R -= S a
 //     Z = M R                         (one V-cycle)
-//     E = S^T Z,  W <- Z - W (T T^T E)
-// This is block CG with A-orthonormal blocks P = W T (X += P (P^T R), W' = Z - P (P^T A Z)),
+//     E = S^T Z,  W <- Z - W (TT E)
+// This is block CG with A-orthonormal blocks P = W T for any T with T T^T = TT
+// (X += P (P^T R), W' = Z - P (P^T A Z)),
 // breakdown-free because directions whose Schur pivot falls below drop_tol are dropped.
 //
 // Every FP64 reduction is a fixed two-stage tree: kGramBlocks contiguous row chunks, each summed
@@ -1131,120 +1132,103 @@ constexpr int small_threads() {
     return K * K <= kWave ? kWave : K * K;
 }
 
-// G, H -> T (pivoted Cholesky of the unit-diagonal scaling of G), TT = T T^T, a = TT H with the
-// frozen columns zeroed. One wavefront: the factorisation is left-looking, lane i owns row i and
-// keeps its Schur-complement diagonal in a register, so the pivot is a wave-wide max and each step
-// needs one barrier (a single wavefront's barrier is only an LDS ordering point).
+// G, H -> TT, a = TT H with the frozen columns zeroed. TT is the inverse of G on the retained
+// directions and zero elsewhere. G = W^T A W is symmetric, but its FP64 sums of FP32 products are
+// not exactly (an entry small against |W| |A W| keeps a large relative rounding error on each
+// side), so the kernel works on the symmetric part (G + G^T) / 2, with every product formed in an
+// order that keeps the matrix bitwise symmetric through the sweeps: TT then stays symmetric
+// positive definite, as the block recurrence needs. With S = diag(G)^-1/2 and A = S G S, the
+// directions are chosen one at a time as the largest remaining Schur-complement diagonal of A
+// (lowest index on ties) while it exceeds drop_tol, which is the pivot order and drop rule of a
+// diagonally pivoted Cholesky of A, and TT = S (A_PP)^-1 S on the chosen set P. A_PP^-1 comes from
+// the symmetric sweep operator (Goodnight, "A tutorial on the SWEEP operator", 1979): sweeping
+// pivot p maps a_pp -> -1/a_pp, a_ip -> a_ip/a_pp, a_ij -> a_ij - a_ip a_pj / a_pp, so after the
+// sweeps the P block holds -A_PP^-1 and the unswept diagonal holds the Schur complements the next
+// pivot is chosen from. One wavefront; lane l keeps entries l, l + 64, ... in registers, and a step
+// is a pivot scan of the diagonal (every lane reads it from LDS and finds the same pivot), one
+// division, the pivot row through LDS, and one fused update per entry.
 template <int K>
-__global__ void __launch_bounds__(kWave) coef_kernel(const double* __restrict__ partials, int nb,
-                                                     double drop_tol, double* __restrict__ G_out,
-                                                     double* __restrict__ T, double* __restrict__ TT,
-                                                     double* __restrict__ AL, int* __restrict__ state,
-                                                     int independent, int* __restrict__ ran) {
-    static_assert(K <= kWave, "one lane per row");
+__global__ void __launch_bounds__(kWave) coef_kernel(const double* __restrict__ partials, double drop_tol,
+                                                     double* __restrict__ TT, double* __restrict__ AL,
+                                                     int* __restrict__ state, int independent, int* __restrict__ ran) {
+    static_assert(K <= 32, "the pivot sets are 32-bit masks");
     if (finished(state)) return;
     if (threadIdx.x == 0) *ran = 1;  // this window slot's coefficients are to be applied
     constexpr int kE = K * K;
-    __shared__ double A[kE];   // G scaled to unit diagonal
+    constexpr int U = (kE + kWave - 1) / kWave;
     __shared__ double H[kE];
-    __shared__ double Lc[kE];  // Lc[j * K + i]: column j of L at original row i
-    __shared__ double Tm[kE];  // T, then T T^T
+    __shared__ double M[kE];  // TT, for the product with H
+    __shared__ double prow[K];
+    __shared__ double diag[K];
     __shared__ double sc[K];
-    __shared__ int idx[K];
-    load_reduced<kE>(partials, 0, A);
-    load_reduced<kE>(partials, kE, H);
     const int t = threadIdx.x;
-    if (independent) {  // K separate systems: each column is its own CG chain
-        for (int e = t; e < kE; e += kWave)
-            if ((e / K) != (e % K)) {
-                A[e] = 0.0;
-                H[e] = 0.0;
-            }
-        __syncthreads();
-    }
-    for (int e = t; e < kE; e += kWave) G_out[e] = A[e];
-    bool rm = false;  // lane t < K: row t is still a pivot candidate
+    // Independent columns (K separate CG chains) keep only the diagonals of G and H.
     if (t < K) {
-        const double d = A[t * K + t];
-        rm = d > 0.0 && isfinite(d);
-        sc[t] = rm ? 1.0 / sqrt(d) : 0.0;
+        const double d = partials[t * K + t];
+        sc[t] = d > 0.0 && isfinite(d) ? 1.0 / sqrt(d) : 0.0;
     }
+    for (int e = t; e < kE; e += kWave) H[e] = independent && (e / K) != (e % K) ? 0.0 : partials[kE + e];
     __syncthreads();
-    for (int e = t; e < kE; e += kWave) {
-        A[e] = A[e] * sc[e / K] * sc[e % K];
-        Tm[e] = 0.0;
-    }
-    __syncthreads();
-    double d = t < K ? A[t * K + t] : 0.0;  // Schur-complement diagonal of row t
-    int r = 0;
-    for (int j = 0; j < K; ++j) {
-        // pivot: the largest remaining diagonal, lowest index on ties; every lane ends with it
-        double v = rm ? d : -1.0;
-        int p = t < K ? t : K;
+    unsigned cand = 0;  // rows that may still be pivots: a positive, finite diagonal in G
+    for (int i = 0; i < K; ++i)
+        if (sc[i] > 0.0) cand |= 1u << i;
+    double a[U];
 #pragma unroll
-        for (int off = kWave / 2; off > 0; off >>= 1) {
-            const double ov = __shfl_xor(v, off, kWave);
-            const int op = __shfl_xor(p, off, kWave);
-            if (ov > v || (ov == v && op < p)) {
-                v = ov;
-                p = op;
+    for (int u = 0; u < U; ++u) {
+        const int e = t + u * kWave, i = e / K, j = e % K;
+        const bool keep = e < kE && ((cand >> i) & (cand >> j) & 1u) && !(independent && i != j);
+        a[u] = keep ? (0.5 * (partials[e] + partials[j * K + i])) * (sc[i] * sc[j]) : 0.0;
+        if (e < kE && i == j) diag[i] = a[u];
+    }
+    __syncthreads();
+    unsigned swept = 0;
+    for (int step = 0; step < K; ++step) {
+        double v = -1.0;
+        int p = K;
+        for (int i = 0; i < K; ++i) {
+            const double di = diag[i];
+            if (((cand >> i) & 1u) && di > v) {
+                v = di;
+                p = i;
             }
         }
         if (p >= K || !(v > drop_tol)) break;
-        if (t == p) rm = false;
-        const double ljj = sqrt(v);
-        if (t < K) {
-            double c = 0.0;
-            if (t == p) {
-                c = ljj;
-            } else if (rm) {
-                double s = A[t * K + p];
-                for (int k = 0; k < j; ++k) s -= Lc[k * K + t] * Lc[k * K + p];
-                c = s / ljj;
-                d -= c * c;
-            }
-            Lc[j * K + t] = c;
+        cand &= ~(1u << p);
+        swept |= 1u << p;
+        const double inv = 1.0 / v;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = t + u * kWave;
+            if (e < kE && e / K == p) prow[e % K] = a[u];  // row p = column p before the sweep
         }
-        if (t == 0) idx[j] = p;
-        r = j + 1;
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = t + u * kWave, i = e / K, j = e % K;
+            if (e >= kE) continue;
+            if (i == p && j == p)
+                a[u] = -inv;
+            else if (i == p || j == p)
+                a[u] = a[u] * inv;
+            else
+                a[u] = fma(-(prow[i] * prow[j]), inv, a[u]);
+            if (i == j) diag[i] = a[u];
+        }
         __syncthreads();
     }
-    if (t < r) {
-        // column b = t of Lp^-1 by forward substitution, Lp(a, q) = Lc[q * K + idx[a]];
-        // T[idx[a]][b] = sc[idx[a]] Lp^-1(b, a), so lane b writes row idx[b] of T
-        const int b = t;
-        double y[K];
 #pragma unroll
-        for (int a = 0; a < K; ++a) {
-            y[a] = 0.0;
-            if (a < r && a >= b) {
-                const int ia = idx[a];
-                double s = a == b ? 1.0 : 0.0;
-#pragma unroll
-                for (int q = 0; q < a; ++q)
-                    if (q >= b) s -= Lc[q * K + ia] * y[q];
-                y[a] = s / Lc[a * K + ia];
-            }
-        }
-        const int ib = idx[b];
-#pragma unroll
-        for (int a = 0; a < K; ++a)
-            if (a < r && a >= b) Tm[ib * K + a] = sc[ib] * y[a];
-    }
-    __syncthreads();
-    for (int e = t; e < kE; e += kWave) {  // TT = T T^T (into Lc, dead now)
-        const int i = e / K, j = e % K;
-        double s = 0.0;
-        for (int b = 0; b < K; ++b) s += Tm[i * K + b] * Tm[j * K + b];
-        Lc[e] = s;
-        T[e] = Tm[e];
-        TT[e] = s;
+    for (int u = 0; u < U; ++u) {
+        const int e = t + u * kWave, i = e / K, j = e % K;
+        if (e >= kE) continue;
+        const double tt = ((swept >> i) & (swept >> j) & 1u) ? -a[u] * (sc[i] * sc[j]) : 0.0;
+        M[e] = tt;
+        TT[e] = tt;
     }
     __syncthreads();
     for (int e = t; e < kE; e += kWave) {  // a = TT H, frozen columns zero
         const int i = e / K, c = e % K;
         double s = 0.0;
-        for (int b = 0; b < K; ++b) s += Lc[i * K + b] * H[b * K + c];
+        for (int b = 0; b < K; ++b) s += M[i * K + b] * H[b * K + c];
         AL[e] = state[2 + c] ? 0.0 : s;
     }
 }
@@ -1314,6 +1298,32 @@ __global__ void column_norms_kernel(const double* __restrict__ partials, int nb,
     __shared__ double v[K];
     reduce_partials<K>(partials, nb, K, 0, v);
     if (threadIdx.x < K) out[threadIdx.x] = sqrt(v[threadIdx.x]);
+}
+
+// The first round of a solve from zero, set on the device so that no host read precedes it: a
+// column is active when it is a real (unpadded) column with b_c != 0; an active column's target is
+// margin x tolerance x ||b_c||, an inactive one is frozen from the start; the converged-at counters
+// start at zero and the done flag is set when no column is active.
+template <int K>
+__global__ void __launch_bounds__(kWave) round_start_kernel(int k, double tolerance, double margin,
+                                                            const double* __restrict__ bnorm,
+                                                            double* __restrict__ target, int* __restrict__ state) {
+    __shared__ int active_cols;
+    const int t = threadIdx.x;
+    for (int i = t; i < 2 + 2 * kMaxBlock; i += kWave) state[i] = 0;
+    bool active = false;
+    if (t < K) {
+        active = t < k && bnorm[t] > 0.0;
+        target[t] = active ? margin * tolerance * bnorm[t] : 0.0;
+    }
+    if (t == 0) active_cols = 0;
+    __syncthreads();
+    if (t < K) {
+        state[2 + t] = active ? 0 : 1;
+        if (active) atomicAdd(&active_cols, 1);
+    }
+    __syncthreads();
+    if (t == 0) state[0] = active_cols == 0;
 }
 
 std::int64_t gram_rows(std::int64_t n) { return (n + kGramBlocks - 1) / kGramBlocks; }
@@ -1526,6 +1536,7 @@ BlockSolver::GraphSet& BlockSolver::graph_set() {
         }
         for (int i = 0; i < kNumBlockWidths; ++i) {
             retire(pick->win[i]);
+            retire(pick->restart[i]);
             for (auto& x : pick->one[i]) retire(x);
         }
         *pick = GraphSet{};
@@ -1551,7 +1562,7 @@ int BlockSolver::graph_execs() const {
     int n = static_cast<int>(retired_.size());
     for (const auto& g : sets_)
         for (int i = 0; i < kNumBlockWidths; ++i) {
-            n += g.win[i] != nullptr;
+            n += (g.win[i] != nullptr) + (g.restart[i] != nullptr);
             for (auto x : g.one[i]) n += x != nullptr;
         }
     return n;
@@ -1593,6 +1604,7 @@ void BlockSolver::drop_graphs() {
     for (auto& g : sets_) {
         for (int i = 0; i < kNumBlockWidths; ++i) {
             retire(g.win[i]);
+            retire(g.restart[i]);
             for (auto& x : g.one[i]) retire(x);
         }
         g = GraphSet{};
@@ -1629,12 +1641,12 @@ void BlockSolver::ensure(int K) {
     partials_.reset(static_cast<std::size_t>(kGramBlocks) * 2 * K * K, "gram partials");
     red_.reset(static_cast<std::size_t>(2) * K * K, "reduced partials");
     G_.reset(K * K, "G");
-    T_.reset(K * K, "T");
     TT_.reset(K * K, "TT");
     AL_.reset(static_cast<std::size_t>(kLag) * K * K, "alpha");
     ran_.reset(kLag, "window slots run");
     D_.reset(K * K, "D");
     norms_.reset(K, "norms");
+    bnorm_.reset(K, "right-hand-side norms");
     target_.reset(K, "targets");
     state_.reset(2 + 2 * kMaxBlock, "solver state");
     cap_k_ = K;
@@ -1791,7 +1803,7 @@ void BlockSolver::iteration_k(const SolveOptions& opt, int j) {
     mark(0, st);
     spmm_block_k<K>(st, w_in);
     reduce_cols(partials_.get(), kGramBlocks, 2 * K * K, red_.get(), st, s_);
-    coef_kernel<K><<<1, kWave, 0, s_>>>(red_.get(), kGramBlocks, opt.drop_tol, G_.get(), T_.get(), TT_.get(), al, stw,
+    coef_kernel<K><<<1, kWave, 0, s_>>>(red_.get(), opt.drop_tol, TT_.get(), al, stw,
                                         (multi_ || opt.independent) ? 1 : 0, ran_.get() + j);
     update_r_kernel<K><<<kGramBlocks, xr_threads<K>(), 0, s_>>>(nn, S_.get(), al, R_.get(), partials_.get(), rpb, st);
     reduce_cols(partials_.get(), kGramBlocks, K, red_.get(), st, s_);
@@ -1840,7 +1852,8 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
     double* const Xi = X64_.get();  // solver-owned, so a captured graph never names a caller buffer
 
     std::vector<double> bnorm(K), rnorm(K);
-    auto true_residual = [&](bool zero) {
+    // R64 = B - A X in FP64 and its column norms into `norms`, without a wait.
+    auto residual_norms = [&](bool zero, double* norms) {
         if (multi_)
             residual64_kernel<K, true><<<kGramBlocks, kGramThreads, 0, s_>>>(
                 nn, op_.a32.ptr, op_.a32.col, multi_->tissue_vals, op_.a32.nnz, multi_->n_tissue, multi_->sigma, B, Xi,
@@ -1849,41 +1862,65 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
             residual64_kernel<K, false><<<kGramBlocks, kGramThreads, 0, s_>>>(
                 nn, op_.a32.ptr, op_.a32.col, op_.val64, op_.a32.nnz, 0, nullptr, B, Xi, R64_.get(), partials_.get(), rpb,
                 zero);
-        column_norms_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, norms_.get());
+        column_norms_kernel<K><<<1, kSmallThreads, 0, s_>>>(partials_.get(), kGramBlocks, norms);
+    };
+    auto true_residual = [&](bool zero) {
+        residual_norms(zero, norms_.get());
         d2h(rnorm.data(), norms_.get(), K, s_);
         sync(s_, "true residual");
     };
-    true_residual(true);  // x_zero: R64 = B, so these are ||b_c||
-    bnorm = rnorm;
+    auto check_bnorm = [&] {
+        for (int c = 0; c < k; ++c) require(bnorm[c] > 0.0, "zero right-hand-side column " + std::to_string(c));
+    };
+    residual_norms(true, bnorm_.get());  // x_zero: R64 = B, so these are ||b_c||
+    // From zero (the usual case) the first round's targets and flags follow from ||b|| alone, so
+    // they are set on the device and ||b|| comes back with the first convergence read; from a
+    // supplied X0 the host reads both norms first, since the guess may already meet the tolerance.
+    bool bnorm_pending = X0 == nullptr && opt.tolerance < 1.0;
     if (X0 != nullptr) {
+        d2h(bnorm.data(), bnorm_.get(), K, s_);
         d2d(Xi, X0, static_cast<std::size_t>(nk), s_);
         true_residual(false);
+        check_bnorm();
     } else {
         zero_f64_kernel<<<grid_for(nk), kBlock, 0, s_>>>(nk, Xi);
+        if (!bnorm_pending) {
+            d2h(bnorm.data(), bnorm_.get(), K, s_);
+            sync(s_, "right-hand-side norms");
+            check_bnorm();
+            rnorm = bnorm;
+        }
     }
-    for (int c = 0; c < k; ++c) require(bnorm[c] > 0.0, "zero right-hand-side column " + std::to_string(c));
 
     std::vector<int> host_state(2 + 2 * kMaxBlock, 0);
     bool use_graph = opt.use_graph;  // cleared for the rest of this solve if a capture fails
     int total_iters = 0;
     std::vector<int> col_iters(K, 0);
+    int slot = 0;
+    while (kBlockWidths[slot] != K) ++slot;
     for (int round = 0; round < opt.max_refine; ++round) {
-        bool all_ok = true;
-        for (int c = 0; c < k; ++c) {
-            stats.relative_residual[c] = rnorm[c] / bnorm[c];
-            if (!(stats.relative_residual[c] <= opt.tolerance)) all_ok = false;
+        const bool device_start = round == 0 && bnorm_pending;
+        if (!device_start) {
+            bool all_ok = true;
+            for (int c = 0; c < k; ++c) {
+                stats.relative_residual[c] = rnorm[c] / bnorm[c];
+                if (!(stats.relative_residual[c] <= opt.tolerance)) all_ok = false;
+            }
+            if (all_ok || total_iters >= opt.max_iters) break;
         }
-        if (all_ok || total_iters >= opt.max_iters) break;
-        // Inner solve on the current true residual (a restart: W = Z).
-        std::vector<double> tgt(K, 0.0);
+        // Inner solve on the current true residual (a restart: W = M R).
         std::fill(host_state.begin(), host_state.end(), 0);
         host_state[1] = total_iters;
-        for (int c = 0; c < K; ++c) {
-            const bool active = c < k && !(stats.relative_residual[c] <= opt.tolerance);
-            tgt[c] = active ? opt.inner_margin * opt.tolerance * bnorm[c] : 0.0;
-            host_state[2 + c] = active ? 0 : 1;
-        }
-        {
+        if (device_start) {
+            round_start_kernel<K><<<1, kWave, 0, s_>>>(k, opt.tolerance, opt.inner_margin, bnorm_.get(), target_.get(),
+                                                      state_.get());
+        } else {
+            std::vector<double> tgt(K, 0.0);
+            for (int c = 0; c < K; ++c) {
+                const bool active = c < k && !(stats.relative_residual[c] <= opt.tolerance);
+                tgt[c] = active ? opt.inner_margin * opt.tolerance * bnorm[c] : 0.0;
+                host_state[2 + c] = active ? 0 : 1;
+            }
             const std::size_t tb = sizeof(double) * K, sb = sizeof(int) * host_state.size();
             char* h = static_cast<char*>(stage_.fill(tb + sb));
             std::memcpy(h, tgt.data(), tb);
@@ -1892,13 +1929,22 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
             stage_.send(state_.get(), tb, sb, s_);
             stage_.done(s_);
         }
-        cast_f64_f32_kernel<<<grid_for(nk), kBlock, 0, s_>>>(nk, R64_.get(), R_.get());
-        vcycle_k<K>(R_.get(), Z_.get(), nullptr);
-        d2d(W_.get(), Z_.get(), static_cast<std::size_t>(nk), s_);
-        int slot = 0;
-        while (kBlockWidths[slot] != K) ++slot;
+        // The restart writes M R straight into the first W slot (the iteration's V-cycle writes Z
+        // afresh before anything reads it).
+        auto restart = [&] {
+            cast_f64_f32_kernel<<<grid_for(nk), kBlock, 0, s_>>>(nk, R64_.get(), R_.get());
+            vcycle_k<K>(R_.get(), W_.get(), nullptr);
+        };
         const int diag = (multi_ || opt.independent) ? 1 : 0;
         hipGraphExec_t win = nullptr, one[kLag] = {};
+        if (use_graph) {
+            GraphSet& gs = graph_set();
+            if (gs.restart[slot] == nullptr && !capture(gs.restart[slot], restart)) use_graph = false;
+        }
+        if (use_graph)
+            check(hipGraphLaunch(graph_set().restart[slot], s_), "graph launch");
+        else
+            restart();
         if (use_graph) {
             GraphSet& gs = graph_set();
             if (gs.win[slot] == nullptr || gs.drop[slot] != opt.drop_tol || gs.diag[slot] != diag) {
@@ -1915,6 +1961,7 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
                 } else {
                     // A failed capture records and runs nothing: this solve launches its iterations
                     // directly; the next solve tries to capture again.
+                    retire(gs.restart[slot]);
                     retire(gs.win[slot]);
                     for (auto& x : gs.one[slot]) retire(x);
                     use_graph = false;
@@ -1949,8 +1996,13 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
                 pos = (pos + L) % kLag;
             }
             launched += batch;
+            if (bnorm_pending) d2h(bnorm.data(), bnorm_.get(), K, s_);
             d2h(host_state_, state_.get(), 2 + 2 * K, s_);
             sync(s_, "convergence read");
+            if (bnorm_pending) {
+                bnorm_pending = false;
+                check_bnorm();
+            }
             total_iters = host_state_[1];
             if (host_state_[0]) {
                 done = true;
@@ -1960,9 +2012,15 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
         if (pos != 0) flush_pending_k<K>(pos);  // the round ended inside a window
         if (!done) total_iters = launched;
         if (round == 0) last_first_round_[slot] = total_iters - host_state[1];
-        for (int c = 0; c < K; ++c)
-            if (host_state[2 + c] == 0 && host_state_[2 + K + c] > 0) col_iters[c] = host_state_[2 + K + c];
+        for (int c = 0; c < K; ++c)  // set only for columns that converged in this round
+            if (host_state_[2 + K + c] > 0) col_iters[c] = host_state_[2 + K + c];
         stats.vcycles += total_iters - host_state[1] + 1;
+        if (bnorm_pending) {  // the round made no convergence read
+            bnorm_pending = false;
+            d2h(bnorm.data(), bnorm_.get(), K, s_);
+            sync(s_, "right-hand-side norms");
+            check_bnorm();
+        }
         true_residual(false);
         double worst = 0.0;
         for (int c = 0; c < k; ++c) worst = std::max(worst, rnorm[c] / bnorm[c]);

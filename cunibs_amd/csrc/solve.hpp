@@ -219,7 +219,8 @@ class BlockSolver {
     // reductions and small state
     DeviceBuffer<double> partials_;  // kGramBlocks * 2 * K * K
     DeviceBuffer<double> red_;       // the partials summed over the blocks (2 K^2)
-    DeviceBuffer<double> G_, T_, TT_, AL_, D_, norms_, target_;  // AL_: kLag K x K coefficient slots
+    DeviceBuffer<double> G_, TT_, AL_, D_, norms_, target_;  // AL_: kLag K x K coefficient slots
+    DeviceBuffer<double> bnorm_;  // ||b_c||, kept on the device for the first round's targets
     DeviceBuffer<int> ran_;  // per window slot: the iteration ran (its coefficients are to be applied)
     DeviceBuffer<int> state_;  // [0] done, [1] iteration, [2..2+K) frozen, [2+K..2+2K) col_iters
     int* host_state_ = nullptr;  // pinned
@@ -237,7 +238,8 @@ class BlockSolver {
     void mark(int i, const int* state);
     // Captured iterations, one set per operator mode: the single operator, or one multi-matrix
     // layout (its value buffers and width), so that alternating between them captures nothing
-    // again. Per width: a whole window of kLag iterations and one iteration at each ring position.
+    // again. Per width: a whole window of kLag iterations, one iteration at each ring position, and
+    // the restart of a round (the true residual cast to FP32, then W = M R).
     struct GraphSet {
         bool in_use = false;
         bool multi = false;
@@ -245,6 +247,7 @@ class BlockSolver {
         unsigned long long last_use = 0;
         hipGraphExec_t win[kNumBlockWidths] = {};
         hipGraphExec_t one[kNumBlockWidths][kLag] = {};
+        hipGraphExec_t restart[kNumBlockWidths] = {};
         double drop[kNumBlockWidths] = {};  // the drop tolerance each width was captured with
         int diag[kNumBlockWidths] = {};     // and whether its Grams were diagonal
     };

@@ -245,6 +245,21 @@ def test_partial_blocks_agree_with_the_full_block(block8, gctx, coil, k):
         assert rel(sub[i].magnE, res[i].magnE) < 2e-5
 
 
+def test_repeated_placements_in_one_block(block8, gctx, coil):
+    """A block that holds one placement three times and another twice has a singular Gram matrix
+    from the first iteration: the coefficient step keeps one direction of each repeated set and
+    drops the others (their Schur pivots vanish), and every copy still meets the tolerance with
+    the field of the block of distinct placements, within the block method's tolerance."""
+    from cunibs_amd import fem
+    sites, res = block8
+    order = [0, 1, 0, 2, 1, 3, 0, 4]
+    rep = fem.solve_placements_block(gctx, coil.positions_m, coil.moments, [sites[i] for i in order], 1e6)
+    for j, i in enumerate(order):
+        assert rep[j].relative_residual <= 1e-6
+        assert rel(rep[j].magnE, res[i].magnE) < 2e-5
+    assert rel(rep[2].magnE, rep[0].magnE) < 1e-6 and rel(rep[4].magnE, rep[1].magnE) < 1e-6
+
+
 def test_degenerate_placement_raises(gctx, coil):
     """place.cu:153-230: a handle along the scalp normal names the placement in a ValueError. The
     centre is a scalp node (its own projection) and the handle sits on the frame's normal there."""
