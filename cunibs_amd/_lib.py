@@ -64,7 +64,7 @@ class SolveOptions(ctypes.Structure):
     _fields_ = [
         ("tolerance", _D), ("max_iters", _I32), ("max_refine", _I32), ("drop_tol", _D),
         ("use_graph", _I32), ("check_every", _I32), ("recovery", _I32), ("warm_start", _I32), ("collect", _I32),
-        ("sigma", _P), ("n_sigma", _I32), ("independent", _I32),
+        ("sigma", _P), ("n_sigma", _I32), ("independent", _I32), ("recycle", _I32),
     ]
 
 

@@ -51,6 +51,9 @@ int32_t guarded(F&& f) {
 // fem/assembly.py:15-26.
 const double kDefaultSigma[11] = {NAN, 0.126, 0.275, 1.654, 0.010, 0.465, 0.5, 0.008, 0.025, 0.6, 0.16};
 constexpr double kMu0Over4Pi = 1e-7;  // fem/placement.py:16
+// Relative Schur pivot below which a direction of a sweep's Galerkin guess basis drops out: the
+// basis's A-Gram comes from right-hand sides that match A X only to the solve tolerance.
+constexpr double kGuessDropTol = 1e-6;
 
 int device_cu_count() {
     int dev = 0, cus = 0;
@@ -1381,10 +1384,15 @@ void solve_sweep(cn_context& c, const double* pos_m, const double* mom, std::int
         opt.independent = sopt->independent != 0;
     }
     const bool warm = sopt && sopt->warm_start == 1;
+    // Blocks of earlier solutions in each full block's initial guess (none with a warm-start basis,
+    // and at most 32 basis vectors).
+    const int recycle = warm ? 0 : std::min(sopt ? std::max(sopt->recycle, 0) : 2, std::min(2, 32 / K));
+    require(sopt == nullptr || sopt->recycle >= 0, "recycle must be >= 0");
     const int mode = sopt ? sopt->recovery : CN_RECOVERY_RAW;
     const RecoveryOp* rec = mode == CN_RECOVERY_RAW ? nullptr : ensure_recovery(c, mode);
     c.d_X.ensure(nr * K, "solution");
     c.d_X2.ensure(nr * K, "solution");
+    if (recycle > 0) c.solver->guess_reset(recycle, K);
     double* const sol[2] = {c.d_X.get(), c.d_X2.get()};
     // The block outputs run on s3, driven by a worker thread, while the next block solves: every
     // buffer they write is theirs alone, allocated here before any of them starts.
@@ -1399,9 +1407,11 @@ void solve_sweep(cn_context& c, const double* pos_m, const double* mom, std::int
     // s3 after its solve; the worker waits for them before it returns.
     int device = 0;
     check(hipGetDevice(&device), "device");
+    // The caller orders s3 after block b's solve before it starts the worker: a stream wait on an
+    // event of the solve stream is refused while that stream captures a graph, which the next
+    // block's solve may be doing by the time the worker runs.
     auto outputs = [&c, rec, m, s3, metrics_out, &nodal, &sol, device](int b, int first, int k, int Kb, int buf) {
         check(hipSetDevice(device), "device");  // the worker thread works on the context's device
-        check(hipStreamWaitEvent(s3, c.solve_done[b % 2], 0), "wait for the solve");
         const double* X = sol[b % 2];
         if (rec == nullptr) {
             reconstruct(X, c.tets.get(), c.g32.get(), nodal[buf], Kb, m, k, Kb, c.ground, nullptr, c.d_magn.get(), s3);
@@ -1436,12 +1446,18 @@ void solve_sweep(cn_context& c, const double* pos_m, const double* mom, std::int
                 c.d_x0.ensure(nr * K, "x0");
                 c.warm.initial_guess(rhs[buf], k, Kb, c.d_x0.get(), s);
                 dx0 = c.d_x0.get();
+            } else if (recycle > 0 && Kb == K && c.solver->guess_blocks() > 0) {
+                c.d_x0.ensure(nr * K, "x0");
+                c.solver->guess(rhs[buf], K, kGuessDropTol, c.d_x0.get());
+                dx0 = c.d_x0.get();
             }
             SolveStats st = c.solver->solve(rhs[buf], sol[b % 2], dx0, k, opt);
             retry_missed(c, st, opt, rhs[buf], k, Kb, nullptr, sol[b % 2]);
+            if (recycle > 0 && Kb == K) c.solver->guess_push(sol[b % 2], rhs[buf], K);
             check(hipEventRecord(c.solve_done[b % 2], s), "solve event");
             join();
             if (b + 2 < n_blocks) coil_stage(b + 2);
+            check(hipStreamWaitEvent(s3, c.solve_done[b % 2], 0), "wait for the solve");
             pending = std::async(std::launch::async, outputs, b, first, k, Kb, buf);
             for (int i = 0; i < k; ++i) {
                 if (iters_out) iters_out[first + i] = st.column_iterations[i];
@@ -1510,6 +1526,7 @@ void cn_default_solve_options(cn_solve_options* o) {
     o->use_graph = 1;
     o->check_every = 4;
     o->independent = 0;
+    o->recycle = 2;
 }
 
 int32_t cn_context_create(const double* nodes_mm, int64_t n_nodes, const int32_t* tet_nodes, const int32_t* tet_tags,

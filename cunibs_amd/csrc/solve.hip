@@ -1132,6 +1132,57 @@ constexpr int small_threads() {
     return K * K <= kWave ? kWave : K * K;
 }
 
+// Symmetric sweeps of an M x M matrix held by one wavefront, lane l owning entries e = l + 64 u
+// (row e / M, column e % M) in a[u], its diagonal mirrored in diag (LDS, written by the caller
+// before a barrier). The largest remaining diagonal among `cand` (lowest index on ties) is swept
+// while it exceeds drop_tol: p -> -1/a_pp on the diagonal, a_ip -> a_ip / a_pp in row and column
+// p, a_ij -> a_ij - a_ip a_pj / a_pp elsewhere, with the pivot row passed through LDS (prow) and
+// every product formed so that a symmetric matrix stays bitwise symmetric. Returns the swept set;
+// its block of a then holds minus the inverse of that principal block, and the unswept diagonal
+// the Schur complements the next pivot was chosen from (Goodnight, "A tutorial on the SWEEP
+// operator", 1979).
+template <int M, int U>
+__device__ unsigned sweep_pivots(double (&a)[U], double* diag, double* prow, unsigned cand, double drop_tol) {
+    static_assert(M <= 32, "the pivot sets are 32-bit masks");
+    const int t = threadIdx.x;
+    unsigned swept = 0;
+    for (int step = 0; step < M; ++step) {
+        double v = -1.0;
+        int p = M;
+        for (int i = 0; i < M; ++i) {
+            const double di = diag[i];
+            if (((cand >> i) & 1u) && di > v) {
+                v = di;
+                p = i;
+            }
+        }
+        if (p >= M || !(v > drop_tol)) break;
+        cand &= ~(1u << p);
+        swept |= 1u << p;
+        const double inv = 1.0 / v;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = t + u * kWave;
+            if (e < M * M && e / M == p) prow[e % M] = a[u];  // row p = column p before the sweep
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = t + u * kWave, i = e / M, j = e % M;
+            if (e >= M * M) continue;
+            if (i == p && j == p)
+                a[u] = -inv;
+            else if (i == p || j == p)
+                a[u] = a[u] * inv;
+            else
+                a[u] = fma(-(prow[i] * prow[j]), inv, a[u]);
+            if (i == j) diag[i] = a[u];
+        }
+        __syncthreads();
+    }
+    return swept;
+}
+
 // G, H -> TT, a = TT H with the frozen columns zeroed. TT is the inverse of G on the retained
 // directions and zero elsewhere. G = W^T A W is symmetric, but its FP64 sums of FP32 products are
 // not exactly (an entry small against |W| |A W| keeps a large relative rounding error on each
@@ -1140,18 +1191,14 @@ constexpr int small_threads() {
 // positive definite, as the block recurrence needs. With S = diag(G)^-1/2 and A = S G S, the
 // directions are chosen one at a time as the largest remaining Schur-complement diagonal of A
 // (lowest index on ties) while it exceeds drop_tol, which is the pivot order and drop rule of a
-// diagonally pivoted Cholesky of A, and TT = S (A_PP)^-1 S on the chosen set P. A_PP^-1 comes from
-// the symmetric sweep operator (Goodnight, "A tutorial on the SWEEP operator", 1979): sweeping
-// pivot p maps a_pp -> -1/a_pp, a_ip -> a_ip/a_pp, a_ij -> a_ij - a_ip a_pj / a_pp, so after the
-// sweeps the P block holds -A_PP^-1 and the unswept diagonal holds the Schur complements the next
-// pivot is chosen from. One wavefront; lane l keeps entries l, l + 64, ... in registers, and a step
-// is a pivot scan of the diagonal (every lane reads it from LDS and finds the same pivot), one
-// division, the pivot row through LDS, and one fused update per entry.
+// diagonally pivoted Cholesky of A, and TT = S (A_PP)^-1 S on the chosen set P, A_PP^-1 from the
+// symmetric sweeps (sweep_pivots). One wavefront; lane l keeps entries l, l + 64, ... in
+// registers, and a step is a pivot scan of the diagonal (every lane reads it from LDS and finds
+// the same pivot), one division, the pivot row through LDS, and one fused update per entry.
 template <int K>
 __global__ void __launch_bounds__(kWave) coef_kernel(const double* __restrict__ partials, double drop_tol,
                                                      double* __restrict__ TT, double* __restrict__ AL,
                                                      int* __restrict__ state, int independent, int* __restrict__ ran) {
-    static_assert(K <= 32, "the pivot sets are 32-bit masks");
     if (finished(state)) return;
     if (threadIdx.x == 0) *ran = 1;  // this window slot's coefficients are to be applied
     constexpr int kE = K * K;
@@ -1181,41 +1228,7 @@ __global__ void __launch_bounds__(kWave) coef_kernel(const double* __restrict__ 
         if (e < kE && i == j) diag[i] = a[u];
     }
     __syncthreads();
-    unsigned swept = 0;
-    for (int step = 0; step < K; ++step) {
-        double v = -1.0;
-        int p = K;
-        for (int i = 0; i < K; ++i) {
-            const double di = diag[i];
-            if (((cand >> i) & 1u) && di > v) {
-                v = di;
-                p = i;
-            }
-        }
-        if (p >= K || !(v > drop_tol)) break;
-        cand &= ~(1u << p);
-        swept |= 1u << p;
-        const double inv = 1.0 / v;
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int e = t + u * kWave;
-            if (e < kE && e / K == p) prow[e % K] = a[u];  // row p = column p before the sweep
-        }
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int e = t + u * kWave, i = e / K, j = e % K;
-            if (e >= kE) continue;
-            if (i == p && j == p)
-                a[u] = -inv;
-            else if (i == p || j == p)
-                a[u] = a[u] * inv;
-            else
-                a[u] = fma(-(prow[i] * prow[j]), inv, a[u]);
-            if (i == j) diag[i] = a[u];
-        }
-        __syncthreads();
-    }
+    const unsigned swept = sweep_pivots<K, U>(a, diag, prow, cand, drop_tol);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int e = t + u * kWave, i = e / K, j = e % K;
@@ -1231,6 +1244,115 @@ __global__ void __launch_bounds__(kWave) coef_kernel(const double* __restrict__ 
         for (int b = 0; b < K; ++b) s += M[i * K + b] * H[b * K + c];
         AL[e] = state[2 + c] ? 0.0 : s;
     }
+}
+
+// Initial guess of a block in the span of NB earlier blocks' solutions X_i (K columns each, M =
+// NB K basis vectors, X_0 the older), the Galerkin projection that minimises the A-norm error over
+// that span (Fischer, "Projection techniques for iterative solution of Ax = b with successive
+// right-hand sides", 1998): x0 = X c with (X^T A X) c = X^T b. The basis blocks' right-hand sides
+// B_i = A X_i (to their solve tolerance) give X^T A X without a product by A: D_i = X_i^T B_i on
+// the diagonal, Cx = X_0^T B_1 above it and its transpose below (A is symmetric). gb holds
+// [b^T X_0, .., b^T X_{NB-1}] (K x K blocks). C (M x K) = G^+ g on the directions the pivoted
+// sweeps of the symmetric part keep above drop_tol, so near-dependent basis vectors drop out;
+// `pending` receives X_{NB-1}^T b, the cross block of the next basis once b's block joins it.
+template <int K, int NB>
+__global__ void __launch_bounds__(kWave) galerkin_solve_kernel(const double* __restrict__ D0,
+                                                               const double* __restrict__ D1,
+                                                               const double* __restrict__ Cx,
+                                                               const double* __restrict__ gb, double drop_tol,
+                                                               double* __restrict__ C, double* __restrict__ pending) {
+    constexpr int M = NB * K;
+    constexpr int kE = M * M;
+    constexpr int U = (kE + kWave - 1) / kWave;
+    constexpr int KK = K * K;
+    __shared__ double Gs[kE];
+    __shared__ double g[M * K];
+    __shared__ double Tm[kE];
+    __shared__ double prow[M];
+    __shared__ double diag[M];
+    __shared__ double sc[M];
+    const int t = threadIdx.x;
+    for (int e = t; e < kE; e += kWave) {
+        const int I = e / M, J = e % M, bi = I / K, bj = J / K, p = I % K, q = J % K;
+        double v;
+        if (bi == bj)
+            v = (bi == 0 ? D0 : D1)[p * K + q];
+        else
+            v = bi == 0 ? Cx[p * K + q] : Cx[q * K + p];
+        Gs[e] = v;
+    }
+    for (int e = t; e < M * K; e += kWave) {  // g[iK + p][c] = (b^T X_i)[c][p]
+        const int I = e / K, c = e % K;
+        g[e] = gb[(I / K) * KK + c * K + I % K];
+    }
+    for (int e = t; e < KK; e += kWave) pending[e] = gb[(NB - 1) * KK + (e % K) * K + e / K];
+    __syncthreads();
+    if (t < M) {
+        const double d = Gs[t * M + t];
+        sc[t] = d > 0.0 && isfinite(d) ? 1.0 / sqrt(d) : 0.0;
+    }
+    __syncthreads();
+    unsigned cand = 0;
+    for (int i = 0; i < M; ++i)
+        if (sc[i] > 0.0) cand |= 1u << i;
+    double a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int e = t + u * kWave, i = e / M, j = e % M;
+        const bool keep = e < kE && ((cand >> i) & (cand >> j) & 1u);
+        a[u] = keep ? (0.5 * (Gs[e] + Gs[j * M + i])) * (sc[i] * sc[j]) : 0.0;
+        if (e < kE && i == j) diag[i] = a[u];
+    }
+    __syncthreads();
+    const unsigned swept = sweep_pivots<M, U>(a, diag, prow, cand, drop_tol);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int e = t + u * kWave, i = e / M, j = e % M;
+        if (e < kE) Tm[e] = ((swept >> i) & (swept >> j) & 1u) ? -a[u] * (sc[i] * sc[j]) : 0.0;
+    }
+    __syncthreads();
+    for (int e = t; e < M * K; e += kWave) {
+        const int I = e / K, c = e % K;
+        double s = 0.0;
+        for (int J = 0; J < M; ++J) s += Tm[I * M + J] * g[J * K + c];
+        C[e] = s;
+    }
+}
+
+// x0[r][c] = sum over i, p of X_i[r][p] C[iK + p][c]: one thread per row and group of V columns,
+// the coefficients in LDS (FP32 products: the result is only a starting point for the solve).
+template <int K, int NB>
+__global__ void __launch_bounds__(kBlock) galerkin_expand_kernel(std::int64_t n, const float* __restrict__ X0,
+                                                                 const float* __restrict__ X1,
+                                                                 const double* __restrict__ C,
+                                                                 double* __restrict__ x) {
+    constexpr int V = vec_width<K>(), G = K / V;
+    __shared__ float cs[NB * K * K];
+    for (int e = threadIdx.x; e < NB * K * K; e += kBlock) cs[e] = static_cast<float>(C[e]);
+    __syncthreads();
+    const std::int64_t t = static_cast<std::int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const std::int64_t r = t / G;
+    if (r >= n) return;
+    const int c0 = static_cast<int>(t % G) * V;
+    float acc[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const float* xr = (i == 0 ? X0 : X1) + r * K;
+#pragma unroll
+        for (int p0 = 0; p0 < K; p0 += V) {
+            float xv[V];
+            vload<V>(xr + p0, xv);
+#pragma unroll
+            for (int q = 0; q < V; ++q)
+#pragma unroll
+                for (int v = 0; v < V; ++v) acc[v] = fmaf(xv[q], cs[(i * K + p0 + q) * K + c0 + v], acc[v]);
+        }
+    }
+    double* out = x + r * K + c0;
+#pragma unroll
+    for (int v = 0; v < V; ++v) out[v] = acc[v];
 }
 
 // D = TT E.
@@ -1300,20 +1422,22 @@ __global__ void column_norms_kernel(const double* __restrict__ partials, int nb,
     if (threadIdx.x < K) out[threadIdx.x] = sqrt(v[threadIdx.x]);
 }
 
-// The first round of a solve from zero, set on the device so that no host read precedes it: a
-// column is active when it is a real (unpadded) column with b_c != 0; an active column's target is
-// margin x tolerance x ||b_c||, an inactive one is frozen from the start; the converged-at counters
-// start at zero and the done flag is set when no column is active.
+// The first round of a solve, set on the device so that no host read precedes it: a column is
+// active when it is a real (unpadded) column with b_c != 0 whose starting residual (rnorm, or b
+// itself from zero when rnorm is null) is above the tolerance; an active column's target is margin
+// x tolerance x ||b_c||, an inactive one is frozen from the start; the converged-at counters start
+// at zero and the done flag is set when no column is active.
 template <int K>
 __global__ void __launch_bounds__(kWave) round_start_kernel(int k, double tolerance, double margin,
                                                             const double* __restrict__ bnorm,
+                                                            const double* __restrict__ rnorm,
                                                             double* __restrict__ target, int* __restrict__ state) {
     __shared__ int active_cols;
     const int t = threadIdx.x;
     for (int i = t; i < 2 + 2 * kMaxBlock; i += kWave) state[i] = 0;
     bool active = false;
     if (t < K) {
-        active = t < k && bnorm[t] > 0.0;
+        active = t < k && bnorm[t] > 0.0 && (rnorm == nullptr || !(rnorm[t] / bnorm[t] <= tolerance));
         target[t] = active ? margin * tolerance * bnorm[t] : 0.0;
     }
     if (t == 0) active_cols = 0;
@@ -1873,23 +1997,22 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
         for (int c = 0; c < k; ++c) require(bnorm[c] > 0.0, "zero right-hand-side column " + std::to_string(c));
     };
     residual_norms(true, bnorm_.get());  // x_zero: R64 = B, so these are ||b_c||
-    // From zero (the usual case) the first round's targets and flags follow from ||b|| alone, so
-    // they are set on the device and ||b|| comes back with the first convergence read; from a
-    // supplied X0 the host reads both norms first, since the guess may already meet the tolerance.
-    bool bnorm_pending = X0 == nullptr && opt.tolerance < 1.0;
     if (X0 != nullptr) {
-        d2h(bnorm.data(), bnorm_.get(), K, s_);
         d2d(Xi, X0, static_cast<std::size_t>(nk), s_);
-        true_residual(false);
-        check_bnorm();
+        residual_norms(false, norms_.get());
     } else {
         zero_f64_kernel<<<grid_for(nk), kBlock, 0, s_>>>(nk, Xi);
-        if (!bnorm_pending) {
-            d2h(bnorm.data(), bnorm_.get(), K, s_);
-            sync(s_, "right-hand-side norms");
-            check_bnorm();
-            rnorm = bnorm;
-        }
+    }
+    // The first round's targets and flags follow from the norms alone, so they are set on the
+    // device and ||b|| comes back with the first convergence read (a column whose guess already
+    // meets the tolerance starts frozen; if all do, the round runs no iteration).
+    bool bnorm_pending = opt.tolerance < 1.0;
+    if (!bnorm_pending) {
+        d2h(bnorm.data(), bnorm_.get(), K, s_);
+        if (X0 != nullptr) d2h(rnorm.data(), norms_.get(), K, s_);
+        sync(s_, "initial norms");
+        check_bnorm();
+        if (X0 == nullptr) rnorm = bnorm;
     }
 
     std::vector<int> host_state(2 + 2 * kMaxBlock, 0);
@@ -1912,7 +2035,8 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
         std::fill(host_state.begin(), host_state.end(), 0);
         host_state[1] = total_iters;
         if (device_start) {
-            round_start_kernel<K><<<1, kWave, 0, s_>>>(k, opt.tolerance, opt.inner_margin, bnorm_.get(), target_.get(),
+            round_start_kernel<K><<<1, kWave, 0, s_>>>(k, opt.tolerance, opt.inner_margin, bnorm_.get(),
+                                                      X0 != nullptr ? norms_.get() : nullptr, target_.get(),
                                                       state_.get());
         } else {
             std::vector<double> tgt(K, 0.0);
@@ -2088,6 +2212,89 @@ void BlockSolver::gram(const float* A, const float* B1, const float* B2, double*
     reduce_cols(partials_.get(), kGramBlocks, E, red_.get(), nullptr, s_);
     d2h(out_host, red_.get(), E, s_);
     sync(s_, "gram");
+}
+
+void BlockSolver::guess_reset(int capacity, int K) {
+    require(capacity >= 1 && capacity <= 2 && capacity * K <= 32,
+            "the guess basis holds one or two blocks, 32 vectors at most");
+    ensure(K);
+    const std::size_t nk = static_cast<std::size_t>(n()) * K;
+    for (int i = 0; i < capacity; ++i) {
+        g_x_[i].ensure(nk, "guess basis");
+        g_b_[i].ensure(nk, "guess basis right-hand sides");
+    }
+    g_mat_.ensure(static_cast<std::size_t>(8) * K * K, "guess Grams");
+    g_cap_ = capacity;
+    g_k_ = K;
+    g_count_ = 0;
+    g_newest_ = 0;
+    g_pending_ = false;
+}
+
+// g_mat_ layout (K x K blocks): D of slot 0 and 1, the cross block, the pending cross block, the
+// guess's Grams (two blocks) and its coefficients (two blocks).
+void BlockSolver::guess_push(const double* X, const float* B, int K) {
+    require(g_cap_ > 0 && K == g_k_, "guess_push: the basis was reset for another width");
+    const std::int64_t nn = n();
+    const std::int64_t nk = nn * K;
+    const std::int64_t E = static_cast<std::int64_t>(K) * K;
+    const int slot = g_cap_ == 1 ? 0 : 1 - g_newest_;
+    const int older = g_newest_;
+    cast_f64_f32_kernel<<<grid_for(nk), kBlock, 0, s_>>>(nk, X, g_x_[slot].get());
+    d2d(g_b_[slot].get(), B, static_cast<std::size_t>(nk), s_);
+    double* const m = g_mat_.get();
+    dispatch_k(K, [&](auto kc) {
+        constexpr int KK = decltype(kc)::value;
+        const std::int64_t rpb = gram_rows(nn);
+        launch_gram<KK, false>(nn, g_x_[slot].get(), g_b_[slot].get(), nullptr, partials_.get(), rpb, nullptr, s_);
+        reduce_cols(partials_.get(), kGramBlocks, static_cast<int>(E), m + slot * E, nullptr, s_);
+        if (g_cap_ == 2 && g_count_ >= 1) {  // the cross block X_older^T B_new
+            if (g_pending_)  // the last guess was for this block: its projection onto the older block
+                d2d(m + 2 * E, m + 3 * E, static_cast<std::size_t>(E), s_);
+            else {
+                launch_gram<KK, false>(nn, g_x_[older].get(), g_b_[slot].get(), nullptr, partials_.get(), rpb,
+                                       nullptr, s_);
+                reduce_cols(partials_.get(), kGramBlocks, static_cast<int>(E), m + 2 * E, nullptr, s_);
+            }
+        }
+    });
+    g_newest_ = slot;
+    g_count_ = std::min(g_count_ + 1, g_cap_);
+    g_pending_ = false;
+}
+
+void BlockSolver::guess(const float* b, int K, double drop_tol, double* x0) {
+    require(g_count_ > 0 && K == g_k_, "guess: no basis block of this width");
+    const std::int64_t nn = n();
+    const std::int64_t E = static_cast<std::int64_t>(K) * K;
+    double* const m = g_mat_.get();
+    double* const gb = m + 4 * E;
+    double* const coef = m + 6 * E;
+    const int newest = g_newest_, older = 1 - g_newest_;
+    dispatch_k(K, [&](auto kc) {
+        constexpr int KK = decltype(kc)::value;
+        const std::int64_t rpb = gram_rows(nn);
+        const unsigned grid = grid_for(nn * (KK / vec_width<KK>()));
+        if (g_count_ == 2) {
+            if constexpr (2 * KK <= 32) {
+                launch_gram<KK, true>(nn, b, g_x_[older].get(), g_x_[newest].get(), partials_.get(), rpb, nullptr,
+                                      s_);
+                reduce_cols(partials_.get(), kGramBlocks, static_cast<int>(2 * E), gb, nullptr, s_);
+                galerkin_solve_kernel<KK, 2><<<1, kWave, 0, s_>>>(m + older * E, m + newest * E, m + 2 * E, gb,
+                                                                  drop_tol, coef, m + 3 * E);
+                galerkin_expand_kernel<KK, 2><<<grid, kBlock, 0, s_>>>(nn, g_x_[older].get(), g_x_[newest].get(),
+                                                                       coef, x0);
+            }
+        } else {
+            launch_gram<KK, false>(nn, b, g_x_[newest].get(), nullptr, partials_.get(), rpb, nullptr, s_);
+            reduce_cols(partials_.get(), kGramBlocks, static_cast<int>(E), gb, nullptr, s_);
+            galerkin_solve_kernel<KK, 1><<<1, kWave, 0, s_>>>(m + newest * E, nullptr, nullptr, gb, drop_tol, coef,
+                                                              m + 3 * E);
+            galerkin_expand_kernel<KK, 1><<<grid, kBlock, 0, s_>>>(nn, g_x_[newest].get(), nullptr, coef, x0);
+        }
+    });
+    check(hipGetLastError(), "Galerkin guess launch");
+    g_pending_ = true;
 }
 
 void BlockSolver::vcycle(const float* B, float* X, int K) {

@@ -115,6 +115,17 @@ class BlockSolver {
     // G1 = A^T B1 and, with B2, G2 = A^T B2 (FP64, the iteration's Gram kernels and reduction)
     // into out_host (K*K, then K*K more with B2); A, B1, B2 device (n, K).
     void gram(const float* A, const float* B1, const float* B2, double* out_host, int K);
+    // Galerkin initial guesses from earlier blocks' solutions (a sweep's projection basis), all
+    // queued on the solver's stream without a host wait. guess_reset empties a basis of up to
+    // `capacity` (1 or 2) blocks of width K (capacity K <= 32); guess_push adds a solved block (its
+    // FP64 solution and FP32 right-hand side, n x K; the oldest leaves a full basis); guess writes
+    // x0 (n x K, FP64), the projection of the solution of A x = b onto the basis. Each push costs
+    // one Gram of its block; the cross block of a two-block basis is the projection the last guess
+    // already formed when that guess was for the block pushed after it.
+    void guess_reset(int capacity, int K);
+    void guess_push(const double* X, const float* B, int K);
+    void guess(const float* b, int K, double drop_tol, double* x0);
+    int guess_blocks() const { return g_count_; }
     // Average device time (ms) over `reps` launches, HIP events on the solver's stream.
     // mode 0: plain SpMM; 1: differenced SpMM (the iteration's S = A W); 2: 1 followed by the Gram
     // pass G = W^T S, H = W^T R. sub > 0 forces the sub-lanes per column group of the fine level.
@@ -221,6 +232,10 @@ class BlockSolver {
     DeviceBuffer<double> red_;       // the partials summed over the blocks (2 K^2)
     DeviceBuffer<double> G_, TT_, AL_, D_, norms_, target_;  // AL_: kLag K x K coefficient slots
     DeviceBuffer<double> bnorm_;  // ||b_c||, kept on the device for the first round's targets
+    DeviceBuffer<float> g_x_[2], g_b_[2];  // the guess basis: FP32 solutions and their right-hand sides
+    DeviceBuffer<double> g_mat_;           // its Grams, cross blocks and the last guess's coefficients
+    int g_cap_ = 0, g_k_ = 0, g_count_ = 0, g_newest_ = 0;
+    bool g_pending_ = false;
     DeviceBuffer<int> ran_;  // per window slot: the iteration ran (its coefficients are to be applied)
     DeviceBuffer<int> state_;  // [0] done, [1] iteration, [2..2+K) frozen, [2+K..2+2K) col_iters
     int* host_state_ = nullptr;  // pinned

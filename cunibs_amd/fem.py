@@ -626,12 +626,15 @@ def solve_sweep(
     use_graph: bool = True,
     recovery: str = "raw",
     warm_start: bool = False,
+    recycle: int = 2,
 ) -> dict:
     """Every placement of `sites` in blocks of `block_k`, summaries only: the next block's dA/dt
     and right-hand side are computed while the current block solves. Returns "transforms"
     (n, 4, 4), "metrics" (n, METRICS_LEN), "iterations" and "relative_residual" (n) and
-    "block_iterations" (per block), each block's numbers those of `solve_placements_block` on the
-    same placements."""
+    "block_iterations" (per block). Each full block starts from the Galerkin projection of its
+    solution onto the span of the previous `recycle` full blocks' solutions (at most 2; 0 starts
+    every block from zero, and then each block's numbers are those of `solve_placements_block` on
+    the same placements)."""
     _blocks()
     n_place = len(sites)
     if n_place < 1:
@@ -659,6 +662,9 @@ def solve_sweep(
     opt.tolerance = float(tolerance)
     opt.max_iters = int(max_iters)
     opt.use_graph = int(use_graph)
+    if recycle < 0:
+        raise ValueError(f"solve_sweep: recycle={recycle} must be >= 0")
+    opt.recycle = int(recycle)
     check(lib().cn_solve_sweep(ctx.handle, ptr(pos), ptr(mom), pos.shape[0], ptr(centers), ptr(handles), ptr(dists),
                                n_place, int(block_k), float(didt), ctypes.byref(opt), ptr(tf), ptr(met), ptr(it),
                                ptr(rel), ptr(bit)), "cn_solve_sweep")

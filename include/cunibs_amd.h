@@ -179,6 +179,11 @@ typedef struct {
      * block solver (solver/amg/solver.cpp:84-198, fem/solve.py:508-552); 0 (default): the true
      * block method, whose shared Krylov space converges in fewer iterations. */
     int32_t independent;
+    /* cn_solve_sweep: each full block starts from the Galerkin projection of its solution onto the
+     * span of this many earlier full blocks' solutions (at most 2, and 32 vectors), the projection
+     * method for successive right-hand sides; 0: every block from zero. Default 2. No reference
+     * counterpart (its placements start from zero). */
+    int32_t recycle;
 } cn_solve_options;
 
 void cn_default_solve_options(cn_solve_options* out);

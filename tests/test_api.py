@@ -133,8 +133,10 @@ def test_subject_end_to_end_matches_the_block_solver(head4, coil):
 
 @pytest.mark.gpu
 def test_summary_only_streaming_uses_the_pipelined_sweep(head4, coil):
-    """iter_simulate without field outputs streams through the pipelined sweep; each result's
-    summary, frame, iterations and residual equal a block-by-block solve of the same placements."""
+    """iter_simulate without field outputs streams through the pipelined sweep, whose blocks start
+    from the Galerkin projection onto the previous blocks' solutions: each result's frame equals a
+    block-by-block solve of the same placements, its summary matches it to the solve tolerance and
+    its FP64 true residual meets that tolerance."""
     from cunibs_amd import fem, synthetic
     from cunibs_amd.mesh import HeadMesh
     sites = synthetic.cap_sites(5, seed=23)
@@ -153,8 +155,11 @@ def test_summary_only_streaming_uses_the_pipelined_sweep(head4, coil):
             assert a.summary.keys() == want.keys()
             for key, value in want.items():
                 if isinstance(value, dict):
+                    for sub_key, v in value.items():
+                        assert a.summary[key][sub_key] == pytest.approx(v, rel=2e-5), (key, sub_key)
+                elif isinstance(value, str):
                     assert a.summary[key] == value, key
                 else:
-                    np.testing.assert_array_equal(a.summary[key], value, err_msg=key)
+                    np.testing.assert_allclose(a.summary[key], value, rtol=2e-5, err_msg=key)
             np.testing.assert_array_equal(a.transform, b.transform)
-            assert a.iterations == b.iterations and a.relative_residual == b.relative_residual
+            assert a.relative_residual <= 1e-6

@@ -352,9 +352,10 @@ def test_dadt_general_and_parallel_moment_paths(gctx, coil, oracle_mod, octx):
 
 @pytest.mark.parametrize("warm", [False, True])
 def test_pipelined_sweep_equals_blockwise_solves(gctx, coil, warm):
-    """cn_solve_sweep overlaps the next block's dA/dt and RHS with the current block's solve; its
-    summaries equal those of solving the same blocks one at a time, bit for bit (a ragged last
-    block of one placement included, which the solver takes at its own narrower padded width)."""
+    """cn_solve_sweep overlaps the next block's dA/dt and RHS with the current block's solve; with
+    every block starting from zero (recycle=0) its summaries equal those of solving the same blocks
+    one at a time, bit for bit (a ragged last block of one placement included, which the solver
+    takes at its own narrower padded width)."""
     from cunibs_amd import fem, synthetic
     sites = synthetic.cap_sites(9, seed=17)
     if warm:
@@ -362,7 +363,7 @@ def test_pipelined_sweep_equals_blockwise_solves(gctx, coil, warm):
         fem.solve_placements_block(gctx, coil.positions_m, coil.moments, synthetic.cap_sites(16, seed=18), 1e6,
                                    metrics=False, magnitude=False, collect=True)
         gctx.warm_finalize()
-    sweep = fem.solve_sweep(gctx, coil.positions_m, coil.moments, sites, 1e6, block_k=4, warm_start=warm)
+    sweep = fem.solve_sweep(gctx, coil.positions_m, coil.moments, sites, 1e6, block_k=4, warm_start=warm, recycle=0)
     ref = []
     for i in range(0, len(sites), 4):
         ref.extend(fem.solve_placements_block(gctx, coil.positions_m, coil.moments, sites[i:i + 4], 1e6,
@@ -372,6 +373,24 @@ def test_pipelined_sweep_equals_blockwise_solves(gctx, coil, warm):
     np.testing.assert_array_equal(sweep["relative_residual"], [r.relative_residual for r in ref])
     assert len(sweep["block_iterations"]) == 3
     assert np.all(sweep["relative_residual"] <= 1e-6)
+
+
+def test_recycled_sweep_meets_the_tolerance_in_fewer_iterations(gctx, coil):
+    """Each full block of a sweep starts from the Galerkin projection of its solution onto the
+    previous two blocks' solutions: every placement still meets the FP64 true-residual tolerance,
+    its summary matches the block solved from zero to the block method's tolerance, and
+    neighbouring placements (a row of the cap grid) need fewer iterations in all."""
+    from cunibs_amd import fem, synthetic
+    sites = synthetic.grid_sites()[:16]
+    rec = fem.solve_sweep(gctx, coil.positions_m, coil.moments, sites, 1e6, block_k=4)
+    cold = fem.solve_sweep(gctx, coil.positions_m, coil.moments, sites, 1e6, block_k=4, recycle=0)
+    again = fem.solve_sweep(gctx, coil.positions_m, coil.moments, sites, 1e6, block_k=4)
+    assert np.all(rec["relative_residual"] <= 1e-6)
+    np.testing.assert_array_equal(rec["metrics"], again["metrics"])  # deterministic
+    np.testing.assert_array_equal(rec["transforms"], cold["transforms"])
+    np.testing.assert_allclose(rec["metrics"][:, 0], cold["metrics"][:, 0], rtol=2e-5)  # peak |E|
+    assert rec["block_iterations"][0] == cold["block_iterations"][0]  # the first block has no basis
+    assert rec["block_iterations"][1:].sum() < cold["block_iterations"][1:].sum()
 
 
 def test_graph_replays_equal_direct_launches(block8, gctx, coil):
