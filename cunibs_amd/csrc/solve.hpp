@@ -50,7 +50,7 @@ struct SolveOptions {
     int max_iters = 2000;       // inner iterations over all refinement rounds
     int max_refine = kMaxRounds;  // outer FP64 refinement rounds
     double inner_margin = 0.5;  // inner target = margin * tol * ||b||
-    double drop_tol = 1e-6;     // pivoted Cholesky: relative Schur pivot below which a direction drops
+    double drop_tol = 1e-6;     // relative Schur pivot of the block Gram below which a direction drops
     int check_every = 4;        // graph replays between host convergence reads
     bool use_graph = true;
     bool independent = false;   // k independent CG chains (diagonal Grams) instead of block CG
