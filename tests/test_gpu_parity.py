@@ -375,16 +375,18 @@ def test_pipelined_sweep_equals_blockwise_solves(gctx, coil, warm):
     assert np.all(sweep["relative_residual"] <= 1e-6)
 
 
-def test_recycled_sweep_meets_the_tolerance_in_fewer_iterations(gctx, coil):
+@pytest.mark.parametrize("block_k,n_sites", [(1, 12), (4, 16), (32, 64)])
+def test_recycled_sweep_meets_the_tolerance_in_fewer_iterations(gctx, coil, block_k, n_sites):
     """Each full block of a sweep starts from the Galerkin projection of its solution onto the
-    previous two blocks' solutions: every placement still meets the FP64 true-residual tolerance,
-    its summary matches the block solved from zero to the block method's tolerance, and
-    neighbouring placements (a row of the cap grid) need fewer iterations in all."""
+    previous two blocks' solutions (one block at width 32, whose basis holds 32 vectors at most):
+    every placement still meets the FP64 true-residual tolerance, its summary matches the block
+    solved from zero to the block method's tolerance, and neighbouring placements (rows of the cap
+    grid) need fewer iterations in all."""
     from cunibs_amd import fem, synthetic
-    sites = synthetic.grid_sites()[:16]
-    rec = fem.solve_sweep(gctx, coil.positions_m, coil.moments, sites, 1e6, block_k=4)
-    cold = fem.solve_sweep(gctx, coil.positions_m, coil.moments, sites, 1e6, block_k=4, recycle=0)
-    again = fem.solve_sweep(gctx, coil.positions_m, coil.moments, sites, 1e6, block_k=4)
+    sites = synthetic.grid_sites()[:n_sites]
+    rec = fem.solve_sweep(gctx, coil.positions_m, coil.moments, sites, 1e6, block_k=block_k)
+    cold = fem.solve_sweep(gctx, coil.positions_m, coil.moments, sites, 1e6, block_k=block_k, recycle=0)
+    again = fem.solve_sweep(gctx, coil.positions_m, coil.moments, sites, 1e6, block_k=block_k)
     assert np.all(rec["relative_residual"] <= 1e-6)
     np.testing.assert_array_equal(rec["metrics"], again["metrics"])  # deterministic
     np.testing.assert_array_equal(rec["transforms"], cold["transforms"])
