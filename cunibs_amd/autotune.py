@@ -49,6 +49,18 @@ class TuningResult:
                 "width_rates": {str(k): round(v, 2) for k, v in self.width_rates.items()}}
 
 
+WIDTH_TIE = 0.02  # relative rate difference below which two block widths count as tied
+
+
+def pick_width(rates: dict) -> int:
+    """The block width to solve at, from placements/s by width: widths within WIDTH_TIE of the
+    best count as tied and the narrowest of them is taken (a narrower block's FP32 recursion drifts
+    less from the FP64 true residual, and a sweep's recycled guess holds two earlier blocks of up to
+    16 columns but one of 32)."""
+    best = max(rates.values())
+    return min(k for k, r in rates.items() if r >= (1.0 - WIDTH_TIE) * best)
+
+
 def _rate(ctx, pos, mom, sites, didt, k: int, warm_start: bool) -> float:
     """Placements/s over `sites` in blocks of k, after one untimed block (graph capture)."""
     def run(chunk):
@@ -117,7 +129,7 @@ def autotune(ctx: fem.SolverContext, dip_pos_m, dip_moment, sites: Sequence, did
     for k in widths:
         res.width_rates[k] = res.degree_rates[res.degree] if k == base_k else \
             _rate(ctx, dip_pos_m, dip_moment, sites, didt, k, warm_start)
-    res.block_k = max(res.width_rates, key=res.width_rates.get)
+    res.block_k = pick_width(res.width_rates)
     res.seconds = time.perf_counter() - t_start
     return res
 

@@ -1974,6 +1974,7 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
     stats.column_iterations.assign(k, 0);
     stats.relative_residual.assign(k, 0.0);
     double* const Xi = X64_.get();  // solver-owned, so a captured graph never names a caller buffer
+    const double margin = opt.inner_margin > 0.0 ? opt.inner_margin : margin_for_width(K);
 
     std::vector<double> bnorm(K), rnorm(K);
     // R64 = B - A X in FP64 and its column norms into `norms`, without a wait.
@@ -2035,14 +2036,14 @@ SolveStats BlockSolver::solve_k(const float* B, double* X, const double* X0, int
         std::fill(host_state.begin(), host_state.end(), 0);
         host_state[1] = total_iters;
         if (device_start) {
-            round_start_kernel<K><<<1, kWave, 0, s_>>>(k, opt.tolerance, opt.inner_margin, bnorm_.get(),
+            round_start_kernel<K><<<1, kWave, 0, s_>>>(k, opt.tolerance, margin, bnorm_.get(),
                                                       X0 != nullptr ? norms_.get() : nullptr, target_.get(),
                                                       state_.get());
         } else {
             std::vector<double> tgt(K, 0.0);
             for (int c = 0; c < K; ++c) {
                 const bool active = c < k && !(stats.relative_residual[c] <= opt.tolerance);
-                tgt[c] = active ? opt.inner_margin * opt.tolerance * bnorm[c] : 0.0;
+                tgt[c] = active ? margin * opt.tolerance * bnorm[c] : 0.0;
                 host_state[2 + c] = active ? 0 : 1;
             }
             const std::size_t tb = sizeof(double) * K, sb = sizeof(int) * host_state.size();

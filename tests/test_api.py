@@ -32,6 +32,14 @@ def test_placement_validation():
         p.distance_mm = 3.0  # frozen
 
 
+def test_autotuner_breaks_width_ties_toward_the_narrower_block():
+    """Block widths whose rates lie within 2 % of the best count as tied; the narrowest is taken."""
+    from cunibs_amd.autotune import pick_width
+    assert pick_width({4: 311.2, 8: 394.0, 16: 475.2, 32: 476.4}) == 16
+    assert pick_width({4: 300.0, 8: 390.0, 16: 455.0, 32: 476.4}) == 32
+    assert pick_width({8: 100.0}) == 8
+
+
 def test_recovery_modes():
     assert validate_recovery("raw") == "raw"
     with pytest.raises(ValueError):
