@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 
 #include "solve.hpp"
 
@@ -902,8 +903,14 @@ __global__ void __launch_bounds__(xr_threads<K>()) update_r_kernel(
     constexpr int V = vec_width<K>();
     constexpr int G = K / V;
     constexpr int RPW = kWave / G;
-    __shared__ float alf[K * K];
-    for (int e = threadIdx.x; e < K * K; e += blockDim.x) alf[e] = static_cast<float>(a[e]);
+    // Above 16 columns the step S a is formed in FP64 from the FP64 coefficients, as the deferred X
+    // update uses them: a wide block's Gram is worse conditioned, its coefficients larger with more
+    // cancellation, and FP32 products left a 32-column block's recursion 1.5x off the true
+    // residual on the 1.5 mm head (a stall at 1.19e-6 and a second round; 28 iterations instead of
+    // 22). Up to 16 columns the FP32 form meets the true residual within a few per cent.
+    using Acc = std::conditional_t<(K > 16), double, float>;
+    __shared__ Acc alf[K * K];
+    for (int e = threadIdx.x; e < K * K; e += blockDim.x) alf[e] = static_cast<Acc>(a[e]);
     __syncthreads();
     const int lb = logical_block();
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
@@ -917,9 +924,9 @@ __global__ void __launch_bounds__(xr_threads<K>()) update_r_kernel(
     for (std::int64_t r0 = lo + static_cast<std::int64_t>(wave) * RPW; r0 < hi; r0 += WAVES * RPW) {
         const std::int64_t row = r0 + lane / G;
         if (row >= hi) continue;
-        float dr[V];
+        Acc dr[V];
 #pragma unroll
-        for (int v = 0; v < V; ++v) dr[v] = 0.0f;
+        for (int v = 0; v < V; ++v) dr[v] = Acc(0);
 #pragma unroll
         for (int b0 = 0; b0 < K; b0 += V) {
             float sb[V];
@@ -927,7 +934,7 @@ __global__ void __launch_bounds__(xr_threads<K>()) update_r_kernel(
 #pragma unroll
             for (int u = 0; u < V; ++u) {
 #pragma unroll
-                for (int v = 0; v < V; ++v) dr[v] = fmaf(sb[u], alf[(b0 + u) * K + c0 + v], dr[v]);
+                for (int v = 0; v < V; ++v) dr[v] = fma(static_cast<Acc>(sb[u]), alf[(b0 + u) * K + c0 + v], dr[v]);
             }
         }
         const std::int64_t at = row * K + c0;
@@ -935,7 +942,7 @@ __global__ void __launch_bounds__(xr_threads<K>()) update_r_kernel(
         vload<V>(R + at, rr);
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-            rr[v] -= dr[v];
+            rr[v] = static_cast<float>(static_cast<Acc>(rr[v]) - dr[v]);
             nrm[v] = fma(static_cast<double>(rr[v]), static_cast<double>(rr[v]), nrm[v]);
         }
         vstore<V>(R + at, rr);

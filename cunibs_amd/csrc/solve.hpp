@@ -49,7 +49,8 @@ constexpr int kDenseTailMaxRows = 4096;
 // round, so the margin trades iterations against extra rounds. Up to 16 columns the FP64 true
 // residual lands within a few per cent of the recursive one (the differenced SpMM keeps the FP32
 // recursion accurate: 8.2e-7 at most over a 768-placement sweep at 0.8); a 32-column block's
-// drifts further (1.19e-6 after a round at 0.8 on the 1.5 mm head), so it keeps 0.5.
+// recursion, even with its FP64 residual step, ends nearer the tolerance (9.9e-7 after a round at
+// 0.8 on the 1.5 mm head), so it keeps 0.5.
 inline double margin_for_width(int K) { return K <= 16 ? 0.8 : 0.5; }
 
 struct SolveOptions {
