@@ -539,7 +539,11 @@ typedef double doublex4_t __attribute__((ext_vector_type(4)));
 // the products of the FP32 inputs are exact and each input is converted once. Accumulator
 // register v of lane l holds tile row 4 v + l / 16 of tile column l % 16 (the FP64 layout, unlike
 // the FP32 one's rows 4 (l / 16) + v). U quads of rows per step keep 3 U loads in flight.
-constexpr int kGramMfmaThreads = 1024;
+// Four waves per workgroup: the pipelined sweep keeps one persistent N-body workgroup on every CU,
+// and a 1024-thread Gram workgroup (16 wave slots and, at 32 columns with two products, 111 VGPRs
+// a lane) often found no CU with room beside it (at 32 columns the pass took 1.46 ms in the sweep
+// against 0.1 ms alone); small workgroups fill around it.
+constexpr int kGramMfmaThreads = 256;
 
 template <int K, bool TWO>
 __global__ void __launch_bounds__(kGramMfmaThreads) gram_mfma_kernel(
@@ -1132,13 +1136,6 @@ __device__ void load_reduced(const double* __restrict__ red, int off, double* ou
     __syncthreads();
 }
 
-// Threads of the one-block K x K kernels: one per matrix entry (whole wavefronts), so the barriers
-// of the factorisation span as few wavefronts as the work needs.
-template <int K>
-constexpr int small_threads() {
-    return K * K <= kWave ? kWave : K * K;
-}
-
 // Symmetric sweeps of an M x M matrix held by one wavefront, lane l owning entries e = l + 64 u
 // (row e / M, column e % M) in a[u], its diagonal mirrored in diag (LDS, written by the caller
 // before a barrier). The largest remaining diagonal among `cand` (lowest index on ties) is swept
@@ -1370,16 +1367,16 @@ __global__ void __launch_bounds__(kSmallThreads) dmat_kernel(const double* __res
     if (finished(state)) return;
     __shared__ double E[K * K];
     load_reduced<K * K>(partials, 0, E);
-    const int t = threadIdx.x;
     if (independent) {
-        if (t < K * K && (t / K) != (t % K)) E[t] = 0.0;
+        for (int e = threadIdx.x; e < K * K; e += blockDim.x)
+            if ((e / K) != (e % K)) E[e] = 0.0;
         __syncthreads();
     }
-    if (t < K * K) {
-        const int i = t / K, c = t % K;
+    for (int e = threadIdx.x; e < K * K; e += blockDim.x) {
+        const int i = e / K, c = e % K;
         double s = 0.0;
         for (int b = 0; b < K; ++b) s += TT[i * K + b] * E[b * K + c];
-        D[t] = s;
+        D[e] = s;
     }
 }
 
@@ -1389,6 +1386,7 @@ __global__ void __launch_bounds__(kSmallThreads) converge_kernel(const double* _
                                                                  const double* __restrict__ target,
                                                                  double* __restrict__ norms,
                                                                  int* __restrict__ state) {
+    static_assert(K <= kWave, "one lane per column");
     if (finished(state)) return;
     __shared__ double v[K];
     __shared__ int all;
@@ -1938,13 +1936,14 @@ void BlockSolver::iteration_k(const SolveOptions& opt, int j) {
                                         (multi_ || opt.independent) ? 1 : 0, ran_.get() + j);
     update_r_kernel<K><<<kGramBlocks, xr_threads<K>(), 0, s_>>>(nn, S_.get(), al, R_.get(), partials_.get(), rpb, st);
     reduce_cols(partials_.get(), kGramBlocks, K, red_.get(), st, s_);
-    converge_kernel<K><<<1, small_threads<K>(), 0, s_>>>(red_.get(), kGramBlocks, target_.get(), norms_.get(), stw);
+    // one wavefront: the kernel reads K norms (small workgroups find room beside the sweep's N-body)
+    converge_kernel<K><<<1, kWave, 0, s_>>>(red_.get(), kGramBlocks, target_.get(), norms_.get(), stw);
     mark(1, st);
     vcycle_k<K>(R_.get(), Z_.get(), st);
     mark(2, st);
     launch_gram<K, false>(nn, S_.get(), Z_.get(), nullptr, partials_.get(), rpb, st, s_);
     reduce_cols(partials_.get(), kGramBlocks, K * K, red_.get(), st, s_);
-    dmat_kernel<K><<<1, small_threads<K>(), 0, s_>>>(red_.get(), kGramBlocks, TT_.get(), D_.get(), stw,
+    dmat_kernel<K><<<1, kBlock, 0, s_>>>(red_.get(), kGramBlocks, TT_.get(), D_.get(), stw,
                                                 (multi_ || opt.independent) ? 1 : 0);
     if (last)  // before this iteration's W update, which overwrites slot 0
         flush_x_kernel<K><<<grid_for(nn * col_groups<K>(), kRowThreads), kRowThreads, 0, s_>>>(
