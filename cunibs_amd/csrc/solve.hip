@@ -1109,6 +1109,7 @@ constexpr int kReduceThreads = 256;
 __global__ void __launch_bounds__(kReduceThreads) reduce_cols_kernel(const double* __restrict__ partials, int nb,
                                                                      int stride, int E, double* __restrict__ out,
                                                                      const int* __restrict__ state) {
+    __builtin_amdgcn_s_setprio(3);  // a short chain of dependent steps: issue ahead of the sweep's N-body
     if (finished(state)) return;
     const int w = static_cast<int>((static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6);
     const int lane = threadIdx.x & (kWave - 1);
@@ -1207,6 +1208,7 @@ template <int K>
 __global__ void __launch_bounds__(kWave) coef_kernel(const double* __restrict__ partials, double drop_tol,
                                                      double* __restrict__ TT, double* __restrict__ AL,
                                                      int* __restrict__ state, int independent, int* __restrict__ ran) {
+    __builtin_amdgcn_s_setprio(3);  // a short chain of dependent steps: issue ahead of the sweep's N-body
     if (finished(state)) return;
     if (threadIdx.x == 0) *ran = 1;  // this window slot's coefficients are to be applied
     constexpr int kE = K * K;
@@ -1368,6 +1370,7 @@ template <int K>
 __global__ void __launch_bounds__(kSmallThreads) dmat_kernel(const double* __restrict__ partials, int nb,
                                                              const double* __restrict__ TT, double* __restrict__ D,
                                                              int* __restrict__ state, int independent) {
+    __builtin_amdgcn_s_setprio(3);  // a short chain of dependent steps: issue ahead of the sweep's N-body
     if (finished(state)) return;
     __shared__ double E[K * K];
     load_reduced<K * K>(partials, 0, E);
@@ -1390,6 +1393,7 @@ __global__ void __launch_bounds__(kSmallThreads) converge_kernel(const double* _
                                                                  const double* __restrict__ target,
                                                                  double* __restrict__ norms,
                                                                  int* __restrict__ state) {
+    __builtin_amdgcn_s_setprio(3);  // a short chain of dependent steps: issue ahead of the sweep's N-body
     static_assert(K <= kWave, "one lane per column");
     if (finished(state)) return;
     __shared__ double v[K];
