@@ -539,14 +539,18 @@ typedef double doublex4_t __attribute__((ext_vector_type(4)));
 // the products of the FP32 inputs are exact and each input is converted once. Accumulator
 // register v of lane l holds tile row 4 v + l / 16 of tile column l % 16 (the FP64 layout, unlike
 // the FP32 one's rows 4 (l / 16) + v). U quads of rows per step keep 3 U loads in flight.
-// Four waves per workgroup: the pipelined sweep keeps one persistent N-body workgroup on every CU,
-// and a 1024-thread Gram workgroup (16 wave slots and, at 32 columns with two products, 111 VGPRs
-// a lane) often found no CU with room beside it (at 32 columns the pass took 1.46 ms in the sweep
-// against 0.1 ms alone); small workgroups fill around it.
-constexpr int kGramMfmaThreads = 256;
+// Threads per Gram workgroup. The pipelined sweep keeps one persistent N-body workgroup on every
+// CU; above 16 columns a 1024-thread Gram workgroup (16 wave slots, 111 VGPRs a lane with two
+// products) rarely found a CU with room beside it (1.46 ms in the sweep against 0.1 ms alone), and
+// four-wave workgroups fill around it. Up to 16 columns 1024 threads fit and hide more latency
+// (43 µs alone against 84 µs in four-wave workgroups, 105 against 97 µs in the sweep).
+template <int K>
+constexpr int gram_mfma_threads() {
+    return K > 16 ? 256 : 1024;
+}
 
 template <int K, bool TWO>
-__global__ void __launch_bounds__(kGramMfmaThreads) gram_mfma_kernel(
+__global__ void __launch_bounds__(gram_mfma_threads<K>()) gram_mfma_kernel(
     std::int64_t n, const float* __restrict__ A, const float* __restrict__ B1, const float* __restrict__ B2,
     double* __restrict__ partials, std::int64_t rows_per_block, const int* __restrict__ state) {
     static_assert(K == 16 || K == 32, "matrix-core Gram tiles are 16 wide");
@@ -558,7 +562,7 @@ __global__ void __launch_bounds__(kGramMfmaThreads) gram_mfma_kernel(
     constexpr int H = K / 16;          // 16-column halves of a row
     constexpr int NB = TWO ? 2 : 1;
     constexpr int U = 8 / H;           // row quads per step
-    constexpr int WAVES = kGramMfmaThreads / kWave;
+    constexpr int WAVES = gram_mfma_threads<K>() / kWave;
     const int lb = logical_block();
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
     const int col = lane & 15, quad = lane >> 4;
@@ -625,8 +629,8 @@ template <int K, bool TWO>
 void launch_gram(std::int64_t n, const float* A, const float* B1, const float* B2, double* partials,
                  std::int64_t rows_per_block, const int* state, hipStream_t s) {
     if constexpr (K >= 16)
-        gram_mfma_kernel<K, TWO><<<kGramBlocks, kGramMfmaThreads, 0, s>>>(n, A, B1, B2, partials, rows_per_block,
-                                                                          state);
+        gram_mfma_kernel<K, TWO><<<kGramBlocks, gram_mfma_threads<K>(), 0, s>>>(n, A, B1, B2, partials,
+                                                                                rows_per_block, state);
     else
         gram_rows_kernel<K, TWO><<<kGramBlocks, gram_threads<K, TWO>(), 0, s>>>(n, A, B1, B2, partials,
                                                                               rows_per_block, state);
